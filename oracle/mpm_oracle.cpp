@@ -1,0 +1,918 @@
+// CPU oracle for the cuMPM hot path — TEST INFRASTRUCTURE ONLY.
+//
+// This file is a plain C++17/OpenMP restatement of the reference solver's
+// monodisperse spectral-Ewald dipole operator and its Krylov solvers, in FP64.
+// It exists to check the HIP product path (tests/, __graft_entry__.smoke()
+// and the cpu_baseline leg of bench.py are its only callers).  Nothing in the
+// product library links or calls it.
+//
+// What it restates (reference files, read-only):
+//   sizing              src/cpp/electric_field/monodisperse_ewald_electric_field.cu:546-601
+//   real-space tables   monodisperse_ewald_electric_field.cu:92-176, 455-500 (dipole tables,
+//                       self term `selfo`)
+//   table interpolation monodisperse_ewald_electric_field.cu:668-685
+//   spread/contract     monodisperse_ewald_electric_field.cu:687-816, 960-1069, 1160-1298, 1488-1525
+//   k-space scale       monodisperse_ewald_electric_field.cu:1300-1432
+//   real space + self   monodisperse_ewald_electric_field.cu:877-958, 1589-1869
+//   neighbour list      src/cpp/electric_field/neighbor_list.cu:20-223, 242-420
+//   GMRES(m)            src/cpp/numerical_solver/gmres_solver.cu:56-208
+//   BiCGSTAB            src/cpp/numerical_solver/bicgstab_solver.cu:58-192
+//   Jacobi / BLAS-1     src/cpp/numerical_solver/numerical_solver.cu:22-276
+// The FFT is this file's own mixed-radix + Bluestein transform (the reference
+// uses cuFFT Z2Z, unnormalised, forward e^{-i}, inverse e^{+i}).
+//
+// Pinning (see tests/test_oracle_*.py): golden fixture
+// tests/data/ref_dipole_monodisperse.npz, numpy.fft, a textbook point-dipole
+// Ewald lattice sum (non-overlapping spheres), quadrature of the
+// overlapping-sphere field (1 <= r < 2), xi-independence of real+recip, and a
+// dense np.linalg.solve for the Krylov solvers.
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+using cplx = std::complex<double>;
+static const double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------
+// 1-D FFT: recursive mixed radix for lengths whose prime factors are <= 31,
+// Bluestein (chirp-z over a power-of-two convolution) otherwise.
+// ---------------------------------------------------------------------------
+namespace fft {
+
+struct Plan1D {
+    int n = 0;
+    std::vector<int> factors;     // radices, product == n (mixed radix path)
+    std::vector<cplx> tw;         // exp(-2 pi i j / n), j < n
+    bool bluestein = false;
+    int m = 0;                    // Bluestein convolution length (power of two)
+    std::vector<cplx> chirp;      // exp(-i pi j^2 / n), j < n
+    std::vector<cplx> bhat;       // FFT_m of the conjugate chirp, wrapped
+    std::unique_ptr<Plan1D> sub;  // power-of-two plan of length m
+};
+
+static void factorize(int n, std::vector<int>& f) {
+    f.clear();
+    while (n % 4 == 0) { f.push_back(4); n /= 4; }
+    while (n % 2 == 0) { f.push_back(2); n /= 2; }
+    for (int p = 3; p * p <= n; p += 2)
+        while (n % p == 0) { f.push_back(p); n /= p; }
+    if (n > 1) f.push_back(n);
+}
+
+static std::unique_ptr<Plan1D> make_plan(int n);
+
+static void twiddles(int n, std::vector<cplx>& tw) {
+    tw.resize(n);
+    for (int j = 0; j < n; ++j) {
+        double a = -2.0 * kPi * (double)j / (double)n;
+        tw[j] = cplx(std::cos(a), std::sin(a));
+    }
+}
+
+static std::unique_ptr<Plan1D> make_plan(int n) {
+    auto p = std::make_unique<Plan1D>();
+    p->n = n;
+    std::vector<int> f;
+    factorize(n, f);
+    int maxf = 1;
+    for (int x : f) maxf = std::max(maxf, x);
+    if (maxf > 31) {
+        p->bluestein = true;
+        int m = 1;
+        while (m < 2 * n - 1) m <<= 1;
+        p->m = m;
+        p->chirp.resize(n);
+        const long long two_n = 2LL * n;
+        for (int j = 0; j < n; ++j) {
+            long long jj = ((long long)j * j) % two_n;  // keep the phase argument small
+            double a = -kPi * (double)jj / (double)n;
+            p->chirp[j] = cplx(std::cos(a), std::sin(a));
+        }
+        p->sub = make_plan(m);
+        std::vector<cplx> b(m, cplx(0, 0));
+        b[0] = std::conj(p->chirp[0]);
+        for (int j = 1; j < n; ++j) b[j] = b[m - j] = std::conj(p->chirp[j]);
+        p->bhat = b;
+        // forward transform of b (in place through a scratch copy)
+        std::vector<cplx> out(m), scratch(m);
+        extern void exec_plan(const Plan1D&, const cplx*, int, cplx*, cplx*);
+        exec_plan(*p->sub, b.data(), 1, out.data(), scratch.data());
+        p->bhat = out;
+    } else {
+        p->factors = f;
+        twiddles(n, p->tw);
+    }
+    return p;
+}
+
+// out[k] = sum_j in[j*stride] * exp(-2 pi i j k / n) for the sub-length n
+// reached after `level` radix splits (tw is the full-length table, twstep maps
+// the sub-length twiddle index to the full table).
+static void rec(const Plan1D& P, int level, int n, const cplx* in, long stride,
+                cplx* out, int twstep) {
+    if (n == 1) { out[0] = in[0]; return; }
+    const int p = P.factors[level];
+    const int m = n / p;
+    for (int r = 0; r < p; ++r)
+        rec(P, level + 1, m, in + r * stride, stride * p, out + r * m, twstep * p);
+    // combine: X[k + q m] = sum_r (W_n^{r k} Y_r[k]) W_p^{r q}
+    cplx tmp[32];
+    cplx y[32];
+    for (int k = 0; k < m; ++k) {
+        for (int r = 0; r < p; ++r) {
+            long idx = (long)r * k * twstep;
+            y[r] = out[r * m + k] * P.tw[idx % P.n];
+        }
+        for (int q = 0; q < p; ++q) {
+            cplx s(0, 0);
+            for (int r = 0; r < p; ++r) {
+                long idx = ((long)r * q % p) * (P.n / p);
+                s += y[r] * P.tw[idx];
+            }
+            tmp[q] = s;
+        }
+        for (int q = 0; q < p; ++q) out[k + q * m] = tmp[q];
+    }
+}
+
+void exec_plan(const Plan1D& P, const cplx* in, int stride, cplx* out, cplx* scratch) {
+    if (!P.bluestein) {
+        rec(P, 0, P.n, in, stride, out, 1);
+        return;
+    }
+    const int n = P.n, m = P.m;
+    std::vector<cplx> a(m, cplx(0, 0)), A(m), s2(m);
+    for (int j = 0; j < n; ++j) a[j] = in[(long)j * stride] * P.chirp[j];
+    exec_plan(*P.sub, a.data(), 1, A.data(), s2.data());
+    for (int j = 0; j < m; ++j) A[j] *= P.bhat[j];
+    // inverse via conjugation
+    for (int j = 0; j < m; ++j) A[j] = std::conj(A[j]);
+    exec_plan(*P.sub, A.data(), 1, a.data(), s2.data());
+    const double inv_m = 1.0 / (double)m;
+    for (int k = 0; k < n; ++k) out[k] = std::conj(a[k]) * inv_m * P.chirp[k];
+    (void)scratch;
+}
+
+// In-place 3-D transform of a row-major [nx][ny][nz] complex array.
+// sign = -1: forward e^{-i}; sign = +1: inverse e^{+i}; unnormalised.
+static void fft3d(cplx* data, int nx, int ny, int nz, int sign) {
+    const int dims[3] = {nx, ny, nz};
+    const long strides[3] = {(long)ny * nz, (long)nz, 1};
+    for (int ax = 0; ax < 3; ++ax) {
+        const int n = dims[ax];
+        if (n == 1) continue;
+        auto plan = make_plan(n);
+        const long stride = strides[ax];
+        // enumerate all lines along `ax`
+        const int o1 = (ax == 0) ? 1 : 0;          // the two other axes
+        const int o2 = (ax == 2) ? 1 : 2;
+        const long nlines = (long)dims[o1] * dims[o2];
+#pragma omp parallel
+        {
+            std::vector<cplx> line(n), out(n), scratch(n);
+#pragma omp for schedule(static)
+            for (long L = 0; L < nlines; ++L) {
+                const long a = L / dims[o2], b = L % dims[o2];
+                const long base = a * strides[o1] + b * strides[o2];
+                for (int j = 0; j < n; ++j) {
+                    cplx v = data[base + j * stride];
+                    line[j] = (sign > 0) ? std::conj(v) : v;
+                }
+                exec_plan(*plan, line.data(), 1, out.data(), scratch.data());
+                for (int j = 0; j < n; ++j)
+                    data[base + j * stride] = (sign > 0) ? std::conj(out[j]) : out[j];
+            }
+        }
+    }
+}
+
+}  // namespace fft
+
+// ---------------------------------------------------------------------------
+// Ewald sizing and real-space tables (monodisperse, radius 1 after nondim).
+// ---------------------------------------------------------------------------
+struct Sizing {
+    double rc = 0, kcut = 0;
+    int n[3] = {0, 0, 0};
+    double h[3] = {0, 0, 0};
+    double eta[3] = {0, 0, 0};
+    int P = 0;       // num_grid_gaussian
+    int half = 0;    // P / 2
+    int width = 0;   // 2*half + 1 stencil points per axis
+};
+
+// monodisperse_ewald_electric_field.cu:546-601.  grid_override / P_override
+// replace the computed grid or support when positive (the reference ties them
+// to errortol; the override is this build's decoupling knob).
+static int compute_sizing(const double L[3], double errortol, double xi,
+                          const int* grid_override, int P_override, Sizing& S,
+                          std::string& err) {
+    S.rc = std::sqrt(-std::log(errortol)) / xi;
+    if (S.rc < 2.0) S.rc = 2.0;
+    for (int d = 0; d < 3; ++d) {
+        if (S.rc > L[d] / 2.0) {
+            err = "Real space cutoff (" + std::to_string(S.rc) +
+                  ") larger than half the box length.";
+            return -1;
+        }
+    }
+    S.kcut = 2.0 * xi * xi * S.rc;
+    for (int d = 0; d < 3; ++d) {
+        S.n[d] = (int)std::ceil(1.0 + L[d] * S.kcut / kPi);
+        if (grid_override && grid_override[d] > 0) S.n[d] = grid_override[d];
+        S.h[d] = L[d] / S.n[d];
+    }
+    S.P = (int)std::ceil(-2.0 * std::log(errortol) / kPi);
+    if (P_override > 0) S.P = P_override;
+    for (int d = 0; d < 3; ++d) S.eta[d] = S.P * std::pow(S.h[d] * xi, 2) / kPi;
+    S.half = S.P / 2;
+    S.width = 2 * S.half + 1;
+    return 0;
+}
+
+// Volume-averaged field of a uniformly polarised unit sphere on another unit
+// sphere at centre distance r, split into the real-space Ewald part.  Each
+// function is a sum over the three "images" of the pair kernel at r-2, r, r+2:
+//   f(r) = sum_s [A_s(r) exp(-(r+s)^2 xi^2) + B_s(r) erfc((r+s) xi)] + reg(r) [r<2]
+// with s in {+2, -2, 0}.  monodisperse_ewald_electric_field.cu:111-176.
+struct DipoleTables {
+    std::vector<double> r, perp, para;  // 9001 entries, entry 0 = self (r=0)
+    double self_perp = 0;
+};
+
+static void dipole_pair_functions(double r, double xi, double& f_perp, double& f_para) {
+    const double x2 = xi * xi, x4 = x2 * x2, x5 = x4 * xi, x6 = x4 * x2;
+    const double r2 = r * r, r3 = r2 * r, r4 = r2 * r2, r5 = r4 * r, r6 = r4 * r2;
+    const double sp = std::pow(kPi, 1.5);
+    const double Ep = std::exp(-(r + 2.0) * (r + 2.0) * x2);
+    const double Em = std::exp(-(r - 2.0) * (r - 2.0) * x2);
+    const double E0 = std::exp(-r2 * x2);
+    const double Cp = std::erfc((r + 2.0) * xi);
+    const double Cm = std::erfc((r - 2.0) * xi);
+    const double C0 = std::erfc(r * xi);
+    // common polynomial pieces
+    const double c1 = 3.0 - 12.0 * x2 + 32.0 * x4;
+    const double c0 = 2.0 * (3.0 + 4.0 * x2 - 32.0 * x4);
+    const double q4 = 36.0 * x4 * (1.0 - 4.0 * x2);
+
+    // ---- perpendicular (field_dip_1) ----
+    {
+        const double pre_e = 1.0 / (1024.0 * sp * x5 * r3);
+        const double odd = 4.0 * x4 * r5 + 8.0 * x2 * (2.0 - 7.0 * x2) * r3 + c1 * r;
+        const double even = -8.0 * x4 * r4 - 8.0 * x2 * (3.0 + 2.0 * x2) * r2 + c0;
+        const double Ap = pre_e * (odd + even);
+        const double Am = pre_e * (odd - even);
+        const double A0 = (-4.0 * x4 * r4 - 8.0 * x2 * (2.0 - 9.0 * x2) * r2 - 3.0 + 36.0 * x2) /
+                          (512.0 * sp * x5 * r2);
+        const double pre_b = 1.0 / (2048.0 * kPi * x6 * r3);
+        const double bev = -8.0 * x6 * r6 - q4 * r4 - 18.0 * x2 * (1.0 - 8.0 * x2) * r2 + 3.0 -
+                           36.0 * x2 + 256.0 * x6;
+        const double Bp = pre_b * (bev + 256.0 * x6 * r3);
+        const double Bm = pre_b * (bev - 256.0 * x6 * r3);
+        const double B0 = (8.0 * x6 * r6 + q4 * r4 + 18.0 * x2 * (1.0 - 8.0 * x2) * r2 - 3.0 + 36.0 * x2) /
+                          (1024.0 * kPi * x6 * r3);
+        double f = Ap * Ep + Bp * Cp + Am * Em + Bm * Cm + A0 * E0 + B0 * C0;
+        if (r < 2.0) f += -1.0 / (4.0 * kPi * r3) + (1.0 - 9.0 * r / 16.0 + r3 / 32.0) / (4.0 * kPi);
+        f_perp = f;
+    }
+    // ---- parallel (field_dip_2) ----
+    {
+        const double pre_e = 1.0 / (512.0 * sp * x5 * r3);
+        const double odd = 8.0 * x4 * r5 + 2.0 * x2 * (7.0 - 20.0 * x2) * r3 - c1 * r;
+        const double even = -16.0 * x4 * r4 - 4.0 * x2 * (3.0 - 4.0 * x2) * r2 - c0;
+        const double Ap = pre_e * (odd + even);
+        const double Am = pre_e * (odd - even);
+        const double A0 = (-8.0 * x4 * r4 - 2.0 * x2 * (7.0 - 36.0 * x2) * r2 + 3.0 - 36.0 * x2) /
+                          (256.0 * sp * x5 * r2);
+        const double pre_b = 1.0 / (1024.0 * kPi * x6 * r3);
+        const double bev = -16.0 * x6 * r6 - q4 * r4 - 3.0 + 36.0 * x2 - 256.0 * x6;
+        const double Bp = pre_b * (bev + 128.0 * x6 * r3);
+        const double Bm = pre_b * (bev - 128.0 * x6 * r3);
+        const double B0 = (16.0 * x6 * r6 + q4 * r4 + 3.0 - 36.0 * x2) / (512.0 * kPi * x6 * r3);
+        double f = Ap * Ep + Bp * Cp + Am * Em + Bm * Cm + A0 * E0 + B0 * C0;
+        if (r < 2.0) f += 1.0 / (2.0 * kPi * r3) + (1.0 - 9.0 * r / 8.0 + r3 / 8.0) / (4.0 * kPi);
+        f_para = f;
+    }
+}
+
+// monodisperse_ewald_electric_field.cu:455-500: 9000 samples r = 1 + 0.001 j,
+// preceded by the r = 0 self entry.
+static void build_dipole_tables(double xi, DipoleTables& T) {
+    const int nsteps = 9000;
+    T.r.assign(nsteps + 1, 0.0);
+    T.perp.assign(nsteps + 1, 0.0);
+    T.para.assign(nsteps + 1, 0.0);
+    const double x2 = xi * xi, x3 = x2 * xi;
+    const double selfo = (-1.0 + 6.0 * x2 + (1.0 - 2.0 * x2) * std::exp(-4.0 * x2)) /
+                             (16.0 * std::pow(kPi, 1.5) * x3) +
+                         std::erfc(2.0 * xi) / (4.0 * kPi);
+    T.self_perp = selfo;
+    T.r[0] = 0.0;
+    T.perp[0] = selfo;
+    T.para[0] = selfo;
+    for (int j = 0; j < nsteps; ++j) {
+        const double r = 1.0 + j * 0.001;
+        double fp, fa;
+        dipole_pair_functions(r, xi, fp, fa);
+        T.r[j + 1] = r;
+        T.perp[j + 1] = fp;
+        T.para[j + 1] = fa;
+    }
+}
+
+// monodisperse_ewald_electric_field.cu:668-685 (clamps at the last sample).
+static double interp(double r, const std::vector<double>& rt, const std::vector<double>& y) {
+    const size_t ts = y.size();
+    if (r <= 0.0) return y[0];
+    if (r >= rt[ts - 1]) return y[ts - 1];
+    if (r < 1.0) return y[0] * (1.0 - r) + y[1] * r;
+    size_t idx = 1 + (size_t)((r - 1.0) / 0.001);
+    if (idx >= ts - 1) return y[ts - 1];
+    const double r0 = rt[idx], r1 = rt[idx + 1];
+    const double t = (r - r0) / (r1 - r0);
+    return y[idx] * (1.0 - t) + y[idx + 1] * t;
+}
+
+// ---------------------------------------------------------------------------
+// Operator
+// ---------------------------------------------------------------------------
+struct Op {
+    double L[3];
+    double errortol, xi;
+    Sizing S;
+    DipoleTables T;
+    double kbx = 0, kby = 0;  // Bloch wavevector (x, y)
+    // geometry
+    int N = 0;
+    std::vector<double> x, y, z;
+    std::vector<int> nb_start, nb_j;   // CSR over source particle i
+    std::vector<double> nb_perp, nb_para;
+    std::vector<int> sp_idx;           // [i][o] wrapped linear voxel index
+    std::vector<double> sp_e;          // [i][o] exp(-2 xi^2 sum d^2/eta)
+    double C_spread = 0, C_contract = 0;
+    std::vector<double> scale, khat;   // V, 3V
+    size_t V = 0;
+    std::string err;
+};
+
+static void compute_scale(Op& op) {
+    const Sizing& S = op.S;
+    const int nx = S.n[0], ny = S.n[1], nz = S.n[2];
+    op.V = (size_t)nx * ny * nz;
+    op.scale.assign(op.V, 0.0);
+    op.khat.assign(3 * op.V, 0.0);
+    const double xi = op.xi;
+#pragma omp parallel for schedule(static)
+    for (long v = 0; v < (long)op.V; ++v) {
+        const int ix = (int)(v / ((long)ny * nz));
+        const int iy = (int)((v / nz) % ny);
+        const int iz = (int)(v % nz);
+        const double fx = (ix <= (nx - 1) / 2) ? ix : (ix - nx);
+        const double fy = (iy <= (ny - 1) / 2) ? iy : (iy - ny);
+        const double fz = (iz <= (nz - 1) / 2) ? iz : (iz - nz);
+        const double Gx = fx * 2.0 * kPi / op.L[0], Gy = fy * 2.0 * kPi / op.L[1],
+                     Gz = fz * 2.0 * kPi / op.L[2];
+        const double kx = Gx - op.kbx, ky = Gy - op.kby, kz = Gz;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        const double k = std::sqrt(k2);
+        if (k < 1e-12) continue;
+        op.khat[3 * v + 0] = kx / k;
+        op.khat[3 * v + 1] = ky / k;
+        op.khat[3 * v + 2] = kz / k;
+        const double etak2 = k2 - (Gx * Gx * S.eta[0] + Gy * Gy * S.eta[1] + Gz * Gz * S.eta[2]);
+        const double t = std::sin(k) / k - std::cos(k);
+        op.scale[v] = 9.0 * t * t * std::exp(-etak2 / (4.0 * xi * xi)) / (k2 * k2) / (double)op.V;
+    }
+}
+
+// neighbor_list.cu: all j != i with min-image |r|^2 <= rc^2 (cell list when
+// floor(L/rc) >= 3 on every axis, brute force otherwise — same pair set).
+static void build_neighbors(Op& op) {
+    const int N = op.N;
+    const double rc2 = op.S.rc * op.S.rc;
+    const double* L = op.L;
+    std::vector<std::vector<int>> lists(N);
+    int nc[3];
+    for (int d = 0; d < 3; ++d) nc[d] = (int)std::floor(L[d] / op.S.rc);
+    const bool cells = nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3;
+    auto minimg = [&](double dx, double Ld) { return dx - Ld * std::round(dx / Ld); };
+    if (cells) {
+        const double cs[3] = {L[0] / nc[0], L[1] / nc[1], L[2] / nc[2]};
+        const long ncell = (long)nc[0] * nc[1] * nc[2];
+        std::vector<int> cell_of(N);
+        std::vector<std::vector<int>> members(ncell);
+        auto cell_coord = [&](double p, int d) {
+            double w = p - L[d] * std::floor(p / L[d]);
+            int c = (int)std::floor(w / cs[d]);
+            if (c < 0) c = 0;
+            if (c >= nc[d]) c = nc[d] - 1;
+            return c;
+        };
+        for (int j = 0; j < N; ++j) {
+            int cx = cell_coord(op.x[j], 0), cy = cell_coord(op.y[j], 1), cz = cell_coord(op.z[j], 2);
+            members[cx + (long)cy * nc[0] + (long)cz * nc[0] * nc[1]].push_back(j);
+        }
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int i = 0; i < N; ++i) {
+            int cx = cell_coord(op.x[i], 0), cy = cell_coord(op.y[i], 1), cz = cell_coord(op.z[i], 2);
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        int a = (cx + dx + nc[0]) % nc[0], b = (cy + dy + nc[1]) % nc[1],
+                            c = (cz + dz + nc[2]) % nc[2];
+                        for (int j : members[a + (long)b * nc[0] + (long)c * nc[0] * nc[1]]) {
+                            if (j == i) continue;
+                            double rx = minimg(op.x[i] - op.x[j], L[0]);
+                            double ry = minimg(op.y[i] - op.y[j], L[1]);
+                            double rz = minimg(op.z[i] - op.z[j], L[2]);
+                            if (rx * rx + ry * ry + rz * rz <= rc2) lists[i].push_back(j);
+                        }
+                    }
+            std::sort(lists[i].begin(), lists[i].end());
+        }
+    } else {
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int i = 0; i < N; ++i) {
+            for (int j = 0; j < N; ++j) {
+                if (j == i) continue;
+                double rx = minimg(op.x[i] - op.x[j], L[0]);
+                double ry = minimg(op.y[i] - op.y[j], L[1]);
+                double rz = minimg(op.z[i] - op.z[j], L[2]);
+                if (rx * rx + ry * ry + rz * rz <= rc2) lists[i].push_back(j);
+            }
+        }
+    }
+    op.nb_start.assign(N + 1, 0);
+    for (int i = 0; i < N; ++i) op.nb_start[i + 1] = op.nb_start[i] + (int)lists[i].size();
+    const int M = op.nb_start[N];
+    op.nb_j.resize(M);
+    op.nb_perp.assign(M, 0.0);
+    op.nb_para.assign(M, 0.0);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < N; ++i) {
+        int s = op.nb_start[i];
+        for (size_t k = 0; k < lists[i].size(); ++k) {
+            const int j = lists[i][k];
+            op.nb_j[s + k] = j;
+            double rx = minimg(op.x[i] - op.x[j], L[0]);
+            double ry = minimg(op.y[i] - op.y[j], L[1]);
+            double rz = minimg(op.z[i] - op.z[j], L[2]);
+            double d = std::sqrt(rx * rx + ry * ry + rz * rz);
+            if (d < op.S.rc) {
+                double de = d < 1.0 ? 1.0 : d;
+                op.nb_perp[s + k] = interp(de, op.T.r, op.T.perp);
+                op.nb_para[s + k] = interp(de, op.T.r, op.T.para);
+            }
+        }
+    }
+}
+
+// monodisperse_ewald_electric_field.cu:687-816 (weights C*exp(-2 xi^2 sum gd^2/eta)).
+static void build_stencils(Op& op) {
+    const Sizing& S = op.S;
+    const int W = S.width, half = S.half;
+    const long SS = (long)W * W * W;
+    op.sp_idx.assign((size_t)op.N * SS, 0);
+    op.sp_e.assign((size_t)op.N * SS, 0.0);
+    const double prod_eta = S.eta[0] * S.eta[1] * S.eta[2];
+    op.C_spread = std::pow(2.0 * op.xi * op.xi / kPi, 1.5) * std::sqrt(1.0 / prod_eta);
+    op.C_contract = op.C_spread * (S.h[0] * S.h[1] * S.h[2]);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < op.N; ++i) {
+        const double p[3] = {op.x[i], op.y[i], op.z[i]};
+        int g[3];
+        double dd[3];
+        for (int d = 0; d < 3; ++d) {
+            g[d] = (int)std::round(p[d] / S.h[d]);
+            dd[d] = g[d] * S.h[d] - p[d];
+        }
+        long o = 0;
+        for (int ox = -half; ox <= half; ++ox)
+            for (int oy = -half; oy <= half; ++oy)
+                for (int oz = -half; oz <= half; ++oz, ++o) {
+                    const int off[3] = {ox, oy, oz};
+                    int w[3];
+                    double s = 0.0;
+                    for (int d = 0; d < 3; ++d) {
+                        w[d] = ((g[d] + off[d]) % S.n[d] + S.n[d]) % S.n[d];
+                        const double gd = dd[d] + off[d] * S.h[d];
+                        s += gd * gd / S.eta[d];
+                    }
+                    op.sp_idx[(size_t)i * SS + o] = (w[0] * S.n[1] + w[1]) * S.n[2] + w[2];
+                    op.sp_e[(size_t)i * SS + o] = std::exp(-2.0 * op.xi * op.xi * s);
+                }
+    }
+}
+
+// Apply E = A p for one right-hand side (SOLVER_AX mode).  p, E: [N][3] complex.
+// self_coef: [N] complex.  parts: bit 0 real space + self, bit 1 reciprocal.
+static void apply(const Op& op, const cplx* self_coef, const cplx* p, cplx* E, int parts) {
+    const int N = op.N;
+    for (long q = 0; q < 3L * N; ++q) E[q] = 0.0;
+    if (parts & 2) {
+        const Sizing& S = op.S;
+        const size_t V = op.V;
+        const long SS = (long)S.width * S.width * S.width;
+        std::vector<cplx> grid(3 * V, cplx(0, 0));
+        // spread (serial over particles to keep the accumulation race-free)
+        for (int i = 0; i < N; ++i) {
+            for (long o = 0; o < SS; ++o) {
+                const double w = op.C_spread * op.sp_e[(size_t)i * SS + o];
+                const int v = op.sp_idx[(size_t)i * SS + o];
+                for (int c = 0; c < 3; ++c) grid[c * V + v] += w * p[3 * i + c];
+            }
+        }
+        for (int c = 0; c < 3; ++c) fft::fft3d(grid.data() + c * V, S.n[0], S.n[1], S.n[2], -1);
+#pragma omp parallel for schedule(static)
+        for (long v = 0; v < (long)V; ++v) {
+            const double kx = op.khat[3 * v], ky = op.khat[3 * v + 1], kz = op.khat[3 * v + 2];
+            const cplx dot = grid[v] * kx + grid[V + v] * ky + grid[2 * V + v] * kz;
+            const cplx s = dot * op.scale[v];
+            grid[v] = s * kx;
+            grid[V + v] = s * ky;
+            grid[2 * V + v] = s * kz;
+        }
+        for (int c = 0; c < 3; ++c) fft::fft3d(grid.data() + c * V, S.n[0], S.n[1], S.n[2], +1);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < N; ++i) {
+            cplx acc[3] = {0.0, 0.0, 0.0};
+            for (long o = 0; o < SS; ++o) {
+                const double w = op.C_contract * op.sp_e[(size_t)i * SS + o];
+                const int v = op.sp_idx[(size_t)i * SS + o];
+                for (int c = 0; c < 3; ++c) acc[c] += grid[c * V + v] * w;
+            }
+            for (int c = 0; c < 3; ++c) E[3 * i + c] += acc[c];
+        }
+    }
+    if (parts & 1) {
+        for (int i = 0; i < N; ++i) {
+            const cplx f = self_coef[i] + op.T.self_perp;
+            for (int c = 0; c < 3; ++c) E[3 * i + c] += f * p[3 * i + c];
+        }
+        // source-centric scatter, as the reference kernel: E_j += T(r_ij) p_i
+        const bool bloch = (op.kbx != 0.0 || op.kby != 0.0);
+        for (int i = 0; i < N; ++i) {
+            for (int k = op.nb_start[i]; k < op.nb_start[i + 1]; ++k) {
+                const int j = op.nb_j[k];
+                double r[3] = {op.x[i] - op.x[j], op.y[i] - op.y[j], op.z[i] - op.z[j]};
+                for (int d = 0; d < 3; ++d) r[d] -= op.L[d] * std::round(r[d] / op.L[d]);
+                const double d = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+                if (!(d < op.S.rc)) continue;
+                const double de = d < 1.0 ? 1.0 : d;
+                const double u[3] = {r[0] / de, r[1] / de, r[2] / de};
+                const cplx rp = p[3 * i] * u[0] + p[3 * i + 1] * u[1] + p[3 * i + 2] * u[2];
+                cplx e[3];
+                for (int c = 0; c < 3; ++c)
+                    e[c] = (p[3 * i + c] - rp * u[c]) * op.nb_perp[k] + (rp * u[c]) * op.nb_para[k];
+                if (bloch) {
+                    const double ph = -(op.kbx * r[0] + op.kby * r[1]);
+                    const cplx f(std::cos(ph), std::sin(ph));
+                    for (int c = 0; c < 3; ++c) e[c] *= f;
+                }
+                for (int c = 0; c < 3; ++c) E[3 * j + c] += e[c];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Krylov solvers on A (left Jacobi preconditioner M = diag(self_coef)).
+// ---------------------------------------------------------------------------
+static void jacobi(const Op& op, const cplx* self_coef, const cplx* src, cplx* dst) {
+    for (int i = 0; i < op.N; ++i) {
+        const cplx sc = self_coef[i];
+        const double den = std::norm(sc);
+        const cplx inv = den > 1e-30 ? cplx(sc.real() / den, -sc.imag() / den) : cplx(1.0, 0.0);
+        for (int c = 0; c < 3; ++c) dst[3 * i + c] = src[3 * i + c] * inv;
+    }
+}
+static double vnorm(const std::vector<cplx>& a) {
+    double s = 0;
+    for (auto& v : a) s += std::norm(v);
+    return std::sqrt(s);
+}
+static cplx vdot(const cplx* a, const cplx* b, size_t n) {  // conj(a) . b
+    cplx s = 0;
+    for (size_t i = 0; i < n; ++i) s += std::conj(a[i]) * b[i];
+    return s;
+}
+static cplx vdotu(const cplx* a, const cplx* b, size_t n) {  // a . b
+    cplx s = 0;
+    for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+struct SolveReport {
+    int iters;        // operator applies inside the Krylov loop (reference counting)
+    int matvecs;      // all operator applies
+    int converged;
+    double relres_pre;   // ||M^-1 (b - A x)|| / ||b|| (reference criterion)
+    double relres_true;  // ||b - A x|| / ||b||
+};
+
+// gmres_solver.cu:56-208
+static void gmres(const Op& op, const cplx* sc, const cplx* b, cplx* x, double tol, int restart,
+                  int maxiter, bool use_jacobi, SolveReport& rep) {
+    const size_t n = 3 * (size_t)op.N;
+    std::vector<cplx> bb(b, b + n), r(n), w(n), t(n);
+    rep = SolveReport{0, 0, 0, 0, 0};
+    const double bnorm = vnorm(bb);
+    if (bnorm == 0.0) {
+        std::fill(x, x + n, cplx(0, 0));
+        rep.converged = 1;
+        return;
+    }
+    const int m = (int)std::min<size_t>(n, restart);
+    std::vector<cplx> V((size_t)(m + 1) * n);
+    std::vector<std::vector<cplx>> H(m + 1, std::vector<cplx>(m, 0.0));
+    std::vector<cplx> g(m + 1), s(m);
+    std::vector<double> c(m);
+    int iter = 0;
+    while (iter < maxiter) {
+        apply(op, sc, x, r.data(), 3);
+        rep.matvecs++;
+        for (size_t q = 0; q < n; ++q) r[q] = bb[q] - r[q];
+        if (use_jacobi) { jacobi(op, sc, r.data(), t.data()); r = t; }
+        const double rn = vnorm(r);
+        if (rn < tol * bnorm) { rep.converged = 1; break; }
+        g.assign(m + 1, 0.0);
+        g[0] = rn;
+        for (size_t q = 0; q < n; ++q) V[q] = r[q] / rn;
+        int k = 0;
+        for (k = 0; k < m && iter < maxiter; ++k, ++iter) {
+            cplx* Vk = &V[(size_t)k * n];
+            cplx* Vk1 = &V[(size_t)(k + 1) * n];
+            apply(op, sc, Vk, w.data(), 3);
+            rep.matvecs++;
+            if (use_jacobi) { jacobi(op, sc, w.data(), t.data()); w = t; }
+            for (int i = 0; i <= k; ++i) {
+                const cplx* Vi = &V[(size_t)i * n];
+                H[i][k] = vdot(Vi, w.data(), n);
+                for (size_t q = 0; q < n; ++q) w[q] -= H[i][k] * Vi[q];
+            }
+            const double wn = vnorm(w);
+            H[k + 1][k] = wn;
+            if (wn > 0.0)
+                for (size_t q = 0; q < n; ++q) Vk1[q] = w[q] / wn;
+            for (int i = 0; i < k; ++i) {
+                const cplx h1 = H[i][k], h2 = H[i + 1][k];
+                H[i][k] = c[i] * h1 + std::conj(s[i]) * h2;
+                H[i + 1][k] = -s[i] * h1 + c[i] * h2;
+            }
+            const cplx h1 = H[k][k], h2 = H[k + 1][k];
+            double cr = 1.0;
+            cplx sr = 0.0;
+            if (h2 != 0.0) {
+                const double a1 = std::abs(h1);
+                if (a1 == 0.0) {
+                    cr = 0.0;
+                    sr = 1.0;
+                } else {
+                    const double scl = a1 + std::abs(h2);
+                    const double n1 = std::abs(h1 / scl), n2 = std::abs(h2 / scl);
+                    const double hyp = scl * std::sqrt(n1 * n1 + n2 * n2);
+                    cr = a1 / hyp;
+                    sr = (std::conj(h1) / a1) * (h2 / hyp);
+                }
+            }
+            c[k] = cr;
+            s[k] = sr;
+            H[k][k] = cr * h1 + std::conj(sr) * h2;
+            H[k + 1][k] = 0.0;
+            const cplx g1 = g[k], g2 = g[k + 1];
+            g[k] = cr * g1 + std::conj(sr) * g2;
+            g[k + 1] = -sr * g1 + cr * g2;
+            if (std::abs(g[k + 1]) < tol * bnorm) {
+                k++;
+                iter++;
+                break;
+            }
+        }
+        std::vector<cplx> yv(k);
+        for (int i = k - 1; i >= 0; --i) {
+            yv[i] = g[i];
+            for (int j = i + 1; j < k; ++j) yv[i] -= H[i][j] * yv[j];
+            yv[i] /= H[i][i];
+        }
+        for (int i = 0; i < k; ++i)
+            for (size_t q = 0; q < n; ++q) x[q] += yv[i] * V[(size_t)i * n + q];
+    }
+    rep.iters = iter;
+    // final residuals (diagnostic only)
+    apply(op, sc, x, r.data(), 3);
+    for (size_t q = 0; q < n; ++q) r[q] = bb[q] - r[q];
+    rep.relres_true = vnorm(r) / bnorm;
+    jacobi(op, sc, r.data(), t.data());
+    rep.relres_pre = vnorm(t) / bnorm;
+    if (rep.relres_pre < tol) rep.converged = 1;
+}
+
+// bicgstab_solver.cu:58-192 (unconjugated rho and r0~.v; conjugated omega).
+static void bicgstab(const Op& op, const cplx* sc, const cplx* b, cplx* x, double tol, int maxiter,
+                     bool use_jacobi, SolveReport& rep) {
+    const size_t n = 3 * (size_t)op.N;
+    std::vector<cplx> bb(b, b + n), r(n), r0(n), p(n), v(n, 0.0), s(n), t(n), tmp(n);
+    rep = SolveReport{0, 0, 0, 0, 0};
+    const double bnorm = vnorm(bb);
+    if (bnorm == 0.0) {
+        std::fill(x, x + n, cplx(0, 0));
+        rep.converged = 1;
+        return;
+    }
+    auto Aprec = [&](const cplx* in, std::vector<cplx>& out) {
+        apply(op, sc, in, out.data(), 3);
+        rep.matvecs++;
+        if (use_jacobi) { jacobi(op, sc, out.data(), tmp.data()); out = tmp; }
+    };
+    apply(op, sc, x, tmp.data(), 3);
+    rep.matvecs++;
+    for (size_t q = 0; q < n; ++q) r[q] = bb[q] - tmp[q];
+    if (use_jacobi) { jacobi(op, sc, r.data(), tmp.data()); r = tmp; }
+    double rn = vnorm(r);
+    int iter = 0;
+    if (!(rn < tol * bnorm)) {
+        r0 = r;
+        p = r;
+        cplx rho = 1.0, alpha = 1.0, omega = 1.0;
+        for (iter = 0; iter < maxiter; ++iter) {
+            const cplx rho_new = vdotu(r0.data(), r.data(), n);
+            if (std::abs(rho_new) < 1e-16) break;
+            if (iter > 0) {
+                const cplx beta = (rho_new / rho) * (alpha / omega);
+                for (size_t q = 0; q < n; ++q) p[q] = (p[q] - omega * v[q]) * beta + r[q];
+            }
+            rho = rho_new;
+            Aprec(p.data(), v);
+            const cplx rv = vdotu(r0.data(), v.data(), n);
+            if (std::abs(rv) < 1e-16) break;
+            alpha = rho / rv;
+            for (size_t q = 0; q < n; ++q) s[q] = r[q] - alpha * v[q];
+            if (vnorm(s) < tol * bnorm) {
+                for (size_t q = 0; q < n; ++q) x[q] += alpha * p[q];
+                iter++;
+                break;
+            }
+            Aprec(s.data(), t);
+            const cplx ts = vdot(t.data(), s.data(), n);
+            const cplx tt = vdot(t.data(), t.data(), n);
+            if (std::abs(tt) < 1e-16) { iter++; break; }
+            omega = ts / tt.real();
+            for (size_t q = 0; q < n; ++q) x[q] += alpha * p[q] + omega * s[q];
+            for (size_t q = 0; q < n; ++q) r[q] = s[q] - omega * t[q];
+            if (vnorm(r) < tol * bnorm) { iter++; break; }
+            if (std::abs(omega) < 1e-16) { iter++; break; }
+        }
+    }
+    rep.iters = iter;
+    apply(op, sc, x, r.data(), 3);
+    for (size_t q = 0; q < n; ++q) r[q] = bb[q] - r[q];
+    rep.relres_true = vnorm(r) / bnorm;
+    jacobi(op, sc, r.data(), tmp.data());
+    rep.relres_pre = vnorm(tmp) / bnorm;
+    rep.converged = rep.relres_pre < tol ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI for the Python test harness (oracle/oracle.py).
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int orc_fft3d(double* data, int nx, int ny, int nz, int sign) {
+    fft::fft3d(reinterpret_cast<cplx*>(data), nx, ny, nz, sign);
+    return 0;
+}
+
+int orc_sizing(const double* L, double errortol, double xi, const int* grid_override,
+               int P_override, int* n_out, double* h_out, double* eta_out, int* P_out,
+               int* width_out, double* rc_out) {
+    Sizing S;
+    std::string err;
+    if (compute_sizing(L, errortol, xi, grid_override, P_override, S, err)) return -1;
+    for (int d = 0; d < 3; ++d) {
+        n_out[d] = S.n[d];
+        h_out[d] = S.h[d];
+        eta_out[d] = S.eta[d];
+    }
+    *P_out = S.P;
+    *width_out = S.width;
+    *rc_out = S.rc;
+    return 0;
+}
+
+int orc_tables(double xi, double* r, double* perp, double* para, double* self_perp) {
+    DipoleTables T;
+    build_dipole_tables(xi, T);
+    std::memcpy(r, T.r.data(), T.r.size() * sizeof(double));
+    std::memcpy(perp, T.perp.data(), T.perp.size() * sizeof(double));
+    std::memcpy(para, T.para.data(), T.para.size() * sizeof(double));
+    *self_perp = T.self_perp;
+    return (int)T.r.size();
+}
+
+void orc_pair_functions(const double* r, int n, double xi, double* perp, double* para) {
+    for (int i = 0; i < n; ++i) dipole_pair_functions(r[i], xi, perp[i], para[i]);
+}
+
+void* orc_op_create(const double* L, double errortol, double xi, const int* grid_override,
+                    int P_override, char* errbuf, int errlen) {
+    auto op = new Op();
+    for (int d = 0; d < 3; ++d) op->L[d] = L[d];
+    op->errortol = errortol;
+    op->xi = xi;
+    std::string err;
+    if (compute_sizing(L, errortol, xi, grid_override, P_override, op->S, err)) {
+        if (errbuf && errlen > 0) std::snprintf(errbuf, errlen, "%s", err.c_str());
+        delete op;
+        return nullptr;
+    }
+    build_dipole_tables(xi, op->T);
+    compute_scale(*op);
+    return op;
+}
+
+void orc_op_destroy(void* h) { delete static_cast<Op*>(h); }
+
+void orc_op_set_bloch(void* h, double kx, double ky) {
+    Op& op = *static_cast<Op*>(h);
+    op.kbx = kx;
+    op.kby = ky;
+    compute_scale(op);
+}
+
+int orc_op_set_positions(void* h, const double* xyz, int N) {
+    Op& op = *static_cast<Op*>(h);
+    op.N = N;
+    op.x.resize(N);
+    op.y.resize(N);
+    op.z.resize(N);
+    for (int i = 0; i < N; ++i) {
+        op.x[i] = xyz[3 * i];
+        op.y[i] = xyz[3 * i + 1];
+        op.z[i] = xyz[3 * i + 2];
+    }
+    build_neighbors(op);
+    build_stencils(op);
+    return op.nb_start[N];
+}
+
+void orc_op_info(void* h, int* n, double* h_out, double* eta, int* P, int* width, double* rc,
+                 double* self_perp, long* num_pairs) {
+    Op& op = *static_cast<Op*>(h);
+    for (int d = 0; d < 3; ++d) {
+        n[d] = op.S.n[d];
+        h_out[d] = op.S.h[d];
+        eta[d] = op.S.eta[d];
+    }
+    *P = op.S.P;
+    *width = op.S.width;
+    *rc = op.S.rc;
+    *self_perp = op.T.self_perp;
+    *num_pairs = op.N ? op.nb_start[op.N] : 0;
+}
+
+void orc_op_apply(void* h, const double* self_coef, const double* p, double* E, int parts) {
+    const Op& op = *static_cast<Op*>(h);
+    apply(op, reinterpret_cast<const cplx*>(self_coef), reinterpret_cast<const cplx*>(p),
+          reinterpret_cast<cplx*>(E), parts);
+}
+
+// method: 0 = GMRES(restart), 1 = BiCGSTAB.  x holds x0 on entry.
+// report: [iters, matvecs, converged, relres_pre, relres_true] as doubles.
+int orc_solve(void* h, int method, const double* self_coef, const double* b, double* x,
+              double tol, int restart, int maxiter, int use_jacobi, double* report) {
+    const Op& op = *static_cast<Op*>(h);
+    SolveReport rep;
+    const cplx* sc = reinterpret_cast<const cplx*>(self_coef);
+    if (method == 0)
+        gmres(op, sc, reinterpret_cast<const cplx*>(b), reinterpret_cast<cplx*>(x), tol, restart,
+              maxiter, use_jacobi != 0, rep);
+    else
+        bicgstab(op, sc, reinterpret_cast<const cplx*>(b), reinterpret_cast<cplx*>(x), tol, maxiter,
+                 use_jacobi != 0, rep);
+    report[0] = rep.iters;
+    report[1] = rep.matvecs;
+    report[2] = rep.converged;
+    report[3] = rep.relres_pre;
+    report[4] = rep.relres_true;
+    return 0;
+}
+
+int orc_num_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"

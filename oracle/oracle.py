@@ -1,0 +1,297 @@
+"""CPU oracle for the cuMPM hot path — TEST INFRASTRUCTURE ONLY.
+
+Python harness over ``oracle/lib/liboracle_mpm.so`` (built from
+``oracle/mpm_oracle.cpp`` by ``oracle/Makefile``).  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import this module, and only as the checker of the HIP product path.
+
+Besides thin wrappers over the C++ operator and Krylov solvers, this module
+restates the reference's ``Dipole_Solver`` driver
+(src/cpp/dipole_solver.cu:131-607): nondimensionalisation (:184-205), volume
+fraction (:207-221), the mean-field / previous / derivative initial guesses
+(:245-346), the frequency x polarisation loops (:361-489) and the physical
+rescaling of the outputs (:609-647).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "lib", "liboracle_mpm.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build():
+    """Compile the oracle library with its Makefile (g++ + OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_fft3d.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_sizing.argtypes = [_dp, ctypes.c_double, ctypes.c_double, _ip, ctypes.c_int,
+                                 _ip, _dp, _dp, _ip, _ip, _dp]
+        L.orc_tables.argtypes = [ctypes.c_double, _dp, _dp, _dp, _dp]
+        L.orc_pair_functions.argtypes = [_dp, ctypes.c_int, ctypes.c_double, _dp, _dp]
+        L.orc_op_create.restype = ctypes.c_void_p
+        L.orc_op_create.argtypes = [_dp, ctypes.c_double, ctypes.c_double, _ip, ctypes.c_int,
+                                    ctypes.c_char_p, ctypes.c_int]
+        L.orc_op_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_op_set_bloch.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.orc_op_set_positions.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int]
+        L.orc_op_info.argtypes = [ctypes.c_void_p, _ip, _dp, _dp, _ip, _ip, _dp, _dp,
+                                  ctypes.POINTER(ctypes.c_long)]
+        L.orc_op_apply.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_int]
+        L.orc_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double,
+                                ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_num_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def fft3d(a, sign=-1):
+    """Unnormalised 3-D DFT of a complex (nx, ny, nz) array (sign -1 forward)."""
+    a = np.ascontiguousarray(a, dtype=np.complex128).copy()
+    nx, ny, nz = a.shape
+    lib().orc_fft3d(_d(a.view(np.float64)), nx, ny, nz, int(sign))
+    return a
+
+
+def sizing(box, errortol, xi, grid=None, P=None):
+    """Ewald sizing (monodisperse_ewald_electric_field.cu:546-601)."""
+    L = np.asarray(box, dtype=np.float64)
+    n = np.zeros(3, np.int32)
+    h = np.zeros(3)
+    eta = np.zeros(3)
+    Pout = ctypes.c_int()
+    width = ctypes.c_int()
+    rc = ctypes.c_double()
+    g = np.zeros(3, np.int32) if grid is None else np.asarray(grid, np.int32)
+    rcode = lib().orc_sizing(_d(L), float(errortol), float(xi), g.ctypes.data_as(_ip),
+                             int(P or 0), n.ctypes.data_as(_ip), _d(h), _d(eta),
+                             ctypes.byref(Pout), ctypes.byref(width), ctypes.byref(rc))
+    if rcode != 0:
+        raise RuntimeError("Real space cutoff larger than half the box length.")
+    return dict(grid=tuple(int(v) for v in n), h=h, eta=eta, P=Pout.value, width=width.value,
+                rc=rc.value)
+
+
+def tables(xi):
+    """Dipole real-space tables (r, perp, para) and the self term."""
+    n = 9001
+    r = np.zeros(n)
+    pe = np.zeros(n)
+    pa = np.zeros(n)
+    sp = ctypes.c_double()
+    lib().orc_tables(float(xi), _d(r), _d(pe), _d(pa), ctypes.byref(sp))
+    return r, pe, pa, sp.value
+
+
+def pair_functions(r, xi):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    pe = np.zeros_like(r)
+    pa = np.zeros_like(r)
+    lib().orc_pair_functions(_d(r), r.size, float(xi), _d(pe), _d(pa))
+    return pe, pa
+
+
+class Operator:
+    """E = A p for the monodisperse Ewald operator (radius 1, SOLVER_AX mode)."""
+
+    def __init__(self, box, errortol, xi, grid=None, P=None):
+        L = np.asarray(box, dtype=np.float64)
+        g = np.zeros(3, np.int32) if grid is None else np.asarray(grid, np.int32)
+        err = ctypes.create_string_buffer(256)
+        self._h = lib().orc_op_create(_d(L), float(errortol), float(xi), g.ctypes.data_as(_ip),
+                                      int(P or 0), err, 256)
+        if not self._h:
+            raise RuntimeError(err.value.decode())
+        self.N = 0
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.orc_op_destroy(self._h)
+            self._h = None
+
+    def set_bloch(self, kx, ky):
+        lib().orc_op_set_bloch(self._h, float(kx), float(ky))
+
+    def set_positions(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        self.N = pos.shape[0]
+        return lib().orc_op_set_positions(self._h, _d(pos), self.N)
+
+    def info(self):
+        n = np.zeros(3, np.int32)
+        h = np.zeros(3)
+        eta = np.zeros(3)
+        P = ctypes.c_int()
+        w = ctypes.c_int()
+        rc = ctypes.c_double()
+        sp = ctypes.c_double()
+        npairs = ctypes.c_long()
+        lib().orc_op_info(self._h, n.ctypes.data_as(_ip), _d(h), _d(eta), ctypes.byref(P),
+                          ctypes.byref(w), ctypes.byref(rc), ctypes.byref(sp),
+                          ctypes.byref(npairs))
+        return dict(grid=tuple(int(v) for v in n), h=h, eta=eta, P=P.value, width=w.value,
+                    rc=rc.value, self_perp=sp.value, num_pairs=npairs.value)
+
+    def apply(self, p, self_coef, parts=3):
+        """p: (N, 3) complex; self_coef: (N,) complex. parts: 1 real+self, 2 recip, 3 both."""
+        p = np.ascontiguousarray(p, dtype=np.complex128).reshape(self.N, 3)
+        sc = np.ascontiguousarray(np.broadcast_to(self_coef, (self.N,)), dtype=np.complex128)
+        E = np.zeros((self.N, 3), np.complex128)
+        lib().orc_op_apply(self._h, _d(sc.view(np.float64)), _d(p.view(np.float64)),
+                           _d(E.view(np.float64)), int(parts))
+        return E
+
+    def solve(self, b, self_coef, x0=None, tol=1e-6, method="gmres", restart=50, maxiter=1000,
+              jacobi=True):
+        b = np.ascontiguousarray(b, dtype=np.complex128).reshape(-1)
+        x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, np.complex128).reshape(-1).copy()
+        sc = np.ascontiguousarray(np.broadcast_to(self_coef, (self.N,)), dtype=np.complex128)
+        rep = np.zeros(5)
+        lib().orc_solve(self._h, 0 if method == "gmres" else 1, _d(sc.view(np.float64)),
+                        _d(b.view(np.float64)), _d(x.view(np.float64)), float(tol), int(restart),
+                        int(maxiter), int(bool(jacobi)), _d(rep))
+        report = dict(iters=int(rep[0]), matvecs=int(rep[1]), converged=bool(rep[2]),
+                      relres_pre=float(rep[3]), relres_true=float(rep[4]))
+        return x, report
+
+
+def self_coefficient(eps, radius=1.0):
+    """-3 / (4 pi (1 - eps) a^3) with the |1-eps| < 1e-12 guard (dipole_solver.cu:453-460)."""
+    eps = np.asarray(eps, dtype=np.complex128)
+    diff = 1.0 - eps
+    diff = np.where(np.abs(diff) < 1e-12, 1e-12, diff)
+    return -3.0 / (4.0 * np.pi * diff) / np.asarray(radius, dtype=np.float64) ** 3
+
+
+class DipoleSolver:
+    """Restatement of Dipole_Solver for one or more frames (monodisperse Ewald, dipoles).
+
+    eps_p: (W, N) complex in the medium's units (divided by eps_m here);
+    E0: (K, 3) or (K, 3N) incident fields.  Physical outputs as the reference.
+    """
+
+    def __init__(self, box, eps_p, radius, eps_m=1.0, xi=0.5, tol=1e-3, guess_type="derivative",
+                 solver_type="gmres", E0=None, restart=50, maxiter=1000, jacobi=True,
+                 grid=None, P=None, ewald_tol=None):
+        self.box = np.asarray(box, np.float64)
+        self.eps_p = np.asarray(eps_p, np.complex128)
+        self.radius = float(radius)
+        self.eps_m = float(eps_m)
+        self.xi = xi
+        self.tol = tol
+        self.guess_type = guess_type
+        self.solver_type = solver_type
+        self.E0 = np.eye(3, dtype=np.complex128) if E0 is None else np.asarray(E0, np.complex128)
+        self.restart = restart
+        self.maxiter = maxiter
+        self.jacobi = jacobi
+        self.grid = grid
+        self.P = P
+        self.ewald_tol = ewald_tol
+        self.op = None
+        self.dips = []       # per frame (W, N, K, 3)
+        self.alpha = []      # per frame (W, K, 3)
+        self.reports = []
+
+    def _setup(self, N):
+        W = self.eps_p.shape[0] if self.eps_p.ndim >= 1 else 1
+        eps = self.eps_p.reshape(W, -1) if self.eps_p.ndim else self.eps_p.reshape(1, 1)
+        if eps.shape[1] == 1:
+            eps = np.repeat(eps, N, axis=1)
+        self.eps = eps / self.eps_m
+        self.W = eps.shape[0]
+        self.length_scale = self.radius
+        self.L = self.box / self.length_scale
+        self.a = 1.0
+        self.vol_frac = 4.0 / 3.0 * np.pi * N * self.a ** 3 / np.prod(self.L)
+        self.op = Operator(self.L, self.ewald_tol or self.tol, self.xi, self.grid, self.P)
+
+    def _rhs(self, N):
+        K = self.E0.shape[0]
+        out = np.zeros((K, N, 3), np.complex128)
+        for k in range(K):
+            e = self.E0[k]
+            out[k] = np.broadcast_to(e, (N, 3)) if e.size == 3 else e.reshape(N, 3)
+        return out
+
+    def _mean_field(self, w, N):
+        K = self.E0.shape[0]
+        beta = (self.eps[w] - 1.0) / (self.eps[w] + 2.0)
+        val = 4.0 * np.pi * self.a ** 3 * beta / (1.0 - beta * self.vol_frac)
+        rhs = self._rhs(N)  # (K, N, 3)
+        return val[None, :, None] * rhs
+
+    def _guess(self, w, prev, N):
+        if self.guess_type in ("mean_field", "mean-field") or w < 1:
+            return self._mean_field(w, N)
+        if self.guess_type == "previous" or w < 2:
+            return prev[w - 1].copy()
+        if self.guess_type != "derivative":
+            raise RuntimeError(f"Guess type {self.guess_type} not supported.")
+        X = np.zeros_like(self.eps)
+        ok = self.eps != 1.0
+        X[ok] = (self.eps[ok] + 2.0) / (self.eps[ok] - 1.0)
+        run = X[w - 1] - X[w - 2]
+        new_run = X[w] - X[w - 1]
+        small = np.abs(run) < 1e-12
+        run = np.where(small, 1.0, run)
+        new_run = np.where(small, 0.0, new_run)
+        rise = prev[w - 1] - prev[w - 2]
+        return prev[w - 1] + (new_run / run)[None, :, None] * rise
+
+    def compute(self, positions):
+        pos = np.asarray(positions, np.float64).reshape(-1, 3)
+        N = pos.shape[0]
+        if self.op is None:
+            self._setup(N)
+        self.op.set_positions(pos / self.length_scale)
+        K = self.E0.shape[0]
+        rhs = self._rhs(N)
+        frame = np.zeros((self.W, K, N, 3), np.complex128)
+        reps = []
+        for w in range(self.W):
+            sc = self_coefficient(self.eps[w], self.a)
+            if w == 0 and self.dips:
+                guess = np.transpose(self.dips[-1][0], (1, 0, 2))  # previous frame, omega 0
+            else:
+                guess = self._guess(w, frame, N)
+            for k in range(K):
+                x, rep = self.op.solve(rhs[k].reshape(-1), sc, guess[k].reshape(-1), self.tol,
+                                       self.solver_type, self.restart, self.maxiter, self.jacobi)
+                frame[w, k] = x.reshape(N, 3)
+                reps.append(rep)
+        dips = np.transpose(frame, (0, 2, 1, 3))  # (W, N, K, 3)
+        self.dips.append(dips)
+        self.alpha.append(dips.mean(axis=1))
+        self.reports.append(reps)
+
+    def get_eff_polarizability(self, physical=True):
+        f = self.eps_m * self.length_scale ** 3 if physical else 1.0
+        return np.squeeze(np.mean(np.stack(self.alpha), axis=0) * f)
+
+    def get_dipoles(self, physical=True):
+        f = self.eps_m * self.length_scale ** 3 if physical else 1.0
+        return np.squeeze(np.stack(self.dips) * f)

@@ -1,0 +1,98 @@
+"""Independent closed-form checks used to pin the oracle (not a restatement of it).
+
+* ``point_dipole_field`` — the classic Ewald lattice sum of point dipoles with
+  tin-foil (conducting) boundary conditions, written from the textbook
+  formulas (erfc real-space kernel, Gaussian-screened reciprocal sum, dipole
+  self-correction 4 alpha^3 / (3 sqrt(pi))).  For non-overlapping unit spheres
+  the reference operator's off-diagonal blocks are minus this field, and its
+  diagonal is self_coef + 1/(4 pi) minus the field of a particle's own images.
+* ``overlap_correction`` — the volume-averaged field of a uniformly polarised
+  unit sphere on an overlapping unit sphere (centre distance 1 < r < 2), minus
+  the point-dipole field, by 1-D quadrature of the analytic angular integrals.
+
+Units: the field of a dipole p at distance r is (3 r^(r^.p) - p) / (4 pi r^3).
+"""
+import itertools
+
+import numpy as np
+from scipy.integrate import quad
+from scipy.special import erfc
+
+
+def point_dipole_field(pos, p, box, alpha=None, real_images=3, kmax_factor=6.5):
+    """Field at every particle from all *other* dipoles and every periodic image
+    (own images included, own dipole excluded).  pos (N,3), p (N,3) complex."""
+    pos = np.asarray(pos, float)
+    p = np.asarray(p, complex)
+    L = np.asarray(box, float)
+    N = len(pos)
+    V = np.prod(L)
+    if alpha is None:
+        alpha = 5.5 / L.min()
+    E = np.zeros((N, 3), complex)
+    rng = range(-real_images, real_images + 1)
+    for i in range(N):
+        for j in range(N):
+            for n in itertools.product(rng, rng, rng):
+                R = pos[i] - pos[j] + np.array(n) * L
+                d = np.linalg.norm(R)
+                if i == j and d == 0.0:
+                    continue
+                ar = alpha * d
+                g = 2.0 * ar / np.sqrt(np.pi) * np.exp(-ar * ar)
+                B = (erfc(ar) + g) / d ** 3
+                C = (3.0 * erfc(ar) + g * (3.0 + 2.0 * ar * ar)) / d ** 5
+                E[i] += -B * p[j] + C * np.dot(R, p[j]) * R
+    # reciprocal space, k != 0
+    kmax = kmax_factor * alpha * 2.0
+    m = [int(np.ceil(kmax * L[d] / (2 * np.pi))) for d in range(3)]
+    ks = []
+    for a in range(-m[0], m[0] + 1):
+        for b in range(-m[1], m[1] + 1):
+            for c in range(-m[2], m[2] + 1):
+                if a == 0 and b == 0 and c == 0:
+                    continue
+                k = 2 * np.pi * np.array([a, b, c]) / L
+                k2 = k @ k
+                if k2 > kmax * kmax:
+                    continue
+                ks.append(k)
+    ks = np.array(ks)
+    k2 = np.sum(ks * ks, axis=1)
+    w = 4 * np.pi / V * np.exp(-k2 / (4 * alpha * alpha)) / k2
+    for i in range(N):
+        for j in range(N):
+            ph = np.cos(ks @ (pos[i] - pos[j]))
+            kp = ks @ p[j]
+            E[i] += -np.sum((w * ph * kp)[:, None] * ks, axis=0)
+        E[i] += 4.0 * alpha ** 3 / (3.0 * np.sqrt(np.pi)) * p[i]
+    return E / (4 * np.pi)
+
+
+def overlap_correction(r):
+    """(E_avg - E_dip) for unit spheres at centre distance r in (1, 2).
+
+    Returns (perp, para): the field component along p for p perpendicular /
+    parallel to the line of centres, per unit dipole moment."""
+    Vs = 4.0 * np.pi / 3.0
+
+    def lens_u0(rho):
+        return (rho * rho + r * r - 1.0) / (2.0 * rho * r)
+
+    def integrand(rho, which):
+        u0 = max(-1.0, min(1.0, lens_u0(rho)))
+        du = 1.0 - u0
+        # int_{u0}^{1} u^2 du
+        u2 = (1.0 - u0 ** 3) / 3.0
+        uniform = -du / (4 * np.pi)                     # interior field -p/(4 pi)
+        if which == "para":                              # 3u^2 - 1
+            ang = (3.0 * u2 - du)
+        else:                                            # (3/2)(1-u^2) - 1
+            ang = 1.5 * (du - u2) - du
+        dip = -ang / (4 * np.pi * rho ** 3)
+        return 2 * np.pi * rho * rho * (uniform + dip)
+
+    lo = r - 1.0
+    perp = quad(integrand, lo, 1.0, args=("perp",), epsabs=1e-14, epsrel=1e-13, limit=200)[0] / Vs
+    para = quad(integrand, lo, 1.0, args=("para",), epsabs=1e-14, epsrel=1e-13, limit=200)[0] / Vs
+    return perp, para
