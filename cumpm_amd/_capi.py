@@ -1,0 +1,195 @@
+"""ctypes bindings of libcumpm_amd.so (declarations in include/cumpm_amd.h).
+
+The library is loaded from ``cumpm_amd/lib/`` (built in-tree by
+``cumpm_amd/build.py``).  There is no CPU fallback: if the library is missing
+or fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libcumpm_amd.so")
+
+MPM_OK = 0
+MPM_ERR_INVALID_ARGUMENT = 1
+MPM_ERR_RUNTIME = 2
+MPM_ERR_HIP = 3
+MPM_ERR_FFT = 4
+MPM_ERR_UNSUPPORTED = 5
+MPM_ERR_OUT_OF_MEMORY = 6
+
+PREC_DOUBLE, PREC_MIXED, PREC_AUTO = 0, 1, 2
+SOLVE_DOUBLE, SOLVE_MIXED, SOLVE_REFINED = 0, 1, 2
+GMRES, BICGSTAB = 0, 1
+PART_REAL, PART_RECIP, PART_ALL = 1, 2, 3
+GUESS_DERIVATIVE, GUESS_MEAN_FIELD, GUESS_PREVIOUS = 0, 1, 2
+FIELD_AUTO, FIELD_MONODISPERSE, FIELD_POLYDISPERSE, FIELD_DIRECT = 0, 1, 2, 3
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64 = ctypes.c_int64
+
+
+class OpDesc(ctypes.Structure):
+    _fields_ = [("box", ctypes.c_double * 3), ("errortol", ctypes.c_double), ("xi", ctypes.c_double),
+                ("grid", ctypes.c_int * 3), ("support", ctypes.c_int)]
+
+
+class OpInfo(ctypes.Structure):
+    _fields_ = [("num_particles", _i64), ("num_pairs", _i64), ("grid", ctypes.c_int * 3),
+                ("support", ctypes.c_int), ("stencil_width", ctypes.c_int),
+                ("grid_spacing", ctypes.c_double * 3), ("spectral_split", ctypes.c_double * 3),
+                ("rc", ctypes.c_double), ("self_perp", ctypes.c_double),
+                ("self_coef_ewald", ctypes.c_double), ("bloch", ctypes.c_double * 2)]
+
+
+class SolveDesc(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("restart", ctypes.c_int), ("maxiter", ctypes.c_int),
+                ("tol", ctypes.c_double), ("jacobi", ctypes.c_int), ("precision", ctypes.c_int)]
+
+
+class SolveReport(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_int), ("matvecs", ctypes.c_int), ("matvecs_fp64", ctypes.c_int),
+                ("converged", ctypes.c_int), ("relres_pre", ctypes.c_double),
+                ("relres_true", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class DipoleDesc(ctypes.Structure):
+    _fields_ = [("box", ctypes.c_double * 3), ("eps_p", _dp), ("num_wavevectors", _i64),
+                ("eps_cols", _i64), ("radius", _dp), ("num_radii", _i64), ("eps_m", ctypes.c_double),
+                ("xi", ctypes.c_double), ("tol", ctypes.c_double), ("quiet", ctypes.c_int),
+                ("guess", ctypes.c_int), ("method", ctypes.c_int), ("field_type", ctypes.c_int),
+                ("E0", _dp), ("num_E0", _i64), ("E0_len", _i64), ("solve_quadrupoles", ctypes.c_int),
+                ("precision", ctypes.c_int), ("refined", ctypes.c_int), ("batched", ctypes.c_int),
+                ("restart", ctypes.c_int), ("maxiter", ctypes.c_int), ("grid", ctypes.c_int * 3),
+                ("support", ctypes.c_int), ("ewald_tol", ctypes.c_double)]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "mpm_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "mpm_last_error": (ctypes.c_char_p, []),
+    "mpm_op_create": (ctypes.c_int, [ctypes.POINTER(OpDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    "mpm_op_destroy": (None, [ctypes.c_void_p]),
+    "mpm_op_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OpInfo)]),
+    "mpm_op_set_positions": (ctypes.c_int, [ctypes.c_void_p, _dp, _i64]),
+    "mpm_op_set_bloch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    "mpm_op_set_self_coef": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
+    "mpm_op_apply": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "mpm_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolveDesc), ctypes.c_void_p,
+                                 ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(SolveReport),
+                                 ctypes.c_void_p]),
+    "mpm_dipole_solver_create": (ctypes.c_int, [ctypes.POINTER(DipoleDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    "mpm_dipole_solver_destroy": (None, [ctypes.c_void_p]),
+    "mpm_dipole_solver_compute": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _i64]),
+    "mpm_dipole_solver_get_dims": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i64)]),
+    "mpm_dipole_solver_get_eff_polarizability": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
+    "mpm_dipole_solver_get_dipoles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
+    "mpm_dipole_solver_get_reports": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolveReport)]),
+    "mpm_dipole_solver_get_op_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OpInfo)]),
+    "mpm_dipole_solver_set_use_jacobi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "mpm_dipole_solver_get_use_jacobi": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "mpm_dipole_solver_get_timing": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "mpm_buffer_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    "mpm_buffer_free": (ctypes.c_int, [ctypes.c_void_p]),
+    "mpm_buffer_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+    "mpm_buffer_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+    "mpm_device_synchronize": (ctypes.c_int, []),
+    "mpm_profile_enable": (ctypes.c_int, [ctypes.c_int]),
+    "mpm_profile_reset": (ctypes.c_int, []),
+    "mpm_profile_get": (ctypes.c_int, [ctypes.c_char_p, _dp, ctypes.POINTER(_i64)]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+_lib = None
+
+
+class MPMError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(message)
+        self.status = status
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                "or `python -m cumpm_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    """Map an mpm_status to the exception the reference raises for it."""
+    if status == MPM_OK:
+        return
+    L = load()
+    msg = (L.mpm_last_error() or b"").decode() or L.mpm_status_string(status).decode()
+    if status == MPM_ERR_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    if status == MPM_ERR_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    if status == MPM_ERR_RUNTIME:
+        raise RuntimeError(msg)
+    raise MPMError(status, f"{L.mpm_status_string(status).decode()}: {msg}")
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+class DeviceBuffer:
+    """Device allocation owned by the library (plumbing for numpy callers)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        check(load().mpm_buffer_alloc(self.nbytes, ctypes.byref(self.ptr)))
+
+    def upload(self, arr):
+        import numpy as np
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        check(load().mpm_buffer_upload(self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
+
+    def download(self, out):
+        assert out.nbytes <= self.nbytes and out.flags.c_contiguous
+        check(load().mpm_buffer_download(out.ctypes.data_as(ctypes.c_void_p), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr and _lib is not None:
+            _lib.mpm_buffer_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        self.free()
+
+
+def profile_enable(on=True):
+    check(load().mpm_profile_enable(int(bool(on))))
+
+
+def profile_reset():
+    check(load().mpm_profile_reset())
+
+
+def profile_get(name):
+    ms = ctypes.c_double()
+    cnt = _i64()
+    check(load().mpm_profile_get(name.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
+    return ms.value, cnt.value

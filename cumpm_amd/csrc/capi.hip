@@ -1,0 +1,590 @@
+// C ABI of libcumpm_amd (include/cumpm_amd.h) and the Dipole_Solver driver.
+//
+// The driver restates src/cpp/dipole_solver.cu: set_dims (:131-182),
+// nondimensionalize (:184-205), calc_vol_frac (:207-221), the mean-field /
+// previous / derivative guesses (:245-346), compute_tensor / compute_spectrum
+// (:361-489), compute (:495-607) and the physical rescaling of the getters
+// (:609-647).  Solves go through solve_batched(): in the reference order one
+// system at a time (batched = 0), or every frequency x polarisation system of
+// the frame in one batched launch (batched = 1; the derivative guess needs
+// the previous frequencies' solutions, so batched mode starts every system
+// from the mean-field guess).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+
+#include "mpm_internal.h"
+
+struct mpm_op {
+    std::unique_ptr<mpm::EwaldOp> impl;
+};
+
+namespace mpm {
+void profile_set(bool on);
+void profile_reset();
+bool profile_get(const std::string& name, double& ms, int64_t& count);
+}  // namespace mpm
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+mpm_status guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return MPM_OK;
+    } catch (const mpm::Error& e) {
+        g_last_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return MPM_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return MPM_ERR_RUNTIME;
+    }
+}
+
+using mpm::cplx;
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Driver
+// ---------------------------------------------------------------------------
+struct mpm_dipole_solver {
+    // configuration (physical until nondimensionalised)
+    std::vector<double> box;
+    std::vector<std::vector<cplx>> eps_p;  // [W][cols]
+    std::vector<double> radius;
+    double eps_m = 1.0, xi = 0.5, tol = 1e-3;
+    bool quiet = false;
+    int guess = MPM_GUESS_DERIVATIVE, method = MPM_GMRES, field_type = MPM_FIELD_AUTO;
+    std::vector<std::vector<cplx>> E0;
+    int precision = MPM_PREC_AUTO;
+    bool refined = false, batched = false, use_jacobi = true;
+    int restart = 50, maxiter = 1000;
+    int grid[3] = {0, 0, 0};
+    int support = 0;
+    double ewald_tol = 0.0;
+    // state
+    size_t N = 0, W = 0, K = 0, frames = 0;
+    double length_scale = 1.0, eps_scale = 1.0, vol_frac = 0.0;
+    std::unique_ptr<mpm::EwaldOp> op;
+    std::vector<cplx> dips;      // [F][W][N][K][3]
+    std::vector<cplx> avg_dips;  // [F][W][K][3]
+    std::vector<mpm_solve_report> reports;  // [F][W][K]
+    double solve_ms = 0, setup_ms = 0;
+    mpm::DevBuf<double2> d_b, d_x;
+    int indent = 0;
+
+    void print(const std::string& m) const {
+        if (!quiet) std::cout << std::string(indent * 2, ' ') << m << std::endl;
+    }
+
+    void set_dims(size_t n) {
+        N = n;
+        if (radius.empty())
+            radius.assign(N, 1.0);
+        else if (radius.size() == 1)
+            radius.assign(N, radius[0]);
+        else if (radius.size() != N)
+            throw mpm::Error(MPM_ERR_RUNTIME,
+                             "The number of particles is inconsistent with the number of radii provided.");
+        bool mono = true;
+        for (size_t i = 1; i < N; ++i) mono = mono && radius[i] == radius[0];
+        if (field_type == MPM_FIELD_MONODISPERSE && !mono)
+            throw mpm::Error(MPM_ERR_RUNTIME, "field_type was set to 'monodisperse' but radii are not identical.");
+        if (field_type == MPM_FIELD_POLYDISPERSE || (field_type == MPM_FIELD_AUTO && !mono))
+            throw mpm::Error(MPM_ERR_UNSUPPORTED, "the polydisperse Ewald operator is not provided by this build");
+        if (field_type == MPM_FIELD_DIRECT)
+            throw mpm::Error(MPM_ERR_UNSUPPORTED, "the direct (open-boundary) field is not provided by this build");
+        if (eps_p.empty()) throw mpm::Error(MPM_ERR_RUNTIME, "eps_p is empty.");
+        W = eps_p.size();
+        for (auto& row : eps_p) {
+            if (row.size() == 1)
+                row.assign(N, row[0]);
+            else if (row.size() != N)
+                throw mpm::Error(MPM_ERR_RUNTIME, "The number of particles is inconsistent with eps_p provided.");
+        }
+        K = E0.size();
+        for (size_t k = 0; k < K; ++k)
+            if (E0[k].size() != 3 && E0[k].size() != 3 * N)
+                throw mpm::Error(MPM_ERR_RUNTIME, "Incident field E0[" + std::to_string(k) + "] has size " +
+                                                      std::to_string(E0[k].size()) + ", but expected 3, " +
+                                                      std::to_string(3 * N) + ", or " + std::to_string(3 * N) + ".");
+    }
+
+    void nondimensionalize() {
+        length_scale = *std::min_element(radius.begin(), radius.end());
+        eps_scale = eps_m;
+        for (auto& b : box) b /= length_scale;
+        for (auto& r : radius) r /= length_scale;
+        eps_m /= eps_scale;
+        for (auto& row : eps_p)
+            for (auto& e : row) e /= eps_scale;
+        double s3 = 0.0;
+        for (double r : radius) s3 += r * r * r;
+        const double bv = box[0] * box[1] * box[2];
+        vol_frac = bv == 0.0 ? 0.0 : (4.0 / 3.0) * M_PI * s3 / bv;
+    }
+
+    // guess layout [p][K][3]
+    std::vector<cplx> mean_field(size_t w) const {
+        std::vector<cplx> g(N * K * 3, 0.0);
+        for (size_t p = 0; p < N; ++p) {
+            const cplx beta = (eps_p[w][p] - 1.0) / (eps_p[w][p] + 2.0);
+            const cplx val = 4.0 * M_PI * std::pow(radius[p], 3.0) * beta / (1.0 - beta * vol_frac);
+            for (size_t k = 0; k < K; ++k)
+                for (int c = 0; c < 3; ++c) {
+                    const cplx e = E0[k].size() > 3 ? E0[k][p * 3 + c] : E0[k][c];
+                    g[p * K * 3 + k * 3 + c] = val * e;
+                }
+        }
+        return g;
+    }
+    // frame_dip: [W][N][K][3] of the current frame
+    std::vector<cplx> guess_for(size_t w, const std::vector<cplx>& fd) const {
+        const size_t blk = N * K * 3;
+        if (guess == MPM_GUESS_MEAN_FIELD || w < 1) return mean_field(w);
+        if (guess == MPM_GUESS_PREVIOUS || w < 2)
+            return std::vector<cplx>(fd.begin() + (w - 1) * blk, fd.begin() + w * blk);
+        std::vector<cplx> g(blk);
+        auto X = [](cplx e) { return (e - 1.0 == 0.0) ? cplx(0.0) : (e + 2.0) / (e - 1.0); };
+        for (size_t p = 0; p < N; ++p) {
+            cplx run = X(eps_p[w - 1][p]) - X(eps_p[w - 2][p]);
+            cplx new_run = X(eps_p[w][p]) - X(eps_p[w - 1][p]);
+            if (std::abs(run) < 1e-12) {
+                run = 1.0;
+                new_run = 0.0;
+            }
+            for (size_t q = 0; q < K * 3; ++q) {
+                const cplx v1 = fd[(w - 1) * blk + p * K * 3 + q], v2 = fd[(w - 2) * blk + p * K * 3 + q];
+                g[p * K * 3 + q] = v1 + new_run * (v1 - v2) / run;
+            }
+        }
+        return g;
+    }
+
+    std::vector<cplx> self_coef(size_t w) const {
+        std::vector<cplx> sc(N);
+        for (size_t p = 0; p < N; ++p) {
+            cplx diff = 1.0 - eps_p[w][p];
+            if (std::abs(diff) < 1e-12) diff = 1e-12;
+            sc[p] = -3.0 / (4.0 * M_PI * diff) / std::pow(radius[p], 3.0);
+        }
+        return sc;
+    }
+
+    std::vector<cplx> rhs(size_t k) const {
+        std::vector<cplx> e(3 * N);
+        for (size_t p = 0; p < N; ++p)
+            for (int c = 0; c < 3; ++c) e[3 * p + c] = E0[k].size() == 3 ? E0[k][c] : E0[k][3 * p + c];
+        return e;
+    }
+
+    mpm_solve_desc solve_desc() const {
+        mpm_solve_desc d;
+        d.method = method;
+        d.restart = restart;
+        d.maxiter = maxiter;
+        d.tol = tol;
+        d.jacobi = use_jacobi ? 1 : 0;
+        if (precision == MPM_PREC_DOUBLE)
+            d.precision = MPM_SOLVE_DOUBLE;
+        else
+            d.precision = refined ? MPM_SOLVE_REFINED : MPM_SOLVE_MIXED;
+        return d;
+    }
+
+    void compute(const double* x, const double* y, const double* z, size_t n) {
+        if (n == 0) return;
+        auto t0 = std::chrono::steady_clock::now();
+        if (N == 0) {
+            set_dims(n);
+            nondimensionalize();
+            mpm_op_desc od;
+            std::memset(&od, 0, sizeof(od));
+            for (int d = 0; d < 3; ++d) {
+                od.box[d] = box[d];
+                od.grid[d] = grid[d];
+            }
+            od.errortol = ewald_tol > 0 ? ewald_tol : tol;
+            od.xi = xi;
+            od.support = support;
+            op = std::make_unique<mpm::EwaldOp>(od);
+        } else if (N != n) {
+            throw mpm::Error(MPM_ERR_RUNTIME, "The number of particles has changed!");
+        }
+        frames++;
+        std::vector<double> xyz(3 * N);
+        for (size_t i = 0; i < N; ++i) {
+            xyz[3 * i] = x[i] / length_scale;
+            xyz[3 * i + 1] = y[i] / length_scale;
+            xyz[3 * i + 2] = z[i] / length_scale;
+        }
+        op->set_positions(xyz.data(), (int64_t)N);
+        const size_t n3 = 3 * N;
+        // initial guess: mean field, or the previous frame's first frequency
+        std::vector<cplx> init;
+        if (frames == 1) {
+            init = mean_field(0);
+        } else {
+            const size_t off = (frames - 2) * W * N * K * 3;
+            init.assign(dips.begin() + off, dips.begin() + off + N * K * 3);
+        }
+        std::vector<cplx> fd(W * N * K * 3, 0.0);
+        std::vector<mpm_solve_report> reps(W * K);
+        MPM_HIP(hipDeviceSynchronize());
+        auto t1 = std::chrono::steady_clock::now();
+        const mpm_solve_desc sd = solve_desc();
+        auto store = [&](size_t w, size_t k, const std::vector<cplx>& sol) {
+            for (size_t p = 0; p < N; ++p)
+                for (int c = 0; c < 3; ++c) fd[(w * N + p) * K * 3 + k * 3 + c] = sol[3 * p + c];
+        };
+        if (!batched) {
+            print("Wavenumber:");
+            indent++;
+            d_b.reserve(n3);
+            d_x.reserve(n3);
+            for (size_t w = 0; w < W; ++w) {
+                print(std::to_string(w) + " of " + std::to_string(W));
+                auto sc = self_coef(w);
+                op->set_self_coef(reinterpret_cast<const double*>(sc.data()), 1);
+                const std::vector<cplx> g = (w == 0) ? init : guess_for(w, fd);
+                for (size_t k = 0; k < K; ++k) {
+                    auto b = rhs(k);
+                    std::vector<cplx> x0(n3);
+                    for (size_t p = 0; p < N; ++p)
+                        for (int c = 0; c < 3; ++c) x0[3 * p + c] = g[p * K * 3 + k * 3 + c];
+                    MPM_HIP(hipMemcpy(d_b.p, b.data(), n3 * sizeof(double2), hipMemcpyHostToDevice));
+                    MPM_HIP(hipMemcpy(d_x.p, x0.data(), n3 * sizeof(double2), hipMemcpyHostToDevice));
+                    mpm::solve_batched(*op, sd, d_b.p, d_x.p, 1, &reps[w * K + k], nullptr);
+                    std::vector<cplx> sol(n3);
+                    MPM_HIP(hipMemcpy(sol.data(), d_x.p, n3 * sizeof(double2), hipMemcpyDeviceToHost));
+                    store(w, k, sol);
+                }
+            }
+            indent--;
+        } else {
+            const size_t B = W * K;
+            std::vector<cplx> sc_all(W * N);
+            std::vector<cplx> b_all(B * n3), x_all(B * n3);
+            for (size_t w = 0; w < W; ++w) {
+                auto sc = self_coef(w);
+                const std::vector<cplx> g = (w == 0) ? init : mean_field(w);
+                for (size_t k = 0; k < K; ++k) {
+                    auto b = rhs(k);
+                    std::copy(b.begin(), b.end(), b_all.begin() + (w * K + k) * n3);
+                    for (size_t p = 0; p < N; ++p)
+                        for (int c = 0; c < 3; ++c) x_all[(w * K + k) * n3 + 3 * p + c] = g[p * K * 3 + k * 3 + c];
+                }
+                std::copy(sc.begin(), sc.end(), sc_all.begin() + w * N);
+            }
+            // one self-coefficient row per system (rows repeat across polarisations)
+            std::vector<cplx> rows(B * N);
+            for (size_t w = 0; w < W; ++w)
+                for (size_t k = 0; k < K; ++k)
+                    std::copy(sc_all.begin() + w * N, sc_all.begin() + (w + 1) * N, rows.begin() + (w * K + k) * N);
+            op->set_self_coef(reinterpret_cast<const double*>(rows.data()), (int)B);
+            d_b.reserve(B * n3);
+            d_x.reserve(B * n3);
+            MPM_HIP(hipMemcpy(d_b.p, b_all.data(), B * n3 * sizeof(double2), hipMemcpyHostToDevice));
+            MPM_HIP(hipMemcpy(d_x.p, x_all.data(), B * n3 * sizeof(double2), hipMemcpyHostToDevice));
+            mpm::solve_batched(*op, sd, d_b.p, d_x.p, (int)B, reps.data(), nullptr);
+            MPM_HIP(hipMemcpy(x_all.data(), d_x.p, B * n3 * sizeof(double2), hipMemcpyDeviceToHost));
+            for (size_t w = 0; w < W; ++w)
+                for (size_t k = 0; k < K; ++k)
+                    store(w, k, std::vector<cplx>(x_all.begin() + (w * K + k) * n3,
+                                                  x_all.begin() + (w * K + k + 1) * n3));
+        }
+        MPM_HIP(hipDeviceSynchronize());
+        auto t2 = std::chrono::steady_clock::now();
+        setup_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        solve_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+        // alpha_eff: mean dipole per (w, k)
+        for (size_t w = 0; w < W; ++w)
+            for (size_t k = 0; k < K; ++k)
+                for (int c = 0; c < 3; ++c) {
+                    cplx s = 0.0;
+                    for (size_t p = 0; p < N; ++p) s += fd[(w * N + p) * K * 3 + k * 3 + c];
+                    avg_dips.push_back(s / (double)N);
+                }
+        dips.insert(dips.end(), fd.begin(), fd.end());
+        reports.insert(reports.end(), reps.begin(), reps.end());
+    }
+};
+
+// ---------------------------------------------------------------------------
+// extern "C"
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* mpm_status_string(mpm_status s) {
+    switch (s) {
+        case MPM_OK: return "ok";
+        case MPM_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case MPM_ERR_RUNTIME: return "runtime error";
+        case MPM_ERR_HIP: return "HIP error";
+        case MPM_ERR_FFT: return "rocFFT error";
+        case MPM_ERR_UNSUPPORTED: return "unsupported";
+        case MPM_ERR_OUT_OF_MEMORY: return "out of memory";
+    }
+    return "unknown status";
+}
+
+const char* mpm_last_error(void) { return g_last_error.c_str(); }
+
+mpm_status mpm_op_create(const mpm_op_desc* desc, mpm_op** out) {
+    return guarded([&] {
+        MPM_REQUIRE(desc && out, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        auto h = std::make_unique<mpm_op>();
+        h->impl = std::make_unique<mpm::EwaldOp>(*desc);
+        *out = h.release();
+    });
+}
+
+void mpm_op_destroy(mpm_op* op) { delete op; }
+
+mpm_status mpm_op_get_info(const mpm_op* op, mpm_op_info* info) {
+    return guarded([&] {
+        MPM_REQUIRE(op && info, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        op->impl->info(*info);
+    });
+}
+
+mpm_status mpm_op_set_positions(mpm_op* op, const double* xyz, int64_t n) {
+    return guarded([&] {
+        MPM_REQUIRE(op && xyz, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        op->impl->set_positions(xyz, n);
+    });
+}
+
+mpm_status mpm_op_set_bloch(mpm_op* op, double kx, double ky) {
+    return guarded([&] {
+        MPM_REQUIRE(op, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        op->impl->set_bloch(kx, ky);
+    });
+}
+
+mpm_status mpm_op_set_self_coef(mpm_op* op, const double* coef, int rows) {
+    return guarded([&] {
+        MPM_REQUIRE(op && coef, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        op->impl->set_self_coef(coef, rows);
+    });
+}
+
+mpm_status mpm_op_apply(mpm_op* op, const double* x, double* y, int B, int precision, int parts,
+                        void* stream) {
+    return guarded([&] {
+        MPM_REQUIRE(op && x && y, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        MPM_REQUIRE(B >= 1, MPM_ERR_INVALID_ARGUMENT, "B must be >= 1");
+        MPM_REQUIRE(parts >= 1 && parts <= 3, MPM_ERR_INVALID_ARGUMENT, "parts must be 1, 2 or 3");
+        MPM_REQUIRE(precision >= 0 && precision <= 2, MPM_ERR_INVALID_ARGUMENT, "unknown precision");
+        auto& E = *op->impl;
+        MPM_REQUIRE(E.self_rows == 1 || E.self_rows >= B, MPM_ERR_INVALID_ARGUMENT,
+                    "need one self-coefficient row per right-hand side (or a single shared row)");
+        E.apply_contiguous(reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y), B,
+                           precision != MPM_PREC_DOUBLE, parts, static_cast<hipStream_t>(stream));
+        MPM_HIP(hipGetLastError());
+    });
+}
+
+mpm_status mpm_solve(mpm_op* op, const mpm_solve_desc* desc, const double* b, double* x, int B,
+                     mpm_solve_report* reports, void* stream) {
+    return guarded([&] {
+        MPM_REQUIRE(op && desc && b && x, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        mpm::solve_batched(*op->impl, *desc, reinterpret_cast<const double2*>(b), reinterpret_cast<double2*>(x),
+                           B, reports, static_cast<hipStream_t>(stream));
+    });
+}
+
+mpm_status mpm_dipole_solver_create(const mpm_dipole_desc* d, mpm_dipole_solver** out) {
+    return guarded([&] {
+        MPM_REQUIRE(d && out, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        MPM_REQUIRE(d->num_wavevectors >= 1 && d->eps_cols >= 1 && d->eps_p, MPM_ERR_INVALID_ARGUMENT,
+                    "eps_p is empty.");
+        MPM_REQUIRE(!d->solve_quadrupoles, MPM_ERR_UNSUPPORTED, "quadrupoles are not provided by this build");
+        MPM_REQUIRE(d->tol > 0.0, MPM_ERR_INVALID_ARGUMENT, "tol must be positive");
+        auto s = std::make_unique<mpm_dipole_solver>();
+        s->box.assign(d->box, d->box + 3);
+        s->eps_p.resize(d->num_wavevectors);
+        for (int64_t w = 0; w < d->num_wavevectors; ++w) {
+            s->eps_p[w].resize(d->eps_cols);
+            for (int64_t c = 0; c < d->eps_cols; ++c)
+                s->eps_p[w][c] = cplx(d->eps_p[2 * (w * d->eps_cols + c)], d->eps_p[2 * (w * d->eps_cols + c) + 1]);
+        }
+        if (d->num_radii > 0) {
+            MPM_REQUIRE(d->radius, MPM_ERR_INVALID_ARGUMENT, "null radius");
+            s->radius.assign(d->radius, d->radius + d->num_radii);
+            for (double r : s->radius) MPM_REQUIRE(r > 0.0, MPM_ERR_INVALID_ARGUMENT, "radii must be positive");
+        }
+        s->eps_m = d->eps_m;
+        MPM_REQUIRE(s->eps_m > 0.0, MPM_ERR_INVALID_ARGUMENT, "eps_m must be positive");
+        s->xi = d->xi;
+        s->tol = d->tol;
+        s->quiet = d->quiet != 0;
+        MPM_REQUIRE(d->guess >= 0 && d->guess <= 2, MPM_ERR_RUNTIME, "Guess type not supported.");
+        s->guess = d->guess;
+        MPM_REQUIRE(d->method == MPM_GMRES || d->method == MPM_BICGSTAB, MPM_ERR_INVALID_ARGUMENT, "unknown solver");
+        s->method = d->method;
+        MPM_REQUIRE(d->field_type >= 0 && d->field_type <= 3, MPM_ERR_RUNTIME, "Unknown field_type");
+        s->field_type = d->field_type;
+        if (d->num_E0 > 0) {
+            MPM_REQUIRE(d->E0 && d->E0_len > 0, MPM_ERR_INVALID_ARGUMENT, "null E0");
+            s->E0.resize(d->num_E0);
+            for (int64_t k = 0; k < d->num_E0; ++k)
+                for (int64_t c = 0; c < d->E0_len; ++c)
+                    s->E0[k].push_back(cplx(d->E0[2 * (k * d->E0_len + c)], d->E0[2 * (k * d->E0_len + c) + 1]));
+        } else {
+            s->E0 = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+        }
+        s->precision = d->precision;
+        s->refined = d->refined != 0;
+        s->batched = d->batched != 0;
+        if (d->restart > 0) s->restart = d->restart;
+        if (d->maxiter > 0) s->maxiter = d->maxiter;
+        for (int k = 0; k < 3; ++k) s->grid[k] = d->grid[k];
+        s->support = d->support;
+        s->ewald_tol = d->ewald_tol;
+        *out = s.release();
+    });
+}
+
+void mpm_dipole_solver_destroy(mpm_dipole_solver* s) { delete s; }
+
+mpm_status mpm_dipole_solver_compute(mpm_dipole_solver* s, const double* x, const double* y, const double* z,
+                                     int64_t n) {
+    return guarded([&] {
+        MPM_REQUIRE(s && ((x && y && z) || n == 0), MPM_ERR_INVALID_ARGUMENT, "null argument");
+        MPM_REQUIRE(n >= 0, MPM_ERR_INVALID_ARGUMENT, "negative particle count");
+        s->compute(x, y, z, (size_t)n);
+    });
+}
+
+mpm_status mpm_dipole_solver_get_dims(const mpm_dipole_solver* s, int64_t dims[4]) {
+    return guarded([&] {
+        MPM_REQUIRE(s && dims, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        dims[0] = (int64_t)s->frames;
+        dims[1] = (int64_t)(s->N ? s->W : s->eps_p.size());
+        dims[2] = (int64_t)s->N;
+        dims[3] = (int64_t)s->E0.size();
+    });
+}
+
+mpm_status mpm_dipole_solver_get_eff_polarizability(const mpm_dipole_solver* s, int physical, double* out) {
+    return guarded([&] {
+        MPM_REQUIRE(s && out, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        const size_t W = s->N ? s->W : s->eps_p.size(), K = s->E0.size();
+        const double f = physical ? s->eps_scale * std::pow(s->length_scale, 3.0) : 1.0;
+        for (size_t q = 0; q < W * K * 3; ++q) {
+            cplx acc = 0.0;
+            for (size_t fr = 0; fr < s->frames; ++fr) acc += s->avg_dips[fr * W * K * 3 + q];
+            const cplx v = s->frames ? acc / (double)s->frames * f : cplx(0.0);
+            out[2 * q] = v.real();
+            out[2 * q + 1] = v.imag();
+        }
+    });
+}
+
+mpm_status mpm_dipole_solver_get_dipoles(const mpm_dipole_solver* s, int physical, double* out) {
+    return guarded([&] {
+        MPM_REQUIRE(s && out, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        const double f = physical ? s->eps_scale * std::pow(s->length_scale, 3.0) : 1.0;
+        for (size_t q = 0; q < s->dips.size(); ++q) {
+            out[2 * q] = s->dips[q].real() * f;
+            out[2 * q + 1] = s->dips[q].imag() * f;
+        }
+    });
+}
+
+mpm_status mpm_dipole_solver_get_reports(const mpm_dipole_solver* s, mpm_solve_report* out) {
+    return guarded([&] {
+        MPM_REQUIRE(s && out, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        std::copy(s->reports.begin(), s->reports.end(), out);
+    });
+}
+
+mpm_status mpm_dipole_solver_get_op_info(const mpm_dipole_solver* s, mpm_op_info* info) {
+    return guarded([&] {
+        MPM_REQUIRE(s && info, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        MPM_REQUIRE(s->op, MPM_ERR_RUNTIME, "the operator exists after the first compute()");
+        s->op->info(*info);
+    });
+}
+
+mpm_status mpm_dipole_solver_set_use_jacobi(mpm_dipole_solver* s, int enable) {
+    return guarded([&] {
+        MPM_REQUIRE(s, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        s->use_jacobi = enable != 0;
+    });
+}
+
+mpm_status mpm_dipole_solver_get_use_jacobi(const mpm_dipole_solver* s, int* enable) {
+    return guarded([&] {
+        MPM_REQUIRE(s && enable, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        *enable = s->use_jacobi ? 1 : 0;
+    });
+}
+
+mpm_status mpm_dipole_solver_get_timing(const mpm_dipole_solver* s, double* solve_ms, double* setup_ms) {
+    return guarded([&] {
+        MPM_REQUIRE(s && solve_ms && setup_ms, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        *solve_ms = s->solve_ms;
+        *setup_ms = s->setup_ms;
+    });
+}
+
+mpm_status mpm_buffer_alloc(size_t bytes, void** dev) {
+    return guarded([&] {
+        MPM_REQUIRE(dev, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        *dev = nullptr;
+        if (bytes) MPM_HIP(hipMalloc(dev, bytes));
+    });
+}
+
+mpm_status mpm_buffer_free(void* dev) {
+    return guarded([&] {
+        if (dev) MPM_HIP(hipFree(dev));
+    });
+}
+
+mpm_status mpm_buffer_upload(void* dev, const void* host, size_t bytes) {
+    return guarded([&] {
+        if (bytes) MPM_HIP(hipMemcpy(dev, host, bytes, hipMemcpyHostToDevice));
+    });
+}
+
+mpm_status mpm_buffer_download(void* host, const void* dev, size_t bytes) {
+    return guarded([&] {
+        if (bytes) MPM_HIP(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+mpm_status mpm_device_synchronize(void) {
+    return guarded([&] { MPM_HIP(hipDeviceSynchronize()); });
+}
+
+mpm_status mpm_profile_enable(int enable) {
+    return guarded([&] { mpm::profile_set(enable != 0); });
+}
+
+mpm_status mpm_profile_reset(void) {
+    return guarded([&] { mpm::profile_reset(); });
+}
+
+mpm_status mpm_profile_get(const char* name, double* total_ms, int64_t* count) {
+    return guarded([&] {
+        MPM_REQUIRE(name && total_ms && count, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        mpm::profile_get(name, *total_ms, *count);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,958 @@
+// Monodisperse spectral-Ewald dipole operator y = A x for a batch of
+// right-hand sides on gfx950.
+//
+// A = diag(self_coef_b + self_perp) + G_real + G_recip, the operator of
+// src/cpp/electric_field/monodisperse_ewald_electric_field.cu (SOLVER_AX mode,
+// dipoles).  The arithmetic follows the reference; the data movement does not:
+//
+//  * spread  — owner-computes gather: a 256-thread workgroup owns a 4x8x8
+//              voxel tile and pulls in the particles binned around it (LDS
+//              staged), so every grid value is written exactly once, with no
+//              atomics and no separate zero-fill; the reference scatters with
+//              LDS/global atomics after a memset (:1160-1298, :2000-2048).
+//              Gaussian weights are separable, exp(-2 xi^2 sum_d gd_d^2/eta_d)
+//              = prod_d exp(-2 xi^2 gd_d^2/eta_d), so each particle carries
+//              3 x width weights instead of width^3.
+//  * FFT     — rocFFT, 3-D, in place, one plan per (precision, batch bucket),
+//              component-major grids [rhs][component][voxel].
+//  * scale   — k-space Green's tensor projection f <- sc (f.k^) k^ (:1406-1432)
+//              with (k^, sc) packed per voxel and reused across the batch.
+//  * gather  — one wavefront per particle fuses the Gaussian contraction
+//              (:1488-1525), the real-space neighbour sum (:1631-1869, written
+//              as a gather over a symmetric CSR list so it needs no atomics)
+//              and the self term (:1589-1629); a single wave reduction and a
+//              single store per particle and RHS.
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+
+#include "mpm_internal.h"
+
+namespace mpm {
+
+static const double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------
+// Host: sizing, real-space pair functions
+// ---------------------------------------------------------------------------
+
+// monodisperse_ewald_electric_field.cu:546-601
+void compute_sizing(const double L[3], double errortol, double xi, const int grid_override[3],
+                    int P_override, Sizing& S) {
+    MPM_REQUIRE(errortol > 0.0 && errortol < 1.0, MPM_ERR_INVALID_ARGUMENT,
+                "errortol must lie in (0, 1)");
+    MPM_REQUIRE(xi > 0.0, MPM_ERR_INVALID_ARGUMENT, "xi must be positive");
+    for (int d = 0; d < 3; ++d)
+        MPM_REQUIRE(L[d] > 0.0, MPM_ERR_INVALID_ARGUMENT, "box lengths must be positive");
+    S.rc = std::sqrt(-std::log(errortol)) / xi;
+    if (S.rc < 2.0) S.rc = 2.0;
+    for (int d = 0; d < 3; ++d)
+        MPM_REQUIRE(!(S.rc > L[d] / 2.0), MPM_ERR_RUNTIME,
+                    "Real space cutoff (" + std::to_string(S.rc) +
+                        ") larger than half the box length.");
+    S.kcut = 2.0 * xi * xi * S.rc;
+    for (int d = 0; d < 3; ++d) {
+        S.n[d] = (int)std::ceil(1.0 + L[d] * S.kcut / kPi);
+        if (grid_override && grid_override[d] > 0) S.n[d] = grid_override[d];
+        MPM_REQUIRE(S.n[d] >= 1 && S.n[d] <= 4096, MPM_ERR_INVALID_ARGUMENT, "grid size out of range");
+        S.h[d] = L[d] / S.n[d];
+    }
+    S.P = (int)std::ceil(-2.0 * std::log(errortol) / kPi);
+    if (P_override > 0) S.P = P_override;
+    MPM_REQUIRE(S.P >= 1 && S.P <= 40, MPM_ERR_INVALID_ARGUMENT, "support out of range");
+    for (int d = 0; d < 3; ++d) S.eta[d] = S.P * std::pow(S.h[d] * xi, 2) / kPi;
+    S.half = S.P / 2;
+    S.width = 2 * S.half + 1;
+}
+
+// Volume-averaged field of a uniformly polarised unit sphere on a unit sphere
+// at centre distance r, real-space Ewald part, as a sum over the kernel's three
+// images r-2, r, r+2: sum_s [A_s exp(-(r+s)^2 xi^2) + B_s erfc((r+s) xi)] plus
+// a short-range polynomial for overlapping spheres (r < 2).
+// monodisperse_ewald_electric_field.cu:111-176.
+void dipole_pair_functions(double r, double xi, double& f_perp, double& f_para) {
+    const double x2 = xi * xi, x4 = x2 * x2, x5 = x4 * xi, x6 = x4 * x2;
+    const double r2 = r * r, r3 = r2 * r, r4 = r2 * r2, r5 = r4 * r, r6 = r4 * r2;
+    const double sp = std::pow(kPi, 1.5);
+    const double Ep = std::exp(-(r + 2.0) * (r + 2.0) * x2);
+    const double Em = std::exp(-(r - 2.0) * (r - 2.0) * x2);
+    const double E0 = std::exp(-r2 * x2);
+    const double Cp = std::erfc((r + 2.0) * xi);
+    const double Cm = std::erfc((r - 2.0) * xi);
+    const double C0 = std::erfc(r * xi);
+    const double c1 = 3.0 - 12.0 * x2 + 32.0 * x4;
+    const double c0 = 2.0 * (3.0 + 4.0 * x2 - 32.0 * x4);
+    const double q4 = 36.0 * x4 * (1.0 - 4.0 * x2);
+    {   // perpendicular component (reference table field_dip_1)
+        const double pre = 1.0 / (1024.0 * sp * x5 * r3);
+        const double odd = 4.0 * x4 * r5 + 8.0 * x2 * (2.0 - 7.0 * x2) * r3 + c1 * r;
+        const double even = -8.0 * x4 * r4 - 8.0 * x2 * (3.0 + 2.0 * x2) * r2 + c0;
+        const double A0 = (-4.0 * x4 * r4 - 8.0 * x2 * (2.0 - 9.0 * x2) * r2 - 3.0 + 36.0 * x2) /
+                          (512.0 * sp * x5 * r2);
+        const double preb = 1.0 / (2048.0 * kPi * x6 * r3);
+        const double bev = -8.0 * x6 * r6 - q4 * r4 - 18.0 * x2 * (1.0 - 8.0 * x2) * r2 + 3.0 -
+                           36.0 * x2 + 256.0 * x6;
+        const double B0 = (8.0 * x6 * r6 + q4 * r4 + 18.0 * x2 * (1.0 - 8.0 * x2) * r2 - 3.0 +
+                           36.0 * x2) / (1024.0 * kPi * x6 * r3);
+        double f = pre * (odd + even) * Ep + preb * (bev + 256.0 * x6 * r3) * Cp +
+                   pre * (odd - even) * Em + preb * (bev - 256.0 * x6 * r3) * Cm + A0 * E0 + B0 * C0;
+        if (r < 2.0) f += -1.0 / (4.0 * kPi * r3) + (1.0 - 9.0 * r / 16.0 + r3 / 32.0) / (4.0 * kPi);
+        f_perp = f;
+    }
+    {   // parallel component (reference table field_dip_2)
+        const double pre = 1.0 / (512.0 * sp * x5 * r3);
+        const double odd = 8.0 * x4 * r5 + 2.0 * x2 * (7.0 - 20.0 * x2) * r3 - c1 * r;
+        const double even = -16.0 * x4 * r4 - 4.0 * x2 * (3.0 - 4.0 * x2) * r2 - c0;
+        const double A0 = (-8.0 * x4 * r4 - 2.0 * x2 * (7.0 - 36.0 * x2) * r2 + 3.0 - 36.0 * x2) /
+                          (256.0 * sp * x5 * r2);
+        const double preb = 1.0 / (1024.0 * kPi * x6 * r3);
+        const double bev = -16.0 * x6 * r6 - q4 * r4 - 3.0 + 36.0 * x2 - 256.0 * x6;
+        const double B0 = (16.0 * x6 * r6 + q4 * r4 + 3.0 - 36.0 * x2) / (512.0 * kPi * x6 * r3);
+        double f = pre * (odd + even) * Ep + preb * (bev + 128.0 * x6 * r3) * Cp +
+                   pre * (odd - even) * Em + preb * (bev - 128.0 * x6 * r3) * Cm + A0 * E0 + B0 * C0;
+        if (r < 2.0) f += 1.0 / (2.0 * kPi * r3) + (1.0 - 9.0 * r / 8.0 + r3 / 8.0) / (4.0 * kPi);
+        f_para = f;
+    }
+}
+
+// Table entry 0 (monodisperse_ewald_electric_field.cu:455-457).
+double dipole_self_term(double xi) {
+    const double x2 = xi * xi, x3 = x2 * xi;
+    return (-1.0 + 6.0 * x2 + (1.0 - 2.0 * x2) * std::exp(-4.0 * x2)) /
+               (16.0 * std::pow(kPi, 1.5) * x3) +
+           std::erfc(2.0 * xi) / (4.0 * kPi);
+}
+
+// The reference samples the pair functions on r = 1 + 0.001 j (j < 9000) and
+// interpolates linearly, clamping at r >= 9.999 (:92-97, :668-685).  The pair
+// values are therefore computed here the same way, from the same samples.
+struct PairTable {
+    std::vector<double> r, perp, para;
+    explicit PairTable(double xi) {
+        const int n = 9000;
+        r.resize(n + 1);
+        perp.resize(n + 1);
+        para.resize(n + 1);
+        const double s = dipole_self_term(xi);
+        r[0] = 0.0;
+        perp[0] = para[0] = s;
+        for (int j = 0; j < n; ++j) {
+            r[j + 1] = 1.0 + j * 0.001;
+            dipole_pair_functions(r[j + 1], xi, perp[j + 1], para[j + 1]);
+        }
+    }
+    double interp(const std::vector<double>& y, double x) const {
+        const size_t ts = y.size();
+        if (x <= 0.0) return y[0];
+        if (x >= r[ts - 1]) return y[ts - 1];
+        if (x < 1.0) return y[0] * (1.0 - x) + y[1] * x;
+        size_t idx = 1 + (size_t)((x - 1.0) / 0.001);
+        if (idx >= ts - 1) return y[ts - 1];
+        const double t = (x - r[idx]) / (r[idx + 1] - r[idx]);
+        return y[idx] * (1.0 - t) + y[idx + 1] * t;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Profiling
+// ---------------------------------------------------------------------------
+namespace {
+struct ProfEntry {
+    double ms = 0;
+    int64_t count = 0;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::map<std::string, ProfEntry> g_prof;
+struct Pending {
+    std::string name;
+    hipEvent_t e0, e1;
+};
+std::vector<Pending> g_pending;
+}  // namespace
+
+bool profiling_enabled() { return g_prof_on; }
+
+ProfScope::ProfScope(const char* n, hipStream_t s) : name(n), stream(s) {
+    if (!g_prof_on) return;
+    MPM_HIP(hipEventCreate(&e0));
+    MPM_HIP(hipEventCreate(&e1));
+    MPM_HIP(hipEventRecord(e0, stream));
+}
+ProfScope::~ProfScope() {
+    if (!e0) return;
+    (void)hipEventRecord(e1, stream);
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    g_pending.push_back({name, e0, e1});
+}
+
+void profile_flush() {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    for (auto& p : g_pending) {
+        (void)hipEventSynchronize(p.e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, p.e0, p.e1);
+        auto& e = g_prof[p.name];
+        e.ms += ms;
+        e.count += 1;
+        (void)hipEventDestroy(p.e0);
+        (void)hipEventDestroy(p.e1);
+    }
+    g_pending.clear();
+}
+void profile_set(bool on) {
+    profile_flush();
+    g_prof_on = on;
+}
+void profile_reset() {
+    profile_flush();
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    g_prof.clear();
+}
+bool profile_get(const std::string& name, double& ms, int64_t& count) {
+    profile_flush();
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    auto it = g_prof.find(name);
+    if (it == g_prof.end()) {
+        ms = 0;
+        count = 0;
+        return false;
+    }
+    ms = it->second.ms;
+    count = it->second.count;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// rocFFT plans
+// ---------------------------------------------------------------------------
+#define MPM_FFT(call)                                                                          \
+    do {                                                                                       \
+        rocfft_status s_ = (call);                                                             \
+        if (s_ != rocfft_status_success)                                                       \
+            throw ::mpm::Error(MPM_ERR_FFT, std::string(#call) + " failed with rocfft status " + \
+                                                std::to_string((int)s_));                      \
+    } while (0)
+
+static std::once_flag g_rocfft_once;
+
+struct FftPlans {
+    struct Pair {
+        rocfft_plan fwd = nullptr, inv = nullptr;
+        size_t work = 0;
+    };
+    int n[3];
+    std::map<std::pair<int, int>, Pair> plans;  // (fp32, rhs batch) -> plans
+    DevBuf<char> work;
+    rocfft_execution_info info = nullptr;
+
+    explicit FftPlans(const int nn[3]) {
+        std::call_once(g_rocfft_once, [] { rocfft_setup(); });
+        std::memcpy(n, nn, sizeof(n));
+        MPM_FFT(rocfft_execution_info_create(&info));
+    }
+    ~FftPlans() {
+        for (auto& kv : plans) {
+            if (kv.second.fwd) rocfft_plan_destroy(kv.second.fwd);
+            if (kv.second.inv) rocfft_plan_destroy(kv.second.inv);
+        }
+        if (info) rocfft_execution_info_destroy(info);
+    }
+    Pair& get(bool fp32, int batch) {
+        auto key = std::make_pair((int)fp32, batch);
+        auto it = plans.find(key);
+        if (it != plans.end()) return it->second;
+        Pair p;
+        const size_t lengths[3] = {(size_t)n[2], (size_t)n[1], (size_t)n[0]};  // fastest first
+        const rocfft_precision prec = fp32 ? rocfft_precision_single : rocfft_precision_double;
+        MPM_FFT(rocfft_plan_create(&p.fwd, rocfft_placement_inplace,
+                                   rocfft_transform_type_complex_forward, prec, 3, lengths,
+                                   (size_t)batch * 3, nullptr));
+        MPM_FFT(rocfft_plan_create(&p.inv, rocfft_placement_inplace,
+                                   rocfft_transform_type_complex_inverse, prec, 3, lengths,
+                                   (size_t)batch * 3, nullptr));
+        size_t w1 = 0, w2 = 0;
+        MPM_FFT(rocfft_plan_get_work_buffer_size(p.fwd, &w1));
+        MPM_FFT(rocfft_plan_get_work_buffer_size(p.inv, &w2));
+        p.work = std::max(w1, w2);
+        return plans.emplace(key, p).first->second;
+    }
+    // Transform `B` RHS grids (3 components each) starting at `grid`, decomposing the
+    // batch into power-of-two buckets so that a handful of plans cover any B.
+    void exec(void* grid, size_t bytes_per_rhs, int B, bool fp32, bool forward, hipStream_t s) {
+        char* base = static_cast<char*>(grid);
+        int done = 0;
+        while (done < B) {
+            int bucket = 1;
+            while (bucket * 2 <= B - done && bucket * 2 <= 32) bucket *= 2;
+            Pair& p = get(fp32, bucket);
+            work.reserve(p.work);
+            MPM_FFT(rocfft_execution_info_set_stream(info, s));
+            if (p.work) MPM_FFT(rocfft_execution_info_set_work_buffer(info, work.p, p.work));
+            void* buf[1] = {base + (size_t)done * bytes_per_rhs};
+            MPM_FFT(rocfft_execute(forward ? p.fwd : p.inv, buf, nullptr, info));
+            done += bucket;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Device kernels
+// ---------------------------------------------------------------------------
+template <typename R>
+struct C2;
+template <>
+struct C2<double> {
+    using T = double2;
+};
+template <>
+struct C2<float> {
+    using T = float2;
+};
+
+// k-space coefficients: (k^x, k^y, k^z, sc) per voxel.  :1300-1404.
+__global__ void kspace_kernel(double* __restrict__ k64, float* __restrict__ k32, int nx, int ny,
+                              int nz, double Lx, double Ly, double Lz, double kbx, double kby,
+                              double xi, double etax, double etay, double etaz, double invV) {
+    const size_t V = (size_t)nx * ny * nz;
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < V;
+         v += (size_t)gridDim.x * blockDim.x) {
+        const int ix = (int)(v / ((size_t)ny * nz));
+        const int iy = (int)((v / nz) % ny);
+        const int iz = (int)(v % nz);
+        const double fx = (ix <= (nx - 1) / 2) ? ix : (ix - nx);
+        const double fy = (iy <= (ny - 1) / 2) ? iy : (iy - ny);
+        const double fz = (iz <= (nz - 1) / 2) ? iz : (iz - nz);
+        const double twopi = 2.0 * 3.14159265358979323846;
+        const double Gx = fx * twopi / Lx, Gy = fy * twopi / Ly, Gz = fz * twopi / Lz;
+        const double kx = Gx - kbx, ky = Gy - kby, kz = Gz;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        const double k = sqrt(k2);
+        double o[4] = {0.0, 0.0, 0.0, 0.0};
+        if (!(k < 1e-12)) {
+            o[0] = kx / k;
+            o[1] = ky / k;
+            o[2] = kz / k;
+            const double etak2 = k2 - (Gx * Gx * etax + Gy * Gy * etay + Gz * Gz * etaz);
+            const double t = sin(k) / k - cos(k);
+            o[3] = 9.0 * t * t * exp(-etak2 / (4.0 * xi * xi)) / (k2 * k2) * invV;
+        }
+        for (int c = 0; c < 4; ++c) {
+            k64[4 * v + c] = o[c];
+            k32[4 * v + c] = (float)o[c];
+        }
+    }
+}
+
+// Spread: owner-computes gather into a voxel tile.
+constexpr int kTX = 4, kTY = 8, kTZ = 8;  // 256 voxels per workgroup, z fastest
+constexpr int kChunk = 64;                 // particles staged in LDS per round
+constexpr int kMaxWidth = 13;              // stencil points per axis held in LDS
+constexpr int kMaxBins = 512;              // candidate bins per tile (<= 8 per axis)
+
+template <typename R, int G>
+__global__ __launch_bounds__(256) void spread_kernel(
+    SlotTable tab, int B, const int* __restrict__ base, const R* __restrict__ wts,
+    const int* __restrict__ bin_start, const int* __restrict__ bin_items, int N, int nx, int ny,
+    int nz, int nbx, int nby, int nbz, int half, int width, typename C2<R>::T* __restrict__ grid,
+    size_t V) {
+    using CT = typename C2<R>::T;
+    __shared__ int s_bin_lo[kMaxBins];
+    __shared__ int s_bin_off[kMaxBins + 1];
+    __shared__ int s_nbins;
+    __shared__ int s_base[kChunk][3];
+    __shared__ R s_w[kChunk][3 * kMaxWidth];
+    __shared__ CT s_p[kChunk][G][3];
+
+    const int tid = threadIdx.x;
+    const int tiles_y = nby, tiles_z = nbz;  // the bin grid is the tile grid
+    const int t = blockIdx.x;
+    const int btx = t / (tiles_y * tiles_z);
+    const int bty = (t / tiles_z) % tiles_y;
+    const int btz = t % tiles_z;
+    const int g0 = blockIdx.y * G;
+
+    const int lz = tid % kTZ, ly = (tid / kTZ) % kTY, lx = tid / (kTZ * kTY);
+    const int vx = btx * kTX + lx, vy = bty * kTY + ly, vz = btz * kTZ + lz;
+    const bool valid = vx < nx && vy < ny && vz < nz;
+
+    // Candidate bins: every bin holding a base voxel g whose stencil g-half..g+half can
+    // reach this tile, found per axis in voxel space (the last bin of an axis may be
+    // partial, so bin-space offsets would miss wrapped neighbours).
+    if (tid == 0) {
+        int lists[3][8];
+        int cnts[3];
+        const int bt[3] = {btx, bty, btz};
+        const int tl[3] = {kTX, kTY, kTZ};
+        const int nn[3] = {nx, ny, nz};
+        const int nbb[3] = {nbx, nby, nbz};
+        for (int d = 0; d < 3; ++d) {
+            const int lo = bt[d] * tl[d] - half, hi = bt[d] * tl[d] + tl[d] - 1 + half;
+            int cnt = 0;
+            if (hi - lo + 1 >= nn[d]) {
+                for (int b = 0; b < nbb[d] && cnt < 8; ++b) lists[d][cnt++] = b;
+            } else {
+                for (int g = lo; g <= hi; ++g) {
+                    const int b = (((g % nn[d]) + nn[d]) % nn[d]) / tl[d];
+                    bool dup = false;
+                    for (int k = 0; k < cnt; ++k) dup = dup || (lists[d][k] == b);
+                    if (!dup && cnt < 8) lists[d][cnt++] = b;
+                }
+            }
+            cnts[d] = cnt;
+        }
+        int nb = 0, off = 0;
+        for (int a = 0; a < cnts[0]; ++a)
+            for (int b = 0; b < cnts[1]; ++b)
+                for (int c = 0; c < cnts[2]; ++c) {
+                    const int bin = (lists[0][a] * nby + lists[1][b]) * nbz + lists[2][c];
+                    const int blo = bin_start[bin], bhi = bin_start[bin + 1];
+                    if (bhi > blo) {
+                        s_bin_lo[nb] = blo;
+                        s_bin_off[nb] = off;
+                        off += bhi - blo;
+                        ++nb;
+                    }
+                }
+        s_bin_off[nb] = off;
+        s_nbins = nb;
+    }
+    __syncthreads();
+    const int nbins = s_nbins;
+    const int total = s_bin_off[nbins];
+
+    R acc[G][3][2];
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[j][c][0] = acc[j][c][1] = R(0);
+
+    for (int c0 = 0; c0 < total; c0 += kChunk) {
+        const int cnt = min(kChunk, total - c0);
+        __syncthreads();
+        // stage `cnt` candidates: base, 3*width weights and G x 3 dipoles each
+        for (int q = tid; q < cnt; q += blockDim.x) {
+            const int flat = c0 + q;
+            int b = 0;
+            while (b + 1 < nbins && s_bin_off[b + 1] <= flat) ++b;
+            const int pid = bin_items[s_bin_lo[b] + (flat - s_bin_off[b])];
+            s_base[q][0] = base[pid];
+            s_base[q][1] = base[N + pid];
+            s_base[q][2] = base[2 * N + pid];
+            for (int k = 0; k < 3 * width; ++k) s_w[q][k] = wts[(size_t)pid * 3 * width + k];
+            for (int j = 0; j < G; ++j) {
+                const int b2 = g0 + j;
+                for (int c = 0; c < 3; ++c) {
+                    CT v;
+                    if (b2 < B) {
+                        const double2 d = tab.x[b2][3 * (size_t)pid + c];
+                        v.x = (R)d.x;
+                        v.y = (R)d.y;
+                    } else {
+                        v.x = v.y = R(0);
+                    }
+                    s_p[q][j][c] = v;
+                }
+            }
+        }
+        __syncthreads();
+        if (valid) {
+            for (int q = 0; q < cnt; ++q) {
+                // folded per-axis weight: sum over stencil offsets o == (v - g) mod n
+                R w3[3];
+                const int vv[3] = {vx, vy, vz};
+                const int nn[3] = {nx, ny, nz};
+                bool hit = true;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    int dd = vv[d] - s_base[q][d];
+                    if (dd < 0) dd += nn[d];
+                    int o = dd - nn[d] * ((dd + half) / nn[d]);
+                    R ws = R(0);
+                    bool any = false;
+                    for (; o <= half; o += nn[d]) {
+                        ws += s_w[q][d * width + o + half];
+                        any = true;
+                    }
+                    w3[d] = ws;
+                    hit = hit && any;
+                }
+                if (!hit) continue;
+                const R w = w3[0] * w3[1] * w3[2];
+#pragma unroll
+                for (int j = 0; j < G; ++j)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        acc[j][c][0] += w * s_p[q][j][c].x;
+                        acc[j][c][1] += w * s_p[q][j][c].y;
+                    }
+            }
+        }
+    }
+    if (!valid) return;
+    const size_t v = ((size_t)vx * ny + vy) * nz + vz;
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+        const int b = g0 + j;
+        if (b >= B) break;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            CT o;
+            o.x = acc[j][c][0];
+            o.y = acc[j][c][1];
+            grid[((size_t)b * 3 + c) * V + v] = o;
+        }
+    }
+}
+
+// k-space projection f <- sc (f.k^) k^ for every RHS (:1406-1432).
+template <typename R>
+__global__ __launch_bounds__(256) void scale_kernel(typename C2<R>::T* __restrict__ grid,
+                                                    const R* __restrict__ kc, size_t V, int B) {
+    using CT = typename C2<R>::T;
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < V;
+         v += (size_t)gridDim.x * blockDim.x) {
+        const R kx = kc[4 * v], ky = kc[4 * v + 1], kz = kc[4 * v + 2], sc = kc[4 * v + 3];
+        for (int b = 0; b < B; ++b) {
+            CT* g = grid + (size_t)b * 3 * V + v;
+            const CT f0 = g[0], f1 = g[V], f2 = g[2 * V];
+            const R dr = (f0.x * kx + f1.x * ky + f2.x * kz) * sc;
+            const R di = (f0.y * kx + f1.y * ky + f2.y * kz) * sc;
+            CT o;
+            o.x = dr * kx;
+            o.y = di * kx;
+            g[0] = o;
+            o.x = dr * ky;
+            o.y = di * ky;
+            g[V] = o;
+            o.x = dr * kz;
+            o.y = di * kz;
+            g[2 * V] = o;
+        }
+    }
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// One wavefront per particle: contraction + real-space gather + self term.
+template <typename R, int G>
+__global__ __launch_bounds__(256) void gather_kernel(
+    SlotTable tab, int B, int parts, const typename C2<R>::T* __restrict__ grid, size_t V,
+    const int* __restrict__ base, const R* __restrict__ wts, int N, int nx, int ny, int nz,
+    int half, int width, double h3, const double* __restrict__ pos, const int* __restrict__ nb_start,
+    const int* __restrict__ nb_j, const double2* __restrict__ nb_f, double Lx, double Ly,
+    double Lz, double kbx, double kby, const double2* __restrict__ selfc, double self_perp) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (i >= N) return;
+    const int S = width * width * width;
+    int sx = 0, sy = 0, sz = 0;
+    const R* w = wts + (size_t)i * 3 * width;
+    if (parts & MPM_PART_RECIP) {
+        sx = ((base[i] - half) % nx + nx) % nx;
+        sy = ((base[N + i] - half) % ny + ny) % ny;
+        sz = ((base[2 * N + i] - half) % nz + nz) % nz;
+    }
+    const double xi_ = pos[i], yi_ = pos[N + i], zi_ = pos[2 * N + i];
+    const bool bloch = (kbx != 0.0 || kby != 0.0);
+
+    for (int g0 = 0; g0 < B; g0 += G) {
+        double acc[G][3][2];
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[j][c][0] = acc[j][c][1] = 0.0;
+
+        if (parts & MPM_PART_RECIP) {
+            R racc[G][3][2];
+#pragma unroll
+            for (int j = 0; j < G; ++j)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) racc[j][c][0] = racc[j][c][1] = R(0);
+            for (int t = lane; t < S; t += 64) {
+                const int ox = t / (width * width), oy = (t / width) % width, oz = t % width;
+                int vx = sx + ox, vy = sy + oy, vz = sz + oz;
+                while (vx >= nx) vx -= nx;
+                while (vy >= ny) vy -= ny;
+                while (vz >= nz) vz -= nz;
+                const R ww = w[ox] * w[width + oy] * w[2 * width + oz];
+                const size_t v = ((size_t)vx * ny + vy) * nz + vz;
+#pragma unroll
+                for (int j = 0; j < G; ++j) {
+                    if (g0 + j >= B) break;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const auto f = grid[((size_t)(g0 + j) * 3 + c) * V + v];
+                        racc[j][c][0] += ww * f.x;
+                        racc[j][c][1] += ww * f.y;
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < G; ++j)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    acc[j][c][0] = (double)racc[j][c][0] * h3;
+                    acc[j][c][1] = (double)racc[j][c][1] * h3;
+                }
+        }
+        if (parts & MPM_PART_REAL) {
+            const int k0 = nb_start[i], k1 = nb_start[i + 1];
+            for (int k = k0 + lane; k < k1; k += 64) {
+                const int j = nb_j[k];
+                const double2 f = nb_f[k];
+                double rx = pos[j] - xi_, ry = pos[N + j] - yi_, rz = pos[2 * N + j] - zi_;
+                rx -= Lx * round(rx / Lx);
+                ry -= Ly * round(ry / Ly);
+                rz -= Lz * round(rz / Lz);
+                const double d = sqrt(rx * rx + ry * ry + rz * rz);
+                const double de = d < 1.0 ? 1.0 : d;
+                const double ux = rx / de, uy = ry / de, uz = rz / de;
+                double pc = 1.0, ps = 0.0;
+                if (bloch) {
+                    const double ph = -(kbx * rx + kby * ry);
+                    pc = cos(ph);
+                    ps = sin(ph);
+                }
+#pragma unroll
+                for (int jj = 0; jj < G; ++jj) {
+                    if (g0 + jj >= B) break;
+                    const double2* p = tab.x[g0 + jj] + 3 * (size_t)j;
+                    const double2 p0 = p[0], p1 = p[1], p2 = p[2];
+                    const double rpr = p0.x * ux + p1.x * uy + p2.x * uz;
+                    const double rpi = p0.y * ux + p1.y * uy + p2.y * uz;
+                    const double u[3] = {ux, uy, uz};
+                    const double2 pp[3] = {p0, p1, p2};
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const double er = (pp[c].x - rpr * u[c]) * f.x + rpr * u[c] * f.y;
+                        const double ei = (pp[c].y - rpi * u[c]) * f.x + rpi * u[c] * f.y;
+                        acc[jj][c][0] += er * pc - ei * ps;
+                        acc[jj][c][1] += er * ps + ei * pc;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                acc[j][c][0] = wave_sum(acc[j][c][0]);
+                acc[j][c][1] = wave_sum(acc[j][c][1]);
+            }
+        if (lane == 0) {
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                const int b = g0 + j;
+                if (b >= B) break;
+                const double2* p = tab.x[b] + 3 * (size_t)i;
+                double2 fs = make_double2(0.0, 0.0);
+                if (parts & MPM_PART_REAL) {
+                    fs = selfc[(size_t)tab.row[b] * N + i];
+                    fs.x += self_perp;
+                }
+                double2* y = tab.y[b] + 3 * (size_t)i;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double2 pc = p[c];
+                    y[c] = make_double2(acc[j][c][0] + fs.x * pc.x - fs.y * pc.y,
+                                        acc[j][c][1] + fs.x * pc.y + fs.y * pc.x);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// EwaldOp
+// ---------------------------------------------------------------------------
+EwaldOp::EwaldOp(const mpm_op_desc& d) {
+    for (int k = 0; k < 3; ++k) L[k] = d.box[k];
+    errortol = d.errortol;
+    xi = d.xi;
+    compute_sizing(L, errortol, xi, d.grid, d.support, S);
+    MPM_REQUIRE(S.width <= kMaxWidth, MPM_ERR_UNSUPPORTED,
+                "stencil wider than " + std::to_string(kMaxWidth) + " points per axis");
+    self_perp = dipole_self_term(xi);
+    V = (size_t)S.n[0] * S.n[1] * S.n[2];
+    for (int k = 0; k < 3; ++k) {
+        const int tl[3] = {kTX, kTY, kTZ};
+        tile[k] = tl[k];
+        nbins[k] = (S.n[k] + tile[k] - 1) / tile[k];
+    }
+    fft = std::make_unique<FftPlans>(S.n);
+    build_kspace();
+}
+
+EwaldOp::~EwaldOp() = default;
+
+void EwaldOp::build_kspace() {
+    d_k64.resize(4 * V);
+    d_k32.resize(4 * V);
+    const int threads = 256;
+    const int blocks = (int)std::min<size_t>((V + threads - 1) / threads, 65535);
+    hipLaunchKernelGGL(kspace_kernel, dim3(blocks), dim3(threads), 0, nullptr, d_k64.p, d_k32.p,
+                       S.n[0], S.n[1], S.n[2], L[0], L[1], L[2], kb[0], kb[1], xi, S.eta[0],
+                       S.eta[1], S.eta[2], 1.0 / (double)V);
+    MPM_HIP(hipGetLastError());
+    MPM_HIP(hipDeviceSynchronize());
+}
+
+void EwaldOp::set_bloch(double kx, double ky) {
+    if (kx == kb[0] && ky == kb[1]) return;
+    kb[0] = kx;
+    kb[1] = ky;
+    build_kspace();
+}
+
+void EwaldOp::set_positions(const double* xyz, int64_t n) {
+    MPM_REQUIRE(n > 0, MPM_ERR_INVALID_ARGUMENT, "no particles");
+    MPM_REQUIRE(n < (1LL << 31) / 1024, MPM_ERR_INVALID_ARGUMENT, "too many particles");
+    const int Nn = (int)n;
+    for (int64_t q = 0; q < 3 * n; ++q)
+        MPM_REQUIRE(std::isfinite(xyz[q]), MPM_ERR_INVALID_ARGUMENT, "non-finite position");
+    N = n;
+    const int W = S.width, half = S.half;
+    std::vector<double> pos(3 * (size_t)Nn);
+    std::vector<int> base(3 * (size_t)Nn);
+    std::vector<double> w64((size_t)Nn * 3 * W);
+    std::vector<float> w32((size_t)Nn * 3 * W);
+    const double prod_eta = S.eta[0] * S.eta[1] * S.eta[2];
+    const double Cs = std::pow(2.0 * xi * xi / kPi, 1.5) * std::sqrt(1.0 / prod_eta);
+    for (int i = 0; i < Nn; ++i) {
+        for (int d = 0; d < 3; ++d) {
+            const double p = xyz[3 * (size_t)i + d];
+            pos[(size_t)d * Nn + i] = p;
+            const double gd = std::round(p / S.h[d]);
+            MPM_REQUIRE(std::fabs(gd) < 1e9, MPM_ERR_INVALID_ARGUMENT, "position too far from the box");
+            const long long g = (long long)gd;
+            const double dd = g * S.h[d] - p;
+            base[(size_t)d * Nn + i] = (int)(((g % S.n[d]) + S.n[d]) % S.n[d]);
+            for (int o = -half; o <= half; ++o) {
+                const double x = dd + o * S.h[d];
+                double wv = std::exp(-2.0 * xi * xi * x * x / S.eta[d]);
+                if (d == 0) wv *= Cs;
+                w64[((size_t)i * 3 + d) * W + o + half] = wv;
+                w32[((size_t)i * 3 + d) * W + o + half] = (float)wv;
+            }
+        }
+    }
+    // bins = tiles of the spread kernel, keyed by each particle's base voxel
+    const int nb = nbins[0] * nbins[1] * nbins[2];
+    std::vector<int> bin_of(Nn), start(nb + 1, 0), items(Nn);
+    for (int i = 0; i < Nn; ++i) {
+        const int bx = base[i] / tile[0], by = base[Nn + i] / tile[1], bz = base[2 * Nn + i] / tile[2];
+        bin_of[i] = (bx * nbins[1] + by) * nbins[2] + bz;
+        start[bin_of[i] + 1]++;
+    }
+    for (int b = 0; b < nb; ++b) start[b + 1] += start[b];
+    {
+        std::vector<int> fill(start.begin(), start.end() - 1);
+        for (int i = 0; i < Nn; ++i) items[fill[bin_of[i]]++] = i;
+    }
+    // real-space neighbours: all j != i with min-image distance < rc (neighbor_list.cu
+    // keeps d^2 <= rc^2 and the kernel then skips d >= rc; pairs at exactly rc add zero)
+    PairTable tab(xi);
+    const double rc = S.rc, rc2 = rc * rc;
+    int nc[3];
+    for (int d = 0; d < 3; ++d) nc[d] = std::max(1, (int)std::floor(L[d] / rc));
+    const bool cells = nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3;
+    std::vector<std::vector<int>> members;
+    auto cell_coord = [&](double p, int d) {
+        const double wv = p - L[d] * std::floor(p / L[d]);
+        int c = (int)std::floor(wv / (L[d] / nc[d]));
+        return std::min(std::max(c, 0), nc[d] - 1);
+    };
+    if (cells) {
+        members.resize((size_t)nc[0] * nc[1] * nc[2]);
+        for (int j = 0; j < Nn; ++j) {
+            const int cx = cell_coord(pos[j], 0), cy = cell_coord(pos[Nn + j], 1),
+                      cz = cell_coord(pos[2 * Nn + j], 2);
+            members[(size_t)(cz * nc[1] + cy) * nc[0] + cx].push_back(j);
+        }
+    }
+    std::vector<int> nstart(Nn + 1, 0), nj;
+    std::vector<double2> nf;
+    nj.reserve((size_t)Nn * 64);
+    nf.reserve((size_t)Nn * 64);
+    auto consider = [&](int i, int j) {
+        double r[3] = {pos[i] - pos[j], pos[Nn + i] - pos[Nn + j], pos[2 * Nn + i] - pos[2 * Nn + j]};
+        for (int d = 0; d < 3; ++d) r[d] -= L[d] * std::round(r[d] / L[d]);
+        const double d2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        if (!(d2 <= rc2)) return;
+        const double dist = std::sqrt(d2);
+        if (!(dist < rc)) return;
+        const double de = dist < 1.0 ? 1.0 : dist;
+        nj.push_back(j);
+        nf.push_back(make_double2(tab.interp(tab.perp, de), tab.interp(tab.para, de)));
+    };
+    for (int i = 0; i < Nn; ++i) {
+        if (cells) {
+            const int cx = cell_coord(pos[i], 0), cy = cell_coord(pos[Nn + i], 1),
+                      cz = cell_coord(pos[2 * Nn + i], 2);
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int a = (cx + dx + nc[0]) % nc[0], b = (cy + dy + nc[1]) % nc[1],
+                                  c = (cz + dz + nc[2]) % nc[2];
+                        for (int j : members[(size_t)(c * nc[1] + b) * nc[0] + a])
+                            if (j != i) consider(i, j);
+                    }
+        } else {
+            for (int j = 0; j < Nn; ++j)
+                if (j != i) consider(i, j);
+        }
+        nstart[i + 1] = (int)nj.size();
+    }
+    num_pairs = (int64_t)nj.size();
+    d_pos.upload(pos.data(), pos.size());
+    d_base.upload(base.data(), base.size());
+    d_w64.upload(w64.data(), w64.size());
+    d_w32.upload(w32.data(), w32.size());
+    d_bin_start.upload(start.data(), start.size());
+    d_bin_items.upload(items.data(), items.size());
+    d_nb_start.upload(nstart.data(), nstart.size());
+    if (nj.empty()) {
+        nj.push_back(0);
+        nf.push_back(make_double2(0, 0));
+    }
+    d_nb_j.upload(nj.data(), nj.size());
+    d_nb_f.upload(nf.data(), nf.size());
+    // default self coefficients: zero (reference base_electric_field.cu allocates them zeroed)
+    if (self_rows == 0 || d_selfc.n != (size_t)self_rows * Nn) {
+        self_rows = 1;
+        d_selfc.resize(Nn);
+        MPM_HIP(hipMemset(d_selfc.p, 0, Nn * sizeof(double2)));
+    }
+}
+
+void EwaldOp::set_self_coef(const double* coef, int rows) {
+    MPM_REQUIRE(N > 0, MPM_ERR_RUNTIME, "set positions before the self coefficients");
+    MPM_REQUIRE(rows >= 1, MPM_ERR_INVALID_ARGUMENT, "need at least one row of self coefficients");
+    d_selfc.upload(reinterpret_cast<const double2*>(coef), (size_t)rows * N);
+    self_rows = rows;
+}
+
+void EwaldOp::ensure_grid(int B, bool fp32) {
+    if (fp32)
+        d_grid32.reserve((size_t)B * 3 * V);
+    else
+        d_grid64.reserve((size_t)B * 3 * V);
+}
+
+template <typename R>
+static void launch_apply(const SlotTable& t, int B, int parts, typename C2<R>::T* grid, size_t V,
+                         const Sizing& S, const int nbins[3], int N, const int* base, const R* w,
+                         const int* bstart, const int* bitems, const double* pos,
+                         const int* nbs, const int* nbj, const double2* nbf, const double L[3],
+                         const double kb[2], const double2* selfc, double self_perp,
+                         const R* kc, FftPlans& fft, hipStream_t s) {
+    const bool fp32 = std::is_same<R, float>::value;
+    if (parts & MPM_PART_RECIP) {
+        {
+            ProfScope ps("spread", s);
+            const int ntiles = nbins[0] * nbins[1] * nbins[2];
+            if (B == 1) {
+                hipLaunchKernelGGL((spread_kernel<R, 1>), dim3(ntiles, 1), dim3(256), 0, s, t, B, base,
+                                   w, bstart, bitems, N, S.n[0], S.n[1], S.n[2], nbins[0], nbins[1],
+                                   nbins[2], S.half, S.width, grid, V);
+            } else {
+                hipLaunchKernelGGL((spread_kernel<R, 4>), dim3(ntiles, (B + 3) / 4), dim3(256), 0, s, t,
+                                   B, base, w, bstart, bitems, N, S.n[0], S.n[1], S.n[2], nbins[0],
+                                   nbins[1], nbins[2], S.half, S.width, grid, V);
+            }
+            MPM_HIP(hipGetLastError());
+        }
+        const size_t bytes_per_rhs = 3 * V * sizeof(typename C2<R>::T);
+        {
+            ProfScope ps("fft", s);
+            fft.exec(grid, bytes_per_rhs, B, fp32, true, s);
+        }
+        {
+            ProfScope ps("scale", s);
+            const int blocks = (int)std::min<size_t>((V + 255) / 256, 8192);
+            hipLaunchKernelGGL(scale_kernel<R>, dim3(blocks), dim3(256), 0, s, grid, kc, V, B);
+            MPM_HIP(hipGetLastError());
+        }
+        {
+            ProfScope ps("fft", s);
+            fft.exec(grid, bytes_per_rhs, B, fp32, false, s);
+        }
+    }
+    {
+        ProfScope ps("gather", s);
+        const double h3 = S.h[0] * S.h[1] * S.h[2];
+        const int blocks = (N + 3) / 4;
+        if (B == 1)
+            hipLaunchKernelGGL((gather_kernel<R, 1>), dim3(blocks), dim3(256), 0, s, t, B, parts, grid, V,
+                               base, w, N, S.n[0], S.n[1], S.n[2], S.half, S.width, h3, pos, nbs,
+                               nbj, nbf, L[0], L[1], L[2], kb[0], kb[1], selfc, self_perp);
+        else
+            hipLaunchKernelGGL((gather_kernel<R, 4>), dim3(blocks), dim3(256), 0, s, t, B, parts, grid, V,
+                               base, w, N, S.n[0], S.n[1], S.n[2], S.half, S.width, h3, pos, nbs,
+                               nbj, nbf, L[0], L[1], L[2], kb[0], kb[1], selfc, self_perp);
+        MPM_HIP(hipGetLastError());
+    }
+}
+
+void EwaldOp::apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t s) {
+    MPM_REQUIRE(N > 0, MPM_ERR_RUNTIME, "apply before set_positions");
+    MPM_REQUIRE(B >= 1 && B <= kMaxSlots, MPM_ERR_INVALID_ARGUMENT, "bad slot count");
+    for (int b = 0; b < B; ++b)
+        MPM_REQUIRE(t.row[b] >= 0 && t.row[b] < self_rows, MPM_ERR_INVALID_ARGUMENT,
+                    "self-coefficient row out of range");
+    ensure_grid(B, fp32);
+    const int Ni = (int)N;
+    if (fp32)
+        launch_apply<float>(t, B, parts, d_grid32.p, V, S, nbins, Ni, d_base.p, d_w32.p, d_bin_start.p,
+                            d_bin_items.p, d_pos.p, d_nb_start.p, d_nb_j.p, d_nb_f.p, L, kb,
+                            d_selfc.p, self_perp, d_k32.p, *fft, s);
+    else
+        launch_apply<double>(t, B, parts, d_grid64.p, V, S, nbins, Ni, d_base.p, d_w64.p,
+                             d_bin_start.p, d_bin_items.p, d_pos.p, d_nb_start.p, d_nb_j.p, d_nb_f.p,
+                             L, kb, d_selfc.p, self_perp, d_k64.p, *fft, s);
+}
+
+void EwaldOp::apply_contiguous(const double2* x, double2* y, int B, bool fp32, int parts,
+                               hipStream_t s, const int* rows) {
+    const size_t n = 3 * (size_t)N;
+    for (int b0 = 0; b0 < B; b0 += kMaxSlots) {
+        const int cnt = std::min(kMaxSlots, B - b0);
+        SlotTable t;
+        for (int j = 0; j < cnt; ++j) {
+            t.x[j] = x + (size_t)(b0 + j) * n;
+            t.y[j] = y + (size_t)(b0 + j) * n;
+            t.row[j] = rows ? rows[b0 + j] : (self_rows > 1 ? b0 + j : 0);
+        }
+        apply(t, cnt, fp32, parts, s);
+    }
+}
+
+void EwaldOp::info(mpm_op_info& o) const {
+    std::memset(&o, 0, sizeof(o));
+    o.num_particles = N;
+    o.num_pairs = num_pairs;
+    for (int d = 0; d < 3; ++d) {
+        o.grid[d] = S.n[d];
+        o.grid_spacing[d] = S.h[d];
+        o.spectral_split[d] = S.eta[d];
+    }
+    o.support = S.P;
+    o.stencil_width = S.width;
+    o.rc = S.rc;
+    o.self_perp = self_perp;
+    o.self_coef_ewald = -4.0 * std::pow(xi, 3.0) / (3.0 * std::sqrt(kPi));
+    o.bloch[0] = kb[0];
+    o.bloch[1] = kb[1];
+}
+
+}  // namespace mpm

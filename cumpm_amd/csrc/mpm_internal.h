@@ -1,0 +1,165 @@
+// Internal declarations shared by the HIP translation units of libcumpm_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cumpm_amd.h"
+
+namespace mpm {
+
+using cplx = std::complex<double>;
+
+// Errors inside the library are C++ exceptions carrying an mpm_status; the
+// C ABI layer converts them (no exception crosses the boundary).
+struct Error : std::runtime_error {
+    mpm_status status;
+    Error(mpm_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define MPM_HIP(call)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw ::mpm::Error(e_ == hipErrorOutOfMemory ? MPM_ERR_OUT_OF_MEMORY           \
+                                                         : MPM_ERR_HIP,                   \
+                               std::string(#call) + ": " + hipGetErrorString(e_) + " (" + \
+                                   __FILE__ + ":" + std::to_string(__LINE__) + ")");      \
+    } while (0)
+
+#define MPM_REQUIRE(cond, status, msg)          \
+    do {                                        \
+        if (!(cond)) throw ::mpm::Error(status, msg); \
+    } while (0)
+
+// Owning device allocation.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void resize(size_t count) {
+        if (count == n && p) return;
+        release();
+        if (count) MPM_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    // grow-only variant for scratch buffers
+    void reserve(size_t count) {
+        if (count > n) resize(count);
+    }
+    void upload(const T* host, size_t count, hipStream_t s = nullptr) {
+        resize(count);
+        if (count) MPM_HIP(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+        if (count) MPM_HIP(hipStreamSynchronize(s));
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Profiling (HIP events around kernel classes, on the launch stream).
+// ---------------------------------------------------------------------------
+struct ProfScope {
+    ProfScope(const char* name, hipStream_t s);
+    ~ProfScope();
+    const char* name;
+    hipStream_t stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+bool profiling_enabled();
+
+// ---------------------------------------------------------------------------
+// Ewald operator (monodisperse, dipoles).
+// ---------------------------------------------------------------------------
+struct Sizing {
+    double rc = 0, kcut = 0;
+    int n[3] = {0, 0, 0};
+    double h[3] = {0, 0, 0};
+    double eta[3] = {0, 0, 0};
+    int P = 0, half = 0, width = 0;
+};
+
+void compute_sizing(const double L[3], double errortol, double xi, const int grid_override[3],
+                    int P_override, Sizing& S);
+void dipole_pair_functions(double r, double xi, double& f_perp, double& f_para);
+double dipole_self_term(double xi);
+
+// Pointer table passed by value to the batched kernels (one entry per RHS slot).
+constexpr int kMaxSlots = 32;
+struct SlotTable {
+    const double2* x[kMaxSlots];
+    double2* y[kMaxSlots];
+    int row[kMaxSlots];  // self-coefficient row of the slot
+};
+
+struct FftPlans;  // rocFFT plans keyed by (precision, batch)
+
+class EwaldOp {
+   public:
+    EwaldOp(const mpm_op_desc& d);
+    ~EwaldOp();
+    void set_positions(const double* xyz, int64_t N);
+    void set_bloch(double kx, double ky);
+    void set_self_coef(const double* coef, int rows);
+    // y[b] = A x[b] for the slots in `t` (count B <= kMaxSlots)
+    void apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t s);
+    // convenience: contiguous [B][3N] buffers, chunked by kMaxSlots
+    void apply_contiguous(const double2* x, double2* y, int B, bool fp32, int parts,
+                          hipStream_t s, const int* rows = nullptr);
+    void info(mpm_op_info& out) const;
+
+    int64_t N = 0;
+    int self_rows = 0;
+    const double2* self_coef() const { return d_selfc.p; }
+
+   private:
+    void build_kspace();
+    void ensure_grid(int B, bool fp32);
+
+    double L[3], errortol, xi;
+    Sizing S;
+    double self_perp = 0;
+    double kb[2] = {0, 0};
+    size_t V = 0;
+    int64_t num_pairs = 0;
+    // geometry on device
+    DevBuf<double> d_pos;        // [3][N] SoA
+    DevBuf<int> d_base;          // [3][N] wrapped base grid index
+    DevBuf<double> d_w64;        // [N][3][width] per-axis weights (spread constant folded in x)
+    DevBuf<float> d_w32;
+    DevBuf<int> d_bin_start;     // [nbins + 1]
+    DevBuf<int> d_bin_items;     // [N] particle ids sorted by bin
+    int tile[3] = {4, 8, 8};
+    int nbins[3] = {0, 0, 0};
+    DevBuf<int> d_nb_start;      // [N + 1] target-centric CSR
+    DevBuf<int> d_nb_j;          // [M]
+    DevBuf<double2> d_nb_f;      // [M] (perp, para)
+    DevBuf<double2> d_selfc;     // [rows][N]
+    // k space
+    DevBuf<double> d_k64;        // [V][4] (khat x, y, z, scale)
+    DevBuf<float> d_k32;
+    // grids
+    DevBuf<double2> d_grid64;
+    DevBuf<float2> d_grid32;
+    std::unique_ptr<FftPlans> fft;
+};
+
+// Batched Krylov solve (krylov.hip).
+void solve_batched(EwaldOp& op, const mpm_solve_desc& d, const double2* b, double2* x, int B,
+                   mpm_solve_report* reports, hipStream_t s);
+
+}  // namespace mpm

@@ -1,0 +1,271 @@
+"""Operator-level API mirroring src/cuMPM/dev/electric_field.py of the reference.
+
+``MonodisperseEwaldElectricField`` applies the Ewald operator once
+(``calculate``) to dipoles uploaded from the host and reads the field back,
+with the reference's split helpers ``compute_real_field`` and
+``compute_inverse_field`` (:764-778).  Device memory is owned by
+libcumpm_amd; dipoles and fields are [particle][component] complex128.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from .. import _capi as C
+from .._capi import check
+from ..solver import PrecisionMode
+
+
+class FieldCalcMode:
+    SOLVER_AX = 0
+    INTERACTION_FIELD = 1
+
+
+def _parse_precision(precision):
+    if precision is None:
+        return PrecisionMode.AUTO
+    if isinstance(precision, str):
+        p = precision.lower()
+        if p in ("auto",):
+            return PrecisionMode.AUTO
+        if p in ("mixed", "fp32", "single"):
+            return PrecisionMode.MIXED
+        if p in ("double", "fp64"):
+            return PrecisionMode.DOUBLE
+        raise ValueError(f"Unknown precision mode string: '{precision}'. Must be 'auto', 'mixed', or 'double'.")
+    return precision
+
+
+def _vec(v):
+    return np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1))
+
+
+class MonodisperseEwaldElectricField:
+    """Monodisperse spectral-Ewald operator (dev/electric_field.py:430-842 of the reference)."""
+
+    def __init__(self, box_x, box_y, box_z, errortol, xi, calc_inter_dipole, radius=1.0,
+                 solve_quadrupoles=False, quad_idxs=None, precision=None, *, grid=None, support=0):
+        mode = calc_inter_dipole
+        if isinstance(calc_inter_dipole, bool):
+            mode = FieldCalcMode.SOLVER_AX if calc_inter_dipole else FieldCalcMode.INTERACTION_FIELD
+        if mode != FieldCalcMode.SOLVER_AX:
+            raise NotImplementedError("INTERACTION_FIELD mode (near fields) is not provided by this build")
+        if solve_quadrupoles:
+            raise NotImplementedError("quadrupoles are not provided by this build yet")
+        self._prec = _parse_precision(precision)
+        d = C.OpDesc()
+        d.box[:] = [float(box_x), float(box_y), float(box_z)]
+        d.errortol = float(errortol)
+        d.xi = float(xi)
+        if grid is not None:
+            d.grid[:] = [int(g) for g in grid]
+        d.support = int(support)
+        h = ctypes.c_void_p()
+        check(C.load().mpm_op_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self._box = (float(box_x), float(box_y), float(box_z))
+        self._errortol = float(errortol)
+        self._xi = float(xi)
+        self._N = 0
+        self._dip = None      # host copy [N][3] complex
+        self._sc = None
+        self._epoint = None
+        self._xbuf = self._ybuf = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and C._lib is not None:
+            C._lib.mpm_op_destroy(h)
+            self._h = None
+
+    # ---- properties ----
+    def _info(self):
+        info = C.OpInfo()
+        check(C.load().mpm_op_get_info(self._h, ctypes.byref(info)))
+        return info
+
+    @property
+    def is_using_recip_fp32(self) -> bool:
+        return self._prec != PrecisionMode.DOUBLE
+
+    @property
+    def recip_precision_mode(self):
+        return self._prec
+
+    @property
+    def box_x(self):
+        return self._box[0]
+
+    @property
+    def box_y(self):
+        return self._box[1]
+
+    @property
+    def box_z(self):
+        return self._box[2]
+
+    @property
+    def errortol(self):
+        return self._errortol
+
+    @property
+    def xi(self):
+        return self._xi
+
+    @property
+    def rc(self):
+        return self._info().rc
+
+    @property
+    def mode(self):
+        return FieldCalcMode.SOLVER_AX
+
+    @property
+    def calc_inter_dipole(self):
+        return True
+
+    @property
+    def self_coef(self):
+        return self._info().self_coef_ewald
+
+    @property
+    def num_particles(self):
+        return self._N
+
+    @property
+    def num_field_points(self):
+        return self._N
+
+    @property
+    def num_pairs(self):
+        return int(self._info().num_pairs)
+
+    @property
+    def num_offsets(self):
+        return int(self._info().stencil_width) ** 3
+
+    @property
+    def num_spread(self):
+        return self._N * self.num_offsets
+
+    @property
+    def num_contract(self):
+        return self._N * self.num_offsets
+
+    @property
+    def table_size(self):
+        return 9001
+
+    @property
+    def solve_quadrupoles(self):
+        return False
+
+    @property
+    def num_quads(self):
+        return 0
+
+    @property
+    def num_grid(self):
+        return np.array(list(self._info().grid), dtype=np.int32)
+
+    @property
+    def grid_spacing(self):
+        return np.array(list(self._info().grid_spacing))
+
+    @property
+    def spectral_split(self):
+        return np.array(list(self._info().spectral_split))
+
+    # ---- updates ----
+    def update_particle_coordinates(self, x_part, y_part, z_part):
+        x, y, z = _vec(x_part), _vec(y_part), _vec(z_part)
+        if not (x.size == y.size == z.size):
+            raise ValueError("Base_Electric_Field: coordinate vectors must have equal size.")
+        xyz = np.ascontiguousarray(np.stack([x, y, z], axis=1))
+        check(C.load().mpm_op_set_positions(self._h, C.dptr(xyz), x.size))
+        if x.size != self._N:
+            self._N = x.size
+            self._dip = np.zeros((self._N, 3), np.complex128)
+            self._sc = np.zeros(self._N, np.complex128)
+            self._epoint = np.zeros((self._N, 3), np.complex128)
+            self._xbuf = C.DeviceBuffer(self._dip.nbytes)
+            self._ybuf = C.DeviceBuffer(self._dip.nbytes)
+        self._push_self()
+
+    def update_dipoles(self, dip_x, dip_y, dip_z):
+        self.update_dipoles_complex(dip_x, np.zeros(self._N), dip_y, np.zeros(self._N), dip_z, np.zeros(self._N))
+
+    def update_dipoles_complex(self, dip_xr, dip_xi, dip_yr, dip_yi, dip_zr, dip_zi):
+        comps = [_vec(v) for v in (dip_xr, dip_xi, dip_yr, dip_yi, dip_zr, dip_zi)]
+        if any(c.size != self._N for c in comps):
+            raise ValueError(
+                "Base_Electric_Field: complex dipole component vectors must match the allocated number of particles.")
+        self._dip = np.stack([comps[0] + 1j * comps[1], comps[2] + 1j * comps[3], comps[4] + 1j * comps[5]], axis=1)
+
+    def set_dipoles(self, p):
+        """Convenience: p of shape (N, 3) complex."""
+        p = np.asarray(p, np.complex128).reshape(self._N, 3)
+        self._dip = p.copy()
+
+    def set_self_coef(self, self_coef_r, self_coef_i=None):
+        if self_coef_i is None:
+            if np.iterable(self_coef_r):
+                sc = _vec(self_coef_r).astype(np.complex128)
+            else:
+                sc = np.full(self._N, complex(float(self_coef_r), 0.0))
+        else:
+            r = np.broadcast_to(np.asarray(self_coef_r, np.float64), (self._N,))
+            i = np.broadcast_to(np.asarray(self_coef_i, np.float64), (self._N,))
+            sc = (r + 1j * i).astype(np.complex128)
+        if sc.size != self._N:
+            raise ValueError("self coefficients must match the number of particles")
+        self._sc = np.ascontiguousarray(sc)
+        self._push_self()
+
+    def _push_self(self):
+        if self._N and self._sc is not None:
+            check(C.load().mpm_op_set_self_coef(self._h, C.dptr(self._sc.view(np.float64)), 1))
+
+    def set_bloch_wavevector(self, kx: float, ky: float):
+        check(C.load().mpm_op_set_bloch(self._h, float(kx), float(ky)))
+
+    # ---- evaluation ----
+    def _apply(self, parts):
+        if not self._N:
+            raise RuntimeError("electricField: particle coordinates have not been set")
+        self._xbuf.upload(np.ascontiguousarray(self._dip))
+        check(C.load().mpm_op_apply(self._h, self._xbuf.ptr, self._ybuf.ptr, 1, int(self._prec), int(parts), None))
+        out = np.zeros((self._N, 3), np.complex128)
+        self._ybuf.download(out)
+        self._epoint = out
+
+    def calculate(self):
+        self._apply(C.PART_ALL)
+
+    def clear_epoint(self):
+        if self._epoint is not None:
+            self._epoint[...] = 0
+
+    def real_space(self, E_point_ptr=None):
+        self._apply(C.PART_REAL)
+
+    def compute_real_field(self):
+        self.clear_epoint()
+        self.real_space(None)
+        return self.get_epoint_host()
+
+    def compute_inverse_field(self):
+        self.calculate()
+        total = self.get_epoint_host()
+        real = self.compute_real_field()
+        return total - real
+
+    def get_epoint_host(self, reshape=True):
+        e = self._epoint.copy()
+        return e if reshape else e.reshape(-1)
+
+    def get_dipoles_complex_host(self):
+        p = self._dip
+        return (p[:, 0].real.copy(), p[:, 0].imag.copy(), p[:, 1].real.copy(), p[:, 1].imag.copy(),
+                p[:, 2].real.copy(), p[:, 2].imag.copy())

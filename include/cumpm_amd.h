@@ -1,0 +1,211 @@
+/*
+ * cumpm_amd — C ABI of the MI355X mutual-polarization (MPM) solver.
+ *
+ * Plain C types only (pointers, sizes, enums); no exceptions cross this
+ * boundary and every call returns an mpm_status.  Complex numbers are two
+ * consecutive doubles (re, im), i.e. the layout of std::complex<double>,
+ * numpy complex128 and HIP double2.  Vectors of unknowns use the reference's
+ * layout [rhs][particle][component] (dipoles only: n = 3N per right-hand side).
+ * Lengths passed to the operator are nondimensional (radius 1); the driver
+ * (mpm_dipole_solver_*) takes physical units and nondimensionalises exactly
+ * as the reference does.
+ *
+ * Threading: one handle per host thread; handles hold device memory on the
+ * device that was current at creation.  Calls that take a `stream` argument
+ * enqueue on that HIP stream (NULL = the default stream) and return once the
+ * work is enqueued, except where noted.
+ */
+#ifndef CUMPM_AMD_H
+#define CUMPM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    MPM_OK = 0,
+    MPM_ERR_INVALID_ARGUMENT = 1, /* maps to ValueError in Python (reference: std::invalid_argument) */
+    MPM_ERR_RUNTIME = 2,          /* maps to RuntimeError (reference: std::runtime_error) */
+    MPM_ERR_HIP = 3,              /* HIP runtime failure (the reference exit()s the process instead) */
+    MPM_ERR_FFT = 4,              /* rocFFT failure */
+    MPM_ERR_UNSUPPORTED = 5,      /* feature of the reference not yet provided by this build */
+    MPM_ERR_OUT_OF_MEMORY = 6
+} mpm_status;
+
+/* Reciprocal-space arithmetic of one operator apply.
+ * Reference: PrecisionMode {AUTO, MIXED, DOUBLE} (ewald_electric_field_base.cu:28-34);
+ * AUTO and MIXED both select FP32 spread/FFT/scale/contract. */
+typedef enum {
+    MPM_PREC_DOUBLE = 0, /* FP64 throughout */
+    MPM_PREC_MIXED = 1,  /* FP32 reciprocal pipeline, FP64 real space / self / accumulation */
+    MPM_PREC_AUTO = 2    /* same as MIXED (reference default) */
+} mpm_precision;
+
+/* Precision of a whole solve. */
+typedef enum {
+    MPM_SOLVE_DOUBLE = 0,  /* every apply FP64 (reference precision="double") */
+    MPM_SOLVE_MIXED = 1,   /* every apply FP32-reciprocal (reference AUTO/MIXED) */
+    MPM_SOLVE_REFINED = 2  /* FP32-reciprocal Arnoldi applies, FP64 residual b - Ax at every restart */
+} mpm_solve_precision;
+
+typedef enum { MPM_GMRES = 0, MPM_BICGSTAB = 1 } mpm_method;
+
+/* Operator-apply parts (reference: calculate() = both; compute_real_field() = real only). */
+enum { MPM_PART_REAL = 1, MPM_PART_RECIP = 2, MPM_PART_ALL = 3 };
+
+typedef struct mpm_op mpm_op;
+
+typedef struct {
+    double box[3];   /* nondimensional box lengths */
+    double errortol; /* Ewald accuracy (the reference passes the Krylov tol here) */
+    double xi;       /* Ewald splitting parameter */
+    int grid[3];     /* 0 = reference sizing rule; >0 overrides that axis */
+    int support;     /* num_grid_gaussian P; 0 = reference rule ceil(-2 ln tol / pi) */
+} mpm_op_desc;
+
+typedef struct {
+    int64_t num_particles;
+    int64_t num_pairs;   /* real-space neighbour pairs (each ordered pair counted) */
+    int grid[3];
+    int support;         /* P */
+    int stencil_width;   /* 2*floor(P/2)+1 points per axis */
+    double grid_spacing[3];
+    double spectral_split[3];
+    double rc;
+    double self_perp;    /* real-space self term (table entry 0) */
+    double self_coef_ewald; /* -4 xi^3 / (3 sqrt(pi)) */
+    double bloch[2];
+} mpm_op_info;
+
+/* Monodisperse_Ewald_Electric_Field(box_x, box_y, box_z, errortol, xi, SOLVER_AX, radius=1)
+ * — src/cpp/electric_field/monodisperse_ewald_electric_field.cu:28-74, sizing :546-616,
+ * tables :92-542.  Fails with MPM_ERR_RUNTIME if rc > L/2 (:553-556). */
+mpm_status mpm_op_create(const mpm_op_desc* desc, mpm_op** out);
+void mpm_op_destroy(mpm_op* op);
+mpm_status mpm_op_get_info(const mpm_op* op, mpm_op_info* info);
+
+/* updateParticleCoordinates — src/cpp/electric_field/base_electric_field.cu:40-118 plus the
+ * lazy per-frame precalcs of monodisperse_ewald_electric_field.cu:2485-2503 (neighbour list,
+ * real-space pair functions, spread/contract stencils).  xyz: host, [N][3], nondimensional. */
+mpm_status mpm_op_set_positions(mpm_op* op, const double* xyz, int64_t n_particles);
+
+/* setBlochWavevector — src/cpp/electric_field/ewald_electric_field_base.h:136-142. */
+mpm_status mpm_op_set_bloch(mpm_op* op, double kx, double ky);
+
+/* setSelfCoef — src/cpp/electric_field/base_electric_field.cu (setSelfCoef), one row of N
+ * complex coefficients per right-hand side: coef is host [n_rows][N] complex. */
+mpm_status mpm_op_set_self_coef(mpm_op* op, const double* coef, int n_rows);
+
+/* calculate() — monodisperse_ewald_electric_field.cu:2211-2411 for B right-hand sides at
+ * once: y[b] = A x[b] with self-coefficient row b (row 0 if only one row was set).
+ * x, y: device, [B][N][3] complex.  parts: MPM_PART_*.  precision: mpm_precision. */
+mpm_status mpm_op_apply(mpm_op* op, const double* x, double* y, int B, int precision, int parts,
+                        void* stream);
+
+typedef struct {
+    int method;     /* mpm_method */
+    int restart;    /* GMRES restart length (reference default 50) */
+    int maxiter;    /* reference default 1000 (counts Arnoldi steps / BiCGSTAB iterations) */
+    double tol;     /* stop when ||M^-1 (b - A x)|| < tol ||b|| (gmres_solver.cu:99-101) */
+    int jacobi;     /* left Jacobi preconditioner M = diag(self_coef) (numerical_solver.cu:207-270) */
+    int precision;  /* mpm_solve_precision */
+} mpm_solve_desc;
+
+typedef struct {
+    int iters;          /* Arnoldi steps (GMRES) or iterations (BiCGSTAB) */
+    int matvecs;        /* operator applies of either precision */
+    int matvecs_fp64;   /* FP64 applies */
+    int converged;
+    double relres_pre;  /* ||M^-1(b - A x)|| / ||b||, FP64 operator (reference criterion) */
+    double relres_true; /* ||b - A x|| / ||b||, FP64 operator */
+} mpm_solve_report;
+
+/* GMRES_Solver::solve / BiCGSTAB_Solver::solve — src/cpp/numerical_solver/gmres_solver.cu:56-208,
+ * bicgstab_solver.cu:58-192 — for B systems at once.  System b uses self-coefficient row b.
+ * b, x: device [B][N][3] complex; x holds x0 on entry and the solution on return.
+ * reports: host, B entries (may be NULL).  Synchronises `stream` before returning. */
+mpm_status mpm_solve(mpm_op* op, const mpm_solve_desc* desc, const double* b, double* x, int B,
+                     mpm_solve_report* reports, void* stream);
+
+/* ---- Dipole_Solver driver (src/cpp/dipole_solver.cu, dipole_solver_bindings.cpp:80-179) ---- */
+typedef struct mpm_dipole_solver mpm_dipole_solver;
+
+typedef enum { MPM_GUESS_DERIVATIVE = 0, MPM_GUESS_MEAN_FIELD = 1, MPM_GUESS_PREVIOUS = 2 } mpm_guess;
+typedef enum { MPM_FIELD_AUTO = 0, MPM_FIELD_MONODISPERSE = 1, MPM_FIELD_POLYDISPERSE = 2,
+               MPM_FIELD_DIRECT = 3 } mpm_field_type;
+
+typedef struct {
+    double box[3];            /* physical */
+    const double* eps_p;      /* host complex [W][eps_cols], eps_cols = 1 or N (copied) */
+    int64_t num_wavevectors;  /* W */
+    int64_t eps_cols;
+    const double* radius;     /* host [num_radii] (copied); num_radii 0 => 1.0 */
+    int64_t num_radii;
+    double eps_m;
+    double xi;
+    double tol;
+    int quiet;
+    int guess;                /* mpm_guess */
+    int method;               /* mpm_method */
+    int field_type;           /* mpm_field_type */
+    const double* E0;         /* host complex [K][E0_len] (copied); K = 0 => identity (3 RHS) */
+    int64_t num_E0;           /* K */
+    int64_t E0_len;           /* 3 or 3N */
+    int solve_quadrupoles;    /* must be 0 in this build */
+    int precision;            /* mpm_precision: DOUBLE => MPM_SOLVE_DOUBLE, AUTO/MIXED => MPM_SOLVE_MIXED */
+    int refined;              /* 1 => MPM_SOLVE_REFINED instead of MPM_SOLVE_MIXED (ignored for DOUBLE) */
+    int batched;              /* 0 = reference order (one RHS at a time); 1 = all W x K RHS in one solve */
+    int restart;              /* 0 => 50 */
+    int maxiter;              /* 0 => 1000 */
+    int grid[3];              /* Ewald overrides (0 = reference rule) */
+    int support;
+    double ewald_tol;         /* 0 => tol (the reference ties the Ewald accuracy to tol) */
+} mpm_dipole_desc;
+
+mpm_status mpm_dipole_solver_create(const mpm_dipole_desc* desc, mpm_dipole_solver** out);
+void mpm_dipole_solver_destroy(mpm_dipole_solver* s);
+/* Dipole_Solver::compute (dipole_solver.cu:495-607): one frame of positions (physical units). */
+mpm_status mpm_dipole_solver_compute(mpm_dipole_solver* s, const double* x, const double* y,
+                                     const double* z, int64_t n_particles);
+/* dims = {frames, wavevectors, particles, polarizations} */
+mpm_status mpm_dipole_solver_get_dims(const mpm_dipole_solver* s, int64_t dims[4]);
+/* get_eff_polarizability: host complex [W][K][3] (dipole_solver.cu:609-625). */
+mpm_status mpm_dipole_solver_get_eff_polarizability(const mpm_dipole_solver* s, int physical,
+                                                    double* out);
+/* get_dipoles: host complex [F][W][N][K][3] (dipole_solver.cu:627-635). */
+mpm_status mpm_dipole_solver_get_dipoles(const mpm_dipole_solver* s, int physical, double* out);
+/* Per-solve reports, [F][W][K] (this build's addition: the reference reports none). */
+mpm_status mpm_dipole_solver_get_reports(const mpm_dipole_solver* s, mpm_solve_report* out);
+/* Ewald operator of the solver (valid after the first compute), for inspection. */
+mpm_status mpm_dipole_solver_get_op_info(const mpm_dipole_solver* s, mpm_op_info* info);
+mpm_status mpm_dipole_solver_set_use_jacobi(mpm_dipole_solver* s, int enable);
+mpm_status mpm_dipole_solver_get_use_jacobi(const mpm_dipole_solver* s, int* enable);
+/* Wall time (ms) of the last compute's solve phase and setup phase. */
+mpm_status mpm_dipole_solver_get_timing(const mpm_dipole_solver* s, double* solve_ms, double* setup_ms);
+
+/* ---- Plumbing ---- */
+const char* mpm_status_string(mpm_status s);
+/* Message of the last failing call on this thread. */
+const char* mpm_last_error(void);
+/* Device buffers for callers without an allocator of their own (the Python layer). */
+mpm_status mpm_buffer_alloc(size_t bytes, void** dev);
+mpm_status mpm_buffer_free(void* dev);
+mpm_status mpm_buffer_upload(void* dev, const void* host, size_t bytes);
+mpm_status mpm_buffer_download(void* host, const void* dev, size_t bytes);
+mpm_status mpm_device_synchronize(void);
+
+/* Kernel timing with HIP events on the stream each kernel is launched on.  When enabled,
+ * every launch of the named kernel class is bracketed by events and accumulated. */
+mpm_status mpm_profile_enable(int enable);
+mpm_status mpm_profile_reset(void);
+/* name: "spread", "fft", "scale", "gather", "krylov"; total_ms over count launches. */
+mpm_status mpm_profile_get(const char* name, double* total_ms, int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUMPM_AMD_H */

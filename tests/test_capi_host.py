@@ -1,0 +1,94 @@
+"""CPU-side checks of the product boundary: the library loads, exports every
+symbol include/cumpm_amd.h declares, the ctypes structs match the header, and
+the Python API validates its arguments like the reference (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cumpm_amd
+from cumpm_amd import _capi as C
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cumpm_amd.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(mpm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.load()
+    names = _declared()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(C.EXPORTED)
+
+
+def test_status_strings():
+    lib = C.load()
+    for s in range(7):
+        assert lib.mpm_status_string(s)
+
+
+def _c_struct_sizes():
+    """Compile a tiny C program against the header and print sizeof of each struct."""
+    import subprocess
+    import tempfile
+    src = r'''
+#include <stdio.h>
+#include "cumpm_amd.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu\n", sizeof(mpm_op_desc), sizeof(mpm_op_info), sizeof(mpm_solve_desc),
+         sizeof(mpm_solve_report), sizeof(mpm_dipole_desc));
+  return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "s.c")
+        open(c, "w").write(src)
+        exe = os.path.join(d, "s")
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    return [int(v) for v in out]
+
+
+def test_ctypes_structs_match_header():
+    sizes = _c_struct_sizes()
+    py = [ctypes.sizeof(C.OpDesc), ctypes.sizeof(C.OpInfo), ctypes.sizeof(C.SolveDesc),
+          ctypes.sizeof(C.SolveReport), ctypes.sizeof(C.DipoleDesc)]
+    assert sizes == py
+
+
+def test_drude_dielectric_matches_formula():
+    omega = np.linspace(0.2, 0.8, 10)
+    eps = cumpm_amd.drude_dielectric(omega, 0.08, 1.7, 4.0)
+    np.testing.assert_allclose(eps, 4.0 - 1.7 ** 2 / (omega ** 2 + 1j * omega * 0.08), rtol=1e-15)
+    eps2 = cumpm_amd.drude_dielectric(omega, [0.08, 0.1], [1.7, 1.8], [4.0, 3.0])
+    assert eps2.shape == (10, 2)
+    with pytest.raises(ValueError):
+        cumpm_amd.drude_dielectric(omega, [0.08, 0.1], [1.7], [4.0, 3.0])
+
+
+def test_argument_validation_like_reference():
+    # reference tests/test_inputs.py and src/cuMPM/solver.py:93-183
+    with pytest.raises(ValueError):
+        cumpm_amd.dipole_solver([10.0, 10.0], -2.0)
+    with pytest.raises(ValueError):
+        cumpm_amd.dipole_solver([10.0] * 3, -2.0, field_type="spectral")
+    with pytest.raises(ValueError):
+        cumpm_amd.dipole_solver([10.0] * 3, -2.0, precision="half")
+    with pytest.raises(ValueError):
+        cumpm_amd.dipole_solver([10.0] * 3, -2.0, E0=[1.0, 0.0])
+    with pytest.raises(TypeError):
+        cumpm_amd.dipole_solver([10.0] * 3, -2.0, quadrupoles=3.5)
+    with pytest.raises(NotImplementedError):
+        cumpm_amd.dipole_solver([10.0] * 3, -2.0, quadrupoles=True)
+    s = cumpm_amd.dipole_solver([10.0] * 3, [-2.0 + 0.1j, -1.0 + 0.1j], E0=[[1, 0, 0], [0, 0, 1]], quiet=True)
+    assert s.use_jacobi_precond is True
+    s.use_jacobi_precond = False
+    assert s.use_jacobi_precond is False

@@ -1,0 +1,266 @@
+"""GPU parity tests: the HIP path (through the C ABI) against the CPU oracle.
+
+Bars: operator applies bit-for-bit up to summation order (relative 1e-11 in
+FP64); FP32-reciprocal applies within 1e-4 of FP64 (the reference's own
+tolerance, tests/test_electric_fields.py:324-357); solves within 1e-6 of the
+oracle's solution at tol 1e-8 (north-star parity bar), and the reference's
+pyMPM fixture within its 1e-3.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from cumpm_amd import _capi as C
+from oracle import oracle as O
+from gpu_helpers import Op, rel, hex_monolayer
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _case(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+GEOMS = [
+    # name, box, errortol, xi, positions, grid override, support override
+    ("random3d", [16.0, 17.0, 18.0], 1e-6, 0.5, lambda: np.random.default_rng(1).uniform(0, 16, (40, 3)), None, 0),
+    ("outside_box", [16.0, 16.0, 16.0], 1e-6, 0.5, lambda: np.random.default_rng(2).uniform(-9, 30, (25, 3)), None, 0),
+    ("hex_slab", None, 1e-6, 0.5, None, None, 0),
+    ("sc_tight", [24.0] * 3, 1e-8, 0.5,
+     lambda: np.stack(np.meshgrid(*[np.arange(4) * 3.0 + 1.5] * 3, indexing="ij"), -1).reshape(-1, 3), None, 0),
+    ("tiny_grid_wrap", [15.0, 15.0, 15.0], 1e-4, 0.5,
+     lambda: np.random.default_rng(3).uniform(0, 15, (12, 3)), (6, 7, 5), 0),
+    ("override_32_P8", [24.0] * 3, 1e-8, 0.5, lambda: np.random.default_rng(4).uniform(0, 24, (30, 3)), (32, 32, 32), 8),
+    ("single_particle", [20.0] * 3, 1e-6, 0.5, lambda: np.array([[3.0, 4.0, 5.0]]), None, 0),
+]
+
+
+def _geom(name):
+    for g in GEOMS:
+        if g[0] == name:
+            nm, box, tol, xi, pf, grid, P = g
+            if nm == "hex_slab":
+                pos, box = hex_monolayer(10, 8, 3.0, jitter=0.04, seed=5)
+                box = np.array([box[0], box[1], 16.0])
+            else:
+                pos = pf()
+            return np.asarray(box, float), tol, xi, pos, grid, P
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name", [g[0] for g in GEOMS])
+def test_apply_fp64_matches_oracle(name):
+    box, tol, xi, pos, grid, P = _geom(name)
+    _case(f"apply fp64 {name} N={len(pos)}")
+    N = len(pos)
+    rng = np.random.default_rng(11)
+    B = 3
+    p = rng.standard_normal((B, N, 3)) + 1j * rng.standard_normal((B, N, 3))
+    rows = rng.standard_normal((B, N)) + 1j * rng.standard_normal((B, N))
+    op = Op(box, tol, xi, grid, P)
+    op.set_positions(pos)
+    op.set_self_coef(rows)
+    ref_op = O.Operator(box, tol, xi, grid, P)
+    ref_op.set_positions(pos)
+    info = op.info()
+    assert tuple(info.grid) == ref_op.info()["grid"]
+    assert info.num_pairs == ref_op.info()["num_pairs"]
+    for parts in (C.PART_REAL, C.PART_RECIP, C.PART_ALL):
+        got = op.apply(p, C.PREC_DOUBLE, parts)
+        for b in range(B):
+            ref = ref_op.apply(p[b], rows[b], parts)
+            if np.abs(ref).max() == 0:
+                assert np.abs(got[b]).max() == 0
+                continue
+            assert rel(got[b], ref) < 1e-11, (name, parts, b, rel(got[b], ref))
+
+
+@pytest.mark.parametrize("name", ["random3d", "hex_slab", "sc_tight"])
+def test_apply_fp32_reciprocal_within_reference_tolerance(name):
+    box, tol, xi, pos, grid, P = _geom(name)
+    _case(f"apply fp32 {name}")
+    N = len(pos)
+    rng = np.random.default_rng(12)
+    p = rng.standard_normal((2, N, 3)) + 1j * rng.standard_normal((2, N, 3))
+    rows = rng.standard_normal((2, N)) + 1j * rng.standard_normal((2, N))
+    op = Op(box, tol, xi, grid, P)
+    op.set_positions(pos)
+    op.set_self_coef(rows)
+    ref_op = O.Operator(box, tol, xi, grid, P)
+    ref_op.set_positions(pos)
+    got = op.apply(p, C.PREC_MIXED)
+    for b in range(2):
+        ref = ref_op.apply(p[b], rows[b])
+        assert rel(got[b], ref) < 1e-4
+
+
+def test_bloch_wavevector_matches_oracle():
+    box, tol, xi, pos, grid, P = _geom("hex_slab")
+    _case("bloch")
+    N = len(pos)
+    rng = np.random.default_rng(13)
+    p = rng.standard_normal((1, N, 3)) + 1j * rng.standard_normal((1, N, 3))
+    sc = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    op = Op(box, tol, xi)
+    op.set_positions(pos)
+    op.set_self_coef(sc)
+    op.set_bloch(0.15, -0.22)
+    ref_op = O.Operator(box, tol, xi)
+    ref_op.set_positions(pos)
+    ref_op.set_bloch(0.15, -0.22)
+    got = op.apply(p)
+    assert rel(got[0], ref_op.apply(p[0], sc)) < 1e-11
+
+
+@pytest.mark.parametrize("method", [C.GMRES, C.BICGSTAB])
+def test_solve_matches_oracle(method):
+    box, tol, xi, pos, grid, P = _geom("random3d")
+    _case(f"solve method={method}")
+    N = len(pos)
+    eps = np.array([-2.5 + 0.6j, -1.2 + 0.3j, 3.0 + 0.1j])
+    rows = np.stack([O.self_coefficient(np.full(N, e)) for e in eps])
+    op = Op(box, tol, xi)
+    op.set_positions(pos)
+    op.set_self_coef(rows)
+    b = np.zeros((3, N, 3), complex)
+    for k in range(3):
+        b[k, :, k] = 1.0
+    stol = 1e-8 if method == C.GMRES else 1e-7
+    x, reps = op.solve(b, tol=stol, method=method, restart=20)
+    ref_op = O.Operator(box, tol, xi)
+    ref_op.set_positions(pos)
+    for k in range(3):
+        xr, rr = ref_op.solve(b[k], rows[k], tol=stol, method="gmres" if method == C.GMRES else "bicgstab",
+                              restart=20)
+        assert reps[k]["converged"], reps[k]
+        assert rel(x[k].ravel(), xr) < 1e-6, (k, rel(x[k].ravel(), xr), reps[k], rr)
+        if method == C.GMRES:
+            # same algorithm, same stopping rule: Arnoldi step counts agree to +-1
+            assert abs(reps[k]["iters"] - rr["iters"]) <= 1, (reps[k], rr)
+
+
+def test_batched_equals_one_at_a_time():
+    box, tol, xi, pos, grid, P = _geom("hex_slab")
+    _case("batched vs sequential")
+    N = len(pos)
+    eps = np.linspace(-3.0, -1.0, 5) + 0.3j
+    rows = np.stack([O.self_coefficient(np.full(N, e)) for e in eps])
+    b = np.zeros((5, N, 3), complex)
+    for q in range(5):
+        b[q, :, q % 3] = 1.0
+    op = Op(box, tol, xi)
+    op.set_positions(pos)
+    op.set_self_coef(rows)
+    xb, rb = op.solve(b, tol=1e-9, restart=30)
+    for q in range(5):
+        op1 = Op(box, tol, xi)
+        op1.set_positions(pos)
+        op1.set_self_coef(rows[q])
+        x1, r1 = op1.solve(b[q:q + 1], tol=1e-9, restart=30)
+        assert rel(xb[q], x1[0]) < 1e-7
+        assert abs(rb[q]["iters"] - r1[0]["iters"]) <= 1
+
+
+def test_refined_precision_reaches_fp64_accuracy():
+    box, tol, xi, pos, grid, P = _geom("hex_slab")
+    _case("refined")
+    N = len(pos)
+    eps = np.array([-2.2 + 0.2j, -1.5 + 0.4j, 2.0 + 0.1j, -4.0 + 1.0j])
+    rows = np.stack([O.self_coefficient(np.full(N, e)) for e in eps])
+    b = np.zeros((4, N, 3), complex)
+    for q in range(4):
+        b[q, :, q % 3] = 1.0
+    op = Op(box, tol, xi)
+    op.set_positions(pos)
+    op.set_self_coef(rows)
+    x64, r64 = op.solve(b, tol=1e-8, restart=30, precision=C.SOLVE_DOUBLE)
+    xr, rr = op.solve(b, tol=1e-8, restart=30, precision=C.SOLVE_REFINED)
+    for q in range(4):
+        assert rr[q]["converged"] and rr[q]["relres_pre"] < 1e-8, rr[q]
+        assert rel(xr[q], x64[q]) < 1e-6
+        assert rr[q]["matvecs_fp64"] < rr[q]["matvecs"]
+
+
+def test_zero_rhs_returns_zero():
+    box, tol, xi, pos, grid, P = _geom("random3d")
+    _case("zero rhs")
+    N = len(pos)
+    op = Op(box, tol, xi)
+    op.set_positions(pos)
+    op.set_self_coef(O.self_coefficient(np.full(N, -2.0 + 0.5j)))
+    x, reps = op.solve(np.zeros((1, N, 3)), x0=np.ones((1, N, 3)))
+    assert np.all(x == 0) and reps[0]["converged"]
+
+
+def test_maxiter_is_respected():
+    box, tol, xi, pos, grid, P = _geom("random3d")
+    _case("maxiter")
+    N = len(pos)
+    op = Op(box, tol, xi)
+    op.set_positions(pos)
+    op.set_self_coef(O.self_coefficient(np.full(N, -2.0 + 0.01j)))
+    b = np.zeros((1, N, 3), complex)
+    b[0, :, 0] = 1
+    x, reps = op.solve(b, tol=1e-14, maxiter=7, restart=5)
+    assert reps[0]["iters"] == 7 and not reps[0]["converged"]
+
+
+def test_dipole_solver_golden_fixture():
+    """Reference tests/test_dipole_solvers.py:8-60 through the product API."""
+    import cumpm_amd
+    _case("golden fixture")
+    ref = np.load(os.path.join(GOLDEN, "ref_dipole_monodisperse.npz"))
+    d, a, eps_m = 12.0, 4.0, 2.13
+    omega = np.linspace(2000, 6000, 3)
+    eps = 4.0 - 12000 ** 2 / (omega ** 2 + 1j * omega * 700)
+    pos = np.array([[0, 0, 0], [d, 0, 0], [0, d, 0], [d, d, 0]], float)
+    epsm = np.repeat(eps[:, None], 4, axis=1)
+    for solver_type in ("bicgstab", "gmres"):
+        s = cumpm_amd.dipole_solver(np.array([5 * d] * 3), epsm, radius=a, eps_m=eps_m, tol=1e-4,
+                                    field_type="monodisperse", solver_type=solver_type, quiet=True,
+                                    precision="double")
+        s.compute(pos)
+        np.testing.assert_allclose(s.get_eff_polarizability(physical=False), ref["alpha_py"], rtol=1e-3, atol=1e-3)
+        np.testing.assert_allclose(s.get_dipoles(physical=False), ref["dips_py"], rtol=1e-3, atol=1e-3)
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_dipole_solver_matches_oracle_driver(batched):
+    import cumpm_amd
+    _case(f"driver batched={batched}")
+    pos, box = hex_monolayer(10, 12, 11.0, jitter=0.2, seed=2)
+    box = np.array([box[0], box[1], 100.0])
+    omega = np.linspace(0.2, 0.8, 4)
+    eps = 4.0 - 1.7 ** 2 / (omega ** 2 + 1j * omega * 0.08)
+    s = cumpm_amd.dipole_solver(box, eps, radius=5.0, eps_m=1.0, tol=1e-8, quiet=True, precision="double",
+                                batched=batched)
+    s.compute(pos)
+    od = O.DipoleSolver(box, eps.reshape(-1, 1), 5.0, 1.0, tol=1e-8)
+    od.compute(pos)
+    assert rel(s.get_dipoles(), od.get_dipoles()) < 1e-6
+    assert rel(s.get_eff_polarizability(), od.get_eff_polarizability()) < 1e-6
+    reps = s.get_reports()
+    assert all(r["converged"] for w in reps[0] for r in w)
+
+
+def test_dipole_solver_input_errors_need_gpu():
+    import cumpm_amd
+    _case("input errors")
+    with pytest.raises(RuntimeError, match="half the box"):
+        s = cumpm_amd.dipole_solver([8.0, 8.0, 8.0], -2.0 + 0.1j, tol=1e-6, quiet=True)
+        s.compute(np.array([[1.0, 1.0, 1.0], [4.0, 4.0, 4.0]]))
+    s = cumpm_amd.dipole_solver([40.0] * 3, -2.0 + 0.1j, radius=[1.0, 1.2], field_type="monodisperse", quiet=True)
+    with pytest.raises(RuntimeError, match="not identical"):
+        s.compute(np.array([[1.0, 1.0, 1.0], [6.0, 6.0, 6.0]]))
+    s = cumpm_amd.dipole_solver([40.0] * 3, -2.0 + 0.1j, radius=[1.0, 1.2, 1.0], quiet=True)
+    with pytest.raises(RuntimeError, match="inconsistent"):
+        s.compute(np.array([[1.0, 1.0, 1.0], [6.0, 6.0, 6.0]]))
+    s = cumpm_amd.dipole_solver([40.0] * 3, -2.0 + 0.1j, quiet=True)
+    s.compute(np.array([[1.0, 1.0, 1.0], [6.0, 6.0, 6.0]]))
+    with pytest.raises(RuntimeError, match="changed"):
+        s.compute(np.array([[1.0, 1.0, 1.0]]))
