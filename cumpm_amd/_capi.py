@@ -94,7 +94,8 @@ _SIGS = {
     "mpm_dipole_solver_get_op_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OpInfo)]),
     "mpm_dipole_solver_set_use_jacobi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "mpm_dipole_solver_get_use_jacobi": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
-    "mpm_dipole_solver_get_timing": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "mpm_dipole_solver_get_timing": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "mpm_dipole_solver_reset": (ctypes.c_int, [ctypes.c_void_p]),
     "mpm_buffer_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
     "mpm_buffer_free": (ctypes.c_int, [ctypes.c_void_p]),
     "mpm_buffer_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),

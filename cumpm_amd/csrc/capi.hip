@@ -81,7 +81,7 @@ struct mpm_dipole_solver {
     std::vector<cplx> dips;      // [F][W][N][K][3]
     std::vector<cplx> avg_dips;  // [F][W][K][3]
     std::vector<mpm_solve_report> reports;  // [F][W][K]
-    double solve_ms = 0, setup_ms = 0;
+    double solve_ms = 0, setup_ms = 0, total_ms = 0;
     mpm::DevBuf<double2> d_b, d_x;
     int indent = 0;
 
@@ -244,7 +244,14 @@ struct mpm_dipole_solver {
         std::vector<mpm_solve_report> reps(W * K);
         MPM_HIP(hipDeviceSynchronize());
         auto t1 = std::chrono::steady_clock::now();
-        const mpm_solve_desc sd = solve_desc();
+        double device_ms = 0.0;  // wall time of the device solves only (inputs resident)
+        auto timed_solve = [&](const double2* b, double2* x, int B, mpm_solve_report* r) {
+            MPM_HIP(hipDeviceSynchronize());
+            auto a = std::chrono::steady_clock::now();
+            mpm::solve_batched(*op, solve_desc(), b, x, B, r, nullptr);
+            MPM_HIP(hipDeviceSynchronize());
+            device_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+        };
         auto store = [&](size_t w, size_t k, const std::vector<cplx>& sol) {
             for (size_t p = 0; p < N; ++p)
                 for (int c = 0; c < 3; ++c) fd[(w * N + p) * K * 3 + k * 3 + c] = sol[3 * p + c];
@@ -266,7 +273,7 @@ struct mpm_dipole_solver {
                         for (int c = 0; c < 3; ++c) x0[3 * p + c] = g[p * K * 3 + k * 3 + c];
                     MPM_HIP(hipMemcpy(d_b.p, b.data(), n3 * sizeof(double2), hipMemcpyHostToDevice));
                     MPM_HIP(hipMemcpy(d_x.p, x0.data(), n3 * sizeof(double2), hipMemcpyHostToDevice));
-                    mpm::solve_batched(*op, sd, d_b.p, d_x.p, 1, &reps[w * K + k], nullptr);
+                    timed_solve(d_b.p, d_x.p, 1, &reps[w * K + k]);
                     std::vector<cplx> sol(n3);
                     MPM_HIP(hipMemcpy(sol.data(), d_x.p, n3 * sizeof(double2), hipMemcpyDeviceToHost));
                     store(w, k, sol);
@@ -298,7 +305,7 @@ struct mpm_dipole_solver {
             d_x.reserve(B * n3);
             MPM_HIP(hipMemcpy(d_b.p, b_all.data(), B * n3 * sizeof(double2), hipMemcpyHostToDevice));
             MPM_HIP(hipMemcpy(d_x.p, x_all.data(), B * n3 * sizeof(double2), hipMemcpyHostToDevice));
-            mpm::solve_batched(*op, sd, d_b.p, d_x.p, (int)B, reps.data(), nullptr);
+            timed_solve(d_b.p, d_x.p, (int)B, reps.data());
             MPM_HIP(hipMemcpy(x_all.data(), d_x.p, B * n3 * sizeof(double2), hipMemcpyDeviceToHost));
             for (size_t w = 0; w < W; ++w)
                 for (size_t k = 0; k < K; ++k)
@@ -308,7 +315,8 @@ struct mpm_dipole_solver {
         MPM_HIP(hipDeviceSynchronize());
         auto t2 = std::chrono::steady_clock::now();
         setup_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-        solve_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+        total_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
+        solve_ms = device_ms;
         // alpha_eff: mean dipole per (w, k)
         for (size_t w = 0; w < W; ++w)
             for (size_t k = 0; k < K; ++k)
@@ -534,11 +542,23 @@ mpm_status mpm_dipole_solver_get_use_jacobi(const mpm_dipole_solver* s, int* ena
     });
 }
 
-mpm_status mpm_dipole_solver_get_timing(const mpm_dipole_solver* s, double* solve_ms, double* setup_ms) {
+mpm_status mpm_dipole_solver_get_timing(const mpm_dipole_solver* s, double* solve_ms, double* setup_ms,
+                                        double* total_ms) {
     return guarded([&] {
-        MPM_REQUIRE(s && solve_ms && setup_ms, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        MPM_REQUIRE(s && solve_ms && setup_ms && total_ms, MPM_ERR_INVALID_ARGUMENT, "null argument");
         *solve_ms = s->solve_ms;
         *setup_ms = s->setup_ms;
+        *total_ms = s->total_ms;
+    });
+}
+
+mpm_status mpm_dipole_solver_reset(mpm_dipole_solver* s) {
+    return guarded([&] {
+        MPM_REQUIRE(s, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        s->frames = 0;
+        s->dips.clear();
+        s->avg_dips.clear();
+        s->reports.clear();
     });
 }
 

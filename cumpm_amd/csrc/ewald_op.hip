@@ -690,6 +690,7 @@ EwaldOp::EwaldOp(const mpm_op_desc& d) {
         nbins[k] = (S.n[k] + tile[k] - 1) / tile[k];
     }
     fft = std::make_unique<FftPlans>(S.n);
+    table = std::make_unique<PairTable>(xi);
     build_kspace();
 }
 
@@ -720,6 +721,11 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     const int Nn = (int)n;
     for (int64_t q = 0; q < 3 * n; ++q)
         MPM_REQUIRE(std::isfinite(xyz[q]), MPM_ERR_INVALID_ARGUMENT, "non-finite position");
+    // unchanged positions: the per-frame precalcs are still valid
+    if (n == N && last_xyz.size() == (size_t)(3 * n) &&
+        std::memcmp(last_xyz.data(), xyz, 3 * n * sizeof(double)) == 0)
+        return;
+    last_xyz.assign(xyz, xyz + 3 * n);
     N = n;
     const int W = S.width, half = S.half;
     std::vector<double> pos(3 * (size_t)Nn);
@@ -761,7 +767,7 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     }
     // real-space neighbours: all j != i with min-image distance < rc (neighbor_list.cu
     // keeps d^2 <= rc^2 and the kernel then skips d >= rc; pairs at exactly rc add zero)
-    PairTable tab(xi);
+    const PairTable& tab = *table;
     const double rc = S.rc, rc2 = rc * rc;
     int nc[3];
     for (int d = 0; d < 3; ++d) nc[d] = std::max(1, (int)std::floor(L[d] / rc));

@@ -106,7 +106,8 @@ struct SlotTable {
     int row[kMaxSlots];  // self-coefficient row of the slot
 };
 
-struct FftPlans;  // rocFFT plans keyed by (precision, batch)
+struct FftPlans;   // rocFFT plans keyed by (precision, batch)
+struct PairTable;  // sampled real-space pair functions
 
 class EwaldOp {
    public:
@@ -156,6 +157,8 @@ class EwaldOp {
     DevBuf<double2> d_grid64;
     DevBuf<float2> d_grid32;
     std::unique_ptr<FftPlans> fft;
+    std::unique_ptr<PairTable> table;
+    std::vector<double> last_xyz;  // positions of the current precalcs
 };
 
 // Batched Krylov solve (krylov.hip).

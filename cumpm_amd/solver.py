@@ -236,10 +236,16 @@ class dipole_solver:
         return [[[flat[(f * W + w) * K + k] for k in range(K)] for w in range(W)] for f in range(F)]
 
     def get_timing(self):
-        """(solve_ms, setup_ms) of the last compute()."""
-        a, b = ctypes.c_double(), ctypes.c_double()
-        check(C.load().mpm_dipole_solver_get_timing(self._h, ctypes.byref(a), ctypes.byref(b)))
-        return a.value, b.value
+        """(solve_ms, setup_ms, total_ms) of the last compute(); solve_ms covers the device
+        solves only (inputs already in device memory)."""
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(C.load().mpm_dipole_solver_get_timing(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def reset(self):
+        """Drop stored frames (keeps the operator and FFT plans); the next compute() is a
+        first frame again."""
+        check(C.load().mpm_dipole_solver_reset(self._h))
 
     def get_op_info(self):
         info = C.OpInfo()

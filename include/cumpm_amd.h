@@ -183,8 +183,15 @@ mpm_status mpm_dipole_solver_get_reports(const mpm_dipole_solver* s, mpm_solve_r
 mpm_status mpm_dipole_solver_get_op_info(const mpm_dipole_solver* s, mpm_op_info* info);
 mpm_status mpm_dipole_solver_set_use_jacobi(mpm_dipole_solver* s, int enable);
 mpm_status mpm_dipole_solver_get_use_jacobi(const mpm_dipole_solver* s, int* enable);
-/* Wall time (ms) of the last compute's solve phase and setup phase. */
-mpm_status mpm_dipole_solver_get_timing(const mpm_dipole_solver* s, double* solve_ms, double* setup_ms);
+/* Wall time (ms) of the last compute: device solves only (right-hand sides and guesses
+ * already in device memory), set-up (first-call operator construction and per-frame
+ * precalcs), and the whole call. */
+mpm_status mpm_dipole_solver_get_timing(const mpm_dipole_solver* s, double* solve_ms, double* setup_ms,
+                                        double* total_ms);
+/* Forget the stored frames (results and warm-start state) but keep the operator, its FFT
+ * plans and per-frame precalcs; the next compute() starts from the mean-field guess as a
+ * first frame does.  No counterpart in the reference (its results only accumulate). */
+mpm_status mpm_dipole_solver_reset(mpm_dipole_solver* s);
 
 /* ---- Plumbing ---- */
 const char* mpm_status_string(mpm_status s);
