@@ -228,6 +228,7 @@ bool profile_get(const std::string& name, double& ms, int64_t& count) {
     return true;
 }
 
+
 // ---------------------------------------------------------------------------
 // rocFFT plans
 // ---------------------------------------------------------------------------
@@ -241,19 +242,22 @@ bool profile_get(const std::string& name, double& ms, int64_t& count) {
 
 static std::once_flag g_rocfft_once;
 
+// Batched in-place complex transforms of `per_rhs` contiguous arrays per right-hand
+// side (3 components for the full 3-D grid; 3 x Zs planes for the slab's 2-D
+// transforms).  Plans are created per (precision, batch bucket) on first use.
 struct FftPlans {
     struct Pair {
         rocfft_plan fwd = nullptr, inv = nullptr;
         size_t work = 0;
     };
-    int n[3];
-    std::map<std::pair<int, int>, Pair> plans;  // (fp32, rhs batch) -> plans
+    std::vector<size_t> lengths;  // fastest first
+    int per_rhs = 3;
+    std::map<std::pair<int, int>, Pair> plans;
     DevBuf<char> work;
     rocfft_execution_info info = nullptr;
 
-    explicit FftPlans(const int nn[3]) {
+    FftPlans(std::vector<size_t> len, int per) : lengths(std::move(len)), per_rhs(per) {
         std::call_once(g_rocfft_once, [] { rocfft_setup(); });
-        std::memcpy(n, nn, sizeof(n));
         MPM_FFT(rocfft_execution_info_create(&info));
     }
     ~FftPlans() {
@@ -268,28 +272,25 @@ struct FftPlans {
         auto it = plans.find(key);
         if (it != plans.end()) return it->second;
         Pair p;
-        const size_t lengths[3] = {(size_t)n[2], (size_t)n[1], (size_t)n[0]};  // fastest first
         const rocfft_precision prec = fp32 ? rocfft_precision_single : rocfft_precision_double;
-        MPM_FFT(rocfft_plan_create(&p.fwd, rocfft_placement_inplace,
-                                   rocfft_transform_type_complex_forward, prec, 3, lengths,
-                                   (size_t)batch * 3, nullptr));
-        MPM_FFT(rocfft_plan_create(&p.inv, rocfft_placement_inplace,
-                                   rocfft_transform_type_complex_inverse, prec, 3, lengths,
-                                   (size_t)batch * 3, nullptr));
+        MPM_FFT(rocfft_plan_create(&p.fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward, prec,
+                                   lengths.size(), lengths.data(), (size_t)batch * per_rhs, nullptr));
+        MPM_FFT(rocfft_plan_create(&p.inv, rocfft_placement_inplace, rocfft_transform_type_complex_inverse, prec,
+                                   lengths.size(), lengths.data(), (size_t)batch * per_rhs, nullptr));
         size_t w1 = 0, w2 = 0;
         MPM_FFT(rocfft_plan_get_work_buffer_size(p.fwd, &w1));
         MPM_FFT(rocfft_plan_get_work_buffer_size(p.inv, &w2));
         p.work = std::max(w1, w2);
         return plans.emplace(key, p).first->second;
     }
-    // Transform `B` RHS grids (3 components each) starting at `grid`, decomposing the
-    // batch into power-of-two buckets so that a handful of plans cover any B.
+    // Transform B right-hand sides starting at `grid`, decomposing the batch into
+    // power-of-two buckets so that a handful of plans cover any B.
     void exec(void* grid, size_t bytes_per_rhs, int B, bool fp32, bool forward, hipStream_t s) {
         char* base = static_cast<char*>(grid);
         int done = 0;
         while (done < B) {
             int bucket = 1;
-            while (bucket * 2 <= B - done && bucket * 2 <= 32) bucket *= 2;
+            while (bucket * 2 <= B - done && bucket * 2 <= kMaxSlots) bucket *= 2;
             Pair& p = get(fp32, bucket);
             work.reserve(p.work);
             MPM_FFT(rocfft_execution_info_set_stream(info, s));
@@ -315,33 +316,41 @@ struct C2<float> {
     using T = float2;
 };
 
-// k-space coefficients: (k^x, k^y, k^z, sc) per voxel.  :1300-1404.
-__global__ void kspace_kernel(double* __restrict__ k64, float* __restrict__ k32, int nx, int ny,
-                              int nz, double Lx, double Ly, double Lz, double kbx, double kby,
-                              double xi, double etax, double etay, double etaz, double invV) {
-    const size_t V = (size_t)nx * ny * nz;
-    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < V;
-         v += (size_t)gridDim.x * blockDim.x) {
-        const int ix = (int)(v / ((size_t)ny * nz));
-        const int iy = (int)((v / nz) % ny);
-        const int iz = (int)(v % nz);
-        const double fx = (ix <= (nx - 1) / 2) ? ix : (ix - nx);
-        const double fy = (iy <= (ny - 1) / 2) ? iy : (iy - ny);
-        const double fz = (iz <= (nz - 1) / 2) ? iz : (iz - nz);
-        const double twopi = 2.0 * 3.14159265358979323846;
-        const double Gx = fx * twopi / Lx, Gy = fy * twopi / Ly, Gz = fz * twopi / Lz;
-        const double kx = Gx - kbx, ky = Gy - kby, kz = Gz;
-        const double k2 = kx * kx + ky * ky + kz * kz;
-        const double k = sqrt(k2);
-        double o[4] = {0.0, 0.0, 0.0, 0.0};
-        if (!(k < 1e-12)) {
-            o[0] = kx / k;
-            o[1] = ky / k;
-            o[2] = kz / k;
-            const double etak2 = k2 - (Gx * Gx * etax + Gy * Gy * etay + Gz * Gz * etaz);
-            const double t = sin(k) / k - cos(k);
-            o[3] = 9.0 * t * t * exp(-etak2 / (4.0 * xi * xi)) / (k2 * k2) * invV;
-        }
+// k-space Green's tensor at wave vector (Gx - kbx, Gy - kby, Gz): (k^x, k^y, k^z, sc)
+// with sc = 9 (sin k / k - cos k)^2 exp(-(k^2 - sum_d G_d^2 eta_d) / (4 xi^2)) / k^4 / V
+// and zero at k = 0.  monodisperse_ewald_electric_field.cu:1300-1404.
+struct KParams {
+    int nx, ny, nz;
+    double Lx, Ly, Lz, kbx, kby, xi, etax, etay, etaz, invV;
+};
+
+__device__ inline void kcoef(const KParams& q, int ix, int iy, int iz, double o[4]) {
+    const double fx = (ix <= (q.nx - 1) / 2) ? ix : (ix - q.nx);
+    const double fy = (iy <= (q.ny - 1) / 2) ? iy : (iy - q.ny);
+    const double fz = (iz <= (q.nz - 1) / 2) ? iz : (iz - q.nz);
+    const double twopi = 2.0 * 3.14159265358979323846;
+    const double Gx = fx * twopi / q.Lx, Gy = fy * twopi / q.Ly, Gz = fz * twopi / q.Lz;
+    const double kx = Gx - q.kbx, ky = Gy - q.kby, kz = Gz;
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    const double k = sqrt(k2);
+    o[0] = o[1] = o[2] = o[3] = 0.0;
+    if (k < 1e-12) return;
+    o[0] = kx / k;
+    o[1] = ky / k;
+    o[2] = kz / k;
+    const double etak2 = k2 - (Gx * Gx * q.etax + Gy * Gy * q.etay + Gz * Gz * q.etaz);
+    const double t = sin(k) / k - cos(k);
+    o[3] = 9.0 * t * t * exp(-etak2 / (4.0 * q.xi * q.xi)) / (k2 * k2) * q.invV;
+}
+
+__global__ void kspace_kernel(double* __restrict__ k64, float* __restrict__ k32, KParams q) {
+    const size_t V = (size_t)q.nx * q.ny * q.nz;
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < V; v += (size_t)gridDim.x * blockDim.x) {
+        const int ix = (int)(v / ((size_t)q.ny * q.nz));
+        const int iy = (int)((v / q.nz) % q.ny);
+        const int iz = (int)(v % q.nz);
+        double o[4];
+        kcoef(q, ix, iy, iz, o);
         for (int c = 0; c < 4; ++c) {
             k64[4 * v + c] = o[c];
             k32[4 * v + c] = (float)o[c];
@@ -349,82 +358,138 @@ __global__ void kspace_kernel(double* __restrict__ k64, float* __restrict__ k32,
     }
 }
 
-// Spread: owner-computes gather into a voxel tile.
-constexpr int kTX = 4, kTY = 8, kTZ = 8;  // 256 voxels per workgroup, z fastest
-constexpr int kChunk = 64;                 // particles staged in LDS per round
-constexpr int kMaxWidth = 13;              // stencil points per axis held in LDS
-constexpr int kMaxBins = 512;              // candidate bins per tile (<= 8 per axis)
+// ---- spread: owner-computes gather into a voxel tile ----
+//
+// Full grid: tiles of 4 x 8 x 8 voxels (z fastest), layout [rhs][comp][x][y][z].
+// Slab: tiles of 8 x 16 x 2 voxels (y fastest), layout [rhs][comp][plane][x][y]
+// over the Zs occupied z-planes only.  Particles are binned by base voxel; a
+// tile stages (through LDS) every particle of every bin whose stencil can reach
+// it, found per axis in voxel space, and each thread sums the contributions to
+// its voxel.  Grid values are written exactly once, without atomics.
+template <bool SLAB>
+struct Tile;
+template <>
+struct Tile<false> {
+    static constexpr int X = 4, Y = 8, Z = 8;
+};
+template <>
+struct Tile<true> {
+    static constexpr int X = 8, Y = 16, Z = 2;
+};
 
-template <typename R, int G>
-__global__ __launch_bounds__(256) void spread_kernel(
-    SlotTable tab, int B, const int* __restrict__ base, const R* __restrict__ wts,
-    const int* __restrict__ bin_start, const int* __restrict__ bin_items, int N, int nx, int ny,
-    int nz, int nbx, int nby, int nbz, int half, int width, typename C2<R>::T* __restrict__ grid,
-    size_t V) {
+struct SpreadGeom {
+    int nx, ny, nz;
+    int half, width;
+    int bx, by, bz;     // bin size (voxels)
+    int nbx, nby, nbz;  // bins per axis
+    int ntx, nty;       // tiles per axis (x, y)
+    int Zs;             // slab planes
+};
+
+constexpr int kChunk = 64;       // particles staged in LDS per round
+constexpr int kMaxWidth = 13;    // stencil points per axis
+constexpr int kListMax = 12;     // candidate bins per axis
+constexpr int kMaxCombos = kListMax * kListMax * kListMax;
+
+template <typename R, int G, bool SLAB>
+__global__ __launch_bounds__(256) void spread_kernel(SlotTable tab, int B, SpreadGeom g, const int* __restrict__ base,
+                                                     const R* __restrict__ wts, const int* __restrict__ bin_start,
+                                                     const int* __restrict__ bin_items, const int* __restrict__ ztiles,
+                                                     const int* __restrict__ zmap, int N,
+                                                     typename C2<R>::T* __restrict__ out) {
     using CT = typename C2<R>::T;
-    __shared__ int s_bin_lo[kMaxBins];
-    __shared__ int s_bin_off[kMaxBins + 1];
-    __shared__ int s_nbins;
+    constexpr int TX = Tile<SLAB>::X, TY = Tile<SLAB>::Y, TZ = Tile<SLAB>::Z;
+    __shared__ int s_list[3][kListMax];
+    __shared__ int s_n[3];
+    __shared__ int s_pre[kMaxCombos + 1];
+    __shared__ int s_part[256];
     __shared__ int s_base[kChunk][3];
     __shared__ R s_w[kChunk][3 * kMaxWidth];
     __shared__ CT s_p[kChunk][G][3];
 
     const int tid = threadIdx.x;
-    const int tiles_y = nby, tiles_z = nbz;  // the bin grid is the tile grid
     const int t = blockIdx.x;
-    const int btx = t / (tiles_y * tiles_z);
-    const int bty = (t / tiles_z) % tiles_y;
-    const int btz = t % tiles_z;
+    const int btz = ztiles[t / (g.ntx * g.nty)];
+    const int rem = t % (g.ntx * g.nty);
+    const int btx = rem / g.nty, bty = rem % g.nty;
     const int g0 = blockIdx.y * G;
 
-    const int lz = tid % kTZ, ly = (tid / kTZ) % kTY, lx = tid / (kTZ * kTY);
-    const int vx = btx * kTX + lx, vy = bty * kTY + ly, vz = btz * kTZ + lz;
-    const bool valid = vx < nx && vy < ny && vz < nz;
+    int lx, ly, lz;
+    if (SLAB) {
+        ly = tid % TY;
+        lx = (tid / TY) % TX;
+        lz = tid / (TX * TY);
+    } else {
+        lz = tid % TZ;
+        ly = (tid / TZ) % TY;
+        lx = tid / (TZ * TY);
+    }
+    const int vx = btx * TX + lx, vy = bty * TY + ly, vz = btz * TZ + lz;
+    bool valid = vx < g.nx && vy < g.ny && vz < g.nz;
+    int js = 0;
+    if (SLAB && valid) {
+        js = zmap[vz];
+        valid = js >= 0;
+    }
 
-    // Candidate bins: every bin holding a base voxel g whose stencil g-half..g+half can
-    // reach this tile, found per axis in voxel space (the last bin of an axis may be
-    // partial, so bin-space offsets would miss wrapped neighbours).
-    if (tid == 0) {
-        int lists[3][8];
-        int cnts[3];
-        const int bt[3] = {btx, bty, btz};
-        const int tl[3] = {kTX, kTY, kTZ};
-        const int nn[3] = {nx, ny, nz};
-        const int nbb[3] = {nbx, nby, nbz};
-        for (int d = 0; d < 3; ++d) {
-            const int lo = bt[d] * tl[d] - half, hi = bt[d] * tl[d] + tl[d] - 1 + half;
-            int cnt = 0;
-            if (hi - lo + 1 >= nn[d]) {
-                for (int b = 0; b < nbb[d] && cnt < 8; ++b) lists[d][cnt++] = b;
-            } else {
-                for (int g = lo; g <= hi; ++g) {
-                    const int b = (((g % nn[d]) + nn[d]) % nn[d]) / tl[d];
-                    bool dup = false;
-                    for (int k = 0; k < cnt; ++k) dup = dup || (lists[d][k] == b);
-                    if (!dup && cnt < 8) lists[d][cnt++] = b;
-                }
+    // candidate bins per axis
+    if (tid < 3) {
+        const int d = tid;
+        const int T = d == 0 ? TX : (d == 1 ? TY : TZ);
+        const int n = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
+        const int bs = d == 0 ? g.bx : (d == 1 ? g.by : g.bz);
+        const int nb = d == 0 ? g.nbx : (d == 1 ? g.nby : g.nbz);
+        const int bt = d == 0 ? btx : (d == 1 ? bty : btz);
+        const int lo = bt * T - g.half, hi = bt * T + T - 1 + g.half;
+        int cnt = 0;
+        if (hi - lo + 1 >= n) {
+            for (int b = 0; b < nb && cnt < kListMax; ++b) s_list[d][cnt++] = b;
+        } else {
+            int last = -1;
+            for (int v = lo; v <= hi; ++v) {
+                const int b = (((v % n) + n) % n) / bs;
+                if (b == last) continue;
+                bool dup = false;
+                for (int k = 0; k < cnt; ++k) dup = dup || (s_list[d][k] == b);
+                if (!dup && cnt < kListMax) s_list[d][cnt++] = b;
+                last = b;
             }
-            cnts[d] = cnt;
         }
-        int nb = 0, off = 0;
-        for (int a = 0; a < cnts[0]; ++a)
-            for (int b = 0; b < cnts[1]; ++b)
-                for (int c = 0; c < cnts[2]; ++c) {
-                    const int bin = (lists[0][a] * nby + lists[1][b]) * nbz + lists[2][c];
-                    const int blo = bin_start[bin], bhi = bin_start[bin + 1];
-                    if (bhi > blo) {
-                        s_bin_lo[nb] = blo;
-                        s_bin_off[nb] = off;
-                        off += bhi - blo;
-                        ++nb;
-                    }
-                }
-        s_bin_off[nb] = off;
-        s_nbins = nb;
+        s_n[d] = cnt;
     }
     __syncthreads();
-    const int nbins = s_nbins;
-    const int total = s_bin_off[nbins];
+    const int ncx = s_n[0], ncy = s_n[1], ncz = s_n[2];
+    const int nc = ncx * ncy * ncz;
+    // deterministic exclusive scan of the bin sizes over the combos
+    {
+        const int per = (nc + 255) / 256;
+        const int c0 = tid * per;
+        int sum = 0;
+        for (int k = c0; k < min(nc, c0 + per); ++k) {
+            const int a = k / (ncy * ncz), b = (k / ncz) % ncy, c = k % ncz;
+            const int bin = (s_list[0][a] * g.nby + s_list[1][b]) * g.nbz + s_list[2][c];
+            const int cnt = bin_start[bin + 1] - bin_start[bin];
+            s_pre[k] = cnt;  // temporarily the count
+            sum += cnt;
+        }
+        s_part[tid] = sum;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const int v = tid >= off ? s_part[tid - off] : 0;
+            __syncthreads();
+            s_part[tid] += v;
+            __syncthreads();
+        }
+        int run = s_part[tid] - sum;  // exclusive prefix of this thread's first combo
+        for (int k = c0; k < min(nc, c0 + per); ++k) {
+            const int cnt = s_pre[k];
+            s_pre[k] = run;
+            run += cnt;
+        }
+        if (tid == 255) s_pre[nc] = s_part[255];
+    }
+    __syncthreads();
+    const int total = s_pre[nc];
 
     R acc[G][3][2];
 #pragma unroll
@@ -432,41 +497,48 @@ __global__ __launch_bounds__(256) void spread_kernel(
 #pragma unroll
         for (int c = 0; c < 3; ++c) acc[j][c][0] = acc[j][c][1] = R(0);
 
+    const int width = g.width, half = g.half;
     for (int c0 = 0; c0 < total; c0 += kChunk) {
         const int cnt = min(kChunk, total - c0);
         __syncthreads();
-        // stage `cnt` candidates: base, 3*width weights and G x 3 dipoles each
         for (int q = tid; q < cnt; q += blockDim.x) {
             const int flat = c0 + q;
-            int b = 0;
-            while (b + 1 < nbins && s_bin_off[b + 1] <= flat) ++b;
-            const int pid = bin_items[s_bin_lo[b] + (flat - s_bin_off[b])];
+            int lo = 0, hi = nc;  // largest k with s_pre[k] <= flat
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_pre[mid] <= flat) lo = mid;
+                else hi = mid;
+            }
+            const int k = lo;
+            const int a = k / (ncy * ncz), b = (k / ncz) % ncy, c = k % ncz;
+            const int bin = (s_list[0][a] * g.nby + s_list[1][b]) * g.nbz + s_list[2][c];
+            const int pid = bin_items[bin_start[bin] + (flat - s_pre[k])];
             s_base[q][0] = base[pid];
             s_base[q][1] = base[N + pid];
             s_base[q][2] = base[2 * N + pid];
-            for (int k = 0; k < 3 * width; ++k) s_w[q][k] = wts[(size_t)pid * 3 * width + k];
+            for (int kk = 0; kk < 3 * width; ++kk) s_w[q][kk] = wts[(size_t)pid * 3 * width + kk];
             for (int j = 0; j < G; ++j) {
                 const int b2 = g0 + j;
-                for (int c = 0; c < 3; ++c) {
+                for (int cc = 0; cc < 3; ++cc) {
                     CT v;
                     if (b2 < B) {
-                        const double2 d = tab.x[b2][3 * (size_t)pid + c];
-                        v.x = (R)d.x;
-                        v.y = (R)d.y;
+                        const double2 dv = tab.x[b2][3 * (size_t)pid + cc];
+                        v.x = (R)dv.x;
+                        v.y = (R)dv.y;
                     } else {
                         v.x = v.y = R(0);
                     }
-                    s_p[q][j][c] = v;
+                    s_p[q][j][cc] = v;
                 }
             }
         }
         __syncthreads();
         if (valid) {
+            const int vv[3] = {vx, vy, vz};
+            const int nn[3] = {g.nx, g.ny, g.nz};
             for (int q = 0; q < cnt; ++q) {
-                // folded per-axis weight: sum over stencil offsets o == (v - g) mod n
+                // folded per-axis weight: sum over stencil offsets o == (v - base) mod n
                 R w3[3];
-                const int vv[3] = {vx, vy, vz};
-                const int nn[3] = {nx, ny, nz};
                 bool hit = true;
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
@@ -495,7 +567,7 @@ __global__ __launch_bounds__(256) void spread_kernel(
         }
     }
     if (!valid) return;
-    const size_t v = ((size_t)vx * ny + vy) * nz + vz;
+    const size_t nxy = (size_t)g.nx * g.ny;
 #pragma unroll
     for (int j = 0; j < G; ++j) {
         const int b = g0 + j;
@@ -505,34 +577,136 @@ __global__ __launch_bounds__(256) void spread_kernel(
             CT o;
             o.x = acc[j][c][0];
             o.y = acc[j][c][1];
-            grid[((size_t)b * 3 + c) * V + v] = o;
+            size_t idx;
+            if (SLAB)
+                idx = (((size_t)b * 3 + c) * g.Zs + js) * nxy + (size_t)vx * g.ny + vy;
+            else
+                idx = ((size_t)b * 3 + c) * nxy * g.nz + ((size_t)vx * g.ny + vy) * g.nz + vz;
+            out[idx] = o;
         }
     }
 }
 
-// k-space projection f <- sc (f.k^) k^ for every RHS (:1406-1432).
+// Full grid: k-space projection f <- sc (f.k^) k^ for every RHS (:1406-1432).
 template <typename R>
-__global__ __launch_bounds__(256) void scale_kernel(typename C2<R>::T* __restrict__ grid,
-                                                    const R* __restrict__ kc, size_t V, int B) {
+__global__ __launch_bounds__(256) void scale_kernel(typename C2<R>::T* __restrict__ grid, const R* __restrict__ kc,
+                                                    size_t V, int B) {
     using CT = typename C2<R>::T;
-    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < V;
-         v += (size_t)gridDim.x * blockDim.x) {
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < V; v += (size_t)gridDim.x * blockDim.x) {
         const R kx = kc[4 * v], ky = kc[4 * v + 1], kz = kc[4 * v + 2], sc = kc[4 * v + 3];
         for (int b = 0; b < B; ++b) {
-            CT* g = grid + (size_t)b * 3 * V + v;
-            const CT f0 = g[0], f1 = g[V], f2 = g[2 * V];
+            CT* gp = grid + (size_t)b * 3 * V + v;
+            const CT f0 = gp[0], f1 = gp[V], f2 = gp[2 * V];
             const R dr = (f0.x * kx + f1.x * ky + f2.x * kz) * sc;
             const R di = (f0.y * kx + f1.y * ky + f2.y * kz) * sc;
             CT o;
             o.x = dr * kx;
             o.y = di * kx;
-            g[0] = o;
+            gp[0] = o;
             o.x = dr * ky;
             o.y = di * ky;
-            g[V] = o;
+            gp[V] = o;
             o.x = dr * kz;
             o.y = di * kz;
-            g[2 * V] = o;
+            gp[2 * V] = o;
+        }
+    }
+}
+
+// Slab: for every in-plane wave vector (kx, ky) — one column of the slab after
+// the 2-D transforms — evaluate the z-transform from the Zs occupied planes to
+// all nz wave numbers, apply the Green's tensor projection, and transform back to
+// the Zs planes the contraction reads.  The 3-D k-space grid never exists in
+// memory.  A block covers (256 / RB) columns x RB right-hand sides; the twiddles
+// exp(-2 pi i kz z_j / nz) sit in LDS, the Green's tensor (k^, sc) is read from a
+// [kz][column] table shared by the RB right-hand sides of a column.
+template <typename R, int ZS, int RB>
+__global__ __launch_bounds__(256) void zfused_kernel(typename C2<R>::T* __restrict__ slab, int B, int ncols, int nz,
+                                                     int Zs, const int* __restrict__ zlist,
+                                                     const R* __restrict__ kc) {
+    using CT = typename C2<R>::T;
+    extern __shared__ unsigned char s_raw[];
+    CT* s_tw = reinterpret_cast<CT*>(s_raw);  // [ZS][nz]
+    constexpr int CB = 256 / RB;
+    const int tid = threadIdx.x;
+    const int tc = tid % CB, tb = tid / CB;
+    for (int e = tid; e < ZS * nz; e += 256) {
+        const int j = e / nz, kz = e % nz;
+        CT w;
+        if (j < Zs) {
+            const long long m = ((long long)kz * zlist[j]) % nz;
+            double sv, cv;
+            sincospi(2.0 * (double)m / (double)nz, &sv, &cv);
+            w.x = (R)cv;
+            w.y = (R)(-sv);
+        } else {
+            w.x = w.y = R(0);
+        }
+        s_tw[e] = w;
+    }
+    __syncthreads();
+    const int col = blockIdx.x * CB + tc;
+    const int b = blockIdx.y * RB + tb;
+    if (col >= ncols || b >= B) return;
+    CT f[3][ZS], gacc[3][ZS];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int j = 0; j < ZS; ++j) {
+            if (j < Zs) {
+                f[c][j] = slab[(((size_t)b * 3 + c) * Zs + j) * ncols + col];
+            } else {
+                f[c][j].x = f[c][j].y = R(0);
+            }
+            gacc[c][j].x = gacc[c][j].y = R(0);
+        }
+    for (int kz = 0; kz < nz; ++kz) {
+        const R* k4 = kc + ((size_t)kz * ncols + col) * 4;
+        const R kx = k4[0], ky = k4[1], kzh = k4[2], sc = k4[3];
+        R Fr[3] = {R(0), R(0), R(0)}, Fi[3] = {R(0), R(0), R(0)};
+#pragma unroll
+        for (int j = 0; j < ZS; ++j) {
+            const CT w = s_tw[j * nz + kz];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                Fr[c] += f[c][j].x * w.x - f[c][j].y * w.y;
+                Fi[c] += f[c][j].x * w.y + f[c][j].y * w.x;
+            }
+        }
+        const R pr = sc * (kx * Fr[0] + ky * Fr[1] + kzh * Fr[2]);
+        const R pi = sc * (kx * Fi[0] + ky * Fi[1] + kzh * Fi[2]);
+        const R kk[3] = {kx, ky, kzh};
+#pragma unroll
+        for (int j = 0; j < ZS; ++j) {
+            const CT w = s_tw[j * nz + kz];  // inverse uses conj(w)
+            const R ur = pr * w.x + pi * w.y;
+            const R ui = pi * w.x - pr * w.y;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                gacc[c][j].x += kk[c] * ur;
+                gacc[c][j].y += kk[c] * ui;
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int j = 0; j < ZS; ++j)
+            if (j < Zs) slab[(((size_t)b * 3 + c) * Zs + j) * ncols + col] = gacc[c][j];
+}
+
+// Green's tensor table for the slab path, [kz][column][4].
+__global__ void kspace_slab_kernel(double* __restrict__ k64, float* __restrict__ k32, KParams q) {
+    const size_t ncols = (size_t)q.nx * q.ny;
+    const size_t V = ncols * q.nz;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < V; e += (size_t)gridDim.x * blockDim.x) {
+        const int kz = (int)(e / ncols);
+        const size_t col = e % ncols;
+        double o[4];
+        kcoef(q, (int)(col / q.ny), (int)(col % q.ny), kz, o);
+        for (int c = 0; c < 4; ++c) {
+            k64[4 * e + c] = o[c];
+            k32[4 * e + c] = (float)o[c];
         }
     }
 }
@@ -543,18 +717,24 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-// One wavefront per particle: contraction + real-space gather + self term.
-template <typename R, int G>
+// One wavefront per particle: Gaussian contraction of the (inverse-transformed)
+// grid, real-space gather over the symmetric neighbour list, self term.
+template <typename R, int G, bool SLAB>
 __global__ __launch_bounds__(256) void gather_kernel(
-    SlotTable tab, int B, int parts, const typename C2<R>::T* __restrict__ grid, size_t V,
-    const int* __restrict__ base, const R* __restrict__ wts, int N, int nx, int ny, int nz,
-    int half, int width, double h3, const double* __restrict__ pos, const int* __restrict__ nb_start,
-    const int* __restrict__ nb_j, const double2* __restrict__ nb_f, double Lx, double Ly,
-    double Lz, double kbx, double kby, const double2* __restrict__ selfc, double self_perp) {
+    SlotTable tab, int B, int parts, const typename C2<R>::T* __restrict__ grid, SpreadGeom g,
+    const int* __restrict__ zmap, const int* __restrict__ base, const R* __restrict__ wts, int N, double h3,
+    const double* __restrict__ pos, const int* __restrict__ nb_start, const int* __restrict__ nb_j,
+    const double2* __restrict__ nb_f, double Lx, double Ly, double Lz, double kbx, double kby,
+    const double2* __restrict__ selfc, double self_perp) {
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (i >= N) return;
+    const int width = g.width, half = g.half;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int S = width * width * width;
+    const size_t nxy = (size_t)nx * ny;
+    const size_t rhs_stride = SLAB ? 3 * (size_t)g.Zs * nxy : 3 * nxy * nz;
+    const size_t comp_stride = SLAB ? (size_t)g.Zs * nxy : nxy * nz;
     int sx = 0, sy = 0, sz = 0;
     const R* w = wts + (size_t)i * 3 * width;
     if (parts & MPM_PART_RECIP) {
@@ -579,19 +759,32 @@ __global__ __launch_bounds__(256) void gather_kernel(
 #pragma unroll
                 for (int c = 0; c < 3; ++c) racc[j][c][0] = racc[j][c][1] = R(0);
             for (int t = lane; t < S; t += 64) {
-                const int ox = t / (width * width), oy = (t / width) % width, oz = t % width;
+                int ox, oy, oz;
+                if (SLAB) {  // y fastest, matching the slab layout
+                    oy = t % width;
+                    ox = (t / width) % width;
+                    oz = t / (width * width);
+                } else {
+                    oz = t % width;
+                    oy = (t / width) % width;
+                    ox = t / (width * width);
+                }
                 int vx = sx + ox, vy = sy + oy, vz = sz + oz;
                 while (vx >= nx) vx -= nx;
                 while (vy >= ny) vy -= ny;
                 while (vz >= nz) vz -= nz;
                 const R ww = w[ox] * w[width + oy] * w[2 * width + oz];
-                const size_t v = ((size_t)vx * ny + vy) * nz + vz;
+                size_t v;
+                if (SLAB)
+                    v = (size_t)zmap[vz] * nxy + (size_t)vx * ny + vy;
+                else
+                    v = ((size_t)vx * ny + vy) * nz + vz;
 #pragma unroll
                 for (int j = 0; j < G; ++j) {
                     if (g0 + j >= B) break;
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
-                        const auto f = grid[((size_t)(g0 + j) * 3 + c) * V + v];
+                        const auto f = grid[(size_t)(g0 + j) * rhs_stride + c * comp_stride + v];
                         racc[j][c][0] += ww * f.x;
                         racc[j][c][1] += ww * f.y;
                     }
@@ -680,32 +873,56 @@ EwaldOp::EwaldOp(const mpm_op_desc& d) {
     errortol = d.errortol;
     xi = d.xi;
     compute_sizing(L, errortol, xi, d.grid, d.support, S);
+    MPM_REQUIRE(d.layout == 0 || d.layout == 1, MPM_ERR_INVALID_ARGUMENT, "layout must be 0 or 1");
+    force_full = d.layout;
     MPM_REQUIRE(S.width <= kMaxWidth, MPM_ERR_UNSUPPORTED,
                 "stencil wider than " + std::to_string(kMaxWidth) + " points per axis");
     self_perp = dipole_self_term(xi);
     V = (size_t)S.n[0] * S.n[1] * S.n[2];
-    for (int k = 0; k < 3; ++k) {
-        const int tl[3] = {kTX, kTY, kTZ};
-        tile[k] = tl[k];
-        nbins[k] = (S.n[k] + tile[k] - 1) / tile[k];
-    }
-    fft = std::make_unique<FftPlans>(S.n);
     table = std::make_unique<PairTable>(xi);
     build_kspace();
 }
 
 EwaldOp::~EwaldOp() = default;
 
+KParams EwaldOp::kparams() const {
+    KParams q;
+    q.nx = S.n[0];
+    q.ny = S.n[1];
+    q.nz = S.n[2];
+    q.Lx = L[0];
+    q.Ly = L[1];
+    q.Lz = L[2];
+    q.kbx = kb[0];
+    q.kby = kb[1];
+    q.xi = xi;
+    q.etax = S.eta[0];
+    q.etay = S.eta[1];
+    q.etaz = S.eta[2];
+    q.invV = 1.0 / (double)V;
+    return q;
+}
+
 void EwaldOp::build_kspace() {
+    // the full-grid coefficient table is built lazily (only the full-grid path reads it)
+    kspace_ready = false;
+}
+
+void EwaldOp::ensure_kspace_table() {
+    const int layout = slab ? 1 : 0;
+    if (kspace_ready && kspace_layout == layout) return;
     d_k64.resize(4 * V);
     d_k32.resize(4 * V);
     const int threads = 256;
     const int blocks = (int)std::min<size_t>((V + threads - 1) / threads, 65535);
-    hipLaunchKernelGGL(kspace_kernel, dim3(blocks), dim3(threads), 0, nullptr, d_k64.p, d_k32.p,
-                       S.n[0], S.n[1], S.n[2], L[0], L[1], L[2], kb[0], kb[1], xi, S.eta[0],
-                       S.eta[1], S.eta[2], 1.0 / (double)V);
+    if (slab)
+        hipLaunchKernelGGL(kspace_slab_kernel, dim3(blocks), dim3(threads), 0, nullptr, d_k64.p, d_k32.p, kparams());
+    else
+        hipLaunchKernelGGL(kspace_kernel, dim3(blocks), dim3(threads), 0, nullptr, d_k64.p, d_k32.p, kparams());
     MPM_HIP(hipGetLastError());
     MPM_HIP(hipDeviceSynchronize());
+    kspace_ready = true;
+    kspace_layout = layout;
 }
 
 void EwaldOp::set_bloch(double kx, double ky) {
@@ -713,6 +930,15 @@ void EwaldOp::set_bloch(double kx, double ky) {
     kb[0] = kx;
     kb[1] = ky;
     build_kspace();
+}
+
+// LDS bytes of zfused_kernel for slab size zs_pad and precision size rs.
+static size_t zfused_lds(int zs_pad, int nz, size_t rs) { return 2 * rs * zs_pad * nz; }
+static int zs_bucket(int Zs) {
+    static const int buckets[] = {4, 6, 8, 10, 12, 14, 16};
+    for (int b : buckets)
+        if (Zs <= b) return b;
+    return 0;
 }
 
 void EwaldOp::set_positions(const double* xyz, int64_t n) {
@@ -740,9 +966,9 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
             pos[(size_t)d * Nn + i] = p;
             const double gd = std::round(p / S.h[d]);
             MPM_REQUIRE(std::fabs(gd) < 1e9, MPM_ERR_INVALID_ARGUMENT, "position too far from the box");
-            const long long g = (long long)gd;
-            const double dd = g * S.h[d] - p;
-            base[(size_t)d * Nn + i] = (int)(((g % S.n[d]) + S.n[d]) % S.n[d]);
+            const long long gi = (long long)gd;
+            const double dd = gi * S.h[d] - p;
+            base[(size_t)d * Nn + i] = (int)(((gi % S.n[d]) + S.n[d]) % S.n[d]);
             for (int o = -half; o <= half; ++o) {
                 const double x = dd + o * S.h[d];
                 double wv = std::exp(-2.0 * xi * xi * x * x / S.eta[d]);
@@ -752,11 +978,44 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
             }
         }
     }
-    // bins = tiles of the spread kernel, keyed by each particle's base voxel
+    // ---- reciprocal layout: slab of occupied z-planes, or the full grid ----
+    const int nz = S.n[2];
+    std::vector<char> occ(nz, 0);
+    for (int i = 0; i < Nn; ++i)
+        for (int o = -half; o <= half; ++o) occ[(((base[2 * (size_t)Nn + i] + o) % nz) + nz) % nz] = 1;
+    std::vector<int> zl;
+    for (int z = 0; z < nz; ++z)
+        if (occ[z]) zl.push_back(z);
+    const int zsb = zs_bucket((int)zl.size());
+    const bool want_slab = force_full == 0 && zsb > 0 && (int)zl.size() * 2 <= nz &&
+                           zfused_lds(zsb, nz, sizeof(double)) <= 64 * 1024;
+    slab = want_slab;
+    Zs = slab ? (int)zl.size() : 0;
+    zlist = zl;
+    std::vector<int> zmap(nz, -1);
+    for (size_t j = 0; j < zl.size(); ++j) zmap[zl[j]] = (int)j;
+    // tiles and bins
+    if (slab) {
+        tile[0] = 8; tile[1] = 16; tile[2] = 2;
+        binsz[0] = 4; binsz[1] = 4; binsz[2] = 2;
+    } else {
+        tile[0] = 4; tile[1] = 8; tile[2] = 8;
+        binsz[0] = 4; binsz[1] = 8; binsz[2] = 8;
+    }
+    for (int d = 0; d < 3; ++d) {
+        nbins[d] = (S.n[d] + binsz[d] - 1) / binsz[d];
+        ntiles[d] = (S.n[d] + tile[d] - 1) / tile[d];
+    }
+    std::vector<int> ztl;  // z-tiles that hold at least one computed voxel
+    for (int t = 0; t < ntiles[2]; ++t) {
+        bool any = !slab;
+        for (int z = t * tile[2]; z < std::min(nz, (t + 1) * tile[2]) && !any; ++z) any = occ[z] != 0;
+        if (any) ztl.push_back(t);
+    }
     const int nb = nbins[0] * nbins[1] * nbins[2];
     std::vector<int> bin_of(Nn), start(nb + 1, 0), items(Nn);
     for (int i = 0; i < Nn; ++i) {
-        const int bx = base[i] / tile[0], by = base[Nn + i] / tile[1], bz = base[2 * Nn + i] / tile[2];
+        const int bx = base[i] / binsz[0], by = base[Nn + i] / binsz[1], bz = base[2 * Nn + i] / binsz[2];
         bin_of[i] = (bx * nbins[1] + by) * nbins[2] + bz;
         start[bin_of[i] + 1]++;
     }
@@ -781,8 +1040,7 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     if (cells) {
         members.resize((size_t)nc[0] * nc[1] * nc[2]);
         for (int j = 0; j < Nn; ++j) {
-            const int cx = cell_coord(pos[j], 0), cy = cell_coord(pos[Nn + j], 1),
-                      cz = cell_coord(pos[2 * Nn + j], 2);
+            const int cx = cell_coord(pos[j], 0), cy = cell_coord(pos[Nn + j], 1), cz = cell_coord(pos[2 * Nn + j], 2);
             members[(size_t)(cz * nc[1] + cy) * nc[0] + cx].push_back(j);
         }
     }
@@ -803,8 +1061,7 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     };
     for (int i = 0; i < Nn; ++i) {
         if (cells) {
-            const int cx = cell_coord(pos[i], 0), cy = cell_coord(pos[Nn + i], 1),
-                      cz = cell_coord(pos[2 * Nn + i], 2);
+            const int cx = cell_coord(pos[i], 0), cy = cell_coord(pos[Nn + i], 1), cz = cell_coord(pos[2 * Nn + i], 2);
             for (int dz = -1; dz <= 1; ++dz)
                 for (int dy = -1; dy <= 1; ++dy)
                     for (int dx = -1; dx <= 1; ++dx) {
@@ -826,6 +1083,11 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     d_w32.upload(w32.data(), w32.size());
     d_bin_start.upload(start.data(), start.size());
     d_bin_items.upload(items.data(), items.size());
+    d_ztiles.upload(ztl.data(), ztl.size());
+    n_ztiles = (int)ztl.size();
+    d_zmap.upload(zmap.data(), zmap.size());
+    if (zl.empty()) zl.push_back(0);
+    d_zlist.upload(zl.data(), zl.size());
     d_nb_start.upload(nstart.data(), nstart.size());
     if (nj.empty()) {
         nj.push_back(0);
@@ -833,6 +1095,15 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     }
     d_nb_j.upload(nj.data(), nj.size());
     d_nb_f.upload(nf.data(), nf.size());
+    // FFT plans for the chosen layout
+    const int key = slab ? Zs : -1;
+    if (!fft || fft_key != key) {
+        if (slab)
+            fft = std::make_unique<FftPlans>(std::vector<size_t>{(size_t)S.n[1], (size_t)S.n[0]}, 3 * Zs);
+        else
+            fft = std::make_unique<FftPlans>(std::vector<size_t>{(size_t)S.n[2], (size_t)S.n[1], (size_t)S.n[0]}, 3);
+        fft_key = key;
+    }
     // default self coefficients: zero (reference base_electric_field.cu allocates them zeroed)
     if (self_rows == 0 || d_selfc.n != (size_t)self_rows * Nn) {
         self_rows = 1;
@@ -848,64 +1119,107 @@ void EwaldOp::set_self_coef(const double* coef, int rows) {
     self_rows = rows;
 }
 
-void EwaldOp::ensure_grid(int B, bool fp32) {
-    if (fp32)
-        d_grid32.reserve((size_t)B * 3 * V);
-    else
-        d_grid64.reserve((size_t)B * 3 * V);
+size_t EwaldOp::grid_elems_per_rhs() const {
+    return slab ? 3 * (size_t)Zs * S.n[0] * S.n[1] : 3 * V;
 }
 
-template <typename R>
-static void launch_apply(const SlotTable& t, int B, int parts, typename C2<R>::T* grid, size_t V,
-                         const Sizing& S, const int nbins[3], int N, const int* base, const R* w,
-                         const int* bstart, const int* bitems, const double* pos,
-                         const int* nbs, const int* nbj, const double2* nbf, const double L[3],
-                         const double kb[2], const double2* selfc, double self_perp,
-                         const R* kc, FftPlans& fft, hipStream_t s) {
+template <typename R, bool SLAB>
+void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s) {
+    using CT = typename C2<R>::T;
     const bool fp32 = std::is_same<R, float>::value;
+    CT* grid = fp32 ? reinterpret_cast<CT*>(d_grid32.p) : reinterpret_cast<CT*>(d_grid64.p);
+    const R* w = fp32 ? reinterpret_cast<const R*>(d_w32.p) : reinterpret_cast<const R*>(d_w64.p);
+    SpreadGeom g;
+    g.nx = S.n[0];
+    g.ny = S.n[1];
+    g.nz = S.n[2];
+    g.half = S.half;
+    g.width = S.width;
+    g.bx = binsz[0];
+    g.by = binsz[1];
+    g.bz = binsz[2];
+    g.nbx = nbins[0];
+    g.nby = nbins[1];
+    g.nbz = nbins[2];
+    g.ntx = ntiles[0];
+    g.nty = ntiles[1];
+    g.Zs = Zs;
+    const int Ni = (int)N;
     if (parts & MPM_PART_RECIP) {
         {
             ProfScope ps("spread", s);
-            const int ntiles = nbins[0] * nbins[1] * nbins[2];
-            if (B == 1) {
-                hipLaunchKernelGGL((spread_kernel<R, 1>), dim3(ntiles, 1), dim3(256), 0, s, t, B, base,
-                                   w, bstart, bitems, N, S.n[0], S.n[1], S.n[2], nbins[0], nbins[1],
-                                   nbins[2], S.half, S.width, grid, V);
-            } else {
-                hipLaunchKernelGGL((spread_kernel<R, 4>), dim3(ntiles, (B + 3) / 4), dim3(256), 0, s, t,
-                                   B, base, w, bstart, bitems, N, S.n[0], S.n[1], S.n[2], nbins[0],
-                                   nbins[1], nbins[2], S.half, S.width, grid, V);
-            }
+            const int nt = ntiles[0] * ntiles[1] * n_ztiles;
+            if (B == 1)
+                hipLaunchKernelGGL((spread_kernel<R, 1, SLAB>), dim3(nt, 1), dim3(256), 0, s, t, B, g, d_base.p, w,
+                                   d_bin_start.p, d_bin_items.p, d_ztiles.p, d_zmap.p, Ni, grid);
+            else
+                hipLaunchKernelGGL((spread_kernel<R, 4, SLAB>), dim3(nt, (B + 3) / 4), dim3(256), 0, s, t, B, g,
+                                   d_base.p, w, d_bin_start.p, d_bin_items.p, d_ztiles.p, d_zmap.p, Ni, grid);
             MPM_HIP(hipGetLastError());
         }
-        const size_t bytes_per_rhs = 3 * V * sizeof(typename C2<R>::T);
+        const size_t bytes_per_rhs = grid_elems_per_rhs() * sizeof(CT);
         {
             ProfScope ps("fft", s);
-            fft.exec(grid, bytes_per_rhs, B, fp32, true, s);
+            fft->exec(grid, bytes_per_rhs, B, fp32, true, s);
         }
-        {
+        if (SLAB) {
+            ProfScope ps("zfused", s);
+            const int ncols = S.n[0] * S.n[1];
+            const int zsb = zs_bucket(Zs);
+            const size_t lds = zfused_lds(zsb, S.n[2], sizeof(R));
+            const R* kc = fp32 ? reinterpret_cast<const R*>(d_k32.p) : reinterpret_cast<const R*>(d_k64.p);
+            const int rb = B >= 16 ? 16 : (B >= 4 ? 4 : 1);
+            const dim3 gridz((ncols + 256 / rb - 1) / (256 / rb), (B + rb - 1) / rb);
+#define MPM_ZF2(ZSV, RBV)                                                                                     \
+    hipLaunchKernelGGL((zfused_kernel<R, ZSV, RBV>), gridz, dim3(256), lds, s, grid, B, ncols, S.n[2], Zs, \
+                       d_zlist.p, kc)
+#define MPM_ZF(ZSV)                     \
+    case ZSV:                           \
+        if (rb == 16)                   \
+            MPM_ZF2(ZSV, 16);           \
+        else if (rb == 4)               \
+            MPM_ZF2(ZSV, 4);            \
+        else                            \
+            MPM_ZF2(ZSV, 1);            \
+        break;
+            switch (zsb) {
+                MPM_ZF(4)
+                MPM_ZF(6)
+                MPM_ZF(8)
+                MPM_ZF(10)
+                MPM_ZF(12)
+                MPM_ZF(14)
+                MPM_ZF(16)
+                default:
+                    throw Error(MPM_ERR_RUNTIME, "slab size out of range");
+            }
+#undef MPM_ZF
+#undef MPM_ZF2
+            MPM_HIP(hipGetLastError());
+        } else {
             ProfScope ps("scale", s);
             const int blocks = (int)std::min<size_t>((V + 255) / 256, 8192);
+            const R* kc = fp32 ? reinterpret_cast<const R*>(d_k32.p) : reinterpret_cast<const R*>(d_k64.p);
             hipLaunchKernelGGL(scale_kernel<R>, dim3(blocks), dim3(256), 0, s, grid, kc, V, B);
             MPM_HIP(hipGetLastError());
         }
         {
             ProfScope ps("fft", s);
-            fft.exec(grid, bytes_per_rhs, B, fp32, false, s);
+            fft->exec(grid, bytes_per_rhs, B, fp32, false, s);
         }
     }
     {
         ProfScope ps("gather", s);
         const double h3 = S.h[0] * S.h[1] * S.h[2];
-        const int blocks = (N + 3) / 4;
+        const int blocks = (Ni + 3) / 4;
         if (B == 1)
-            hipLaunchKernelGGL((gather_kernel<R, 1>), dim3(blocks), dim3(256), 0, s, t, B, parts, grid, V,
-                               base, w, N, S.n[0], S.n[1], S.n[2], S.half, S.width, h3, pos, nbs,
-                               nbj, nbf, L[0], L[1], L[2], kb[0], kb[1], selfc, self_perp);
+            hipLaunchKernelGGL((gather_kernel<R, 1, SLAB>), dim3(blocks), dim3(256), 0, s, t, B, parts, grid, g,
+                               d_zmap.p, d_base.p, w, Ni, h3, d_pos.p, d_nb_start.p, d_nb_j.p, d_nb_f.p, L[0], L[1],
+                               L[2], kb[0], kb[1], d_selfc.p, self_perp);
         else
-            hipLaunchKernelGGL((gather_kernel<R, 4>), dim3(blocks), dim3(256), 0, s, t, B, parts, grid, V,
-                               base, w, N, S.n[0], S.n[1], S.n[2], S.half, S.width, h3, pos, nbs,
-                               nbj, nbf, L[0], L[1], L[2], kb[0], kb[1], selfc, self_perp);
+            hipLaunchKernelGGL((gather_kernel<R, 4, SLAB>), dim3(blocks), dim3(256), 0, s, t, B, parts, grid, g,
+                               d_zmap.p, d_base.p, w, Ni, h3, d_pos.p, d_nb_start.p, d_nb_j.p, d_nb_f.p, L[0], L[1],
+                               L[2], kb[0], kb[1], d_selfc.p, self_perp);
         MPM_HIP(hipGetLastError());
     }
 }
@@ -916,20 +1230,23 @@ void EwaldOp::apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t
     for (int b = 0; b < B; ++b)
         MPM_REQUIRE(t.row[b] >= 0 && t.row[b] < self_rows, MPM_ERR_INVALID_ARGUMENT,
                     "self-coefficient row out of range");
-    ensure_grid(B, fp32);
-    const int Ni = (int)N;
+    if (parts & MPM_PART_RECIP) ensure_kspace_table();
+    const size_t elems = (size_t)B * grid_elems_per_rhs();
     if (fp32)
-        launch_apply<float>(t, B, parts, d_grid32.p, V, S, nbins, Ni, d_base.p, d_w32.p, d_bin_start.p,
-                            d_bin_items.p, d_pos.p, d_nb_start.p, d_nb_j.p, d_nb_f.p, L, kb,
-                            d_selfc.p, self_perp, d_k32.p, *fft, s);
+        d_grid32.reserve(elems);
     else
-        launch_apply<double>(t, B, parts, d_grid64.p, V, S, nbins, Ni, d_base.p, d_w64.p,
-                             d_bin_start.p, d_bin_items.p, d_pos.p, d_nb_start.p, d_nb_j.p, d_nb_f.p,
-                             L, kb, d_selfc.p, self_perp, d_k64.p, *fft, s);
+        d_grid64.reserve(elems);
+    if (fp32) {
+        if (slab) launch<float, true>(t, B, parts, s);
+        else launch<float, false>(t, B, parts, s);
+    } else {
+        if (slab) launch<double, true>(t, B, parts, s);
+        else launch<double, false>(t, B, parts, s);
+    }
 }
 
-void EwaldOp::apply_contiguous(const double2* x, double2* y, int B, bool fp32, int parts,
-                               hipStream_t s, const int* rows) {
+void EwaldOp::apply_contiguous(const double2* x, double2* y, int B, bool fp32, int parts, hipStream_t s,
+                               const int* rows) {
     const size_t n = 3 * (size_t)N;
     for (int b0 = 0; b0 < B; b0 += kMaxSlots) {
         const int cnt = std::min(kMaxSlots, B - b0);
@@ -959,6 +1276,7 @@ void EwaldOp::info(mpm_op_info& o) const {
     o.self_coef_ewald = -4.0 * std::pow(xi, 3.0) / (3.0 * std::sqrt(kPi));
     o.bloch[0] = kb[0];
     o.bloch[1] = kb[1];
+    o.slab_planes = slab ? Zs : 0;
 }
 
 }  // namespace mpm

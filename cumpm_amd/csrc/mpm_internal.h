@@ -108,6 +108,7 @@ struct SlotTable {
 
 struct FftPlans;   // rocFFT plans keyed by (precision, batch)
 struct PairTable;  // sampled real-space pair functions
+struct KParams;    // k-space geometry passed to the kernels
 
 class EwaldOp {
    public:
@@ -129,7 +130,11 @@ class EwaldOp {
 
    private:
     void build_kspace();
-    void ensure_grid(int B, bool fp32);
+    void ensure_kspace_table();
+    KParams kparams() const;
+    size_t grid_elems_per_rhs() const;
+    template <typename R, bool SLAB>
+    void launch(const SlotTable& t, int B, int parts, hipStream_t s);
 
     double L[3], errortol, xi;
     Sizing S;
@@ -137,6 +142,16 @@ class EwaldOp {
     double kb[2] = {0, 0};
     size_t V = 0;
     int64_t num_pairs = 0;
+    int force_full = 0;          // 1: never use the slab layout
+    // reciprocal layout: the full grid, or the slab of occupied z-planes
+    bool slab = false;
+    int Zs = 0;
+    std::vector<int> zlist;
+    int tile[3] = {4, 8, 8};
+    int binsz[3] = {4, 8, 8};
+    int nbins[3] = {0, 0, 0};
+    int ntiles[3] = {0, 0, 0};
+    int n_ztiles = 0;
     // geometry on device
     DevBuf<double> d_pos;        // [3][N] SoA
     DevBuf<int> d_base;          // [3][N] wrapped base grid index
@@ -144,21 +159,27 @@ class EwaldOp {
     DevBuf<float> d_w32;
     DevBuf<int> d_bin_start;     // [nbins + 1]
     DevBuf<int> d_bin_items;     // [N] particle ids sorted by bin
-    int tile[3] = {4, 8, 8};
-    int nbins[3] = {0, 0, 0};
+    DevBuf<int> d_ztiles;        // z-tile indices the spread visits
+    DevBuf<int> d_zmap;          // [nz] slab plane of each z (or -1)
+    DevBuf<int> d_zlist;         // [Zs] z of each slab plane
     DevBuf<int> d_nb_start;      // [N + 1] target-centric CSR
     DevBuf<int> d_nb_j;          // [M]
     DevBuf<double2> d_nb_f;      // [M] (perp, para)
     DevBuf<double2> d_selfc;     // [rows][N]
-    // k space
-    DevBuf<double> d_k64;        // [V][4] (khat x, y, z, scale)
+    // full-grid k-space table (khat x, y, z, scale) per voxel
+    bool kspace_ready = false;
+    int kspace_layout = -1;
+    DevBuf<double> d_k64;
     DevBuf<float> d_k32;
     // grids
     DevBuf<double2> d_grid64;
     DevBuf<float2> d_grid32;
     std::unique_ptr<FftPlans> fft;
+    int fft_key = -2;
     std::unique_ptr<PairTable> table;
     std::vector<double> last_xyz;  // positions of the current precalcs
+
+    friend struct OpAccess;
 };
 
 // Batched Krylov solve (krylov.hip).

@@ -64,6 +64,8 @@ typedef struct {
     double xi;       /* Ewald splitting parameter */
     int grid[3];     /* 0 = reference sizing rule; >0 overrides that axis */
     int support;     /* num_grid_gaussian P; 0 = reference rule ceil(-2 ln tol / pi) */
+    int layout;      /* reciprocal-space layout: 0 = automatic (the z-plane slab when the
+                        particle stencils touch at most half of the z-planes), 1 = full grid */
 } mpm_op_desc;
 
 typedef struct {
@@ -78,6 +80,7 @@ typedef struct {
     double self_perp;    /* real-space self term (table entry 0) */
     double self_coef_ewald; /* -4 xi^3 / (3 sqrt(pi)) */
     double bloch[2];
+    int slab_planes;     /* occupied z-planes of the slab layout; 0 = full grid */
 } mpm_op_info;
 
 /* Monodisperse_Ewald_Electric_Field(box_x, box_y, box_z, errortol, xi, SOLVER_AX, radius=1)

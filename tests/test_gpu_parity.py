@@ -37,6 +37,7 @@ GEOMS = [
      lambda: np.random.default_rng(3).uniform(0, 15, (12, 3)), (6, 7, 5), 0),
     ("override_32_P8", [24.0] * 3, 1e-8, 0.5, lambda: np.random.default_rng(4).uniform(0, 24, (30, 3)), (32, 32, 32), 8),
     ("single_particle", [20.0] * 3, 1e-6, 0.5, lambda: np.array([[3.0, 4.0, 5.0]]), None, 0),
+    ("bilayer", None, 1e-4, 0.5, None, None, 0),
 ]
 
 
@@ -47,27 +48,39 @@ def _geom(name):
             if nm == "hex_slab":
                 pos, box = hex_monolayer(10, 8, 3.0, jitter=0.04, seed=5)
                 box = np.array([box[0], box[1], 16.0])
+            elif nm == "bilayer":  # two layers: the occupied z-planes are not contiguous
+                p1, box = hex_monolayer(8, 6, 2.5, jitter=0.05, seed=6)
+                p2 = p1 + np.array([1.25, 0.7, 12.0])
+                pos = np.concatenate([p1, p2])
+                box = np.array([box[0], box[1], 40.0])
             else:
                 pos = pf()
             return np.asarray(box, float), tol, xi, pos, grid, P
     raise KeyError(name)
 
 
-@pytest.mark.parametrize("name", [g[0] for g in GEOMS])
-def test_apply_fp64_matches_oracle(name):
+SLAB_GEOMS = ("hex_slab", "bilayer")
+
+
+@pytest.mark.parametrize("name,layout", [(g[0], 0) for g in GEOMS] + [(g, 1) for g in SLAB_GEOMS])
+def test_apply_fp64_matches_oracle(name, layout):
     box, tol, xi, pos, grid, P = _geom(name)
-    _case(f"apply fp64 {name} N={len(pos)}")
+    _case(f"apply fp64 {name} layout={layout} N={len(pos)}")
     N = len(pos)
     rng = np.random.default_rng(11)
     B = 3
     p = rng.standard_normal((B, N, 3)) + 1j * rng.standard_normal((B, N, 3))
     rows = rng.standard_normal((B, N)) + 1j * rng.standard_normal((B, N))
-    op = Op(box, tol, xi, grid, P)
+    op = Op(box, tol, xi, grid, P, layout=layout)
     op.set_positions(pos)
     op.set_self_coef(rows)
     ref_op = O.Operator(box, tol, xi, grid, P)
     ref_op.set_positions(pos)
     info = op.info()
+    if layout == 1:
+        assert info.slab_planes == 0
+    elif name in SLAB_GEOMS:
+        assert info.slab_planes > 0
     assert tuple(info.grid) == ref_op.info()["grid"]
     assert info.num_pairs == ref_op.info()["num_pairs"]
     for parts in (C.PART_REAL, C.PART_RECIP, C.PART_ALL):
@@ -80,15 +93,16 @@ def test_apply_fp64_matches_oracle(name):
             assert rel(got[b], ref) < 1e-11, (name, parts, b, rel(got[b], ref))
 
 
-@pytest.mark.parametrize("name", ["random3d", "hex_slab", "sc_tight"])
-def test_apply_fp32_reciprocal_within_reference_tolerance(name):
+@pytest.mark.parametrize("name,layout", [("random3d", 0), ("hex_slab", 0), ("hex_slab", 1), ("sc_tight", 0),
+                                         ("bilayer", 0)])
+def test_apply_fp32_reciprocal_within_reference_tolerance(name, layout):
     box, tol, xi, pos, grid, P = _geom(name)
-    _case(f"apply fp32 {name}")
+    _case(f"apply fp32 {name} layout={layout}")
     N = len(pos)
     rng = np.random.default_rng(12)
     p = rng.standard_normal((2, N, 3)) + 1j * rng.standard_normal((2, N, 3))
     rows = rng.standard_normal((2, N)) + 1j * rng.standard_normal((2, N))
-    op = Op(box, tol, xi, grid, P)
+    op = Op(box, tol, xi, grid, P, layout=layout)
     op.set_positions(pos)
     op.set_self_coef(rows)
     ref_op = O.Operator(box, tol, xi, grid, P)
