@@ -62,17 +62,23 @@ def run_once(s, pos):
 
 
 def algorithmic_bytes(cls, info, N, B, fp32, pairs):
-    """Algorithmic HBM bytes of one launch of a kernel class over B right-hand sides."""
-    V = int(np.prod(list(info.grid)))
+    """Algorithmic HBM bytes of one launch of a kernel class over B right-hand sides.
+
+    The reciprocal grid is the full nx*ny*nz grid, or in the slab layout only the
+    occupied z-planes (slab_planes * nx * ny voxels); one FFT launch is one
+    multi-axis transform, counted as one read and one write of the grid."""
+    nx, ny, nz = list(info.grid)
+    V = nx * ny * nz
+    Vp = info.slab_planes * nx * ny if info.slab_planes else V
     c = 8 if fp32 else 16
     r = 4 if fp32 else 8
     W = info.stencil_width
-    grid = 3 * V * B * c
+    grid = 3 * Vp * B * c
     if cls == "spread":
         return grid + 48 * N * B + N * (3 * W * r + 12)
     if cls == "fft":
         return 2 * grid
-    if cls == "scale":
+    if cls in ("scale", "zfused"):
         return 2 * grid + 4 * V * r
     if cls == "gather":
         return grid + 2 * 48 * N * B + 16 * N * B + N * (3 * W * r + 12) + pairs * 20 + 24 * N
@@ -169,7 +175,7 @@ def main():
     run_once(s_mix, pos)
     _capi.profile_enable(False)
     classes = {}
-    for cls in ("spread", "fft", "scale", "gather", "krylov"):
+    for cls in ("spread", "fft", "scale", "zfused", "gather", "krylov"):
         ms, cnt = _capi.profile_get(cls)
         classes[cls] = {"total_ms": ms, "launches": cnt}
 
@@ -207,7 +213,7 @@ def main():
 
     # ---- roofline of the dominant kernel class ----
     pairs = int(info.num_pairs)
-    dom = max(("spread", "scale", "gather", "fft"), key=lambda c: classes[c]["total_ms"])
+    dom = max(("spread", "scale", "zfused", "gather", "fft"), key=lambda c: classes[c]["total_ms"])
     # average batch size of the launches: total RHS-applies / launches of the spread
     mv_total = sum(r["matvecs"] for w in reps_mix for r in w)
     mv64 = sum(r["matvecs_fp64"] for w in reps_mix for r in w)
@@ -251,6 +257,7 @@ def main():
                         "box 880x476.31x250 nm), Drude ITO eps_inf=4 wp=1.7 eV gamma=0.08 eV, "
                         "10 freqs 0.2-0.8 eV x 3 pol = 30 RHS, GMRES(50)+Jacobi, tol 1e-6",
             "grid": list(info.grid), "stencil": int(info.stencil_width), "rc": info.rc,
+            "slab_planes": int(info.slab_planes), "fft": "built-in" if info.fft_backend == 0 else "rocFFT",
             "rhs": B, "particles": N, "parallelism": "single GPU, all RHS batched",
         },
         "plain_fp64": plain,

@@ -33,7 +33,8 @@ _i64 = ctypes.c_int64
 
 class OpDesc(ctypes.Structure):
     _fields_ = [("box", ctypes.c_double * 3), ("errortol", ctypes.c_double), ("xi", ctypes.c_double),
-                ("grid", ctypes.c_int * 3), ("support", ctypes.c_int), ("layout", ctypes.c_int)]
+                ("grid", ctypes.c_int * 3), ("support", ctypes.c_int), ("layout", ctypes.c_int),
+                ("fft_backend", ctypes.c_int)]
 
 
 class OpInfo(ctypes.Structure):
@@ -42,7 +43,7 @@ class OpInfo(ctypes.Structure):
                 ("grid_spacing", ctypes.c_double * 3), ("spectral_split", ctypes.c_double * 3),
                 ("rc", ctypes.c_double), ("self_perp", ctypes.c_double),
                 ("self_coef_ewald", ctypes.c_double), ("bloch", ctypes.c_double * 2),
-                ("slab_planes", ctypes.c_int)]
+                ("slab_planes", ctypes.c_int), ("fft_backend", ctypes.c_int)]
 
 
 class SolveDesc(ctypes.Structure):

@@ -316,6 +316,343 @@ struct C2<float> {
     using T = float2;
 };
 
+// ---------------------------------------------------------------------------
+// Built-in batched line FFT: unnormalised complex DFTs of length n along one
+// axis of a strided array, computed in LDS with the Stockham auto-sort
+// formulation: a stage of radix R maps butterfly j of a line (j < n/R) to
+//   out[(j/Ns) Ns R + j%Ns + q Ns] = sum_r in[j + r n/R] W_{Ns R}^{(j%Ns) r} W_R^{r q}.
+// Radices 2, 3, 4, 5 and 7 run as register codelets; any other prime factor runs
+// as a generic stage in which a thread computes whole outputs from the
+// twiddle table (a single-stage direct DFT when n is prime).  Each stage stages
+// its results in registers and writes back in place, so a block needs one LDS
+// buffer of TL lines (<= 32 KB) plus the twiddles; one global read and one
+// global write per element.
+// ---------------------------------------------------------------------------
+constexpr int kMaxStages = 24;
+constexpr int kLdsElems32 = 2048;  // lines x length per block, FP32
+constexpr int kLdsElems64 = 2048;  // FP64
+
+struct LinePlan {
+    int n;
+    int nstages;
+    int radix[kMaxStages];
+};
+
+struct LineGeom {
+    long long nlines;
+    long long inner;         // lines per outer block
+    long long inner_stride;  // distance between consecutive lines of a block
+    long long outer;         // distance between blocks
+    long long elem_stride;   // distance between consecutive elements of a line
+};
+
+__device__ inline long long line_base(const LineGeom& g, long long l) {
+    return (l / g.inner) * g.outer + (l % g.inner) * g.inner_stride;
+}
+
+// x / d and x % d for 0 <= x < 2^22 via a float reciprocal and one correction.
+__device__ inline void fdivmod(int x, int d, float rcp, int& q, int& r) {
+    q = (int)((float)x * rcp);
+    r = x - q * d;
+    if (r < 0) {
+        --q;
+        r += d;
+    } else if (r >= d) {
+        ++q;
+        r -= d;
+    }
+}
+
+template <typename CT>
+__device__ inline CT cmul_t(CT a, CT b) {
+    CT r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+
+// In-place DFT of RAD values, forward (sgn = -1) or inverse (sgn = +1).
+template <typename R, int RAD>
+__device__ inline void dft_small(typename C2<R>::T* v, R sgn) {
+    using CT = typename C2<R>::T;
+    if (RAD == 2) {
+        const CT a = v[0], b = v[1];
+        v[0].x = a.x + b.x; v[0].y = a.y + b.y;
+        v[1].x = a.x - b.x; v[1].y = a.y - b.y;
+    } else if (RAD == 4) {
+        const CT a = v[0], b = v[1], c = v[2], d = v[3];
+        CT t0, t1, t2, t3;
+        t0.x = a.x + c.x; t0.y = a.y + c.y;
+        t1.x = a.x - c.x; t1.y = a.y - c.y;
+        t2.x = b.x + d.x; t2.y = b.y + d.y;
+        t3.x = b.x - d.x; t3.y = b.y - d.y;
+        // multiply t3 by sgn*i
+        CT u;
+        u.x = -sgn * t3.y;
+        u.y = sgn * t3.x;
+        v[0].x = t0.x + t2.x; v[0].y = t0.y + t2.y;
+        v[2].x = t0.x - t2.x; v[2].y = t0.y - t2.y;
+        v[1].x = t1.x + u.x; v[1].y = t1.y + u.y;
+        v[3].x = t1.x - u.x; v[3].y = t1.y - u.y;
+    } else {
+        // direct DFT with compile-time roots cos/sin(2 pi k / RAD)
+        R cs[RAD], sn[RAD];
+#pragma unroll
+        for (int k = 0; k < RAD; ++k) {
+            const double ang = 2.0 * 3.14159265358979323846 * k / RAD;
+            cs[k] = (R)__builtin_cos(ang);
+            sn[k] = (R)__builtin_sin(ang);
+        }
+        CT out[RAD];
+#pragma unroll
+        for (int q = 0; q < RAD; ++q) {
+            R ar = v[0].x, ai = v[0].y;
+#pragma unroll
+            for (int r = 1; r < RAD; ++r) {
+                const int k = (r * q) % RAD;
+                const R wr = cs[k], wi = sgn * sn[k];
+                ar += v[r].x * wr - v[r].y * wi;
+                ai += v[r].x * wi + v[r].y * wr;
+            }
+            out[q].x = ar;
+            out[q].y = ai;
+        }
+#pragma unroll
+        for (int q = 0; q < RAD; ++q) v[q] = out[q];
+    }
+}
+
+// One Stockham stage with a register codelet of radix RAD, in place on buf.
+template <typename R, int RAD>
+__device__ inline void stage_codelet(typename C2<R>::T* buf, int n, int nl, int Ns, const typename C2<R>::T* tw,
+                                     R sgn) {
+    using CT = typename C2<R>::T;
+    constexpr int KMAX = (kLdsElems32 / RAD + 255) / 256;
+    const int nb = n / RAD;
+    const int total = nl * nb;
+    const int step = n / (Ns * RAD);
+    const float rnb = 1.0f / (float)nb, rns = 1.0f / (float)Ns;
+    CT v[KMAX][RAD];
+    int dst[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        const int bf = threadIdx.x + 256 * k;
+        dst[k] = -1;
+        if (bf < total) {
+            int li, j, jd, a;
+            fdivmod(bf, nb, rnb, li, j);
+            fdivmod(j, Ns, rns, jd, a);
+            const CT* src = buf + li * n + j;
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) {
+                CT x = src[r * nb];
+                if (r > 0 && a > 0) x = cmul_t(x, tw[a * r * step]);
+                v[k][r] = x;
+            }
+            dft_small<R, RAD>(v[k], sgn);
+            dst[k] = li * n + jd * Ns * RAD + a;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k)
+        if (dst[k] >= 0)
+#pragma unroll
+            for (int q = 0; q < RAD; ++q) buf[dst[k] + q * Ns] = v[k][q];
+    __syncthreads();
+}
+
+// Generic Stockham stage of any radix R.  Pass 1 multiplies every input
+// x[j + r m] by its stage twiddle W_{Ns R}^{(j%Ns) r} in place; pass 2 computes
+// each output as a length-R DFT of the twiddled inputs with the R-th roots held
+// in a compact LDS table (lanes mostly read the same root: broadcasts).
+template <typename R>
+__device__ inline void stage_generic(typename C2<R>::T* buf, typename C2<R>::T* roots, int n, int nl, int Ns,
+                                     int Rx, const typename C2<R>::T* tw) {
+    using CT = typename C2<R>::T;
+    constexpr int KO = kLdsElems32 / 256;
+    const int m = n / Rx;
+    const int step = n / (Ns * Rx);
+    const int tot = nl * n;
+    const float rn = 1.0f / (float)n, rns = 1.0f / (float)Ns, rr = 1.0f / (float)Rx, rm = 1.0f / (float)m;
+    for (int k = threadIdx.x; k < Rx; k += blockDim.x) roots[k] = tw[k * m];
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        int li, p, r, j, jd, a;
+        fdivmod(e, n, rn, li, p);
+        fdivmod(p, m, rm, r, j);
+        fdivmod(j, Ns, rns, jd, a);
+        if (r > 0 && a > 0) buf[e] = cmul_t(buf[e], tw[a * r * step]);
+    }
+    __syncthreads();
+    CT o[KO];
+#pragma unroll
+    for (int k = 0; k < KO; ++k) {
+        const int e = threadIdx.x + 256 * k;
+        if (e < tot) {
+            int li, oi, rest, a, jd, q;
+            fdivmod(e, n, rn, li, oi);
+            fdivmod(oi, Ns, rns, rest, a);
+            fdivmod(rest, Rx, rr, jd, q);
+            const CT* src = buf + li * n + jd * Ns + a;
+            R ar = R(0), ai = R(0);
+            int kq = 0;
+            for (int r = 0; r < Rx; ++r) {
+                const CT x = src[r * m];
+                const CT w = roots[kq];
+                ar += x.x * w.x - x.y * w.y;
+                ai += x.x * w.y + x.y * w.x;
+                kq += q;
+                if (kq >= Rx) kq -= Rx;
+            }
+            o[k].x = ar;
+            o[k].y = ai;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KO; ++k) {
+        const int e = threadIdx.x + 256 * k;
+        if (e < tot) buf[e] = o[k];  // outputs are in natural (li, o) order
+    }
+    __syncthreads();
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void line_fft_kernel(typename C2<R>::T* __restrict__ data, LineGeom lg, LinePlan pl,
+                                                       const typename C2<R>::T* __restrict__ twiddle, int TL,
+                                                       int inverse) {
+    using CT = typename C2<R>::T;
+    extern __shared__ unsigned char s_raw[];
+    const int n = pl.n;
+    CT* buf = reinterpret_cast<CT*>(s_raw);
+    CT* tw = buf + (size_t)TL * n;
+    CT* roots = tw + n;
+    const int tid = threadIdx.x;
+    const long long l0 = (long long)blockIdx.x * TL;
+    const int nl = (int)min((long long)TL, lg.nlines - l0);
+    const R sgn = inverse ? R(1) : R(-1);
+    for (int k = tid; k < n; k += blockDim.x) {
+        CT w = twiddle[k];
+        if (inverse) w.y = -w.y;
+        tw[k] = w;
+    }
+    __shared__ long long s_lb[64];
+    if (tid < nl) s_lb[tid] = line_base(lg, l0 + tid);
+    __syncthreads();
+    const int tot = nl * n;
+    const float rn = 1.0f / (float)n, rnl = 1.0f / (float)nl;
+    if (lg.elem_stride == 1) {
+        for (int e = tid; e < tot; e += blockDim.x) {
+            int li, k;
+            fdivmod(e, n, rn, li, k);
+            buf[e] = data[s_lb[li] + k];
+        }
+    } else {
+        for (int e = tid; e < tot; e += blockDim.x) {
+            int k, li;
+            fdivmod(e, nl, rnl, k, li);
+            buf[li * n + k] = data[s_lb[li] + k * lg.elem_stride];
+        }
+    }
+    __syncthreads();
+    int Ns = 1;
+    for (int st = 0; st < pl.nstages; ++st) {
+        const int Rx = pl.radix[st];
+        switch (Rx) {
+            case 2: stage_codelet<R, 2>(buf, n, nl, Ns, tw, sgn); break;
+            case 3: stage_codelet<R, 3>(buf, n, nl, Ns, tw, sgn); break;
+            case 4: stage_codelet<R, 4>(buf, n, nl, Ns, tw, sgn); break;
+            case 5: stage_codelet<R, 5>(buf, n, nl, Ns, tw, sgn); break;
+            case 7: stage_codelet<R, 7>(buf, n, nl, Ns, tw, sgn); break;
+            default: stage_generic<R>(buf, roots, n, nl, Ns, Rx, tw); break;
+        }
+        Ns *= Rx;
+    }
+    if (lg.elem_stride == 1) {
+        for (int e = tid; e < tot; e += blockDim.x) {
+            int li, k;
+            fdivmod(e, n, rn, li, k);
+            data[s_lb[li] + k] = buf[e];
+        }
+    } else {
+        for (int e = tid; e < tot; e += blockDim.x) {
+            int k, li;
+            fdivmod(e, nl, rnl, k, li);
+            data[s_lb[li] + k * lg.elem_stride] = buf[li * n + k];
+        }
+    }
+}
+
+// One axis of the built-in FFT: plan + twiddle tables in both precisions.
+struct LineAxis {
+    LinePlan plan{};
+    DevBuf<double2> tw64;
+    DevBuf<float2> tw32;
+    explicit LineAxis(int n) {
+        plan.n = n;
+        plan.nstages = 0;
+        int m = n;
+        auto push = [&](int r) {
+            MPM_REQUIRE(plan.nstages < kMaxStages, MPM_ERR_UNSUPPORTED, "FFT length has too many factors");
+            plan.radix[plan.nstages++] = r;
+        };
+        while (m % 4 == 0) { push(4); m /= 4; }
+        while (m % 2 == 0) { push(2); m /= 2; }
+        for (int p = 3; p * p <= m; p += 2)
+            while (m % p == 0) { push(p); m /= p; }
+        if (m > 1) push(m);
+        std::vector<double2> t64(n);
+        std::vector<float2> t32(n);
+        for (int k = 0; k < n; ++k) {
+            double sv, cv;
+            // exp(-2 pi i k / n) with the argument reduced exactly
+            const double a = -2.0 * kPi * (double)k / (double)n;
+            sv = std::sin(a);
+            cv = std::cos(a);
+            t64[k] = make_double2(cv, sv);
+            t32[k] = make_float2((float)cv, (float)sv);
+        }
+        tw64.upload(t64.data(), n);
+        tw32.upload(t32.data(), n);
+    }
+};
+
+// Built-in batched FFT over the axes of one reciprocal layout.
+struct LineFft {
+    std::vector<std::unique_ptr<LineAxis>> axes;  // one per transformed axis
+    std::vector<LineGeom> geom_per_rhs;            // nlines counted for one RHS
+    long long rhs_elems = 0;                       // elements per RHS (grid size)
+
+    // largest power of two TL with TL * n within the per-block element budget
+    static int lines_per_block(int n, size_t cs) {
+        const int cap = cs == sizeof(float2) ? kLdsElems32 : kLdsElems64;
+        int tl = 1;
+        while (tl * 2 * n <= cap && tl < 64) tl *= 2;
+        return tl;
+    }
+    static bool fits(int n, size_t cs) { return n <= (cs == sizeof(float2) ? kLdsElems32 : kLdsElems64); }
+
+    template <typename R>
+    void exec(typename C2<R>::T* grid, int B, bool forward, hipStream_t s) {
+        using CT = typename C2<R>::T;
+        for (size_t a = 0; a < axes.size(); ++a) {
+            LineAxis& ax = *axes[a];
+            // the line descriptors of one RHS extend unchanged over the B stacked RHS
+            LineGeom g = geom_per_rhs[a];
+            g.nlines *= B;
+            const int n = ax.plan.n;
+            const int TL = lines_per_block(n, sizeof(CT));
+            const size_t lds = ((size_t)TL * n + 2 * n) * sizeof(CT);
+            const long long blocks = (g.nlines + TL - 1) / TL;
+            const CT* tw = std::is_same<R, float>::value ? reinterpret_cast<const CT*>(ax.tw32.p)
+                                                         : reinterpret_cast<const CT*>(ax.tw64.p);
+            hipLaunchKernelGGL(line_fft_kernel<R>, dim3((unsigned)blocks), dim3(256), lds, s, grid, g, ax.plan, tw, TL,
+                               forward ? 0 : 1);
+            MPM_HIP(hipGetLastError());
+        }
+    }
+};
+
 // k-space Green's tensor at wave vector (Gx - kbx, Gy - kby, Gz): (k^x, k^y, k^z, sc)
 // with sc = 9 (sin k / k - cos k)^2 exp(-(k^2 - sum_d G_d^2 eta_d) / (4 xi^2)) / k^4 / V
 // and zero at k = 0.  monodisperse_ewald_electric_field.cu:1300-1404.
@@ -380,6 +717,7 @@ struct Tile<true> {
 struct SpreadGeom {
     int nx, ny, nz;
     int half, width;
+    int nofold;         // 1 when nx, ny, nz >= width (no stencil point folds onto another)
     int bx, by, bz;     // bin size (voxels)
     int nbx, nby, nbz;  // bins per axis
     int ntx, nty;       // tiles per axis (x, y)
@@ -533,7 +871,31 @@ __global__ __launch_bounds__(256) void spread_kernel(SlotTable tab, int B, Sprea
             }
         }
         __syncthreads();
-        if (valid) {
+        if (valid && g.nofold) {
+            // every axis is at least as wide as the stencil: one wrap, one weight per axis
+            for (int q = 0; q < cnt; ++q) {
+                int dx = vx - s_base[q][0];
+                dx += (dx < -half) ? g.nx : 0;
+                dx -= (dx > half) ? g.nx : 0;
+                if (dx < -half || dx > half) continue;
+                int dy = vy - s_base[q][1];
+                dy += (dy < -half) ? g.ny : 0;
+                dy -= (dy > half) ? g.ny : 0;
+                if (dy < -half || dy > half) continue;
+                int dz = vz - s_base[q][2];
+                dz += (dz < -half) ? g.nz : 0;
+                dz -= (dz > half) ? g.nz : 0;
+                if (dz < -half || dz > half) continue;
+                const R w = s_w[q][dx + half] * s_w[q][width + dy + half] * s_w[q][2 * width + dz + half];
+#pragma unroll
+                for (int j = 0; j < G; ++j)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        acc[j][c][0] += w * s_p[q][j][c].x;
+                        acc[j][c][1] += w * s_p[q][j][c].y;
+                    }
+            }
+        } else if (valid) {
             const int vv[3] = {vx, vy, vz};
             const int nn[3] = {g.nx, g.ny, g.nz};
             for (int q = 0; q < cnt; ++q) {
@@ -725,11 +1087,29 @@ __global__ __launch_bounds__(256) void gather_kernel(
     const int* __restrict__ zmap, const int* __restrict__ base, const R* __restrict__ wts, int N, double h3,
     const double* __restrict__ pos, const int* __restrict__ nb_start, const int* __restrict__ nb_j,
     const double2* __restrict__ nb_f, double Lx, double Ly, double Lz, double kbx, double kby,
-    const double2* __restrict__ selfc, double self_perp) {
+    const double2* __restrict__ selfc, double self_perp, int jacobi) {
+    __shared__ int s_off[kMaxWidth * kMaxWidth * kMaxWidth];
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (i >= N) return;
     const int width = g.width, half = g.half;
+    {   // stencil offsets (ox, oy, oz) packed, in the order matching the grid layout
+        const int S0 = width * width * width;
+        for (int t = threadIdx.x; t < S0; t += blockDim.x) {
+            int ox, oy, oz;
+            if (SLAB) {
+                oy = t % width;
+                ox = (t / width) % width;
+                oz = t / (width * width);
+            } else {
+                oz = t % width;
+                oy = (t / width) % width;
+                ox = t / (width * width);
+            }
+            s_off[t] = ox | (oy << 8) | (oz << 16);
+        }
+    }
+    __syncthreads();
+    if (i >= N) return;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int S = width * width * width;
     const size_t nxy = (size_t)nx * ny;
@@ -759,20 +1139,18 @@ __global__ __launch_bounds__(256) void gather_kernel(
 #pragma unroll
                 for (int c = 0; c < 3; ++c) racc[j][c][0] = racc[j][c][1] = R(0);
             for (int t = lane; t < S; t += 64) {
-                int ox, oy, oz;
-                if (SLAB) {  // y fastest, matching the slab layout
-                    oy = t % width;
-                    ox = (t / width) % width;
-                    oz = t / (width * width);
-                } else {
-                    oz = t % width;
-                    oy = (t / width) % width;
-                    ox = t / (width * width);
-                }
+                const int pk = s_off[t];
+                const int ox = pk & 255, oy = (pk >> 8) & 255, oz = pk >> 16;
                 int vx = sx + ox, vy = sy + oy, vz = sz + oz;
-                while (vx >= nx) vx -= nx;
-                while (vy >= ny) vy -= ny;
-                while (vz >= nz) vz -= nz;
+                if (g.nofold) {
+                    vx -= (vx >= nx) ? nx : 0;
+                    vy -= (vy >= ny) ? ny : 0;
+                    vz -= (vz >= nz) ? nz : 0;
+                } else {
+                    while (vx >= nx) vx -= nx;
+                    while (vy >= ny) vy -= ny;
+                    while (vz >= nz) vz -= nz;
+                }
                 const R ww = w[ox] * w[width + oy] * w[2 * width + oz];
                 size_t v;
                 if (SLAB)
@@ -854,11 +1232,19 @@ __global__ __launch_bounds__(256) void gather_kernel(
                     fs.x += self_perp;
                 }
                 double2* y = tab.y[b] + 3 * (size_t)i;
+                // optional left Jacobi: y <- y / self_coef (numerical_solver.cu:207-242)
+                double2 inv = make_double2(1.0, 0.0);
+                if (jacobi) {
+                    const double2 sc = selfc[(size_t)tab.row[b] * N + i];
+                    const double den = sc.x * sc.x + sc.y * sc.y;
+                    if (den > 1e-30) inv = make_double2(sc.x / den, -sc.y / den);
+                }
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const double2 pc = p[c];
-                    y[c] = make_double2(acc[j][c][0] + fs.x * pc.x - fs.y * pc.y,
-                                        acc[j][c][1] + fs.x * pc.y + fs.y * pc.x);
+                    const double yr = acc[j][c][0] + fs.x * pc.x - fs.y * pc.y;
+                    const double yi = acc[j][c][1] + fs.x * pc.y + fs.y * pc.x;
+                    y[c] = make_double2(yr * inv.x - yi * inv.y, yr * inv.y + yi * inv.x);
                 }
             }
         }
@@ -875,6 +1261,8 @@ EwaldOp::EwaldOp(const mpm_op_desc& d) {
     compute_sizing(L, errortol, xi, d.grid, d.support, S);
     MPM_REQUIRE(d.layout == 0 || d.layout == 1, MPM_ERR_INVALID_ARGUMENT, "layout must be 0 or 1");
     force_full = d.layout;
+    MPM_REQUIRE(d.fft_backend == 0 || d.fft_backend == 1, MPM_ERR_INVALID_ARGUMENT, "fft_backend must be 0 or 1");
+    fft_backend = d.fft_backend;
     MPM_REQUIRE(S.width <= kMaxWidth, MPM_ERR_UNSUPPORTED,
                 "stencil wider than " + std::to_string(kMaxWidth) + " points per axis");
     self_perp = dipole_self_term(xi);
@@ -1097,6 +1485,29 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     d_nb_f.upload(nf.data(), nf.size());
     // FFT plans for the chosen layout
     const int key = slab ? Zs : -1;
+    if (!lfft || fft_key != key) {
+        const int nx = S.n[0], ny = S.n[1], nzz = S.n[2];
+        const long long nxy = (long long)nx * ny;
+        auto lf = std::make_unique<LineFft>();
+        bool ok = true;
+        auto add = [&](int n, LineGeom g) {
+            ok = ok && LineFft::fits(n, sizeof(double2));
+            if (!ok) return;
+            lf->axes.push_back(std::make_unique<LineAxis>(n));
+            lf->geom_per_rhs.push_back(g);
+        };
+        if (slab) {  // [3][Zs][nx][ny]: y lines (contiguous), then x lines (stride ny)
+            lf->rhs_elems = 3LL * Zs * nxy;
+            if (ny > 1) add(ny, LineGeom{3LL * Zs * nx, 1, 0, ny, 1});
+            if (nx > 1) add(nx, LineGeom{3LL * Zs * ny, ny, 1, nxy, ny});
+        } else {     // [3][nx][ny][nz]: z, y, x lines
+            lf->rhs_elems = 3LL * nxy * nzz;
+            if (nzz > 1) add(nzz, LineGeom{3LL * nxy, 1, 0, nzz, 1});
+            if (ny > 1) add(ny, LineGeom{3LL * nx * nzz, nzz, 1, (long long)ny * nzz, nzz});
+            if (nx > 1) add(nx, LineGeom{3LL * ny * nzz, (long long)ny * nzz, 1, nxy * nzz, (long long)ny * nzz});
+        }
+        lfft = ok ? std::move(lf) : nullptr;
+    }
     if (!fft || fft_key != key) {
         if (slab)
             fft = std::make_unique<FftPlans>(std::vector<size_t>{(size_t)S.n[1], (size_t)S.n[0]}, 3 * Zs);
@@ -1124,7 +1535,7 @@ size_t EwaldOp::grid_elems_per_rhs() const {
 }
 
 template <typename R, bool SLAB>
-void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s) {
+void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int jacobi) {
     using CT = typename C2<R>::T;
     const bool fp32 = std::is_same<R, float>::value;
     CT* grid = fp32 ? reinterpret_cast<CT*>(d_grid32.p) : reinterpret_cast<CT*>(d_grid64.p);
@@ -1135,6 +1546,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s) {
     g.nz = S.n[2];
     g.half = S.half;
     g.width = S.width;
+    g.nofold = (S.n[0] >= S.width && S.n[1] >= S.width && S.n[2] >= S.width) ? 1 : 0;
     g.bx = binsz[0];
     g.by = binsz[1];
     g.bz = binsz[2];
@@ -1160,7 +1572,10 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s) {
         const size_t bytes_per_rhs = grid_elems_per_rhs() * sizeof(CT);
         {
             ProfScope ps("fft", s);
-            fft->exec(grid, bytes_per_rhs, B, fp32, true, s);
+            if (lfft && fft_backend == 0)
+                lfft->exec<R>(grid, B, true, s);
+            else
+                fft->exec(grid, bytes_per_rhs, B, fp32, true, s);
         }
         if (SLAB) {
             ProfScope ps("zfused", s);
@@ -1205,7 +1620,10 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s) {
         }
         {
             ProfScope ps("fft", s);
-            fft->exec(grid, bytes_per_rhs, B, fp32, false, s);
+            if (lfft && fft_backend == 0)
+                lfft->exec<R>(grid, B, false, s);
+            else
+                fft->exec(grid, bytes_per_rhs, B, fp32, false, s);
         }
     }
     {
@@ -1215,16 +1633,16 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s) {
         if (B == 1)
             hipLaunchKernelGGL((gather_kernel<R, 1, SLAB>), dim3(blocks), dim3(256), 0, s, t, B, parts, grid, g,
                                d_zmap.p, d_base.p, w, Ni, h3, d_pos.p, d_nb_start.p, d_nb_j.p, d_nb_f.p, L[0], L[1],
-                               L[2], kb[0], kb[1], d_selfc.p, self_perp);
+                               L[2], kb[0], kb[1], d_selfc.p, self_perp, jacobi);
         else
             hipLaunchKernelGGL((gather_kernel<R, 4, SLAB>), dim3(blocks), dim3(256), 0, s, t, B, parts, grid, g,
                                d_zmap.p, d_base.p, w, Ni, h3, d_pos.p, d_nb_start.p, d_nb_j.p, d_nb_f.p, L[0], L[1],
-                               L[2], kb[0], kb[1], d_selfc.p, self_perp);
+                               L[2], kb[0], kb[1], d_selfc.p, self_perp, jacobi);
         MPM_HIP(hipGetLastError());
     }
 }
 
-void EwaldOp::apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t s) {
+void EwaldOp::apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t s, bool jacobi) {
     MPM_REQUIRE(N > 0, MPM_ERR_RUNTIME, "apply before set_positions");
     MPM_REQUIRE(B >= 1 && B <= kMaxSlots, MPM_ERR_INVALID_ARGUMENT, "bad slot count");
     for (int b = 0; b < B; ++b)
@@ -1236,12 +1654,13 @@ void EwaldOp::apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t
         d_grid32.reserve(elems);
     else
         d_grid64.reserve(elems);
+    const int jac = jacobi ? 1 : 0;
     if (fp32) {
-        if (slab) launch<float, true>(t, B, parts, s);
-        else launch<float, false>(t, B, parts, s);
+        if (slab) launch<float, true>(t, B, parts, s, jac);
+        else launch<float, false>(t, B, parts, s, jac);
     } else {
-        if (slab) launch<double, true>(t, B, parts, s);
-        else launch<double, false>(t, B, parts, s);
+        if (slab) launch<double, true>(t, B, parts, s, jac);
+        else launch<double, false>(t, B, parts, s, jac);
     }
 }
 
@@ -1277,6 +1696,7 @@ void EwaldOp::info(mpm_op_info& o) const {
     o.bloch[0] = kb[0];
     o.bloch[1] = kb[1];
     o.slab_planes = slab ? Zs : 0;
+    o.fft_backend = (lfft && fft_backend == 0) ? 0 : 1;
 }
 
 }  // namespace mpm

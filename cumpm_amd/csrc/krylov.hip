@@ -149,15 +149,6 @@ __global__ __launch_bounds__(256) void residual_kernel(Ptrs p, size_t n, const d
     p.b[slot][e] = r;
 }
 
-__global__ __launch_bounds__(256) void jacobi_kernel(Ptrs p, size_t n, const double2* __restrict__ selfc,
-                                                     const int* __restrict__ rows, int N) {
-    const int slot = blockIdx.y;
-    const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const int s = p.sys[slot];
-    p.b[slot][e] = cmul(p.b[slot][e], jac_inv(selfc[(size_t)rows[s] * N + e / 3]));
-}
-
 // b = a * (1/norm[sys]) when norm > 0
 __global__ __launch_bounds__(256) void scale_copy_kernel(Ptrs p, size_t n, const double* __restrict__ norm2) {
     const int slot = blockIdx.y;
@@ -378,16 +369,17 @@ struct Ctx {
         if (host_pinned) (void)hipHostFree(host_pinned);
     }
 
-    // batched apply with per-system pointers; fp32 selects the reciprocal precision
+    // batched apply with per-system pointers; fp32 selects the reciprocal precision,
+    // jacobi fuses the left preconditioner into the operator's store
     void apply(const std::vector<int>& sys, const std::vector<const double2*>& x,
-               const std::vector<double2*>& y, bool fp32) {
+               const std::vector<double2*>& y, bool fp32, bool jacobi = false) {
         SlotTable t;
         for (size_t j = 0; j < sys.size(); ++j) {
             t.x[j] = x[j];
             t.y[j] = y[j];
             t.row[j] = rows_h[sys[j]];
         }
-        op.apply(t, (int)sys.size(), fp32, MPM_PART_ALL, s);
+        op.apply(t, (int)sys.size(), fp32, MPM_PART_ALL, s, jacobi);
     }
 
     // squared norms of vectors b[slot] -> W.nrm[sys]
@@ -562,17 +554,10 @@ static void gmres_group(Ctx& C, const double2* b, double2* x, int B, mpm_solve_r
                     xs.push_back(Vk(q, k));
                     ys.push_back(W.w.p + q * n);
                 }
-                C.apply(in_cycle, xs, ys, inner32);
+                C.apply(in_cycle, xs, ys, inner32, d.jacobi != 0);
                 for (int q : in_cycle) {
                     matvecs[q]++;
                     if (!inner32) mv64[q]++;
-                }
-                if (d.jacobi) {
-                    Ptrs pj = make_ptrs(in_cycle, [&](int q) { return W.w.p + q * n; },
-                                        [&](int q) { return W.w.p + q * n; });
-                    hipLaunchKernelGGL(jacobi_kernel, grid_for(n, ns), dim3(256), 0, C.s, pj, n,
-                                       C.op.self_coef(), W.rows.p, C.N);
-                    MPM_HIP(hipGetLastError());
                 }
             }
             // CGS2: h = V^H w ; w -= V h ; h2 = V^H w ; w -= V h2 ; h += h2
@@ -762,16 +747,10 @@ static void bicgstab_group(Ctx& C, const double2* b, double2* x, int B, mpm_solv
             xs.push_back(in_of(q));
             ys.push_back(vec(out, q));
         }
-        C.apply(sys, xs, ys, use32);
+        C.apply(sys, xs, ys, use32, d.jacobi != 0);
         for (int q : sys) {
             matvecs[q]++;
             if (!use32) mv64[q]++;
-        }
-        if (d.jacobi) {
-            Ptrs pj = make_ptrs(sys, [&](int q) { return vec(out, q); }, [&](int q) { return vec(out, q); });
-            hipLaunchKernelGGL(jacobi_kernel, grid_for(n, (int)sys.size()), dim3(256), 0, C.s, pj, n,
-                               C.op.self_coef(), W.rows.p, C.N);
-            MPM_HIP(hipGetLastError());
         }
     };
     // b norms

@@ -109,6 +109,7 @@ struct SlotTable {
 struct FftPlans;   // rocFFT plans keyed by (precision, batch)
 struct PairTable;  // sampled real-space pair functions
 struct KParams;    // k-space geometry passed to the kernels
+struct LineFft;    // built-in batched line FFT
 
 class EwaldOp {
    public:
@@ -117,8 +118,9 @@ class EwaldOp {
     void set_positions(const double* xyz, int64_t N);
     void set_bloch(double kx, double ky);
     void set_self_coef(const double* coef, int rows);
-    // y[b] = A x[b] for the slots in `t` (count B <= kMaxSlots)
-    void apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t s);
+    // y[b] = A x[b] (or M^-1 A x[b] with the left Jacobi preconditioner) for the slots
+    // in `t` (count B <= kMaxSlots)
+    void apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t s, bool jacobi = false);
     // convenience: contiguous [B][3N] buffers, chunked by kMaxSlots
     void apply_contiguous(const double2* x, double2* y, int B, bool fp32, int parts,
                           hipStream_t s, const int* rows = nullptr);
@@ -134,7 +136,7 @@ class EwaldOp {
     KParams kparams() const;
     size_t grid_elems_per_rhs() const;
     template <typename R, bool SLAB>
-    void launch(const SlotTable& t, int B, int parts, hipStream_t s);
+    void launch(const SlotTable& t, int B, int parts, hipStream_t s, int jacobi);
 
     double L[3], errortol, xi;
     Sizing S;
@@ -175,7 +177,9 @@ class EwaldOp {
     DevBuf<double2> d_grid64;
     DevBuf<float2> d_grid32;
     std::unique_ptr<FftPlans> fft;
+    std::unique_ptr<LineFft> lfft;
     int fft_key = -2;
+    int fft_backend = 0;         // 0: built-in LDS FFT, 1: rocFFT
     std::unique_ptr<PairTable> table;
     std::vector<double> last_xyz;  // positions of the current precalcs
 

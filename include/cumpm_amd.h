@@ -66,6 +66,8 @@ typedef struct {
     int support;     /* num_grid_gaussian P; 0 = reference rule ceil(-2 ln tol / pi) */
     int layout;      /* reciprocal-space layout: 0 = automatic (the z-plane slab when the
                         particle stencils touch at most half of the z-planes), 1 = full grid */
+    int fft_backend; /* 0 = built-in batched LDS FFT (rocFFT when a length does not fit),
+                        1 = rocFFT */
 } mpm_op_desc;
 
 typedef struct {
@@ -81,6 +83,7 @@ typedef struct {
     double self_coef_ewald; /* -4 xi^3 / (3 sqrt(pi)) */
     double bloch[2];
     int slab_planes;     /* occupied z-planes of the slab layout; 0 = full grid */
+    int fft_backend;     /* FFT in use: 0 = built-in, 1 = rocFFT */
 } mpm_op_info;
 
 /* Monodisperse_Ewald_Electric_Field(box_x, box_y, box_z, errortol, xi, SOLVER_AX, radius=1)
@@ -211,7 +214,7 @@ mpm_status mpm_device_synchronize(void);
  * every launch of the named kernel class is bracketed by events and accumulated. */
 mpm_status mpm_profile_enable(int enable);
 mpm_status mpm_profile_reset(void);
-/* name: "spread", "fft", "scale", "gather", "krylov"; total_ms over count launches. */
+/* name: "spread", "fft", "scale", "zfused", "gather", "krylov"; total_ms over count launches. */
 mpm_status mpm_profile_get(const char* name, double* total_ms, int64_t* count);
 
 #ifdef __cplusplus

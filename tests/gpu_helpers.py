@@ -9,9 +9,10 @@ from cumpm_amd import _capi as C
 class Op:
     """Owns an mpm_op plus device buffers; every call goes through the C ABI."""
 
-    def __init__(self, box, errortol, xi, grid=None, support=0, layout=0):
+    def __init__(self, box, errortol, xi, grid=None, support=0, layout=0, fft_backend=0):
         d = C.OpDesc()
         d.layout = int(layout)
+        d.fft_backend = int(fft_backend)
         d.box[:] = [float(v) for v in box]
         d.errortol = float(errortol)
         d.xi = float(xi)

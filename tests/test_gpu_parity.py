@@ -62,21 +62,23 @@ def _geom(name):
 SLAB_GEOMS = ("hex_slab", "bilayer")
 
 
-@pytest.mark.parametrize("name,layout", [(g[0], 0) for g in GEOMS] + [(g, 1) for g in SLAB_GEOMS])
-def test_apply_fp64_matches_oracle(name, layout):
+@pytest.mark.parametrize("name,layout,fft", [(g[0], 0, 0) for g in GEOMS] + [(g, 1, 0) for g in SLAB_GEOMS] +
+                         [("random3d", 0, 1), ("hex_slab", 0, 1), ("hex_slab", 1, 1), ("sc_tight", 0, 1)])
+def test_apply_fp64_matches_oracle(name, layout, fft):
     box, tol, xi, pos, grid, P = _geom(name)
-    _case(f"apply fp64 {name} layout={layout} N={len(pos)}")
+    _case(f"apply fp64 {name} layout={layout} fft={fft} N={len(pos)}")
     N = len(pos)
     rng = np.random.default_rng(11)
     B = 3
     p = rng.standard_normal((B, N, 3)) + 1j * rng.standard_normal((B, N, 3))
     rows = rng.standard_normal((B, N)) + 1j * rng.standard_normal((B, N))
-    op = Op(box, tol, xi, grid, P, layout=layout)
+    op = Op(box, tol, xi, grid, P, layout=layout, fft_backend=fft)
     op.set_positions(pos)
     op.set_self_coef(rows)
     ref_op = O.Operator(box, tol, xi, grid, P)
     ref_op.set_positions(pos)
     info = op.info()
+    assert info.fft_backend == fft
     if layout == 1:
         assert info.slab_planes == 0
     elif name in SLAB_GEOMS:
@@ -93,16 +95,16 @@ def test_apply_fp64_matches_oracle(name, layout):
             assert rel(got[b], ref) < 1e-11, (name, parts, b, rel(got[b], ref))
 
 
-@pytest.mark.parametrize("name,layout", [("random3d", 0), ("hex_slab", 0), ("hex_slab", 1), ("sc_tight", 0),
-                                         ("bilayer", 0)])
-def test_apply_fp32_reciprocal_within_reference_tolerance(name, layout):
+@pytest.mark.parametrize("name,layout,fft", [("random3d", 0, 0), ("hex_slab", 0, 0), ("hex_slab", 1, 0),
+                                             ("sc_tight", 0, 0), ("bilayer", 0, 0), ("hex_slab", 0, 1)])
+def test_apply_fp32_reciprocal_within_reference_tolerance(name, layout, fft):
     box, tol, xi, pos, grid, P = _geom(name)
-    _case(f"apply fp32 {name} layout={layout}")
+    _case(f"apply fp32 {name} layout={layout} fft={fft}")
     N = len(pos)
     rng = np.random.default_rng(12)
     p = rng.standard_normal((2, N, 3)) + 1j * rng.standard_normal((2, N, 3))
     rows = rng.standard_normal((2, N)) + 1j * rng.standard_normal((2, N))
-    op = Op(box, tol, xi, grid, P, layout=layout)
+    op = Op(box, tol, xi, grid, P, layout=layout, fft_backend=fft)
     op.set_positions(pos)
     op.set_self_coef(rows)
     ref_op = O.Operator(box, tol, xi, grid, P)
