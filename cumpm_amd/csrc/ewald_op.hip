@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <numeric>
 
@@ -321,16 +322,23 @@ struct C2<float> {
 // axis of a strided array, computed in LDS with the Stockham auto-sort
 // formulation: a stage of radix R maps butterfly j of a line (j < n/R) to
 //   out[(j/Ns) Ns R + j%Ns + q Ns] = sum_r in[j + r n/R] W_{Ns R}^{(j%Ns) r} W_R^{r q}.
-// Radices 2, 3, 4, 5 and 7 run as register codelets; any other prime factor runs
-// as a generic stage in which a thread computes whole outputs from the
-// twiddle table (a single-stage direct DFT when n is prime).  Each stage stages
-// its results in registers and writes back in place, so a block needs one LDS
-// buffer of TL lines (<= 32 KB) plus the twiddles; one global read and one
-// global write per element.
+// The planner packs the prime factors of n into as few radices as possible
+// (composites up to 16, primes up to 23 on their own: 210 -> 14 x 15,
+// 114 -> 19 x 6), and every radix runs as a register DFT of compile-time size:
+// odd primes with the symmetric-pair direct form, composites as an in-register
+// four-step of smaller DFTs with constant twiddles.  Prime factors above 23
+// run as a generic stage in which a thread computes whole outputs from the
+// twiddle table.  Stages alternate between two LDS buffers of TL lines (32 KB
+// together) plus the twiddles; one global read and one global write per element.
 // ---------------------------------------------------------------------------
 constexpr int kMaxStages = 24;
-constexpr int kLdsElems32 = 2048;  // lines x length per block, FP32
-constexpr int kLdsElems64 = 2048;  // FP64
+constexpr int kMaxPrimeRadix = 23;  // larger prime factors use the generic stage
+constexpr int kMaxCompositeRadix = 16;
+
+template <typename R>
+constexpr int fft_elems() {  // lines x length per block (two LDS buffers of this size)
+    return sizeof(R) == 4 ? 2048 : 1024;
+}
 
 struct LinePlan {
     int n;
@@ -371,106 +379,215 @@ __device__ inline CT cmul_t(CT a, CT b) {
     return r;
 }
 
-// In-place DFT of RAD values, forward (sgn = -1) or inverse (sgn = +1).
-template <typename R, int RAD>
-__device__ inline void dft_small(typename C2<R>::T* v, R sgn) {
+template <int P>
+struct SmallestFactor {
+    static constexpr int v = (P % 2 == 0)    ? 2
+                             : (P % 3 == 0)  ? 3
+                             : (P % 5 == 0)  ? 5
+                             : (P % 7 == 0)  ? 7
+                             : (P % 11 == 0) ? 11
+                             : (P % 13 == 0) ? 13
+                                             : P;
+};
+
+// cos and sin of 2 pi m / P; exact on the multiples of pi / 4.  Called with
+// compile-time arguments only (inside fully unrolled loops), so it folds to
+// constants.
+__device__ __forceinline__ void unit_root(int m, int P, double& c, double& s) {
+    m = ((m % P) + P) % P;
+    if ((8 * m) % P == 0) {
+        const int o = 8 * m / P;
+        const double h = 0.70710678118654752440;
+        const double cs[8] = {1.0, h, 0.0, -h, -1.0, -h, 0.0, h};
+        const double sn[8] = {0.0, h, 1.0, h, 0.0, -h, -1.0, -h};
+        c = cs[o];
+        s = sn[o];
+    } else {
+        const double a = 2.0 * 3.14159265358979323846 * (double)m / (double)P;
+        c = __builtin_cos(a);
+        s = __builtin_sin(a);
+    }
+}
+
+// x * exp(-+2 pi i m / P) (minus for the forward transform), with the trivial
+// rotations done as swaps and negations.
+template <typename CT, bool INV>
+__device__ __forceinline__ CT rot(CT x, int m, int P) {
+    m = ((m % P) + P) % P;
+    if (m == 0) return x;
+    CT r;
+    if (2 * m == P) {
+        r.x = -x.x;
+        r.y = -x.y;
+        return r;
+    }
+    if (4 * m == P || 4 * m == 3 * P) {  // multiply by +-i
+        const bool plus_i = (4 * m == P) == INV;
+        if (plus_i) {
+            r.x = -x.y;
+            r.y = x.x;
+        } else {
+            r.x = x.y;
+            r.y = -x.x;
+        }
+        return r;
+    }
+    double c, s;
+    unit_root(m, P, c, s);
+    if (!INV) s = -s;
+    using R = decltype(x.x);
+    const R cr = (R)c, sr = (R)s;
+    r.x = x.x * cr - x.y * sr;
+    r.y = x.x * sr + x.y * cr;
+    return r;
+}
+
+// In-place DFT of P values held in registers, natural order in and out.
+template <typename R, int P, bool INV>
+__device__ __forceinline__ void dft_reg(typename C2<R>::T* v);
+
+// Odd prime P: X_0 = sum x_j; for k = 1..(P-1)/2 with s_j = x_j + x_{P-j},
+// d_j = x_j - x_{P-j}: A_k = x_0 + sum_j s_j cos(2 pi jk/P),
+// B_k = sum_j d_j sin(2 pi jk/P), X_k = A_k -+ i B_k, X_{P-k} = A_k +- i B_k.
+template <typename R, int P, bool INV>
+__device__ __forceinline__ void dft_odd_prime(typename C2<R>::T* v) {
     using CT = typename C2<R>::T;
-    if (RAD == 2) {
+    constexpr int H = (P - 1) / 2;
+    CT sm[H + 1], df[H + 1];
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+        sm[j].x = v[j].x + v[P - j].x;
+        sm[j].y = v[j].y + v[P - j].y;
+        df[j].x = v[j].x - v[P - j].x;
+        df[j].y = v[j].y - v[P - j].y;
+    }
+    const CT x0 = v[0];
+    CT X0 = x0;
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+        X0.x += sm[j].x;
+        X0.y += sm[j].y;
+    }
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+        R ar = x0.x, ai = x0.y, br = R(0), bi = R(0);
+#pragma unroll
+        for (int j = 1; j <= H; ++j) {
+            double c, s;
+            unit_root(j * k, P, c, s);
+            ar += sm[j].x * (R)c;
+            ai += sm[j].y * (R)c;
+            br += df[j].x * (R)s;
+            bi += df[j].y * (R)s;
+        }
+        if (!INV) {
+            v[k].x = ar + bi;
+            v[k].y = ai - br;
+            v[P - k].x = ar - bi;
+            v[P - k].y = ai + br;
+        } else {
+            v[k].x = ar - bi;
+            v[k].y = ai + br;
+            v[P - k].x = ar + bi;
+            v[P - k].y = ai - br;
+        }
+    }
+    v[0] = X0;
+}
+
+template <typename R, int P, bool INV>
+__device__ __forceinline__ void dft_reg(typename C2<R>::T* v) {
+    using CT = typename C2<R>::T;
+    if constexpr (P == 1) {
+        return;
+    } else if constexpr (P == 2) {
         const CT a = v[0], b = v[1];
-        v[0].x = a.x + b.x; v[0].y = a.y + b.y;
-        v[1].x = a.x - b.x; v[1].y = a.y - b.y;
-    } else if (RAD == 4) {
+        v[0].x = a.x + b.x;
+        v[0].y = a.y + b.y;
+        v[1].x = a.x - b.x;
+        v[1].y = a.y - b.y;
+    } else if constexpr (P == 4) {
         const CT a = v[0], b = v[1], c = v[2], d = v[3];
-        CT t0, t1, t2, t3;
+        CT t0, t1, t2, t3, u;
         t0.x = a.x + c.x; t0.y = a.y + c.y;
         t1.x = a.x - c.x; t1.y = a.y - c.y;
         t2.x = b.x + d.x; t2.y = b.y + d.y;
         t3.x = b.x - d.x; t3.y = b.y - d.y;
-        // multiply t3 by sgn*i
-        CT u;
-        u.x = -sgn * t3.y;
-        u.y = sgn * t3.x;
+        if (INV) {  // u = +i t3
+            u.x = -t3.y;
+            u.y = t3.x;
+        } else {    // u = -i t3
+            u.x = t3.y;
+            u.y = -t3.x;
+        }
         v[0].x = t0.x + t2.x; v[0].y = t0.y + t2.y;
         v[2].x = t0.x - t2.x; v[2].y = t0.y - t2.y;
         v[1].x = t1.x + u.x; v[1].y = t1.y + u.y;
         v[3].x = t1.x - u.x; v[3].y = t1.y - u.y;
+    } else if constexpr (SmallestFactor<P>::v == P) {
+        dft_odd_prime<R, P, INV>(v);
     } else {
-        // direct DFT with compile-time roots cos/sin(2 pi k / RAD)
-        R cs[RAD], sn[RAD];
+        // P = A B: input j1 + A j2, output k2 + B k1
+        constexpr int A = (P % 4 == 0) ? 4 : SmallestFactor<P>::v;
+        constexpr int B = P / A;
+        CT y[A][B];
 #pragma unroll
-        for (int k = 0; k < RAD; ++k) {
-            const double ang = 2.0 * 3.14159265358979323846 * k / RAD;
-            cs[k] = (R)__builtin_cos(ang);
-            sn[k] = (R)__builtin_sin(ang);
-        }
-        CT out[RAD];
+        for (int j1 = 0; j1 < A; ++j1) {
 #pragma unroll
-        for (int q = 0; q < RAD; ++q) {
-            R ar = v[0].x, ai = v[0].y;
+            for (int j2 = 0; j2 < B; ++j2) y[j1][j2] = v[j1 + A * j2];
+            dft_reg<R, B, INV>(y[j1]);
 #pragma unroll
-            for (int r = 1; r < RAD; ++r) {
-                const int k = (r * q) % RAD;
-                const R wr = cs[k], wi = sgn * sn[k];
-                ar += v[r].x * wr - v[r].y * wi;
-                ai += v[r].x * wi + v[r].y * wr;
-            }
-            out[q].x = ar;
-            out[q].y = ai;
+            for (int k2 = 0; k2 < B; ++k2) y[j1][k2] = rot<CT, INV>(y[j1][k2], j1 * k2, P);
         }
 #pragma unroll
-        for (int q = 0; q < RAD; ++q) v[q] = out[q];
+        for (int k2 = 0; k2 < B; ++k2) {
+            CT t[A];
+#pragma unroll
+            for (int j1 = 0; j1 < A; ++j1) t[j1] = y[j1][k2];
+            dft_reg<R, A, INV>(t);
+#pragma unroll
+            for (int k1 = 0; k1 < A; ++k1) v[k2 + B * k1] = t[k1];
+        }
     }
 }
 
-// One Stockham stage with a register codelet of radix RAD, in place on buf.
-template <typename R, int RAD>
-__device__ inline void stage_codelet(typename C2<R>::T* buf, int n, int nl, int Ns, const typename C2<R>::T* tw,
-                                     R sgn) {
+// One Stockham stage of radix P (register DFT), src -> dst (LDS ping-pong).
+template <typename R, int P, bool INV>
+__device__ __forceinline__ void fft_stage(const typename C2<R>::T* src, typename C2<R>::T* dst, int n, int nl,
+                                          int Ns, const typename C2<R>::T* tw) {
     using CT = typename C2<R>::T;
-    constexpr int KMAX = (kLdsElems32 / RAD + 255) / 256;
-    const int nb = n / RAD;
-    const int total = nl * nb;
-    const int step = n / (Ns * RAD);
-    const float rnb = 1.0f / (float)nb, rns = 1.0f / (float)Ns;
-    CT v[KMAX][RAD];
-    int dst[KMAX];
+    const int m = n / P;
+    const int total = nl * m;
+    const int step = n / (Ns * P);
+    const float rm = 1.0f / (float)m, rns = 1.0f / (float)Ns;
+    for (int bf = threadIdx.x; bf < total; bf += blockDim.x) {
+        int li, j, jd, a;
+        fdivmod(bf, m, rm, li, j);
+        fdivmod(j, Ns, rns, jd, a);
+        const CT* sp = src + li * n + j;
+        CT v[P];
 #pragma unroll
-    for (int k = 0; k < KMAX; ++k) {
-        const int bf = threadIdx.x + 256 * k;
-        dst[k] = -1;
-        if (bf < total) {
-            int li, j, jd, a;
-            fdivmod(bf, nb, rnb, li, j);
-            fdivmod(j, Ns, rns, jd, a);
-            const CT* src = buf + li * n + j;
+        for (int r = 0; r < P; ++r) v[r] = sp[r * m];
+        if (a > 0) {
+            const int ta = a * step;
 #pragma unroll
-            for (int r = 0; r < RAD; ++r) {
-                CT x = src[r * nb];
-                if (r > 0 && a > 0) x = cmul_t(x, tw[a * r * step]);
-                v[k][r] = x;
-            }
-            dft_small<R, RAD>(v[k], sgn);
-            dst[k] = li * n + jd * Ns * RAD + a;
+            for (int r = 1; r < P; ++r) v[r] = cmul_t(v[r], tw[ta * r]);
         }
+        dft_reg<R, P, INV>(v);
+        CT* dp = dst + li * n + jd * Ns * P + a;
+#pragma unroll
+        for (int q = 0; q < P; ++q) dp[q * Ns] = v[q];
     }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < KMAX; ++k)
-        if (dst[k] >= 0)
-#pragma unroll
-            for (int q = 0; q < RAD; ++q) buf[dst[k] + q * Ns] = v[k][q];
-    __syncthreads();
 }
 
-// Generic Stockham stage of any radix R.  Pass 1 multiplies every input
-// x[j + r m] by its stage twiddle W_{Ns R}^{(j%Ns) r} in place; pass 2 computes
-// each output as a length-R DFT of the twiddled inputs with the R-th roots held
-// in a compact LDS table (lanes mostly read the same root: broadcasts).
+// Generic Stockham stage of any radix R, src -> dst.  Pass 1 multiplies every
+// input x[j + r m] of src by its stage twiddle W_{Ns R}^{(j%Ns) r} in place;
+// pass 2 computes each output as a length-R DFT of the twiddled inputs with the
+// R-th roots held in a compact LDS table (lanes mostly read the same root).
 template <typename R>
-__device__ inline void stage_generic(typename C2<R>::T* buf, typename C2<R>::T* roots, int n, int nl, int Ns,
-                                     int Rx, const typename C2<R>::T* tw) {
+__device__ inline void stage_generic(typename C2<R>::T* src, typename C2<R>::T* dst, typename C2<R>::T* roots, int n,
+                                     int nl, int Ns, int Rx, const typename C2<R>::T* tw) {
     using CT = typename C2<R>::T;
-    constexpr int KO = kLdsElems32 / 256;
     const int m = n / Rx;
     const int step = n / (Ns * Rx);
     const int tot = nl * n;
@@ -481,67 +598,108 @@ __device__ inline void stage_generic(typename C2<R>::T* buf, typename C2<R>::T* 
         fdivmod(e, n, rn, li, p);
         fdivmod(p, m, rm, r, j);
         fdivmod(j, Ns, rns, jd, a);
-        if (r > 0 && a > 0) buf[e] = cmul_t(buf[e], tw[a * r * step]);
+        if (r > 0 && a > 0) src[e] = cmul_t(src[e], tw[a * r * step]);
     }
     __syncthreads();
-    CT o[KO];
-#pragma unroll
-    for (int k = 0; k < KO; ++k) {
-        const int e = threadIdx.x + 256 * k;
-        if (e < tot) {
-            int li, oi, rest, a, jd, q;
-            fdivmod(e, n, rn, li, oi);
-            fdivmod(oi, Ns, rns, rest, a);
-            fdivmod(rest, Rx, rr, jd, q);
-            const CT* src = buf + li * n + jd * Ns + a;
-            R ar = R(0), ai = R(0);
-            int kq = 0;
-            for (int r = 0; r < Rx; ++r) {
-                const CT x = src[r * m];
-                const CT w = roots[kq];
-                ar += x.x * w.x - x.y * w.y;
-                ai += x.x * w.y + x.y * w.x;
-                kq += q;
-                if (kq >= Rx) kq -= Rx;
-            }
-            o[k].x = ar;
-            o[k].y = ai;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        int li, oi, rest, a, jd, q;
+        fdivmod(e, n, rn, li, oi);
+        fdivmod(oi, Ns, rns, rest, a);
+        fdivmod(rest, Rx, rr, jd, q);
+        const CT* sp = src + li * n + jd * Ns + a;
+        R ar = R(0), ai = R(0);
+        int kq = 0;
+        for (int r = 0; r < Rx; ++r) {
+            const CT x = sp[r * m];
+            const CT w = roots[kq];
+            ar += x.x * w.x - x.y * w.y;
+            ai += x.x * w.y + x.y * w.x;
+            kq += q;
+            if (kq >= Rx) kq -= Rx;
         }
+        CT o;
+        o.x = ar;
+        o.y = ai;
+        dst[e] = o;  // outputs are in natural (li, o) order
     }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < KO; ++k) {
-        const int e = threadIdx.x + 256 * k;
-        if (e < tot) buf[e] = o[k];  // outputs are in natural (li, o) order
-    }
-    __syncthreads();
 }
 
-template <typename R>
+// Radix classes: the kernel is compiled for the radices of its class only, so
+// a plan of small radices is not charged the registers of the large ones.
+//   0: radices up to 8;  1: up to 16;  2: up to 16 and the primes 17, 19, 23.
+template <typename R, bool INV, int CLS>
+__device__ __forceinline__ void run_stage(typename C2<R>::T* src, typename C2<R>::T* dst, typename C2<R>::T* roots,
+                                          int n, int nl, int Ns, int Rx, const typename C2<R>::T* tw) {
+#define MPM_ST(P)                                    \
+    case P:                                          \
+        fft_stage<R, P, INV>(src, dst, n, nl, Ns, tw);    \
+        return;
+#define MPM_ST_C(P, C)                                    \
+    case P:                                               \
+        if constexpr (CLS >= C) {                         \
+            fft_stage<R, P, INV>(src, dst, n, nl, Ns, tw);     \
+            return;                                       \
+        }                                                 \
+        break;
+    switch (Rx) {
+        MPM_ST(2)
+        MPM_ST(3)
+        MPM_ST(4)
+        MPM_ST(5)
+        MPM_ST(6)
+        MPM_ST(7)
+        MPM_ST(8)
+        MPM_ST_C(9, 1)
+        MPM_ST_C(10, 1)
+        MPM_ST_C(11, 1)
+        MPM_ST_C(12, 1)
+        MPM_ST_C(13, 1)
+        MPM_ST_C(14, 1)
+        MPM_ST_C(15, 1)
+        MPM_ST_C(16, 1)
+        MPM_ST_C(17, 2)
+        MPM_ST_C(19, 2)
+        MPM_ST_C(23, 2)
+        default:
+            break;
+    }
+#undef MPM_ST
+#undef MPM_ST_C
+    stage_generic<R>(src, dst, roots, n, nl, Ns, Rx, tw);
+}
+
+template <typename R, bool INV, int CLS>
 __global__ __launch_bounds__(256) void line_fft_kernel(typename C2<R>::T* __restrict__ data, LineGeom lg, LinePlan pl,
-                                                       const typename C2<R>::T* __restrict__ twiddle, int TL,
-                                                       int inverse) {
+                                                       const typename C2<R>::T* __restrict__ twiddle, int TL) {
     using CT = typename C2<R>::T;
     extern __shared__ unsigned char s_raw[];
     const int n = pl.n;
-    CT* buf = reinterpret_cast<CT*>(s_raw);
-    CT* tw = buf + (size_t)TL * n;
+    CT* bufs[2];
+    bufs[0] = reinterpret_cast<CT*>(s_raw);
+    bufs[1] = bufs[0] + (size_t)TL * n;
+    CT* tw = bufs[1] + (size_t)TL * n;
     CT* roots = tw + n;
     const int tid = threadIdx.x;
     const long long l0 = (long long)blockIdx.x * TL;
     const int nl = (int)min((long long)TL, lg.nlines - l0);
-    const R sgn = inverse ? R(1) : R(-1);
     for (int k = tid; k < n; k += blockDim.x) {
         CT w = twiddle[k];
-        if (inverse) w.y = -w.y;
+        if (INV) w.y = -w.y;
         tw[k] = w;
     }
-    __shared__ long long s_lb[64];
-    if (tid < nl) s_lb[tid] = line_base(lg, l0 + tid);
-    __syncthreads();
     const int tot = nl * n;
+    // the lines of a block are one contiguous run when each line is contiguous
+    // and consecutive lines follow each other
+    const bool run = lg.elem_stride == 1 && lg.inner == 1 && lg.outer == n;
+    __shared__ long long s_lb[64];
+    if (!run && tid < nl) s_lb[tid] = line_base(lg, l0 + tid);
+    __syncthreads();
     const float rn = 1.0f / (float)n, rnl = 1.0f / (float)nl;
-    if (lg.elem_stride == 1) {
+    CT* buf = bufs[0];
+    if (run) {
+        const CT* src = data + l0 * n;
+        for (int e = tid; e < tot; e += blockDim.x) buf[e] = src[e];
+    } else if (lg.elem_stride == 1) {
         for (int e = tid; e < tot; e += blockDim.x) {
             int li, k;
             fdivmod(e, n, rn, li, k);
@@ -555,20 +713,19 @@ __global__ __launch_bounds__(256) void line_fft_kernel(typename C2<R>::T* __rest
         }
     }
     __syncthreads();
-    int Ns = 1;
+    int Ns = 1, cur = 0;
     for (int st = 0; st < pl.nstages; ++st) {
         const int Rx = pl.radix[st];
-        switch (Rx) {
-            case 2: stage_codelet<R, 2>(buf, n, nl, Ns, tw, sgn); break;
-            case 3: stage_codelet<R, 3>(buf, n, nl, Ns, tw, sgn); break;
-            case 4: stage_codelet<R, 4>(buf, n, nl, Ns, tw, sgn); break;
-            case 5: stage_codelet<R, 5>(buf, n, nl, Ns, tw, sgn); break;
-            case 7: stage_codelet<R, 7>(buf, n, nl, Ns, tw, sgn); break;
-            default: stage_generic<R>(buf, roots, n, nl, Ns, Rx, tw); break;
-        }
+        run_stage<R, INV, CLS>(bufs[cur], bufs[cur ^ 1], roots, n, nl, Ns, Rx, tw);
+        __syncthreads();
+        cur ^= 1;
         Ns *= Rx;
     }
-    if (lg.elem_stride == 1) {
+    buf = bufs[cur];
+    if (run) {
+        CT* dstp = data + l0 * n;
+        for (int e = tid; e < tot; e += blockDim.x) dstp[e] = buf[e];
+    } else if (lg.elem_stride == 1) {
         for (int e = tid; e < tot; e += blockDim.x) {
             int li, k;
             fdivmod(e, n, rn, li, k);
@@ -583,32 +740,58 @@ __global__ __launch_bounds__(256) void line_fft_kernel(typename C2<R>::T* __rest
     }
 }
 
+// Stage radices for length n: primes above kMaxPrimeRadix stand alone (generic
+// stage), primes from 17 up stand alone, and the small primes are packed,
+// largest first, into the fullest radix that stays <= kMaxCompositeRadix.
+static std::vector<int> plan_radices(int n) {
+    std::vector<int> primes;
+    int m = n;
+    for (int p = 2; p * p <= m; ++p)
+        while (m % p == 0) {
+            primes.push_back(p);
+            m /= p;
+        }
+    if (m > 1) primes.push_back(m);
+    std::sort(primes.begin(), primes.end(), std::greater<int>());
+    std::vector<int> out, groups;
+    for (int p : primes) {
+        if (p > kMaxCompositeRadix) {
+            out.push_back(p);
+            continue;
+        }
+        int best = -1;
+        for (size_t g = 0; g < groups.size(); ++g)
+            if (groups[g] * p <= kMaxCompositeRadix && (best < 0 || groups[g] > groups[best])) best = (int)g;
+        if (best >= 0)
+            groups[best] *= p;
+        else
+            groups.push_back(p);
+    }
+    out.insert(out.end(), groups.begin(), groups.end());
+    return out;
+}
+
 // One axis of the built-in FFT: plan + twiddle tables in both precisions.
 struct LineAxis {
     LinePlan plan{};
+    int cls = 0;  // radix class of the compiled kernel
     DevBuf<double2> tw64;
     DevBuf<float2> tw32;
     explicit LineAxis(int n) {
         plan.n = n;
         plan.nstages = 0;
-        int m = n;
-        auto push = [&](int r) {
+        for (int r : plan_radices(n)) {
             MPM_REQUIRE(plan.nstages < kMaxStages, MPM_ERR_UNSUPPORTED, "FFT length has too many factors");
             plan.radix[plan.nstages++] = r;
-        };
-        while (m % 4 == 0) { push(4); m /= 4; }
-        while (m % 2 == 0) { push(2); m /= 2; }
-        for (int p = 3; p * p <= m; p += 2)
-            while (m % p == 0) { push(p); m /= p; }
-        if (m > 1) push(m);
+            if (r > kMaxPrimeRadix) continue;  // generic stage
+            cls = std::max(cls, r <= 8 ? 0 : (r <= 16 ? 1 : 2));
+        }
         std::vector<double2> t64(n);
         std::vector<float2> t32(n);
         for (int k = 0; k < n; ++k) {
-            double sv, cv;
-            // exp(-2 pi i k / n) with the argument reduced exactly
+            // exp(-2 pi i k / n)
             const double a = -2.0 * kPi * (double)k / (double)n;
-            sv = std::sin(a);
-            cv = std::cos(a);
+            const double sv = std::sin(a), cv = std::cos(a);
             t64[k] = make_double2(cv, sv);
             t32[k] = make_float2((float)cv, (float)sv);
         }
@@ -623,14 +806,18 @@ struct LineFft {
     std::vector<LineGeom> geom_per_rhs;            // nlines counted for one RHS
     long long rhs_elems = 0;                       // elements per RHS (grid size)
 
-    // largest power of two TL with TL * n within the per-block element budget
-    static int lines_per_block(int n, size_t cs) {
-        const int cap = cs == sizeof(float2) ? kLdsElems32 : kLdsElems64;
-        int tl = 1;
-        while (tl * 2 * n <= cap && tl < 64) tl *= 2;
-        return tl;
+    // lines per block: as many as fit the per-block element budget (at most 64)
+    template <typename R>
+    static int lines_per_block(int n) {
+        return std::max(1, std::min(64, fft_elems<R>() / n));
     }
-    static bool fits(int n, size_t cs) { return n <= (cs == sizeof(float2) ? kLdsElems32 : kLdsElems64); }
+    static bool fits(int n) { return n <= fft_elems<double>(); }
+
+    template <typename R, bool INV, int CLS>
+    static void launch(dim3 grid, size_t lds, hipStream_t s, typename C2<R>::T* data, const LineGeom& g,
+                       const LinePlan& pl, const typename C2<R>::T* tw, int TL) {
+        hipLaunchKernelGGL((line_fft_kernel<R, INV, CLS>), grid, dim3(256), lds, s, data, g, pl, tw, TL);
+    }
 
     template <typename R>
     void exec(typename C2<R>::T* grid, int B, bool forward, hipStream_t s) {
@@ -641,13 +828,21 @@ struct LineFft {
             LineGeom g = geom_per_rhs[a];
             g.nlines *= B;
             const int n = ax.plan.n;
-            const int TL = lines_per_block(n, sizeof(CT));
-            const size_t lds = ((size_t)TL * n + 2 * n) * sizeof(CT);
+            const int TL = lines_per_block<R>(n);
+            const size_t lds = ((size_t)2 * TL * n + 2 * n) * sizeof(CT);
             const long long blocks = (g.nlines + TL - 1) / TL;
             const CT* tw = std::is_same<R, float>::value ? reinterpret_cast<const CT*>(ax.tw32.p)
                                                          : reinterpret_cast<const CT*>(ax.tw64.p);
-            hipLaunchKernelGGL(line_fft_kernel<R>, dim3((unsigned)blocks), dim3(256), lds, s, grid, g, ax.plan, tw, TL,
-                               forward ? 0 : 1);
+            const dim3 gd((unsigned)blocks);
+            if (forward) {
+                if (ax.cls == 0) launch<R, false, 0>(gd, lds, s, grid, g, ax.plan, tw, TL);
+                else if (ax.cls == 1) launch<R, false, 1>(gd, lds, s, grid, g, ax.plan, tw, TL);
+                else launch<R, false, 2>(gd, lds, s, grid, g, ax.plan, tw, TL);
+            } else {
+                if (ax.cls == 0) launch<R, true, 0>(gd, lds, s, grid, g, ax.plan, tw, TL);
+                else if (ax.cls == 1) launch<R, true, 1>(gd, lds, s, grid, g, ax.plan, tw, TL);
+                else launch<R, true, 2>(gd, lds, s, grid, g, ax.plan, tw, TL);
+            }
             MPM_HIP(hipGetLastError());
         }
     }
@@ -975,6 +1170,267 @@ __global__ __launch_bounds__(256) void scale_kernel(typename C2<R>::T* __restric
     }
 }
 
+// ---- spread on the matrix cores (slab layout, FP32) ----
+//
+// For an xy-region of RX x 16 voxels over all Zs slab planes and 16 real
+// columns q = 6 b + 2 c + (0: real, 1: imaginary) of the right-hand sides,
+//   grid^T[q][voxel] = sum_k V[k][q] W[k][voxel],
+//   W[k][(x, y, plane)] = wx_k(x) wy_k(y) wz_k(plane)
+// over the candidate particles k of the region (every particle whose stencil
+// footprint meets it, in ascending id order), one v_mfma_f32_16x16x4_f32 per
+// four candidates and (x, plane) pair: A = V^T (16 columns x 4 candidates),
+// B = W^T (4 candidates x 16 consecutive y).  The region-local separable weights
+// (stencil folding on small grids included) are staged per candidate chunk in
+// LDS, so the products cost three LDS reads and two multiplies per lane.  Every
+// voxel of the slab is written exactly once, without atomics; the sum order is
+// fixed (deterministic).  A block is four waves = 64 real columns of one region.
+constexpr int kMfmaRY = 16;    // y voxels per region (the MFMA N dimension)
+constexpr int kMfmaKC = 64;    // candidates staged per chunk
+
+typedef float mpm_f32x4 __attribute__((ext_vector_type(4)));
+
+template <int RX, int ZSB>
+__global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, SpreadGeom g,
+                                                          const int* __restrict__ zlist, const int* __restrict__ base,
+                                                          const float* __restrict__ wts,
+                                                          const int* __restrict__ reg_start,
+                                                          const int* __restrict__ reg_items, int nry, int N,
+                                                          float2* __restrict__ out) {
+    constexpr int RY = kMfmaRY, KC = kMfmaKC;
+    constexpr int NW = RY + RX + ZSB;  // region-local weights per candidate
+    __shared__ float s_v[KC][64 + 1];
+    __shared__ float s_w[KC][NW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int reg = blockIdx.x;
+    const int x0 = (reg / nry) * RX, y0 = (reg % nry) * RY;
+    const int col_base = blockIdx.y * 64;
+    const int wcol0 = col_base + wave * 16;
+    const int ncols = 6 * B;
+    const int Zs = g.Zs, width = g.width, half = g.half;
+    const int k0 = reg_start[reg], k1 = reg_start[reg + 1];
+
+    mpm_f32x4 acc[RX][ZSB];
+#pragma unroll
+    for (int xi = 0; xi < RX; ++xi)
+#pragma unroll
+        for (int p = 0; p < ZSB; ++p) acc[xi][p] = mpm_f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int c0 = k0; c0 < k1; c0 += KC) {
+        const int cnt = min(KC, k1 - c0);
+        __syncthreads();
+        // values: s_v[k][col] = component (b, c), real or imaginary part, of candidate k
+        for (int e = tid; e < KC * 64; e += 256) {
+            const int q = e >> 6, col = e & 63;
+            const int qc = col_base + col;
+            float v = 0.f;
+            if (q < cnt && qc < ncols) {
+                const int pid = reg_items[c0 + q];
+                const int b = qc / 6, c = (qc >> 1) % 3;
+                const double2 dv = tab.x[b][3 * (size_t)pid + c];
+                v = (float)((qc & 1) ? dv.y : dv.x);
+            }
+            s_v[q][col] = v;
+        }
+        // region-local weights: y0 + j (j < RY), x0 + j - RY, slab plane j - RY - RX;
+        // the sum over stencil points covers stencils that fold onto themselves
+        for (int e = tid; e < KC * NW; e += 256) {
+            const int q = e / NW, j = e % NW;
+            float w = 0.f;
+            if (q < cnt) {
+                const int pid = reg_items[c0 + q];
+                int d, v = -1, n = 1;
+                if (j < RY) {
+                    d = 1;
+                    v = y0 + j;
+                    n = g.ny;
+                    if (v >= n) v = -1;
+                } else if (j < RY + RX) {
+                    d = 0;
+                    v = x0 + j - RY;
+                    n = g.nx;
+                    if (v >= n) v = -1;
+                } else {
+                    d = 2;
+                    const int p = j - RY - RX;
+                    n = g.nz;
+                    v = p < Zs ? zlist[p] : -1;
+                }
+                if (v >= 0) {
+                    const int b0 = base[(size_t)d * N + pid] - half;
+                    const float* wp = wts + ((size_t)pid * 3 + d) * width;
+                    int off = (v - b0) % n;  // stencil index of the first point on voxel v
+                    if (off < 0) off += n;
+                    for (int o = off; o < width; o += n) w += wp[o];
+                }
+            }
+            s_w[q][j] = w;
+        }
+        __syncthreads();
+        if (wcol0 < ncols) {
+            const int i = lane & 15;
+            for (int ks = 0; ks < cnt; ks += 4) {
+                const int kq = ks + (lane >> 4);  // rows >= cnt hold zeros
+                const float a = s_v[kq][wave * 16 + i];
+                const float wy = s_w[kq][i];
+                float wx[RX], wz[ZSB];
+#pragma unroll
+                for (int xi = 0; xi < RX; ++xi) wx[xi] = s_w[kq][RY + xi] * wy;
+#pragma unroll
+                for (int p = 0; p < ZSB; ++p) wz[p] = s_w[kq][RY + RX + p];
+#pragma unroll
+                for (int p = 0; p < ZSB; ++p) {
+                    if (p < Zs) {
+#pragma unroll
+                        for (int xi = 0; xi < RX; ++xi)
+                            acc[xi][p] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, wx[xi] * wz[p], acc[xi][p], 0, 0, 0);
+                    }
+                }
+            }
+        }
+    }
+    // lane holds D[4 (lane >> 4) + r][lane & 15]: real columns wcol0 + 4 (lane >> 4) + r
+    // (r = 0, 1: one complex column, r = 2, 3: the next) at y = y0 + (lane & 15)
+    const int y = y0 + (lane & 15);
+    if (y >= g.ny) return;
+    const size_t nxy = (size_t)g.nx * g.ny;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int qc = wcol0 + 4 * (lane >> 4) + 2 * h;
+        if (qc >= ncols) continue;
+        const int b = qc / 6, c = (qc >> 1) % 3;
+        float2* o = out + ((size_t)b * 3 + c) * Zs * nxy + y;
+#pragma unroll
+        for (int xi = 0; xi < RX; ++xi) {
+            const int x = x0 + xi;
+            if (x >= g.nx) continue;
+#pragma unroll
+            for (int p = 0; p < ZSB; ++p)
+                if (p < Zs) o[p * nxy + (size_t)x * g.ny] = make_float2(acc[xi][p][2 * h], acc[xi][p][2 * h + 1]);
+        }
+    }
+}
+
+// ---- contraction on the matrix cores (slab layout, FP32) ----
+//
+// The transpose of spread_mfma_kernel over the same regions: for the candidate
+// particles k of a region and 16 real columns q,
+//   D[k][q] = sum_{voxels v of the region} W[k][v] G[v][q],
+// one v_mfma_f32_16x16x4_f32 per four voxels and 16 candidates (A = W, B = G).
+// The region's grid values of one slab plane at a time are staged in LDS with
+// coalesced reads (every voxel of the slab is read exactly once per column
+// block), and D is written to the particle's slot of the region (the CSR
+// position of the pair); the per-particle sum over its ~(9/RX + 1)(9/16 + 1)
+// regions happens in gather_kernel, in a fixed order.
+template <int RX, int ZSB>
+__global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g, const int* __restrict__ zlist,
+                                                            const int* __restrict__ base,
+                                                            const float* __restrict__ wts,
+                                                            const int* __restrict__ reg_start,
+                                                            const int* __restrict__ reg_items, int nry, int N,
+                                                            const float2* __restrict__ grid,
+                                                            float* __restrict__ partial) {
+    constexpr int RY = kMfmaRY, KC = kMfmaKC;
+    constexpr int NV = RX * RY;        // voxels of the region in one plane
+    constexpr int NW = RY + RX + ZSB;  // region-local weights per candidate
+    __shared__ float s_g[NV][64 + 1];
+    __shared__ float s_w[KC][NW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int reg = blockIdx.x;
+    const int x0 = (reg / nry) * RX, y0 = (reg % nry) * RY;
+    const int col_base = blockIdx.y * 64;
+    const int wcol0 = col_base + wave * 16;
+    const int ncols = 6 * B;
+    const int Zs = g.Zs, width = g.width, half = g.half;
+    const size_t nxy = (size_t)g.nx * g.ny;
+    const int k0 = reg_start[reg], k1 = reg_start[reg + 1];
+
+    for (int c0 = k0; c0 < k1; c0 += KC) {
+        const int cnt = min(KC, k1 - c0);
+        const int nblk = (cnt + 15) >> 4;
+        __syncthreads();
+        for (int e = tid; e < KC * NW; e += 256) {
+            const int q = e / NW, j = e % NW;
+            float w = 0.f;
+            if (q < cnt) {
+                const int pid = reg_items[c0 + q];
+                int d, v = -1, n = 1;
+                if (j < RY) {
+                    d = 1;
+                    v = y0 + j;
+                    n = g.ny;
+                    if (v >= n) v = -1;
+                } else if (j < RY + RX) {
+                    d = 0;
+                    v = x0 + j - RY;
+                    n = g.nx;
+                    if (v >= n) v = -1;
+                } else {
+                    d = 2;
+                    const int p = j - RY - RX;
+                    n = g.nz;
+                    v = p < Zs ? zlist[p] : -1;
+                }
+                if (v >= 0) {
+                    const int b0 = base[(size_t)d * N + pid] - half;
+                    const float* wp = wts + ((size_t)pid * 3 + d) * width;
+                    int off = (v - b0) % n;
+                    if (off < 0) off += n;
+                    for (int o = off; o < width; o += n) w += wp[o];
+                }
+            }
+            s_w[q][j] = w;
+        }
+        mpm_f32x4 acc[KC / 16];
+#pragma unroll
+        for (int nb = 0; nb < KC / 16; ++nb) acc[nb] = mpm_f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int p = 0; p < Zs; ++p) {
+            __syncthreads();
+            // s_g[xi * 16 + yi][col]: consecutive threads read consecutive y of one column pair
+            for (int e = tid; e < 32 * NV; e += 256) {
+                const int yi = e % RY, xi = (e / RY) % RX, cp = e / NV;
+                const int x = x0 + xi, y = y0 + yi, qc = col_base + 2 * cp;
+                float2 v = make_float2(0.f, 0.f);
+                if (x < g.nx && y < g.ny && qc < ncols) {
+                    const int b = qc / 6, c = (qc >> 1) % 3;
+                    v = grid[(((size_t)b * 3 + c) * Zs + p) * nxy + (size_t)x * g.ny + y];
+                }
+                s_g[xi * RY + yi][2 * cp] = v.x;
+                s_g[xi * RY + yi][2 * cp + 1] = v.y;
+            }
+            __syncthreads();
+            if (wcol0 < ncols) {
+                float wzp[KC / 16];
+#pragma unroll
+                for (int nb = 0; nb < KC / 16; ++nb) wzp[nb] = s_w[nb * 16 + (lane & 15)][RY + RX + p];
+                for (int ks = 0; ks < NV; ks += 4) {
+                    const int vox = ks + (lane >> 4);
+                    const int xi = vox / RY, yi = vox % RY;
+                    const float bval = s_g[vox][wave * 16 + (lane & 15)];
+#pragma unroll
+                    for (int nb = 0; nb < KC / 16; ++nb) {
+                        if (nb < nblk) {
+                            const int k = nb * 16 + (lane & 15);
+                            const float a = s_w[k][yi] * s_w[k][RY + xi] * wzp[nb];
+                            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bval, acc[nb], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+        }
+        // lane holds D[16 nb + 4 (lane >> 4) + r][lane & 15]
+        const int qc = wcol0 + (lane & 15);
+        if (qc < ncols) {
+#pragma unroll
+            for (int nb = 0; nb < KC / 16; ++nb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int k = nb * 16 + 4 * (lane >> 4) + r;
+                    if (k < cnt) partial[(size_t)(c0 + k) * ncols + qc] = acc[nb][r];
+                }
+        }
+    }
+}
+
 // Slab: for every in-plane wave vector (kx, ky) — one column of the slab after
 // the 2-D transforms — evaluate the z-transform from the Zs occupied planes to
 // all nz wave numbers, apply the Green's tensor projection, and transform back to
@@ -1073,13 +1529,136 @@ __global__ void kspace_slab_kernel(double* __restrict__ k64, float* __restrict__
     }
 }
 
+// Slab with cyclically contiguous planes z_a = z_0 + a (mod nz): the z-transform,
+// Green's tensor and inverse z-transform of one (kx, ky) column collapse into a
+// Toeplitz coupling of the planes,
+//   g[a] = sum_b T_{a-b} f[b],  T_d = sum_kz exp(+2 pi i kz d / nz) sc k^ k^T,
+// a complex symmetric 3x3 per (d, column) with T_{-d} = conj(T_d) (the projector
+// sc k^ k^T is real).  Table layout [d][6][column], d = 0..Zs-1, components
+// (xx, yy, zz, xy, xz, yz), built in FP64 once per grid / Bloch vector / slab size.
+__global__ void ztoeplitz_table_kernel(double2* __restrict__ t64, float2* __restrict__ t32, KParams q, int Zs) {
+    const int ncols = q.nx * q.ny;
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int d = blockIdx.y;
+    if (col >= ncols || d >= Zs) return;
+    double tr[6] = {0, 0, 0, 0, 0, 0}, ti[6] = {0, 0, 0, 0, 0, 0};
+    for (int kz = 0; kz < q.nz; ++kz) {
+        double o[4];
+        kcoef(q, col / q.ny, col % q.ny, kz, o);
+        if (o[3] == 0.0) continue;
+        double sv, cv;
+        sincospi(2.0 * (double)(((long long)kz * d) % q.nz) / (double)q.nz, &sv, &cv);
+        const double m[6] = {o[0] * o[0], o[1] * o[1], o[2] * o[2], o[0] * o[1], o[0] * o[2], o[1] * o[2]};
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            tr[c] += o[3] * m[c] * cv;
+            ti[c] += o[3] * m[c] * sv;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        const size_t e = ((size_t)d * 6 + c) * ncols + col;
+        t64[e] = make_double2(tr[c], ti[c]);
+        t32[e] = make_float2((float)tr[c], (float)ti[c]);
+    }
+}
+
+// Output planes per accumulation pass of ztoeplitz_kernel: all of them in FP32;
+// in FP64 as many as keep f (6 ZS doubles), the accumulators and one table line
+// within the 256-register budget.
+template <typename R, int ZS>
+constexpr int ztoe_as() {
+    return sizeof(R) == 4 ? (ZS <= 12 ? ZS : 8) : (ZS <= 10 ? (ZS + 1) / 2 : (ZS <= 12 ? 2 : 1));
+}
+
+// Applies the Toeplitz plane coupling in place on the slab [B][3][Zs][column].
+// One thread per (column, right-hand side); a block covers (256 / rb) columns x rb
+// right-hand sides so the rb threads of a column share the table lines in cache.
+// ZS >= Zs is the compile-time plane bound (planes Zs..ZS-1 are skipped), AS the
+// number of output planes accumulated per pass (register budget).
+template <typename R, int ZS, int AS>
+__global__ __launch_bounds__(256) void ztoeplitz_kernel(typename C2<R>::T* __restrict__ slab, int B, int ncols,
+                                                        int Zs, int rb, const typename C2<R>::T* __restrict__ T) {
+    using CT = typename C2<R>::T;
+    const int cb = 256 / rb;
+    const int tc = threadIdx.x % cb, tb = threadIdx.x / cb;
+    const int col = blockIdx.x * cb + tc;
+    const int b = blockIdx.y * rb + tb;
+    if (col >= ncols || b >= B) return;
+    CT* base = slab + (size_t)b * 3 * Zs * ncols + col;
+    CT f[3][ZS];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int j = 0; j < ZS; ++j) {
+            if (j < Zs) {
+                f[c][j] = base[((size_t)c * Zs + j) * ncols];
+            } else {
+                f[c][j].x = f[c][j].y = R(0);
+            }
+        }
+#pragma unroll
+    for (int a0 = 0; a0 < ZS; a0 += AS) {
+        if (a0 >= Zs) continue;
+        R ar[3][AS], ai[3][AS];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int k = 0; k < AS; ++k) ar[c][k] = ai[c][k] = R(0);
+#pragma unroll
+        for (int dd = -(ZS - 1); dd <= ZS - 1; ++dd) {
+            const int ad = dd < 0 ? -dd : dd;
+            // pairs (a, a - dd) with a in [a0, a0 + AS) and both planes inside [0, ZS)
+            const int alo = a0 > dd ? a0 : dd;                        // a - dd >= 0
+            const int ahi = (a0 + AS < ZS + dd ? a0 + AS : ZS + dd);  // a - dd < ZS, a < a0 + AS
+            if (alo >= ahi || ad >= Zs) continue;
+            const CT* tp = T + (size_t)ad * 6 * ncols + col;
+            R tr[6], ti[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                const CT v = tp[(size_t)c * ncols];
+                tr[c] = v.x;
+                ti[c] = dd < 0 ? -v.y : v.y;
+            }
+#pragma unroll
+            for (int k = 0; k < AS; ++k) {
+                const int a = a0 + k;
+                const int bb = a - dd;
+                if (a < alo || a >= ahi) continue;
+                if (a >= Zs || bb >= Zs) continue;
+                const CT fx = f[0][bb], fy = f[1][bb], fz = f[2][bb];
+                // row x: (xx, xy, xz); row y: (xy, yy, yz); row z: (xz, yz, zz)
+                ar[0][k] += tr[0] * fx.x - ti[0] * fx.y + tr[3] * fy.x - ti[3] * fy.y + tr[4] * fz.x - ti[4] * fz.y;
+                ai[0][k] += tr[0] * fx.y + ti[0] * fx.x + tr[3] * fy.y + ti[3] * fy.x + tr[4] * fz.y + ti[4] * fz.x;
+                ar[1][k] += tr[3] * fx.x - ti[3] * fx.y + tr[1] * fy.x - ti[1] * fy.y + tr[5] * fz.x - ti[5] * fz.y;
+                ai[1][k] += tr[3] * fx.y + ti[3] * fx.x + tr[1] * fy.y + ti[1] * fy.x + tr[5] * fz.y + ti[5] * fz.x;
+                ar[2][k] += tr[4] * fx.x - ti[4] * fx.y + tr[5] * fy.x - ti[5] * fy.y + tr[2] * fz.x - ti[2] * fz.y;
+                ai[2][k] += tr[4] * fx.y + ti[4] * fx.x + tr[5] * fy.y + ti[5] * fy.x + tr[2] * fz.y + ti[2] * fz.x;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < AS; ++k) {
+            const int a = a0 + k;
+            if (a >= Zs) continue;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                CT o;
+                o.x = ar[c][k];
+                o.y = ai[c][k];
+                base[((size_t)c * Zs + a) * ncols] = o;
+            }
+        }
+    }
+}
+
 __device__ inline double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
 
-// One wavefront per particle: Gaussian contraction of the (inverse-transformed)
+// One wavefront per (particle, group of G right-hand sides; blockIdx.y strides the
+// groups): Gaussian contraction of the (inverse-transformed)
 // grid, real-space gather over the symmetric neighbour list, self term.
 template <typename R, int G, bool SLAB>
 __global__ __launch_bounds__(256) void gather_kernel(
@@ -1087,7 +1666,8 @@ __global__ __launch_bounds__(256) void gather_kernel(
     const int* __restrict__ zmap, const int* __restrict__ base, const R* __restrict__ wts, int N, double h3,
     const double* __restrict__ pos, const int* __restrict__ nb_start, const int* __restrict__ nb_j,
     const double2* __restrict__ nb_f, double Lx, double Ly, double Lz, double kbx, double kby,
-    const double2* __restrict__ selfc, double self_perp, int jacobi) {
+    const double2* __restrict__ selfc, double self_perp, int jacobi, const float* __restrict__ partial,
+    const int* __restrict__ ps_start, const int* __restrict__ ps_items) {
     __shared__ int s_off[kMaxWidth * kMaxWidth * kMaxWidth];
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -1125,7 +1705,7 @@ __global__ __launch_bounds__(256) void gather_kernel(
     const double xi_ = pos[i], yi_ = pos[N + i], zi_ = pos[2 * N + i];
     const bool bloch = (kbx != 0.0 || kby != 0.0);
 
-    for (int g0 = 0; g0 < B; g0 += G) {
+    for (int g0 = blockIdx.y * G; g0 < B; g0 += G * gridDim.y) {
         double acc[G][3][2];
 #pragma unroll
         for (int j = 0; j < G; ++j)
@@ -1138,7 +1718,22 @@ __global__ __launch_bounds__(256) void gather_kernel(
             for (int j = 0; j < G; ++j)
 #pragma unroll
                 for (int c = 0; c < 3; ++c) racc[j][c][0] = racc[j][c][1] = R(0);
-            for (int t = lane; t < S; t += 64) {
+            // contraction already done per region on the matrix cores: sum the
+            // particle's region slots (ascending region order)
+            const int s1 = partial ? ps_start[i + 1] : 0;
+            for (int t = (partial ? ps_start[i] : 0) + lane; t < s1; t += 64) {
+                const float* pp = partial + (size_t)ps_items[t] * (6 * B);
+#pragma unroll
+                for (int j = 0; j < G; ++j) {
+                    if (g0 + j >= B) break;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        racc[j][c][0] += (R)pp[6 * (g0 + j) + 2 * c];
+                        racc[j][c][1] += (R)pp[6 * (g0 + j) + 2 * c + 1];
+                    }
+                }
+            }
+            for (int t = lane; t < (partial ? 0 : S); t += 64) {
                 const int pk = s_off[t];
                 const int ox = pk & 255, oy = (pk >> 8) & 255, oz = pk >> 16;
                 int vx = sx + ox, vy = sy + oy, vz = sz + oz;
@@ -1297,20 +1892,29 @@ void EwaldOp::build_kspace() {
 }
 
 void EwaldOp::ensure_kspace_table() {
-    const int layout = slab ? 1 : 0;
-    if (kspace_ready && kspace_layout == layout) return;
-    d_k64.resize(4 * V);
-    d_k32.resize(4 * V);
+    const int key = !slab ? 0 : (zcontig ? 2 + Zs : 1);
+    if (kspace_ready && kspace_key == key) return;
     const int threads = 256;
-    const int blocks = (int)std::min<size_t>((V + threads - 1) / threads, 65535);
-    if (slab)
-        hipLaunchKernelGGL(kspace_slab_kernel, dim3(blocks), dim3(threads), 0, nullptr, d_k64.p, d_k32.p, kparams());
-    else
-        hipLaunchKernelGGL(kspace_kernel, dim3(blocks), dim3(threads), 0, nullptr, d_k64.p, d_k32.p, kparams());
+    if (slab && zcontig) {
+        const size_t ncols = (size_t)S.n[0] * S.n[1];
+        d_toe64.resize((size_t)Zs * 6 * ncols);
+        d_toe32.resize((size_t)Zs * 6 * ncols);
+        hipLaunchKernelGGL(ztoeplitz_table_kernel, dim3((unsigned)((ncols + threads - 1) / threads), Zs),
+                           dim3(threads), 0, nullptr, d_toe64.p, d_toe32.p, kparams(), Zs);
+    } else {
+        d_k64.resize(4 * V);
+        d_k32.resize(4 * V);
+        const int blocks = (int)std::min<size_t>((V + threads - 1) / threads, 65535);
+        if (slab)
+            hipLaunchKernelGGL(kspace_slab_kernel, dim3(blocks), dim3(threads), 0, nullptr, d_k64.p, d_k32.p,
+                               kparams());
+        else
+            hipLaunchKernelGGL(kspace_kernel, dim3(blocks), dim3(threads), 0, nullptr, d_k64.p, d_k32.p, kparams());
+    }
     MPM_HIP(hipGetLastError());
     MPM_HIP(hipDeviceSynchronize());
     kspace_ready = true;
-    kspace_layout = layout;
+    kspace_key = key;
 }
 
 void EwaldOp::set_bloch(double kx, double ky) {
@@ -1374,11 +1978,24 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     std::vector<int> zl;
     for (int z = 0; z < nz; ++z)
         if (occ[z]) zl.push_back(z);
+    // one cyclic run of occupied planes: order the slab from the run's first plane
+    // so that slab plane a sits at z0 + a (mod nz)
+    int runs = 0, z0 = 0;
+    for (int z = 0; z < nz; ++z)
+        if (occ[z] && !occ[(z + nz - 1) % nz]) {
+            ++runs;
+            z0 = z;
+        }
+    if (runs == 1) {
+        const int nzl = (int)zl.size();
+        for (int a = 0; a < nzl; ++a) zl[a] = (z0 + a) % nz;
+    }
     const int zsb = zs_bucket((int)zl.size());
     const bool want_slab = force_full == 0 && zsb > 0 && (int)zl.size() * 2 <= nz &&
                            zfused_lds(zsb, nz, sizeof(double)) <= 64 * 1024;
     slab = want_slab;
     Zs = slab ? (int)zl.size() : 0;
+    zcontig = slab && runs == 1;
     zlist = zl;
     std::vector<int> zmap(nz, -1);
     for (size_t j = 0; j < zl.size(); ++j) zmap[zl[j]] = (int)j;
@@ -1393,6 +2010,51 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     for (int d = 0; d < 3; ++d) {
         nbins[d] = (S.n[d] + binsz[d] - 1) / binsz[d];
         ntiles[d] = (S.n[d] + tile[d] - 1) / tile[d];
+    }
+    // regions of the matrix-core spread: every particle joins each region its
+    // xy stencil footprint meets
+    reg_rx = 0;
+    if (slab) {
+        reg_rx = zs_bucket(Zs) <= 10 ? 4 : 2;
+        const int RX = reg_rx, RY = 16;
+        reg_nrx = (S.n[0] + RX - 1) / RX;
+        reg_nry = (S.n[1] + RY - 1) / RY;
+        const int nreg = reg_nrx * reg_nry;
+        std::vector<std::vector<int>> lists(nreg);
+        std::vector<int> rxs, rys;
+        for (int i = 0; i < Nn; ++i) {
+            auto regions_of = [&](int d, int R, std::vector<int>& outv) {
+                outv.clear();
+                const int nd = S.n[d];
+                for (int o = -half; o <= half; ++o) {
+                    const int v = (((base[(size_t)d * Nn + i] + o) % nd) + nd) % nd;
+                    const int r = v / R;
+                    if (std::find(outv.begin(), outv.end(), r) == outv.end()) outv.push_back(r);
+                }
+            };
+            regions_of(0, RX, rxs);
+            regions_of(1, RY, rys);
+            for (int rx : rxs)
+                for (int ry : rys) lists[(size_t)rx * reg_nry + ry].push_back(i);
+        }
+        std::vector<int> rs(nreg + 1, 0), ri;
+        for (int r = 0; r < nreg; ++r) rs[r + 1] = rs[r] + (int)lists[r].size();
+        ri.reserve(rs[nreg] + 1);
+        for (int r = 0; r < nreg; ++r) ri.insert(ri.end(), lists[r].begin(), lists[r].end());
+        // slots of each particle, in ascending region order
+        std::vector<int> pss(Nn + 1, 0), psi(std::max<size_t>(ri.size(), 1));
+        for (size_t q = 0; q < (size_t)rs[nreg]; ++q) pss[ri[q] + 1]++;
+        for (int i = 0; i < Nn; ++i) pss[i + 1] += pss[i];
+        {
+            std::vector<int> fill(pss.begin(), pss.end() - 1);
+            for (int q = 0; q < rs[nreg]; ++q) psi[fill[ri[q]]++] = q;
+        }
+        if (ri.empty()) ri.push_back(0);
+        d_reg_start.upload(rs.data(), rs.size());
+        d_reg_items.upload(ri.data(), ri.size());
+        d_ps_start.upload(pss.data(), pss.size());
+        d_ps_items.upload(psi.data(), psi.size());
+        d_partial.resize((size_t)std::max(rs[nreg], 1) * 6 * kMaxSlots);
     }
     std::vector<int> ztl;  // z-tiles that hold at least one computed voxel
     for (int t = 0; t < ntiles[2]; ++t) {
@@ -1491,7 +2153,7 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         auto lf = std::make_unique<LineFft>();
         bool ok = true;
         auto add = [&](int n, LineGeom g) {
-            ok = ok && LineFft::fits(n, sizeof(double2));
+            ok = ok && LineFft::fits(n);
             if (!ok) return;
             lf->axes.push_back(std::make_unique<LineAxis>(n));
             lf->geom_per_rhs.push_back(g);
@@ -1558,7 +2220,30 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
     g.Zs = Zs;
     const int Ni = (int)N;
     if (parts & MPM_PART_RECIP) {
-        {
+        if (SLAB && fp32 && reg_rx > 0) {
+            ProfScope ps("spread", s);
+            const dim3 gs((unsigned)(reg_nrx * reg_nry), (unsigned)((6 * B + 63) / 64));
+            const float* w32 = d_w32.p;
+            float2* g32 = reinterpret_cast<float2*>(grid);
+#define MPM_SM(RXV, ZSV)                                                                                       \
+    case ZSV:                                                                                                  \
+        hipLaunchKernelGGL((spread_mfma_kernel<RXV, ZSV>), gs, dim3(256), 0, s, t, B, g, d_zlist.p, d_base.p, \
+                           w32, d_reg_start.p, d_reg_items.p, reg_nry, Ni, g32);                              \
+        break;
+            switch (zs_bucket(Zs)) {
+                MPM_SM(4, 4)
+                MPM_SM(4, 6)
+                MPM_SM(4, 8)
+                MPM_SM(4, 10)
+                MPM_SM(2, 12)
+                MPM_SM(2, 14)
+                MPM_SM(2, 16)
+                default:
+                    throw Error(MPM_ERR_RUNTIME, "slab size out of range");
+            }
+#undef MPM_SM
+            MPM_HIP(hipGetLastError());
+        } else {
             ProfScope ps("spread", s);
             const int nt = ntiles[0] * ntiles[1] * n_ztiles;
             if (B == 1)
@@ -1577,7 +2262,33 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
             else
                 fft->exec(grid, bytes_per_rhs, B, fp32, true, s);
         }
-        if (SLAB) {
+        if (SLAB && zcontig) {
+            ProfScope ps("zfused", s);
+            const int ncols = S.n[0] * S.n[1];
+            const CT* T = reinterpret_cast<const CT*>(fp32 ? static_cast<const void*>(d_toe32.p)
+                                                           : static_cast<const void*>(d_toe64.p));
+            const int rb = B >= 16 ? 16 : (B >= 4 ? 4 : 1);
+            const int cb = 256 / rb;
+            const dim3 gridz((ncols + cb - 1) / cb, (B + rb - 1) / rb);
+#define MPM_ZT(ZSV)                                                                                           \
+    case ZSV:                                                                                                 \
+        hipLaunchKernelGGL((ztoeplitz_kernel<R, ZSV, ztoe_as<R, ZSV>()>), gridz, dim3(256), 0, s, grid, B, \
+                           ncols, Zs, rb, T);                                                                 \
+        break;
+            switch (zs_bucket(Zs)) {
+                MPM_ZT(4)
+                MPM_ZT(6)
+                MPM_ZT(8)
+                MPM_ZT(10)
+                MPM_ZT(12)
+                MPM_ZT(14)
+                MPM_ZT(16)
+                default:
+                    throw Error(MPM_ERR_RUNTIME, "slab size out of range");
+            }
+#undef MPM_ZT
+            MPM_HIP(hipGetLastError());
+        } else if (SLAB) {
             ProfScope ps("zfused", s);
             const int ncols = S.n[0] * S.n[1];
             const int zsb = zs_bucket(Zs);
@@ -1626,18 +2337,45 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
                 fft->exec(grid, bytes_per_rhs, B, fp32, false, s);
         }
     }
+    const float* part = nullptr;
+    if (SLAB && fp32 && reg_rx > 0 && (parts & MPM_PART_RECIP)) {
+        ProfScope ps("gather", s);
+        const dim3 gs((unsigned)(reg_nrx * reg_nry), (unsigned)((6 * B + 63) / 64));
+        const float* w32 = d_w32.p;
+        const float2* g32 = reinterpret_cast<const float2*>(grid);
+#define MPM_CM(RXV, ZSV)                                                                                          \
+    case ZSV:                                                                                                     \
+        hipLaunchKernelGGL((contract_mfma_kernel<RXV, ZSV>), gs, dim3(256), 0, s, B, g, d_zlist.p, d_base.p, w32, \
+                           d_reg_start.p, d_reg_items.p, reg_nry, Ni, g32, d_partial.p);                         \
+        break;
+        switch (zs_bucket(Zs)) {
+            MPM_CM(4, 4)
+            MPM_CM(4, 6)
+            MPM_CM(4, 8)
+            MPM_CM(4, 10)
+            MPM_CM(2, 12)
+            MPM_CM(2, 14)
+            MPM_CM(2, 16)
+            default:
+                throw Error(MPM_ERR_RUNTIME, "slab size out of range");
+        }
+#undef MPM_CM
+        MPM_HIP(hipGetLastError());
+        part = d_partial.p;
+    }
     {
         ProfScope ps("gather", s);
         const double h3 = S.h[0] * S.h[1] * S.h[2];
         const int blocks = (Ni + 3) / 4;
         if (B == 1)
-            hipLaunchKernelGGL((gather_kernel<R, 1, SLAB>), dim3(blocks), dim3(256), 0, s, t, B, parts, grid, g,
+            hipLaunchKernelGGL((gather_kernel<R, 1, SLAB>), dim3(blocks, 1), dim3(256), 0, s, t, B, parts, grid, g,
                                d_zmap.p, d_base.p, w, Ni, h3, d_pos.p, d_nb_start.p, d_nb_j.p, d_nb_f.p, L[0], L[1],
-                               L[2], kb[0], kb[1], d_selfc.p, self_perp, jacobi);
+                               L[2], kb[0], kb[1], d_selfc.p, self_perp, jacobi, part, d_ps_start.p, d_ps_items.p);
         else
-            hipLaunchKernelGGL((gather_kernel<R, 4, SLAB>), dim3(blocks), dim3(256), 0, s, t, B, parts, grid, g,
+            hipLaunchKernelGGL((gather_kernel<R, 4, SLAB>), dim3(blocks, (B + 3) / 4), dim3(256), 0, s, t, B, parts,
+                               grid, g,
                                d_zmap.p, d_base.p, w, Ni, h3, d_pos.p, d_nb_start.p, d_nb_j.p, d_nb_f.p, L[0], L[1],
-                               L[2], kb[0], kb[1], d_selfc.p, self_perp, jacobi);
+                               L[2], kb[0], kb[1], d_selfc.p, self_perp, jacobi, part, d_ps_start.p, d_ps_items.p);
         MPM_HIP(hipGetLastError());
     }
 }

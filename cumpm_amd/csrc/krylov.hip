@@ -33,7 +33,7 @@ namespace mpm {
 namespace {
 
 constexpr int kRedBlocks = 64;   // max partial blocks per system in reductions
-constexpr int kRedChunk = 2048;  // elements per reduction block
+constexpr int kRedChunk = 1024;  // elements per reduction block
 
 struct Ptrs {
     double2* a[kMaxSlots];
@@ -55,32 +55,67 @@ __device__ inline double wsum(double v) {
 
 // partial[slot][blk][i] = sum_{e in chunk} op(A_i[e], b[e]), A_i = a + i*stride.
 // mode 0: conj(A_i) . b ; mode 1: A_i . b (unconjugated); mode 2: |b|^2 (real, i = 0)
+// Grid (chunks, slots * groups): one block reduces one element chunk of one
+// system against kDotGroup consecutive A_i, so a Gram-Schmidt pass against k
+// basis vectors launches chunks * systems * ceil(k / kDotGroup) blocks and every
+// thread keeps kDotGroup independent loads in flight.
+constexpr int kDotGroup = 8;
 __global__ __launch_bounds__(256) void dots_kernel(Ptrs p, int kk, size_t stride, size_t n, size_t chunk,
                                                    int mode, double2* __restrict__ partial) {
-    const int slot = blockIdx.y;
+    __shared__ double s_red[4][2 * kDotGroup];
+    const int ng = (kk + kDotGroup - 1) / kDotGroup;
+    const int slot = blockIdx.y / ng, grp = blockIdx.y % ng;
     const int blk = blockIdx.x;
     const size_t e0 = (size_t)blk * chunk;
     const size_t e1 = min(n, e0 + chunk);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const double2* A = p.a[slot];
-    const double2* B = p.b[slot];
-    for (int i = wave; i < kk; i += 4) {
-        const double2* Ai = A + (size_t)i * stride;
-        double sr = 0.0, si = 0.0;
-        for (size_t e = e0 + lane; e < e1; e += 64) {
-            const double2 bv = B[e];
-            if (mode == 2) {
-                sr += bv.x * bv.x + bv.y * bv.y;
-            } else {
-                const double2 av = Ai[e];
-                const double2 t = (mode == 0) ? cmulc(av, bv) : cmul(av, bv);
-                sr += t.x;
-                si += t.y;
+    const int i0 = grp * kDotGroup;
+    const int ni = min(kDotGroup, kk - i0);
+    const double2* A = p.a[slot] + (size_t)i0 * stride;
+    const double2* Bv = p.b[slot];
+    double sr[kDotGroup], si[kDotGroup];
+#pragma unroll
+    for (int i = 0; i < kDotGroup; ++i) sr[i] = si[i] = 0.0;
+    for (size_t e = e0 + threadIdx.x; e < e1; e += 256) {
+        const double2 bv = Bv[e];
+        if (mode == 2) {
+            sr[0] += bv.x * bv.x + bv.y * bv.y;
+        } else {
+#pragma unroll
+            for (int i = 0; i < kDotGroup; ++i) {
+                if (i < ni) {
+                    const double2 av = A[(size_t)i * stride + e];
+                    const double2 t = (mode == 0) ? cmulc(av, bv) : cmul(av, bv);
+                    sr[i] += t.x;
+                    si[i] += t.y;
+                }
             }
         }
-        sr = wsum(sr);
-        si = wsum(si);
-        if (lane == 0) partial[((size_t)slot * kRedBlocks + blk) * 64 + i] = make_double2(sr, si);
+    }
+#pragma unroll
+    for (int i = 0; i < kDotGroup; ++i) {
+        if (i < ni) {
+            sr[i] = wsum(sr[i]);
+            si[i] = wsum(si[i]);
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < kDotGroup; ++i) {
+            s_red[wave][2 * i] = sr[i];
+            s_red[wave][2 * i + 1] = si[i];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < ni) {
+        const int i = threadIdx.x;
+        double r = 0.0, im = 0.0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            r += s_red[w][2 * i];
+            im += s_red[w][2 * i + 1];
+        }
+        partial[((size_t)slot * kRedBlocks + blk) * 64 + i0 + i] = make_double2(r, im);
     }
 }
 
@@ -339,6 +374,9 @@ inline size_t chunk_for(size_t n) {
     return (n + nb - 1) / nb;
 }
 inline dim3 grid_for(size_t n, int nslots) { return dim3((unsigned)((n + 255) / 256), (unsigned)nslots); }
+inline dim3 dots_grid(size_t n, int nslots, int kk) {
+    return dim3((unsigned)nblk_for(n), (unsigned)(nslots * ((kk + kDotGroup - 1) / kDotGroup)));
+}
 
 // Scratch owned by one solve call.
 struct Work {
@@ -385,7 +423,7 @@ struct Ctx {
     // squared norms of vectors b[slot] -> W.nrm[sys]
     void norms(Ptrs& p, int ns) {
         const int nb = nblk_for(n);
-        hipLaunchKernelGGL(dots_kernel, dim3(nb, ns), dim3(256), 0, s, p, 1, (size_t)0, n, chunk_for(n), 2, W.partial.p);
+        hipLaunchKernelGGL(dots_kernel, dots_grid(n, ns, 1), dim3(256), 0, s, p, 1, (size_t)0, n, chunk_for(n), 2, W.partial.p);
         MPM_HIP(hipGetLastError());
         // reduce into nrm (stored as double2 real part) via h scratch then copy
         hipLaunchKernelGGL(reduce_kernel, dim3((ns + 255) / 256), dim3(256), 0, s, p, ns, 1, nb,
@@ -566,7 +604,7 @@ static void gmres_group(Ctx& C, const double2* b, double2* x, int B, mpm_solve_r
             for (int pass = 0; pass < 2; ++pass) {
                 {
                     ProfScope ps("krylov", C.s);
-                    hipLaunchKernelGGL(dots_kernel, dim3(nb, ns), dim3(256), 0, C.s, pw, k + 1, n, n, chunk_for(n), 0, W.partial.p);
+                    hipLaunchKernelGGL(dots_kernel, dots_grid(n, ns, k + 1), dim3(256), 0, C.s, pw, k + 1, n, n, chunk_for(n), 0, W.partial.p);
                     hipLaunchKernelGGL(reduce_kernel, dim3((ns * (k + 1) + 255) / 256), dim3(256), 0, C.s, pw, ns,
                                        k + 1, nb, W.partial.p, W.h.p, m + 1, pass, W.hlast.p);
                     hipLaunchKernelGGL(multi_axpy_kernel, grid_for(n, ns), dim3(256), 0, C.s, pw, k + 1, n, n,
@@ -579,7 +617,7 @@ static void gmres_group(Ctx& C, const double2* b, double2* x, int B, mpm_solve_r
             // norms() reuses W.h as reduction scratch: keep h safe by reducing into hlast
             {
                 const int nb2 = nblk_for(n);
-                hipLaunchKernelGGL(dots_kernel, dim3(nb2, ns), dim3(256), 0, C.s, pn, 1, (size_t)0, n, chunk_for(n), 2, W.partial.p);
+                hipLaunchKernelGGL(dots_kernel, dots_grid(n, ns, 1), dim3(256), 0, C.s, pn, 1, (size_t)0, n, chunk_for(n), 2, W.partial.p);
                 hipLaunchKernelGGL(reduce_kernel, dim3((ns + 255) / 256), dim3(256), 0, C.s, pn, ns, 1, nb2,
                                    W.partial.p, W.hlast.p, m + 1, 0, (double2*)nullptr);
                 hipLaunchKernelGGL(extract_kernel, dim3((ns + 63) / 64), dim3(64), 0, C.s, pn, ns, W.hlast.p, m + 1,
@@ -722,7 +760,7 @@ static void bicgstab_group(Ctx& C, const double2* b, double2* x, int B, mpm_solv
         Ptrs p = make_ptrs(sys, [&](int q) { return vec(A, q); }, [&](int q) { return vec(Bv, q); });
         const int nb = nblk_for(n);
         const int ns = (int)sys.size();
-        hipLaunchKernelGGL(dots_kernel, dim3(nb, ns), dim3(256), 0, C.s, p, 1, (size_t)0, n, chunk_for(n), mode, W.partial.p);
+        hipLaunchKernelGGL(dots_kernel, dots_grid(n, ns, 1), dim3(256), 0, C.s, p, 1, (size_t)0, n, chunk_for(n), mode, W.partial.p);
         hipLaunchKernelGGL(reduce_kernel, dim3((ns + 255) / 256), dim3(256), 0, C.s, p, ns, 1, nb, W.partial.p,
                            W.h.p, 2, 0, (double2*)nullptr);
         MPM_HIP(hipGetLastError());

@@ -168,11 +168,24 @@ class EwaldOp {
     DevBuf<int> d_nb_j;          // [M]
     DevBuf<double2> d_nb_f;      // [M] (perp, para)
     DevBuf<double2> d_selfc;     // [rows][N]
-    // full-grid k-space table (khat x, y, z, scale) per voxel
+    bool zcontig = false;        // slab planes form one cyclic run (zlist[a] = z0 + a mod nz)
+    // matrix-core spread (slab, FP32): xy-regions of reg_rx x 16 voxels and the
+    // candidate particles of each region (CSR, ascending ids)
+    int reg_rx = 0, reg_nrx = 0, reg_nry = 0;
+    DevBuf<int> d_reg_start;
+    DevBuf<int> d_reg_items;
+    DevBuf<int> d_ps_start;      // [N + 1] region slots of each particle
+    DevBuf<int> d_ps_items;      // slot ids (CSR positions of d_reg_items)
+    DevBuf<float> d_partial;     // [slot][6 B] per-region contractions
+    // k-space tables: (khat x, y, z, scale) per voxel for the full grid and the
+    // z-DFT slab kernel, or the Toeplitz plane coupling [Zs][6][nx*ny] for a
+    // contiguous slab; kspace_key records which one is current
     bool kspace_ready = false;
-    int kspace_layout = -1;
+    int kspace_key = -1;
     DevBuf<double> d_k64;
     DevBuf<float> d_k32;
+    DevBuf<double2> d_toe64;
+    DevBuf<float2> d_toe32;
     // grids
     DevBuf<double2> d_grid64;
     DevBuf<float2> d_grid32;

@@ -38,6 +38,13 @@ GEOMS = [
     ("override_32_P8", [24.0] * 3, 1e-8, 0.5, lambda: np.random.default_rng(4).uniform(0, 24, (30, 3)), (32, 32, 32), 8),
     ("single_particle", [20.0] * 3, 1e-6, 0.5, lambda: np.array([[3.0, 4.0, 5.0]]), None, 0),
     ("bilayer", None, 1e-4, 0.5, None, None, 0),
+    # FFT lengths covering the register radices 3..17, 23 and the generic stage (29, 61 = prime)
+    ("radix_mix_a", [24.0, 22.0, 26.0], 1e-6, 0.5, lambda: np.random.default_rng(5).uniform(0, 22, (30, 3)),
+     (39, 34, 46), 0),
+    ("radix_mix_b", [24.0, 21.0, 30.0], 1e-6, 0.5, lambda: np.random.default_rng(6).uniform(0, 21, (30, 3)),
+     (58, 27, 121), 0),
+    ("radix_mix_c", [24.0, 24.0, 24.0], 1e-6, 0.5, lambda: np.random.default_rng(7).uniform(0, 24, (30, 3)),
+     (61, 44, 45), 0),
 ]
 
 
@@ -96,7 +103,8 @@ def test_apply_fp64_matches_oracle(name, layout, fft):
 
 
 @pytest.mark.parametrize("name,layout,fft", [("random3d", 0, 0), ("hex_slab", 0, 0), ("hex_slab", 1, 0),
-                                             ("sc_tight", 0, 0), ("bilayer", 0, 0), ("hex_slab", 0, 1)])
+                                             ("sc_tight", 0, 0), ("bilayer", 0, 0), ("hex_slab", 0, 1),
+                                             ("radix_mix_a", 0, 0), ("radix_mix_b", 0, 0)])
 def test_apply_fp32_reciprocal_within_reference_tolerance(name, layout, fft):
     box, tol, xi, pos, grid, P = _geom(name)
     _case(f"apply fp32 {name} layout={layout} fft={fft}")
