@@ -340,10 +340,18 @@ constexpr int fft_elems() {  // lines x length per block (two LDS buffers of thi
     return sizeof(R) == 4 ? 2048 : 1024;
 }
 
+// Stage constants are computed on the host so that the kernel does no integer
+// division: stage st has radix radix[st], m = n / radix butterflies per line,
+// Ns = product of the earlier radices, step = n / (Ns radix).
 struct LinePlan {
     int n;
     int nstages;
     int radix[kMaxStages];
+    int m[kMaxStages];
+    int ns[kMaxStages];
+    int step[kMaxStages];
+    float rm[kMaxStages];
+    float rns[kMaxStages];
 };
 
 struct LineGeom {
@@ -358,10 +366,11 @@ __device__ inline long long line_base(const LineGeom& g, long long l) {
     return (l / g.inner) * g.outer + (l % g.inner) * g.inner_stride;
 }
 
-// x / d and x % d for 0 <= x < 2^22 via a float reciprocal and one correction.
+// x / d and x % d for 0 <= x < 2^22 via a float reciprocal and one correction
+// (24-bit multiply: full rate).
 __device__ inline void fdivmod(int x, int d, float rcp, int& q, int& r) {
     q = (int)((float)x * rcp);
-    r = x - q * d;
+    r = x - __mul24(q, d);
     if (r < 0) {
         --q;
         r += d;
@@ -554,27 +563,27 @@ __device__ __forceinline__ void dft_reg(typename C2<R>::T* v) {
 // One Stockham stage of radix P (register DFT), src -> dst (LDS ping-pong).
 template <typename R, int P, bool INV>
 __device__ __forceinline__ void fft_stage(const typename C2<R>::T* src, typename C2<R>::T* dst, int n, int nl,
-                                          int Ns, const typename C2<R>::T* tw) {
+                                          const LinePlan& pl, int st, const typename C2<R>::T* tw) {
     using CT = typename C2<R>::T;
-    const int m = n / P;
-    const int total = nl * m;
-    const int step = n / (Ns * P);
-    const float rm = 1.0f / (float)m, rns = 1.0f / (float)Ns;
+    const int m = pl.m[st], Ns = pl.ns[st], step = pl.step[st];
+    const float rm = pl.rm[st], rns = pl.rns[st];
+    const int total = __mul24(nl, m);
     for (int bf = threadIdx.x; bf < total; bf += blockDim.x) {
         int li, j, jd, a;
         fdivmod(bf, m, rm, li, j);
         fdivmod(j, Ns, rns, jd, a);
-        const CT* sp = src + li * n + j;
+        const int lb = __mul24(li, n);
+        const CT* sp = src + lb + j;
         CT v[P];
 #pragma unroll
         for (int r = 0; r < P; ++r) v[r] = sp[r * m];
         if (a > 0) {
-            const int ta = a * step;
+            const int ta = __mul24(a, step);
 #pragma unroll
             for (int r = 1; r < P; ++r) v[r] = cmul_t(v[r], tw[ta * r]);
         }
         dft_reg<R, P, INV>(v);
-        CT* dp = dst + li * n + jd * Ns * P + a;
+        CT* dp = dst + lb + __mul24(jd, Ns * P) + a;
 #pragma unroll
         for (int q = 0; q < P; ++q) dp[q * Ns] = v[q];
     }
@@ -629,15 +638,16 @@ __device__ inline void stage_generic(typename C2<R>::T* src, typename C2<R>::T* 
 //   0: radices up to 8;  1: up to 16;  2: up to 16 and the primes 17, 19, 23.
 template <typename R, bool INV, int CLS>
 __device__ __forceinline__ void run_stage(typename C2<R>::T* src, typename C2<R>::T* dst, typename C2<R>::T* roots,
-                                          int n, int nl, int Ns, int Rx, const typename C2<R>::T* tw) {
+                                          int n, int nl, const LinePlan& pl, int st, const typename C2<R>::T* tw) {
+    const int Rx = pl.radix[st];
 #define MPM_ST(P)                                    \
     case P:                                          \
-        fft_stage<R, P, INV>(src, dst, n, nl, Ns, tw);    \
+        fft_stage<R, P, INV>(src, dst, n, nl, pl, st, tw);    \
         return;
 #define MPM_ST_C(P, C)                                    \
     case P:                                               \
         if constexpr (CLS >= C) {                         \
-            fft_stage<R, P, INV>(src, dst, n, nl, Ns, tw);     \
+            fft_stage<R, P, INV>(src, dst, n, nl, pl, st, tw);     \
             return;                                       \
         }                                                 \
         break;
@@ -665,77 +675,92 @@ __device__ __forceinline__ void run_stage(typename C2<R>::T* src, typename C2<R>
     }
 #undef MPM_ST
 #undef MPM_ST_C
-    stage_generic<R>(src, dst, roots, n, nl, Ns, Rx, tw);
+    stage_generic<R>(src, dst, roots, n, nl, pl.ns[st], Rx, tw);
 }
 
+// Line base offsets fit in 32 bits: the library limits one transform call to
+// fewer than 2^31 elements.
+__device__ inline int line_base32(const LineGeom& g, int l) {
+    return (int)((l / g.inner) * g.outer + (l % g.inner) * g.inner_stride);
+}
+
+// One axis of the transform over TL lines per block: lines are read with the
+// input geometry and written with the output geometry (in place when both
+// buffers and geometries agree; the slab's y pass writes transposed).
 template <typename R, bool INV, int CLS>
-__global__ __launch_bounds__(256) void line_fft_kernel(typename C2<R>::T* __restrict__ data, LineGeom lg, LinePlan pl,
+__global__ __launch_bounds__(256) void line_fft_kernel(const typename C2<R>::T* __restrict__ in,
+                                                       typename C2<R>::T* __restrict__ out, LineGeom gin,
+                                                       LineGeom gout, LinePlan pl,
                                                        const typename C2<R>::T* __restrict__ twiddle, int TL) {
     using CT = typename C2<R>::T;
     extern __shared__ unsigned char s_raw[];
     const int n = pl.n;
-    CT* bufs[2];
-    bufs[0] = reinterpret_cast<CT*>(s_raw);
-    bufs[1] = bufs[0] + (size_t)TL * n;
-    CT* tw = bufs[1] + (size_t)TL * n;
+    // two stage buffers of TL lines, then the twiddles and the generic-stage roots;
+    // the buffers are selected by offset so that every access stays an LDS access
+    CT* const lds0 = reinterpret_cast<CT*>(s_raw);
+    const int boff = TL * n;
+    CT* tw = lds0 + 2 * boff;
     CT* roots = tw + n;
     const int tid = threadIdx.x;
-    const long long l0 = (long long)blockIdx.x * TL;
-    const int nl = (int)min((long long)TL, lg.nlines - l0);
+    const int l0 = blockIdx.x * TL;
+    const int nl = min(TL, (int)gin.nlines - l0);
     for (int k = tid; k < n; k += blockDim.x) {
         CT w = twiddle[k];
         if (INV) w.y = -w.y;
         tw[k] = w;
     }
-    const int tot = nl * n;
-    // the lines of a block are one contiguous run when each line is contiguous
-    // and consecutive lines follow each other
-    const bool run = lg.elem_stride == 1 && lg.inner == 1 && lg.outer == n;
-    __shared__ long long s_lb[64];
-    if (!run && tid < nl) s_lb[tid] = line_base(lg, l0 + tid);
+    // a run: each line contiguous and consecutive lines adjacent
+    const bool run_in = gin.elem_stride == 1 && gin.inner == 1 && gin.outer == n;
+    const bool run_out = gout.elem_stride == 1 && gout.inner == 1 && gout.outer == n;
+    __shared__ int s_lbi[64], s_lbo[64];
+    if (tid < nl) {
+        s_lbi[tid] = line_base32(gin, l0 + tid);
+        s_lbo[tid] = line_base32(gout, l0 + tid);
+    }
     __syncthreads();
+    const int tot = __mul24(nl, n);
     const float rn = 1.0f / (float)n, rnl = 1.0f / (float)nl;
-    CT* buf = bufs[0];
-    if (run) {
-        const CT* src = data + l0 * n;
+    CT* buf = lds0;
+    if (run_in) {
+        const CT* src = in + __mul24(l0, n);
         for (int e = tid; e < tot; e += blockDim.x) buf[e] = src[e];
-    } else if (lg.elem_stride == 1) {
+    } else if (gin.elem_stride == 1) {
         for (int e = tid; e < tot; e += blockDim.x) {
             int li, k;
             fdivmod(e, n, rn, li, k);
-            buf[e] = data[s_lb[li] + k];
+            buf[e] = in[s_lbi[li] + k];
         }
     } else {
+        const int es = (int)gin.elem_stride;
         for (int e = tid; e < tot; e += blockDim.x) {
             int k, li;
             fdivmod(e, nl, rnl, k, li);
-            buf[li * n + k] = data[s_lb[li] + k * lg.elem_stride];
+            buf[__mul24(li, n) + k] = in[s_lbi[li] + __mul24(k, es)];
         }
     }
     __syncthreads();
-    int Ns = 1, cur = 0;
+    int cur = 0;
     for (int st = 0; st < pl.nstages; ++st) {
-        const int Rx = pl.radix[st];
-        run_stage<R, INV, CLS>(bufs[cur], bufs[cur ^ 1], roots, n, nl, Ns, Rx, tw);
+        run_stage<R, INV, CLS>(lds0 + (cur ? boff : 0), lds0 + (cur ? 0 : boff), roots, n, nl, pl, st, tw);
         __syncthreads();
         cur ^= 1;
-        Ns *= Rx;
     }
-    buf = bufs[cur];
-    if (run) {
-        CT* dstp = data + l0 * n;
+    buf = lds0 + (cur ? boff : 0);
+    if (run_out) {
+        CT* dstp = out + __mul24(l0, n);
         for (int e = tid; e < tot; e += blockDim.x) dstp[e] = buf[e];
-    } else if (lg.elem_stride == 1) {
+    } else if (gout.elem_stride == 1) {
         for (int e = tid; e < tot; e += blockDim.x) {
             int li, k;
             fdivmod(e, n, rn, li, k);
-            data[s_lb[li] + k] = buf[e];
+            out[s_lbo[li] + k] = buf[e];
         }
     } else {
+        const int es = (int)gout.elem_stride;
         for (int e = tid; e < tot; e += blockDim.x) {
             int k, li;
             fdivmod(e, nl, rnl, k, li);
-            data[s_lb[li] + k * lg.elem_stride] = buf[li * n + k];
+            out[s_lbo[li] + __mul24(k, es)] = buf[__mul24(li, n) + k];
         }
     }
 }
@@ -780,9 +805,17 @@ struct LineAxis {
     explicit LineAxis(int n) {
         plan.n = n;
         plan.nstages = 0;
+        int ns = 1;
         for (int r : plan_radices(n)) {
             MPM_REQUIRE(plan.nstages < kMaxStages, MPM_ERR_UNSUPPORTED, "FFT length has too many factors");
-            plan.radix[plan.nstages++] = r;
+            const int st = plan.nstages++;
+            plan.radix[st] = r;
+            plan.m[st] = n / r;
+            plan.ns[st] = ns;
+            plan.step[st] = n / (ns * r);
+            plan.rm[st] = 1.0f / (float)plan.m[st];
+            plan.rns[st] = 1.0f / (float)ns;
+            ns *= r;
             if (r > kMaxPrimeRadix) continue;  // generic stage
             cls = std::max(cls, r <= 8 ? 0 : (r <= 16 ? 1 : 2));
         }
@@ -800,11 +833,21 @@ struct LineAxis {
     }
 };
 
-// Built-in batched FFT over the axes of one reciprocal layout.
+// Built-in batched FFT over the axes of one reciprocal layout.  A pass
+// transforms one axis, reading buffer `src` with geometry `gin` and writing
+// buffer `dst` with `gout` (0: the spread/contraction grid, 1: the k-space
+// buffer); the line descriptors count the lines of one right-hand side and
+// extend unchanged over B stacked right-hand sides.
+struct LinePass {
+    int axis;
+    LineGeom gin, gout;
+    int src, dst;
+};
+
 struct LineFft {
-    std::vector<std::unique_ptr<LineAxis>> axes;  // one per transformed axis
-    std::vector<LineGeom> geom_per_rhs;            // nlines counted for one RHS
-    long long rhs_elems = 0;                       // elements per RHS (grid size)
+    std::vector<std::unique_ptr<LineAxis>> axes;
+    std::vector<LinePass> fwd, inv;
+    long long rhs_elems = 0;  // elements per RHS (grid size)
 
     // lines per block: as many as fit the per-block element budget (at most 64)
     template <typename R>
@@ -814,34 +857,39 @@ struct LineFft {
     static bool fits(int n) { return n <= fft_elems<double>(); }
 
     template <typename R, bool INV, int CLS>
-    static void launch(dim3 grid, size_t lds, hipStream_t s, typename C2<R>::T* data, const LineGeom& g,
-                       const LinePlan& pl, const typename C2<R>::T* tw, int TL) {
-        hipLaunchKernelGGL((line_fft_kernel<R, INV, CLS>), grid, dim3(256), lds, s, data, g, pl, tw, TL);
+    static void launch(dim3 grid, size_t lds, hipStream_t s, const typename C2<R>::T* in, typename C2<R>::T* out,
+                       const LineGeom& gi, const LineGeom& go, const LinePlan& pl, const typename C2<R>::T* tw,
+                       int TL) {
+        hipLaunchKernelGGL((line_fft_kernel<R, INV, CLS>), grid, dim3(256), lds, s, in, out, gi, go, pl, tw, TL);
     }
 
     template <typename R>
-    void exec(typename C2<R>::T* grid, int B, bool forward, hipStream_t s) {
+    void exec(typename C2<R>::T* grid, typename C2<R>::T* kgrid, int B, bool forward, hipStream_t s) {
         using CT = typename C2<R>::T;
-        for (size_t a = 0; a < axes.size(); ++a) {
-            LineAxis& ax = *axes[a];
-            // the line descriptors of one RHS extend unchanged over the B stacked RHS
-            LineGeom g = geom_per_rhs[a];
-            g.nlines *= B;
+        MPM_REQUIRE((long long)B * rhs_elems < (1LL << 31), MPM_ERR_UNSUPPORTED,
+                    "reciprocal grid too large for one transform call");
+        for (const LinePass& ps : (forward ? fwd : inv)) {
+            LineAxis& ax = *axes[ps.axis];
+            LineGeom gi = ps.gin, go = ps.gout;
+            gi.nlines *= B;
+            go.nlines *= B;
             const int n = ax.plan.n;
             const int TL = lines_per_block<R>(n);
             const size_t lds = ((size_t)2 * TL * n + 2 * n) * sizeof(CT);
-            const long long blocks = (g.nlines + TL - 1) / TL;
+            const long long blocks = (gi.nlines + TL - 1) / TL;
             const CT* tw = std::is_same<R, float>::value ? reinterpret_cast<const CT*>(ax.tw32.p)
                                                          : reinterpret_cast<const CT*>(ax.tw64.p);
+            const CT* in = ps.src ? kgrid : grid;
+            CT* out = ps.dst ? kgrid : grid;
             const dim3 gd((unsigned)blocks);
             if (forward) {
-                if (ax.cls == 0) launch<R, false, 0>(gd, lds, s, grid, g, ax.plan, tw, TL);
-                else if (ax.cls == 1) launch<R, false, 1>(gd, lds, s, grid, g, ax.plan, tw, TL);
-                else launch<R, false, 2>(gd, lds, s, grid, g, ax.plan, tw, TL);
+                if (ax.cls == 0) launch<R, false, 0>(gd, lds, s, in, out, gi, go, ax.plan, tw, TL);
+                else if (ax.cls == 1) launch<R, false, 1>(gd, lds, s, in, out, gi, go, ax.plan, tw, TL);
+                else launch<R, false, 2>(gd, lds, s, in, out, gi, go, ax.plan, tw, TL);
             } else {
-                if (ax.cls == 0) launch<R, true, 0>(gd, lds, s, grid, g, ax.plan, tw, TL);
-                else if (ax.cls == 1) launch<R, true, 1>(gd, lds, s, grid, g, ax.plan, tw, TL);
-                else launch<R, true, 2>(gd, lds, s, grid, g, ax.plan, tw, TL);
+                if (ax.cls == 0) launch<R, true, 0>(gd, lds, s, in, out, gi, go, ax.plan, tw, TL);
+                else if (ax.cls == 1) launch<R, true, 1>(gd, lds, s, in, out, gi, go, ax.plan, tw, TL);
+                else launch<R, true, 2>(gd, lds, s, in, out, gi, go, ax.plan, tw, TL);
             }
             MPM_HIP(hipGetLastError());
         }
@@ -853,8 +901,19 @@ struct LineFft {
 // and zero at k = 0.  monodisperse_ewald_electric_field.cu:1300-1404.
 struct KParams {
     int nx, ny, nz;
+    int tcols;  // slab columns ordered ky * nx + kx (else kx * ny + ky)
     double Lx, Ly, Lz, kbx, kby, xi, etax, etay, etaz, invV;
 };
+
+__device__ inline void col_xy(const KParams& q, int col, int& ix, int& iy) {
+    if (q.tcols) {
+        iy = col / q.nx;
+        ix = col - iy * q.nx;
+    } else {
+        ix = col / q.ny;
+        iy = col - ix * q.ny;
+    }
+}
 
 __device__ inline void kcoef(const KParams& q, int ix, int iy, int iz, double o[4]) {
     const double fx = (ix <= (q.nx - 1) / 2) ? ix : (ix - q.nx);
@@ -1519,9 +1578,11 @@ __global__ void kspace_slab_kernel(double* __restrict__ k64, float* __restrict__
     const size_t V = ncols * q.nz;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < V; e += (size_t)gridDim.x * blockDim.x) {
         const int kz = (int)(e / ncols);
-        const size_t col = e % ncols;
+        const int col = (int)(e % ncols);
+        int ix, iy;
+        col_xy(q, col, ix, iy);
         double o[4];
-        kcoef(q, (int)(col / q.ny), (int)(col % q.ny), kz, o);
+        kcoef(q, ix, iy, kz, o);
         for (int c = 0; c < 4; ++c) {
             k64[4 * e + c] = o[c];
             k32[4 * e + c] = (float)o[c];
@@ -1542,9 +1603,11 @@ __global__ void ztoeplitz_table_kernel(double2* __restrict__ t64, float2* __rest
     const int d = blockIdx.y;
     if (col >= ncols || d >= Zs) return;
     double tr[6] = {0, 0, 0, 0, 0, 0}, ti[6] = {0, 0, 0, 0, 0, 0};
+    int ix, iy;
+    col_xy(q, col, ix, iy);
     for (int kz = 0; kz < q.nz; ++kz) {
         double o[4];
-        kcoef(q, col / q.ny, col % q.ny, kz, o);
+        kcoef(q, ix, iy, kz, o);
         if (o[3] == 0.0) continue;
         double sv, cv;
         sincospi(2.0 * (double)(((long long)kz * d) % q.nz) / (double)q.nz, &sv, &cv);
@@ -1873,6 +1936,7 @@ KParams EwaldOp::kparams() const {
     q.nx = S.n[0];
     q.ny = S.n[1];
     q.nz = S.n[2];
+    q.tcols = (slab && kt) ? 1 : 0;
     q.Lx = L[0];
     q.Ly = L[1];
     q.Lz = L[2];
@@ -1892,7 +1956,7 @@ void EwaldOp::build_kspace() {
 }
 
 void EwaldOp::ensure_kspace_table() {
-    const int key = !slab ? 0 : (zcontig ? 2 + Zs : 1);
+    const int key = !slab ? 0 : (zcontig ? 2 + Zs : 1) + (kt ? 1000 : 0);
     if (kspace_ready && kspace_key == key) return;
     const int threads = 256;
     if (slab && zcontig) {
@@ -2152,24 +2216,59 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         const long long nxy = (long long)nx * ny;
         auto lf = std::make_unique<LineFft>();
         bool ok = true;
-        auto add = [&](int n, LineGeom g) {
+        auto axis = [&](int n) {
             ok = ok && LineFft::fits(n);
-            if (!ok) return;
-            lf->axes.push_back(std::make_unique<LineAxis>(n));
-            lf->geom_per_rhs.push_back(g);
+            lf->axes.push_back(std::make_unique<LineAxis>(ok ? n : 2));
+            return (int)lf->axes.size() - 1;
         };
-        if (slab) {  // [3][Zs][nx][ny]: y lines (contiguous), then x lines (stride ny)
+        auto pass = [&](std::vector<LinePass>& v, int a, LineGeom gi, LineGeom go, int src, int dst) {
+            v.push_back(LinePass{a, gi, go, src, dst});
+        };
+        if (slab && nx > 1 && ny > 1) {
+            // [3][Zs][nx][ny] -> y transform written transposed to the k-space
+            // buffer [3][Zs][ky][nx] -> x transform in place on contiguous lines
             lf->rhs_elems = 3LL * Zs * nxy;
-            if (ny > 1) add(ny, LineGeom{3LL * Zs * nx, 1, 0, ny, 1});
-            if (nx > 1) add(nx, LineGeom{3LL * Zs * ny, ny, 1, nxy, ny});
-        } else {     // [3][nx][ny][nz]: z, y, x lines
+            const int ay = axis(ny), ax = axis(nx);
+            const LineGeom yrun{3LL * Zs * nx, 1, 0, ny, 1};
+            const LineGeom ytr{3LL * Zs * nx, nx, 1, nxy, nx};
+            const LineGeom xrun{3LL * Zs * ny, 1, 0, nx, 1};
+            pass(lf->fwd, ay, yrun, ytr, 0, 1);
+            pass(lf->fwd, ax, xrun, xrun, 1, 1);
+            pass(lf->inv, ax, xrun, xrun, 1, 1);
+            pass(lf->inv, ay, ytr, yrun, 1, 0);
+        } else if (slab) {  // a one-voxel axis: in place on [3][Zs][nx][ny]
+            lf->rhs_elems = 3LL * Zs * nxy;
+            if (ny > 1) {
+                const LineGeom g{3LL * Zs * nx, 1, 0, ny, 1};
+                const int a = axis(ny);
+                pass(lf->fwd, a, g, g, 0, 0);
+                pass(lf->inv, a, g, g, 0, 0);
+            }
+            if (nx > 1) {
+                const LineGeom g{3LL * Zs * ny, ny, 1, nxy, ny};
+                const int a = axis(nx);
+                pass(lf->fwd, a, g, g, 0, 0);
+                pass(lf->inv, a, g, g, 0, 0);
+            }
+        } else {  // [3][nx][ny][nz]: z, y, x lines in place
             lf->rhs_elems = 3LL * nxy * nzz;
-            if (nzz > 1) add(nzz, LineGeom{3LL * nxy, 1, 0, nzz, 1});
-            if (ny > 1) add(ny, LineGeom{3LL * nx * nzz, nzz, 1, (long long)ny * nzz, nzz});
-            if (nx > 1) add(nx, LineGeom{3LL * ny * nzz, (long long)ny * nzz, 1, nxy * nzz, (long long)ny * nzz});
+            const LineGeom gz{3LL * nxy, 1, 0, nzz, 1};
+            const LineGeom gy{3LL * nx * nzz, nzz, 1, (long long)ny * nzz, nzz};
+            const LineGeom gx{3LL * ny * nzz, (long long)ny * nzz, 1, nxy * nzz, (long long)ny * nzz};
+            const int n3[3] = {nzz, ny, nx};
+            const LineGeom g3[3] = {gz, gy, gx};
+            for (int d = 0; d < 3; ++d) {
+                if (n3[d] <= 1) continue;
+                const int a = axis(n3[d]);
+                pass(lf->fwd, a, g3[d], g3[d], 0, 0);
+                pass(lf->inv, a, g3[d], g3[d], 0, 0);
+            }
         }
         lfft = ok ? std::move(lf) : nullptr;
     }
+    // the slab's k-space data sits transposed ([ky][kx] columns) in its own
+    // buffer when the built-in transform runs the slab with two real axes
+    kt = slab && lfft && fft_backend == 0 && S.n[0] > 1 && S.n[1] > 1;
     if (!fft || fft_key != key) {
         if (slab)
             fft = std::make_unique<FftPlans>(std::vector<size_t>{(size_t)S.n[1], (size_t)S.n[0]}, 3 * Zs);
@@ -2255,10 +2354,12 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
             MPM_HIP(hipGetLastError());
         }
         const size_t bytes_per_rhs = grid_elems_per_rhs() * sizeof(CT);
+        CT* kgrid = grid;
+        if (kt) kgrid = fp32 ? reinterpret_cast<CT*>(d_kgrid32.p) : reinterpret_cast<CT*>(d_kgrid64.p);
         {
             ProfScope ps("fft", s);
             if (lfft && fft_backend == 0)
-                lfft->exec<R>(grid, B, true, s);
+                lfft->exec<R>(grid, kgrid, B, true, s);
             else
                 fft->exec(grid, bytes_per_rhs, B, fp32, true, s);
         }
@@ -2272,7 +2373,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
             const dim3 gridz((ncols + cb - 1) / cb, (B + rb - 1) / rb);
 #define MPM_ZT(ZSV)                                                                                           \
     case ZSV:                                                                                                 \
-        hipLaunchKernelGGL((ztoeplitz_kernel<R, ZSV, ztoe_as<R, ZSV>()>), gridz, dim3(256), 0, s, grid, B, \
+        hipLaunchKernelGGL((ztoeplitz_kernel<R, ZSV, ztoe_as<R, ZSV>()>), gridz, dim3(256), 0, s, kgrid, B, \
                            ncols, Zs, rb, T);                                                                 \
         break;
             switch (zs_bucket(Zs)) {
@@ -2297,7 +2398,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
             const int rb = B >= 16 ? 16 : (B >= 4 ? 4 : 1);
             const dim3 gridz((ncols + 256 / rb - 1) / (256 / rb), (B + rb - 1) / rb);
 #define MPM_ZF2(ZSV, RBV)                                                                                     \
-    hipLaunchKernelGGL((zfused_kernel<R, ZSV, RBV>), gridz, dim3(256), lds, s, grid, B, ncols, S.n[2], Zs, \
+    hipLaunchKernelGGL((zfused_kernel<R, ZSV, RBV>), gridz, dim3(256), lds, s, kgrid, B, ncols, S.n[2], Zs, \
                        d_zlist.p, kc)
 #define MPM_ZF(ZSV)                     \
     case ZSV:                           \
@@ -2332,7 +2433,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
         {
             ProfScope ps("fft", s);
             if (lfft && fft_backend == 0)
-                lfft->exec<R>(grid, B, false, s);
+                lfft->exec<R>(grid, kgrid, B, false, s);
             else
                 fft->exec(grid, bytes_per_rhs, B, fp32, false, s);
         }
@@ -2388,10 +2489,13 @@ void EwaldOp::apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t
                     "self-coefficient row out of range");
     if (parts & MPM_PART_RECIP) ensure_kspace_table();
     const size_t elems = (size_t)B * grid_elems_per_rhs();
-    if (fp32)
+    if (fp32) {
         d_grid32.reserve(elems);
-    else
+        if (kt) d_kgrid32.reserve(elems);
+    } else {
         d_grid64.reserve(elems);
+        if (kt) d_kgrid64.reserve(elems);
+    }
     const int jac = jacobi ? 1 : 0;
     if (fp32) {
         if (slab) launch<float, true>(t, B, parts, s, jac);

@@ -169,6 +169,7 @@ class EwaldOp {
     DevBuf<double2> d_nb_f;      // [M] (perp, para)
     DevBuf<double2> d_selfc;     // [rows][N]
     bool zcontig = false;        // slab planes form one cyclic run (zlist[a] = z0 + a mod nz)
+    bool kt = false;             // slab k-space data in its own buffer, columns ordered ky * nx + kx
     // matrix-core spread (slab, FP32): xy-regions of reg_rx x 16 voxels and the
     // candidate particles of each region (CSR, ascending ids)
     int reg_rx = 0, reg_nrx = 0, reg_nry = 0;
@@ -189,6 +190,8 @@ class EwaldOp {
     // grids
     DevBuf<double2> d_grid64;
     DevBuf<float2> d_grid32;
+    DevBuf<double2> d_kgrid64;   // slab k-space buffer (transposed columns)
+    DevBuf<float2> d_kgrid32;
     std::unique_ptr<FftPlans> fft;
     std::unique_ptr<LineFft> lfft;
     int fft_key = -2;
