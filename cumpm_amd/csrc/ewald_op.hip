@@ -1248,6 +1248,59 @@ constexpr int kMfmaKC = 64;    // candidates staged per chunk
 
 typedef float mpm_f32x4 __attribute__((ext_vector_type(4)));
 
+// Region-local separable weights of a candidate, one LDS row per candidate:
+// x weights at [0, RX), slab-plane weights at [4, 4 + ZSB), y weights at
+// [4 + ZSB4, 4 + ZSB4 + 16) (ZSB4 = ZSB rounded up to 4), so that 4-wide
+// groups start on 16-byte boundaries.  A voxel's weight sums the stencil points
+// that land on it (stencils fold onto themselves on grids narrower than the
+// stencil).
+template <int RX, int ZSB>
+struct RegionW {
+    static constexpr int ZSB4 = (ZSB + 3) & ~3;
+    static constexpr int X = 0, Z = 4, Y = 4 + ZSB4;
+    static constexpr int NW = Y + kMfmaRY;
+};
+
+template <int RX, int ZSB>
+__device__ inline void stage_region_weights(float* s_w, int cnt, int x0, int y0,
+                                            const SpreadGeom& g, const int* __restrict__ zlist,
+                                            const int* __restrict__ base, const float* __restrict__ wts,
+                                            const int* __restrict__ items, int N) {
+    using L = RegionW<RX, ZSB>;
+    for (int e = threadIdx.x; e < kMfmaKC * L::NW; e += blockDim.x) {
+        const int q = e / L::NW, j = e % L::NW;
+        float w = 0.f;
+        if (q < cnt) {
+            int d = -1, v = -1, n = 1;
+            if (j >= L::Y) {
+                d = 1;
+                n = g.ny;
+                v = y0 + j - L::Y;
+                if (v >= n) v = -1;
+            } else if (j >= L::Z) {
+                const int p = j - L::Z;
+                d = 2;
+                n = g.nz;
+                v = p < g.Zs ? zlist[p] : -1;
+            } else if (j < RX) {
+                d = 0;
+                n = g.nx;
+                v = x0 + j;
+                if (v >= n) v = -1;
+            }
+            if (v >= 0) {
+                const int pid = items[q];
+                const int b0 = base[(size_t)d * N + pid] - g.half;
+                const float* wp = wts + ((size_t)pid * 3 + d) * g.width;
+                int off = (v - b0) % n;  // stencil index of the first point on voxel v
+                if (off < 0) off += n;
+                for (int o = off; o < g.width; o += n) w += wp[o];
+            }
+        }
+        s_w[e] = w;  // row q = e / NW, entry j = e % NW
+    }
+}
+
 template <int RX, int ZSB>
 __global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, SpreadGeom g,
                                                           const int* __restrict__ zlist, const int* __restrict__ base,
@@ -1256,16 +1309,16 @@ __global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, 
                                                           const int* __restrict__ reg_items, int nry, int N,
                                                           float2* __restrict__ out) {
     constexpr int RY = kMfmaRY, KC = kMfmaKC;
-    constexpr int NW = RY + RX + ZSB;  // region-local weights per candidate
+    using L = RegionW<RX, ZSB>;
     __shared__ float s_v[KC][64 + 1];
-    __shared__ float s_w[KC][NW];
+    __shared__ __align__(16) float s_w[KC][L::NW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int reg = blockIdx.x;
     const int x0 = (reg / nry) * RX, y0 = (reg % nry) * RY;
     const int col_base = blockIdx.y * 64;
     const int wcol0 = col_base + wave * 16;
     const int ncols = 6 * B;
-    const int Zs = g.Zs, width = g.width, half = g.half;
+    const int Zs = g.Zs;
     const int k0 = reg_start[reg], k1 = reg_start[reg + 1];
 
     mpm_f32x4 acc[RX][ZSB];
@@ -1290,52 +1343,19 @@ __global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, 
             }
             s_v[q][col] = v;
         }
-        // region-local weights: y0 + j (j < RY), x0 + j - RY, slab plane j - RY - RX;
-        // the sum over stencil points covers stencils that fold onto themselves
-        for (int e = tid; e < KC * NW; e += 256) {
-            const int q = e / NW, j = e % NW;
-            float w = 0.f;
-            if (q < cnt) {
-                const int pid = reg_items[c0 + q];
-                int d, v = -1, n = 1;
-                if (j < RY) {
-                    d = 1;
-                    v = y0 + j;
-                    n = g.ny;
-                    if (v >= n) v = -1;
-                } else if (j < RY + RX) {
-                    d = 0;
-                    v = x0 + j - RY;
-                    n = g.nx;
-                    if (v >= n) v = -1;
-                } else {
-                    d = 2;
-                    const int p = j - RY - RX;
-                    n = g.nz;
-                    v = p < Zs ? zlist[p] : -1;
-                }
-                if (v >= 0) {
-                    const int b0 = base[(size_t)d * N + pid] - half;
-                    const float* wp = wts + ((size_t)pid * 3 + d) * width;
-                    int off = (v - b0) % n;  // stencil index of the first point on voxel v
-                    if (off < 0) off += n;
-                    for (int o = off; o < width; o += n) w += wp[o];
-                }
-            }
-            s_w[q][j] = w;
-        }
+        stage_region_weights<RX, ZSB>(&s_w[0][0], cnt, x0, y0, g, zlist, base, wts, reg_items + c0, N);
         __syncthreads();
         if (wcol0 < ncols) {
             const int i = lane & 15;
             for (int ks = 0; ks < cnt; ks += 4) {
                 const int kq = ks + (lane >> 4);  // rows >= cnt hold zeros
                 const float a = s_v[kq][wave * 16 + i];
-                const float wy = s_w[kq][i];
+                const float wy = s_w[kq][L::Y + i];
                 float wx[RX], wz[ZSB];
 #pragma unroll
-                for (int xi = 0; xi < RX; ++xi) wx[xi] = s_w[kq][RY + xi] * wy;
+                for (int xi = 0; xi < RX; ++xi) wx[xi] = s_w[kq][L::X + xi] * wy;
 #pragma unroll
-                for (int p = 0; p < ZSB; ++p) wz[p] = s_w[kq][RY + RX + p];
+                for (int p = 0; p < ZSB; ++p) wz[p] = s_w[kq][L::Z + p];
 #pragma unroll
                 for (int p = 0; p < ZSB; ++p) {
                     if (p < Zs) {
@@ -1375,11 +1395,12 @@ __global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, 
 // particles k of a region and 16 real columns q,
 //   D[k][q] = sum_{voxels v of the region} W[k][v] G[v][q],
 // one v_mfma_f32_16x16x4_f32 per four voxels and 16 candidates (A = W, B = G).
-// The region's grid values of one slab plane at a time are staged in LDS with
-// coalesced reads (every voxel of the slab is read exactly once per column
-// block), and D is written to the particle's slot of the region (the CSR
-// position of the pair); the per-particle sum over its ~(9/RX + 1)(9/16 + 1)
-// regions happens in gather_kernel, in a fixed order.
+// Each wave reads its B operands straight from the grid: lane (j, u) loads
+// column j at y = 4 u + (lane >> 4), so the four instructions of a 16-voxel row
+// consume whole cache lines, and waves run without barriers after the one
+// weight-staging step.  D goes to the particle's slot of the region (the CSR
+// position of the pair); the per-particle sum over its regions happens in
+// gather_kernel, in a fixed order.
 template <int RX, int ZSB>
 __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g, const int* __restrict__ zlist,
                                                             const int* __restrict__ base,
@@ -1389,98 +1410,82 @@ __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g,
                                                             const float2* __restrict__ grid,
                                                             float* __restrict__ partial) {
     constexpr int RY = kMfmaRY, KC = kMfmaKC;
-    constexpr int NV = RX * RY;        // voxels of the region in one plane
-    constexpr int NW = RY + RX + ZSB;  // region-local weights per candidate
-    __shared__ float s_g[NV][64 + 1];
-    __shared__ float s_w[KC][NW];
+    using L = RegionW<RX, ZSB>;
+    constexpr int NB = KC / 16;  // candidate blocks of 16
+    __shared__ __align__(16) float s_w[KC][L::NW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int reg = blockIdx.x;
     const int x0 = (reg / nry) * RX, y0 = (reg % nry) * RY;
-    const int col_base = blockIdx.y * 64;
-    const int wcol0 = col_base + wave * 16;
+    const int wcol0 = blockIdx.y * 64 + wave * 16;
     const int ncols = 6 * B;
-    const int Zs = g.Zs, width = g.width, half = g.half;
+    const int Zs = g.Zs;
     const size_t nxy = (size_t)g.nx * g.ny;
     const int k0 = reg_start[reg], k1 = reg_start[reg + 1];
+
+    // this lane's B-operand column and its rows of the region
+    const int qc = wcol0 + (lane & 15);
+    const bool cvalid = qc < ncols;
+    const float* gcol = nullptr;
+    if (cvalid) {
+        const int b = qc / 6, c = (qc >> 1) % 3;
+        gcol = reinterpret_cast<const float*>(grid + ((size_t)b * 3 + c) * Zs * nxy) + (qc & 1);
+    }
+    int goff[RX][4];  // float offset of (x0 + xi, y0 + 4 u + lane / 16) in a plane, -1 outside
+#pragma unroll
+    for (int xi = 0; xi < RX; ++xi)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int x = x0 + xi, y = y0 + 4 * u + (lane >> 4);
+            goff[xi][u] = (cvalid && x < g.nx && y < g.ny) ? 2 * (x * g.ny + y) : -1;
+        }
 
     for (int c0 = k0; c0 < k1; c0 += KC) {
         const int cnt = min(KC, k1 - c0);
         const int nblk = (cnt + 15) >> 4;
         __syncthreads();
-        for (int e = tid; e < KC * NW; e += 256) {
-            const int q = e / NW, j = e % NW;
-            float w = 0.f;
-            if (q < cnt) {
-                const int pid = reg_items[c0 + q];
-                int d, v = -1, n = 1;
-                if (j < RY) {
-                    d = 1;
-                    v = y0 + j;
-                    n = g.ny;
-                    if (v >= n) v = -1;
-                } else if (j < RY + RX) {
-                    d = 0;
-                    v = x0 + j - RY;
-                    n = g.nx;
-                    if (v >= n) v = -1;
-                } else {
-                    d = 2;
-                    const int p = j - RY - RX;
-                    n = g.nz;
-                    v = p < Zs ? zlist[p] : -1;
-                }
-                if (v >= 0) {
-                    const int b0 = base[(size_t)d * N + pid] - half;
-                    const float* wp = wts + ((size_t)pid * 3 + d) * width;
-                    int off = (v - b0) % n;
-                    if (off < 0) off += n;
-                    for (int o = off; o < width; o += n) w += wp[o];
-                }
-            }
-            s_w[q][j] = w;
+        stage_region_weights<RX, ZSB>(&s_w[0][0], cnt, x0, y0, g, zlist, base, wts, reg_items + c0, N);
+        __syncthreads();
+        if (wcol0 >= ncols) continue;
+        // per-lane weights: candidate k = 16 nb + (lane & 15) at y = 4 u + (lane >> 4), x = x0 + xi
+        float wyl[NB][4], wxl[NB][RX];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const int k = nb * 16 + (lane & 15);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) wyl[nb][u] = s_w[k][L::Y + 4 * u + (lane >> 4)];
+#pragma unroll
+            for (int xi = 0; xi < RX; ++xi) wxl[nb][xi] = s_w[k][L::X + xi];
         }
-        mpm_f32x4 acc[KC / 16];
+        mpm_f32x4 acc[NB];
 #pragma unroll
-        for (int nb = 0; nb < KC / 16; ++nb) acc[nb] = mpm_f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int nb = 0; nb < NB; ++nb) acc[nb] = mpm_f32x4{0.f, 0.f, 0.f, 0.f};
         for (int p = 0; p < Zs; ++p) {
-            __syncthreads();
-            // s_g[xi * 16 + yi][col]: consecutive threads read consecutive y of one column pair
-            for (int e = tid; e < 32 * NV; e += 256) {
-                const int yi = e % RY, xi = (e / RY) % RX, cp = e / NV;
-                const int x = x0 + xi, y = y0 + yi, qc = col_base + 2 * cp;
-                float2 v = make_float2(0.f, 0.f);
-                if (x < g.nx && y < g.ny && qc < ncols) {
-                    const int b = qc / 6, c = (qc >> 1) % 3;
-                    v = grid[(((size_t)b * 3 + c) * Zs + p) * nxy + (size_t)x * g.ny + y];
-                }
-                s_g[xi * RY + yi][2 * cp] = v.x;
-                s_g[xi * RY + yi][2 * cp + 1] = v.y;
-            }
-            __syncthreads();
-            if (wcol0 < ncols) {
-                float wzp[KC / 16];
+            const float* gp = gcol ? gcol + 2 * (size_t)p * nxy : nullptr;
+            float gv[RX][4];
 #pragma unroll
-                for (int nb = 0; nb < KC / 16; ++nb) wzp[nb] = s_w[nb * 16 + (lane & 15)][RY + RX + p];
-                for (int ks = 0; ks < NV; ks += 4) {
-                    const int vox = ks + (lane >> 4);
-                    const int xi = vox / RY, yi = vox % RY;
-                    const float bval = s_g[vox][wave * 16 + (lane & 15)];
+            for (int xi = 0; xi < RX; ++xi)
 #pragma unroll
-                    for (int nb = 0; nb < KC / 16; ++nb) {
-                        if (nb < nblk) {
-                            const int k = nb * 16 + (lane & 15);
-                            const float a = s_w[k][yi] * s_w[k][RY + xi] * wzp[nb];
-                            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bval, acc[nb], 0, 0, 0);
-                        }
+                for (int u = 0; u < 4; ++u) gv[xi][u] = goff[xi][u] >= 0 ? gp[goff[xi][u]] : 0.f;
+            float wz[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) wz[nb] = s_w[nb * 16 + (lane & 15)][L::Z + p];
+#pragma unroll
+            for (int xi = 0; xi < RX; ++xi)
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) {
+                    if (nb < nblk) {
+                        const float wxz = wxl[nb][xi] * wz[nb];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(wyl[nb][u] * wxz, gv[xi][u], acc[nb],
+                                                                           0, 0, 0);
                     }
                 }
-            }
         }
         // lane holds D[16 nb + 4 (lane >> 4) + r][lane & 15]
-        const int qc = wcol0 + (lane & 15);
-        if (qc < ncols) {
+        if (cvalid) {
 #pragma unroll
-            for (int nb = 0; nb < KC / 16; ++nb)
+            for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int k = nb * 16 + 4 * (lane >> 4) + r;
