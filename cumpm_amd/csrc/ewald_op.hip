@@ -1229,24 +1229,62 @@ __global__ __launch_bounds__(256) void scale_kernel(typename C2<R>::T* __restric
     }
 }
 
-// ---- spread on the matrix cores (slab layout, FP32) ----
+// ---- spread on the matrix cores (slab layout) ----
 //
-// For an xy-region of RX x 16 voxels over all Zs slab planes and 16 real
-// columns q = 6 b + 2 c + (0: real, 1: imaginary) of the right-hand sides,
+// For an xy-region of RX x 16 voxels over the slab planes and 16 real columns
+// q = 6 b + 2 c + (0: real, 1: imaginary) of the right-hand sides,
 //   grid^T[q][voxel] = sum_k V[k][q] W[k][voxel],
 //   W[k][(x, y, plane)] = wx_k(x) wy_k(y) wz_k(plane)
 // over the candidate particles k of the region (every particle whose stencil
-// footprint meets it, in ascending id order), one v_mfma_f32_16x16x4_f32 per
-// four candidates and (x, plane) pair: A = V^T (16 columns x 4 candidates),
+// footprint meets it, in ascending id order), one 16x16x4 MFMA (FP32 or FP64)
+// per four candidates and (x, plane) pair: A = V^T (16 columns x 4 candidates),
 // B = W^T (4 candidates x 16 consecutive y).  The region-local separable weights
 // (stencil folding on small grids included) are staged per candidate chunk in
-// LDS, so the products cost three LDS reads and two multiplies per lane.  Every
-// voxel of the slab is written exactly once, without atomics; the sum order is
-// fixed (deterministic).  A block is four waves = 64 real columns of one region.
+// LDS.  Every voxel of the slab is written exactly once, without atomics; the
+// sum order is fixed (deterministic).
 constexpr int kMfmaRY = 16;    // y voxels per region (the MFMA N dimension)
 constexpr int kMfmaKC = 64;    // candidates staged per chunk
 
 typedef float mpm_f32x4 __attribute__((ext_vector_type(4)));
+typedef double mpm_f64x4 __attribute__((ext_vector_type(4)));
+
+// 16x16x4 MFMA: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]; its
+// accumulator entry r is D[drow(l / 16, r)][l % 16], with drow = 4 g + r for
+// FP32 and g + 4 r for FP64 (measured on gfx950).  rowcol(i) is the real
+// column a spread wave feeds into A row i so that accumulator entries (0, 1)
+// and (2, 3) of every lane are the real and imaginary parts of one complex
+// column: complex column 2 g + h of the wave sits in entries (2 h, 2 h + 1).
+template <typename R>
+struct Mfma;
+template <>
+struct Mfma<float> {
+    using V4 = mpm_f32x4;
+    static __device__ __forceinline__ V4 mac(float a, float b, V4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int drow(int g, int r) { return 4 * g + r; }
+    static __device__ __forceinline__ int rowcol(int i) { return i; }
+};
+template <>
+struct Mfma<double> {
+    using V4 = mpm_f64x4;
+    static __device__ __forceinline__ V4 mac(double a, double b, V4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int drow(int g, int r) { return g + 4 * r; }
+    // row i = g + 4 r -> complex column 2 g + r / 2, part r % 2
+    static __device__ __forceinline__ int rowcol(int i) { return 2 * (2 * (i & 3) + (i >> 3)) + ((i >> 2) & 1); }
+};
+
+// Block shape of spread_mfma_kernel: 4 waves = (CPB / 16 column groups) x
+// (plane groups of ZH planes).  FP32: 64 columns, all planes per wave; FP64:
+// 32 columns, the planes split over two waves (accumulator registers).
+template <typename R, int RX, int ZSB>
+struct SpreadShape {
+    static constexpr int CPB = sizeof(R) == 4 ? 64 : 32;
+    static constexpr int NPG = 4 / (CPB / 16);
+    static constexpr int ZH = (ZSB + NPG - 1) / NPG;
+};
 
 // Region-local separable weights of a candidate, one LDS row per candidate:
 // x weights at [0, RX), slab-plane weights at [4, 4 + ZSB), y weights at
@@ -1261,15 +1299,32 @@ struct RegionW {
     static constexpr int NW = Y + kMfmaRY;
 };
 
-template <int RX, int ZSB>
-__device__ inline void stage_region_weights(float* s_w, int cnt, int x0, int y0,
-                                            const SpreadGeom& g, const int* __restrict__ zlist,
-                                            const int* __restrict__ base, const float* __restrict__ wts,
-                                            const int* __restrict__ items, int N) {
+// Stages a chunk of cnt <= kMfmaKC candidates: ids and stencil origins
+// (base - half) first, then the region-local weights, with every global load of
+// a phase independent of the others so that they are all in flight together.
+// Needs blockDim.x == 256 and ends with a barrier.
+template <typename R, int RX, int ZSB>
+__device__ inline void stage_candidates(int* s_pid, int (*s_org)[3], R* s_w, int cnt, int x0, int y0,
+                                        const SpreadGeom& g, const int* __restrict__ zlist,
+                                        const int* __restrict__ base, const R* __restrict__ wts,
+                                        const int* __restrict__ items, int N) {
     using L = RegionW<RX, ZSB>;
-    for (int e = threadIdx.x; e < kMfmaKC * L::NW; e += blockDim.x) {
+    const int tid = threadIdx.x;
+    if (tid < kMfmaKC) {
+        const int pid = tid < cnt ? items[tid] : 0;
+        s_pid[tid] = pid;
+        s_org[tid][0] = base[pid] - g.half;
+        s_org[tid][1] = base[(size_t)N + pid] - g.half;
+        s_org[tid][2] = base[2 * (size_t)N + pid] - g.half;
+    }
+    __syncthreads();
+    constexpr int T = kMfmaKC * L::NW / 256;
+    static_assert(kMfmaKC * L::NW % 256 == 0, "weight rows must tile the block");
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int e = tid + 256 * t;
         const int q = e / L::NW, j = e % L::NW;
-        float w = 0.f;
+        R w = R(0);
         if (q < cnt) {
             int d = -1, v = -1, n = 1;
             if (j >= L::Y) {
@@ -1289,130 +1344,145 @@ __device__ inline void stage_region_weights(float* s_w, int cnt, int x0, int y0,
                 if (v >= n) v = -1;
             }
             if (v >= 0) {
-                const int pid = items[q];
-                const int b0 = base[(size_t)d * N + pid] - g.half;
-                const float* wp = wts + ((size_t)pid * 3 + d) * g.width;
-                int off = (v - b0) % n;  // stencil index of the first point on voxel v
+                const R* wp = wts + ((size_t)s_pid[q] * 3 + d) * g.width;
+                int off = (v - s_org[q][d]) % n;  // stencil index of the first point on voxel v
                 if (off < 0) off += n;
                 for (int o = off; o < g.width; o += n) w += wp[o];
             }
         }
         s_w[e] = w;  // row q = e / NW, entry j = e % NW
     }
+    __syncthreads();
 }
 
-template <int RX, int ZSB>
+template <typename R, int RX, int ZSB>
 __global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, SpreadGeom g,
                                                           const int* __restrict__ zlist, const int* __restrict__ base,
-                                                          const float* __restrict__ wts,
+                                                          const R* __restrict__ wts,
                                                           const int* __restrict__ reg_start,
                                                           const int* __restrict__ reg_items, int nry, int N,
-                                                          float2* __restrict__ out) {
-    constexpr int RY = kMfmaRY, KC = kMfmaKC;
+                                                          typename C2<R>::T* __restrict__ out) {
+    using CT = typename C2<R>::T;
+    using M = Mfma<R>;
+    using Sh = SpreadShape<R, RX, ZSB>;
+    constexpr int RY = kMfmaRY, KC = kMfmaKC, CPB = Sh::CPB, ZH = Sh::ZH;
     using L = RegionW<RX, ZSB>;
-    __shared__ float s_v[KC][64 + 1];
-    __shared__ __align__(16) float s_w[KC][L::NW];
+    __shared__ R s_v[KC][CPB + 1];
+    __shared__ __align__(16) R s_w[KC][L::NW];
+    __shared__ int s_pid[KC];
+    __shared__ int s_org[KC][3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int cg = wave % (CPB / 16), pg = wave / (CPB / 16);
     const int reg = blockIdx.x;
     const int x0 = (reg / nry) * RX, y0 = (reg % nry) * RY;
-    const int col_base = blockIdx.y * 64;
-    const int wcol0 = col_base + wave * 16;
+    const int col_base = blockIdx.y * CPB;
+    const int wcol0 = col_base + cg * 16;
     const int ncols = 6 * B;
     const int Zs = g.Zs;
+    const int p0 = pg * ZH;  // this wave's slab planes [p0, p0 + ZH)
     const int k0 = reg_start[reg], k1 = reg_start[reg + 1];
 
-    mpm_f32x4 acc[RX][ZSB];
+    typename M::V4 acc[RX][ZH];
 #pragma unroll
     for (int xi = 0; xi < RX; ++xi)
 #pragma unroll
-        for (int p = 0; p < ZSB; ++p) acc[xi][p] = mpm_f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int p = 0; p < ZH; ++p) acc[xi][p] = typename M::V4{R(0), R(0), R(0), R(0)};
 
+    // staged columns of this thread: s_v[q][scol] for q = tid / CPB + (256 / CPB) t
+    constexpr int QS = 256 / CPB;
+    const int scol = tid % CPB, sqc = col_base + scol;
+    const double* xcol = nullptr;
+    if (sqc < ncols) xcol = reinterpret_cast<const double*>(tab.x[sqc / 6]) + 2 * ((sqc >> 1) % 3) + (sqc & 1);
     for (int c0 = k0; c0 < k1; c0 += KC) {
         const int cnt = min(KC, k1 - c0);
         __syncthreads();
+        stage_candidates<R, RX, ZSB>(s_pid, s_org, &s_w[0][0], cnt, x0, y0, g, zlist, base, wts, reg_items + c0,
+                                     N);
         // values: s_v[k][col] = component (b, c), real or imaginary part, of candidate k
-        for (int e = tid; e < KC * 64; e += 256) {
-            const int q = e >> 6, col = e & 63;
-            const int qc = col_base + col;
-            float v = 0.f;
-            if (q < cnt && qc < ncols) {
-                const int pid = reg_items[c0 + q];
-                const int b = qc / 6, c = (qc >> 1) % 3;
-                const double2 dv = tab.x[b][3 * (size_t)pid + c];
-                v = (float)((qc & 1) ? dv.y : dv.x);
-            }
-            s_v[q][col] = v;
+        R vv[KC / QS];
+#pragma unroll
+        for (int t = 0; t < KC / QS; ++t) {
+            const int q = tid / CPB + QS * t;
+            vv[t] = (xcol && q < cnt) ? (R)xcol[6 * (size_t)s_pid[q]] : R(0);
         }
-        stage_region_weights<RX, ZSB>(&s_w[0][0], cnt, x0, y0, g, zlist, base, wts, reg_items + c0, N);
+#pragma unroll
+        for (int t = 0; t < KC / QS; ++t) s_v[tid / CPB + QS * t][scol] = vv[t];
         __syncthreads();
-        if (wcol0 < ncols) {
+        if (wcol0 < ncols && p0 < Zs) {
             const int i = lane & 15;
             for (int ks = 0; ks < cnt; ks += 4) {
                 const int kq = ks + (lane >> 4);  // rows >= cnt hold zeros
-                const float a = s_v[kq][wave * 16 + i];
-                const float wy = s_w[kq][L::Y + i];
-                float wx[RX], wz[ZSB];
+                const R a = s_v[kq][cg * 16 + M::rowcol(i)];
+                const R wy = s_w[kq][L::Y + i];
+                R wx[RX], wz[ZH];
 #pragma unroll
                 for (int xi = 0; xi < RX; ++xi) wx[xi] = s_w[kq][L::X + xi] * wy;
 #pragma unroll
-                for (int p = 0; p < ZSB; ++p) wz[p] = s_w[kq][L::Z + p];
+                for (int p = 0; p < ZH; ++p) wz[p] = (p0 + p < ZSB) ? s_w[kq][L::Z + p0 + p] : R(0);
 #pragma unroll
-                for (int p = 0; p < ZSB; ++p) {
-                    if (p < Zs) {
+                for (int p = 0; p < ZH; ++p) {
+                    if (p0 + p < Zs) {
 #pragma unroll
-                        for (int xi = 0; xi < RX; ++xi)
-                            acc[xi][p] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, wx[xi] * wz[p], acc[xi][p], 0, 0, 0);
+                        for (int xi = 0; xi < RX; ++xi) acc[xi][p] = M::mac(a, wx[xi] * wz[p], acc[xi][p]);
                     }
                 }
             }
         }
     }
-    // lane holds D[4 (lane >> 4) + r][lane & 15]: real columns wcol0 + 4 (lane >> 4) + r
-    // (r = 0, 1: one complex column, r = 2, 3: the next) at y = y0 + (lane & 15)
+    // lane holds complex columns 2 (lane >> 4) + h of the wave (entries 2 h, 2 h + 1)
+    // at y = y0 + (lane & 15)
     const int y = y0 + (lane & 15);
-    if (y >= g.ny) return;
+    if (y >= g.ny || wcol0 >= ncols) return;
     const size_t nxy = (size_t)g.nx * g.ny;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int qc = wcol0 + 4 * (lane >> 4) + 2 * h;
         if (qc >= ncols) continue;
         const int b = qc / 6, c = (qc >> 1) % 3;
-        float2* o = out + ((size_t)b * 3 + c) * Zs * nxy + y;
+        CT* o = out + ((size_t)b * 3 + c) * Zs * nxy + y;
 #pragma unroll
         for (int xi = 0; xi < RX; ++xi) {
             const int x = x0 + xi;
             if (x >= g.nx) continue;
 #pragma unroll
-            for (int p = 0; p < ZSB; ++p)
-                if (p < Zs) o[p * nxy + (size_t)x * g.ny] = make_float2(acc[xi][p][2 * h], acc[xi][p][2 * h + 1]);
+            for (int p = 0; p < ZH; ++p)
+                if (p0 + p < Zs) {
+                    CT v;
+                    v.x = acc[xi][p][2 * h];
+                    v.y = acc[xi][p][2 * h + 1];
+                    o[(p0 + p) * nxy + (size_t)x * g.ny] = v;
+                }
         }
     }
 }
 
-// ---- contraction on the matrix cores (slab layout, FP32) ----
+// ---- contraction on the matrix cores (slab layout) ----
 //
 // The transpose of spread_mfma_kernel over the same regions: for the candidate
 // particles k of a region and 16 real columns q,
 //   D[k][q] = sum_{voxels v of the region} W[k][v] G[v][q],
-// one v_mfma_f32_16x16x4_f32 per four voxels and 16 candidates (A = W, B = G).
-// Each wave reads its B operands straight from the grid: lane (j, u) loads
-// column j at y = 4 u + (lane >> 4), so the four instructions of a 16-voxel row
-// consume whole cache lines, and waves run without barriers after the one
-// weight-staging step.  D goes to the particle's slot of the region (the CSR
-// position of the pair); the per-particle sum over its regions happens in
-// gather_kernel, in a fixed order.
-template <int RX, int ZSB>
+// one 16x16x4 MFMA per four voxels and 16 candidates (A = W, B = G).  Each wave
+// reads its B operands straight from the grid: lane (j, u) loads column j at
+// y = 4 u + (lane >> 4), so the four instructions of a 16-voxel row consume
+// whole cache lines, and waves run without barriers after the one staging
+// step.  D goes to the particle's slot of the region (the CSR position of the
+// pair); the per-particle sum over its regions happens in gather_kernel, in a
+// fixed order.
+template <typename R, int RX, int ZSB>
 __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g, const int* __restrict__ zlist,
                                                             const int* __restrict__ base,
-                                                            const float* __restrict__ wts,
+                                                            const R* __restrict__ wts,
                                                             const int* __restrict__ reg_start,
                                                             const int* __restrict__ reg_items, int nry, int N,
-                                                            const float2* __restrict__ grid,
-                                                            float* __restrict__ partial) {
+                                                            const typename C2<R>::T* __restrict__ grid,
+                                                            R* __restrict__ partial) {
+    using M = Mfma<R>;
     constexpr int RY = kMfmaRY, KC = kMfmaKC;
     using L = RegionW<RX, ZSB>;
     constexpr int NB = KC / 16;  // candidate blocks of 16
-    __shared__ __align__(16) float s_w[KC][L::NW];
+    __shared__ __align__(16) R s_w[KC][L::NW];
+    __shared__ int s_pid[KC];
+    __shared__ int s_org[KC][3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int reg = blockIdx.x;
     const int x0 = (reg / nry) * RX, y0 = (reg % nry) * RY;
@@ -1425,12 +1495,12 @@ __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g,
     // this lane's B-operand column and its rows of the region
     const int qc = wcol0 + (lane & 15);
     const bool cvalid = qc < ncols;
-    const float* gcol = nullptr;
+    const R* gcol = nullptr;
     if (cvalid) {
         const int b = qc / 6, c = (qc >> 1) % 3;
-        gcol = reinterpret_cast<const float*>(grid + ((size_t)b * 3 + c) * Zs * nxy) + (qc & 1);
+        gcol = reinterpret_cast<const R*>(grid + ((size_t)b * 3 + c) * Zs * nxy) + (qc & 1);
     }
-    int goff[RX][4];  // float offset of (x0 + xi, y0 + 4 u + lane / 16) in a plane, -1 outside
+    int goff[RX][4];  // offset of (x0 + xi, y0 + 4 u + lane / 16) in a plane (in R), -1 outside
 #pragma unroll
     for (int xi = 0; xi < RX; ++xi)
 #pragma unroll
@@ -1443,11 +1513,11 @@ __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g,
         const int cnt = min(KC, k1 - c0);
         const int nblk = (cnt + 15) >> 4;
         __syncthreads();
-        stage_region_weights<RX, ZSB>(&s_w[0][0], cnt, x0, y0, g, zlist, base, wts, reg_items + c0, N);
-        __syncthreads();
+        stage_candidates<R, RX, ZSB>(s_pid, s_org, &s_w[0][0], cnt, x0, y0, g, zlist, base, wts, reg_items + c0,
+                                     N);
         if (wcol0 >= ncols) continue;
         // per-lane weights: candidate k = 16 nb + (lane & 15) at y = 4 u + (lane >> 4), x = x0 + xi
-        float wyl[NB][4], wxl[NB][RX];
+        R wyl[NB][4], wxl[NB][RX];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
             const int k = nb * 16 + (lane & 15);
@@ -1456,17 +1526,17 @@ __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g,
 #pragma unroll
             for (int xi = 0; xi < RX; ++xi) wxl[nb][xi] = s_w[k][L::X + xi];
         }
-        mpm_f32x4 acc[NB];
+        typename M::V4 acc[NB];
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) acc[nb] = mpm_f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int nb = 0; nb < NB; ++nb) acc[nb] = typename M::V4{R(0), R(0), R(0), R(0)};
         for (int p = 0; p < Zs; ++p) {
-            const float* gp = gcol ? gcol + 2 * (size_t)p * nxy : nullptr;
-            float gv[RX][4];
+            const R* gp = gcol ? gcol + 2 * (size_t)p * nxy : nullptr;
+            R gv[RX][4];
 #pragma unroll
             for (int xi = 0; xi < RX; ++xi)
 #pragma unroll
-                for (int u = 0; u < 4; ++u) gv[xi][u] = goff[xi][u] >= 0 ? gp[goff[xi][u]] : 0.f;
-            float wz[NB];
+                for (int u = 0; u < 4; ++u) gv[xi][u] = goff[xi][u] >= 0 ? gp[goff[xi][u]] : R(0);
+            R wz[NB];
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) wz[nb] = s_w[nb * 16 + (lane & 15)][L::Z + p];
 #pragma unroll
@@ -1474,21 +1544,19 @@ __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g,
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb) {
                     if (nb < nblk) {
-                        const float wxz = wxl[nb][xi] * wz[nb];
+                        const R wxz = wxl[nb][xi] * wz[nb];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u)
-                            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(wyl[nb][u] * wxz, gv[xi][u], acc[nb],
-                                                                           0, 0, 0);
+                        for (int u = 0; u < 4; ++u) acc[nb] = M::mac(wyl[nb][u] * wxz, gv[xi][u], acc[nb]);
                     }
                 }
         }
-        // lane holds D[16 nb + 4 (lane >> 4) + r][lane & 15]
+        // lane holds D[16 nb + drow(lane >> 4, r)][lane & 15]
         if (cvalid) {
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int k = nb * 16 + 4 * (lane >> 4) + r;
+                    const int k = nb * 16 + M::drow(lane >> 4, r);
                     if (k < cnt) partial[(size_t)(c0 + k) * ncols + qc] = acc[nb][r];
                 }
         }
@@ -1734,7 +1802,7 @@ __global__ __launch_bounds__(256) void gather_kernel(
     const int* __restrict__ zmap, const int* __restrict__ base, const R* __restrict__ wts, int N, double h3,
     const double* __restrict__ pos, const int* __restrict__ nb_start, const int* __restrict__ nb_j,
     const double2* __restrict__ nb_f, double Lx, double Ly, double Lz, double kbx, double kby,
-    const double2* __restrict__ selfc, double self_perp, int jacobi, const float* __restrict__ partial,
+    const double2* __restrict__ selfc, double self_perp, int jacobi, const R* __restrict__ partial,
     const int* __restrict__ ps_start, const int* __restrict__ ps_items) {
     __shared__ int s_off[kMaxWidth * kMaxWidth * kMaxWidth];
     const int lane = threadIdx.x & 63;
@@ -1790,7 +1858,7 @@ __global__ __launch_bounds__(256) void gather_kernel(
             // particle's region slots (ascending region order)
             const int s1 = partial ? ps_start[i + 1] : 0;
             for (int t = (partial ? ps_start[i] : 0) + lane; t < s1; t += 64) {
-                const float* pp = partial + (size_t)ps_items[t] * (6 * B);
+                const R* pp = partial + (size_t)ps_items[t] * (6 * B);
 #pragma unroll
                 for (int j = 0; j < G; ++j) {
                     if (g0 + j >= B) break;
@@ -2124,6 +2192,7 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         d_ps_start.upload(pss.data(), pss.size());
         d_ps_items.upload(psi.data(), psi.size());
         d_partial.resize((size_t)std::max(rs[nreg], 1) * 6 * kMaxSlots);
+        d_partial64.resize((size_t)std::max(rs[nreg], 1) * 6 * kMaxSlots);
     }
     std::vector<int> ztl;  // z-tiles that hold at least one computed voxel
     for (int t = 0; t < ntiles[2]; ++t) {
@@ -2324,15 +2393,14 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
     g.Zs = Zs;
     const int Ni = (int)N;
     if (parts & MPM_PART_RECIP) {
-        if (SLAB && fp32 && reg_rx > 0) {
+        if (SLAB && reg_rx > 0) {
             ProfScope ps("spread", s);
-            const dim3 gs((unsigned)(reg_nrx * reg_nry), (unsigned)((6 * B + 63) / 64));
-            const float* w32 = d_w32.p;
-            float2* g32 = reinterpret_cast<float2*>(grid);
-#define MPM_SM(RXV, ZSV)                                                                                       \
-    case ZSV:                                                                                                  \
-        hipLaunchKernelGGL((spread_mfma_kernel<RXV, ZSV>), gs, dim3(256), 0, s, t, B, g, d_zlist.p, d_base.p, \
-                           w32, d_reg_start.p, d_reg_items.p, reg_nry, Ni, g32);                              \
+            constexpr int CPB = sizeof(R) == 4 ? 64 : 32;
+            const dim3 gs((unsigned)(reg_nrx * reg_nry), (unsigned)((6 * B + CPB - 1) / CPB));
+#define MPM_SM(RXV, ZSV)                                                                                          \
+    case ZSV:                                                                                                     \
+        hipLaunchKernelGGL((spread_mfma_kernel<R, RXV, ZSV>), gs, dim3(256), 0, s, t, B, g, d_zlist.p, d_base.p, \
+                           w, d_reg_start.p, d_reg_items.p, reg_nry, Ni, grid);                                  \
         break;
             switch (zs_bucket(Zs)) {
                 MPM_SM(4, 4)
@@ -2443,16 +2511,15 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
                 fft->exec(grid, bytes_per_rhs, B, fp32, false, s);
         }
     }
-    const float* part = nullptr;
-    if (SLAB && fp32 && reg_rx > 0 && (parts & MPM_PART_RECIP)) {
+    const R* part = nullptr;
+    if (SLAB && reg_rx > 0 && (parts & MPM_PART_RECIP)) {
         ProfScope ps("gather", s);
         const dim3 gs((unsigned)(reg_nrx * reg_nry), (unsigned)((6 * B + 63) / 64));
-        const float* w32 = d_w32.p;
-        const float2* g32 = reinterpret_cast<const float2*>(grid);
-#define MPM_CM(RXV, ZSV)                                                                                          \
-    case ZSV:                                                                                                     \
-        hipLaunchKernelGGL((contract_mfma_kernel<RXV, ZSV>), gs, dim3(256), 0, s, B, g, d_zlist.p, d_base.p, w32, \
-                           d_reg_start.p, d_reg_items.p, reg_nry, Ni, g32, d_partial.p);                         \
+        R* pbuf = fp32 ? reinterpret_cast<R*>(d_partial.p) : reinterpret_cast<R*>(d_partial64.p);
+#define MPM_CM(RXV, ZSV)                                                                                     \
+    case ZSV:                                                                                                \
+        hipLaunchKernelGGL((contract_mfma_kernel<R, RXV, ZSV>), gs, dim3(256), 0, s, B, g, d_zlist.p, d_base.p, \
+                           w, d_reg_start.p, d_reg_items.p, reg_nry, Ni, grid, pbuf);                        \
         break;
         switch (zs_bucket(Zs)) {
             MPM_CM(4, 4)
@@ -2467,7 +2534,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
         }
 #undef MPM_CM
         MPM_HIP(hipGetLastError());
-        part = d_partial.p;
+        part = pbuf;
     }
     {
         ProfScope ps("gather", s);

@@ -525,7 +525,12 @@ static void gmres_group(Ctx& C, const double2* b, double2* x, int B, mpm_solve_r
         if (!done[q]) active.push_back(q);
 
     const double tol = d.tol;
+    // REFINED: the initial residual may carry the FP32 operator's error (the
+    // end-of-cycle residual is FP64 and alone decides convergence), so the first
+    // outer residual is FP32 and is never taken as converged
+    bool first = true;
     while (!active.empty()) {
+        const bool res32 = outer32 || (inner32 && first);
         // ---- outer residual r = M^-1 (b - A x) ----
         {
             std::vector<const double2*> xs;
@@ -534,10 +539,10 @@ static void gmres_group(Ctx& C, const double2* b, double2* x, int B, mpm_solve_r
                 xs.push_back(x + q * n);
                 ys.push_back(W.w.p + q * n);
             }
-            C.apply(active, xs, ys, outer32);
+            C.apply(active, xs, ys, res32);
             for (int q : active) {
                 matvecs[q]++;
-                if (!outer32) mv64[q]++;
+                if (!res32) mv64[q]++;
             }
             Ptrs p = make_ptrs(active, [&](int q) { return W.w.p + q * n; }, [&](int q) { return Vk(q, 0); });
             hipLaunchKernelGGL(residual_kernel, grid_for(n, (int)active.size()), dim3(256), 0, C.s, p, n,
@@ -559,8 +564,8 @@ static void gmres_group(Ctx& C, const double2* b, double2* x, int B, mpm_solve_r
                 const double rn = std::sqrt(rn2[q]);
                 last_pre[q] = rn / bnorm[q];
                 last_true[q] = std::sqrt(tn2[q]) / bnorm[q];
-                fresh64[q] = !outer32;
-                if (rn < tol * bnorm[q]) {
+                fresh64[q] = !res32;
+                if (rn < tol * bnorm[q] && (!(res32 && !outer32) || rn == 0.0)) {
                     done[q] = 1;
                     conv[q] = 1;
                 } else if (iters[q] >= d.maxiter) {
@@ -570,6 +575,7 @@ static void gmres_group(Ctx& C, const double2* b, double2* x, int B, mpm_solve_r
                 }
             }
             active.swap(still);
+            first = false;
             if (active.empty()) break;
             // V_0 = r / ||r||, g = (||r||, 0, ...)
             Ptrs pv = make_ptrs(active, [&](int q) { return Vk(q, 0); }, [&](int q) { return Vk(q, 0); });

@@ -170,14 +170,15 @@ class EwaldOp {
     DevBuf<double2> d_selfc;     // [rows][N]
     bool zcontig = false;        // slab planes form one cyclic run (zlist[a] = z0 + a mod nz)
     bool kt = false;             // slab k-space data in its own buffer, columns ordered ky * nx + kx
-    // matrix-core spread (slab, FP32): xy-regions of reg_rx x 16 voxels and the
+    // matrix-core spread and contraction (slab): xy-regions of reg_rx x 16 voxels and the
     // candidate particles of each region (CSR, ascending ids)
     int reg_rx = 0, reg_nrx = 0, reg_nry = 0;
     DevBuf<int> d_reg_start;
     DevBuf<int> d_reg_items;
     DevBuf<int> d_ps_start;      // [N + 1] region slots of each particle
     DevBuf<int> d_ps_items;      // slot ids (CSR positions of d_reg_items)
-    DevBuf<float> d_partial;     // [slot][6 B] per-region contractions
+    DevBuf<float> d_partial;     // [slot][6 B] per-region contractions (FP32, FP64)
+    DevBuf<double> d_partial64;
     // k-space tables: (khat x, y, z, scale) per voxel for the full grid and the
     // z-DFT slab kernel, or the Toeplitz plane coupling [Zs][6][nx*ny] for a
     // contiguous slab; kspace_key records which one is current

@@ -48,7 +48,8 @@ typedef enum {
 typedef enum {
     MPM_SOLVE_DOUBLE = 0,  /* every apply FP64 (reference precision="double") */
     MPM_SOLVE_MIXED = 1,   /* every apply FP32-reciprocal (reference AUTO/MIXED) */
-    MPM_SOLVE_REFINED = 2  /* FP32-reciprocal Arnoldi applies, FP64 residual b - Ax at every restart */
+    MPM_SOLVE_REFINED = 2  /* FP32-reciprocal Arnoldi applies and initial residual; FP64 residual b - Ax at
+                              the end of every cycle decides convergence */
 } mpm_solve_precision;
 
 typedef enum { MPM_GMRES = 0, MPM_BICGSTAB = 1 } mpm_method;

@@ -208,6 +208,14 @@ def test_refined_precision_reaches_fp64_accuracy():
         assert rr[q]["converged"] and rr[q]["relres_pre"] < 1e-8, rr[q]
         assert rel(xr[q], x64[q]) < 1e-6
         assert rr[q]["matvecs_fp64"] < rr[q]["matvecs"]
+    # non-zero initial guesses: a perturbed solution, and the FP64 solution itself
+    # (its FP32 initial residual is not below tol, its FP64 end-of-cycle one is)
+    rng = np.random.default_rng(11)
+    for x0 in (x64 * (1 + 1e-3 * rng.standard_normal(x64.shape)), x64):
+        xg, rg = op.solve(b, x0=x0, tol=1e-8, restart=30, precision=C.SOLVE_REFINED)
+        for q in range(4):
+            assert rg[q]["converged"] and rg[q]["relres_pre"] < 1e-8, rg[q]
+            assert rel(xg[q], x64[q]) < 1e-6
 
 
 def test_zero_rhs_returns_zero():
