@@ -1475,19 +1475,24 @@ __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g,
                                                             const int* __restrict__ reg_start,
                                                             const int* __restrict__ reg_items, int nry, int N,
                                                             const typename C2<R>::T* __restrict__ grid,
-                                                            R* __restrict__ partial) {
+                                                            R* __restrict__ partial, int ncg) {
     using M = Mfma<R>;
+    using V4 = typename M::V4;
     constexpr int RY = kMfmaRY, KC = kMfmaKC;
     using L = RegionW<RX, ZSB>;
     constexpr int NB = KC / 16;  // candidate blocks of 16
     __shared__ __align__(16) R s_w[KC][L::NW];
     __shared__ int s_pid[KC];
     __shared__ int s_org[KC][3];
+    __shared__ V4 s_red[4][NB][64];  // per-wave partial accumulators
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // 4 waves = ncg column groups x (4 / ncg) plane groups
+    const int npg = 4 / ncg, cg = wave % ncg, pg = wave / ncg;
     const int reg = blockIdx.x;
     const int x0 = (reg / nry) * RX, y0 = (reg % nry) * RY;
-    const int wcol0 = blockIdx.y * 64 + wave * 16;
+    const int wcol0 = (blockIdx.y * ncg + cg) * 16;
     const int ncols = 6 * B;
+    const bool active = wcol0 < ncols;
     const int Zs = g.Zs;
     const size_t nxy = (size_t)g.nx * g.ny;
     const int k0 = reg_start[reg], k1 = reg_start[reg + 1];
@@ -1515,43 +1520,55 @@ __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g,
         __syncthreads();
         stage_candidates<R, RX, ZSB>(s_pid, s_org, &s_w[0][0], cnt, x0, y0, g, zlist, base, wts, reg_items + c0,
                                      N);
-        if (wcol0 >= ncols) continue;
-        // per-lane weights: candidate k = 16 nb + (lane & 15) at y = 4 u + (lane >> 4), x = x0 + xi
-        R wyl[NB][4], wxl[NB][RX];
+        V4 acc[NB];
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            const int k = nb * 16 + (lane & 15);
+        for (int nb = 0; nb < NB; ++nb) acc[nb] = V4{R(0), R(0), R(0), R(0)};
+        if (active) {
+            // per-lane weights: candidate k = 16 nb + (lane & 15) at y = 4 u + (lane >> 4), x = x0 + xi
+            R wyl[NB][4], wxl[NB][RX];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) wyl[nb][u] = s_w[k][L::Y + 4 * u + (lane >> 4)];
+            for (int nb = 0; nb < NB; ++nb) {
+                const int k = nb * 16 + (lane & 15);
 #pragma unroll
-            for (int xi = 0; xi < RX; ++xi) wxl[nb][xi] = s_w[k][L::X + xi];
-        }
-        typename M::V4 acc[NB];
+                for (int u = 0; u < 4; ++u) wyl[nb][u] = s_w[k][L::Y + 4 * u + (lane >> 4)];
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) acc[nb] = typename M::V4{R(0), R(0), R(0), R(0)};
-        for (int p = 0; p < Zs; ++p) {
-            const R* gp = gcol ? gcol + 2 * (size_t)p * nxy : nullptr;
-            R gv[RX][4];
+                for (int xi = 0; xi < RX; ++xi) wxl[nb][xi] = s_w[k][L::X + xi];
+            }
+            for (int p = pg; p < Zs; p += npg) {
+                const R* gp = gcol ? gcol + 2 * (size_t)p * nxy : nullptr;
+                R gv[RX][4];
 #pragma unroll
-            for (int xi = 0; xi < RX; ++xi)
+                for (int xi = 0; xi < RX; ++xi)
 #pragma unroll
-                for (int u = 0; u < 4; ++u) gv[xi][u] = goff[xi][u] >= 0 ? gp[goff[xi][u]] : R(0);
-            R wz[NB];
+                    for (int u = 0; u < 4; ++u) gv[xi][u] = goff[xi][u] >= 0 ? gp[goff[xi][u]] : R(0);
+                R wz[NB];
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) wz[nb] = s_w[nb * 16 + (lane & 15)][L::Z + p];
+                for (int nb = 0; nb < NB; ++nb) wz[nb] = s_w[nb * 16 + (lane & 15)][L::Z + p];
 #pragma unroll
-            for (int xi = 0; xi < RX; ++xi)
+                for (int xi = 0; xi < RX; ++xi)
 #pragma unroll
-                for (int nb = 0; nb < NB; ++nb) {
-                    if (nb < nblk) {
-                        const R wxz = wxl[nb][xi] * wz[nb];
+                    for (int nb = 0; nb < NB; ++nb) {
+                        if (nb < nblk) {
+                            const R wxz = wxl[nb][xi] * wz[nb];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) acc[nb] = M::mac(wyl[nb][u] * wxz, gv[xi][u], acc[nb]);
+                            for (int u = 0; u < 4; ++u) acc[nb] = M::mac(wyl[nb][u] * wxz, gv[xi][u], acc[nb]);
+                        }
                     }
-                }
+            }
+        }
+        // plane groups of one column group are summed in plane-group order
+        if (npg > 1) {
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) s_red[wave][nb][lane] = acc[nb];
+            __syncthreads();
+            if (pg == 0) {
+                for (int q = 1; q < npg; ++q)
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) acc[nb] += s_red[q * ncg + cg][nb][lane];
+            }
         }
         // lane holds D[16 nb + drow(lane >> 4, r)][lane & 15]
-        if (cvalid) {
+        if (pg == 0 && cvalid) {
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
@@ -2514,12 +2531,16 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
     const R* part = nullptr;
     if (SLAB && reg_rx > 0 && (parts & MPM_PART_RECIP)) {
         ProfScope ps("gather", s);
-        const dim3 gs((unsigned)(reg_nrx * reg_nry), (unsigned)((6 * B + 63) / 64));
+        // column groups of 16 per block: 4 when there are enough right-hand sides,
+        // else fewer, with the slab planes split over the remaining waves
+        const int ngrp = (6 * B + 15) / 16;
+        const int ncg = ngrp >= 4 ? 4 : (ngrp >= 2 ? 2 : 1);
+        const dim3 gs((unsigned)(reg_nrx * reg_nry), (unsigned)((ngrp + ncg - 1) / ncg));
         R* pbuf = fp32 ? reinterpret_cast<R*>(d_partial.p) : reinterpret_cast<R*>(d_partial64.p);
 #define MPM_CM(RXV, ZSV)                                                                                     \
     case ZSV:                                                                                                \
         hipLaunchKernelGGL((contract_mfma_kernel<R, RXV, ZSV>), gs, dim3(256), 0, s, B, g, d_zlist.p, d_base.p, \
-                           w, d_reg_start.p, d_reg_items.p, reg_nry, Ni, grid, pbuf);                        \
+                           w, d_reg_start.p, d_reg_items.p, reg_nry, Ni, grid, pbuf, ncg);                   \
         break;
         switch (zs_bucket(Zs)) {
             MPM_CM(4, 4)
