@@ -1999,6 +1999,97 @@ __global__ __launch_bounds__(256) void gather_kernel(
     }
 }
 
+// Thread per (particle i, right-hand side b), b fastest: the reciprocal part
+// as the sum of the particle's region contractions (slot order), the real-space
+// part over the neighbour list with the precomputed pair coefficients
+// (ux, uy, uz, perp, para - perp, cos, sin) -- the field of p_j at i is
+//   perp p_j + (para - perp) u (u . p_j), times the Bloch phase --
+// then the self term and the optional left Jacobi scaling.  Used with the
+// matrix-core contraction when the batch is wide enough to fill the GPU.
+template <typename R>
+__global__ __launch_bounds__(256) void finish_kernel(SlotTable tab, int B, int parts, int N,
+                                                     const R* __restrict__ partial, const int* __restrict__ ps_start,
+                                                     const int* __restrict__ ps_items, double h3,
+                                                     const int* __restrict__ nb_start, const int* __restrict__ nb_j,
+                                                     const double* __restrict__ pairc,
+                                                     const double2* __restrict__ selfc, double self_perp,
+                                                     int jacobi) {
+    __shared__ const double2* s_x[kMaxSlots];
+    __shared__ double2* s_y[kMaxSlots];
+    __shared__ int s_row[kMaxSlots];
+#pragma unroll
+    for (int k = 0; k < kMaxSlots; ++k)  // uniform indices: no per-lane walk of the argument
+        if (threadIdx.x == k) {
+            s_x[k] = tab.x[k];
+            s_y[k] = tab.y[k];
+            s_row[k] = tab.row[k];
+        }
+    __syncthreads();
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / B, b = t - i * B;
+    if (i >= N) return;
+    const double2* xb = s_x[b];
+    double ar[3] = {0.0, 0.0, 0.0}, ai[3] = {0.0, 0.0, 0.0};
+    if (parts & MPM_PART_RECIP) {
+        const int s0 = ps_start[i], s1 = ps_start[i + 1];
+        for (int sl = s0; sl < s1; ++sl) {
+            const R* pp = partial + (size_t)ps_items[sl] * (6 * B) + 6 * b;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                ar[c] += (double)pp[2 * c];
+                ai[c] += (double)pp[2 * c + 1];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            ar[c] *= h3;
+            ai[c] *= h3;
+        }
+    }
+    if (parts & MPM_PART_REAL) {
+        const int k0 = nb_start[i], k1 = nb_start[i + 1];
+#pragma unroll 4
+        for (int k = k0; k < k1; ++k) {
+            const int j = nb_j[k];
+            const double* pc = pairc + 8 * (size_t)k;
+            const double ux = pc[0], uy = pc[1], uz = pc[2], cp = pc[3], cd = pc[4], ec = pc[5], es = pc[6];
+            const double2 p0 = xb[3 * (size_t)j], p1 = xb[3 * (size_t)j + 1], p2 = xb[3 * (size_t)j + 2];
+            const double rr = cd * (p0.x * ux + p1.x * uy + p2.x * uz);
+            const double ri = cd * (p0.y * ux + p1.y * uy + p2.y * uz);
+            const double u[3] = {ux, uy, uz};
+            const double2 pp[3] = {p0, p1, p2};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double er = cp * pp[c].x + rr * u[c];
+                const double ei = cp * pp[c].y + ri * u[c];
+                ar[c] += er * ec - ei * es;
+                ai[c] += er * es + ei * ec;
+            }
+        }
+    }
+    const double2* p = xb + 3 * (size_t)i;
+    double2 fs = make_double2(0.0, 0.0);
+    if (parts & MPM_PART_REAL) {
+        fs = selfc[(size_t)s_row[b] * N + i];
+        fs.x += self_perp;
+    }
+    // optional left Jacobi: y <- y / self_coef (numerical_solver.cu:207-242)
+    double2 inv = make_double2(1.0, 0.0);
+    if (jacobi) {
+        const double2 sc = selfc[(size_t)s_row[b] * N + i];
+        const double den = sc.x * sc.x + sc.y * sc.y;
+        if (den > 1e-30) inv = make_double2(sc.x / den, -sc.y / den);
+    }
+    double2* y = s_y[b] + 3 * (size_t)i;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double2 pc = p[c];
+        const double yr = ar[c] + fs.x * pc.x - fs.y * pc.y;
+        const double yi = ai[c] + fs.x * pc.y + fs.y * pc.x;
+        y[c] = make_double2(yr * inv.x - yi * inv.y, yr * inv.y + yi * inv.x);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // EwaldOp
 // ---------------------------------------------------------------------------
@@ -2076,6 +2167,32 @@ void EwaldOp::set_bloch(double kx, double ky) {
     kb[0] = kx;
     kb[1] = ky;
     build_kspace();
+    if (N > 0) build_pair_coefs();
+}
+
+void EwaldOp::build_pair_coefs() {
+    const size_t M = h_pair_f.size();
+    std::vector<double> c(std::max<size_t>(M, 1) * 8, 0.0);
+    const bool bloch = kb[0] != 0.0 || kb[1] != 0.0;
+    for (size_t k = 0; k < M; ++k) {
+        const double* r = &h_pair_r[3 * k];
+        const double d = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        const double de = d < 1.0 ? 1.0 : d;
+        double* o = &c[8 * k];
+        o[0] = r[0] / de;
+        o[1] = r[1] / de;
+        o[2] = r[2] / de;
+        o[3] = h_pair_f[k].x;
+        o[4] = h_pair_f[k].y - h_pair_f[k].x;
+        o[5] = 1.0;
+        o[6] = 0.0;
+        if (bloch) {
+            const double ph = -(kb[0] * r[0] + kb[1] * r[1]);
+            o[5] = std::cos(ph);
+            o[6] = std::sin(ph);
+        }
+    }
+    d_pairc.upload(c.data(), c.size());
 }
 
 // LDS bytes of zfused_kernel for slab size zs_pad and precision size rs.
@@ -2251,8 +2368,10 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     }
     std::vector<int> nstart(Nn + 1, 0), nj;
     std::vector<double2> nf;
+    std::vector<double> pr;
     nj.reserve((size_t)Nn * 64);
     nf.reserve((size_t)Nn * 64);
+    pr.reserve((size_t)Nn * 64 * 3);
     auto consider = [&](int i, int j) {
         double r[3] = {pos[i] - pos[j], pos[Nn + i] - pos[Nn + j], pos[2 * Nn + i] - pos[2 * Nn + j]};
         for (int d = 0; d < 3; ++d) r[d] -= L[d] * std::round(r[d] / L[d]);
@@ -2263,6 +2382,10 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         const double de = dist < 1.0 ? 1.0 : dist;
         nj.push_back(j);
         nf.push_back(make_double2(tab.interp(tab.perp, de), tab.interp(tab.para, de)));
+        // x_j - x_i, the orientation the gather uses
+        double rj[3] = {pos[j] - pos[i], pos[Nn + j] - pos[Nn + i], pos[2 * Nn + j] - pos[2 * Nn + i]};
+        for (int d = 0; d < 3; ++d) rj[d] -= L[d] * std::round(rj[d] / L[d]);
+        pr.insert(pr.end(), rj, rj + 3);
     };
     for (int i = 0; i < Nn; ++i) {
         if (cells) {
@@ -2298,8 +2421,12 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         nj.push_back(0);
         nf.push_back(make_double2(0, 0));
     }
+    h_pair_r = std::move(pr);
+    h_pair_f = nf;
+    h_pair_f.resize(num_pairs);
     d_nb_j.upload(nj.data(), nj.size());
     d_nb_f.upload(nf.data(), nf.size());
+    build_pair_coefs();
     // FFT plans for the chosen layout
     const int key = slab ? Zs : -1;
     if (!lfft || fft_key != key) {
@@ -2557,7 +2684,14 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
         MPM_HIP(hipGetLastError());
         part = pbuf;
     }
-    {
+    if (part && B >= 4) {
+        ProfScope ps("gather", s);
+        const double h3 = S.h[0] * S.h[1] * S.h[2];
+        const int blocks = (int)(((size_t)Ni * B + 255) / 256);
+        hipLaunchKernelGGL((finish_kernel<R>), dim3(blocks), dim3(256), 0, s, t, B, parts, Ni, part, d_ps_start.p,
+                           d_ps_items.p, h3, d_nb_start.p, d_nb_j.p, d_pairc.p, d_selfc.p, self_perp, jacobi);
+        MPM_HIP(hipGetLastError());
+    } else {
         ProfScope ps("gather", s);
         const double h3 = S.h[0] * S.h[1] * S.h[2];
         const int blocks = (Ni + 3) / 4;

@@ -167,6 +167,13 @@ class EwaldOp {
     DevBuf<int> d_nb_start;      // [N + 1] target-centric CSR
     DevBuf<int> d_nb_j;          // [M]
     DevBuf<double2> d_nb_f;      // [M] (perp, para)
+    // per pair, for the thread-per-(particle, RHS) finish: (ux, uy, uz, perp,
+    // para - perp, cos, sin, 0) with u = r / max(|r|, 1), r = x_j - x_i (min
+    // image) and the Bloch phase -(kb . r); rebuilt when positions or kb change
+    std::vector<double> h_pair_r;  // [M][3]
+    std::vector<double2> h_pair_f;
+    DevBuf<double> d_pairc;        // [M][8]
+    void build_pair_coefs();
     DevBuf<double2> d_selfc;     // [rows][N]
     bool zcontig = false;        // slab planes form one cyclic run (zlist[a] = z0 + a mod nz)
     bool kt = false;             // slab k-space data in its own buffer, columns ordered ky * nx + kx

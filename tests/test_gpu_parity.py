@@ -76,7 +76,9 @@ def test_apply_fp64_matches_oracle(name, layout, fft):
     _case(f"apply fp64 {name} layout={layout} fft={fft} N={len(pos)}")
     N = len(pos)
     rng = np.random.default_rng(11)
-    B = 3
+    # 12 right-hand sides: wide-batch kernels (four column groups, thread-per-RHS
+    # finish); the first 3 again on their own: narrow-batch kernels
+    B = 12
     p = rng.standard_normal((B, N, 3)) + 1j * rng.standard_normal((B, N, 3))
     rows = rng.standard_normal((B, N)) + 1j * rng.standard_normal((B, N))
     op = Op(box, tol, xi, grid, P, layout=layout, fft_backend=fft)
@@ -94,12 +96,15 @@ def test_apply_fp64_matches_oracle(name, layout, fft):
     assert info.num_pairs == ref_op.info()["num_pairs"]
     for parts in (C.PART_REAL, C.PART_RECIP, C.PART_ALL):
         got = op.apply(p, C.PREC_DOUBLE, parts)
+        got3 = op.apply(p[:3], C.PREC_DOUBLE, parts)
         for b in range(B):
             ref = ref_op.apply(p[b], rows[b], parts)
             if np.abs(ref).max() == 0:
                 assert np.abs(got[b]).max() == 0
                 continue
             assert rel(got[b], ref) < 1e-11, (name, parts, b, rel(got[b], ref))
+            if b < 3:
+                assert rel(got3[b], ref) < 1e-11, (name, parts, b, rel(got3[b], ref))
 
 
 @pytest.mark.parametrize("name,layout,fft", [("random3d", 0, 0), ("hex_slab", 0, 0), ("hex_slab", 1, 0),
@@ -128,17 +133,22 @@ def test_bloch_wavevector_matches_oracle():
     _case("bloch")
     N = len(pos)
     rng = np.random.default_rng(13)
-    p = rng.standard_normal((1, N, 3)) + 1j * rng.standard_normal((1, N, 3))
+    p = rng.standard_normal((6, N, 3)) + 1j * rng.standard_normal((6, N, 3))
     sc = rng.standard_normal(N) + 1j * rng.standard_normal(N)
     op = Op(box, tol, xi)
     op.set_positions(pos)
     op.set_self_coef(sc)
-    op.set_bloch(0.15, -0.22)
     ref_op = O.Operator(box, tol, xi)
     ref_op.set_positions(pos)
-    ref_op.set_bloch(0.15, -0.22)
-    got = op.apply(p)
-    assert rel(got[0], ref_op.apply(p[0], sc)) < 1e-11
+    for kb in ((0.15, -0.22), (-0.05, 0.31)):  # the second call re-phases the pair table
+        op.set_bloch(*kb)
+        ref_op.set_bloch(*kb)
+        got1 = op.apply(p[:1])
+        got6 = op.apply(p)
+        for b in range(6):
+            ref = ref_op.apply(p[b], sc)
+            assert rel(got6[b], ref) < 1e-11, (kb, b)
+        assert rel(got1[0], ref_op.apply(p[0], sc)) < 1e-11
 
 
 @pytest.mark.parametrize("method", [C.GMRES, C.BICGSTAB])
