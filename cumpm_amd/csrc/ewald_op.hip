@@ -688,8 +688,8 @@ __device__ inline int line_base32(const LineGeom& g, int l) {
 // input geometry and written with the output geometry (in place when both
 // buffers and geometries agree; the slab's y pass writes transposed).
 template <typename R, bool INV, int CLS>
-__global__ __launch_bounds__(256) void line_fft_kernel(const typename C2<R>::T* __restrict__ in,
-                                                       typename C2<R>::T* __restrict__ out, LineGeom gin,
+__global__ __launch_bounds__(256) void line_fft_kernel(const typename C2<R>::T* in, typename C2<R>::T* out,
+                                                       LineGeom gin,
                                                        LineGeom gout, LinePlan pl,
                                                        const typename C2<R>::T* __restrict__ twiddle, int TL) {
     using CT = typename C2<R>::T;
@@ -765,6 +765,132 @@ __global__ __launch_bounds__(256) void line_fft_kernel(const typename C2<R>::T* 
     }
 }
 
+// Two-stage transform n = P1 P2 (the common case: 210 = 14 x 15, 114 = 19 x 6)
+// with one LDS round trip: stage 1 (radix P1, no twiddles) reads its inputs
+// x[j + r P2] straight from global memory and leaves y[j P1 + q] in LDS; stage
+// 2 (radix P2) reads y[j' + r P1], applies W_n^{j' r} and writes X[j' + q P1]
+// straight to global memory.  Work items are mapped line-fastest when the
+// global side of a stage is strided (coalesced across lines), element-fastest
+// when it is contiguous.
+template <typename R, int P, bool INV>
+__device__ __forceinline__ void fft2_s1(const typename C2<R>::T* in, int ib, int istride,
+                                        typename C2<R>::T* y, int yb) {
+    using CT = typename C2<R>::T;
+    CT v[P];
+#pragma unroll
+    for (int r = 0; r < P; ++r) v[r] = in[ib + r * istride];
+    dft_reg<R, P, INV>(v);
+#pragma unroll
+    for (int q = 0; q < P; ++q) y[yb + q] = v[q];
+}
+
+template <typename R, int P, bool INV>
+__device__ __forceinline__ void fft2_s2(const typename C2<R>::T* y, int yb, int ystride,
+                                        const typename C2<R>::T* tw, int a, typename C2<R>::T* out,
+                                        int ob, int ostride) {
+    using CT = typename C2<R>::T;
+    CT v[P];
+#pragma unroll
+    for (int r = 0; r < P; ++r) v[r] = y[yb + r * ystride];
+    if (a > 0) {
+#pragma unroll
+        for (int r = 1; r < P; ++r) v[r] = cmul_t(v[r], tw[a * r]);
+    }
+    dft_reg<R, P, INV>(v);
+#pragma unroll
+    for (int q = 0; q < P; ++q) out[ob + q * ostride] = v[q];
+}
+
+#define MPM_FFT2_CASES(CALL)   \
+    CALL(2, 0)                 \
+    CALL(3, 0)                 \
+    CALL(4, 0)                 \
+    CALL(5, 0)                 \
+    CALL(6, 0)                 \
+    CALL(7, 0)                 \
+    CALL(8, 0)                 \
+    CALL(9, 1)                 \
+    CALL(10, 1)                \
+    CALL(11, 1)                \
+    CALL(12, 1)                \
+    CALL(13, 1)                \
+    CALL(14, 1)                \
+    CALL(15, 1)                \
+    CALL(16, 1)                \
+    CALL(17, 2)                \
+    CALL(19, 2)                \
+    CALL(23, 2)
+
+template <typename R, bool INV, int CLS>
+__global__ __launch_bounds__(256) void line_fft2_kernel(const typename C2<R>::T* in, typename C2<R>::T* out,
+                                                        LineGeom gin,
+                                                        LineGeom gout, int n, int P1, int P2,
+                                                        const typename C2<R>::T* __restrict__ twiddle, int TL) {
+    using CT = typename C2<R>::T;
+    extern __shared__ unsigned char s_raw[];
+    CT* const y = reinterpret_cast<CT*>(s_raw);
+    CT* tw = y + TL * n;
+    const int tid = threadIdx.x;
+    const int l0 = blockIdx.x * TL;
+    const int nl = min(TL, (int)gin.nlines - l0);
+    for (int k = tid; k < n; k += blockDim.x) {
+        CT w = twiddle[k];
+        if (INV) w.y = -w.y;
+        tw[k] = w;
+    }
+    __shared__ int s_lbi[64], s_lbo[64];
+    if (tid < nl) {
+        s_lbi[tid] = line_base32(gin, l0 + tid);
+        s_lbo[tid] = line_base32(gout, l0 + tid);
+    }
+    __syncthreads();
+    const int esi = (int)gin.elem_stride, eso = (int)gout.elem_stride;
+    const float rnl = 1.0f / (float)nl;
+    // stage 1: butterflies j < P2 of radix P1
+    {
+        const float rm = 1.0f / (float)P2;
+        const int tot = nl * P2;
+        for (int w = tid; w < tot; w += blockDim.x) {
+            int li, j;
+            if (esi == 1) fdivmod(w, P2, rm, li, j);
+            else fdivmod(w, nl, rnl, j, li);
+            const int ib = s_lbi[li] + j * esi, yb = li * n + j * P1;
+            switch (P1) {
+#define MPM_S1(P, C)                                                                 \
+    case P:                                                                          \
+        if constexpr (CLS >= C) fft2_s1<R, P, INV>(in, ib, P2 * esi, y, yb);         \
+        break;
+                MPM_FFT2_CASES(MPM_S1)
+#undef MPM_S1
+                default:
+                    break;
+            }
+        }
+    }
+    __syncthreads();
+    // stage 2: butterflies j' < P1 of radix P2
+    {
+        const float rm = 1.0f / (float)P1;
+        const int tot = nl * P1;
+        for (int w = tid; w < tot; w += blockDim.x) {
+            int li, j;
+            if (eso == 1) fdivmod(w, P1, rm, li, j);
+            else fdivmod(w, nl, rnl, j, li);
+            const int yb = li * n + j, ob = s_lbo[li] + j * eso;
+            switch (P2) {
+#define MPM_S2(P, C)                                                                       \
+    case P:                                                                                \
+        if constexpr (CLS >= C) fft2_s2<R, P, INV>(y, yb, P1, tw, j, out, ob, P1 * eso);   \
+        break;
+                MPM_FFT2_CASES(MPM_S2)
+#undef MPM_S2
+                default:
+                    break;
+            }
+        }
+    }
+}
+
 // Stage radices for length n: primes above kMaxPrimeRadix stand alone (generic
 // stage), primes from 17 up stand alone, and the small primes are packed,
 // largest first, into the fullest radix that stays <= kMaxCompositeRadix.
@@ -799,7 +925,8 @@ static std::vector<int> plan_radices(int n) {
 // One axis of the built-in FFT: plan + twiddle tables in both precisions.
 struct LineAxis {
     LinePlan plan{};
-    int cls = 0;  // radix class of the compiled kernel
+    int cls = 0;       // radix class of the compiled kernel
+    bool two = false;  // two register stages: line_fft2_kernel
     DevBuf<double2> tw64;
     DevBuf<float2> tw32;
     explicit LineAxis(int n) {
@@ -819,6 +946,7 @@ struct LineAxis {
             if (r > kMaxPrimeRadix) continue;  // generic stage
             cls = std::max(cls, r <= 8 ? 0 : (r <= 16 ? 1 : 2));
         }
+        two = plan.nstages == 2 && plan.radix[0] <= kMaxPrimeRadix && plan.radix[1] <= kMaxPrimeRadix;
         std::vector<double2> t64(n);
         std::vector<float2> t32(n);
         for (int k = 0; k < n; ++k) {
@@ -863,6 +991,33 @@ struct LineFft {
         hipLaunchKernelGGL((line_fft_kernel<R, INV, CLS>), grid, dim3(256), lds, s, in, out, gi, go, pl, tw, TL);
     }
 
+    // lines per block of the two-stage kernel: the most lines per unit of work,
+    // counting the rounds of 256 threads each stage takes (radix-weighted), within
+    // one LDS buffer of the per-block element budget (two stage buffers' worth)
+    template <typename R>
+    static int lines_per_block2(int n, int P1, int P2) {
+        const int tlmax = std::max(1, std::min(64, 2 * fft_elems<R>() / n));
+        int best = 1;
+        double best_eff = 0.0;
+        for (int tl = 1; tl <= tlmax; ++tl) {
+            const double cost = (double)((tl * P2 + 255) / 256) * P1 + (double)((tl * P1 + 255) / 256) * P2;
+            const double eff = tl / cost;
+            if (eff > best_eff * 1.0001) {
+                best_eff = eff;
+                best = tl;
+            }
+        }
+        return best;
+    }
+
+    template <typename R, bool INV, int CLS>
+    static void launch2(dim3 grid, size_t lds, hipStream_t s, const typename C2<R>::T* in, typename C2<R>::T* out,
+                        const LineGeom& gi, const LineGeom& go, const LinePlan& pl, const typename C2<R>::T* tw,
+                        int TL) {
+        hipLaunchKernelGGL((line_fft2_kernel<R, INV, CLS>), grid, dim3(256), lds, s, in, out, gi, go, pl.n,
+                           pl.radix[0], pl.radix[1], tw, TL);
+    }
+
     template <typename R>
     void exec(typename C2<R>::T* grid, typename C2<R>::T* kgrid, int B, bool forward, hipStream_t s) {
         using CT = typename C2<R>::T;
@@ -874,13 +1029,30 @@ struct LineFft {
             gi.nlines *= B;
             go.nlines *= B;
             const int n = ax.plan.n;
-            const int TL = lines_per_block<R>(n);
-            const size_t lds = ((size_t)2 * TL * n + 2 * n) * sizeof(CT);
-            const long long blocks = (gi.nlines + TL - 1) / TL;
             const CT* tw = std::is_same<R, float>::value ? reinterpret_cast<const CT*>(ax.tw32.p)
                                                          : reinterpret_cast<const CT*>(ax.tw64.p);
             const CT* in = ps.src ? kgrid : grid;
             CT* out = ps.dst ? kgrid : grid;
+            if (ax.two) {
+                const int TL = lines_per_block2<R>(n, ax.plan.radix[0], ax.plan.radix[1]);
+                const size_t lds = ((size_t)TL * n + n) * sizeof(CT);
+                const dim3 gd((unsigned)((gi.nlines + TL - 1) / TL));
+#define MPM_L2(INV)                                                                       \
+    if (ax.cls == 0) launch2<R, INV, 0>(gd, lds, s, in, out, gi, go, ax.plan, tw, TL);    \
+    else if (ax.cls == 1) launch2<R, INV, 1>(gd, lds, s, in, out, gi, go, ax.plan, tw, TL); \
+    else launch2<R, INV, 2>(gd, lds, s, in, out, gi, go, ax.plan, tw, TL);
+                if (forward) {
+                    MPM_L2(false)
+                } else {
+                    MPM_L2(true)
+                }
+#undef MPM_L2
+                MPM_HIP(hipGetLastError());
+                continue;
+            }
+            const int TL = lines_per_block<R>(n);
+            const size_t lds = ((size_t)2 * TL * n + 2 * n) * sizeof(CT);
+            const long long blocks = (gi.nlines + TL - 1) / TL;
             const dim3 gd((unsigned)blocks);
             if (forward) {
                 if (ax.cls == 0) launch<R, false, 0>(gd, lds, s, in, out, gi, go, ax.plan, tw, TL);
