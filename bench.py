@@ -65,8 +65,13 @@ def algorithmic_bytes(cls, info, N, B, fp32, pairs):
     """Algorithmic HBM bytes of one launch of a kernel class over B right-hand sides.
 
     The reciprocal grid is the full nx*ny*nz grid, or in the slab layout only the
-    occupied z-planes (slab_planes * nx * ny voxels); one FFT launch is one
-    multi-axis transform, counted as one read and one write of the grid."""
+    occupied z-planes (slab_planes * nx * ny voxels).  "fft_axisK" is one pass of
+    the transform (one read and one write of the grid); "contract" reads the grid
+    once and produces the reciprocal field of every particle (3 complex FP64 per
+    particle and RHS); "zfused" is the slab Green's-tensor pass (grid read and
+    write plus its coefficient table); "gather" reads the dipoles of the
+    neighbours and of the particle and writes the result (pair data on the fly
+    counts 20 B per pair)."""
     nx, ny, nz = list(info.grid)
     V = nx * ny * nz
     Vp = info.slab_planes * nx * ny if info.slab_planes else V
@@ -76,13 +81,30 @@ def algorithmic_bytes(cls, info, N, B, fp32, pairs):
     grid = 3 * Vp * B * c
     if cls == "spread":
         return grid + 48 * N * B + N * (3 * W * r + 12)
-    if cls == "fft":
+    if cls.startswith("fft_axis"):
         return 2 * grid
-    if cls in ("scale", "zfused"):
+    if cls == "fft":
+        return 2 * 2 * grid if info.slab_planes else 3 * 2 * grid
+    if cls == "zfused":
+        return 2 * grid + (6 * info.slab_planes * nx * ny * c if info.slab_planes else 4 * V * r)
+    if cls == "scale":
         return 2 * grid + 4 * V * r
+    if cls == "contract":
+        return grid + 48 * N * B + N * (3 * W * r + 12)
     if cls == "gather":
-        return grid + 2 * 48 * N * B + 16 * N * B + N * (3 * W * r + 12) + pairs * 20 + 24 * N
+        return 2 * 48 * N * B + 16 * N * B + pairs * 20 + 24 * N
     return 0
+
+
+# HBM traffic per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
+# workload (profiles/hex4000_sweep_batched_refined.md; FETCH_SIZE scaled by the copy-kernel
+# calibration recorded there), and the mean batch of the profiled launches.
+# Per-launch values are dispatch-weighted over the FP32 and FP64 launches of the class.
+MEASURED_TRAFFIC = {
+    "contract": (116 * (147.6 + 6.7) + 4 * (416.7 + 24.1)) / 120 * 1e6,
+    "zfused": (116 * (155.9 + 73.6) + 4 * (451.5 + 289.6)) / 120 * 1e6,
+}
+MEASURED_MEAN_B = 14.68
 
 
 def cpu_baseline(pos, box, a, eps, iters_needed, budget_s=20.0):
@@ -175,9 +197,11 @@ def main():
     run_once(s_mix, pos)
     _capi.profile_enable(False)
     classes = {}
-    for cls in ("spread", "fft", "scale", "zfused", "gather", "krylov"):
+    for cls in ("spread", "fft", "fft_axis0", "fft_axis1", "fft_axis2", "scale", "zfused", "contract", "gather",
+                "krylov"):
         ms, cnt = _capi.profile_get(cls)
-        classes[cls] = {"total_ms": ms, "launches": cnt}
+        if cnt:
+            classes[cls] = {"total_ms": ms, "launches": cnt}
 
     # ---- plain FP64, reference order ----
     plain = None
@@ -211,26 +235,36 @@ def main():
     else:
         accuracy = None
 
-    # ---- roofline of the dominant kernel class ----
+    # ---- roofline of the dominant kernel (single-kernel classes only) ----
     pairs = int(info.num_pairs)
-    dom = max(("spread", "scale", "zfused", "gather", "fft"), key=lambda c: classes[c]["total_ms"])
+    single = [c for c in ("spread", "fft_axis0", "fft_axis1", "fft_axis2", "scale", "zfused", "contract", "gather")
+              if c in classes]
+    dom = max(single, key=lambda c: classes[c]["total_ms"])
     # average batch size of the launches: total RHS-applies / launches of the spread
     mv_total = sum(r["matvecs"] for w in reps_mix for r in w)
     mv64 = sum(r["matvecs_fp64"] for w in reps_mix for r in w)
     n_launch = max(classes["spread"]["launches"], 1)
     mean_B = mv_total / n_launch
     frac32 = (mv_total - mv64) / max(mv_total, 1)
-    bytes_per_launch = (frac32 * algorithmic_bytes(dom, info, N, mean_B, True, pairs) +
-                        (1 - frac32) * algorithmic_bytes(dom, info, N, mean_B, False, pairs))
-    cls = classes[dom]
-    per_launch_ms = cls["total_ms"] / max(cls["launches"], 1)
+
+    def launch_bytes(c):
+        return (frac32 * algorithmic_bytes(c, info, N, mean_B, True, pairs) +
+                (1 - frac32) * algorithmic_bytes(c, info, N, mean_B, False, pairs))
+
+    for c, v in classes.items():
+        v["avg_ms"] = v["total_ms"] / max(v["launches"], 1)
+        if c != "krylov":
+            v["algorithmic_GBps"] = launch_bytes(c) / (v["avg_ms"] / 1e3) / 1e9
+    per_launch_ms = classes[dom]["avg_ms"]
+    bytes_per_launch = launch_bytes(dom)
     achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
+    traffic = None
+    if dom in MEASURED_TRAFFIC and MEASURED_MEAN_B:
+        traffic = MEASURED_TRAFFIC[dom] * mean_B / MEASURED_MEAN_B
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                 "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": per_launch_ms,
                 "mean_rhs_per_launch": mean_B}
-    for c in classes.values():
-        c["avg_ms"] = c["total_ms"] / max(c["launches"], 1)
 
     # ---- CPU oracle ----
     cpu = None

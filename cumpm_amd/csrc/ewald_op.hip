@@ -1023,7 +1023,9 @@ struct LineFft {
         using CT = typename C2<R>::T;
         MPM_REQUIRE((long long)B * rhs_elems < (1LL << 31), MPM_ERR_UNSUPPORTED,
                     "reciprocal grid too large for one transform call");
+        static const char* kPassNames[] = {"fft_axis0", "fft_axis1", "fft_axis2"};
         for (const LinePass& ps : (forward ? fwd : inv)) {
+            ProfScope pscope(kPassNames[std::min(ps.axis, 2)], s);
             LineAxis& ax = *axes[ps.axis];
             LineGeom gi = ps.gin, go = ps.gout;
             gi.nlines *= B;
@@ -2829,7 +2831,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
     }
     const R* part = nullptr;
     if (SLAB && reg_rx > 0 && (parts & MPM_PART_RECIP)) {
-        ProfScope ps("gather", s);
+        ProfScope ps("contract", s);
         // column groups of 16 per block: 4 when there are enough right-hand sides,
         // else fewer, with the slab planes split over the remaining waves
         const int ngrp = (6 * B + 15) / 16;

@@ -215,7 +215,9 @@ mpm_status mpm_device_synchronize(void);
  * every launch of the named kernel class is bracketed by events and accumulated. */
 mpm_status mpm_profile_enable(int enable);
 mpm_status mpm_profile_reset(void);
-/* name: "spread", "fft", "scale", "zfused", "gather", "krylov"; total_ms over count launches. */
+/* name: "spread", "fft" (one 2-D/3-D transform), "fft_axis0".."fft_axis2" (its passes), "scale",
+ * "zfused" (slab Green's tensor), "contract" (matrix-core contraction), "gather" (contraction and/or
+ * real space, self term), "krylov"; total_ms over count launches. */
 mpm_status mpm_profile_get(const char* name, double* total_ms, int64_t* count);
 
 #ifdef __cplusplus
