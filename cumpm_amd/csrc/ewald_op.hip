@@ -1075,21 +1075,27 @@ struct LineFft {
 // and zero at k = 0.  monodisperse_ewald_electric_field.cu:1300-1404.
 struct KParams {
     int nx, ny, nz;
-    int tcols;  // slab columns ordered ky * nx + kx (else kx * ny + ky)
+    int tcols;  // slab columns ordered ky * nxp + kx (else kx * ny + ky)
+    int nxp;    // row stride of the transposed columns (nx rounded up to whole cache lines)
     double Lx, Ly, Lz, kbx, kby, xi, etax, etay, etaz, invV;
 };
 
+// (kx, ky) of a slab column; ix >= nx marks a padding column
 __device__ inline void col_xy(const KParams& q, int col, int& ix, int& iy) {
     if (q.tcols) {
-        iy = col / q.nx;
-        ix = col - iy * q.nx;
+        iy = col / q.nxp;
+        ix = col - iy * q.nxp;
     } else {
         ix = col / q.ny;
         iy = col - ix * q.ny;
     }
 }
 
+__device__ inline int kcols(const KParams& q) { return q.tcols ? q.ny * q.nxp : q.nx * q.ny; }
+
 __device__ inline void kcoef(const KParams& q, int ix, int iy, int iz, double o[4]) {
+    o[0] = o[1] = o[2] = o[3] = 0.0;
+    if (ix >= q.nx) return;  // padding column
     const double fx = (ix <= (q.nx - 1) / 2) ? ix : (ix - q.nx);
     const double fy = (iy <= (q.ny - 1) / 2) ? iy : (iy - q.ny);
     const double fz = (iz <= (q.nz - 1) / 2) ? iz : (iz - q.nz);
@@ -1150,6 +1156,7 @@ struct SpreadGeom {
     int nbx, nby, nbz;  // bins per axis
     int ntx, nty;       // tiles per axis (x, y)
     int Zs;             // slab planes
+    int nyp;            // slab row stride (ny rounded up to whole cache lines when padded)
 };
 
 constexpr int kChunk = 64;       // particles staged in LDS per round
@@ -1369,7 +1376,7 @@ __global__ __launch_bounds__(256) void spread_kernel(SlotTable tab, int B, Sprea
             o.y = acc[j][c][1];
             size_t idx;
             if (SLAB)
-                idx = (((size_t)b * 3 + c) * g.Zs + js) * nxy + (size_t)vx * g.ny + vy;
+                idx = (((size_t)b * 3 + c) * g.Zs + js) * g.nx * g.nyp + (size_t)vx * g.nyp + vy;
             else
                 idx = ((size_t)b * 3 + c) * nxy * g.nz + ((size_t)vx * g.ny + vy) * g.nz + vz;
             out[idx] = o;
@@ -1607,13 +1614,13 @@ __global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, 
     // at y = y0 + (lane & 15)
     const int y = y0 + (lane & 15);
     if (y >= g.ny || wcol0 >= ncols) return;
-    const size_t nxy = (size_t)g.nx * g.ny;
+    const size_t plane = (size_t)g.nx * g.nyp;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int qc = wcol0 + 4 * (lane >> 4) + 2 * h;
         if (qc >= ncols) continue;
         const int b = qc / 6, c = (qc >> 1) % 3;
-        CT* o = out + ((size_t)b * 3 + c) * Zs * nxy + y;
+        CT* o = out + ((size_t)b * 3 + c) * Zs * plane + y;
 #pragma unroll
         for (int xi = 0; xi < RX; ++xi) {
             const int x = x0 + xi;
@@ -1624,7 +1631,7 @@ __global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, 
                     CT v;
                     v.x = acc[xi][p][2 * h];
                     v.y = acc[xi][p][2 * h + 1];
-                    o[(p0 + p) * nxy + (size_t)x * g.ny] = v;
+                    o[(p0 + p) * plane + (size_t)x * g.nyp] = v;
                 }
         }
     }
@@ -1668,25 +1675,26 @@ __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g,
     const int ncols = 6 * B;
     const bool active = wcol0 < ncols;
     const int Zs = g.Zs;
-    const size_t nxy = (size_t)g.nx * g.ny;
+    const size_t plane = (size_t)g.nx * g.nyp;
     const int k0 = reg_start[reg], k1 = reg_start[reg + 1];
 
-    // this lane's B-operand column and its rows of the region
+    // this lane's B-operand column and its rows of the region, as 32-bit offsets
+    // (in R) from the uniform grid base: the grid holds fewer than 2^31 complex
+    const R* gf = reinterpret_cast<const R*>(grid);
     const int qc = wcol0 + (lane & 15);
     const bool cvalid = qc < ncols;
-    const R* gcol = nullptr;
+    unsigned coff = 0, vmask = 0;  // column offset; bit 4 xi + u: (x0 + xi, y0 + 4 u + lane / 16) inside
     if (cvalid) {
         const int b = qc / 6, c = (qc >> 1) % 3;
-        gcol = reinterpret_cast<const R*>(grid + ((size_t)b * 3 + c) * Zs * nxy) + (qc & 1);
+        coff = (unsigned)(((b * 3 + c) * Zs) * plane * 2) + (qc & 1) +
+               2u * (unsigned)(x0 * g.nyp + y0 + (lane >> 4));
+#pragma unroll
+        for (int xi = 0; xi < RX; ++xi)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (x0 + xi < g.nx && y0 + 4 * u + (lane >> 4) < g.ny) vmask |= 1u << (4 * xi + u);
     }
-    int goff[RX][4];  // offset of (x0 + xi, y0 + 4 u + lane / 16) in a plane (in R), -1 outside
-#pragma unroll
-    for (int xi = 0; xi < RX; ++xi)
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int x = x0 + xi, y = y0 + 4 * u + (lane >> 4);
-            goff[xi][u] = (cvalid && x < g.nx && y < g.ny) ? 2 * (x * g.ny + y) : -1;
-        }
+    const unsigned xstep = 2u * (unsigned)g.nyp;
 
     for (int c0 = k0; c0 < k1; c0 += KC) {
         const int cnt = min(KC, k1 - c0);
@@ -1709,12 +1717,13 @@ __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g,
                 for (int xi = 0; xi < RX; ++xi) wxl[nb][xi] = s_w[k][L::X + xi];
             }
             for (int p = pg; p < Zs; p += npg) {
-                const R* gp = gcol ? gcol + 2 * (size_t)p * nxy : nullptr;
+                const unsigned poff = coff + 2u * (unsigned)p * (unsigned)plane;
                 R gv[RX][4];
 #pragma unroll
                 for (int xi = 0; xi < RX; ++xi)
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) gv[xi][u] = goff[xi][u] >= 0 ? gp[goff[xi][u]] : R(0);
+                    for (int u = 0; u < 4; ++u)
+                        gv[xi][u] = (vmask >> (4 * xi + u)) & 1u ? gf[poff + xi * xstep + 8u * u] : R(0);
                 R wz[NB];
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb) wz[nb] = s_w[nb * 16 + (lane & 15)][L::Z + p];
@@ -1838,7 +1847,7 @@ __global__ __launch_bounds__(256) void zfused_kernel(typename C2<R>::T* __restri
 
 // Green's tensor table for the slab path, [kz][column][4].
 __global__ void kspace_slab_kernel(double* __restrict__ k64, float* __restrict__ k32, KParams q) {
-    const size_t ncols = (size_t)q.nx * q.ny;
+    const size_t ncols = (size_t)kcols(q);
     const size_t V = ncols * q.nz;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < V; e += (size_t)gridDim.x * blockDim.x) {
         const int kz = (int)(e / ncols);
@@ -1862,7 +1871,7 @@ __global__ void kspace_slab_kernel(double* __restrict__ k64, float* __restrict__
 // sc k^ k^T is real).  Table layout [d][6][column], d = 0..Zs-1, components
 // (xx, yy, zz, xy, xz, yz), built in FP64 once per grid / Bloch vector / slab size.
 __global__ void ztoeplitz_table_kernel(double2* __restrict__ t64, float2* __restrict__ t32, KParams q, int Zs) {
-    const int ncols = q.nx * q.ny;
+    const int ncols = kcols(q);
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const int d = blockIdx.y;
     if (col >= ncols || d >= Zs) return;
@@ -1903,7 +1912,11 @@ constexpr int ztoe_as() {
 // right-hand sides so the rb threads of a column share the table lines in cache.
 // ZS >= Zs is the compile-time plane bound (planes Zs..ZS-1 are skipped), AS the
 // number of output planes accumulated per pass (register budget).
-template <typename R, int ZS, int AS>
+// RI: with an odd nz every kz != 0 pairs with -kz, and the projector's xx, yy, zz,
+// xy entries are even in kz while xz, yz are odd, so T_d's first four entries are
+// real and the last two imaginary: each entry then costs two real FMAs per
+// complex input instead of four.
+template <typename R, int ZS, int AS, bool RI>
 __global__ __launch_bounds__(256) void ztoeplitz_kernel(typename C2<R>::T* __restrict__ slab, int B, int ncols,
                                                         int Zs, int rb, const typename C2<R>::T* __restrict__ T) {
     using CT = typename C2<R>::T;
@@ -1946,6 +1959,24 @@ __global__ __launch_bounds__(256) void ztoeplitz_kernel(typename C2<R>::T* __res
                 const CT v = tp[(size_t)c * ncols];
                 tr[c] = v.x;
                 ti[c] = dd < 0 ? -v.y : v.y;
+            }
+            if (RI) {
+                // real xx, yy, zz, xy (tr[0..3]); imaginary xz, yz (i ti[4], i ti[5])
+#pragma unroll
+                for (int k = 0; k < AS; ++k) {
+                    const int a = a0 + k;
+                    const int bb = a - dd;
+                    if (a < alo || a >= ahi) continue;
+                    if (a >= Zs || bb >= Zs) continue;
+                    const CT fx = f[0][bb], fy = f[1][bb], fz = f[2][bb];
+                    ar[0][k] += tr[0] * fx.x + tr[3] * fy.x - ti[4] * fz.y;
+                    ai[0][k] += tr[0] * fx.y + tr[3] * fy.y + ti[4] * fz.x;
+                    ar[1][k] += tr[3] * fx.x + tr[1] * fy.x - ti[5] * fz.y;
+                    ai[1][k] += tr[3] * fx.y + tr[1] * fy.y + ti[5] * fz.x;
+                    ar[2][k] += tr[2] * fz.x - ti[4] * fx.y - ti[5] * fy.y;
+                    ai[2][k] += tr[2] * fz.y + ti[4] * fx.x + ti[5] * fy.x;
+                }
+                continue;
             }
 #pragma unroll
             for (int k = 0; k < AS; ++k) {
@@ -2020,8 +2051,9 @@ __global__ __launch_bounds__(256) void gather_kernel(
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int S = width * width * width;
     const size_t nxy = (size_t)nx * ny;
-    const size_t rhs_stride = SLAB ? 3 * (size_t)g.Zs * nxy : 3 * nxy * nz;
-    const size_t comp_stride = SLAB ? (size_t)g.Zs * nxy : nxy * nz;
+    const size_t plane = SLAB ? (size_t)nx * g.nyp : nxy;
+    const size_t rhs_stride = SLAB ? 3 * (size_t)g.Zs * plane : 3 * nxy * nz;
+    const size_t comp_stride = SLAB ? (size_t)g.Zs * plane : nxy * nz;
     int sx = 0, sy = 0, sz = 0;
     const R* w = wts + (size_t)i * 3 * width;
     if (parts & MPM_PART_RECIP) {
@@ -2076,7 +2108,7 @@ __global__ __launch_bounds__(256) void gather_kernel(
                 const R ww = w[ox] * w[width + oy] * w[2 * width + oz];
                 size_t v;
                 if (SLAB)
-                    v = (size_t)zmap[vz] * nxy + (size_t)vx * ny + vy;
+                    v = (size_t)zmap[vz] * plane + (size_t)vx * g.nyp + vy;
                 else
                     v = ((size_t)vx * ny + vy) * nz + vz;
 #pragma unroll
@@ -2292,6 +2324,7 @@ KParams EwaldOp::kparams() const {
     q.ny = S.n[1];
     q.nz = S.n[2];
     q.tcols = (slab && kt) ? 1 : 0;
+    q.nxp = nxp;
     q.Lx = L[0];
     q.Ly = L[1];
     q.Lz = L[2];
@@ -2315,15 +2348,17 @@ void EwaldOp::ensure_kspace_table() {
     if (kspace_ready && kspace_key == key) return;
     const int threads = 256;
     if (slab && zcontig) {
-        const size_t ncols = (size_t)S.n[0] * S.n[1];
+        const size_t ncols = (size_t)kspace_cols();
         d_toe64.resize((size_t)Zs * 6 * ncols);
         d_toe32.resize((size_t)Zs * 6 * ncols);
         hipLaunchKernelGGL(ztoeplitz_table_kernel, dim3((unsigned)((ncols + threads - 1) / threads), Zs),
                            dim3(threads), 0, nullptr, d_toe64.p, d_toe32.p, kparams(), Zs);
     } else {
-        d_k64.resize(4 * V);
-        d_k32.resize(4 * V);
-        const int blocks = (int)std::min<size_t>((V + threads - 1) / threads, 65535);
+        // [kz][column][4] over the slab's k-space columns, or [voxel][4] on the full grid
+        const size_t nk = slab ? (size_t)kspace_cols() * S.n[2] : V;
+        d_k64.resize(4 * nk);
+        d_k32.resize(4 * nk);
+        const int blocks = (int)std::min<size_t>((nk + threads - 1) / threads, 65535);
         if (slab)
             hipLaunchKernelGGL(kspace_slab_kernel, dim3(blocks), dim3(threads), 0, nullptr, d_k64.p, d_k32.p,
                                kparams());
@@ -2601,7 +2636,13 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     d_nb_j.upload(nj.data(), nj.size());
     d_nb_f.upload(nf.data(), nf.size());
     build_pair_coefs();
-    // FFT plans for the chosen layout
+    // FFT plans for the chosen layout.  With the built-in transform the slab rows
+    // are padded to whole 128-byte cache lines (16 FP32 / 8 FP64 complex; 16 for
+    // both), so that the region tiles and transposed passes touch whole lines.
+    const bool kt_ok = slab && fft_backend == 0 && S.n[0] > 1 && S.n[1] > 1 && LineFft::fits(S.n[0]) &&
+                       LineFft::fits(S.n[1]);
+    nyp = kt_ok ? (S.n[1] + 15) / 16 * 16 : S.n[1];
+    nxp = kt_ok ? (S.n[0] + 15) / 16 * 16 : S.n[0];
     const int key = slab ? Zs : -1;
     if (!lfft || fft_key != key) {
         const int nx = S.n[0], ny = S.n[1], nzz = S.n[2];
@@ -2616,14 +2657,14 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         auto pass = [&](std::vector<LinePass>& v, int a, LineGeom gi, LineGeom go, int src, int dst) {
             v.push_back(LinePass{a, gi, go, src, dst});
         };
-        if (slab && nx > 1 && ny > 1) {
-            // [3][Zs][nx][ny] -> y transform written transposed to the k-space
-            // buffer [3][Zs][ky][nx] -> x transform in place on contiguous lines
-            lf->rhs_elems = 3LL * Zs * nxy;
+        if (kt_ok) {
+            // [3][Zs][nx][nyp] -> y transform written transposed to the k-space
+            // buffer [3][Zs][ky][nxp] -> x transform in place on contiguous lines
+            lf->rhs_elems = 3LL * Zs * std::max((long long)nx * nyp, (long long)ny * nxp);
             const int ay = axis(ny), ax = axis(nx);
-            const LineGeom yrun{3LL * Zs * nx, 1, 0, ny, 1};
-            const LineGeom ytr{3LL * Zs * nx, nx, 1, nxy, nx};
-            const LineGeom xrun{3LL * Zs * ny, 1, 0, nx, 1};
+            const LineGeom yrun{3LL * Zs * nx, 1, 0, nyp, 1};
+            const LineGeom ytr{3LL * Zs * nx, nx, 1, (long long)ny * nxp, nxp};
+            const LineGeom xrun{3LL * Zs * ny, 1, 0, nxp, 1};
             pass(lf->fwd, ay, yrun, ytr, 0, 1);
             pass(lf->fwd, ax, xrun, xrun, 1, 1);
             pass(lf->inv, ax, xrun, xrun, 1, 1);
@@ -2684,8 +2725,11 @@ void EwaldOp::set_self_coef(const double* coef, int rows) {
 }
 
 size_t EwaldOp::grid_elems_per_rhs() const {
-    return slab ? 3 * (size_t)Zs * S.n[0] * S.n[1] : 3 * V;
+    return slab ? 3 * (size_t)Zs * S.n[0] * nyp : 3 * V;
 }
+size_t EwaldOp::kgrid_elems_per_rhs() const { return 3 * (size_t)Zs * kspace_cols(); }
+// columns of one slab plane in the k-space buffer (padding columns included)
+int EwaldOp::kspace_cols() const { return kt ? S.n[1] * nxp : S.n[0] * S.n[1]; }
 
 template <typename R, bool SLAB>
 void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int jacobi) {
@@ -2709,6 +2753,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
     g.ntx = ntiles[0];
     g.nty = ntiles[1];
     g.Zs = Zs;
+    g.nyp = slab ? nyp : S.n[1];
     const int Ni = (int)N;
     if (parts & MPM_PART_RECIP) {
         if (SLAB && reg_rx > 0) {
@@ -2756,16 +2801,21 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
         }
         if (SLAB && zcontig) {
             ProfScope ps("zfused", s);
-            const int ncols = S.n[0] * S.n[1];
+            const int ncols = kspace_cols();
             const CT* T = reinterpret_cast<const CT*>(fp32 ? static_cast<const void*>(d_toe32.p)
                                                            : static_cast<const void*>(d_toe64.p));
             const int rb = B >= 16 ? 16 : (B >= 4 ? 4 : 1);
             const int cb = 256 / rb;
             const dim3 gridz((ncols + cb - 1) / cb, (B + rb - 1) / rb);
+            const bool ri = (S.n[2] & 1) != 0;  // odd nz: real / imaginary table entries
 #define MPM_ZT(ZSV)                                                                                           \
     case ZSV:                                                                                                 \
-        hipLaunchKernelGGL((ztoeplitz_kernel<R, ZSV, ztoe_as<R, ZSV>()>), gridz, dim3(256), 0, s, kgrid, B, \
-                           ncols, Zs, rb, T);                                                                 \
+        if (ri)                                                                                               \
+            hipLaunchKernelGGL((ztoeplitz_kernel<R, ZSV, ztoe_as<R, ZSV>(), true>), gridz, dim3(256), 0, s,   \
+                               kgrid, B, ncols, Zs, rb, T);                                                   \
+        else                                                                                                  \
+            hipLaunchKernelGGL((ztoeplitz_kernel<R, ZSV, ztoe_as<R, ZSV>(), false>), gridz, dim3(256), 0, s,  \
+                               kgrid, B, ncols, Zs, rb, T);                                                   \
         break;
             switch (zs_bucket(Zs)) {
                 MPM_ZT(4)
@@ -2782,7 +2832,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
             MPM_HIP(hipGetLastError());
         } else if (SLAB) {
             ProfScope ps("zfused", s);
-            const int ncols = S.n[0] * S.n[1];
+            const int ncols = kspace_cols();
             const int zsb = zs_bucket(Zs);
             const size_t lds = zfused_lds(zsb, S.n[2], sizeof(R));
             const R* kc = fp32 ? reinterpret_cast<const R*>(d_k32.p) : reinterpret_cast<const R*>(d_k64.p);
@@ -2890,12 +2940,13 @@ void EwaldOp::apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t
                     "self-coefficient row out of range");
     if (parts & MPM_PART_RECIP) ensure_kspace_table();
     const size_t elems = (size_t)B * grid_elems_per_rhs();
+    const size_t kelems = (size_t)B * kgrid_elems_per_rhs();
     if (fp32) {
         d_grid32.reserve(elems);
-        if (kt) d_kgrid32.reserve(elems);
+        if (kt) d_kgrid32.reserve(kelems);
     } else {
         d_grid64.reserve(elems);
-        if (kt) d_kgrid64.reserve(elems);
+        if (kt) d_kgrid64.reserve(kelems);
     }
     const int jac = jacobi ? 1 : 0;
     if (fp32) {

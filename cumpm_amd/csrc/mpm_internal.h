@@ -176,7 +176,10 @@ class EwaldOp {
     void build_pair_coefs();
     DevBuf<double2> d_selfc;     // [rows][N]
     bool zcontig = false;        // slab planes form one cyclic run (zlist[a] = z0 + a mod nz)
-    bool kt = false;             // slab k-space data in its own buffer, columns ordered ky * nx + kx
+    bool kt = false;             // slab k-space data in its own buffer, columns ordered ky * nxp + kx
+    int nyp = 0, nxp = 0;        // slab row strides of the grid ([x][nyp]) and k-space ([ky][nxp]) buffers
+    size_t kgrid_elems_per_rhs() const;
+    int kspace_cols() const;
     // matrix-core spread and contraction (slab): xy-regions of reg_rx x 16 voxels and the
     // candidate particles of each region (CSR, ascending ids)
     int reg_rx = 0, reg_nrx = 0, reg_nry = 0;

@@ -31,6 +31,8 @@ GEOMS = [
     ("random3d", [16.0, 17.0, 18.0], 1e-6, 0.5, lambda: np.random.default_rng(1).uniform(0, 16, (40, 3)), None, 0),
     ("outside_box", [16.0, 16.0, 16.0], 1e-6, 0.5, lambda: np.random.default_rng(2).uniform(-9, 30, (25, 3)), None, 0),
     ("hex_slab", None, 1e-6, 0.5, None, None, 0),
+    # odd nz (21): the slab Green's-tensor table has real / imaginary entries
+    ("hex_slab_odd", None, 1e-6, 0.5, None, None, 0),
     ("sc_tight", [24.0] * 3, 1e-8, 0.5,
      lambda: np.stack(np.meshgrid(*[np.arange(4) * 3.0 + 1.5] * 3, indexing="ij"), -1).reshape(-1, 3), None, 0),
     ("tiny_grid_wrap", [15.0, 15.0, 15.0], 1e-4, 0.5,
@@ -52,9 +54,9 @@ def _geom(name):
     for g in GEOMS:
         if g[0] == name:
             nm, box, tol, xi, pf, grid, P = g
-            if nm == "hex_slab":
+            if nm in ("hex_slab", "hex_slab_odd"):
                 pos, box = hex_monolayer(10, 8, 3.0, jitter=0.04, seed=5)
-                box = np.array([box[0], box[1], 16.0])
+                box = np.array([box[0], box[1], 16.0 if nm == "hex_slab" else 16.5])
             elif nm == "bilayer":  # two layers: the occupied z-planes are not contiguous
                 p1, box = hex_monolayer(8, 6, 2.5, jitter=0.05, seed=6)
                 p2 = p1 + np.array([1.25, 0.7, 12.0])
@@ -66,7 +68,7 @@ def _geom(name):
     raise KeyError(name)
 
 
-SLAB_GEOMS = ("hex_slab", "bilayer")
+SLAB_GEOMS = ("hex_slab", "hex_slab_odd", "bilayer")
 
 
 @pytest.mark.parametrize("name,layout,fft", [(g[0], 0, 0) for g in GEOMS] + [(g, 1, 0) for g in SLAB_GEOMS] +
@@ -109,7 +111,7 @@ def test_apply_fp64_matches_oracle(name, layout, fft):
 
 @pytest.mark.parametrize("name,layout,fft", [("random3d", 0, 0), ("hex_slab", 0, 0), ("hex_slab", 1, 0),
                                              ("sc_tight", 0, 0), ("bilayer", 0, 0), ("hex_slab", 0, 1),
-                                             ("radix_mix_a", 0, 0), ("radix_mix_b", 0, 0)])
+                                             ("radix_mix_a", 0, 0), ("radix_mix_b", 0, 0), ("hex_slab_odd", 0, 0)])
 def test_apply_fp32_reciprocal_within_reference_tolerance(name, layout, fft):
     box, tol, xi, pos, grid, P = _geom(name)
     _case(f"apply fp32 {name} layout={layout} fft={fft}")
