@@ -1458,11 +1458,11 @@ struct Mfma<double> {
 };
 
 // Block shape of spread_mfma_kernel: 4 waves = (CPB / 16 column groups) x
-// (plane groups of ZH planes).  FP32: 64 columns, all planes per wave; FP64:
-// 32 columns, the planes split over two waves (accumulator registers).
+// (plane groups of ZH planes): 32 columns, the planes split over two waves, which
+// halves the accumulator registers and lets more blocks overlap their staging.
 template <typename R, int RX, int ZSB>
 struct SpreadShape {
-    static constexpr int CPB = sizeof(R) == 4 ? 64 : 32;
+    static constexpr int CPB = 32;
     static constexpr int NPG = 4 / (CPB / 16);
     static constexpr int ZH = (ZSB + NPG - 1) / NPG;
 };
@@ -1480,66 +1480,19 @@ struct RegionW {
     static constexpr int NW = Y + kMfmaRY;
 };
 
-// Stages a chunk of cnt <= kMfmaKC candidates: ids and stencil origins
-// (base - half) first, then the region-local weights, with every global load of
-// a phase independent of the others so that they are all in flight together.
-// Needs blockDim.x == 256 and ends with a barrier.
+// Copies the precomputed region-local weight rows of a chunk of cnt <= kMfmaKC
+// candidates into LDS (zero rows past cnt) and ends with a barrier.
 template <typename R, int RX, int ZSB>
-__device__ inline void stage_candidates(int* s_pid, int (*s_org)[3], R* s_w, int cnt, int x0, int y0,
-                                        const SpreadGeom& g, const int* __restrict__ zlist,
-                                        const int* __restrict__ base, const R* __restrict__ wts,
-                                        const int* __restrict__ items, int N) {
+__device__ inline void stage_weights(R* s_w, int cnt, const R* __restrict__ regw) {
     using L = RegionW<RX, ZSB>;
-    const int tid = threadIdx.x;
-    if (tid < kMfmaKC) {
-        const int pid = tid < cnt ? items[tid] : 0;
-        s_pid[tid] = pid;
-        s_org[tid][0] = base[pid] - g.half;
-        s_org[tid][1] = base[(size_t)N + pid] - g.half;
-        s_org[tid][2] = base[2 * (size_t)N + pid] - g.half;
-    }
-    __syncthreads();
-    constexpr int T = kMfmaKC * L::NW / 256;
-    static_assert(kMfmaKC * L::NW % 256 == 0, "weight rows must tile the block");
-#pragma unroll
-    for (int t = 0; t < T; ++t) {
-        const int e = tid + 256 * t;
-        const int q = e / L::NW, j = e % L::NW;
-        R w = R(0);
-        if (q < cnt) {
-            int d = -1, v = -1, n = 1;
-            if (j >= L::Y) {
-                d = 1;
-                n = g.ny;
-                v = y0 + j - L::Y;
-                if (v >= n) v = -1;
-            } else if (j >= L::Z) {
-                const int p = j - L::Z;
-                d = 2;
-                n = g.nz;
-                v = p < g.Zs ? zlist[p] : -1;
-            } else if (j < RX) {
-                d = 0;
-                n = g.nx;
-                v = x0 + j;
-                if (v >= n) v = -1;
-            }
-            if (v >= 0) {
-                const R* wp = wts + ((size_t)s_pid[q] * 3 + d) * g.width;
-                int off = (v - s_org[q][d]) % n;  // stencil index of the first point on voxel v
-                if (off < 0) off += n;
-                for (int o = off; o < g.width; o += n) w += wp[o];
-            }
-        }
-        s_w[e] = w;  // row q = e / NW, entry j = e % NW
-    }
+    const int n = cnt * L::NW;
+    for (int e = threadIdx.x; e < kMfmaKC * L::NW; e += blockDim.x) s_w[e] = e < n ? regw[e] : R(0);
     __syncthreads();
 }
 
 template <typename R, int RX, int ZSB>
 __global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, SpreadGeom g,
-                                                          const int* __restrict__ zlist, const int* __restrict__ base,
-                                                          const R* __restrict__ wts,
+                                                          const R* __restrict__ regw,
                                                           const int* __restrict__ reg_start,
                                                           const int* __restrict__ reg_items, int nry, int N,
                                                           typename C2<R>::T* __restrict__ out) {
@@ -1550,8 +1503,6 @@ __global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, 
     using L = RegionW<RX, ZSB>;
     __shared__ R s_v[KC][CPB + 1];
     __shared__ __align__(16) R s_w[KC][L::NW];
-    __shared__ int s_pid[KC];
-    __shared__ int s_org[KC][3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int cg = wave % (CPB / 16), pg = wave / (CPB / 16);
     const int reg = blockIdx.x;
@@ -1577,15 +1528,14 @@ __global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, 
     for (int c0 = k0; c0 < k1; c0 += KC) {
         const int cnt = min(KC, k1 - c0);
         __syncthreads();
-        stage_candidates<R, RX, ZSB>(s_pid, s_org, &s_w[0][0], cnt, x0, y0, g, zlist, base, wts, reg_items + c0,
-                                     N);
         // values: s_v[k][col] = component (b, c), real or imaginary part, of candidate k
         R vv[KC / QS];
 #pragma unroll
         for (int t = 0; t < KC / QS; ++t) {
             const int q = tid / CPB + QS * t;
-            vv[t] = (xcol && q < cnt) ? (R)xcol[6 * (size_t)s_pid[q]] : R(0);
+            vv[t] = (xcol && q < cnt) ? (R)xcol[6 * (size_t)reg_items[c0 + q]] : R(0);
         }
+        stage_weights<R, RX, ZSB>(&s_w[0][0], cnt, regw + (size_t)c0 * L::NW);
 #pragma unroll
         for (int t = 0; t < KC / QS; ++t) s_v[tid / CPB + QS * t][scol] = vv[t];
         __syncthreads();
@@ -1642,17 +1592,15 @@ __global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, 
 // The transpose of spread_mfma_kernel over the same regions: for the candidate
 // particles k of a region and 16 real columns q,
 //   D[k][q] = sum_{voxels v of the region} W[k][v] G[v][q],
-// one 16x16x4 MFMA per four voxels and 16 candidates (A = W, B = G).  Each wave
-// reads its B operands straight from the grid: lane (j, u) loads column j at
-// y = 4 u + (lane >> 4), so the four instructions of a 16-voxel row consume
-// whole cache lines, and waves run without barriers after the one staging
-// step.  D goes to the particle's slot of the region (the CSR position of the
-// pair); the per-particle sum over its regions happens in gather_kernel, in a
-// fixed order.
+// one 16x16x4 MFMA per four voxels and 16 candidates (A = W, B = G), every
+// loaded operand feeding all candidate blocks.  Each wave reads its B operands
+// straight from the grid: lane (j, u) loads column j at y = 4 u + (lane >> 4),
+// so the four loads of a 16-voxel row consume whole cache lines; waves run
+// without barriers after the one weight-staging step.  D goes to the particle's
+// slot of the region (the CSR position of the pair); the per-particle sum over
+// its regions happens in the finish step, in a fixed order.
 template <typename R, int RX, int ZSB>
-__global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g, const int* __restrict__ zlist,
-                                                            const int* __restrict__ base,
-                                                            const R* __restrict__ wts,
+__global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g, const R* __restrict__ regw,
                                                             const int* __restrict__ reg_start,
                                                             const int* __restrict__ reg_items, int nry, int N,
                                                             const typename C2<R>::T* __restrict__ grid,
@@ -1663,8 +1611,6 @@ __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g,
     using L = RegionW<RX, ZSB>;
     constexpr int NB = KC / 16;  // candidate blocks of 16
     __shared__ __align__(16) R s_w[KC][L::NW];
-    __shared__ int s_pid[KC];
-    __shared__ int s_org[KC][3];
     __shared__ V4 s_red[4][NB][64];  // per-wave partial accumulators
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // 4 waves = ncg column groups x (4 / ncg) plane groups
@@ -1675,7 +1621,7 @@ __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g,
     const int ncols = 6 * B;
     const bool active = wcol0 < ncols;
     const int Zs = g.Zs;
-    const size_t plane = (size_t)g.nx * g.nyp;
+    const unsigned plane = (unsigned)(g.nx * g.nyp);
     const int k0 = reg_start[reg], k1 = reg_start[reg + 1];
 
     // this lane's B-operand column and its rows of the region, as 32-bit offsets
@@ -1686,7 +1632,7 @@ __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g,
     unsigned coff = 0, vmask = 0;  // column offset; bit 4 xi + u: (x0 + xi, y0 + 4 u + lane / 16) inside
     if (cvalid) {
         const int b = qc / 6, c = (qc >> 1) % 3;
-        coff = (unsigned)(((b * 3 + c) * Zs) * plane * 2) + (qc & 1) +
+        coff = 2u * (unsigned)((b * 3 + c) * Zs) * plane + (unsigned)(qc & 1) +
                2u * (unsigned)(x0 * g.nyp + y0 + (lane >> 4));
 #pragma unroll
         for (int xi = 0; xi < RX; ++xi)
@@ -1700,8 +1646,7 @@ __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g,
         const int cnt = min(KC, k1 - c0);
         const int nblk = (cnt + 15) >> 4;
         __syncthreads();
-        stage_candidates<R, RX, ZSB>(s_pid, s_org, &s_w[0][0], cnt, x0, y0, g, zlist, base, wts, reg_items + c0,
-                                     N);
+        stage_weights<R, RX, ZSB>(&s_w[0][0], cnt, regw + (size_t)c0 * L::NW);
         V4 acc[NB];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) acc[nb] = V4{R(0), R(0), R(0), R(0)};
@@ -1717,7 +1662,7 @@ __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g,
                 for (int xi = 0; xi < RX; ++xi) wxl[nb][xi] = s_w[k][L::X + xi];
             }
             for (int p = pg; p < Zs; p += npg) {
-                const unsigned poff = coff + 2u * (unsigned)p * (unsigned)plane;
+                const unsigned poff = coff + 2u * (unsigned)p * plane;
                 R gv[RX][4];
 #pragma unroll
                 for (int xi = 0; xi < RX; ++xi)
@@ -1759,6 +1704,92 @@ __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g,
                     const int k = nb * 16 + M::drow(lane >> 4, r);
                     if (k < cnt) partial[(size_t)(c0 + k) * ncols + qc] = acc[nb][r];
                 }
+        }
+    }
+}
+
+// Narrow-batch variant of the contraction: a block is one column group of 16
+// and four waves, one per block of 16 candidates, so a wave holds few registers
+// and, when only one or two column groups exist, the region's candidates still
+// spread over four waves; the next plane's operands are loaded while the
+// current plane's products run.
+template <typename R, int RX, int ZSB>
+__global__ __launch_bounds__(256) void contract_nb_mfma_kernel(int B, SpreadGeom g, const R* __restrict__ regw,
+                                                               const int* __restrict__ reg_start,
+                                                               const int* __restrict__ reg_items, int nry, int N,
+                                                               const typename C2<R>::T* __restrict__ grid,
+                                                               R* __restrict__ partial) {
+    using M = Mfma<R>;
+    using V4 = typename M::V4;
+    constexpr int RY = kMfmaRY, KC = kMfmaKC;
+    using L = RegionW<RX, ZSB>;
+    __shared__ __align__(16) R s_w[KC][L::NW];
+    const int tid = threadIdx.x, lane = tid & 63, nb = tid >> 6;  // candidate block of this wave
+    const int reg = blockIdx.x;
+    const int x0 = (reg / nry) * RX, y0 = (reg % nry) * RY;
+    const int wcol0 = blockIdx.y * 16;
+    const int ncols = 6 * B;
+    const int Zs = g.Zs;
+    const unsigned plane = (unsigned)(g.nx * g.nyp);
+    const int k0 = reg_start[reg], k1 = reg_start[reg + 1];
+    const R* gf = reinterpret_cast<const R*>(grid);
+    const int qc = wcol0 + (lane & 15);
+    const bool cvalid = qc < ncols;
+    unsigned coff = 0, vmask = 0;  // column offset; bit 4 xi + u: (x0 + xi, y0 + 4 u + lane / 16) inside
+    if (cvalid) {
+        const int b = qc / 6, c = (qc >> 1) % 3;
+        coff = 2u * (unsigned)((b * 3 + c) * Zs) * plane + (unsigned)(qc & 1) +
+               2u * (unsigned)(x0 * g.nyp + y0 + (lane >> 4));
+#pragma unroll
+        for (int xi = 0; xi < RX; ++xi)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (x0 + xi < g.nx && y0 + 4 * u + (lane >> 4) < g.ny) vmask |= 1u << (4 * xi + u);
+    }
+    const unsigned xstep = 2u * (unsigned)g.nyp;
+    auto load_plane = [&](int p, R (&dst)[RX][4]) {
+        const unsigned poff = coff + 2u * (unsigned)p * plane;
+#pragma unroll
+        for (int xi = 0; xi < RX; ++xi)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                dst[xi][u] = (vmask >> (4 * xi + u)) & 1u ? gf[poff + xi * xstep + 8u * u] : R(0);
+    };
+    for (int c0 = k0; c0 < k1; c0 += KC) {
+        const int cnt = min(KC, k1 - c0);
+        __syncthreads();
+        stage_weights<R, RX, ZSB>(&s_w[0][0], cnt, regw + (size_t)c0 * L::NW);
+        if (nb * 16 >= cnt) continue;  // no candidates for this wave (no barrier follows in this chunk)
+        const int k = nb * 16 + (lane & 15);
+        R wyl[4], wxl[RX];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) wyl[u] = s_w[k][L::Y + 4 * u + (lane >> 4)];
+#pragma unroll
+        for (int xi = 0; xi < RX; ++xi) wxl[xi] = s_w[k][L::X + xi];
+        V4 acc = V4{R(0), R(0), R(0), R(0)};
+        R gv[RX][4], gn[RX][4];
+        load_plane(0, gv);
+        for (int p = 0; p < Zs; ++p) {
+            if (p + 1 < Zs) load_plane(p + 1, gn);
+            const R wz = s_w[k][L::Z + p];
+#pragma unroll
+            for (int xi = 0; xi < RX; ++xi) {
+                const R wxz = wxl[xi] * wz;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) acc = M::mac(wyl[u] * wxz, gv[xi][u], acc);
+            }
+#pragma unroll
+            for (int xi = 0; xi < RX; ++xi)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) gv[xi][u] = gn[xi][u];
+        }
+        // lane holds D[16 nb + drow(lane >> 4, r)][lane & 15]
+        if (cvalid) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int kk = nb * 16 + M::drow(lane >> 4, r);
+                if (kk < cnt) partial[(size_t)(c0 + kk) * ncols + qc] = acc[r];
+            }
         }
     }
 }
@@ -2536,6 +2567,36 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         d_ps_items.upload(psi.data(), psi.size());
         d_partial.resize((size_t)std::max(rs[nreg], 1) * 6 * kMaxSlots);
         d_partial64.resize((size_t)std::max(rs[nreg], 1) * 6 * kMaxSlots);
+        // region-local separable weights of every (region, candidate) slot, in the
+        // row layout of the kernels: x at [0, RX), slab planes at [4, 4 + ZSB),
+        // y at [4 + ZSB4, 4 + ZSB4 + 16); sums over stencil points that fold
+        const int zsb = zs_bucket(Zs), zsb4 = (zsb + 3) & ~3;
+        const int NW = 4 + zsb4 + RY, X = 0, Zo = 4, Yo = 4 + zsb4;
+        std::vector<double> rw64((size_t)std::max(rs[nreg], 1) * NW, 0.0);
+        auto wsum = [&](int pid, int d, int v) {
+            const int nd = S.n[d];
+            const int b0 = base[(size_t)d * Nn + pid] - half;
+            int off = (v - b0) % nd;
+            if (off < 0) off += nd;
+            double acc = 0.0;
+            for (int o = off; o < W; o += nd) acc += w64[((size_t)pid * 3 + d) * W + o];
+            return acc;
+        };
+        for (int r = 0; r < nreg; ++r) {
+            const int x0 = (r / reg_nry) * RX, y0 = (r % reg_nry) * RY;
+            for (int q = rs[r]; q < rs[r + 1]; ++q) {
+                const int pid = ri[q];
+                double* row = &rw64[(size_t)q * NW];
+                for (int j = 0; j < RX; ++j)
+                    if (x0 + j < S.n[0]) row[X + j] = wsum(pid, 0, x0 + j);
+                for (int p = 0; p < Zs; ++p) row[Zo + p] = wsum(pid, 2, zl[p]);
+                for (int j = 0; j < RY; ++j)
+                    if (y0 + j < S.n[1]) row[Yo + j] = wsum(pid, 1, y0 + j);
+            }
+        }
+        std::vector<float> rw32(rw64.begin(), rw64.end());
+        d_regw64.upload(rw64.data(), rw64.size());
+        d_regw32.upload(rw32.data(), rw32.size());
     }
     std::vector<int> ztl;  // z-tiles that hold at least one computed voxel
     for (int t = 0; t < ntiles[2]; ++t) {
@@ -2754,16 +2815,17 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
     g.nty = ntiles[1];
     g.Zs = Zs;
     g.nyp = slab ? nyp : S.n[1];
+    const R* regw = fp32 ? reinterpret_cast<const R*>(d_regw32.p) : reinterpret_cast<const R*>(d_regw64.p);
     const int Ni = (int)N;
     if (parts & MPM_PART_RECIP) {
         if (SLAB && reg_rx > 0) {
             ProfScope ps("spread", s);
-            constexpr int CPB = sizeof(R) == 4 ? 64 : 32;
+            constexpr int CPB = 32;  // SpreadShape<R, ...>::CPB
             const dim3 gs((unsigned)(reg_nrx * reg_nry), (unsigned)((6 * B + CPB - 1) / CPB));
 #define MPM_SM(RXV, ZSV)                                                                                          \
     case ZSV:                                                                                                     \
-        hipLaunchKernelGGL((spread_mfma_kernel<R, RXV, ZSV>), gs, dim3(256), 0, s, t, B, g, d_zlist.p, d_base.p, \
-                           w, d_reg_start.p, d_reg_items.p, reg_nry, Ni, grid);                                  \
+        hipLaunchKernelGGL((spread_mfma_kernel<R, RXV, ZSV>), gs, dim3(256), 0, s, t, B, g, regw, d_reg_start.p, \
+                           d_reg_items.p, reg_nry, Ni, grid);                                                    \
         break;
             switch (zs_bucket(Zs)) {
                 MPM_SM(4, 4)
@@ -2883,15 +2945,21 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
     if (SLAB && reg_rx > 0 && (parts & MPM_PART_RECIP)) {
         ProfScope ps("contract", s);
         // column groups of 16 per block: 4 when there are enough right-hand sides,
-        // else fewer, with the slab planes split over the remaining waves
+        // else fewer, with the slab planes split over the remaining waves; one or
+        // two column groups: one wave per candidate block instead
         const int ngrp = (6 * B + 15) / 16;
         const int ncg = ngrp >= 4 ? 4 : (ngrp >= 2 ? 2 : 1);
         const dim3 gs((unsigned)(reg_nrx * reg_nry), (unsigned)((ngrp + ncg - 1) / ncg));
+        const dim3 gn((unsigned)(reg_nrx * reg_nry), (unsigned)ngrp);
         R* pbuf = fp32 ? reinterpret_cast<R*>(d_partial.p) : reinterpret_cast<R*>(d_partial64.p);
-#define MPM_CM(RXV, ZSV)                                                                                     \
-    case ZSV:                                                                                                \
-        hipLaunchKernelGGL((contract_mfma_kernel<R, RXV, ZSV>), gs, dim3(256), 0, s, B, g, d_zlist.p, d_base.p, \
-                           w, d_reg_start.p, d_reg_items.p, reg_nry, Ni, grid, pbuf, ncg);                   \
+#define MPM_CM(RXV, ZSV)                                                                                          \
+    case ZSV:                                                                                                     \
+        if (ngrp <= 2)                                                                                            \
+            hipLaunchKernelGGL((contract_nb_mfma_kernel<R, RXV, ZSV>), gn, dim3(256), 0, s, B, g, regw,          \
+                               d_reg_start.p, d_reg_items.p, reg_nry, Ni, grid, pbuf);                           \
+        else                                                                                                      \
+            hipLaunchKernelGGL((contract_mfma_kernel<R, RXV, ZSV>), gs, dim3(256), 0, s, B, g, regw, d_reg_start.p, \
+                               d_reg_items.p, reg_nry, Ni, grid, pbuf, ncg);                                     \
         break;
         switch (zs_bucket(Zs)) {
             MPM_CM(4, 4)

@@ -187,6 +187,8 @@ class EwaldOp {
     DevBuf<int> d_reg_items;
     DevBuf<int> d_ps_start;      // [N + 1] region slots of each particle
     DevBuf<int> d_ps_items;      // slot ids (CSR positions of d_reg_items)
+    DevBuf<double> d_regw64;     // [slot][row] region-local separable weights (FP64, FP32)
+    DevBuf<float> d_regw32;
     DevBuf<float> d_partial;     // [slot][6 B] per-region contractions (FP32, FP64)
     DevBuf<double> d_partial64;
     // k-space tables: (khat x, y, z, scale) per voxel for the full grid and the
