@@ -206,85 +206,117 @@ __global__ __launch_bounds__(256) void axpy_kernel(Ptrs p, size_t n, const doubl
     p.b[slot][e] = make_double2(v.x + t.x, v.y + t.y);
 }
 
-// GMRES column update + Givens rotations on the device, one thread per system.
+// GMRES column update + Givens rotations on the device, one wave per system.
 // H: [sys][(m+1) x m] row-major (row i, column k); h: new column coefficients [sys][m+1];
 // hn2: squared norm of the orthogonalised vector.  gmres_solver.cu:139-178.
-__global__ void givens_kernel(Ptrs p, int nslots, int k, int m, double2* __restrict__ H,
-                              const double2* __restrict__ h, const double* __restrict__ hn2,
-                              double2* __restrict__ g, double* __restrict__ cs,
-                              double2* __restrict__ sn, double* __restrict__ resid) {
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void givens_kernel(Ptrs p, int nslots, int k, int m, double2* __restrict__ H,
+                                                    const double2* __restrict__ h, const double* __restrict__ hn2,
+                                                    double2* __restrict__ g, double* __restrict__ cs,
+                                                    double2* __restrict__ sn, double* __restrict__ resid) {
+    // one wave per system: the new Hessenberg column and the earlier rotations are
+    // staged in LDS, one lane applies the rotations in order
+    __shared__ double2 s_col[64 + 2];
+    __shared__ double s_c[64];
+    __shared__ double2 s_s[64];
+    const int slot = blockIdx.x;
     if (slot >= nslots) return;
+    const int lane = threadIdx.x;
     const int s = p.sys[slot];
     double2* Hs = H + (size_t)s * (m + 1) * m;
     const double2* hs = h + (size_t)s * (m + 1);
     double2* gs = g + (size_t)s * (m + 1);
     double* c = cs + (size_t)s * m;
     double2* sv = sn + (size_t)s * m;
-    for (int i = 0; i <= k; ++i) Hs[i * m + k] = hs[i];
-    Hs[(k + 1) * m + k] = make_double2(sqrt(hn2[s]), 0.0);
-    for (int i = 0; i < k; ++i) {
-        const double2 h1 = Hs[i * m + k], h2 = Hs[(i + 1) * m + k];
-        const double2 a = cmulc(sv[i], h2);                // conj(s) h2
-        const double2 b = cmul(sv[i], h1);                 // s h1
-        Hs[i * m + k] = make_double2(c[i] * h1.x + a.x, c[i] * h1.y + a.y);
-        Hs[(i + 1) * m + k] = make_double2(-b.x + c[i] * h2.x, -b.y + c[i] * h2.y);
+    for (int i = lane; i <= k; i += 64) s_col[i] = hs[i];
+    for (int i = lane; i < k; i += 64) {
+        s_c[i] = c[i];
+        s_s[i] = sv[i];
     }
-    const double2 h1 = Hs[k * m + k], h2 = Hs[(k + 1) * m + k];
-    double cr = 1.0;
-    double2 sr = make_double2(0.0, 0.0);
-    if (h2.x != 0.0 || h2.y != 0.0) {
-        const double a1 = hypot(h1.x, h1.y);
-        if (a1 == 0.0) {
-            cr = 0.0;
-            sr = make_double2(1.0, 0.0);
-        } else {
-            const double a2 = hypot(h2.x, h2.y);
-            const double scl = a1 + a2;
-            const double n1 = a1 / scl, n2 = a2 / scl;
-            const double hyp = scl * sqrt(n1 * n1 + n2 * n2);
-            cr = a1 / hyp;
-            // (conj(h1)/|h1|) * (h2 / hyp)
-            const double2 u = make_double2(h1.x / a1, -h1.y / a1);
-            sr = cmul(u, make_double2(h2.x / hyp, h2.y / hyp));
+    if (lane == 0) s_col[k + 1] = make_double2(sqrt(hn2[s]), 0.0);
+    __syncthreads();
+    if (lane == 0) {
+        double2 h1 = s_col[0];  // row i of the column after the rotations before i
+        for (int i = 0; i < k; ++i) {
+            const double2 h2 = s_col[i + 1];
+            const double ci = s_c[i];
+            const double2 si = s_s[i];
+            const double2 a = cmulc(si, h2);  // conj(s) h2
+            const double2 b = cmul(si, h1);   // s h1
+            s_col[i] = make_double2(ci * h1.x + a.x, ci * h1.y + a.y);
+            h1 = make_double2(-b.x + ci * h2.x, -b.y + ci * h2.y);
         }
-    }
-    c[k] = cr;
-    sv[k] = sr;
-    {
+        const double2 h2 = s_col[k + 1];
+        double cr = 1.0;
+        double2 sr = make_double2(0.0, 0.0);
+        if (h2.x != 0.0 || h2.y != 0.0) {
+            const double a1 = hypot(h1.x, h1.y);
+            if (a1 == 0.0) {
+                cr = 0.0;
+                sr = make_double2(1.0, 0.0);
+            } else {
+                const double a2 = hypot(h2.x, h2.y);
+                const double scl = a1 + a2;
+                const double n1 = a1 / scl, n2 = a2 / scl;
+                const double hyp = scl * sqrt(n1 * n1 + n2 * n2);
+                cr = a1 / hyp;
+                // (conj(h1)/|h1|) * (h2 / hyp)
+                const double2 u = make_double2(h1.x / a1, -h1.y / a1);
+                sr = cmul(u, make_double2(h2.x / hyp, h2.y / hyp));
+            }
+        }
+        c[k] = cr;
+        sv[k] = sr;
         const double2 a = cmulc(sr, h2);
-        Hs[k * m + k] = make_double2(cr * h1.x + a.x, cr * h1.y + a.y);
-        Hs[(k + 1) * m + k] = make_double2(0.0, 0.0);
+        s_col[k] = make_double2(cr * h1.x + a.x, cr * h1.y + a.y);
+        s_col[k + 1] = make_double2(0.0, 0.0);
+        const double2 g1 = gs[k], g2 = gs[k + 1];
+        const double2 ga = cmulc(sr, g2), gb = cmul(sr, g1);
+        gs[k] = make_double2(cr * g1.x + ga.x, cr * g1.y + ga.y);
+        const double2 gk1 = make_double2(-gb.x + cr * g2.x, -gb.y + cr * g2.y);
+        gs[k + 1] = gk1;
+        resid[s] = hypot(gk1.x, gk1.y);
     }
-    const double2 g1 = gs[k], g2 = gs[k + 1];
-    const double2 a = cmulc(sr, g2), b = cmul(sr, g1);
-    gs[k] = make_double2(cr * g1.x + a.x, cr * g1.y + a.y);
-    gs[k + 1] = make_double2(-b.x + cr * g2.x, -b.y + cr * g2.y);
-    resid[s] = hypot(gs[k + 1].x, gs[k + 1].y);
+    __syncthreads();
+    for (int i = lane; i <= k + 1; i += 64) Hs[i * m + k] = s_col[i];
 }
 
 // y = H^-1 g (upper triangular, kk columns per system); gmres_solver.cu:187-195.
-__global__ void backsolve_kernel(Ptrs p, int nslots, const int* __restrict__ kk_of, int m,
-                                 const double2* __restrict__ H, const double2* __restrict__ g,
-                                 double2* __restrict__ y) {
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per system: the upper triangle is staged in LDS (packed by rows), then
+// column by column y_i = g_i / H_ii and g_j -= H_ji y_i for j < i across the lanes.
+__global__ __launch_bounds__(64) void backsolve_kernel(Ptrs p, int nslots, const int* __restrict__ kk_of, int m,
+                                                       const double2* __restrict__ H, const double2* __restrict__ g,
+                                                       double2* __restrict__ y) {
+    extern __shared__ double2 s_h[];  // row i holds H_ij, j = i..kk-1, from offset tri(i)
+    __shared__ double2 s_g[64];
+    __shared__ double2 s_y[64];
+    const int slot = blockIdx.x;
     if (slot >= nslots) return;
+    const int lane = threadIdx.x;
     const int s = p.sys[slot];
     const int kk = kk_of[s];
     const double2* Hs = H + (size_t)s * (m + 1) * m;
     const double2* gs = g + (size_t)s * (m + 1);
     double2* ys = y + (size_t)s * (m + 1);
+    auto tri = [&](int i) { return i * kk - i * (i - 1) / 2; };  // start of row i (its diagonal)
+    for (int i = 0; i < kk; ++i)
+        for (int j = i + lane; j < kk; j += 64) s_h[tri(i) + j - i] = Hs[i * m + j];
+    for (int i = lane; i < kk; i += 64) s_g[i] = gs[i];
+    __syncthreads();
     for (int i = kk - 1; i >= 0; --i) {
-        double2 v = gs[i];
-        for (int j = i + 1; j < kk; ++j) {
-            const double2 t = cmul(Hs[i * m + j], ys[j]);
-            v.x -= t.x;
-            v.y -= t.y;
+        if (lane == 0) {
+            const double2 d = s_h[tri(i)], v = s_g[i];
+            const double den = d.x * d.x + d.y * d.y;
+            s_y[i] = make_double2((v.x * d.x + v.y * d.y) / den, (v.y * d.x - v.x * d.y) / den);
         }
-        const double2 d = Hs[i * m + i];
-        const double den = d.x * d.x + d.y * d.y;
-        ys[i] = make_double2((v.x * d.x + v.y * d.y) / den, (v.y * d.x - v.x * d.y) / den);
+        __syncthreads();
+        const double2 yi = s_y[i];
+        for (int j = lane; j < i; j += 64) {
+            const double2 t = cmul(s_h[tri(j) + i - j], yi);
+            s_g[j] = make_double2(s_g[j].x - t.x, s_g[j].y - t.y);
+        }
+        __syncthreads();
     }
+    for (int i = lane; i < kk; i += 64) ys[i] = s_y[i];
 }
 
 // x += sum_{i<kk} y_i V_i  (a = V_0, b = x)
@@ -632,7 +664,7 @@ static void gmres_group(Ctx& C, const double2* b, double2* x, int B, mpm_solve_r
             }
             Ptrs pc = make_ptrs(in_cycle, [&](int q) { return W.w.p + q * n; }, [&](int q) { return Vk(q, k + 1); });
             hipLaunchKernelGGL(scale_copy_kernel, grid_for(n, ns), dim3(256), 0, C.s, pc, n, W.nrm.p);
-            hipLaunchKernelGGL(givens_kernel, dim3((ns + 63) / 64), dim3(64), 0, C.s, pw, ns, k, m, W.H.p, W.h.p,
+            hipLaunchKernelGGL(givens_kernel, dim3(ns), dim3(64), 0, C.s, pw, ns, k, m, W.H.p, W.h.p,
                                W.nrm.p, W.g.p, W.cs.p, W.sn.p, W.resid.p);
             MPM_HIP(hipGetLastError());
             std::vector<double> res;
@@ -653,7 +685,9 @@ static void gmres_group(Ctx& C, const double2* b, double2* x, int B, mpm_solve_r
             std::vector<int> kkB(B, 0);
             for (int q : cyc) kkB[q] = kk[q];
             W.kk.upload(kkB.data(), B, C.s);
-            hipLaunchKernelGGL(backsolve_kernel, dim3((ns + 63) / 64), dim3(64), 0, C.s, px, ns, W.kk.p, m, W.H.p,
+            hipLaunchKernelGGL(backsolve_kernel, dim3(ns), dim3(64), (size_t)m * (m + 1) / 2 * sizeof(double2), C.s,
+                               px, ns,
+                               W.kk.p, m, W.H.p,
                                W.g.p, W.y.p);
             hipLaunchKernelGGL(xupdate_kernel, grid_for(n, ns), dim3(256), 0, C.s, px, n, n, W.kk.p, W.y.p, m + 1);
             MPM_HIP(hipGetLastError());
