@@ -86,6 +86,9 @@ _SIGS = {
     "mpm_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolveDesc), ctypes.c_void_p,
                                  ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(SolveReport),
                                  ctypes.c_void_p]),
+    "mpm_solve_adjoint": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolveDesc), ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(SolveReport),
+                                         ctypes.c_void_p]),
     "mpm_dipole_solver_create": (ctypes.c_int, [ctypes.POINTER(DipoleDesc), ctypes.POINTER(ctypes.c_void_p)]),
     "mpm_dipole_solver_destroy": (None, [ctypes.c_void_p]),
     "mpm_dipole_solver_compute": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _i64]),

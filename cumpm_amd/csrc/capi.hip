@@ -414,6 +414,29 @@ mpm_status mpm_solve(mpm_op* op, const mpm_solve_desc* desc, const double* b, do
     });
 }
 
+mpm_status mpm_solve_adjoint(mpm_op* op, const mpm_solve_desc* desc, const double* g, double* y, int B,
+                             mpm_solve_report* reports, void* stream) {
+    return guarded([&] {
+        MPM_REQUIRE(op && desc && g && y, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        // A^H = (field operator, Hermitian) + conj(self coefficients)
+        struct Conj {
+            mpm::EwaldOp& o;
+            hipStream_t s;
+            Conj(mpm::EwaldOp& op_, hipStream_t st) : o(op_), s(st) { o.conj_self_coef(s); }
+            ~Conj() {
+                try {
+                    o.conj_self_coef(s);
+                    (void)hipStreamSynchronize(s);
+                } catch (...) {
+                }
+            }
+        } conj(*op->impl, s);
+        mpm::solve_batched(*op->impl, *desc, reinterpret_cast<const double2*>(g), reinterpret_cast<double2*>(y), B,
+                           reports, s);
+    });
+}
+
 mpm_status mpm_dipole_solver_create(const mpm_dipole_desc* d, mpm_dipole_solver** out) {
     return guarded([&] {
         MPM_REQUIRE(d && out, MPM_ERR_INVALID_ARGUMENT, "null argument");

@@ -2402,6 +2402,19 @@ void EwaldOp::ensure_kspace_table() {
     kspace_key = key;
 }
 
+__global__ void conj_kernel(double2* __restrict__ v, size_t n) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        v[e].y = -v[e].y;
+}
+
+void EwaldOp::conj_self_coef(hipStream_t s) {
+    const size_t n = (size_t)self_rows * N;
+    if (n == 0) return;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(conj_kernel, dim3(blocks), dim3(256), 0, s, d_selfc.p, n);
+    MPM_HIP(hipGetLastError());
+}
+
 void EwaldOp::set_bloch(double kx, double ky) {
     if (kx == kb[0] && ky == kb[1]) return;
     kb[0] = kx;

@@ -118,6 +118,8 @@ class EwaldOp {
     void set_positions(const double* xyz, int64_t N);
     void set_bloch(double kx, double ky);
     void set_self_coef(const double* coef, int rows);
+    // conjugates the device self coefficients in place (the adjoint operator's)
+    void conj_self_coef(hipStream_t s);
     // y[b] = A x[b] (or M^-1 A x[b] with the left Jacobi preconditioner) for the slots
     // in `t` (count B <= kMaxSlots)
     void apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t s, bool jacobi = false);

@@ -137,6 +137,14 @@ typedef struct {
 mpm_status mpm_solve(mpm_op* op, const mpm_solve_desc* desc, const double* b, double* x, int B,
                      mpm_solve_report* reports, void* stream);
 
+/* Adjoint solve A^H y = g for B systems (the gradient solve of an autograd caller;
+ * mpm_solve_adjoint of the proposed C ABI, no counterpart in the reference code).
+ * The field operator is Hermitian (real symmetric without a Bloch wave vector),
+ * so A^H differs from A only in its conjugated self coefficients: the same
+ * batched solver runs with them.  Arguments as mpm_solve, with g for b and y for x. */
+mpm_status mpm_solve_adjoint(mpm_op* op, const mpm_solve_desc* desc, const double* g, double* y, int B,
+                             mpm_solve_report* reports, void* stream);
+
 /* ---- Dipole_Solver driver (src/cpp/dipole_solver.cu, dipole_solver_bindings.cpp:80-179) ---- */
 typedef struct mpm_dipole_solver mpm_dipole_solver;
 

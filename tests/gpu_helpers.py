@@ -55,8 +55,12 @@ class Op:
         yb.download(out)
         return out
 
+    def solve_adjoint(self, g, y0=None, **kw):
+        """A^H y = g through mpm_solve_adjoint."""
+        return self.solve(g, y0, _entry="mpm_solve_adjoint", **kw)
+
     def solve(self, b, x0=None, tol=1e-8, method=C.GMRES, restart=50, maxiter=1000, jacobi=True,
-              precision=C.SOLVE_DOUBLE):
+              precision=C.SOLVE_DOUBLE, _entry="mpm_solve"):
         b = np.ascontiguousarray(b, np.complex128).reshape(-1, self.N, 3)
         B = b.shape[0]
         x0 = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, np.complex128).reshape(b.shape)
@@ -67,7 +71,7 @@ class Op:
         d = C.SolveDesc(method=method, restart=restart, maxiter=maxiter, tol=tol, jacobi=int(jacobi),
                         precision=precision)
         reps = (C.SolveReport * B)()
-        C.check(C.load().mpm_solve(self.h, ctypes.byref(d), bb.ptr, xb.ptr, B, reps, None))
+        C.check(getattr(C.load(), _entry)(self.h, ctypes.byref(d), bb.ptr, xb.ptr, B, reps, None))
         x = np.zeros_like(b)
         xb.download(x)
         return x, [reps[i].as_dict() for i in range(B)]
