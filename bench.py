@@ -142,6 +142,35 @@ def cpu_baseline(pos, box, a, eps, iters_needed, budget_s=20.0):
     }
 
 
+def copy_calibration(_capi, mib=256, reps=20):
+    """Launch the library's plain copy kernel on `mib` MiB (read once, written once) and print
+    its duration: under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per dispatch, divided by the
+    true bytes, is the factor that turns the counters of the operator's kernels into bytes."""
+    import ctypes
+    lib = _capi.load()
+    nbytes = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    _capi.check(lib.mpm_buffer_alloc(nbytes, ctypes.byref(src)))
+    _capi.check(lib.mpm_buffer_alloc(nbytes, ctypes.byref(dst)))
+    try:
+        host = np.ones(nbytes // 8)
+        _capi.check(lib.mpm_buffer_upload(src, host.ctypes.data, nbytes))
+        del host
+        for _ in range(5):
+            _capi.check(lib.mpm_profile_copy(dst, src, nbytes, None))
+        _capi.check(lib.mpm_device_synchronize())
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _capi.check(lib.mpm_profile_copy(dst, src, nbytes, None))
+        _capi.check(lib.mpm_device_synchronize())
+        dt = (time.perf_counter() - t0) / reps
+    finally:
+        lib.mpm_buffer_free(src)
+        lib.mpm_buffer_free(dst)
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": reps + 5,
+                      "us_per_launch": dt * 1e6, "GBps": 2 * nbytes / dt / 1e9}))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +181,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 reference-order solve")
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup + steps of the batched solve only (for rocprofv3)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 256 MiB device copy (the known-traffic kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the sweep's 30 independent systems fit one MI355X, and "
@@ -161,6 +193,10 @@ def main():
 
     import cumpm_amd
     from cumpm_amd import _capi
+
+    if args.copy_calibration:
+        copy_calibration(_capi)
+        return
 
     pos, box, a, omega, eps = hex_monolayer_config()
     N = len(pos)
@@ -188,7 +224,9 @@ def main():
     info = s_mix.get_op_info()
     log(f"batched+refined: {ms_step:.1f} ms/step -> {value:.1f} RHS-solves/s")
     if args.profile_only:
-        print(json.dumps({"metric": METRIC, "value": value, "ms_per_step": ms_step}))
+        print(json.dumps({"metric": METRIC, "value": value, "ms_per_step": ms_step,
+                          "matvecs_per_solve": sum(r["matvecs"] for w in reps_mix for r in w),
+                          "solves": max(args.warmup, 1) + args.steps}))
         return
 
     # ---- kernel-class timings (HIP events on the launch stream) ----

@@ -109,6 +109,7 @@ _SIGS = {
     "mpm_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "mpm_profile_reset": (ctypes.c_int, []),
     "mpm_profile_get": (ctypes.c_int, [ctypes.c_char_p, _dp, ctypes.POINTER(_i64)]),
+    "mpm_profile_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
 }
 
 EXPORTED = tuple(_SIGS)

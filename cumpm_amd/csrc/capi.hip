@@ -615,6 +615,20 @@ mpm_status mpm_device_synchronize(void) {
     return guarded([&] { MPM_HIP(hipDeviceSynchronize()); });
 }
 
+}  // extern "C"
+
+namespace {
+// Plain vector copy, 16 B per thread and iteration: the kernel of known traffic against which
+// the HBM counters (FETCH_SIZE, WRITE_SIZE) of the operator's kernels are scaled.
+__global__ void __launch_bounds__(256) profile_copy_kernel(const float4* __restrict__ src,
+                                                           float4* __restrict__ dst, size_t n) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" {
+
 mpm_status mpm_profile_enable(int enable) {
     return guarded([&] { mpm::profile_set(enable != 0); });
 }
@@ -627,6 +641,20 @@ mpm_status mpm_profile_get(const char* name, double* total_ms, int64_t* count) {
     return guarded([&] {
         MPM_REQUIRE(name && total_ms && count, MPM_ERR_INVALID_ARGUMENT, "null argument");
         mpm::profile_get(name, *total_ms, *count);
+    });
+}
+
+mpm_status mpm_profile_copy(void* dst, const void* src, size_t bytes, void* stream) {
+    return guarded([&] {
+        MPM_REQUIRE(dst && src, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        MPM_REQUIRE(bytes % 16 == 0 && reinterpret_cast<uintptr_t>(dst) % 16 == 0 &&
+                        reinterpret_cast<uintptr_t>(src) % 16 == 0,
+                    MPM_ERR_INVALID_ARGUMENT, "copy needs 16-byte aligned pointers and size");
+        const size_t n = bytes / 16;
+        if (!n) return;
+        hipLaunchKernelGGL(profile_copy_kernel, dim3(4096), dim3(256), 0, static_cast<hipStream_t>(stream),
+                           static_cast<const float4*>(src), static_cast<float4*>(dst), n);
+        MPM_HIP(hipGetLastError());
     });
 }
 

@@ -227,6 +227,9 @@ mpm_status mpm_profile_reset(void);
  * "zfused" (slab Green's tensor), "contract" (matrix-core contraction), "gather" (contraction and/or
  * real space, self term), "krylov"; total_ms over count launches. */
 mpm_status mpm_profile_get(const char* name, double* total_ms, int64_t* count);
+/* Plain device copy of `bytes` (a multiple of 16, 16-byte aligned) from src to dst on `stream`:
+ * no counterpart in the reference; a kernel of known traffic for calibrating hardware counters. */
+mpm_status mpm_profile_copy(void* dst, const void* src, size_t bytes, void* stream);
 
 #ifdef __cplusplus
 }

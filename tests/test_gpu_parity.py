@@ -337,3 +337,19 @@ def test_dipole_solver_input_errors_need_gpu():
     s.compute(np.array([[1.0, 1.0, 1.0], [6.0, 6.0, 6.0]]))
     with pytest.raises(RuntimeError, match="changed"):
         s.compute(np.array([[1.0, 1.0, 1.0]]))
+
+
+def test_profile_copy_moves_exact_bytes():
+    """mpm_profile_copy (the counter-calibration kernel) copies exactly `bytes`."""
+    _case("profile copy")
+    src = np.random.default_rng(3).standard_normal(4099 * 2)
+    a, b = C.DeviceBuffer(src.nbytes), C.DeviceBuffer(src.nbytes)
+    a.upload(src)
+    b.upload(np.zeros_like(src))
+    n = (src.nbytes // 16 - 1) * 16
+    C.check(C.load().mpm_profile_copy(b.ptr, a.ptr, n, None))
+    out = np.zeros_like(src)
+    b.download(out)
+    assert np.array_equal(out[: n // 8], src[: n // 8]) and not out[n // 8:].any()
+    with pytest.raises(ValueError):
+        C.check(C.load().mpm_profile_copy(b.ptr, a.ptr, 24, None))
