@@ -96,15 +96,26 @@ def algorithmic_bytes(cls, info, N, B, fp32, pairs):
     return 0
 
 
-# HBM traffic per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
-# workload (profiles/hex4000_sweep_batched_refined.md; FETCH_SIZE scaled by the copy-kernel
-# calibration recorded there), and the mean batch of the profiled launches.
-# Per-launch values are dispatch-weighted over the FP32 and FP64 launches of the class.
+def kernel_class(name):
+    """Kernel class of a kernel name as profilers print it (template arguments included)."""
+    base = name.split("<")[0]
+    return {"spread_mfma_kernel": "spread", "spread_kernel": "spread", "line_fft2_kernel": "fft_axis",
+            "line_fft_kernel": "fft_axis", "ztoeplitz_kernel": "zfused", "zfused_kernel": "zfused",
+            "scale_kernel": "scale", "contract_mfma_kernel": "contract", "contract_nb_mfma_kernel": "contract",
+            "finish_kernel": "gather", "gather_kernel": "gather"}[base]
+
+
+def kernel_bytes(name, info, N, B, pairs):
+    return algorithmic_bytes(kernel_class(name), info, N, B, "<float" in name, pairs)
+
+
+# HBM bytes per dispatch (FETCH_SIZE + WRITE_SIZE, FETCH_SIZE scaled by the copy-kernel
+# calibration) measured with rocprofv3 --pmc on this workload, and the mean right-hand
+# sides per dispatch of that kernel in the profiled run; from
+# profiles/hex4000_sweep_batched_refined.md.
 MEASURED_TRAFFIC = {
-    "contract": (116 * (147.6 + 6.7) + 4 * (416.7 + 24.1)) / 120 * 1e6,
-    "zfused": (116 * (155.9 + 73.6) + 4 * (451.5 + 289.6)) / 120 * 1e6,
+    "contract_mfma_kernel<float, 4, 10>": ((110.1 + 8.4) * 1e6, 18.12),
 }
-MEASURED_MEAN_B = 14.68
 
 
 def cpu_baseline(pos, box, a, eps, iters_needed, budget_s=20.0):
@@ -205,6 +216,9 @@ def main():
 
     # ---- batched + mixed (refined) ----
     s_mix = make_solver(cumpm_amd, box, eps, a, "refined", True, args.tol)
+    if args.profile_only:  # per-kernel launch and RHS counts of every solve below
+        _capi.profile_reset()
+        _capi.profile_enable(True)
     t0 = time.perf_counter()
     for _ in range(max(args.warmup, 1)):
         run_once(s_mix, pos)
@@ -224,9 +238,15 @@ def main():
     info = s_mix.get_op_info()
     log(f"batched+refined: {ms_step:.1f} ms/step -> {value:.1f} RHS-solves/s")
     if args.profile_only:
+        _capi.profile_enable(False)
+        kern = {}
+        for n in _capi.profile_names():
+            if "<" in n:
+                ms, cnt, units = _capi.profile_get(n)
+                kern[n] = {"launches": cnt, "mean_rhs": units / max(cnt, 1)}
         print(json.dumps({"metric": METRIC, "value": value, "ms_per_step": ms_step,
                           "matvecs_per_solve": sum(r["matvecs"] for w in reps_mix for r in w),
-                          "solves": max(args.warmup, 1) + args.steps}))
+                          "solves": max(args.warmup, 1) + args.steps, "kernels": kern}))
         return
 
     # ---- kernel-class timings (HIP events on the launch stream) ----
@@ -234,12 +254,12 @@ def main():
     _capi.profile_enable(True)
     run_once(s_mix, pos)
     _capi.profile_enable(False)
-    classes = {}
-    for cls in ("spread", "fft", "fft_axis0", "fft_axis1", "fft_axis2", "scale", "zfused", "contract", "gather",
-                "krylov"):
-        ms, cnt = _capi.profile_get(cls)
+    classes, kernels = {}, {}
+    for name in _capi.profile_names():
+        ms, cnt, units = _capi.profile_get(name)
         if cnt:
-            classes[cls] = {"total_ms": ms, "launches": cnt}
+            (kernels if "<" in name else classes)[name] = {"total_ms": ms, "launches": cnt,
+                                                           "mean_rhs": units / cnt}
 
     # ---- plain FP64, reference order ----
     plain = None
@@ -273,36 +293,25 @@ def main():
     else:
         accuracy = None
 
-    # ---- roofline of the dominant kernel (single-kernel classes only) ----
+    # ---- roofline of the dominant kernel (one template instance, HIP events) ----
     pairs = int(info.num_pairs)
-    single = [c for c in ("spread", "fft_axis0", "fft_axis1", "fft_axis2", "scale", "zfused", "contract", "gather")
-              if c in classes]
-    dom = max(single, key=lambda c: classes[c]["total_ms"])
-    # average batch size of the launches: total RHS-applies / launches of the spread
-    mv_total = sum(r["matvecs"] for w in reps_mix for r in w)
-    mv64 = sum(r["matvecs_fp64"] for w in reps_mix for r in w)
-    n_launch = max(classes["spread"]["launches"], 1)
-    mean_B = mv_total / n_launch
-    frac32 = (mv_total - mv64) / max(mv_total, 1)
-
-    def launch_bytes(c):
-        return (frac32 * algorithmic_bytes(c, info, N, mean_B, True, pairs) +
-                (1 - frac32) * algorithmic_bytes(c, info, N, mean_B, False, pairs))
-
+    for name, v in kernels.items():
+        v["avg_ms"] = v["total_ms"] / v["launches"]
+        v["algorithmic_GBps"] = kernel_bytes(name, info, N, v["mean_rhs"], pairs) / (v["avg_ms"] / 1e3) / 1e9
     for c, v in classes.items():
-        v["avg_ms"] = v["total_ms"] / max(v["launches"], 1)
-        if c != "krylov":
-            v["algorithmic_GBps"] = launch_bytes(c) / (v["avg_ms"] / 1e3) / 1e9
-    per_launch_ms = classes[dom]["avg_ms"]
-    bytes_per_launch = launch_bytes(dom)
-    achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
+        v["avg_ms"] = v["total_ms"] / v["launches"]
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    kd = kernels[dom]
+    bytes_per_launch = kernel_bytes(dom, info, N, kd["mean_rhs"], pairs)
+    achieved = bytes_per_launch / (kd["avg_ms"] / 1e3) / 1e9
     traffic = None
-    if dom in MEASURED_TRAFFIC and MEASURED_MEAN_B:
-        traffic = MEASURED_TRAFFIC[dom] * mean_B / MEASURED_MEAN_B
+    if dom in MEASURED_TRAFFIC:  # scaled to this run's right-hand sides per launch
+        t_bytes, t_rhs = MEASURED_TRAFFIC[dom]
+        traffic = t_bytes * kd["mean_rhs"] / t_rhs
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
-                "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": per_launch_ms,
-                "mean_rhs_per_launch": mean_B}
+                "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": kd["avg_ms"],
+                "mean_rhs_per_launch": kd["mean_rhs"], "launches": kd["launches"]}
 
     # ---- CPU oracle ----
     cpu = None
@@ -339,6 +348,7 @@ def main():
         "batched_matvecs_fp64_per_rhs": [r["matvecs_fp64"] for w in reps_mix for r in w],
         "end_to_end_wall_rhs_per_s": B * args.steps / wall,
         "kernel_classes": classes,
+        "kernels": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -108,7 +108,8 @@ _SIGS = {
     "mpm_device_synchronize": (ctypes.c_int, []),
     "mpm_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "mpm_profile_reset": (ctypes.c_int, []),
-    "mpm_profile_get": (ctypes.c_int, [ctypes.c_char_p, _dp, ctypes.POINTER(_i64)]),
+    "mpm_profile_get": (ctypes.c_int, [ctypes.c_char_p, _dp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "mpm_profile_names": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "mpm_profile_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
 }
 
@@ -196,7 +197,18 @@ def profile_reset():
 
 
 def profile_get(name):
+    """(total ms, launches, right-hand sides summed over the launches) of a class or kernel."""
     ms = ctypes.c_double()
     cnt = _i64()
-    check(load().mpm_profile_get(name.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
-    return ms.value, cnt.value
+    units = _i64()
+    check(load().mpm_profile_get(name.encode(), ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(units)))
+    return ms.value, cnt.value, units.value
+
+
+def profile_names():
+    """Names (classes and kernels) with recorded launches."""
+    need = ctypes.c_size_t()
+    check(load().mpm_profile_names(None, 0, ctypes.byref(need)))
+    buf = ctypes.create_string_buffer(need.value + 64)
+    check(load().mpm_profile_names(buf, len(buf), ctypes.byref(need)))
+    return [n for n in buf.value.decode().split("\n") if n]

@@ -27,7 +27,8 @@ struct mpm_op {
 namespace mpm {
 void profile_set(bool on);
 void profile_reset();
-bool profile_get(const std::string& name, double& ms, int64_t& count);
+bool profile_get(const std::string& name, double& ms, int64_t& count, int64_t& units);
+std::string profile_names();
 }  // namespace mpm
 
 namespace {
@@ -637,10 +638,21 @@ mpm_status mpm_profile_reset(void) {
     return guarded([&] { mpm::profile_reset(); });
 }
 
-mpm_status mpm_profile_get(const char* name, double* total_ms, int64_t* count) {
+mpm_status mpm_profile_get(const char* name, double* total_ms, int64_t* count, int64_t* units) {
     return guarded([&] {
         MPM_REQUIRE(name && total_ms && count, MPM_ERR_INVALID_ARGUMENT, "null argument");
-        mpm::profile_get(name, *total_ms, *count);
+        int64_t u = 0;
+        mpm::profile_get(name, *total_ms, *count, u);
+        if (units) *units = u;
+    });
+}
+
+mpm_status mpm_profile_names(char* buf, size_t cap, size_t* needed) {
+    return guarded([&] {
+        MPM_REQUIRE(needed && (buf || cap == 0), MPM_ERR_INVALID_ARGUMENT, "null argument");
+        const std::string names = mpm::profile_names();
+        *needed = names.size() + 1;
+        if (cap >= names.size() + 1) std::memcpy(buf, names.c_str(), names.size() + 1);
     });
 }
 

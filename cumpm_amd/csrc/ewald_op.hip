@@ -166,12 +166,14 @@ namespace {
 struct ProfEntry {
     double ms = 0;
     int64_t count = 0;
+    int64_t units = 0;
 };
 std::mutex g_prof_mu;
 bool g_prof_on = false;
 std::map<std::string, ProfEntry> g_prof;
 struct Pending {
-    std::string name;
+    std::string name, kname;
+    int64_t units;
     hipEvent_t e0, e1;
 };
 std::vector<Pending> g_pending;
@@ -179,17 +181,29 @@ std::vector<Pending> g_pending;
 
 bool profiling_enabled() { return g_prof_on; }
 
-ProfScope::ProfScope(const char* n, hipStream_t s) : name(n), stream(s) {
+ProfScope::ProfScope(const char* n, hipStream_t s, int64_t u) : name(n), units(u), stream(s) {
     if (!g_prof_on) return;
     MPM_HIP(hipEventCreate(&e0));
     MPM_HIP(hipEventCreate(&e1));
     MPM_HIP(hipEventRecord(e0, stream));
 }
+void ProfScope::kernel(const char* base, std::initializer_list<std::string> targs) {
+    if (!e0) return;
+    kname = base;
+    kname += '<';
+    bool first = true;
+    for (const std::string& a : targs) {
+        if (!first) kname += ", ";
+        kname += a;
+        first = false;
+    }
+    kname += '>';
+}
 ProfScope::~ProfScope() {
     if (!e0) return;
     (void)hipEventRecord(e1, stream);
     std::lock_guard<std::mutex> g(g_prof_mu);
-    g_pending.push_back({name, e0, e1});
+    g_pending.push_back({name, kname, units, e0, e1});
 }
 
 void profile_flush() {
@@ -198,9 +212,13 @@ void profile_flush() {
         (void)hipEventSynchronize(p.e1);
         float ms = 0;
         (void)hipEventElapsedTime(&ms, p.e0, p.e1);
-        auto& e = g_prof[p.name];
-        e.ms += ms;
-        e.count += 1;
+        for (const std::string* k : {&p.name, &p.kname}) {
+            if (k->empty()) continue;
+            auto& e = g_prof[*k];
+            e.ms += ms;
+            e.count += 1;
+            e.units += p.units;
+        }
         (void)hipEventDestroy(p.e0);
         (void)hipEventDestroy(p.e1);
     }
@@ -215,18 +233,30 @@ void profile_reset() {
     std::lock_guard<std::mutex> g(g_prof_mu);
     g_prof.clear();
 }
-bool profile_get(const std::string& name, double& ms, int64_t& count) {
+bool profile_get(const std::string& name, double& ms, int64_t& count, int64_t& units) {
     profile_flush();
     std::lock_guard<std::mutex> g(g_prof_mu);
     auto it = g_prof.find(name);
     if (it == g_prof.end()) {
         ms = 0;
         count = 0;
+        units = 0;
         return false;
     }
     ms = it->second.ms;
     count = it->second.count;
+    units = it->second.units;
     return true;
+}
+std::string profile_names() {
+    profile_flush();
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    std::string out;
+    for (const auto& kv : g_prof) {
+        out += kv.first;
+        out += '\n';
+    }
+    return out;
 }
 
 
@@ -1025,7 +1055,7 @@ struct LineFft {
                     "reciprocal grid too large for one transform call");
         static const char* kPassNames[] = {"fft_axis0", "fft_axis1", "fft_axis2"};
         for (const LinePass& ps : (forward ? fwd : inv)) {
-            ProfScope pscope(kPassNames[std::min(ps.axis, 2)], s);
+            ProfScope pscope(kPassNames[std::min(ps.axis, 2)], s, B);
             LineAxis& ax = *axes[ps.axis];
             LineGeom gi = ps.gin, go = ps.gout;
             gi.nlines *= B;
@@ -1039,6 +1069,7 @@ struct LineFft {
                 const int TL = lines_per_block2<R>(n, ax.plan.radix[0], ax.plan.radix[1]);
                 const size_t lds = ((size_t)TL * n + n) * sizeof(CT);
                 const dim3 gd((unsigned)((gi.nlines + TL - 1) / TL));
+                pscope.kernel("line_fft2_kernel", {type_name<R>(), tbool(!forward), std::to_string(ax.cls)});
 #define MPM_L2(INV)                                                                       \
     if (ax.cls == 0) launch2<R, INV, 0>(gd, lds, s, in, out, gi, go, ax.plan, tw, TL);    \
     else if (ax.cls == 1) launch2<R, INV, 1>(gd, lds, s, in, out, gi, go, ax.plan, tw, TL); \
@@ -1056,6 +1087,7 @@ struct LineFft {
             const size_t lds = ((size_t)2 * TL * n + 2 * n) * sizeof(CT);
             const long long blocks = (gi.nlines + TL - 1) / TL;
             const dim3 gd((unsigned)blocks);
+            pscope.kernel("line_fft_kernel", {type_name<R>(), tbool(!forward), std::to_string(ax.cls)});
             if (forward) {
                 if (ax.cls == 0) launch<R, false, 0>(gd, lds, s, in, out, gi, go, ax.plan, tw, TL);
                 else if (ax.cls == 1) launch<R, false, 1>(gd, lds, s, in, out, gi, go, ax.plan, tw, TL);
@@ -2832,11 +2864,12 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
     const int Ni = (int)N;
     if (parts & MPM_PART_RECIP) {
         if (SLAB && reg_rx > 0) {
-            ProfScope ps("spread", s);
+            ProfScope ps("spread", s, B);
             constexpr int CPB = 32;  // SpreadShape<R, ...>::CPB
             const dim3 gs((unsigned)(reg_nrx * reg_nry), (unsigned)((6 * B + CPB - 1) / CPB));
 #define MPM_SM(RXV, ZSV)                                                                                          \
     case ZSV:                                                                                                     \
+        ps.kernel("spread_mfma_kernel", {type_name<R>(), #RXV, #ZSV});                                            \
         hipLaunchKernelGGL((spread_mfma_kernel<R, RXV, ZSV>), gs, dim3(256), 0, s, t, B, g, regw, d_reg_start.p, \
                            d_reg_items.p, reg_nry, Ni, grid);                                                    \
         break;
@@ -2854,7 +2887,8 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
 #undef MPM_SM
             MPM_HIP(hipGetLastError());
         } else {
-            ProfScope ps("spread", s);
+            ProfScope ps("spread", s, B);
+            ps.kernel("spread_kernel", {type_name<R>(), B == 1 ? "1" : "4", tbool(SLAB)});
             const int nt = ntiles[0] * ntiles[1] * n_ztiles;
             if (B == 1)
                 hipLaunchKernelGGL((spread_kernel<R, 1, SLAB>), dim3(nt, 1), dim3(256), 0, s, t, B, g, d_base.p, w,
@@ -2868,14 +2902,14 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
         CT* kgrid = grid;
         if (kt) kgrid = fp32 ? reinterpret_cast<CT*>(d_kgrid32.p) : reinterpret_cast<CT*>(d_kgrid64.p);
         {
-            ProfScope ps("fft", s);
+            ProfScope ps("fft", s, B);
             if (lfft && fft_backend == 0)
                 lfft->exec<R>(grid, kgrid, B, true, s);
             else
                 fft->exec(grid, bytes_per_rhs, B, fp32, true, s);
         }
         if (SLAB && zcontig) {
-            ProfScope ps("zfused", s);
+            ProfScope ps("zfused", s, B);
             const int ncols = kspace_cols();
             const CT* T = reinterpret_cast<const CT*>(fp32 ? static_cast<const void*>(d_toe32.p)
                                                            : static_cast<const void*>(d_toe64.p));
@@ -2885,6 +2919,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
             const bool ri = (S.n[2] & 1) != 0;  // odd nz: real / imaginary table entries
 #define MPM_ZT(ZSV)                                                                                           \
     case ZSV:                                                                                                 \
+        ps.kernel("ztoeplitz_kernel", {type_name<R>(), #ZSV, std::to_string(ztoe_as<R, ZSV>()), tbool(ri)}); \
         if (ri)                                                                                               \
             hipLaunchKernelGGL((ztoeplitz_kernel<R, ZSV, ztoe_as<R, ZSV>(), true>), gridz, dim3(256), 0, s,   \
                                kgrid, B, ncols, Zs, rb, T);                                                   \
@@ -2906,7 +2941,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
 #undef MPM_ZT
             MPM_HIP(hipGetLastError());
         } else if (SLAB) {
-            ProfScope ps("zfused", s);
+            ProfScope ps("zfused", s, B);
             const int ncols = kspace_cols();
             const int zsb = zs_bucket(Zs);
             const size_t lds = zfused_lds(zsb, S.n[2], sizeof(R));
@@ -2914,8 +2949,11 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
             const int rb = B >= 16 ? 16 : (B >= 4 ? 4 : 1);
             const dim3 gridz((ncols + 256 / rb - 1) / (256 / rb), (B + rb - 1) / rb);
 #define MPM_ZF2(ZSV, RBV)                                                                                     \
-    hipLaunchKernelGGL((zfused_kernel<R, ZSV, RBV>), gridz, dim3(256), lds, s, kgrid, B, ncols, S.n[2], Zs, \
-                       d_zlist.p, kc)
+    do {                                                                                                      \
+        ps.kernel("zfused_kernel", {type_name<R>(), #ZSV, #RBV});                                             \
+        hipLaunchKernelGGL((zfused_kernel<R, ZSV, RBV>), gridz, dim3(256), lds, s, kgrid, B, ncols, S.n[2],  \
+                           Zs, d_zlist.p, kc);                                                                \
+    } while (0)
 #define MPM_ZF(ZSV)                     \
     case ZSV:                           \
         if (rb == 16)                   \
@@ -2940,14 +2978,15 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
 #undef MPM_ZF2
             MPM_HIP(hipGetLastError());
         } else {
-            ProfScope ps("scale", s);
+            ProfScope ps("scale", s, B);
+            ps.kernel("scale_kernel", {type_name<R>()});
             const int blocks = (int)std::min<size_t>((V + 255) / 256, 8192);
             const R* kc = fp32 ? reinterpret_cast<const R*>(d_k32.p) : reinterpret_cast<const R*>(d_k64.p);
             hipLaunchKernelGGL(scale_kernel<R>, dim3(blocks), dim3(256), 0, s, grid, kc, V, B);
             MPM_HIP(hipGetLastError());
         }
         {
-            ProfScope ps("fft", s);
+            ProfScope ps("fft", s, B);
             if (lfft && fft_backend == 0)
                 lfft->exec<R>(grid, kgrid, B, false, s);
             else
@@ -2956,7 +2995,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
     }
     const R* part = nullptr;
     if (SLAB && reg_rx > 0 && (parts & MPM_PART_RECIP)) {
-        ProfScope ps("contract", s);
+        ProfScope ps("contract", s, B);
         // column groups of 16 per block: 4 when there are enough right-hand sides,
         // else fewer, with the slab planes split over the remaining waves; one or
         // two column groups: one wave per candidate block instead
@@ -2967,6 +3006,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
         R* pbuf = fp32 ? reinterpret_cast<R*>(d_partial.p) : reinterpret_cast<R*>(d_partial64.p);
 #define MPM_CM(RXV, ZSV)                                                                                          \
     case ZSV:                                                                                                     \
+        ps.kernel(ngrp <= 2 ? "contract_nb_mfma_kernel" : "contract_mfma_kernel", {type_name<R>(), #RXV, #ZSV});   \
         if (ngrp <= 2)                                                                                            \
             hipLaunchKernelGGL((contract_nb_mfma_kernel<R, RXV, ZSV>), gn, dim3(256), 0, s, B, g, regw,          \
                                d_reg_start.p, d_reg_items.p, reg_nry, Ni, grid, pbuf);                           \
@@ -2990,14 +3030,16 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
         part = pbuf;
     }
     if (part && B >= 4) {
-        ProfScope ps("gather", s);
+        ProfScope ps("gather", s, B);
+        ps.kernel("finish_kernel", {type_name<R>()});
         const double h3 = S.h[0] * S.h[1] * S.h[2];
         const int blocks = (int)(((size_t)Ni * B + 255) / 256);
         hipLaunchKernelGGL((finish_kernel<R>), dim3(blocks), dim3(256), 0, s, t, B, parts, Ni, part, d_ps_start.p,
                            d_ps_items.p, h3, d_nb_start.p, d_nb_j.p, d_pairc.p, d_selfc.p, self_perp, jacobi);
         MPM_HIP(hipGetLastError());
     } else {
-        ProfScope ps("gather", s);
+        ProfScope ps("gather", s, B);
+        ps.kernel("gather_kernel", {type_name<R>(), B == 1 ? "1" : "4", tbool(SLAB)});
         const double h3 = S.h[0] * S.h[1] * S.h[2];
         const int blocks = (Ni + 3) / 4;
         if (B == 1)

@@ -5,10 +5,12 @@
 
 #include <complex>
 #include <cstdint>
+#include <initializer_list>
 #include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "cumpm_amd.h"
@@ -74,12 +76,21 @@ struct DevBuf {
 // Profiling (HIP events around kernel classes, on the launch stream).
 // ---------------------------------------------------------------------------
 struct ProfScope {
-    ProfScope(const char* name, hipStream_t s);
+    // units: right-hand sides the bracketed launches process (summed per class)
+    ProfScope(const char* name, hipStream_t s, int64_t units = 0);
     ~ProfScope();
+    // Also account the bracketed launch under the kernel's own name, written the way
+    // profilers print template instances: base<arg0, arg1, ...>.
+    void kernel(const char* base, std::initializer_list<std::string> targs);
     const char* name;
+    std::string kname;
+    int64_t units;
     hipStream_t stream;
     hipEvent_t e0 = nullptr, e1 = nullptr;
 };
+template <typename R>
+inline std::string type_name() { return std::is_same<R, float>::value ? "float" : "double"; }
+inline std::string tbool(bool b) { return b ? "true" : "false"; }
 bool profiling_enabled();
 
 // ---------------------------------------------------------------------------

@@ -223,10 +223,15 @@ mpm_status mpm_device_synchronize(void);
  * every launch of the named kernel class is bracketed by events and accumulated. */
 mpm_status mpm_profile_enable(int enable);
 mpm_status mpm_profile_reset(void);
-/* name: "spread", "fft" (one 2-D/3-D transform), "fft_axis0".."fft_axis2" (its passes), "scale",
- * "zfused" (slab Green's tensor), "contract" (matrix-core contraction), "gather" (contraction and/or
- * real space, self term), "krylov"; total_ms over count launches. */
-mpm_status mpm_profile_get(const char* name, double* total_ms, int64_t* count);
+/* name: a kernel class, "spread", "fft" (one 2-D/3-D transform), "fft_axis0".."fft_axis2" (its
+ * passes), "scale", "zfused" (slab Green's tensor), "contract" (matrix-core contraction), "gather"
+ * (contraction and/or real space, self term), "krylov"; or a kernel's own name with its template
+ * arguments as profilers print it (e.g. "contract_mfma_kernel<float, 4, 10>").  total_ms over count
+ * launches; units (may be NULL): right-hand sides summed over those launches. */
+mpm_status mpm_profile_get(const char* name, double* total_ms, int64_t* count, int64_t* units);
+/* Newline-separated names with recorded launches.  Writes them to buf when cap >= *needed
+ * (the length including the terminating zero). */
+mpm_status mpm_profile_names(char* buf, size_t cap, size_t* needed);
 /* Plain device copy of `bytes` (a multiple of 16, 16-byte aligned) from src to dst on `stream`:
  * no counterpart in the reference; a kernel of known traffic for calibrating hardware counters. */
 mpm_status mpm_profile_copy(void* dst, const void* src, size_t bytes, void* stream);
