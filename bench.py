@@ -69,7 +69,8 @@ def algorithmic_bytes(cls, info, N, B, fp32, pairs):
     the transform (one read and one write of the grid); "contract" reads the grid
     once and produces the reciprocal field of every particle (3 complex FP64 per
     particle and RHS); "zfused" is the slab Green's-tensor pass (grid read and
-    write plus its coefficient table); "gather" reads the dipoles of the
+    write plus its coefficient table); "xtoeplitz" is the same with the slab's x
+    transform and inverse x transform fused in (still one read and one write); "gather" reads the dipoles of the
     neighbours and of the particle and writes the result (pair data on the fly
     counts 20 B per pair)."""
     nx, ny, nz = list(info.grid)
@@ -85,6 +86,8 @@ def algorithmic_bytes(cls, info, N, B, fp32, pairs):
         return 2 * grid
     if cls == "fft":
         return 2 * 2 * grid if info.slab_planes else 3 * 2 * grid
+    if cls == "xtoeplitz":  # x transform + plane coupling + inverse x transform: one read, one write
+        return 2 * grid + 6 * info.slab_planes * nx * ny * c
     if cls == "zfused":
         return 2 * grid + (6 * info.slab_planes * nx * ny * c if info.slab_planes else 4 * V * r)
     if cls == "scale":
@@ -100,7 +103,7 @@ def kernel_class(name):
     """Kernel class of a kernel name as profilers print it (template arguments included)."""
     base = name.split("<")[0]
     return {"spread_mfma_kernel": "spread", "spread_kernel": "spread", "line_fft2_kernel": "fft_axis",
-            "line_fft_kernel": "fft_axis", "ztoeplitz_kernel": "zfused", "zfused_kernel": "zfused",
+            "line_fft_kernel": "fft_axis", "ztoeplitz_kernel": "zfused", "zfused_kernel": "zfused", "xtoeplitz_kernel": "xtoeplitz",
             "scale_kernel": "scale", "contract_mfma_kernel": "contract", "contract_nb_mfma_kernel": "contract",
             "finish_kernel": "gather", "gather_kernel": "gather"}[base]
 

@@ -1006,6 +1006,7 @@ struct LineFft {
     std::vector<std::unique_ptr<LineAxis>> axes;
     std::vector<LinePass> fwd, inv;
     long long rhs_elems = 0;  // elements per RHS (grid size)
+    int kt_x = -1;            // transposed slab layout: the axis of the contiguous x lines
 
     // lines per block: as many as fit the per-block element budget (at most 64)
     template <typename R>
@@ -1048,13 +1049,16 @@ struct LineFft {
                            pl.radix[0], pl.radix[1], tw, TL);
     }
 
+    // skip_axis: passes of this axis are left out (run fused with the Green's tensor)
     template <typename R>
-    void exec(typename C2<R>::T* grid, typename C2<R>::T* kgrid, int B, bool forward, hipStream_t s) {
+    void exec(typename C2<R>::T* grid, typename C2<R>::T* kgrid, int B, bool forward, hipStream_t s,
+              int skip_axis = -1) {
         using CT = typename C2<R>::T;
         MPM_REQUIRE((long long)B * rhs_elems < (1LL << 31), MPM_ERR_UNSUPPORTED,
                     "reciprocal grid too large for one transform call");
         static const char* kPassNames[] = {"fft_axis0", "fft_axis1", "fft_axis2"};
         for (const LinePass& ps : (forward ? fwd : inv)) {
+            if (ps.axis == skip_axis) continue;
             ProfScope pscope(kPassNames[std::min(ps.axis, 2)], s, B);
             LineAxis& ax = *axes[ps.axis];
             LineGeom gi = ps.gin, go = ps.gout;
@@ -1456,6 +1460,7 @@ __global__ __launch_bounds__(256) void scale_kernel(typename C2<R>::T* __restric
 // LDS.  Every voxel of the slab is written exactly once, without atomics; the
 // sum order is fixed (deterministic).
 constexpr int kMfmaRY = 16;    // y voxels per region (the MFMA N dimension)
+constexpr size_t kMaxFusedLds = 64 * 1024;  // LDS of one xtoeplitz_kernel block
 constexpr int kMfmaKC = 64;    // candidates staged per chunk
 
 typedef float mpm_f32x4 __attribute__((ext_vector_type(4)));
@@ -1582,12 +1587,11 @@ __global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, 
                 for (int xi = 0; xi < RX; ++xi) wx[xi] = s_w[kq][L::X + xi] * wy;
 #pragma unroll
                 for (int p = 0; p < ZH; ++p) wz[p] = (p0 + p < ZSB) ? s_w[kq][L::Z + p0 + p] : R(0);
+                // planes past Zs carry zero weights: no branch around the accumulators
 #pragma unroll
                 for (int p = 0; p < ZH; ++p) {
-                    if (p0 + p < Zs) {
 #pragma unroll
-                        for (int xi = 0; xi < RX; ++xi) acc[xi][p] = M::mac(a, wx[xi] * wz[p], acc[xi][p]);
-                    }
+                    for (int xi = 0; xi < RX; ++xi) acc[xi][p] = M::mac(a, wx[xi] * wz[p], acc[xi][p]);
                 }
             }
         }
@@ -1676,7 +1680,6 @@ __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g,
 
     for (int c0 = k0; c0 < k1; c0 += KC) {
         const int cnt = min(KC, k1 - c0);
-        const int nblk = (cnt + 15) >> 4;
         __syncthreads();
         stage_weights<R, RX, ZSB>(&s_w[0][0], cnt, regw + (size_t)c0 * L::NW);
         V4 acc[NB];
@@ -1693,27 +1696,41 @@ __global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g,
 #pragma unroll
                 for (int xi = 0; xi < RX; ++xi) wxl[nb][xi] = s_w[k][L::X + xi];
             }
-            for (int p = pg; p < Zs; p += npg) {
+            // the next plane's operands are loaded while this plane's products run
+            auto load_plane = [&](int p, R (&dst)[RX][4]) {
                 const unsigned poff = coff + 2u * (unsigned)p * plane;
-                R gv[RX][4];
 #pragma unroll
                 for (int xi = 0; xi < RX; ++xi)
 #pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        gv[xi][u] = (vmask >> (4 * xi + u)) & 1u ? gf[poff + xi * xstep + 8u * u] : R(0);
+                    for (int u = 0; u < 4; ++u) {
+                        // lanes outside the grid read their own in-grid voxel and drop it:
+                        // no divergent branch around the load
+                        const bool in = (vmask >> (4 * xi + u)) & 1u;
+                        const R v = gf[in ? poff + xi * xstep + 8u * u : poff];
+                        dst[xi][u] = in ? v : R(0);
+                    }
+            };
+            R gv[RX][4], gn[RX][4];
+            if (pg < Zs) load_plane(pg, gv);
+            for (int p = pg; p < Zs; p += npg) {
+                if (p + npg < Zs) load_plane(p + npg, gn);
                 R wz[NB];
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb) wz[nb] = s_w[nb * 16 + (lane & 15)][L::Z + p];
+                // every candidate block, filled or not (rows past cnt hold zero weights):
+                // a branch here would make the compiler copy the accumulators around it
 #pragma unroll
                 for (int xi = 0; xi < RX; ++xi)
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) {
-                        if (nb < nblk) {
-                            const R wxz = wxl[nb][xi] * wz[nb];
+                        const R wxz = wxl[nb][xi] * wz[nb];
 #pragma unroll
-                            for (int u = 0; u < 4; ++u) acc[nb] = M::mac(wyl[nb][u] * wxz, gv[xi][u], acc[nb]);
-                        }
+                        for (int u = 0; u < 4; ++u) acc[nb] = M::mac(wyl[nb][u] * wxz, gv[xi][u], acc[nb]);
                     }
+#pragma unroll
+                for (int xi = 0; xi < RX; ++xi)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) gv[xi][u] = gn[xi][u];
             }
         }
         // plane groups of one column group are summed in plane-group order
@@ -1784,8 +1801,12 @@ __global__ __launch_bounds__(256) void contract_nb_mfma_kernel(int B, SpreadGeom
 #pragma unroll
         for (int xi = 0; xi < RX; ++xi)
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                dst[xi][u] = (vmask >> (4 * xi + u)) & 1u ? gf[poff + xi * xstep + 8u * u] : R(0);
+            for (int u = 0; u < 4; ++u) {
+                // lanes outside the grid read their own in-grid voxel and drop it
+                const bool in = (vmask >> (4 * xi + u)) & 1u;
+                const R v = gf[in ? poff + xi * xstep + 8u * u : poff];
+                dst[xi][u] = in ? v : R(0);
+            }
     };
     for (int c0 = k0; c0 < k1; c0 += KC) {
         const int cnt = min(KC, k1 - c0);
@@ -1970,32 +1991,26 @@ constexpr int ztoe_as() {
     return sizeof(R) == 4 ? (ZS <= 12 ? ZS : 8) : (ZS <= 10 ? (ZS + 1) / 2 : (ZS <= 12 ? 2 : 1));
 }
 
-// Applies the Toeplitz plane coupling in place on the slab [B][3][Zs][column].
-// One thread per (column, right-hand side); a block covers (256 / rb) columns x rb
-// right-hand sides so the rb threads of a column share the table lines in cache.
-// ZS >= Zs is the compile-time plane bound (planes Zs..ZS-1 are skipped), AS the
-// number of output planes accumulated per pass (register budget).
+// Toeplitz plane coupling of one column: reads f[c][j] = base[(c Zs + j) ld],
+// writes g[c][a] = sum_b T_{a-b} f[c'][b] back over it.  Tc points at the
+// column's entry of the table [Zs][6][ncols].  ZS >= Zs is the compile-time plane
+// bound (planes Zs..ZS-1 are skipped), AS the number of output planes
+// accumulated per pass (register budget).
 // RI: with an odd nz every kz != 0 pairs with -kz, and the projector's xx, yy, zz,
 // xy entries are even in kz while xz, yz are odd, so T_d's first four entries are
 // real and the last two imaginary: each entry then costs two real FMAs per
 // complex input instead of four.
 template <typename R, int ZS, int AS, bool RI>
-__global__ __launch_bounds__(256) void ztoeplitz_kernel(typename C2<R>::T* __restrict__ slab, int B, int ncols,
-                                                        int Zs, int rb, const typename C2<R>::T* __restrict__ T) {
+__device__ __forceinline__ void toeplitz_column(typename C2<R>::T* base, int ld, int Zs,
+                                                const typename C2<R>::T* __restrict__ Tc, int ncols) {
     using CT = typename C2<R>::T;
-    const int cb = 256 / rb;
-    const int tc = threadIdx.x % cb, tb = threadIdx.x / cb;
-    const int col = blockIdx.x * cb + tc;
-    const int b = blockIdx.y * rb + tb;
-    if (col >= ncols || b >= B) return;
-    CT* base = slab + (size_t)b * 3 * Zs * ncols + col;
     CT f[3][ZS];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int j = 0; j < ZS; ++j) {
             if (j < Zs) {
-                f[c][j] = base[((size_t)c * Zs + j) * ncols];
+                f[c][j] = base[(size_t)(c * Zs + j) * ld];
             } else {
                 f[c][j].x = f[c][j].y = R(0);
             }
@@ -2015,7 +2030,7 @@ __global__ __launch_bounds__(256) void ztoeplitz_kernel(typename C2<R>::T* __res
             const int alo = a0 > dd ? a0 : dd;                        // a - dd >= 0
             const int ahi = (a0 + AS < ZS + dd ? a0 + AS : ZS + dd);  // a - dd < ZS, a < a0 + AS
             if (alo >= ahi || ad >= Zs) continue;
-            const CT* tp = T + (size_t)ad * 6 * ncols + col;
+            const CT* tp = Tc + (size_t)ad * 6 * ncols;
             R tr[6], ti[6];
 #pragma unroll
             for (int c = 0; c < 6; ++c) {
@@ -2066,8 +2081,159 @@ __global__ __launch_bounds__(256) void ztoeplitz_kernel(typename C2<R>::T* __res
                 CT o;
                 o.x = ar[c][k];
                 o.y = ai[c][k];
-                base[((size_t)c * Zs + a) * ncols] = o;
+                base[(size_t)(c * Zs + a) * ld] = o;
             }
+        }
+    }
+}
+
+// Applies the Toeplitz plane coupling in place on the slab [B][3][Zs][column].
+// One thread per (column, right-hand side); a block covers (256 / rb) columns x rb
+// right-hand sides so the rb threads of a column share the table lines in cache.
+template <typename R, int ZS, int AS, bool RI>
+__global__ __launch_bounds__(256) void ztoeplitz_kernel(typename C2<R>::T* __restrict__ slab, int B, int ncols,
+                                                        int Zs, int rb, const typename C2<R>::T* __restrict__ T) {
+    const int cb = 256 / rb;
+    const int tc = threadIdx.x % cb, tb = threadIdx.x / cb;
+    const int col = blockIdx.x * cb + tc;
+    const int b = blockIdx.y * rb + tb;
+    if (col >= ncols || b >= B) return;
+    toeplitz_column<R, ZS, AS, RI>(slab + (size_t)b * 3 * Zs * ncols + col, ncols, Zs, T + col, ncols);
+}
+
+// ---- x transform, Toeplitz coupling and inverse x transform in one pass ----
+//
+// One block per (ky row, right-hand side) of the transposed k-space buffer
+// [B][3][Zs][ky][nxp] (after the y transform): the 3 Zs x-lines of the row are
+// transformed along x into LDS with two register stages n = P1 P2 (decimation
+// in time: global -> LDS, then in place), coupled across planes column by
+// column (toeplitz_column on LDS), transformed back (decimation in frequency:
+// in place, then LDS -> global) and written over their input.  With input index
+// j = j2 + P2 j1 and output index k = k1 + P1 k2,
+//   X[k] = sum_j2 W_P2^{j2 k2} W_n^{j2 k1} sum_j1 W_P1^{j1 k1} x[j2 + P2 j1].
+// It replaces the forward x pass, the Toeplitz pass and the inverse x pass
+// (three reads and three writes of the slab) by one read and one write.
+// Blocks are numbered so that the rows of one XCD are ky = xcd (mod 8)
+// (workgroups go to the 8 XCDs round-robin): each XCD's L2 then holds one
+// eighth of the plane-coupling table, re-read by every right-hand side.
+template <typename R, int P, bool INV>
+__device__ __forceinline__ void dft_strided(const typename C2<R>::T* in, int ib, int is, typename C2<R>::T* out,
+                                            int ob, int os) {
+    using CT = typename C2<R>::T;
+    CT v[P];
+#pragma unroll
+    for (int r = 0; r < P; ++r) v[r] = in[ib + r * is];
+    dft_reg<R, P, INV>(v);
+#pragma unroll
+    for (int q = 0; q < P; ++q) out[ob + q * os] = v[q];
+}
+
+// In-place decimation-in-frequency stage: x[base + r stride] (r < P) -> DFT ->
+// output k1 multiplied by tw[j2 k1].
+template <typename R, int P, bool INV>
+__device__ __forceinline__ void dif_stage(typename C2<R>::T* x, int base, int stride,
+                                          const typename C2<R>::T* tw, int j2) {
+    using CT = typename C2<R>::T;
+    CT v[P];
+#pragma unroll
+    for (int r = 0; r < P; ++r) v[r] = x[base + r * stride];
+    dft_reg<R, P, INV>(v);
+    if (j2 > 0) {
+#pragma unroll
+        for (int k = 1; k < P; ++k) v[k] = cmul_t(v[k], tw[j2 * k]);
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) x[base + k * stride] = v[k];
+}
+
+template <typename R, int ZS, int CLS, bool RI>
+__global__ __launch_bounds__(256) void xtoeplitz_kernel(typename C2<R>::T* __restrict__ kbuf, int Zs, int ny,
+                                                        int nxp, int n, int P1, int P2,
+                                                        const typename C2<R>::T* __restrict__ twiddle,
+                                                        const typename C2<R>::T* __restrict__ T) {
+    using CT = typename C2<R>::T;
+    constexpr int AS = ztoe_as<R, ZS>();
+    extern __shared__ unsigned char s_raw[];
+    const int nl = 3 * Zs;
+    CT* const X = reinterpret_cast<CT*>(s_raw);  // [nl][n], then the forward and inverse twiddles
+    CT* const twf = X + nl * n;
+    CT* const twi = twf + n;
+    const int tid = threadIdx.x;
+    const int nky8 = (ny + 7) >> 3;
+    const int slot = blockIdx.x >> 3;
+    const int ky = ((slot % nky8) << 3) + (blockIdx.x & 7), b = slot / nky8;
+    if (ky >= ny) return;  // whole block
+    const size_t plane = (size_t)ny * nxp;
+    CT* const row0 = kbuf + (size_t)b * nl * plane + (size_t)ky * nxp;  // line li at row0 + li plane
+    for (int k = tid; k < n; k += blockDim.x) {
+        CT w = twiddle[k];
+        twf[k] = w;
+        w.y = -w.y;
+        twi[k] = w;
+    }
+    // forward, radix P1: x[j2 + P2 j1] -> X[li n + P1 j2 + k1]
+    for (int w = tid; w < nl * P2; w += blockDim.x) {
+        const int li = w / P2, j2 = w - li * P2;
+        const CT* in = row0 + li * plane;
+        switch (P1) {
+#define MPM_XS(P, C)                                                                  \
+    case P:                                                                           \
+        if constexpr (CLS >= C) dft_strided<R, P, false>(in, j2, P2, X, li * n + P1 * j2, 1); \
+        break;
+            MPM_FFT2_CASES(MPM_XS)
+#undef MPM_XS
+            default:
+                break;
+        }
+    }
+    __syncthreads();
+    // forward, radix P2 with twiddles W_n^{k1 j2}, in place: X[li n + k1 + P1 r]
+    for (int w = tid; w < nl * P1; w += blockDim.x) {
+        const int li = w / P1, k1 = w - li * P1;
+        switch (P2) {
+#define MPM_XS(P, C)                                                                                 \
+    case P:                                                                                          \
+        if constexpr (CLS >= C) fft2_s2<R, P, false>(X, li * n + k1, P1, twf, k1, X, li * n + k1, P1); \
+        break;
+            MPM_FFT2_CASES(MPM_XS)
+#undef MPM_XS
+            default:
+                break;
+        }
+    }
+    __syncthreads();
+    const int ncols = ny * nxp;
+    for (int kx = tid; kx < n; kx += blockDim.x)
+        toeplitz_column<R, ZS, AS, RI>(X + kx, n, Zs, T + (size_t)ky * nxp + kx, ncols);
+    __syncthreads();
+    // inverse, radix P1, in place: X[li n + j2 + P2 r] -> DFT -> * conj W_n^{j2 k1}
+    for (int w = tid; w < nl * P2; w += blockDim.x) {
+        const int li = w / P2, j2 = w - li * P2;
+        switch (P1) {
+#define MPM_XS(P, C)                                                              \
+    case P:                                                                       \
+        if constexpr (CLS >= C) dif_stage<R, P, true>(X, li * n + j2, P2, twi, j2); \
+        break;
+            MPM_FFT2_CASES(MPM_XS)
+#undef MPM_XS
+            default:
+                break;
+        }
+    }
+    __syncthreads();
+    // inverse, radix P2: X[li n + P2 k1 + j2] -> out[k1 + P1 k2]
+    for (int w = tid; w < nl * P1; w += blockDim.x) {
+        const int li = w / P1, k1 = w - li * P1;
+        CT* out = row0 + li * plane;
+        switch (P2) {
+#define MPM_XS(P, C)                                                                       \
+    case P:                                                                                \
+        if constexpr (CLS >= C) dft_strided<R, P, true>(X, li * n + P2 * k1, 1, out, k1, P1); \
+        break;
+            MPM_FFT2_CASES(MPM_XS)
+#undef MPM_XS
+            default:
+                break;
         }
     }
 }
@@ -2268,13 +2434,18 @@ __global__ __launch_bounds__(256) void gather_kernel(
     }
 }
 
-// Thread per (particle i, right-hand side b), b fastest: the reciprocal part
-// as the sum of the particle's region contractions (slot order), the real-space
-// part over the neighbour list with the precomputed pair coefficients
-// (ux, uy, uz, perp, para - perp, cos, sin) -- the field of p_j at i is
-//   perp p_j + (para - perp) u (u . p_j), times the Bloch phase --
-// then the self term and the optional left Jacobi scaling.  Used with the
-// matrix-core contraction when the batch is wide enough to fill the GPU.
+// kFinishLanes lanes per (particle i, right-hand side b), lanes then b fastest:
+// the reciprocal part as the sum of the particle's region contractions, the
+// real-space part over the neighbour list with the precomputed pair
+// coefficients (ux, uy, uz, perp, para - perp, cos, sin) -- the field of p_j at
+// i is perp p_j + (para - perp) u (u . p_j), times the Bloch phase -- each lane
+// taking every kFinishLanes-th slot and neighbour, then a butterfly sum over the
+// lanes (fixed order), the self term and the optional left Jacobi scaling.
+// Used with the matrix-core contraction when the batch is wide enough to fill
+// the GPU; splitting the neighbour loop over lanes keeps enough independent
+// loads in flight at a few thousand particles.
+constexpr int kFinishLanes = 4;
+
 template <typename R>
 __global__ __launch_bounds__(256) void finish_kernel(SlotTable tab, int B, int parts, int N,
                                                      const R* __restrict__ partial, const int* __restrict__ ps_start,
@@ -2283,6 +2454,7 @@ __global__ __launch_bounds__(256) void finish_kernel(SlotTable tab, int B, int p
                                                      const double* __restrict__ pairc,
                                                      const double2* __restrict__ selfc, double self_perp,
                                                      int jacobi) {
+    constexpr int NL = kFinishLanes;
     __shared__ const double2* s_x[kMaxSlots];
     __shared__ double2* s_y[kMaxSlots];
     __shared__ int s_row[kMaxSlots];
@@ -2295,13 +2467,14 @@ __global__ __launch_bounds__(256) void finish_kernel(SlotTable tab, int B, int p
         }
     __syncthreads();
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = t / B, b = t - i * B;
-    if (i >= N) return;
+    const int item = t / NL, sub = t % NL;  // NL divides the wave: the lanes of an item share it
+    const int i = item / B, b = item - i * B;
+    if (i >= N) return;  // whole items only: all lanes of an item leave together
     const double2* xb = s_x[b];
     double ar[3] = {0.0, 0.0, 0.0}, ai[3] = {0.0, 0.0, 0.0};
     if (parts & MPM_PART_RECIP) {
         const int s0 = ps_start[i], s1 = ps_start[i + 1];
-        for (int sl = s0; sl < s1; ++sl) {
+        for (int sl = s0 + sub; sl < s1; sl += NL) {
             const R* pp = partial + (size_t)ps_items[sl] * (6 * B) + 6 * b;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -2309,16 +2482,12 @@ __global__ __launch_bounds__(256) void finish_kernel(SlotTable tab, int B, int p
                 ai[c] += (double)pp[2 * c + 1];
             }
         }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            ar[c] *= h3;
-            ai[c] *= h3;
-        }
     }
+    double br[3] = {0.0, 0.0, 0.0}, bi[3] = {0.0, 0.0, 0.0};
     if (parts & MPM_PART_REAL) {
         const int k0 = nb_start[i], k1 = nb_start[i + 1];
-#pragma unroll 4
-        for (int k = k0; k < k1; ++k) {
+#pragma unroll 2
+        for (int k = k0 + sub; k < k1; k += NL) {
             const int j = nb_j[k];
             const double* pc = pairc + 8 * (size_t)k;
             const double ux = pc[0], uy = pc[1], uz = pc[2], cp = pc[3], cd = pc[4], ec = pc[5], es = pc[6];
@@ -2331,11 +2500,23 @@ __global__ __launch_bounds__(256) void finish_kernel(SlotTable tab, int B, int p
             for (int c = 0; c < 3; ++c) {
                 const double er = cp * pp[c].x + rr * u[c];
                 const double ei = cp * pp[c].y + ri * u[c];
-                ar[c] += er * ec - ei * es;
-                ai[c] += er * es + ei * ec;
+                br[c] += er * ec - ei * es;
+                bi[c] += er * es + ei * ec;
             }
         }
     }
+    // butterfly over the item's lanes: every lane ends with the same sums
+#pragma unroll
+    for (int m = 1; m < NL; m <<= 1)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            ar[c] += __shfl_xor(ar[c], m, NL);
+            ai[c] += __shfl_xor(ai[c], m, NL);
+            br[c] += __shfl_xor(br[c], m, NL);
+            bi[c] += __shfl_xor(bi[c], m, NL);
+        }
+    if (sub >= 3) return;
+    const int c = sub;  // lane c of the item writes component c
     const double2* p = xb + 3 * (size_t)i;
     double2 fs = make_double2(0.0, 0.0);
     if (parts & MPM_PART_REAL) {
@@ -2349,14 +2530,14 @@ __global__ __launch_bounds__(256) void finish_kernel(SlotTable tab, int B, int p
         const double den = sc.x * sc.x + sc.y * sc.y;
         if (den > 1e-30) inv = make_double2(sc.x / den, -sc.y / den);
     }
-    double2* y = s_y[b] + 3 * (size_t)i;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const double2 pc = p[c];
-        const double yr = ar[c] + fs.x * pc.x - fs.y * pc.y;
-        const double yi = ai[c] + fs.x * pc.y + fs.y * pc.x;
-        y[c] = make_double2(yr * inv.x - yi * inv.y, yr * inv.y + yi * inv.x);
-    }
+    const double2 pc = p[c];
+    const double fr = c == 0 ? ar[0] : (c == 1 ? ar[1] : ar[2]);
+    const double fi = c == 0 ? ai[0] : (c == 1 ? ai[1] : ai[2]);
+    const double gr = c == 0 ? br[0] : (c == 1 ? br[1] : br[2]);
+    const double gi = c == 0 ? bi[0] : (c == 1 ? bi[1] : bi[2]);
+    const double yr = fr * h3 + gr + fs.x * pc.x - fs.y * pc.y;
+    const double yi = fi * h3 + gi + fs.x * pc.y + fs.y * pc.x;
+    s_y[b][3 * (size_t)i + c] = make_double2(yr * inv.x - yi * inv.y, yr * inv.y + yi * inv.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -2768,6 +2949,7 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
             // buffer [3][Zs][ky][nxp] -> x transform in place on contiguous lines
             lf->rhs_elems = 3LL * Zs * std::max((long long)nx * nyp, (long long)ny * nxp);
             const int ay = axis(ny), ax = axis(nx);
+            lf->kt_x = ax;
             const LineGeom yrun{3LL * Zs * nx, 1, 0, nyp, 1};
             const LineGeom ytr{3LL * Zs * nx, nx, 1, (long long)ny * nxp, nxp};
             const LineGeom xrun{3LL * Zs * ny, 1, 0, nxp, 1};
@@ -2901,14 +3083,51 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
         const size_t bytes_per_rhs = grid_elems_per_rhs() * sizeof(CT);
         CT* kgrid = grid;
         if (kt) kgrid = fp32 ? reinterpret_cast<CT*>(d_kgrid32.p) : reinterpret_cast<CT*>(d_kgrid64.p);
+        // contiguous slab, transposed k-space rows, a two-stage x transform whose rows
+        // fit in LDS: x transform, plane coupling and inverse x transform in one kernel
+        const int xaxis = (SLAB && zcontig && kt && lfft && fft_backend == 0) ? lfft->kt_x : -1;
+        const size_t xlds = (size_t)(3 * Zs + 2) * S.n[0] * sizeof(CT);
+        const bool fusedx = fp32 && xaxis >= 0 && lfft->axes[xaxis]->two && xlds <= kMaxFusedLds;
         {
             ProfScope ps("fft", s, B);
             if (lfft && fft_backend == 0)
-                lfft->exec<R>(grid, kgrid, B, true, s);
+                lfft->exec<R>(grid, kgrid, B, true, s, fusedx ? xaxis : -1);
             else
                 fft->exec(grid, bytes_per_rhs, B, fp32, true, s);
         }
-        if (SLAB && zcontig) {
+        if (fusedx) {
+            ProfScope ps("zfused", s, B);
+            const CT* T = reinterpret_cast<const CT*>(d_toe32.p);
+            const LineAxis& ax = *lfft->axes[xaxis];
+            const CT* tw = reinterpret_cast<const CT*>(ax.tw32.p);
+            const bool ri = (S.n[2] & 1) != 0;
+            const dim3 gx((unsigned)(8 * ((S.n[1] + 7) / 8) * B));  // XCD-major row order
+#define MPM_XT(ZSV)                                                                                              \
+    case ZSV:                                                                                                    \
+        ps.kernel("xtoeplitz_kernel", {type_name<R>(), #ZSV, "2", tbool(ri)});                                   \
+        if (ri)                                                                                                  \
+            hipLaunchKernelGGL((xtoeplitz_kernel<R, ZSV, 2, true>), gx, dim3(256), xlds, s, kgrid, Zs, S.n[1],   \
+                               nxp, S.n[0], ax.plan.radix[0], ax.plan.radix[1], tw, T);                         \
+        else                                                                                                     \
+            hipLaunchKernelGGL((xtoeplitz_kernel<R, ZSV, 2, false>), gx, dim3(256), xlds, s, kgrid, Zs, S.n[1],  \
+                               nxp, S.n[0], ax.plan.radix[0], ax.plan.radix[1], tw, T);                         \
+        break;
+            if constexpr (std::is_same<R, float>::value) {
+                switch (zs_bucket(Zs)) {
+                    MPM_XT(4)
+                    MPM_XT(6)
+                    MPM_XT(8)
+                    MPM_XT(10)
+                    MPM_XT(12)
+                    MPM_XT(14)
+                    MPM_XT(16)
+                    default:
+                        throw Error(MPM_ERR_RUNTIME, "slab size out of range");
+                }
+            }
+#undef MPM_XT
+            MPM_HIP(hipGetLastError());
+        } else if (SLAB && zcontig) {
             ProfScope ps("zfused", s, B);
             const int ncols = kspace_cols();
             const CT* T = reinterpret_cast<const CT*>(fp32 ? static_cast<const void*>(d_toe32.p)
@@ -2988,7 +3207,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
         {
             ProfScope ps("fft", s, B);
             if (lfft && fft_backend == 0)
-                lfft->exec<R>(grid, kgrid, B, false, s);
+                lfft->exec<R>(grid, kgrid, B, false, s, fusedx ? xaxis : -1);
             else
                 fft->exec(grid, bytes_per_rhs, B, fp32, false, s);
         }
@@ -3033,7 +3252,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
         ProfScope ps("gather", s, B);
         ps.kernel("finish_kernel", {type_name<R>()});
         const double h3 = S.h[0] * S.h[1] * S.h[2];
-        const int blocks = (int)(((size_t)Ni * B + 255) / 256);
+        const int blocks = (int)(((size_t)Ni * B * kFinishLanes + 255) / 256);
         hipLaunchKernelGGL((finish_kernel<R>), dim3(blocks), dim3(256), 0, s, t, B, parts, Ni, part, d_ps_start.p,
                            d_ps_items.p, h3, d_nb_start.p, d_nb_j.p, d_pairc.p, d_selfc.p, self_perp, jacobi);
         MPM_HIP(hipGetLastError());

@@ -130,6 +130,34 @@ def test_apply_fp32_reciprocal_within_reference_tolerance(name, layout, fft):
         assert rel(got[b], ref) < 1e-4
 
 
+@pytest.mark.parametrize("name", ["hex_slab", "hex_slab_odd"])
+def test_fused_x_transform_slab_fp32_matches_fp64(name):
+    """FP32 reciprocal part (x transform, plane coupling and inverse x transform in one
+    kernel) against the FP64 one (separate passes) at FP32 rounding, with a Bloch phase."""
+    box, tol, xi, pos, grid, P = _geom(name)
+    grid = (40, 26, 20 if name == "hex_slab" else 21)  # x = 10 x 4: two register stages
+    _case(f"fused x transform {name} grid={grid}")
+    N = len(pos)
+    rng = np.random.default_rng(21)
+    p = rng.standard_normal((5, N, 3)) + 1j * rng.standard_normal((5, N, 3))
+    op = Op(box, tol, xi, grid, P)
+    op.set_positions(pos)
+    op.set_bloch(0.11, -0.07)
+    op.set_self_coef(np.ones(N))
+    C.profile_reset()
+    C.profile_enable(True)
+    got = op.apply(p, C.PREC_MIXED, C.PART_RECIP)
+    C.profile_enable(False)
+    names = C.profile_names()
+    assert any(n.startswith("xtoeplitz_kernel<float") for n in names), names
+    ref = op.apply(p, C.PREC_DOUBLE, C.PART_RECIP)
+    assert rel(got, ref) < 1e-5
+    ref_op = O.Operator(box, tol, xi, grid, P)
+    ref_op.set_positions(pos)
+    ref_op.set_bloch(0.11, -0.07)
+    assert rel(ref[0], ref_op.apply(p[0], np.ones(N), parts=2)) < 1e-11
+
+
 def test_bloch_wavevector_matches_oracle():
     box, tol, xi, pos, grid, P = _geom("hex_slab")
     _case("bloch")
