@@ -142,21 +142,81 @@ __global__ void reduce_kernel(Ptrs p, int nslots, int kk, int nblk, const double
     }
 }
 
-// w -= sum_{i<kk} c[sys][i] V_i   (a = V_0, b = w)
-__global__ __launch_bounds__(256) void multi_axpy_kernel(Ptrs p, int kk, size_t stride, size_t n,
-                                                         const double2* __restrict__ coef, int cstride) {
+// One classical Gram-Schmidt pass of the Arnoldi step, after dots_kernel:
+// c_i = sum_blk partial[slot][blk][i] (every block sums the projections in block
+// order), w -= sum_{i<kk} c_i V_i (a = V_0, b = w); block 0 of a system stores c
+// into h[sys] (pass 0) or adds it (pass 1).  With pnorm (the second pass), each
+// block also writes the sum of |w_new|^2 over its elements to
+// pnorm[slot * pstride + block], the partial norms of the next basis vector.
+__global__ __launch_bounds__(256) void cgs_axpy_kernel(Ptrs p, int kk, size_t stride, size_t n, int nblk,
+                                                       const double2* __restrict__ partial, double2* __restrict__ h,
+                                                       int hstride, int accumulate, double* __restrict__ pnorm,
+                                                       int pstride) {
+    __shared__ double2 s_c[64];
+    __shared__ double s_n[4];
     const int slot = blockIdx.y;
-    const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const double2* V = p.a[slot];
-    const double2* c = coef + (size_t)p.sys[slot] * cstride;
-    double2 acc = p.b[slot][e];
-    for (int i = 0; i < kk; ++i) {
-        const double2 t = cmul(c[i], V[(size_t)i * stride + e]);
-        acc.x -= t.x;
-        acc.y -= t.y;
+    if (threadIdx.x < kk) {
+        double sr = 0.0, si = 0.0;
+        for (int b = 0; b < nblk; ++b) {
+            const double2 v = partial[((size_t)slot * kRedBlocks + b) * 64 + threadIdx.x];
+            sr += v.x;
+            si += v.y;
+        }
+        s_c[threadIdx.x] = make_double2(sr, si);
+        if (blockIdx.x == 0) {
+            double2* o = h + (size_t)p.sys[slot] * hstride + threadIdx.x;
+            if (accumulate) {
+                o->x += sr;
+                o->y += si;
+            } else {
+                *o = make_double2(sr, si);
+            }
+        }
     }
-    p.b[slot][e] = acc;
+    __syncthreads();
+    const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    double nn = 0.0;
+    if (e < n) {
+        const double2* V = p.a[slot];
+        double2 acc = p.b[slot][e];
+        for (int i = 0; i < kk; ++i) {
+            const double2 t = cmul(s_c[i], V[(size_t)i * stride + e]);
+            acc.x -= t.x;
+            acc.y -= t.y;
+        }
+        p.b[slot][e] = acc;
+        nn = acc.x * acc.x + acc.y * acc.y;
+    }
+    if (pnorm) {
+        nn = wsum(nn);
+        if ((threadIdx.x & 63) == 0) s_n[threadIdx.x >> 6] = nn;
+        __syncthreads();
+        if (threadIdx.x == 0) pnorm[(size_t)slot * pstride + blockIdx.x] = ((s_n[0] + s_n[1]) + s_n[2]) + s_n[3];
+    }
+}
+
+// b = a / sqrt(nn) with nn = sum_blk pnorm[slot][blk] (summed by the first wave of
+// every block in a fixed order); block 0 stores nn into norm2[sys].
+__global__ __launch_bounds__(256) void scale_norm_kernel(Ptrs p, size_t n, const double* __restrict__ pnorm,
+                                                         int pstride, int nbx, double* __restrict__ norm2) {
+    __shared__ double s_nn;
+    const int slot = blockIdx.y;
+    if (threadIdx.x < 64) {
+        double t = 0.0;
+        for (int b = threadIdx.x; b < nbx; b += 64) t += pnorm[(size_t)slot * pstride + b];
+        t = wsum(t);
+        if (threadIdx.x == 0) {
+            s_nn = t;
+            if (blockIdx.x == 0) norm2[p.sys[slot]] = t;
+        }
+    }
+    __syncthreads();
+    const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const double nn = sqrt(s_nn);
+    if (e < n && nn > 0.0) {
+        const double2 v = p.a[slot][e];
+        p.b[slot][e] = make_double2(v.x / nn, v.y / nn);
+    }
 }
 
 // Left Jacobi (numerical_solver.cu:207-242): v[i][c] *= 1/self_coef[row][i]; in place on b.
@@ -414,9 +474,9 @@ inline dim3 dots_grid(size_t n, int nslots, int kk) {
 struct Work {
     size_t n = 0;
     int B = 0, m = 0;
-    DevBuf<double2> V, w, r, t, H, g, h, hlast, y, partial;
+    DevBuf<double2> V, w, r, t, H, g, h, y, partial;
     DevBuf<double2> pb, sb, vb, tb, r0b, alpha, omega, beta;
-    DevBuf<double> nrm, resid, cs;
+    DevBuf<double> nrm, resid, cs, pnorm;
     DevBuf<int> rows, kk;
     DevBuf<double2> sn;
 };
@@ -506,9 +566,10 @@ static void gmres_group(Ctx& C, const double2* b, double2* x, int B, mpm_solve_r
     W.H.reserve((size_t)B * (m + 1) * m);
     W.g.reserve((size_t)B * (m + 1));
     W.h.reserve((size_t)B * (m + 1));
-    W.hlast.reserve((size_t)B * (m + 1));
     W.y.reserve((size_t)B * (m + 1));
     W.partial.reserve((size_t)kMaxSlots * kRedBlocks * 64);
+    const int nbx = (int)((n + 255) / 256);  // blocks of the element-wise kernels
+    W.pnorm.reserve((size_t)kMaxSlots * nbx);
     W.nrm.reserve(B);
     W.resid.reserve(B);
     W.cs.reserve((size_t)B * m);
@@ -636,34 +697,24 @@ static void gmres_group(Ctx& C, const double2* b, double2* x, int B, mpm_solve_r
                     if (!inner32) mv64[q]++;
                 }
             }
-            // CGS2: h = V^H w ; w -= V h ; h2 = V^H w ; w -= V h2 ; h += h2
+            // CGS2: h = V^H w ; w -= V h ; h2 = V^H w ; w -= V h2 ; h += h2 (the
+            // projections summed inside the update; the second update also leaves
+            // the partial norms of w)
             Ptrs pw = make_ptrs(in_cycle, [&](int q) { return Vk(q, 0); }, [&](int q) { return W.w.p + q * n; });
             const int nb = nblk_for(n);
             for (int pass = 0; pass < 2; ++pass) {
-                {
-                    ProfScope ps("krylov", C.s);
-                    hipLaunchKernelGGL(dots_kernel, dots_grid(n, ns, k + 1), dim3(256), 0, C.s, pw, k + 1, n, n, chunk_for(n), 0, W.partial.p);
-                    hipLaunchKernelGGL(reduce_kernel, dim3((ns * (k + 1) + 255) / 256), dim3(256), 0, C.s, pw, ns,
-                                       k + 1, nb, W.partial.p, W.h.p, m + 1, pass, W.hlast.p);
-                    hipLaunchKernelGGL(multi_axpy_kernel, grid_for(n, ns), dim3(256), 0, C.s, pw, k + 1, n, n,
-                                       (const double2*)W.hlast.p, m + 1);
-                    MPM_HIP(hipGetLastError());
-                }
-            }
-            // ||w||, V_{k+1} = w / ||w||
-            Ptrs pn = make_ptrs(in_cycle, [&](int q) { return W.w.p + q * n; }, [&](int q) { return W.w.p + q * n; });
-            // norms() reuses W.h as reduction scratch: keep h safe by reducing into hlast
-            {
-                const int nb2 = nblk_for(n);
-                hipLaunchKernelGGL(dots_kernel, dots_grid(n, ns, 1), dim3(256), 0, C.s, pn, 1, (size_t)0, n, chunk_for(n), 2, W.partial.p);
-                hipLaunchKernelGGL(reduce_kernel, dim3((ns + 255) / 256), dim3(256), 0, C.s, pn, ns, 1, nb2,
-                                   W.partial.p, W.hlast.p, m + 1, 0, (double2*)nullptr);
-                hipLaunchKernelGGL(extract_kernel, dim3((ns + 63) / 64), dim3(64), 0, C.s, pn, ns, W.hlast.p, m + 1,
-                                   W.nrm.p);
+                ProfScope ps("krylov", C.s);
+                hipLaunchKernelGGL(dots_kernel, dots_grid(n, ns, k + 1), dim3(256), 0, C.s, pw, k + 1, n, n,
+                                   chunk_for(n), 0, W.partial.p);
+                hipLaunchKernelGGL(cgs_axpy_kernel, grid_for(n, ns), dim3(256), 0, C.s, pw, k + 1, n, n, nb,
+                                   (const double2*)W.partial.p, W.h.p, m + 1, pass,
+                                   pass == 1 ? W.pnorm.p : (double*)nullptr, nbx);
                 MPM_HIP(hipGetLastError());
             }
+            // ||w||^2 -> nrm, V_{k+1} = w / ||w||
             Ptrs pc = make_ptrs(in_cycle, [&](int q) { return W.w.p + q * n; }, [&](int q) { return Vk(q, k + 1); });
-            hipLaunchKernelGGL(scale_copy_kernel, grid_for(n, ns), dim3(256), 0, C.s, pc, n, W.nrm.p);
+            hipLaunchKernelGGL(scale_norm_kernel, grid_for(n, ns), dim3(256), 0, C.s, pc, n, (const double*)W.pnorm.p,
+                               nbx, nbx, W.nrm.p);
             hipLaunchKernelGGL(givens_kernel, dim3(ns), dim3(64), 0, C.s, pw, ns, k, m, W.H.p, W.h.p,
                                W.nrm.p, W.g.p, W.cs.p, W.sn.p, W.resid.p);
             MPM_HIP(hipGetLastError());
@@ -762,7 +813,6 @@ static void bicgstab_group(Ctx& C, const double2* b, double2* x, int B, mpm_solv
     W.r0b.reserve((size_t)B * n);
     W.r.reserve((size_t)B * n);
     W.h.reserve((size_t)B * 2);
-    W.hlast.reserve((size_t)B * 2);
     W.partial.reserve((size_t)kMaxSlots * kRedBlocks * 64);
     W.nrm.reserve(B);
     W.alpha.reserve(B);
