@@ -18,10 +18,14 @@
 //   GMRES(m)            src/cpp/numerical_solver/gmres_solver.cu:56-208
 //   BiCGSTAB            src/cpp/numerical_solver/bicgstab_solver.cu:58-192
 //   Jacobi / BLAS-1     src/cpp/numerical_solver/numerical_solver.cu:22-276
+//   polydisperse        src/cpp/electric_field/polydisperse_ewald_electric_field.cu:27-127
+//                       (unique radii), 190-350 (finite-sphere spread/contract weights),
+//                       565-640 (scalar k-space coefficient), 654-828 (contract, self,
+//                       real space), 1434-1698 (pair functions and tables), 2039-2105 (sizing)
 // The FFT is this file's own mixed-radix + Bluestein transform (the reference
 // uses cuFFT Z2Z, unnormalised, forward e^{-i}, inverse e^{+i}).
 //
-// Pinning (see tests/test_oracle_*.py): golden fixture
+// Pinning (see tests/test_oracle*.py): golden fixture
 // tests/data/ref_dipole_monodisperse.npz, numpy.fft, a textbook point-dipole
 // Ewald lattice sum (non-overlapping spheres), quadrature of the
 // overlapping-sphere field (1 <= r < 2), xi-independence of real+recip, and a
@@ -344,6 +348,149 @@ static double interp(double r, const std::vector<double>& rt, const std::vector<
 }
 
 // ---------------------------------------------------------------------------
+// Polydisperse pieces (polydisperse_ewald_electric_field.cu)
+// ---------------------------------------------------------------------------
+// Real-space pair functions of uniformly polarised spheres of radii a, b at
+// centre distance r.  Both are odd extensions in the radii:
+//   f(r; a, b) = sum_{s, t = +-1} [ A(r; s a, t b) exp(-(r + s a + t b)^2 xi^2)
+//                                  + C(r; s a, t b) erfc((r + s a + t b) xi) ] + reg(r; a, b)
+// where A and C include the factor 1/(a^3 b^3) of their own (signed) arguments,
+// which supplies the signs of the four terms.  polydisperse...cu:1434-1621.
+struct PolyCoefs {
+    double A, C;
+};
+static PolyCoefs poly_perp_terms(double r, double a, double b, double xi) {
+    const double x2 = xi * xi, x4 = x2 * x2, x5 = x4 * xi, x6 = x4 * x2;
+    const double r2 = r * r, r3 = r2 * r, r4 = r2 * r2, r5 = r4 * r, r6 = r4 * r2;
+    const double a2 = a * a, b2 = b * b, a3 = a2 * a, b3 = b2 * b, s = a + b;
+    const double q = a2 - 4.0 * a * b + b2;
+    const double c3 = 4.0 * a3 - 3.0 * a2 * b - 3.0 * a * b2 + 4.0 * b3;
+    const double ab3 = a3 * b3;
+    const double A = (4.0 * x4 * r5 - 4.0 * s * x4 * r4 + (16.0 * x2 + 8.0 * (-4.0 * a2 + a * b - 4.0 * b2) * x4) * r3 +
+                      (-12.0 * s * x2 - 8.0 * c3 * x4) * r2 +
+                      (3.0 - 12.0 * (a2 - a * b + b2) * x2 - 4.0 * s * s * q * x4) * r + 3.0 * s + 4.0 * c3 * x2 +
+                      4.0 * s * s * s * q * x4) /
+                     (1024.0 * std::pow(kPi, 1.5) * ab3 * r3 * x5);
+    const double d2 = (a2 - b2) * (a2 - b2);
+    const double C = (-8.0 * x6 * r6 + 36.0 * x4 * (-1.0 + 2.0 * (a2 + b2) * x2) * r4 + 128.0 * (a3 + b3) * x6 * r3 +
+                      (-18.0 * x2 + 72.0 * (a2 + b2) * x4 + 72.0 * d2 * x6) * r2 + 3.0 - 18.0 * (a2 + b2) * x2 -
+                      36.0 * d2 * x4 - 8.0 * s * s * s * s * q * x6) /
+                     (2048.0 * kPi * ab3 * r3 * x6);
+    return {A, C};
+}
+static PolyCoefs poly_para_terms(double r, double a, double b, double xi) {
+    const double x2 = xi * xi, x4 = x2 * x2, x5 = x4 * xi, x6 = x4 * x2;
+    const double r2 = r * r, r3 = r2 * r, r4 = r2 * r2, r5 = r4 * r, r6 = r4 * r2;
+    const double a2 = a * a, b2 = b * b, a3 = a2 * a, b3 = b2 * b, s = a + b;
+    const double q = a2 - 4.0 * a * b + b2;
+    const double c3 = 4.0 * a3 - 3.0 * a2 * b - 3.0 * a * b2 + 4.0 * b3;
+    const double e3 = a3 - 3.0 * a2 * b - 3.0 * a * b2 + b3;
+    const double ab3 = a3 * b3;
+    const double A = (8.0 * x4 * r5 - 8.0 * s * x4 * r4 + (14.0 * x2 - 4.0 * (7.0 * a2 - 4.0 * a * b + 7.0 * b2) * x4) * r3 +
+                      (-6.0 * s * x2 - 4.0 * e3 * x4) * r2 +
+                      (-3.0 + 12.0 * (a2 - a * b + b2) * x2 + 4.0 * s * s * q * x4) * r - 3.0 * s - 4.0 * c3 * x2 -
+                      4.0 * s * s * s * q * x4) /
+                     (512.0 * std::pow(kPi, 1.5) * ab3 * r3 * x5);
+    const double d2 = (a2 - b2) * (a2 - b2);
+    const double C = (-16.0 * x6 * r6 + 36.0 * x4 * (-1.0 + 2.0 * (a2 + b2) * x2) * r4 + 64.0 * (a3 + b3) * x6 * r3 - 3.0 +
+                      18.0 * (a2 + b2) * x2 + 36.0 * d2 * x4 + 8.0 * s * s * s * s * q * x6) /
+                     (1024.0 * kPi * ab3 * r3 * x6);
+    return {A, C};
+}
+
+static void poly_pair_functions(double r, double a, double b, double xi, double& fperp, double& fpara) {
+    double fp = 0.0, fa = 0.0;
+    for (int sa = -1; sa <= 1; sa += 2)
+        for (int sb = -1; sb <= 1; sb += 2) {
+            const double aa = sa * a, bb = sb * b;
+            const double u = r + aa + bb;
+            const double e = std::exp(-u * u * xi * xi), c = std::erfc(u * xi);
+            const PolyCoefs tp = poly_perp_terms(r, aa, bb, xi), ta = poly_para_terms(r, aa, bb, xi);
+            fp += tp.A * e + tp.C * c;
+            fa += ta.A * e + ta.C * c;
+        }
+    // direct (unscreened) overlap of the two spheres
+    const double r3 = r * r * r, a3 = a * a * a, b3 = b * b * b;
+    if (r < a + b && r >= std::abs(a - b)) {
+        const double t = a + b - r;
+        fp += t * t * t / (128.0 * kPi * a3 * b3 * r3) *
+              (-r3 - 3.0 * (a + b) * r * r + 3.0 * (a * a - 4.0 * a * b + b * b) * r + a3 - 3.0 * a * a * b -
+               3.0 * a * b * b + b3);
+        fa += -(std::pow(a + b, 4) * (a * a - 4.0 * a * b + b * b) - 8.0 * (a3 + b3) * r3 +
+                9.0 * (a * a + b * b) * r * r * r * r - 2.0 * r3 * r3) /
+              (64.0 * kPi * a3 * b3 * r3);
+    } else if (r < b - a) {  // sphere a inside sphere b
+        fp += (r3 - b3) / (4.0 * kPi * b3 * r3);
+        fa += (r3 + 2.0 * b3) / (4.0 * kPi * b3 * r3);
+    } else if (r < a - b) {  // sphere b inside sphere a
+        fp += (r3 - a3) / (4.0 * kPi * a3 * r3);
+        fa += (r3 + 2.0 * a3) / (4.0 * kPi * a3 * r3);
+    }
+    fperp = fp;
+    fpara = fa;
+}
+
+// r -> 0 limit of the pair functions (concentric spheres): the self term of a
+// radius pair; a = b gives the particle's real-space self term.  :1611-1620.
+static double poly_self(double a, double b, double xi) {
+    const double x2 = xi * xi, x3 = x2 * xi;
+    const double a3 = a * a * a, b3 = b * b * b;
+    return ((1.0 - 2.0 * a * a * x2 + 2.0 * a * b * x2 - 2.0 * b * b * x2) * std::exp(-(a + b) * (a + b) * x2) +
+            (-1.0 + 2.0 * a * a * x2 + 2.0 * a * b * x2 + 2.0 * b * b * x2) * std::exp(-(a - b) * (a - b) * x2)) /
+               (16.0 * std::pow(kPi, 1.5) * a3 * b3 * x3) +
+           ((a3 - b3) * std::erf((a - b) * xi) - (a3 + b3) * std::erf((a + b) * xi)) / (8.0 * kPi * a3 * b3) +
+           std::min(a3, b3) / (4.0 * kPi * a3 * b3);
+}
+
+// Finite-sphere spread/contract weight of a particle of radius a at the grid
+// offset vector gd (grid point minus particle centre): the vector C(d) gd/d.
+// :243-262 (the d < 1e-6 branch included as written there).
+static void poly_weight(const double gd[3], double a, double xi, double eta, double w[3]) {
+    const double d = std::sqrt(gd[0] * gd[0] + gd[1] * gd[1] + gd[2] * gd[2]);
+    const double x2 = xi * xi;
+    if (d >= 1e-6) {
+        const double k = 3.0 / (8.0 * std::sqrt(2.0) * std::pow(kPi, 1.5) * a * a * a * std::sqrt(eta) * xi * d * d);
+        const double tp = (eta + 4.0 * a * x2 * d) * std::exp(-2.0 * (d + a) * (d + a) * x2 / eta);
+        const double tm = (eta - 4.0 * a * x2 * d) * std::exp(-2.0 * (d - a) * (d - a) * x2 / eta);
+        const double c = k * (tp - tm) / d;
+        for (int q = 0; q < 3; ++q) w[q] = c * gd[q];
+    } else {
+        const double c = 8.0 * std::sqrt(2.0) * std::pow(xi, 5.0) * std::exp(-2.0 * a * x2 / eta) /
+                         (std::pow(kPi, 1.5) * std::pow(eta, 2.5));
+        for (int q = 0; q < 3; ++q) w[q] = c * gd[q];
+    }
+}
+
+// Polydisperse sizing: rc = max(sqrt(-ln tol)/xi, 2 a_max), the grid rule of the
+// monodisperse operator, P = ceil(2/h_mean - 2 ln tol / pi) points from
+// -floor((P-1)/2), eta = P (h_mean xi)^2 / pi.  :2039-2105.
+static int compute_sizing_poly(const double L[3], double errortol, double xi, double amax,
+                               const int* grid_override, int P_override, Sizing& S, std::string& err) {
+    S.rc = std::sqrt(-std::log(errortol)) / xi;
+    if (S.rc < 2.0 * amax) S.rc = 2.0 * amax;
+    for (int d = 0; d < 3; ++d) {
+        if (S.rc > L[d] / 2.0) {
+            err = "Real space cutoff (" + std::to_string(S.rc) + ") larger than half the box length.";
+            return -1;
+        }
+    }
+    S.kcut = 2.0 * xi * xi * S.rc;
+    for (int d = 0; d < 3; ++d) {
+        S.n[d] = (int)std::ceil(1.0 + L[d] * S.kcut / kPi);
+        if (grid_override && grid_override[d] > 0) S.n[d] = grid_override[d];
+        S.h[d] = L[d] / S.n[d];
+    }
+    const double hm = (S.h[0] + S.h[1] + S.h[2]) / 3.0;
+    S.P = (int)std::ceil(2.0 / hm - 2.0 * std::log(errortol) / kPi);
+    if (P_override > 0) S.P = P_override;
+    const double eta = S.P * std::pow(hm * xi, 2) / kPi;
+    for (int d = 0; d < 3; ++d) S.eta[d] = eta;
+    S.half = (S.P - 1) / 2;  // offsets -half .. P - 1 - half
+    S.width = S.P;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // Operator
 // ---------------------------------------------------------------------------
 struct Op {
@@ -351,6 +498,14 @@ struct Op {
     double errortol, xi;
     Sizing S;
     DipoleTables T;
+    // polydisperse: per-particle radii, unique radii (merged within 1e-5), the
+    // pair tables [entry][pair column] and the per-radius self terms
+    bool poly = false;
+    std::vector<double> radius, uniq;
+    std::vector<int> ridx;
+    std::vector<double> pr, pperp, ppara, uself;  // r samples, tables [entry][col], self per unique radius
+    int ncol = 0;
+    std::vector<double> pw;  // [i][o][3] spread/contract weight vectors
     double kbx = 0, kby = 0;  // Bloch wavevector (x, y)
     // geometry
     int N = 0;
@@ -384,6 +539,11 @@ static void compute_scale(Op& op) {
                      Gz = fz * 2.0 * kPi / op.L[2];
         const double kx = Gx - op.kbx, ky = Gy - op.kby, kz = Gz;
         const double k2 = kx * kx + ky * ky + kz * kz;
+        if (op.poly) {  // exp(-(1 - eta) k^2 / 4 xi^2) / k^2 / V (:565-640)
+            if (k2 >= 1e-12)
+                op.scale[v] = std::exp(-(1.0 - S.eta[0]) * k2 / (4.0 * xi * xi)) / k2 / (double)op.V;
+            continue;
+        }
         const double k = std::sqrt(k2);
         if (k < 1e-12) continue;
         op.khat[3 * v + 0] = kx / k;
@@ -393,6 +553,56 @@ static void compute_scale(Op& op) {
         const double t = std::sin(k) / k - std::cos(k);
         op.scale[v] = 9.0 * t * t * std::exp(-etak2 / (4.0 * xi * xi)) / (k2 * k2) / (double)op.V;
     }
+}
+
+// Pair column of two radius classes (upper triangle, row-major) and the table
+// lookup with the monodisperse interpolation rule.  :56-68, 168-188.
+static int poly_col(const Op& op, int u, int v) {
+    if (u > v) std::swap(u, v);
+    const int U = (int)op.uniq.size();
+    return u * U - u * (u - 1) / 2 + (v - u);
+}
+static double poly_interp(const Op& op, double r, int u, int v, const std::vector<double>& tab) {
+    const int col = poly_col(op, u, v);
+    const size_t ts = op.pr.size();
+    auto y = [&](size_t e) { return tab[e * op.ncol + col]; };
+    if (r <= 0.0) return y(0);
+    if (r >= op.pr[ts - 1]) return y(ts - 1);
+    if (r < 1.0) return y(0) * (1.0 - r) + y(1) * r;
+    size_t idx = 1 + (size_t)((r - 1.0) / 0.001);
+    if (idx >= ts - 1) return y(ts - 1);
+    const double t = (r - op.pr[idx]) / (op.pr[idx + 1] - op.pr[idx]);
+    return y(idx) * (1.0 - t) + y(idx + 1) * t;
+}
+
+// Tables of every radius pair: entry 0 = the pair's self term, then samples at
+// r = 1 + 0.001 k up to the cutoff.  :1629-1698.
+static void build_poly_tables(Op& op) {
+    const double rcv = op.S.rc < 1.001 ? 1.001 : op.S.rc;
+    const size_t steps = (size_t)std::ceil((rcv - 1.0) / 0.001);
+    const int U = (int)op.uniq.size();
+    op.ncol = U * (U + 1) / 2;
+    op.pr.assign(steps + 1, 0.0);
+    for (size_t k = 0; k < steps; ++k) op.pr[k + 1] = 1.0 + k * 0.001;
+    op.pperp.assign((steps + 1) * op.ncol, 0.0);
+    op.ppara.assign((steps + 1) * op.ncol, 0.0);
+    op.uself.assign(U, 0.0);
+    for (int u = 0; u < U; ++u)
+        for (int v = u; v < U; ++v) {
+            const int col = poly_col(op, u, v);
+            const double a = op.uniq[u], b = op.uniq[v];
+            const double s0 = poly_self(a, b, op.xi);
+            op.pperp[col] = s0;
+            op.ppara[col] = s0;
+            if (u == v) op.uself[u] = s0;
+#pragma omp parallel for schedule(static)
+            for (long k = 0; k < (long)steps; ++k) {
+                double fp, fa;
+                poly_pair_functions(op.pr[k + 1], a, b, op.xi, fp, fa);
+                op.pperp[(k + 1) * op.ncol + col] = fp;
+                op.ppara[(k + 1) * op.ncol + col] = fa;
+            }
+        }
 }
 
 // neighbor_list.cu: all j != i with min-image |r|^2 <= rc^2 (cell list when
@@ -468,7 +678,11 @@ static void build_neighbors(Op& op) {
             double ry = minimg(op.y[i] - op.y[j], L[1]);
             double rz = minimg(op.z[i] - op.z[j], L[2]);
             double d = std::sqrt(rx * rx + ry * ry + rz * rz);
-            if (d < op.S.rc) {
+            if (d < op.S.rc && op.poly) {
+                const double de = d < 1e-6 ? 1e-6 : d;
+                op.nb_perp[s + k] = poly_interp(op, de, op.ridx[i], op.ridx[j], op.pperp);
+                op.nb_para[s + k] = poly_interp(op, de, op.ridx[i], op.ridx[j], op.ppara);
+            } else if (d < op.S.rc) {
                 double de = d < 1.0 ? 1.0 : d;
                 op.nb_perp[s + k] = interp(de, op.T.r, op.T.perp);
                 op.nb_para[s + k] = interp(de, op.T.r, op.T.para);
@@ -477,8 +691,46 @@ static void build_neighbors(Op& op) {
     }
 }
 
+// Polydisperse stencils: P^3 points per particle (x-major offsets from
+// -floor((P-1)/2)), wrapped voxel index and weight vector.  :190-350.
+static void build_stencils_poly(Op& op) {
+    const Sizing& S = op.S;
+    const int P = S.P, o0 = -S.half;
+    const long SS = (long)P * P * P;
+    op.sp_idx.assign((size_t)op.N * SS, 0);
+    op.pw.assign((size_t)op.N * SS * 3, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < op.N; ++i) {
+        const double p[3] = {op.x[i], op.y[i], op.z[i]};
+        int g[3];
+        double dd[3];
+        for (int d = 0; d < 3; ++d) {
+            g[d] = (int)std::round(p[d] / S.h[d]);
+            dd[d] = g[d] * S.h[d] - p[d];
+        }
+        long o = 0;
+        for (int ox = o0; ox < o0 + P; ++ox)
+            for (int oy = o0; oy < o0 + P; ++oy)
+                for (int oz = o0; oz < o0 + P; ++oz, ++o) {
+                    const int off[3] = {ox, oy, oz};
+                    int w[3];
+                    double gd[3];
+                    for (int d = 0; d < 3; ++d) {
+                        w[d] = ((g[d] + off[d]) % S.n[d] + S.n[d]) % S.n[d];
+                        gd[d] = dd[d] + off[d] * S.h[d];
+                    }
+                    op.sp_idx[(size_t)i * SS + o] = (w[0] * S.n[1] + w[1]) * S.n[2] + w[2];
+                    poly_weight(gd, op.radius[i], op.xi, S.eta[0], &op.pw[((size_t)i * SS + o) * 3]);
+                }
+    }
+}
+
 // monodisperse_ewald_electric_field.cu:687-816 (weights C*exp(-2 xi^2 sum gd^2/eta)).
 static void build_stencils(Op& op) {
+    if (op.poly) {
+        build_stencils_poly(op);
+        return;
+    }
     const Sizing& S = op.S;
     const int W = S.width, half = S.half;
     const long SS = (long)W * W * W;
@@ -519,7 +771,32 @@ static void build_stencils(Op& op) {
 static void apply(const Op& op, const cplx* self_coef, const cplx* p, cplx* E, int parts) {
     const int N = op.N;
     for (long q = 0; q < 3L * N; ++q) E[q] = 0.0;
-    if (parts & 2) {
+    if ((parts & 2) && op.poly) {
+        // scalar grid S = sum C.p, scaled in k-space, E_i = h^3 sum C_i S (:444-694)
+        const Sizing& S = op.S;
+        const size_t V = op.V;
+        const long SS = (long)S.P * S.P * S.P;
+        std::vector<cplx> grid(V, cplx(0, 0));
+        for (int i = 0; i < N; ++i)
+            for (long o = 0; o < SS; ++o) {
+                const double* w = &op.pw[((size_t)i * SS + o) * 3];
+                grid[op.sp_idx[(size_t)i * SS + o]] += w[0] * p[3 * i] + w[1] * p[3 * i + 1] + w[2] * p[3 * i + 2];
+            }
+        fft::fft3d(grid.data(), S.n[0], S.n[1], S.n[2], -1);
+        for (size_t v = 0; v < V; ++v) grid[v] *= op.scale[v];
+        fft::fft3d(grid.data(), S.n[0], S.n[1], S.n[2], +1);
+        const double h3 = S.h[0] * S.h[1] * S.h[2];
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < N; ++i) {
+            cplx acc[3] = {0.0, 0.0, 0.0};
+            for (long o = 0; o < SS; ++o) {
+                const double* w = &op.pw[((size_t)i * SS + o) * 3];
+                const cplx g = grid[op.sp_idx[(size_t)i * SS + o]];
+                for (int c = 0; c < 3; ++c) acc[c] += g * w[c];
+            }
+            for (int c = 0; c < 3; ++c) E[3 * i + c] += h3 * acc[c];
+        }
+    } else if (parts & 2) {
         const Sizing& S = op.S;
         const size_t V = op.V;
         const long SS = (long)S.width * S.width * S.width;
@@ -556,7 +833,7 @@ static void apply(const Op& op, const cplx* self_coef, const cplx* p, cplx* E, i
     }
     if (parts & 1) {
         for (int i = 0; i < N; ++i) {
-            const cplx f = self_coef[i] + op.T.self_perp;
+            const cplx f = self_coef[i] + (op.poly ? op.uself[op.ridx[i]] : op.T.self_perp);
             for (int c = 0; c < 3; ++c) E[3 * i + c] += f * p[3 * i + c];
         }
         // source-centric scatter, as the reference kernel: E_j += T(r_ij) p_i
@@ -568,7 +845,7 @@ static void apply(const Op& op, const cplx* self_coef, const cplx* p, cplx* E, i
                 for (int d = 0; d < 3; ++d) r[d] -= op.L[d] * std::round(r[d] / op.L[d]);
                 const double d = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
                 if (!(d < op.S.rc)) continue;
-                const double de = d < 1.0 ? 1.0 : d;
+                const double de = op.poly ? (d < 1e-6 ? 1e-6 : d) : (d < 1.0 ? 1.0 : d);
                 const double u[3] = {r[0] / de, r[1] / de, r[2] / de};
                 const cplx rp = p[3 * i] * u[0] + p[3 * i + 1] * u[1] + p[3 * i + 2] * u[2];
                 cplx e[3];
@@ -840,6 +1117,48 @@ void* orc_op_create(const double* L, double errortol, double xi, const int* grid
     return op;
 }
 
+// Polydisperse operator: radii[N] (nondimensional), merged within 1e-5 into
+// sorted unique radii (:40-83).  Positions must then come with the same N.
+void* orc_poly_create(const double* L, double errortol, double xi, const double* radii, int N,
+                      const int* grid_override, int P_override, char* errbuf, int errlen) {
+    auto op = new Op();
+    for (int d = 0; d < 3; ++d) op->L[d] = L[d];
+    op->errortol = errortol;
+    op->xi = xi;
+    op->poly = true;
+    op->radius.assign(radii, radii + N);
+    for (double r : op->radius) {
+        bool seen = false;
+        for (double u : op->uniq) seen = seen || std::abs(u - r) < 1e-5;
+        if (!seen) op->uniq.push_back(r);
+    }
+    std::sort(op->uniq.begin(), op->uniq.end());
+    op->ridx.assign(N, 0);
+    for (int i = 0; i < N; ++i)
+        for (size_t u = 0; u < op->uniq.size(); ++u)
+            if (std::abs(op->uniq[u] - op->radius[i]) < 1e-5) {
+                op->ridx[i] = (int)u;
+                break;
+            }
+    std::string err;
+    if (N < 1 || compute_sizing_poly(L, errortol, xi, op->uniq.back(), grid_override, P_override, op->S, err)) {
+        if (errbuf && errlen > 0) std::snprintf(errbuf, errlen, "%s", N < 1 ? "no radii" : err.c_str());
+        delete op;
+        return nullptr;
+    }
+    build_poly_tables(*op);
+    compute_scale(*op);
+    return op;
+}
+
+void orc_poly_pair_functions(const double* r, int n, double a, double b, double xi, double* perp, double* para) {
+    for (int i = 0; i < n; ++i) poly_pair_functions(r[i], a, b, xi, perp[i], para[i]);
+}
+
+double orc_poly_self(double a, double b, double xi) { return poly_self(a, b, xi); }
+
+void orc_poly_weight(const double* gd, double a, double xi, double eta, double* w) { poly_weight(gd, a, xi, eta, w); }
+
 void orc_op_destroy(void* h) { delete static_cast<Op*>(h); }
 
 void orc_op_set_bloch(void* h, double kx, double ky) {
@@ -851,6 +1170,7 @@ void orc_op_set_bloch(void* h, double kx, double ky) {
 
 int orc_op_set_positions(void* h, const double* xyz, int N) {
     Op& op = *static_cast<Op*>(h);
+    if (op.poly && N != (int)op.radius.size()) return -1;
     op.N = N;
     op.x.resize(N);
     op.y.resize(N);
@@ -876,7 +1196,7 @@ void orc_op_info(void* h, int* n, double* h_out, double* eta, int* P, int* width
     *P = op.S.P;
     *width = op.S.width;
     *rc = op.S.rc;
-    *self_perp = op.T.self_perp;
+    *self_perp = op.poly ? op.uself[0] : op.T.self_perp;
     *num_pairs = op.N ? op.nb_start[op.N] : 0;
 }
 

@@ -56,6 +56,14 @@ def lib():
         L.orc_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double,
                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_poly_create.restype = ctypes.c_void_p
+        L.orc_poly_create.argtypes = [_dp, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int, _ip,
+                                      ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        L.orc_poly_pair_functions.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_double, _dp, _dp]
+        L.orc_poly_self.restype = ctypes.c_double
+        L.orc_poly_self.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.orc_poly_weight.argtypes = [_dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
         _lib = L
     return _lib
 
@@ -112,6 +120,27 @@ def pair_functions(r, xi):
     pa = np.zeros_like(r)
     lib().orc_pair_functions(_d(r), r.size, float(xi), _d(pe), _d(pa))
     return pe, pa
+
+
+def poly_pair_functions(r, a, b, xi):
+    """Polydisperse real-space pair functions (perp, para) of radii a, b at distances r."""
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    pe = np.zeros_like(r)
+    pa = np.zeros_like(r)
+    lib().orc_poly_pair_functions(_d(r), r.size, float(a), float(b), float(xi), _d(pe), _d(pa))
+    return pe, pa
+
+
+def poly_self(a, b, xi):
+    return lib().orc_poly_self(float(a), float(b), float(xi))
+
+
+def poly_weight(gd, a, xi, eta):
+    """Finite-sphere spread weight vector of radius a at grid offset gd (3,)."""
+    gd = np.ascontiguousarray(gd, dtype=np.float64)
+    w = np.zeros(3)
+    lib().orc_poly_weight(_d(gd), float(a), float(xi), float(eta), _d(w))
+    return w
 
 
 class Operator:
@@ -295,3 +324,25 @@ class DipoleSolver:
     def get_dipoles(self, physical=True):
         f = self.eps_m * self.length_scale ** 3 if physical else 1.0
         return np.squeeze(np.stack(self.dips) * f)
+
+
+class PolyOperator(Operator):
+    """E = A p for the polydisperse Ewald operator (per-particle radii, nondimensional)."""
+
+    def __init__(self, box, errortol, xi, radii, grid=None, P=None):
+        L = np.asarray(box, dtype=np.float64)
+        g = np.zeros(3, np.int32) if grid is None else np.asarray(grid, np.int32)
+        r = np.ascontiguousarray(radii, dtype=np.float64)
+        err = ctypes.create_string_buffer(256)
+        self._h = lib().orc_poly_create(_d(L), float(errortol), float(xi), _d(r), int(r.size),
+                                        g.ctypes.data_as(_ip), int(P or 0), err, 256)
+        if not self._h:
+            raise RuntimeError(err.value.decode())
+        self.N = 0
+        self.radii = r
+
+    def set_positions(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        if pos.shape[0] != self.radii.size:
+            raise ValueError("one radius per particle")
+        return super().set_positions(pos)

@@ -96,3 +96,31 @@ def overlap_correction(r):
     perp = quad(integrand, lo, 1.0, args=("perp",), epsabs=1e-14, epsrel=1e-13, limit=200)[0] / Vs
     para = quad(integrand, lo, 1.0, args=("para",), epsabs=1e-14, epsrel=1e-13, limit=200)[0] / Vs
     return perp, para
+
+
+def overlap_correction_radii(r, a, b):
+    """(E_avg - E_dip) on a sphere of radius b from a uniformly polarised sphere of
+    radius a (unit dipole), centre distance r < a + b: the interior field -p/(4 pi a^3)
+    minus the point-dipole field, integrated over the part of sphere a inside sphere b
+    (shells of radius rho about a's centre, caps u >= u0(rho)), divided by b's volume.
+
+    Returns (perp, para) as ``overlap_correction``."""
+    Vb = 4.0 * np.pi * b ** 3 / 3.0
+
+    def integrand(rho, which):
+        u0 = (rho * rho + r * r - b * b) / (2.0 * rho * r)
+        u0 = max(-1.0, min(1.0, u0))
+        du = 1.0 - u0
+        u2 = (1.0 - u0 ** 3) / 3.0
+        uniform = -du / (4 * np.pi * a ** 3)
+        ang = (3.0 * u2 - du) if which == "para" else (1.5 * (du - u2) - du)
+        dip = -ang / (4 * np.pi * rho ** 3)
+        return 2 * np.pi * rho * rho * (uniform + dip)
+
+    lo, hi = max(r - b, 0.0), min(a, r + b)
+    pts = [p for p in (b - r,) if lo < p < hi]
+    perp = quad(integrand, lo, hi, args=("perp",), points=pts or None, epsabs=1e-14, epsrel=1e-13,
+                limit=400)[0] / Vb
+    para = quad(integrand, lo, hi, args=("para",), points=pts or None, epsabs=1e-14, epsrel=1e-13,
+                limit=400)[0] / Vb
+    return perp, para
