@@ -360,6 +360,16 @@ mpm_status mpm_op_create(const mpm_op_desc* desc, mpm_op** out) {
     });
 }
 
+mpm_status mpm_op_create_poly(const mpm_op_desc* desc, const double* radii, int64_t n_radii, mpm_op** out) {
+    return guarded([&] {
+        MPM_REQUIRE(desc && radii && out, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        MPM_REQUIRE(n_radii > 0, MPM_ERR_INVALID_ARGUMENT, "no radii");
+        auto h = std::make_unique<mpm_op>();
+        h->impl = std::make_unique<mpm::EwaldOp>(*desc, radii, n_radii);
+        *out = h.release();
+    });
+}
+
 void mpm_op_destroy(mpm_op* op) { delete op; }
 
 mpm_status mpm_op_get_info(const mpm_op* op, mpm_op_info* info) {

@@ -129,6 +129,127 @@ double dipole_self_term(double xi) {
            std::erfc(2.0 * xi) / (4.0 * kPi);
 }
 
+// ---- polydisperse (polydisperse_ewald_electric_field.cu) ----
+
+// rc = max(sqrt(-ln tol)/xi, 2 a_max); the grid rule of the monodisperse operator;
+// P = ceil(2/h_mean - 2 ln tol/pi) points from -floor((P-1)/2); eta = P (h_mean xi)^2/pi
+// (one value for every axis).  :2039-2105.
+static void compute_sizing_poly(const double L[3], double errortol, double xi, double amax,
+                                const int grid_override[3], int P_override, Sizing& S) {
+    compute_sizing(L, errortol, xi, grid_override, 0, S);  // validation and the grid rule at rc >= 2
+    if (S.rc < 2.0 * amax) {
+        S.rc = 2.0 * amax;
+        for (int d = 0; d < 3; ++d)
+            MPM_REQUIRE(!(S.rc > L[d] / 2.0), MPM_ERR_RUNTIME,
+                        "Real space cutoff (" + std::to_string(S.rc) + ") larger than half the box length.");
+        S.kcut = 2.0 * xi * xi * S.rc;
+        for (int d = 0; d < 3; ++d) {
+            S.n[d] = (int)std::ceil(1.0 + L[d] * S.kcut / kPi);
+            if (grid_override && grid_override[d] > 0) S.n[d] = grid_override[d];
+            MPM_REQUIRE(S.n[d] >= 1 && S.n[d] <= 4096, MPM_ERR_INVALID_ARGUMENT, "grid size out of range");
+            S.h[d] = L[d] / S.n[d];
+        }
+    }
+    const double hm = (S.h[0] + S.h[1] + S.h[2]) / 3.0;
+    S.P = (int)std::ceil(2.0 / hm - 2.0 * std::log(errortol) / kPi);
+    if (P_override > 0) S.P = P_override;
+    MPM_REQUIRE(S.P >= 1 && S.P <= 40, MPM_ERR_INVALID_ARGUMENT, "support out of range");
+    const double eta = S.P * std::pow(hm * xi, 2) / kPi;
+    for (int d = 0; d < 3; ++d) S.eta[d] = eta;
+    S.half = (S.P - 1) / 2;
+    S.width = S.P;
+}
+
+// Real-space pair functions of uniformly polarised spheres of radii a, b at
+// centre distance r: odd extensions in the radii,
+//   f = sum_{s,t = +-1} [A(r; s a, t b) exp(-(r + s a + t b)^2 xi^2) + C(r; s a, t b) erfc((r + s a + t b) xi)]
+//       + the direct overlap term,
+// A and C carrying the factor 1/(a^3 b^3) of their own signed arguments.  :1434-1621.
+struct PolyTerm {
+    double Ap, Cp, Aa, Ca;  // perpendicular and parallel coefficients
+};
+static PolyTerm poly_terms(double r, double a, double b, double xi) {
+    const double x2 = xi * xi, x4 = x2 * x2, x5 = x4 * xi, x6 = x4 * x2;
+    const double r2 = r * r, r3 = r2 * r, r4 = r2 * r2, r5 = r4 * r, r6 = r4 * r2;
+    const double a2 = a * a, b2 = b * b, a3 = a2 * a, b3 = b2 * b, s = a + b, s3 = s * s * s;
+    const double q = a2 - 4.0 * a * b + b2, d2 = (a2 - b2) * (a2 - b2);
+    const double c3 = 4.0 * a3 - 3.0 * a2 * b - 3.0 * a * b2 + 4.0 * b3;
+    const double e3 = a3 - 3.0 * a2 * b - 3.0 * a * b2 + b3;
+    const double m2 = a2 - a * b + b2;
+    const double de = std::pow(kPi, 1.5) * a3 * b3 * r3 * x5, dc = kPi * a3 * b3 * r3 * x6;
+    PolyTerm t;
+    t.Ap = (4.0 * x4 * r5 - 4.0 * s * x4 * r4 + (16.0 * x2 + 8.0 * (-4.0 * a2 + a * b - 4.0 * b2) * x4) * r3 +
+            (-12.0 * s * x2 - 8.0 * c3 * x4) * r2 + (3.0 - 12.0 * m2 * x2 - 4.0 * s * s * q * x4) * r + 3.0 * s +
+            4.0 * c3 * x2 + 4.0 * s3 * q * x4) / (1024.0 * de);
+    t.Cp = (-8.0 * x6 * r6 + 36.0 * x4 * (-1.0 + 2.0 * (a2 + b2) * x2) * r4 + 128.0 * (a3 + b3) * x6 * r3 +
+            (-18.0 * x2 + 72.0 * (a2 + b2) * x4 + 72.0 * d2 * x6) * r2 + 3.0 - 18.0 * (a2 + b2) * x2 - 36.0 * d2 * x4 -
+            8.0 * s3 * s * q * x6) / (2048.0 * dc);
+    t.Aa = (8.0 * x4 * r5 - 8.0 * s * x4 * r4 + (14.0 * x2 - 4.0 * (7.0 * a2 - 4.0 * a * b + 7.0 * b2) * x4) * r3 +
+            (-6.0 * s * x2 - 4.0 * e3 * x4) * r2 + (-3.0 + 12.0 * m2 * x2 + 4.0 * s * s * q * x4) * r - 3.0 * s -
+            4.0 * c3 * x2 - 4.0 * s3 * q * x4) / (512.0 * de);
+    t.Ca = (-16.0 * x6 * r6 + 36.0 * x4 * (-1.0 + 2.0 * (a2 + b2) * x2) * r4 + 64.0 * (a3 + b3) * x6 * r3 - 3.0 +
+            18.0 * (a2 + b2) * x2 + 36.0 * d2 * x4 + 8.0 * s3 * s * q * x6) / (1024.0 * dc);
+    return t;
+}
+
+static void poly_pair_functions(double r, double a, double b, double xi, double& fperp, double& fpara) {
+    double fp = 0.0, fa = 0.0;
+    for (int sa = -1; sa <= 1; sa += 2)
+        for (int sb = -1; sb <= 1; sb += 2) {
+            const double u = r + sa * a + sb * b;
+            const double e = std::exp(-u * u * xi * xi), c = std::erfc(u * xi);
+            const PolyTerm t = poly_terms(r, sa * a, sb * b, xi);
+            fp += t.Ap * e + t.Cp * c;
+            fa += t.Aa * e + t.Ca * c;
+        }
+    const double r3 = r * r * r, a3 = a * a * a, b3 = b * b * b;
+    if (r < a + b && r >= std::abs(a - b)) {  // lens-shaped overlap
+        const double w = a + b - r;
+        fp += w * w * w / (128.0 * kPi * a3 * b3 * r3) *
+              (-r3 - 3.0 * (a + b) * r * r + 3.0 * (a * a - 4.0 * a * b + b * b) * r + a3 - 3.0 * a * a * b -
+               3.0 * a * b * b + b3);
+        fa += -(std::pow(a + b, 4) * (a * a - 4.0 * a * b + b * b) - 8.0 * (a3 + b3) * r3 +
+                9.0 * (a * a + b * b) * r3 * r - 2.0 * r3 * r3) / (64.0 * kPi * a3 * b3 * r3);
+    } else if (r < b - a) {  // sphere a inside sphere b
+        fp += (r3 - b3) / (4.0 * kPi * b3 * r3);
+        fa += (r3 + 2.0 * b3) / (4.0 * kPi * b3 * r3);
+    } else if (r < a - b) {  // sphere b inside sphere a
+        fp += (r3 - a3) / (4.0 * kPi * a3 * r3);
+        fa += (r3 + 2.0 * a3) / (4.0 * kPi * a3 * r3);
+    }
+    fperp = fp;
+    fpara = fa;
+}
+
+// r -> 0 limit (concentric spheres): a radius pair's table entry 0, and for a = b
+// a particle's real-space self term.  :1611-1620.
+static double poly_self_term(double a, double b, double xi) {
+    const double x2 = xi * xi, x3 = x2 * xi, a3 = a * a * a, b3 = b * b * b;
+    return ((1.0 - 2.0 * a * a * x2 + 2.0 * a * b * x2 - 2.0 * b * b * x2) * std::exp(-(a + b) * (a + b) * x2) +
+            (-1.0 + 2.0 * a * a * x2 + 2.0 * a * b * x2 + 2.0 * b * b * x2) * std::exp(-(a - b) * (a - b) * x2)) /
+               (16.0 * std::pow(kPi, 1.5) * a3 * b3 * x3) +
+           ((a3 - b3) * std::erf((a - b) * xi) - (a3 + b3) * std::erf((a + b) * xi)) / (8.0 * kPi * a3 * b3) +
+           std::min(a3, b3) / (4.0 * kPi * a3 * b3);
+}
+
+// Finite-sphere spread/contract weight vector of a particle of radius a at grid
+// offset gd (grid point minus centre): C(|gd|) gd / |gd|, with the reference's
+// separate form below |gd| = 1e-6.  :243-262.
+static void poly_weight(const double gd[3], double a, double xi, double eta, double w[3]) {
+    const double d = std::sqrt(gd[0] * gd[0] + gd[1] * gd[1] + gd[2] * gd[2]);
+    const double x2 = xi * xi;
+    double c;
+    if (d >= 1e-6) {
+        const double k = 3.0 / (8.0 * std::sqrt(2.0) * std::pow(kPi, 1.5) * a * a * a * std::sqrt(eta) * xi * d * d);
+        c = k * ((eta + 4.0 * a * x2 * d) * std::exp(-2.0 * (d + a) * (d + a) * x2 / eta) -
+                 (eta - 4.0 * a * x2 * d) * std::exp(-2.0 * (d - a) * (d - a) * x2 / eta)) / d;
+    } else {
+        c = 8.0 * std::sqrt(2.0) * std::pow(xi, 5.0) * std::exp(-2.0 * a * x2 / eta) /
+            (std::pow(kPi, 1.5) * std::pow(eta, 2.5));
+    }
+    for (int q = 0; q < 3; ++q) w[q] = c * gd[q];
+}
+
 // The reference samples the pair functions on r = 1 + 0.001 j (j < 9000) and
 // interpolates linearly, clamping at r >= 9.999 (:92-97, :668-685).  The pair
 // values are therefore computed here the same way, from the same samples.
@@ -2541,6 +2662,198 @@ __global__ __launch_bounds__(256) void finish_kernel(SlotTable tab, int B, int p
 }
 
 // ---------------------------------------------------------------------------
+// Polydisperse operator kernels (full grid [B][nx][ny][nz], one complex per voxel)
+// ---------------------------------------------------------------------------
+
+// Scalar k-space coefficient exp(-(1 - eta) k^2 / 4 xi^2) / k^2 / V at k = G - k_bloch,
+// zero at k = 0 (polydisperse...cu:565-640).
+__global__ void poly_kspace_kernel(double* __restrict__ k64, float* __restrict__ k32, KParams q, double eta) {
+    const size_t V = (size_t)q.nx * q.ny * q.nz;
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < V; v += (size_t)gridDim.x * blockDim.x) {
+        const int ix = (int)(v / ((size_t)q.ny * q.nz)), iy = (int)((v / q.nz) % q.ny), iz = (int)(v % q.nz);
+        const double fx = ix <= (q.nx - 1) / 2 ? ix : ix - q.nx;
+        const double fy = iy <= (q.ny - 1) / 2 ? iy : iy - q.ny;
+        const double fz = iz <= (q.nz - 1) / 2 ? iz : iz - q.nz;
+        const double kx = fx * 2.0 * kPi / q.Lx - q.kbx, ky = fy * 2.0 * kPi / q.Ly - q.kby, kz = fz * 2.0 * kPi / q.Lz;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        const double sc = k2 < 1e-12 ? 0.0 : exp(-(1.0 - eta) * k2 / (4.0 * q.xi * q.xi)) / k2 * q.invV;
+        k64[v] = sc;
+        k32[v] = (float)sc;
+    }
+}
+
+// Owner-computes spread of the scalar grid S = sum_i C_i . p_i: a thread per
+// voxel and G right-hand sides walks the voxel's (particle, stencil point)
+// entries in ascending order (no atomics, a fixed summation order).
+template <typename R, int G>
+__global__ __launch_bounds__(256) void poly_spread_kernel(SlotTable tab, int B, int V, int SS,
+                                                          const int* __restrict__ vox_start,
+                                                          const int* __restrict__ vox_items,
+                                                          const R* __restrict__ pw,
+                                                          typename C2<R>::T* __restrict__ out) {
+    using CT = typename C2<R>::T;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const int g0 = blockIdx.y * G;
+    R ar[G], ai[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j) ar[j] = ai[j] = R(0);
+    const int e0 = vox_start[v], e1 = vox_start[v + 1];
+    for (int e = e0; e < e1; ++e) {
+        const int item = vox_items[e];
+        const int pid = item / SS;
+        const R w0 = pw[3 * (size_t)item], w1 = pw[3 * (size_t)item + 1], w2 = pw[3 * (size_t)item + 2];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            if (g0 + j < B) {
+                const double2* x = tab.x[g0 + j] + 3 * (size_t)pid;
+                const double2 p0 = x[0], p1 = x[1], p2 = x[2];
+                ar[j] += w0 * (R)p0.x + w1 * (R)p1.x + w2 * (R)p2.x;
+                ai[j] += w0 * (R)p0.y + w1 * (R)p1.y + w2 * (R)p2.y;
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+        if (g0 + j < B) {
+            CT o;
+            o.x = ar[j];
+            o.y = ai[j];
+            out[(size_t)(g0 + j) * V + v] = o;
+        }
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void poly_scale_kernel(typename C2<R>::T* __restrict__ grid,
+                                                         const R* __restrict__ sc, size_t V, size_t total) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const R f = sc[e % V];
+        grid[e].x *= f;
+        grid[e].y *= f;
+    }
+}
+
+// A wave per (particle i, group of G right-hand sides): the reciprocal field
+// h^3 sum_o C_i(o) S[v(i, o)] over the particle's P^3 stencil points, the
+// real-space sum over its neighbours with the pair coefficients of the
+// monodisperse finish step, the self term (coefficient + the particle's own
+// real-space self value) and the optional left Jacobi scaling; FP64 sums, one
+// fixed-order wave reduction, one store.  polydisperse...cu:654-828.
+template <typename R, int G>
+__global__ __launch_bounds__(256) void poly_gather_kernel(SlotTable tab, int B, int parts, int N, int V, int SS,
+                                                          const typename C2<R>::T* __restrict__ grid,
+                                                          const R* __restrict__ pw, const int* __restrict__ pidx,
+                                                          double h3, const int* __restrict__ nb_start,
+                                                          const int* __restrict__ nb_j,
+                                                          const double* __restrict__ pairc,
+                                                          const double2* __restrict__ selfc,
+                                                          const double* __restrict__ selfp, int jacobi) {
+    using CT = typename C2<R>::T;
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= N) return;  // whole wave
+    const int g0 = blockIdx.y * G;
+    double acc[G][3][2];
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[j][c][0] = acc[j][c][1] = 0.0;
+    if (parts & MPM_PART_RECIP) {
+        for (int o = lane; o < SS; o += 64) {
+            const size_t item = (size_t)i * SS + o;
+            const int v = pidx[item];
+            const double w[3] = {(double)pw[3 * item], (double)pw[3 * item + 1], (double)pw[3 * item + 2]};
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                if (g0 + j < B) {
+                    const CT sv = grid[(size_t)(g0 + j) * V + v];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        acc[j][c][0] += w[c] * (double)sv.x;
+                        acc[j][c][1] += w[c] * (double)sv.y;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                acc[j][c][0] *= h3;
+                acc[j][c][1] *= h3;
+            }
+    }
+    if (parts & MPM_PART_REAL) {
+        const int k0 = nb_start[i], k1 = nb_start[i + 1];
+        for (int k = k0 + lane; k < k1; k += 64) {
+            const int jn = nb_j[k];
+            const double* pc = pairc + 8 * (size_t)k;
+            const double u[3] = {pc[0], pc[1], pc[2]};
+            const double cp = pc[3], cd = pc[4], ec = pc[5], es = pc[6];
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                if (g0 + j < B) {
+                    const double2* x = tab.x[g0 + j] + 3 * (size_t)jn;
+                    const double2 pp[3] = {x[0], x[1], x[2]};
+                    const double rr = cd * (pp[0].x * u[0] + pp[1].x * u[1] + pp[2].x * u[2]);
+                    const double ri = cd * (pp[0].y * u[0] + pp[1].y * u[1] + pp[2].y * u[2]);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const double er = cp * pp[c].x + rr * u[c];
+                        const double ei = cp * pp[c].y + ri * u[c];
+                        acc[j][c][0] += er * ec - ei * es;
+                        acc[j][c][1] += er * es + ei * ec;
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            acc[j][c][0] = wave_sum(acc[j][c][0]);
+            acc[j][c][1] = wave_sum(acc[j][c][1]);
+        }
+    if (lane >= G * 3) return;
+    const int j = lane / 3, c = lane % 3;  // lane (j, c) stores component c of right-hand side g0 + j
+    const int b = g0 + j;
+    if (b >= B) return;
+    double yr = 0.0, yi = 0.0;
+    const double2* xb = nullptr;
+    double2* yb = nullptr;
+    int row = 0;
+#pragma unroll
+    for (int jj = 0; jj < G; ++jj) {  // uniform indices into the argument, selected per lane
+        if (jj == j) {
+            xb = tab.x[g0 + jj];
+            yb = tab.y[g0 + jj];
+            row = tab.row[g0 + jj];
+        }
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+            if (jj == j && cc == c) {
+                yr = acc[jj][cc][0];
+                yi = acc[jj][cc][1];
+            }
+    }
+    const double2 pcv = xb[3 * (size_t)i + c];
+    const double2 sc = selfc[(size_t)row * N + i];
+    if (parts & MPM_PART_REAL) {
+        const double fr = sc.x + selfp[i], fi = sc.y;
+        yr += fr * pcv.x - fi * pcv.y;
+        yi += fr * pcv.y + fi * pcv.x;
+    }
+    double2 inv = make_double2(1.0, 0.0);
+    if (jacobi) {
+        const double den = sc.x * sc.x + sc.y * sc.y;
+        if (den > 1e-30) inv = make_double2(sc.x / den, -sc.y / den);
+    }
+    yb[3 * (size_t)i + c] = make_double2(yr * inv.x - yi * inv.y, yr * inv.y + yi * inv.x);
+}
+
+// ---------------------------------------------------------------------------
 // EwaldOp
 // ---------------------------------------------------------------------------
 EwaldOp::EwaldOp(const mpm_op_desc& d) {
@@ -2558,6 +2871,129 @@ EwaldOp::EwaldOp(const mpm_op_desc& d) {
     V = (size_t)S.n[0] * S.n[1] * S.n[2];
     table = std::make_unique<PairTable>(xi);
     build_kspace();
+}
+
+EwaldOp::EwaldOp(const mpm_op_desc& d, const double* radii, int64_t nr) : EwaldOp(d) {
+    MPM_REQUIRE(radii && nr > 0, MPM_ERR_INVALID_ARGUMENT, "no radii");
+    MPM_REQUIRE(d.layout == 0 || d.layout == 1, MPM_ERR_INVALID_ARGUMENT, "layout must be 0 or 1");
+    MPM_REQUIRE(d.fft_backend == 0, MPM_ERR_UNSUPPORTED,
+                "the polydisperse operator runs on the built-in FFT only (fft_backend 0)");
+    poly = true;
+    force_full = 1;
+    rad_h.assign(radii, radii + nr);
+    for (double r : rad_h) {
+        MPM_REQUIRE(std::isfinite(r) && r > 0.0, MPM_ERR_INVALID_ARGUMENT, "radii must be positive");
+        bool seen = false;
+        for (double u : uniq_h) seen = seen || std::abs(u - r) < 1e-5;
+        if (!seen) uniq_h.push_back(r);
+    }
+    std::sort(uniq_h.begin(), uniq_h.end());
+    MPM_REQUIRE(uniq_h.size() <= 64, MPM_ERR_UNSUPPORTED, "more than 64 distinct radii");
+    ridx_h.assign(nr, 0);
+    for (int64_t i = 0; i < nr; ++i)
+        for (size_t u = 0; u < uniq_h.size(); ++u)
+            if (std::abs(uniq_h[u] - rad_h[i]) < 1e-5) {
+                ridx_h[i] = (int)u;
+                break;
+            }
+    compute_sizing_poly(L, errortol, xi, uniq_h.back(), d.grid, d.support, S);
+    peta = S.eta[0];
+    poff = -S.half;
+    V = (size_t)S.n[0] * S.n[1] * S.n[2];
+    // tables of every radius pair: entry 0 the pair's self value, then r = 1 + 0.001 k up to rc
+    const double rcv = S.rc < 1.001 ? 1.001 : S.rc;
+    const size_t steps = (size_t)std::ceil((rcv - 1.0) / 0.001);
+    const int U = (int)uniq_h.size();
+    pt_ncol = U * (U + 1) / 2;
+    pt_r.assign(steps + 1, 0.0);
+    for (size_t k = 0; k < steps; ++k) pt_r[k + 1] = 1.0 + k * 0.001;
+    pt_perp.assign((steps + 1) * pt_ncol, 0.0);
+    pt_para.assign((steps + 1) * pt_ncol, 0.0);
+    uself_h.assign(U, 0.0);
+    int col = 0;
+    for (int u = 0; u < U; ++u)
+        for (int v = u; v < U; ++v, ++col) {
+            const double a = uniq_h[u], b = uniq_h[v];
+            const double s0 = poly_self_term(a, b, xi);
+            pt_perp[col] = pt_para[col] = s0;
+            if (u == v) uself_h[u] = s0;
+            for (size_t k = 0; k < steps; ++k)
+                poly_pair_functions(pt_r[k + 1], a, b, xi, pt_perp[(k + 1) * pt_ncol + col],
+                                    pt_para[(k + 1) * pt_ncol + col]);
+        }
+    self_perp = uself_h[0];
+    build_kspace();
+}
+
+// Table value of radius classes (u, v) at distance r: pair column of the upper
+// triangle (row-major), the monodisperse interpolation rule (clamped at the last
+// sample).  polydisperse...cu:56-68, 168-188.
+double EwaldOp::poly_lookup(const std::vector<double>& tab, double r, int u, int v) const {
+    if (u > v) std::swap(u, v);
+    const int U = (int)uniq_h.size();
+    const int col = u * U - u * (u - 1) / 2 + (v - u);
+    const size_t ts = pt_r.size();
+    auto y = [&](size_t e) { return tab[e * pt_ncol + col]; };
+    if (r <= 0.0) return y(0);
+    if (r >= pt_r[ts - 1]) return y(ts - 1);
+    if (r < 1.0) return y(0) * (1.0 - r) + y(1) * r;
+    const size_t idx = 1 + (size_t)((r - 1.0) / 0.001);
+    if (idx >= ts - 1) return y(ts - 1);
+    const double t = (r - pt_r[idx]) / (pt_r[idx + 1] - pt_r[idx]);
+    return y(idx) * (1.0 - t) + y(idx + 1) * t;
+}
+
+// Per-frame stencils of the polydisperse operator: the P^3 points of every
+// particle (x-major offsets from -floor((P-1)/2)), their wrapped voxels and
+// weight vectors, and the voxel -> entry lists of the owner-computes spread.
+void EwaldOp::setup_poly_stencils(const std::vector<double>& pos, int Nn) {
+    const int P = S.P;
+    const int SS = P * P * P;
+    MPM_REQUIRE((int64_t)Nn * SS < (1LL << 31), MPM_ERR_UNSUPPORTED, "too many stencil points");
+    std::vector<double> pw((size_t)Nn * SS * 3);
+    std::vector<int> pidx((size_t)Nn * SS);
+    std::vector<int> cnt(V + 1, 0);
+    for (int i = 0; i < Nn; ++i) {
+        int g[3];
+        double dd[3];
+        for (int d = 0; d < 3; ++d) {
+            const double p = pos[(size_t)d * Nn + i];
+            g[d] = (int)std::round(p / S.h[d]);
+            dd[d] = g[d] * S.h[d] - p;
+        }
+        int o = 0;
+        for (int ox = poff; ox < poff + P; ++ox)
+            for (int oy = poff; oy < poff + P; ++oy)
+                for (int oz = poff; oz < poff + P; ++oz, ++o) {
+                    const int off[3] = {ox, oy, oz};
+                    int w[3];
+                    double gd[3];
+                    for (int d = 0; d < 3; ++d) {
+                        w[d] = ((g[d] + off[d]) % S.n[d] + S.n[d]) % S.n[d];
+                        gd[d] = dd[d] + off[d] * S.h[d];
+                    }
+                    const size_t item = (size_t)i * SS + o;
+                    const int v = (w[0] * S.n[1] + w[1]) * S.n[2] + w[2];
+                    pidx[item] = v;
+                    poly_weight(gd, rad_h[i], xi, peta, &pw[3 * item]);
+                    cnt[v + 1]++;
+                }
+    }
+    for (size_t v = 0; v < V; ++v) cnt[v + 1] += cnt[v];
+    std::vector<int> items((size_t)Nn * SS);
+    {
+        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+        for (size_t item = 0; item < (size_t)Nn * SS; ++item) items[fill[pidx[item]]++] = (int)item;
+    }
+    std::vector<float> pw32(pw.begin(), pw.end());
+    std::vector<double> sp(Nn);
+    for (int i = 0; i < Nn; ++i) sp[i] = uself_h[ridx_h[i]];
+    d_pw64.upload(pw.data(), pw.size());
+    d_pw32.upload(pw32.data(), pw32.size());
+    d_pidx.upload(pidx.data(), pidx.size());
+    d_vox_start.upload(cnt.data(), cnt.size());
+    d_vox_items.upload(items.data(), items.size());
+    d_selfp.upload(sp.data(), sp.size());
 }
 
 EwaldOp::~EwaldOp() = default;
@@ -2588,10 +3024,16 @@ void EwaldOp::build_kspace() {
 }
 
 void EwaldOp::ensure_kspace_table() {
-    const int key = !slab ? 0 : (zcontig ? 2 + Zs : 1) + (kt ? 1000 : 0);
+    const int key = poly ? 5000 : (!slab ? 0 : (zcontig ? 2 + Zs : 1) + (kt ? 1000 : 0));
     if (kspace_ready && kspace_key == key) return;
     const int threads = 256;
-    if (slab && zcontig) {
+    if (poly) {
+        d_k64.resize(V);
+        d_k32.resize(V);
+        const int blocks = (int)std::min<size_t>((V + threads - 1) / threads, 65535);
+        hipLaunchKernelGGL(poly_kspace_kernel, dim3(blocks), dim3(threads), 0, nullptr, d_k64.p, d_k32.p, kparams(),
+                           peta);
+    } else if (slab && zcontig) {
         const size_t ncols = (size_t)kspace_cols();
         d_toe64.resize((size_t)Zs * 6 * ncols);
         d_toe32.resize((size_t)Zs * 6 * ncols);
@@ -2643,7 +3085,8 @@ void EwaldOp::build_pair_coefs() {
     for (size_t k = 0; k < M; ++k) {
         const double* r = &h_pair_r[3 * k];
         const double d = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-        const double de = d < 1.0 ? 1.0 : d;
+        const double dmin = poly ? 1e-6 : 1.0;  // d_eff of the unit vector
+        const double de = d < dmin ? dmin : d;
         double* o = &c[8 * k];
         o[0] = r[0] / de;
         o[1] = r[1] / de;
@@ -2673,6 +3116,8 @@ static int zs_bucket(int Zs) {
 void EwaldOp::set_positions(const double* xyz, int64_t n) {
     MPM_REQUIRE(n > 0, MPM_ERR_INVALID_ARGUMENT, "no particles");
     MPM_REQUIRE(n < (1LL << 31) / 1024, MPM_ERR_INVALID_ARGUMENT, "too many particles");
+    MPM_REQUIRE(!poly || n == (int64_t)rad_h.size(), MPM_ERR_INVALID_ARGUMENT,
+                "the polydisperse operator needs one radius per particle");
     const int Nn = (int)n;
     for (int64_t q = 0; q < 3 * n; ++q)
         MPM_REQUIRE(std::isfinite(xyz[q]), MPM_ERR_INVALID_ARGUMENT, "non-finite position");
@@ -2875,9 +3320,15 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         if (!(d2 <= rc2)) return;
         const double dist = std::sqrt(d2);
         if (!(dist < rc)) return;
-        const double de = dist < 1.0 ? 1.0 : dist;
         nj.push_back(j);
-        nf.push_back(make_double2(tab.interp(tab.perp, de), tab.interp(tab.para, de)));
+        if (poly) {  // d_eff = max(d, 1e-6) and the radius pair's table
+            const double de = dist < 1e-6 ? 1e-6 : dist;
+            nf.push_back(make_double2(poly_lookup(pt_perp, de, ridx_h[i], ridx_h[j]),
+                                      poly_lookup(pt_para, de, ridx_h[i], ridx_h[j])));
+        } else {
+            const double de = dist < 1.0 ? 1.0 : dist;
+            nf.push_back(make_double2(tab.interp(tab.perp, de), tab.interp(tab.para, de)));
+        }
         // x_j - x_i, the orientation the gather uses
         double rj[3] = {pos[j] - pos[i], pos[Nn + j] - pos[Nn + i], pos[2 * Nn + j] - pos[2 * Nn + i]};
         for (int d = 0; d < 3; ++d) rj[d] -= L[d] * std::round(rj[d] / L[d]);
@@ -2971,11 +3422,12 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
                 pass(lf->fwd, a, g, g, 0, 0);
                 pass(lf->inv, a, g, g, 0, 0);
             }
-        } else {  // [3][nx][ny][nz]: z, y, x lines in place
-            lf->rhs_elems = 3LL * nxy * nzz;
-            const LineGeom gz{3LL * nxy, 1, 0, nzz, 1};
-            const LineGeom gy{3LL * nx * nzz, nzz, 1, (long long)ny * nzz, nzz};
-            const LineGeom gx{3LL * ny * nzz, (long long)ny * nzz, 1, nxy * nzz, (long long)ny * nzz};
+        } else {  // [comps][nx][ny][nz]: z, y, x lines in place (one component for the polydisperse grid)
+            const long long nc = poly ? 1 : 3;
+            lf->rhs_elems = nc * nxy * nzz;
+            const LineGeom gz{nc * nxy, 1, 0, nzz, 1};
+            const LineGeom gy{nc * nx * nzz, nzz, 1, (long long)ny * nzz, nzz};
+            const LineGeom gx{nc * ny * nzz, (long long)ny * nzz, 1, nxy * nzz, (long long)ny * nzz};
             const int n3[3] = {nzz, ny, nx};
             const LineGeom g3[3] = {gz, gy, gx};
             for (int d = 0; d < 3; ++d) {
@@ -2994,8 +3446,13 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         if (slab)
             fft = std::make_unique<FftPlans>(std::vector<size_t>{(size_t)S.n[1], (size_t)S.n[0]}, 3 * Zs);
         else
-            fft = std::make_unique<FftPlans>(std::vector<size_t>{(size_t)S.n[2], (size_t)S.n[1], (size_t)S.n[0]}, 3);
+            fft = std::make_unique<FftPlans>(std::vector<size_t>{(size_t)S.n[2], (size_t)S.n[1], (size_t)S.n[0]},
+                                             poly ? 1 : 3);
         fft_key = key;
+    }
+    if (poly) {
+        MPM_REQUIRE(lfft != nullptr, MPM_ERR_UNSUPPORTED, "grid length not supported by the built-in FFT");
+        setup_poly_stencils(pos, Nn);
     }
     // default self coefficients: zero (reference base_electric_field.cu allocates them zeroed)
     if (self_rows == 0 || d_selfc.n != (size_t)self_rows * Nn) {
@@ -3013,6 +3470,7 @@ void EwaldOp::set_self_coef(const double* coef, int rows) {
 }
 
 size_t EwaldOp::grid_elems_per_rhs() const {
+    if (poly) return V;
     return slab ? 3 * (size_t)Zs * S.n[0] * nyp : 3 * V;
 }
 size_t EwaldOp::kgrid_elems_per_rhs() const { return 3 * (size_t)Zs * kspace_cols(); }
@@ -3274,6 +3732,53 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
     }
 }
 
+static_assert(kMaxSlots % 4 == 0, "the polydisperse kernels take right-hand sides in groups of 4");
+
+template <typename R>
+void EwaldOp::launch_poly(const SlotTable& t, int B, int parts, hipStream_t s, int jacobi) {
+    using CT = typename C2<R>::T;
+    const bool fp32 = std::is_same<R, float>::value;
+    CT* grid = fp32 ? reinterpret_cast<CT*>(d_grid32.p) : reinterpret_cast<CT*>(d_grid64.p);
+    const R* pw = fp32 ? reinterpret_cast<const R*>(d_pw32.p) : reinterpret_cast<const R*>(d_pw64.p);
+    const int SS = S.P * S.P * S.P;
+    const int Vi = (int)V;
+    const int Ni = (int)N;
+    constexpr int G = 4;
+    if (parts & MPM_PART_RECIP) {
+        {
+            ProfScope ps("spread", s, B);
+            ps.kernel("poly_spread_kernel", {type_name<R>(), "4"});
+            hipLaunchKernelGGL((poly_spread_kernel<R, G>), dim3((unsigned)((V + 255) / 256), (unsigned)((B + G - 1) / G)),
+                               dim3(256), 0, s, t, B, Vi, SS, d_vox_start.p, d_vox_items.p, pw, grid);
+            MPM_HIP(hipGetLastError());
+        }
+        {
+            ProfScope ps("fft", s, B);
+            lfft->exec<R>(grid, grid, B, true, s);
+        }
+        {
+            ProfScope ps("scale", s, B);
+            ps.kernel("poly_scale_kernel", {type_name<R>()});
+            const R* kc = fp32 ? reinterpret_cast<const R*>(d_k32.p) : reinterpret_cast<const R*>(d_k64.p);
+            const size_t total = (size_t)B * V;
+            const int blocks = (int)std::min<size_t>((total + 255) / 256, 16384);
+            hipLaunchKernelGGL((poly_scale_kernel<R>), dim3(blocks), dim3(256), 0, s, grid, kc, V, total);
+            MPM_HIP(hipGetLastError());
+        }
+        {
+            ProfScope ps("fft", s, B);
+            lfft->exec<R>(grid, grid, B, false, s);
+        }
+    }
+    ProfScope ps("gather", s, B);
+    ps.kernel("poly_gather_kernel", {type_name<R>(), "4"});
+    const double h3 = S.h[0] * S.h[1] * S.h[2];
+    hipLaunchKernelGGL((poly_gather_kernel<R, G>), dim3((unsigned)((Ni + 3) / 4), (unsigned)((B + G - 1) / G)),
+                       dim3(256), 0, s, t, B, parts, Ni, Vi, SS, grid, pw, d_pidx.p, h3, d_nb_start.p, d_nb_j.p,
+                       d_pairc.p, d_selfc.p, d_selfp.p, jacobi);
+    MPM_HIP(hipGetLastError());
+}
+
 void EwaldOp::apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t s, bool jacobi) {
     MPM_REQUIRE(N > 0, MPM_ERR_RUNTIME, "apply before set_positions");
     MPM_REQUIRE(B >= 1 && B <= kMaxSlots, MPM_ERR_INVALID_ARGUMENT, "bad slot count");
@@ -3291,6 +3796,11 @@ void EwaldOp::apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t
         if (kt) d_kgrid64.reserve(kelems);
     }
     const int jac = jacobi ? 1 : 0;
+    if (poly) {
+        if (fp32) launch_poly<float>(t, B, parts, s, jac);
+        else launch_poly<double>(t, B, parts, s, jac);
+        return;
+    }
     if (fp32) {
         if (slab) launch<float, true>(t, B, parts, s, jac);
         else launch<float, false>(t, B, parts, s, jac);
@@ -3333,6 +3843,8 @@ void EwaldOp::info(mpm_op_info& o) const {
     o.bloch[1] = kb[1];
     o.slab_planes = slab ? Zs : 0;
     o.fft_backend = (lfft && fft_backend == 0) ? 0 : 1;
+    o.polydisperse = poly ? 1 : 0;
+    o.num_unique_radii = poly ? (int)uniq_h.size() : 1;
 }
 
 }  // namespace mpm

@@ -125,6 +125,9 @@ struct LineFft;    // built-in batched line FFT
 class EwaldOp {
    public:
     EwaldOp(const mpm_op_desc& d);
+    // polydisperse operator: one radius per particle (nondimensional), merged
+    // within 1e-5 into unique radii; positions must then come with the same N
+    EwaldOp(const mpm_op_desc& d, const double* radii, int64_t n_radii);
     ~EwaldOp();
     void set_positions(const double* xyz, int64_t N);
     void set_bloch(double kx, double ky);
@@ -150,6 +153,8 @@ class EwaldOp {
     size_t grid_elems_per_rhs() const;
     template <typename R, bool SLAB>
     void launch(const SlotTable& t, int B, int parts, hipStream_t s, int jacobi);
+    template <typename R>
+    void launch_poly(const SlotTable& t, int B, int parts, hipStream_t s, int jacobi);
 
     double L[3], errortol, xi;
     Sizing S;
@@ -224,6 +229,26 @@ class EwaldOp {
     int fft_backend = 0;         // 0: built-in LDS FFT, 1: rocFFT
     std::unique_ptr<PairTable> table;
     std::vector<double> last_xyz;  // positions of the current precalcs
+
+    // ---- polydisperse operator ----
+    bool poly = false;
+    std::vector<double> rad_h;       // [N] radii
+    std::vector<double> uniq_h;      // sorted unique radii
+    std::vector<int> ridx_h;         // [N] index into uniq_h
+    std::vector<double> pt_r;        // table samples: 0, then 1 + 0.001 k
+    std::vector<double> pt_perp, pt_para;  // [entry][pair column]
+    std::vector<double> uself_h;     // real-space self term per unique radius
+    int pt_ncol = 0;
+    double peta = 0;                 // scalar spectral split of the finite-sphere kernel
+    int poff = 0;                    // first stencil offset, -floor((P - 1) / 2)
+    DevBuf<double> d_selfp;          // [N] real-space self term of each particle
+    DevBuf<double> d_pw64;           // [N][P^3][3] spread/contract weight vectors
+    DevBuf<float> d_pw32;
+    DevBuf<int> d_pidx;              // [N][P^3] wrapped voxel of each stencil point
+    DevBuf<int> d_vox_start;         // [V + 1] per voxel, its (particle, point) entries
+    DevBuf<int> d_vox_items;         //   entry = particle * P^3 + point, ascending
+    double poly_lookup(const std::vector<double>& tab, double r, int u, int v) const;
+    void setup_poly_stencils(const std::vector<double>& pos, int Nn);
 
     friend struct OpAccess;
 };

@@ -85,12 +85,22 @@ typedef struct {
     double bloch[2];
     int slab_planes;     /* occupied z-planes of the slab layout; 0 = full grid */
     int fft_backend;     /* FFT in use: 0 = built-in, 1 = rocFFT */
+    int polydisperse;    /* 1 for an operator made by mpm_op_create_poly */
+    int num_unique_radii;
 } mpm_op_info;
 
 /* Monodisperse_Ewald_Electric_Field(box_x, box_y, box_z, errortol, xi, SOLVER_AX, radius=1)
  * — src/cpp/electric_field/monodisperse_ewald_electric_field.cu:28-74, sizing :546-616,
  * tables :92-542.  Fails with MPM_ERR_RUNTIME if rc > L/2 (:553-556). */
 mpm_status mpm_op_create(const mpm_op_desc* desc, mpm_op** out);
+
+/* Polydisperse_Ewald_Electric_Field(box_x, box_y, box_z, errortol, xi, SOLVER_AX, particle_radii)
+ * — src/cpp/electric_field/polydisperse_ewald_electric_field.cu:27-127, sizing :2039-2105, pair
+ * tables :1434-1698.  radii: host [n_radii], nondimensional; radii within 1e-5 share a class.
+ * rc = max(sqrt(-ln tol)/xi, 2 a_max); P = ceil(2/h_mean - 2 ln tol/pi) unless desc->support > 0.
+ * Full-grid layout and the built-in FFT only (desc->layout and desc->fft_backend must be 0 or
+ * request them); MPM_ERR_UNSUPPORTED for more than 64 radius classes. */
+mpm_status mpm_op_create_poly(const mpm_op_desc* desc, const double* radii, int64_t n_radii, mpm_op** out);
 void mpm_op_destroy(mpm_op* op);
 mpm_status mpm_op_get_info(const mpm_op* op, mpm_op_info* info);
 

@@ -9,7 +9,7 @@ from cumpm_amd import _capi as C
 class Op:
     """Owns an mpm_op plus device buffers; every call goes through the C ABI."""
 
-    def __init__(self, box, errortol, xi, grid=None, support=0, layout=0, fft_backend=0):
+    def __init__(self, box, errortol, xi, grid=None, support=0, layout=0, fft_backend=0, radii=None):
         d = C.OpDesc()
         d.layout = int(layout)
         d.fft_backend = int(fft_backend)
@@ -20,7 +20,11 @@ class Op:
             d.grid[:] = [int(g) for g in grid]
         d.support = int(support)
         self.h = ctypes.c_void_p()
-        C.check(C.load().mpm_op_create(ctypes.byref(d), ctypes.byref(self.h)))
+        if radii is None:
+            C.check(C.load().mpm_op_create(ctypes.byref(d), ctypes.byref(self.h)))
+        else:
+            r = np.ascontiguousarray(radii, np.float64)
+            C.check(C.load().mpm_op_create_poly(ctypes.byref(d), C.dptr(r), r.size, ctypes.byref(self.h)))
         self.N = 0
 
     def __del__(self):

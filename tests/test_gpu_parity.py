@@ -381,3 +381,104 @@ def test_profile_copy_moves_exact_bytes():
     assert np.array_equal(out[: n // 8], src[: n // 8]) and not out[n // 8:].any()
     with pytest.raises(ValueError):
         C.check(C.load().mpm_profile_copy(b.ptr, a.ptr, 24, None))
+
+
+POLY_GEOMS = [
+    # name, box, errortol, xi, positions, radii, grid override, bloch
+    ("poly_random", [22.0, 23.0, 24.0], 1e-6, 0.5,
+     lambda: np.random.default_rng(31).uniform(0, 22, (24, 3)),
+     lambda: np.random.default_rng(32).choice([1.0, 1.4, 2.0], 24), None, None),
+    ("poly_overlap_bloch", [20.0, 21.0, 22.0], 1e-6, 0.5,
+     lambda: np.array([[5.0, 5.0, 5.0], [5.0, 5.0, 6.5], [5.0, 5.9, 5.0], [12.0, 3.0, 17.0], [12.4, 3.1, 17.2]]),
+     lambda: np.array([1.0, 1.6, 1.2, 2.5, 1.0]), None, (0.13, -0.21)),
+    ("poly_two_classes_grid", [24.0, 24.0, 24.0], 1e-5, 0.55,
+     lambda: np.random.default_rng(33).uniform(0, 24, (30, 3)),
+     lambda: np.where(np.arange(30) % 3 == 0, 1.8, 1.0), (36, 30, 40), None),
+]
+
+
+def _poly(name):
+    for g in POLY_GEOMS:
+        if g[0] == name:
+            _, box, tol, xi, pf, rf, grid, kb = g
+            return np.asarray(box, float), tol, xi, pf(), rf(), grid, kb
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name", [g[0] for g in POLY_GEOMS])
+def test_poly_apply_fp64_matches_oracle(name):
+    box, tol, xi, pos, rad, grid, kb = _poly(name)
+    _case(f"poly apply fp64 {name} N={len(pos)}")
+    N = len(pos)
+    rng = np.random.default_rng(41)
+    p = rng.standard_normal((5, N, 3)) + 1j * rng.standard_normal((5, N, 3))
+    rows = rng.standard_normal((5, N)) + 1j * rng.standard_normal((5, N))
+    op = Op(box, tol, xi, grid, radii=rad)
+    ref_op = O.PolyOperator(box, tol, xi, rad, grid)
+    if kb:
+        op.set_bloch(*kb)
+        ref_op.set_bloch(*kb)
+    op.set_positions(pos)
+    op.set_self_coef(rows)
+    ref_op.set_positions(pos)
+    info = op.info()
+    oi = ref_op.info()
+    assert info.polydisperse == 1 and tuple(info.grid) == oi["grid"] and info.support == oi["P"]
+    assert info.rc == pytest.approx(oi["rc"]) and info.num_pairs == oi["num_pairs"]
+    for parts in (C.PART_REAL, C.PART_RECIP, C.PART_ALL):
+        got = op.apply(p, C.PREC_DOUBLE, parts)
+        for b in range(5):
+            ref = ref_op.apply(p[b], rows[b], parts)
+            assert rel(got[b], ref) < 1e-11, (parts, b)
+
+
+@pytest.mark.parametrize("name", ["poly_random", "poly_overlap_bloch"])
+def test_poly_apply_fp32_within_reference_tolerance(name):
+    box, tol, xi, pos, rad, grid, kb = _poly(name)
+    _case(f"poly apply fp32 {name}")
+    N = len(pos)
+    rng = np.random.default_rng(42)
+    p = rng.standard_normal((3, N, 3)) + 1j * rng.standard_normal((3, N, 3))
+    op = Op(box, tol, xi, grid, radii=rad)
+    if kb:
+        op.set_bloch(*kb)
+    op.set_positions(pos)
+    op.set_self_coef(np.ones(N))
+    ref = op.apply(p, C.PREC_DOUBLE)
+    got = op.apply(p, C.PREC_MIXED)
+    assert rel(got, ref) < 1e-4
+
+
+def test_poly_solve_matches_oracle():
+    box, tol, xi, pos, rad, grid, kb = _poly("poly_random")
+    _case("poly gmres")
+    N = len(pos)
+    eps = -2.0 + 0.4j
+    sc = O.self_coefficient(np.full(N, eps), rad)
+    b = np.zeros((2, N, 3), complex)
+    b[0, :, 0] = 1.0
+    b[1, :, 2] = 1.0
+    op = Op(box, tol, xi, grid, radii=rad)
+    op.set_positions(pos)
+    op.set_self_coef(np.stack([sc, sc]))
+    x, reps = op.solve(b, tol=1e-8)
+    ref_op = O.PolyOperator(box, tol, xi, rad, grid)
+    ref_op.set_positions(pos)
+    for q in range(2):
+        xr, rr = ref_op.solve(b[q], sc, tol=1e-8)
+        assert reps[q]["converged"] and rr["converged"]
+        assert rel(x[q], xr.reshape(N, 3)) < 1e-6
+
+
+def test_poly_identical_radii_matches_monodisperse():
+    """Reference tests/test_electric_fields.py:30-82 on the device (1e-5)."""
+    _case("poly vs mono")
+    pos = np.array([[0.0, 0.0, 0.0], [5.0, 0.0, 0.0], [10.0, 0.0, 0.0], [15.0, 0.0, 0.0]])
+    p = np.zeros((1, 4, 3), complex)
+    p[0, :, 0] = 1.0
+    mono = Op([20.0] * 3, 1e-6, 0.5, layout=1)
+    poly = Op([20.0] * 3, 1e-6, 0.5, radii=[1.0] * 4)
+    for op in (mono, poly):
+        op.set_positions(pos)
+        op.set_self_coef(np.ones(4))
+    np.testing.assert_allclose(poly.apply(p), mono.apply(p), rtol=1e-5, atol=1e-5)
