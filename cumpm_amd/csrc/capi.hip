@@ -79,6 +79,7 @@ struct mpm_dipole_solver {
     size_t N = 0, W = 0, K = 0, frames = 0;
     double length_scale = 1.0, eps_scale = 1.0, vol_frac = 0.0;
     std::unique_ptr<mpm::EwaldOp> op;
+    bool use_poly = false;
     std::vector<cplx> dips;      // [F][W][N][K][3]
     std::vector<cplx> avg_dips;  // [F][W][K][3]
     std::vector<mpm_solve_report> reports;  // [F][W][K]
@@ -103,8 +104,8 @@ struct mpm_dipole_solver {
         for (size_t i = 1; i < N; ++i) mono = mono && radius[i] == radius[0];
         if (field_type == MPM_FIELD_MONODISPERSE && !mono)
             throw mpm::Error(MPM_ERR_RUNTIME, "field_type was set to 'monodisperse' but radii are not identical.");
-        if (field_type == MPM_FIELD_POLYDISPERSE || (field_type == MPM_FIELD_AUTO && !mono))
-            throw mpm::Error(MPM_ERR_UNSUPPORTED, "the polydisperse Ewald operator is not provided by this build");
+        // dipole_solver.cu:527-536: auto selects the polydisperse operator iff radii differ
+        use_poly = field_type == MPM_FIELD_POLYDISPERSE || (field_type == MPM_FIELD_AUTO && !mono);
         if (field_type == MPM_FIELD_DIRECT)
             throw mpm::Error(MPM_ERR_UNSUPPORTED, "the direct (open-boundary) field is not provided by this build");
         if (eps_p.empty()) throw mpm::Error(MPM_ERR_RUNTIME, "eps_p is empty.");
@@ -220,7 +221,10 @@ struct mpm_dipole_solver {
             od.errortol = ewald_tol > 0 ? ewald_tol : tol;
             od.xi = xi;
             od.support = support;
-            op = std::make_unique<mpm::EwaldOp>(od);
+            if (use_poly)
+                op = std::make_unique<mpm::EwaldOp>(od, radius.data(), (int64_t)N);
+            else
+                op = std::make_unique<mpm::EwaldOp>(od);
         } else if (N != n) {
             throw mpm::Error(MPM_ERR_RUNTIME, "The number of particles has changed!");
         }

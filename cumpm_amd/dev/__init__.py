@@ -1,3 +1,3 @@
-from .electric_field import MonodisperseEwaldElectricField, FieldCalcMode
+from .electric_field import MonodisperseEwaldElectricField, PolydisperseEwaldElectricField, FieldCalcMode
 
-__all__ = ["MonodisperseEwaldElectricField", "FieldCalcMode"]
+__all__ = ["MonodisperseEwaldElectricField", "PolydisperseEwaldElectricField", "FieldCalcMode"]

@@ -61,9 +61,7 @@ class MonodisperseEwaldElectricField:
         if grid is not None:
             d.grid[:] = [int(g) for g in grid]
         d.support = int(support)
-        h = ctypes.c_void_p()
-        check(C.load().mpm_op_create(ctypes.byref(d), ctypes.byref(h)))
-        self._h = h
+        self._h = self._create(d)
         self._box = (float(box_x), float(box_y), float(box_z))
         self._errortol = float(errortol)
         self._xi = float(xi)
@@ -72,6 +70,11 @@ class MonodisperseEwaldElectricField:
         self._sc = None
         self._epoint = None
         self._xbuf = self._ybuf = None
+
+    def _create(self, d):
+        h = ctypes.c_void_p()
+        check(C.load().mpm_op_create(ctypes.byref(d), ctypes.byref(h)))
+        return h
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -269,3 +272,44 @@ class MonodisperseEwaldElectricField:
         p = self._dip
         return (p[:, 0].real.copy(), p[:, 0].imag.copy(), p[:, 1].real.copy(), p[:, 1].imag.copy(),
                 p[:, 2].real.copy(), p[:, 2].imag.copy())
+
+
+class PolydisperseEwaldElectricField(MonodisperseEwaldElectricField):
+    """Polydisperse spectral-Ewald operator: one radius per particle (nondimensional).
+    Reference: dev/electric_field.py (PolydisperseEwaldElectricField) over
+    polydisperse_ewald_electric_field.cu; radii within 1e-5 share a class."""
+
+    def __init__(self, box_x, box_y, box_z, errortol, xi, calc_inter_dipole, particle_radii,
+                 solve_quadrupoles=False, quad_idxs=None, precision=None, *, grid=None, support=0):
+        self._radii = _vec(particle_radii)
+        if self._radii.size == 0:
+            raise ValueError("particle_radii is empty")
+        super().__init__(box_x, box_y, box_z, errortol, xi, calc_inter_dipole, 1.0, solve_quadrupoles,
+                         quad_idxs, precision, grid=grid, support=support)
+
+    def _create(self, d):
+        h = ctypes.c_void_p()
+        check(C.load().mpm_op_create_poly(ctypes.byref(d), C.dptr(self._radii), self._radii.size,
+                                          ctypes.byref(h)))
+        return h
+
+    @property
+    def radii(self):
+        return self._radii.copy()
+
+    @property
+    def num_unique_radii(self):
+        return int(self._info().num_unique_radii)
+
+    @property
+    def num_offsets(self):
+        return int(self._info().support) ** 3
+
+    @property
+    def table_size(self):
+        return int(np.ceil((max(self.rc, 1.001) - 1.0) / 0.001)) + 1
+
+    def update_particle_coordinates(self, x_part, y_part, z_part):
+        if _vec(x_part).size != self._radii.size:
+            raise ValueError("the polydisperse operator needs one radius per particle")
+        super().update_particle_coordinates(x_part, y_part, z_part)

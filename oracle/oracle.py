@@ -224,10 +224,11 @@ class DipoleSolver:
 
     def __init__(self, box, eps_p, radius, eps_m=1.0, xi=0.5, tol=1e-3, guess_type="derivative",
                  solver_type="gmres", E0=None, restart=50, maxiter=1000, jacobi=True,
-                 grid=None, P=None, ewald_tol=None):
+                 grid=None, P=None, ewald_tol=None, field_type="auto"):
         self.box = np.asarray(box, np.float64)
         self.eps_p = np.asarray(eps_p, np.complex128)
-        self.radius = float(radius)
+        self.radius = np.asarray(radius, np.float64)  # scalar or one per particle
+        self.field_type = field_type
         self.eps_m = float(eps_m)
         self.xi = xi
         self.tol = tol
@@ -252,11 +253,18 @@ class DipoleSolver:
             eps = np.repeat(eps, N, axis=1)
         self.eps = eps / self.eps_m
         self.W = eps.shape[0]
-        self.length_scale = self.radius
+        rad = np.broadcast_to(self.radius, (N,)).astype(np.float64)
+        # dipole_solver.cu:184-221 (lengths over the smallest radius) and :527-536 (auto:
+        # polydisperse iff the radii differ)
+        self.length_scale = rad.min()
         self.L = self.box / self.length_scale
-        self.a = 1.0
-        self.vol_frac = 4.0 / 3.0 * np.pi * N * self.a ** 3 / np.prod(self.L)
-        self.op = Operator(self.L, self.ewald_tol or self.tol, self.xi, self.grid, self.P)
+        self.a = rad / self.length_scale
+        self.vol_frac = 4.0 / 3.0 * np.pi * np.sum(self.a ** 3) / np.prod(self.L)
+        poly = self.field_type == "polydisperse" or (self.field_type == "auto" and np.any(rad != rad[0]))
+        if poly:
+            self.op = PolyOperator(self.L, self.ewald_tol or self.tol, self.xi, self.a, self.grid, self.P)
+        else:
+            self.op = Operator(self.L, self.ewald_tol or self.tol, self.xi, self.grid, self.P)
 
     def _rhs(self, N):
         K = self.E0.shape[0]
@@ -269,7 +277,7 @@ class DipoleSolver:
     def _mean_field(self, w, N):
         K = self.E0.shape[0]
         beta = (self.eps[w] - 1.0) / (self.eps[w] + 2.0)
-        val = 4.0 * np.pi * self.a ** 3 * beta / (1.0 - beta * self.vol_frac)
+        val = 4.0 * np.pi * self.a ** 3 * beta / (1.0 - beta * self.vol_frac)  # per particle
         rhs = self._rhs(N)  # (K, N, 3)
         return val[None, :, None] * rhs
 

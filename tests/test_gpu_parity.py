@@ -482,3 +482,46 @@ def test_poly_identical_radii_matches_monodisperse():
         op.set_positions(pos)
         op.set_self_coef(np.ones(4))
     np.testing.assert_allclose(poly.apply(p), mono.apply(p), rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("batched,precision", [(False, "double"), (True, "refined")])
+def test_dipole_solver_polydisperse_matches_oracle_driver(batched, precision):
+    """field_type="auto" with two radius classes selects the polydisperse operator
+    (dipole_solver.cu:527-536); physical units, nondimensionalised by the smallest radius."""
+    import cumpm_amd
+    _case(f"poly driver batched={batched} {precision}")
+    rng = np.random.default_rng(7)
+    cols, rows, sp = 6, 5, 11.0
+    pos, box = hex_monolayer(cols, rows, sp, jitter=0.2, seed=3)
+    box = np.array([box[0] + 40.0, box[1] + 40.0, 100.0])
+    radii = np.where(rng.random(len(pos)) < 0.5, 4.0, 5.0)
+    omega = np.linspace(0.3, 0.7, 3)
+    eps = 4.0 - 1.7 ** 2 / (omega ** 2 + 1j * omega * 0.08)
+    s = cumpm_amd.dipole_solver(box, eps, radius=radii, eps_m=1.0, tol=1e-8, quiet=True, precision=precision,
+                                batched=batched)
+    s.compute(pos)
+    od = O.DipoleSolver(box, eps.reshape(-1, 1), radii, 1.0, tol=1e-8)
+    od.compute(pos)
+    assert rel(s.get_dipoles(), od.get_dipoles()) < 1e-6
+    assert rel(s.get_eff_polarizability(), od.get_eff_polarizability()) < 1e-6
+    reps = s.get_reports()
+    assert all(r["converged"] for w in reps[0] for r in w)
+
+
+def test_dev_polydisperse_field_matches_monodisperse_for_identical_radii():
+    """Reference tests/test_electric_fields.py:30-82 through the dev API (1e-5; splits 1e-12)."""
+    from cumpm_amd.dev import MonodisperseEwaldElectricField, PolydisperseEwaldElectricField
+    _case("dev poly vs mono")
+    pos = np.array([[0.0, 0.0, 0.0], [5.0, 0.0, 0.0], [10.0, 0.0, 0.0], [15.0, 0.0, 0.0]])
+    x, y, z = pos[:, 0], pos[:, 1], pos[:, 2]
+    fields = []
+    for ef in (MonodisperseEwaldElectricField(20.0, 20.0, 20.0, 1e-6, 0.5, True, precision="double"),
+               PolydisperseEwaldElectricField(20.0, 20.0, 20.0, 1e-6, 0.5, True, [1.0] * 4, precision="double")):
+        ef.update_particle_coordinates(x, y, z)
+        ef.update_dipoles(np.ones(4), np.zeros(4), np.zeros(4))
+        ef.set_self_coef(np.ones(4), np.zeros(4))
+        ef.calculate()
+        tot = ef.get_epoint_host()
+        np.testing.assert_allclose(tot, ef.compute_real_field() + ef.compute_inverse_field(), rtol=1e-12, atol=1e-12)
+        fields.append(tot)
+    np.testing.assert_allclose(fields[1], fields[0], rtol=1e-5, atol=1e-5)
