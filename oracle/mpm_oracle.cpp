@@ -334,6 +334,179 @@ static void build_dipole_tables(double xi, DipoleTables& T) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Quadrupole pair functions (monodisperse_ewald_electric_field.cu:199-470).
+// Each is  A+ e^{-(r+2)^2 xi^2} + B+ erfc((r+2) xi) + A- e^{-(r-2)^2 xi^2}
+//        + B- erfc((r-2) xi) + A0 e^{-r^2 xi^2} + B0 erfc(r xi) + [r < 2] reg,
+// with A+- = a (O +- E) for the odd and even powers O, E of r and B+- = b (E' +- O').
+// Order: field_quad_1..3 (E from Q: perp_Q, para_Q, Q3), grad_quad_1..4 (G1..G4).
+// ---------------------------------------------------------------------------
+struct Img {
+    double Ap, Am, A0, Bp, Bm, B0, reg;
+};
+static double img_sum(double r, double xi, const Img& c) {
+    const double x2 = xi * xi;
+    double f = c.Ap * std::exp(-(r + 2.0) * (r + 2.0) * x2) + c.Bp * std::erfc((r + 2.0) * xi) +
+               c.Am * std::exp(-(r - 2.0) * (r - 2.0) * x2) + c.Bm * std::erfc((r - 2.0) * xi) +
+               c.A0 * std::exp(-r * r * x2) + c.B0 * std::erfc(r * xi);
+    if (r < 2.0) f += c.reg;
+    return f;
+}
+
+static void quad_pair_functions(double r, double xi, double out[7]) {
+    const double x = xi, x2 = x * x, x4 = x2 * x2, x5 = x4 * x, x6 = x4 * x2, x7 = x6 * x, x8 = x4 * x4,
+                 x9 = x8 * x, x10 = x8 * x2;
+    const double r2 = r * r, r3 = r2 * r, r4 = r2 * r2, r5 = r4 * r, r6 = r4 * r2, r7 = r6 * r, r8 = r4 * r4,
+                 r9 = r8 * r, r10 = r8 * r2;
+    const double s15 = std::pow(kPi, 1.5), pi = kPi;
+    Img c;
+    {   // field_quad_1
+        const double a = 15.0 / (16384.0 * s15 * x7 * r4);
+        const double O = -24.0 * x6 * r7 - 4.0 * x4 * (9.0 - 8.0 * x2) * r5 + 2.0 * x2 * (21.0 - 80.0 * x2 + 64.0 * x4) * r3 -
+                         (45.0 - 120.0 * x2 + 192.0 * x4 - 512.0 * x6) * r;
+        const double E = 48.0 * x6 * r6 + 8.0 * x4 * (3.0 - 8.0 * x2) * r4 - 4.0 * x2 * std::pow(3.0 - 8.0 * x2, 2) * r2 -
+                         2.0 * (45.0 + 24.0 * x2 - 64.0 * x4 + 512.0 * x6);
+        const double b = 15.0 / (32768.0 * pi * x8 * r4);
+        const double Be = 48.0 * x8 * r8 + 32.0 * x6 * (3.0 - 8.0 * x2) * r6 - 24.0 * x4 * (3.0 - 16.0 * x2) * r4 +
+                          72.0 * x2 * (1.0 - 8.0 * x2) * r2 - 45.0 + 480.0 * x2 + 4096.0 * x8;
+        c.Ap = a * (O + E);
+        c.Am = a * (O - E);
+        c.A0 = 15.0 / (8192.0 * s15 * x7 * r3) *
+               (24.0 * x6 * r6 + 4.0 * x4 * (9.0 - 32.0 * x2) * r4 - 2.0 * x2 * (21.0 - 128.0 * x2) * r2 + 45.0 - 480.0 * x2);
+        c.Bp = c.Bm = b * Be;
+        c.B0 = 15.0 / (16384.0 * pi * x8 * r4) * (-(Be - 4096.0 * x8));
+        c.reg = -15.0 / (4.0 * pi * r4) + 15.0 * r2 / (64.0 * pi) * (1.0 - 3.0 * r2 / 16.0);
+        out[0] = img_sum(r, xi, c);
+    }
+    {   // field_quad_2
+        const double a = 3.0 / (16384.0 * s15 * x7 * r4);
+        const double O = -40.0 * x6 * r7 - 20.0 * x4 * (11.0 - 24.0 * x2) * r5 - 2.0 * x2 * (45.0 - 80.0 * x2 + 64.0 * x4) * r3 +
+                         (45.0 - 120.0 * x2 + 192.0 * x4 - 512.0 * x6) * r;
+        const double E = 80.0 * x6 * r6 + 8.0 * x4 * (45.0 + 8.0 * x2) * r4 - 4.0 * x2 * (15.0 + 48.0 * x2 - 64.0 * x4) * r2 +
+                         2.0 * (45.0 + 24.0 * x2 - 64.0 * x4 + 512.0 * x6);
+        const double b = 3.0 / (32768.0 * pi * x8 * r4);
+        const double Be = 80.0 * x8 * r8 + 160.0 * x6 * (3.0 - 8.0 * x2) * r6 + 120.0 * x4 * (3.0 - 16.0 * x2) * r4 -
+                          120.0 * x2 * (1.0 - 8.0 * x2) * r2 + 45.0 - 480.0 * x2 - 4096.0 * x8;
+        const double Bo = -2048.0 * x8 * r5;
+        c.Ap = a * (O + E);
+        c.Am = a * (O - E);
+        c.A0 = 15.0 / (8192.0 * s15 * x7 * r3) *
+               (8.0 * x6 * r6 + 4.0 * x4 * (11.0 - 32.0 * x2) * r4 + 2.0 * x2 * (9.0 - 64.0 * x2) * r2 - 9.0 + 96.0 * x2);
+        c.Bp = b * (Be + Bo);
+        c.Bm = b * (Be - Bo);
+        c.B0 = 15.0 / (16384.0 * pi * x8 * r4) *
+               (-16.0 * x8 * r8 - 32.0 * x6 * (3.0 - 8.0 * x2) * r6 - 24.0 * x4 * (3.0 - 16.0 * x2) * r4 +
+                24.0 * x2 * (1.0 - 8.0 * x2) * r2 - 9.0 + 96.0 * x2);
+        c.reg = 3.0 / (4.0 * pi * r4) - 3.0 * r / (8.0 * pi) * (1.0 - 5.0 * r / 8.0 + 5.0 * r3 / 128.0);
+        out[1] = img_sum(r, xi, c);
+    }
+    {   // field_quad_3
+        const double a = 5.0 / (1024.0 * s15 * x5 * r2);
+        const double O = 4.0 * x4 * r5 + 16.0 * x2 * (1.0 - 2.0 * x2) * r3 + 3.0 * r;
+        const double E = -8.0 * x4 * r4 - 24.0 * x2 * r2 + 6.0;
+        const double b = 5.0 / (2048.0 * pi * x6 * r2);
+        const double Be = -8.0 * x6 * r6 - 12.0 * x4 * (3.0 - 8.0 * x2) * r4 - 6.0 * x2 * (3.0 - 16.0 * x2) * r2 + 3.0 - 24.0 * x2;
+        const double Bo = 128.0 * x6 * r3;
+        c.Ap = a * (O + E);
+        c.Am = a * (O - E);
+        c.A0 = 5.0 / (512.0 * s15 * x5 * r) * (-4.0 * x4 * r4 - 16.0 * x2 * (1.0 - 3.0 * x2) * r2 - 3.0 + 24.0 * x2);
+        c.Bp = b * (Be + Bo);
+        c.Bm = b * (Be - Bo);
+        c.B0 = 5.0 / (1024.0 * pi * x6 * r2) * (-Be);
+        c.reg = 5.0 * r / (8.0 * pi) * (1.0 - 3.0 * r / 4.0 + r3 / 16.0);
+        out[2] = img_sum(r, xi, c);
+    }
+    {   // grad_quad_1
+        const double a = 75.0 / (16384.0 * s15 * x7 * r3);
+        const double O = 8.0 * x6 * r7 + (44.0 * x4 - 32.0 * x6) * r5 + (18.0 * x2 + 32.0 * x4) * r3 - (9.0 + 24.0 * x2) * r;
+        const double E = -16.0 * x6 * r6 - (72.0 * x4 - 64.0 * x6) * r4 + (12.0 * x2 - 64.0 * x4) * r2 - (18.0 - 48.0 * x2);
+        const double Be = 16.0 * x8 * r8 + (96.0 * x6 - 128.0 * x8) * r6 + (72.0 * x4 - 192.0 * x6 + 256.0 * x8) * r4 -
+                          (24.0 * x2 - 96.0 * x4 + 256.0 * x6) * r2 + 9.0 - 48.0 * x2 + 192.0 * x4;
+        c.Ap = a * (O + E);
+        c.Am = a * (O - E);
+        c.A0 = -75.0 / (8192.0 * s15 * x7 * r2) *
+               (8.0 * x6 * r6 + (44.0 * x4 - 64.0 * x6) * r4 + (18.0 * x2 - 64.0 * x4 + 128.0 * x6) * r2 - 9.0 + 48.0 * x2 -
+                192.0 * x4);
+        c.Bp = c.Bm = -75.0 / (32768.0 * pi * x8 * r3) * Be;
+        c.B0 = 75.0 / (16384.0 * pi * x8 * r3) * Be;
+        c.reg = 75.0 * r * (4.0 - r2) * (4.0 - r2) / (1024.0 * pi);
+        out[3] = img_sum(r, xi, c);
+    }
+    {   // grad_quad_2
+        const double a = 3.0 / (32768.0 * s15 * x9 * r5);
+        const double O = -48.0 * x8 * r9 - (576.0 * x6 - 608.0 * x8) * r7 - (1536.0 * x4 - 2576.0 * x6 + 3968.0 * x8) * r5 -
+                         (360.0 * x2 + 360.0 * x4 + 640.0 * x6 - 512.0 * x8) * r3 +
+                         (135.0 + 180.0 * x2 + 480.0 * x4 - 768.0 * x6 + 2048.0 * x8) * r;
+        const double E = 96.0 * x8 * r8 + (1056.0 * x6 - 1216.0 * x8) * r6 + (2160.0 * x4 - 4320.0 * x6 - 256.0 * x8) * r4 -
+                         (360.0 * x2 - 1680.0 * x4 - 768.0 * x6 + 1024.0 * x8) * r2 +
+                         (270.0 - 1080.0 * x2 - 192.0 * x4 + 512.0 * x6 - 4096.0 * x8);
+        const double P = 288.0 * x10 * r10 + (3600.0 * x8 - 4800.0 * x10) * r8 +
+                         (10800.0 * x6 - 28800.0 * x8 + 38400.0 * x10) * r6 + (5400.0 * x4 - 21600.0 * x6 + 57600.0 * x8) * r4 -
+                         (1350.0 * x2 - 7200.0 * x4 + 28800.0 * x6) * r2 + 405.0 - 2700.0 * x2 + 14400.0 * x4;
+        const double b = 1.0 / (65536.0 * pi * x10 * r5);
+        c.Ap = a * (O + E);
+        c.Am = a * (O - E);
+        c.A0 = 1.0 / (16384.0 * s15 * x9 * r4) *
+               (144.0 * x8 * r8 + (1728.0 * x6 - 2400.0 * x8) * r6 + (4608.0 * x4 - 13200.0 * x6 + 19200.0 * x8) * r4 +
+                (1080.0 * x2 - 5400.0 * x4 + 19200.0 * x6) * r2 - 405.0 + 2700.0 * x2 - 14400.0 * x4);
+        c.Bp = b * (P + 49152.0 * x10 + 49152.0 * x10 * r5);
+        c.Bm = b * (P + 49152.0 * x10 - 49152.0 * x10 * r5);
+        c.B0 = -1.0 / (32768.0 * pi * x10 * r5) * P;
+        c.reg = -3.0 / (2.0 * pi) * (1.0 / r5 - 1.0 + 25.0 * r / 32.0 - 25.0 * r3 / 256.0 + 3.0 * r5 / 512.0);
+        out[4] = img_sum(r, xi, c);
+    }
+    {   // grad_quad_3
+        const double a = -15.0 / (65536.0 * s15 * x9 * r5);
+        const double O = 48.0 * x8 * r9 + (336.0 * x6 - 288.0 * x8) * r7 + (216.0 * x4 + 144.0 * x6 + 128.0 * x8) * r5 -
+                         (180.0 * x2 - 120.0 * x4 + 640.0 * x6 - 512.0 * x8) * r3 +
+                         (135.0 + 180.0 * x2 + 480.0 * x4 - 768.0 * x6 + 2048.0 * x8) * r;
+        const double E = -96.0 * x8 * r8 - 576.0 * (x6 - x8) * r6 - (480.0 * x6 + 256.0 * x8) * r4 +
+                         (720.0 * x4 + 768.0 * x6 - 1024.0 * x8) * r2 -
+                         (-270.0 + 1080.0 * x2 + 192.0 * x4 - 512.0 * x6 + 4096.0 * x8);
+        const double P = -96.0 * x10 * r10 + (-720.0 * x8 + 960.0 * x10) * r8 -
+                         (720.0 * x6 - 1920.0 * x8 + 2560.0 * x10) * r6 + (360.0 * x4 - 1440.0 * x6 + 3840.0 * x8) * r4 -
+                         (270.0 * x2 - 1440.0 * x4 + 5760.0 * x6) * r2 + 135.0 - 900.0 * x2 + 4800.0 * x4;
+        c.Ap = a * (O + E);
+        c.Am = a * (O - E);
+        c.A0 = 15.0 / (32768.0 * s15 * x9 * r4) *
+               (48.0 * x8 * r8 + (336.0 * x6 - 480.0 * x8) * r6 + (216.0 * x4 - 720.0 * x6 + 1280.0 * x8) * r4 -
+                (180.0 * x2 - 840.0 * x4 + 2560.0 * x6) * r2 + 135.0 - 900.0 * x2 + 4800.0 * x4);
+        c.Bp = c.Bm = -15.0 / (131072.0 * pi * x10 * r5) * (P + 16384.0 * x10);
+        c.B0 = -15.0 / (65536.0 * pi * x10 * r5) * (-P);
+        c.reg = -15.0 * std::pow(r2 - 4.0, 3) * (3.0 * r4 + 6.0 * r2 + 8.0) / (2048.0 * pi * r5);
+        out[5] = img_sum(r, xi, c);
+    }
+    {   // grad_quad_4
+        const double a = -15.0 / (65536.0 * s15 * x9 * r5);
+        const double O = 144.0 * x8 * r9 + (288.0 * x6 + 96.0 * x8) * r7 - (432.0 * x4 - 912.0 * x6 + 896.0 * x8) * r5 +
+                         (720.0 * x2 - 2280.0 * x4 + 4480.0 * x6 - 3584.0 * x8) * r3 -
+                         (945.0 + 1260.0 * x2 + 3360.0 * x4 - 5376.0 * x6 + 14336.0 * x8) * r;
+        const double E = -288.0 * x8 * r8 - (288.0 * x6 + 192.0 * x8) * r6 + (720.0 * x4 - 480.0 * x6 + 1792.0 * x8) * r4 -
+                         (1080.0 * x2 + 2160.0 * x4 + 5376.0 * x6 - 7168.0 * x8) * r2 -
+                         (1890.0 - 7560.0 * x2 - 1344.0 * x4 + 3584.0 * x6 - 28672.0 * x8);
+        const double P = 288.0 * x10 * r10 + (720.0 * x8 - 960.0 * x10) * r8 - (720.0 * x6 - 1920.0 * x8 + 2560.0 * x10) * r6 +
+                         (1080.0 * x4 - 4320.0 * x6 + 11520.0 * x8) * r4 - (1350.0 * x2 - 7200.0 * x4 + 28800.0 * x6) * r2 +
+                         945.0 - 6300.0 * x2 + 33600.0 * x4;
+        c.Ap = a * (O + E);
+        c.Am = a * (O - E);
+        c.A0 = 15.0 / (32768.0 * s15 * x9 * r4) *
+               (144.0 * x8 * r8 + (288.0 * x6 - 480.0 * x8) * r6 - (432.0 * x4 - 1200.0 * x6 + 1280.0 * x8) * r4 +
+                (720.0 * x2 - 3000.0 * x4 + 6400.0 * x6) * r2 - 945.0 + 6300.0 * x2 - 33600.0 * x4);
+        c.Bp = c.Bm = 15.0 / (131072.0 * pi * x10 * r5) * (P + 114688.0 * x10);
+        c.B0 = -15.0 / (65536.0 * pi * x10 * r5) * P;
+        c.reg = -15.0 * (3584.0 - 80.0 * r6 - 30.0 * r8 + 9.0 * r10) / (2048.0 * pi * r5);
+        out[6] = img_sum(r, xi, c);
+    }
+}
+
+// Gradient self term, the quadrupole tables' entry 0 (:465-466).
+static double quad_self_grad(double xi) {
+    const double x2 = xi * xi, x4 = x2 * x2, x5 = x4 * xi, sp = std::sqrt(kPi);
+    return 0.5 / kPi *
+           (3.0 * (3.0 - 10.0 * x2 + 20.0 * x4) / (8.0 * sp * x5) -
+            3.0 * (3.0 + 2.0 * x2 + 4.0 * x4) * std::exp(-4.0 * x2) / (8.0 * sp * x5) + 3.0 * std::erfc(2.0 * xi));
+}
+
 // monodisperse_ewald_electric_field.cu:668-685 (clamps at the last sample).
 static double interp(double r, const std::vector<double>& rt, const std::vector<double>& y) {
     const size_t ts = y.size();
@@ -506,6 +679,15 @@ struct Op {
     std::vector<double> pr, pperp, ppara, uself;  // r samples, tables [entry][col], self per unique radius
     int ncol = 0;
     std::vector<double> pw;  // [i][o][3] spread/contract weight vectors
+    // quadrupoles (monodisperse): particles qidx[q] carry 5 traceless components
+    // (xx, xy, xz, yy, yz) after the 3N dipole unknowns; qmap[i] = q or -1
+    bool quads = false;
+    std::vector<int> qidx, qmap;
+    std::vector<double> qtab[7];        // field_quad_1..3, grad_quad_1..4 on the dipole table's r
+    double self_grad = 0;
+    std::vector<double> nb_q;           // [M][7] pair values
+    std::vector<double> scQ, scGP, scGQ, Qf, Qd;  // k-space: V, V, V, 5V, 5V
+    size_t n_unknowns() const { return 3 * (size_t)N + 5 * qidx.size(); }
     double kbx = 0, kby = 0;  // Bloch wavevector (x, y)
     // geometry
     int N = 0;
@@ -526,6 +708,13 @@ static void compute_scale(Op& op) {
     op.V = (size_t)nx * ny * nz;
     op.scale.assign(op.V, 0.0);
     op.khat.assign(3 * op.V, 0.0);
+    if (op.quads) {
+        op.scQ.assign(op.V, 0.0);
+        op.scGP.assign(op.V, 0.0);
+        op.scGQ.assign(op.V, 0.0);
+        op.Qf.assign(5 * op.V, 0.0);
+        op.Qd.assign(5 * op.V, 0.0);
+    }
     const double xi = op.xi;
 #pragma omp parallel for schedule(static)
     for (long v = 0; v < (long)op.V; ++v) {
@@ -551,7 +740,23 @@ static void compute_scale(Op& op) {
         op.khat[3 * v + 2] = kz / k;
         const double etak2 = k2 - (Gx * Gx * S.eta[0] + Gy * Gy * S.eta[1] + Gz * Gz * S.eta[2]);
         const double t = std::sin(k) / k - std::cos(k);
-        op.scale[v] = 9.0 * t * t * std::exp(-etak2 / (4.0 * xi * xi)) / (k2 * k2) / (double)op.V;
+        const double ex = std::exp(-etak2 / (4.0 * xi * xi));
+        op.scale[v] = 9.0 * t * t * ex / (k2 * k2) / (double)op.V;
+        if (op.quads) {  // :1367-1398: sphere form factors j1, j2 and the quadrupole projectors
+            const double j1 = t / k;
+            const double j2 = (3.0 / k2 - 1.0) * std::sin(k) / k - 3.0 * std::cos(k) / k2;
+            const double e2 = ex / k2 / (double)op.V;
+            op.scQ[v] = -22.5 * j1 * j2 * e2;
+            op.scGP[v] = 45.0 * j1 * j2 * e2;
+            op.scGQ[v] = 112.5 * j2 * j2 * e2;
+            const double h0 = kx / k, h1 = ky / k, h2 = kz / k;
+            const double qf[5] = {h0 * h0 - 1.0 / 3.0, h0 * h1, h0 * h2, h1 * h1 - 1.0 / 3.0, h1 * h2};
+            const double qd[5] = {h0 * h0 - h2 * h2, 2.0 * h0 * h1, 2.0 * h0 * h2, h1 * h1 - h2 * h2, 2.0 * h1 * h2};
+            for (int c = 0; c < 5; ++c) {
+                op.Qf[5 * v + c] = qf[c];
+                op.Qd[5 * v + c] = qd[c];
+            }
+        }
     }
 }
 
@@ -668,6 +873,7 @@ static void build_neighbors(Op& op) {
     op.nb_j.resize(M);
     op.nb_perp.assign(M, 0.0);
     op.nb_para.assign(M, 0.0);
+    op.nb_q.assign(op.quads ? 7 * (size_t)M : 0, 0.0);
 #pragma omp parallel for schedule(dynamic, 64)
     for (int i = 0; i < N; ++i) {
         int s = op.nb_start[i];
@@ -686,6 +892,8 @@ static void build_neighbors(Op& op) {
                 double de = d < 1.0 ? 1.0 : d;
                 op.nb_perp[s + k] = interp(de, op.T.r, op.T.perp);
                 op.nb_para[s + k] = interp(de, op.T.r, op.T.para);
+                if (op.quads)
+                    for (int t = 0; t < 7; ++t) op.nb_q[7 * (size_t)(s + k) + t] = interp(de, op.T.r, op.qtab[t]);
             }
         }
     }
@@ -770,7 +978,10 @@ static void build_stencils(Op& op) {
 // self_coef: [N] complex.  parts: bit 0 real space + self, bit 1 reciprocal.
 static void apply(const Op& op, const cplx* self_coef, const cplx* p, cplx* E, int parts) {
     const int N = op.N;
-    for (long q = 0; q < 3L * N; ++q) E[q] = 0.0;
+    const int Q = (int)op.qidx.size();
+    for (size_t q = 0; q < op.n_unknowns(); ++q) E[q] = 0.0;
+    const cplx* qv = p + 3 * (size_t)N;  // quadrupoles [Q][5]
+    cplx* G = E + 3 * (size_t)N;
     if ((parts & 2) && op.poly) {
         // scalar grid S = sum C.p, scaled in k-space, E_i = h^3 sum C_i S (:444-694)
         const Sizing& S = op.S;
@@ -800,26 +1011,52 @@ static void apply(const Op& op, const cplx* self_coef, const cplx* p, cplx* E, i
         const Sizing& S = op.S;
         const size_t V = op.V;
         const long SS = (long)S.width * S.width * S.width;
-        std::vector<cplx> grid(3 * V, cplx(0, 0));
-        // spread (serial over particles to keep the accumulation race-free)
+        const int nc = op.quads ? 8 : 3;  // E components, then the 5 quadrupole-gradient components
+        std::vector<cplx> grid(nc * V, cplx(0, 0));
+        // spread (serial over particles to keep the accumulation race-free); quadrupoles
+        // use the dipoles' Gaussian weights (:1871-1999)
         for (int i = 0; i < N; ++i) {
+            const int q = op.quads ? op.qmap[i] : -1;
             for (long o = 0; o < SS; ++o) {
                 const double w = op.C_spread * op.sp_e[(size_t)i * SS + o];
                 const int v = op.sp_idx[(size_t)i * SS + o];
                 for (int c = 0; c < 3; ++c) grid[c * V + v] += w * p[3 * i + c];
+                if (q >= 0)
+                    for (int c = 0; c < 5; ++c) grid[(3 + c) * V + v] += w * qv[5 * q + c];
             }
         }
-        for (int c = 0; c < 3; ++c) fft::fft3d(grid.data() + c * V, S.n[0], S.n[1], S.n[2], -1);
+        for (int c = 0; c < nc; ++c) fft::fft3d(grid.data() + c * V, S.n[0], S.n[1], S.n[2], -1);
 #pragma omp parallel for schedule(static)
         for (long v = 0; v < (long)V; ++v) {
             const double kx = op.khat[3 * v], ky = op.khat[3 * v + 1], kz = op.khat[3 * v + 2];
             const cplx dot = grid[v] * kx + grid[V + v] * ky + grid[2 * V + v] * kz;
-            const cplx s = dot * op.scale[v];
+            cplx s = dot * op.scale[v];
+            if (op.quads) {  // scale_kernel_joint (:1435-1487)
+                cplx gq = 0.0;
+                for (int c = 0; c < 5; ++c) gq += grid[(3 + c) * V + v] * op.Qd[5 * v + c];
+                const cplx I(0.0, 1.0);
+                s += gq * (I * op.scQ[v]);
+                const cplx g = dot * (I * op.scGP[v]) + gq * op.scGQ[v];
+                for (int c = 0; c < 5; ++c) grid[(3 + c) * V + v] = g * op.Qf[5 * v + c];
+            }
             grid[v] = s * kx;
             grid[V + v] = s * ky;
             grid[2 * V + v] = s * kz;
         }
-        for (int c = 0; c < 3; ++c) fft::fft3d(grid.data() + c * V, S.n[0], S.n[1], S.n[2], +1);
+        for (int c = 0; c < nc; ++c) fft::fft3d(grid.data() + c * V, S.n[0], S.n[1], S.n[2], +1);
+        if (op.quads) {
+#pragma omp parallel for schedule(static)
+            for (int q = 0; q < Q; ++q) {
+                const int i = op.qidx[q];
+                cplx acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+                for (long o = 0; o < SS; ++o) {
+                    const double w = op.C_contract * op.sp_e[(size_t)i * SS + o];
+                    const int v = op.sp_idx[(size_t)i * SS + o];
+                    for (int c = 0; c < 5; ++c) acc[c] += grid[(3 + c) * V + v] * w;
+                }
+                for (int c = 0; c < 5; ++c) G[5 * q + c] += acc[c];
+            }
+        }
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < N; ++i) {
             cplx acc[3] = {0.0, 0.0, 0.0};
@@ -836,6 +1073,10 @@ static void apply(const Op& op, const cplx* self_coef, const cplx* p, cplx* E, i
             const cplx f = self_coef[i] + (op.poly ? op.uself[op.ridx[i]] : op.T.self_perp);
             for (int c = 0; c < 3; ++c) E[3 * i + c] += f * p[3 * i + c];
         }
+        for (int q = 0; q < Q; ++q) {  // :1615-1627
+            const cplx f = 2.5 * self_coef[op.qidx[q]] + 0.5 * op.self_grad;
+            for (int c = 0; c < 5; ++c) G[5 * q + c] += f * qv[5 * q + c];
+        }
         // source-centric scatter, as the reference kernel: E_j += T(r_ij) p_i
         const bool bloch = (op.kbx != 0.0 || op.kby != 0.0);
         for (int i = 0; i < N; ++i) {
@@ -851,12 +1092,47 @@ static void apply(const Op& op, const cplx* self_coef, const cplx* p, cplx* E, i
                 cplx e[3];
                 for (int c = 0; c < 3; ++c)
                     e[c] = (p[3 * i + c] - rp * u[c]) * op.nb_perp[k] + (rp * u[c]) * op.nb_para[k];
+                cplx f(1.0, 0.0);
                 if (bloch) {
                     const double ph = -(op.kbx * r[0] + op.kby * r[1]);
-                    const cplx f(std::cos(ph), std::sin(ph));
-                    for (int c = 0; c < 3; ++c) e[c] *= f;
+                    f = cplx(std::cos(ph), std::sin(ph));
                 }
-                for (int c = 0; c < 3; ++c) E[3 * j + c] += e[c];
+                // Dipole-quadrupole cross terms carry the signs of the reference's Python
+                // formulation (tests/quadrupole_field.py:214-215: E += Q terms, G -= P terms),
+                // which keep real + reciprocal independent of xi; the CUDA kernel
+                // (monodisperse...cu:1749-1800) has both signs flipped.
+                if (op.quads) {  // :1631-1869, source i, target j, u = r_ij / max(d, 1)
+                    const double* t = &op.nb_q[7 * (size_t)k];
+                    const int qs = op.qmap[i], qt = op.qmap[j];
+                    const double I5[5] = {1.0, 0.0, 0.0, 1.0, 0.0};
+                    const double rr[5] = {u[0] * u[0], u[0] * u[1], u[0] * u[2], u[1] * u[1], u[1] * u[2]};
+                    cplx Qr[3] = {0.0, 0.0, 0.0}, Qrr = 0.0;
+                    if (qs >= 0) {
+                        const cplx* qq = qv + 5 * qs;
+                        Qr[0] = qq[0] * u[0] + qq[1] * u[1] + qq[2] * u[2];
+                        Qr[1] = qq[1] * u[0] + qq[3] * u[1] + qq[4] * u[2];
+                        Qr[2] = qq[2] * u[0] + qq[4] * u[1] - (qq[0] + qq[3]) * u[2];
+                        Qrr = Qr[0] * u[0] + Qr[1] * u[1] + Qr[2] * u[2];
+                        for (int c = 0; c < 3; ++c) e[c] += 0.5 * (Qrr * (t[0] * u[c]) + Qr[c] * (2.0 * t[1]));
+                    }
+                    if (qt >= 0) {
+                        const cplx* pi = p + 3 * i;
+                        const cplx PrrP[5] = {pi[0] * (2.0 * u[0]), pi[0] * u[1] + pi[1] * u[0], pi[0] * u[2] + pi[2] * u[0],
+                                              pi[1] * (2.0 * u[1]), pi[1] * u[2] + pi[2] * u[1]};
+                        cplx g[5];
+                        for (int c = 0; c < 5; ++c) g[c] = -(rp * (t[0] * rr[c] + (t[1] + t[2]) * I5[c]) + PrrP[c] * t[1]);
+                        if (qs >= 0) {
+                            const cplx* qq = qv + 5 * qs;
+                            const cplx QrrQ[5] = {Qr[0] * (2.0 * u[0]), Qr[0] * u[1] + Qr[1] * u[0], Qr[0] * u[2] + Qr[2] * u[0],
+                                                  Qr[1] * (2.0 * u[1]), Qr[1] * u[2] + Qr[2] * u[1]};
+                            for (int c = 0; c < 5; ++c)
+                                g[c] += 0.5 * (Qrr * ((t[3] + t[5]) * I5[c] + t[6] * rr[c]) + qq[c] * t[4] +
+                                               QrrQ[c] * (2.0 * t[5]));
+                        }
+                        for (int c = 0; c < 5; ++c) G[5 * qt + c] += g[c] * f;
+                    }
+                }
+                for (int c = 0; c < 3; ++c) E[3 * j + c] += e[c] * f;
             }
         }
     }
@@ -871,6 +1147,12 @@ static void jacobi(const Op& op, const cplx* self_coef, const cplx* src, cplx* d
         const double den = std::norm(sc);
         const cplx inv = den > 1e-30 ? cplx(sc.real() / den, -sc.imag() / den) : cplx(1.0, 0.0);
         for (int c = 0; c < 3; ++c) dst[3 * i + c] = src[3 * i + c] * inv;
+    }
+    for (size_t q = 0; q < op.qidx.size(); ++q) {  // numerical_solver.cu:223-233
+        const cplx sc = self_coef[op.qidx[q]];
+        const double den = std::norm(sc);
+        const cplx inv = den > 1e-30 ? cplx(sc.real() / den, -sc.imag() / den) : cplx(1.0, 0.0);
+        for (int c = 0; c < 5; ++c) dst[3 * (size_t)op.N + 5 * q + c] = src[3 * (size_t)op.N + 5 * q + c] * inv;
     }
 }
 static double vnorm(const std::vector<cplx>& a) {
@@ -900,7 +1182,7 @@ struct SolveReport {
 // gmres_solver.cu:56-208
 static void gmres(const Op& op, const cplx* sc, const cplx* b, cplx* x, double tol, int restart,
                   int maxiter, bool use_jacobi, SolveReport& rep) {
-    const size_t n = 3 * (size_t)op.N;
+    const size_t n = op.n_unknowns();
     std::vector<cplx> bb(b, b + n), r(n), w(n), t(n);
     rep = SolveReport{0, 0, 0, 0, 0};
     const double bnorm = vnorm(bb);
@@ -997,7 +1279,7 @@ static void gmres(const Op& op, const cplx* sc, const cplx* b, cplx* x, double t
 // bicgstab_solver.cu:58-192 (unconjugated rho and r0~.v; conjugated omega).
 static void bicgstab(const Op& op, const cplx* sc, const cplx* b, cplx* x, double tol, int maxiter,
                      bool use_jacobi, SolveReport& rep) {
-    const size_t n = 3 * (size_t)op.N;
+    const size_t n = op.n_unknowns();
     std::vector<cplx> bb(b, b + n), r(n), r0(n), p(n), v(n, 0.0), s(n), t(n), tmp(n);
     rep = SolveReport{0, 0, 0, 0, 0};
     const double bnorm = vnorm(bb);
@@ -1158,6 +1440,42 @@ void orc_poly_pair_functions(const double* r, int n, double a, double b, double 
 double orc_poly_self(double a, double b, double xi) { return poly_self(a, b, xi); }
 
 void orc_poly_weight(const double* gd, double a, double xi, double eta, double* w) { poly_weight(gd, a, xi, eta, w); }
+
+// Quadrupoles on the monodisperse operator: quad_idxs[nq] particle indices (all
+// particles when nq < 0).  Call after orc_op_set_positions; unknowns become 3N + 5Q.
+int orc_op_set_quads(void* h, const int* quad_idxs, int nq) {
+    Op& op = *static_cast<Op*>(h);
+    if (op.poly || op.N == 0) return -1;
+    op.quads = true;
+    op.qidx.clear();
+    if (nq < 0)
+        for (int i = 0; i < op.N; ++i) op.qidx.push_back(i);
+    else
+        op.qidx.assign(quad_idxs, quad_idxs + nq);
+    op.qmap.assign(op.N, -1);
+    for (size_t q = 0; q < op.qidx.size(); ++q) {
+        if (op.qidx[q] < 0 || op.qidx[q] >= op.N) return -1;
+        op.qmap[op.qidx[q]] = (int)q;
+    }
+    const size_t ts = op.T.r.size();
+    for (int t = 0; t < 7; ++t) op.qtab[t].assign(ts, 0.0);
+    op.self_grad = quad_self_grad(op.xi);
+    for (int t = 3; t < 7; ++t) op.qtab[t][0] = op.self_grad;
+    for (size_t e = 1; e < ts; ++e) {
+        double v[7];
+        quad_pair_functions(op.T.r[e], op.xi, v);
+        for (int t = 0; t < 7; ++t) op.qtab[t][e] = v[t];
+    }
+    compute_scale(op);
+    build_neighbors(op);
+    return (int)op.qidx.size();
+}
+
+void orc_quad_pair_functions(const double* r, int n, double xi, double* out) {
+    for (int i = 0; i < n; ++i) quad_pair_functions(r[i], xi, out + 7 * i);
+}
+
+double orc_quad_self_grad(double xi) { return quad_self_grad(xi); }
 
 void orc_op_destroy(void* h) { delete static_cast<Op*>(h); }
 

@@ -56,6 +56,10 @@ def lib():
         L.orc_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double,
                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_op_set_quads.argtypes = [ctypes.c_void_p, _ip, ctypes.c_int]
+        L.orc_quad_pair_functions.argtypes = [_dp, ctypes.c_int, ctypes.c_double, _dp]
+        L.orc_quad_self_grad.restype = ctypes.c_double
+        L.orc_quad_self_grad.argtypes = [ctypes.c_double]
         L.orc_poly_create.restype = ctypes.c_void_p
         L.orc_poly_create.argtypes = [_dp, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int, _ip,
                                       ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
@@ -122,6 +126,18 @@ def pair_functions(r, xi):
     return pe, pa
 
 
+def quad_pair_functions(r, xi):
+    """Quadrupole pair functions (field_quad_1..3, grad_quad_1..4) at distances r: (len(r), 7)."""
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.zeros((r.size, 7))
+    lib().orc_quad_pair_functions(_d(r), r.size, float(xi), _d(out))
+    return out
+
+
+def quad_self_grad(xi):
+    return lib().orc_quad_self_grad(float(xi))
+
+
 def poly_pair_functions(r, a, b, xi):
     """Polydisperse real-space pair functions (perp, para) of radii a, b at distances r."""
     r = np.ascontiguousarray(r, dtype=np.float64)
@@ -155,6 +171,24 @@ class Operator:
         if not self._h:
             raise RuntimeError(err.value.decode())
         self.N = 0
+        self.Q = 0
+
+    def set_quads(self, quad_idxs=None):
+        """Quadrupoles on the particles quad_idxs (all when None); after set_positions.
+        Unknowns then are [3N dipole components, 5Q quadrupole components]."""
+        if quad_idxs is None:
+            q = lib().orc_op_set_quads(self._h, None, -1)
+        else:
+            idx = np.ascontiguousarray(quad_idxs, np.int32)
+            q = lib().orc_op_set_quads(self._h, idx.ctypes.data_as(_ip), idx.size)
+        if q < 0:
+            raise ValueError("bad quadrupole indices")
+        self.Q = q
+        return q
+
+    @property
+    def n(self):
+        return 3 * self.N + 5 * self.Q
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
@@ -186,9 +220,10 @@ class Operator:
 
     def apply(self, p, self_coef, parts=3):
         """p: (N, 3) complex; self_coef: (N,) complex. parts: 1 real+self, 2 recip, 3 both."""
-        p = np.ascontiguousarray(p, dtype=np.complex128).reshape(self.N, 3)
+        shape = (self.N, 3) if self.Q == 0 else (self.n,)
+        p = np.ascontiguousarray(p, dtype=np.complex128).reshape(shape)
         sc = np.ascontiguousarray(np.broadcast_to(self_coef, (self.N,)), dtype=np.complex128)
-        E = np.zeros((self.N, 3), np.complex128)
+        E = np.zeros(shape, np.complex128)
         lib().orc_op_apply(self._h, _d(sc.view(np.float64)), _d(p.view(np.float64)),
                            _d(E.view(np.float64)), int(parts))
         return E
