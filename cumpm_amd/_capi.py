@@ -44,7 +44,7 @@ class OpInfo(ctypes.Structure):
                 ("rc", ctypes.c_double), ("self_perp", ctypes.c_double),
                 ("self_coef_ewald", ctypes.c_double), ("bloch", ctypes.c_double * 2),
                 ("slab_planes", ctypes.c_int), ("fft_backend", ctypes.c_int),
-                ("polydisperse", ctypes.c_int), ("num_unique_radii", ctypes.c_int)]
+                ("polydisperse", ctypes.c_int), ("num_unique_radii", ctypes.c_int), ("num_quads", _i64)]
 
 
 class SolveDesc(ctypes.Structure):
@@ -77,6 +77,7 @@ _SIGS = {
     "mpm_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "mpm_last_error": (ctypes.c_char_p, []),
     "mpm_op_create": (ctypes.c_int, [ctypes.POINTER(OpDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    "mpm_op_set_quads": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), _i64]),
     "mpm_op_create_poly": (ctypes.c_int, [ctypes.POINTER(OpDesc), _dp, _i64, ctypes.POINTER(ctypes.c_void_p)]),
     "mpm_op_destroy": (None, [ctypes.c_void_p]),
     "mpm_op_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OpInfo)]),

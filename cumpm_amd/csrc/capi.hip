@@ -376,6 +376,13 @@ mpm_status mpm_op_create_poly(const mpm_op_desc* desc, const double* radii, int6
 
 void mpm_op_destroy(mpm_op* op) { delete op; }
 
+mpm_status mpm_op_set_quads(mpm_op* op, const int* idx, int64_t nq) {
+    return guarded([&] {
+        MPM_REQUIRE(op, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        op->impl->set_quads(idx, nq);
+    });
+}
+
 mpm_status mpm_op_get_info(const mpm_op* op, mpm_op_info* info) {
     return guarded([&] {
         MPM_REQUIRE(op && info, MPM_ERR_INVALID_ARGUMENT, "null argument");

@@ -250,6 +250,177 @@ static void poly_weight(const double gd[3], double a, double xi, double eta, dou
     for (int q = 0; q < 3; ++q) w[q] = c * gd[q];
 }
 
+// ---- quadrupole pair functions (monodisperse_ewald_electric_field.cu:199-470) ----
+// Each is  A+ e^{-(r+2)^2 xi^2} + B+ erfc((r+2) xi) + A- e^{-(r-2)^2 xi^2}
+//        + B- erfc((r-2) xi) + A0 e^{-r^2 xi^2} + B0 erfc(r xi) + [r < 2] reg,
+// with A+- = a (O +- E) for the odd and even powers O, E of r and B+- = b (E' +- O').
+// Order: field_quad_1..3 (field of a quadrupole), grad_quad_1..4 (gradient couplings).
+struct Img {
+    double Ap, Am, A0, Bp, Bm, B0, reg;
+};
+static double img_sum(double r, double xi, const Img& c) {
+    const double x2 = xi * xi;
+    double f = c.Ap * std::exp(-(r + 2.0) * (r + 2.0) * x2) + c.Bp * std::erfc((r + 2.0) * xi) +
+               c.Am * std::exp(-(r - 2.0) * (r - 2.0) * x2) + c.Bm * std::erfc((r - 2.0) * xi) +
+               c.A0 * std::exp(-r * r * x2) + c.B0 * std::erfc(r * xi);
+    if (r < 2.0) f += c.reg;
+    return f;
+}
+
+static void quad_pair_functions(double r, double xi, double out[7]) {
+    const double x = xi, x2 = x * x, x4 = x2 * x2, x5 = x4 * x, x6 = x4 * x2, x7 = x6 * x, x8 = x4 * x4,
+                 x9 = x8 * x, x10 = x8 * x2;
+    const double r2 = r * r, r3 = r2 * r, r4 = r2 * r2, r5 = r4 * r, r6 = r4 * r2, r7 = r6 * r, r8 = r4 * r4,
+                 r9 = r8 * r, r10 = r8 * r2;
+    const double s15 = std::pow(kPi, 1.5), pi = kPi;
+    Img c;
+    {   // field_quad_1
+        const double a = 15.0 / (16384.0 * s15 * x7 * r4);
+        const double O = -24.0 * x6 * r7 - 4.0 * x4 * (9.0 - 8.0 * x2) * r5 + 2.0 * x2 * (21.0 - 80.0 * x2 + 64.0 * x4) * r3 -
+                         (45.0 - 120.0 * x2 + 192.0 * x4 - 512.0 * x6) * r;
+        const double E = 48.0 * x6 * r6 + 8.0 * x4 * (3.0 - 8.0 * x2) * r4 - 4.0 * x2 * std::pow(3.0 - 8.0 * x2, 2) * r2 -
+                         2.0 * (45.0 + 24.0 * x2 - 64.0 * x4 + 512.0 * x6);
+        const double b = 15.0 / (32768.0 * pi * x8 * r4);
+        const double Be = 48.0 * x8 * r8 + 32.0 * x6 * (3.0 - 8.0 * x2) * r6 - 24.0 * x4 * (3.0 - 16.0 * x2) * r4 +
+                          72.0 * x2 * (1.0 - 8.0 * x2) * r2 - 45.0 + 480.0 * x2 + 4096.0 * x8;
+        c.Ap = a * (O + E);
+        c.Am = a * (O - E);
+        c.A0 = 15.0 / (8192.0 * s15 * x7 * r3) *
+               (24.0 * x6 * r6 + 4.0 * x4 * (9.0 - 32.0 * x2) * r4 - 2.0 * x2 * (21.0 - 128.0 * x2) * r2 + 45.0 - 480.0 * x2);
+        c.Bp = c.Bm = b * Be;
+        c.B0 = 15.0 / (16384.0 * pi * x8 * r4) * (-(Be - 4096.0 * x8));
+        c.reg = -15.0 / (4.0 * pi * r4) + 15.0 * r2 / (64.0 * pi) * (1.0 - 3.0 * r2 / 16.0);
+        out[0] = img_sum(r, xi, c);
+    }
+    {   // field_quad_2
+        const double a = 3.0 / (16384.0 * s15 * x7 * r4);
+        const double O = -40.0 * x6 * r7 - 20.0 * x4 * (11.0 - 24.0 * x2) * r5 - 2.0 * x2 * (45.0 - 80.0 * x2 + 64.0 * x4) * r3 +
+                         (45.0 - 120.0 * x2 + 192.0 * x4 - 512.0 * x6) * r;
+        const double E = 80.0 * x6 * r6 + 8.0 * x4 * (45.0 + 8.0 * x2) * r4 - 4.0 * x2 * (15.0 + 48.0 * x2 - 64.0 * x4) * r2 +
+                         2.0 * (45.0 + 24.0 * x2 - 64.0 * x4 + 512.0 * x6);
+        const double b = 3.0 / (32768.0 * pi * x8 * r4);
+        const double Be = 80.0 * x8 * r8 + 160.0 * x6 * (3.0 - 8.0 * x2) * r6 + 120.0 * x4 * (3.0 - 16.0 * x2) * r4 -
+                          120.0 * x2 * (1.0 - 8.0 * x2) * r2 + 45.0 - 480.0 * x2 - 4096.0 * x8;
+        const double Bo = -2048.0 * x8 * r5;
+        c.Ap = a * (O + E);
+        c.Am = a * (O - E);
+        c.A0 = 15.0 / (8192.0 * s15 * x7 * r3) *
+               (8.0 * x6 * r6 + 4.0 * x4 * (11.0 - 32.0 * x2) * r4 + 2.0 * x2 * (9.0 - 64.0 * x2) * r2 - 9.0 + 96.0 * x2);
+        c.Bp = b * (Be + Bo);
+        c.Bm = b * (Be - Bo);
+        c.B0 = 15.0 / (16384.0 * pi * x8 * r4) *
+               (-16.0 * x8 * r8 - 32.0 * x6 * (3.0 - 8.0 * x2) * r6 - 24.0 * x4 * (3.0 - 16.0 * x2) * r4 +
+                24.0 * x2 * (1.0 - 8.0 * x2) * r2 - 9.0 + 96.0 * x2);
+        c.reg = 3.0 / (4.0 * pi * r4) - 3.0 * r / (8.0 * pi) * (1.0 - 5.0 * r / 8.0 + 5.0 * r3 / 128.0);
+        out[1] = img_sum(r, xi, c);
+    }
+    {   // field_quad_3
+        const double a = 5.0 / (1024.0 * s15 * x5 * r2);
+        const double O = 4.0 * x4 * r5 + 16.0 * x2 * (1.0 - 2.0 * x2) * r3 + 3.0 * r;
+        const double E = -8.0 * x4 * r4 - 24.0 * x2 * r2 + 6.0;
+        const double b = 5.0 / (2048.0 * pi * x6 * r2);
+        const double Be = -8.0 * x6 * r6 - 12.0 * x4 * (3.0 - 8.0 * x2) * r4 - 6.0 * x2 * (3.0 - 16.0 * x2) * r2 + 3.0 - 24.0 * x2;
+        const double Bo = 128.0 * x6 * r3;
+        c.Ap = a * (O + E);
+        c.Am = a * (O - E);
+        c.A0 = 5.0 / (512.0 * s15 * x5 * r) * (-4.0 * x4 * r4 - 16.0 * x2 * (1.0 - 3.0 * x2) * r2 - 3.0 + 24.0 * x2);
+        c.Bp = b * (Be + Bo);
+        c.Bm = b * (Be - Bo);
+        c.B0 = 5.0 / (1024.0 * pi * x6 * r2) * (-Be);
+        c.reg = 5.0 * r / (8.0 * pi) * (1.0 - 3.0 * r / 4.0 + r3 / 16.0);
+        out[2] = img_sum(r, xi, c);
+    }
+    {   // grad_quad_1
+        const double a = 75.0 / (16384.0 * s15 * x7 * r3);
+        const double O = 8.0 * x6 * r7 + (44.0 * x4 - 32.0 * x6) * r5 + (18.0 * x2 + 32.0 * x4) * r3 - (9.0 + 24.0 * x2) * r;
+        const double E = -16.0 * x6 * r6 - (72.0 * x4 - 64.0 * x6) * r4 + (12.0 * x2 - 64.0 * x4) * r2 - (18.0 - 48.0 * x2);
+        const double Be = 16.0 * x8 * r8 + (96.0 * x6 - 128.0 * x8) * r6 + (72.0 * x4 - 192.0 * x6 + 256.0 * x8) * r4 -
+                          (24.0 * x2 - 96.0 * x4 + 256.0 * x6) * r2 + 9.0 - 48.0 * x2 + 192.0 * x4;
+        c.Ap = a * (O + E);
+        c.Am = a * (O - E);
+        c.A0 = -75.0 / (8192.0 * s15 * x7 * r2) *
+               (8.0 * x6 * r6 + (44.0 * x4 - 64.0 * x6) * r4 + (18.0 * x2 - 64.0 * x4 + 128.0 * x6) * r2 - 9.0 + 48.0 * x2 -
+                192.0 * x4);
+        c.Bp = c.Bm = -75.0 / (32768.0 * pi * x8 * r3) * Be;
+        c.B0 = 75.0 / (16384.0 * pi * x8 * r3) * Be;
+        c.reg = 75.0 * r * (4.0 - r2) * (4.0 - r2) / (1024.0 * pi);
+        out[3] = img_sum(r, xi, c);
+    }
+    {   // grad_quad_2
+        const double a = 3.0 / (32768.0 * s15 * x9 * r5);
+        const double O = -48.0 * x8 * r9 - (576.0 * x6 - 608.0 * x8) * r7 - (1536.0 * x4 - 2576.0 * x6 + 3968.0 * x8) * r5 -
+                         (360.0 * x2 + 360.0 * x4 + 640.0 * x6 - 512.0 * x8) * r3 +
+                         (135.0 + 180.0 * x2 + 480.0 * x4 - 768.0 * x6 + 2048.0 * x8) * r;
+        const double E = 96.0 * x8 * r8 + (1056.0 * x6 - 1216.0 * x8) * r6 + (2160.0 * x4 - 4320.0 * x6 - 256.0 * x8) * r4 -
+                         (360.0 * x2 - 1680.0 * x4 - 768.0 * x6 + 1024.0 * x8) * r2 +
+                         (270.0 - 1080.0 * x2 - 192.0 * x4 + 512.0 * x6 - 4096.0 * x8);
+        const double P = 288.0 * x10 * r10 + (3600.0 * x8 - 4800.0 * x10) * r8 +
+                         (10800.0 * x6 - 28800.0 * x8 + 38400.0 * x10) * r6 + (5400.0 * x4 - 21600.0 * x6 + 57600.0 * x8) * r4 -
+                         (1350.0 * x2 - 7200.0 * x4 + 28800.0 * x6) * r2 + 405.0 - 2700.0 * x2 + 14400.0 * x4;
+        const double b = 1.0 / (65536.0 * pi * x10 * r5);
+        c.Ap = a * (O + E);
+        c.Am = a * (O - E);
+        c.A0 = 1.0 / (16384.0 * s15 * x9 * r4) *
+               (144.0 * x8 * r8 + (1728.0 * x6 - 2400.0 * x8) * r6 + (4608.0 * x4 - 13200.0 * x6 + 19200.0 * x8) * r4 +
+                (1080.0 * x2 - 5400.0 * x4 + 19200.0 * x6) * r2 - 405.0 + 2700.0 * x2 - 14400.0 * x4);
+        c.Bp = b * (P + 49152.0 * x10 + 49152.0 * x10 * r5);
+        c.Bm = b * (P + 49152.0 * x10 - 49152.0 * x10 * r5);
+        c.B0 = -1.0 / (32768.0 * pi * x10 * r5) * P;
+        c.reg = -3.0 / (2.0 * pi) * (1.0 / r5 - 1.0 + 25.0 * r / 32.0 - 25.0 * r3 / 256.0 + 3.0 * r5 / 512.0);
+        out[4] = img_sum(r, xi, c);
+    }
+    {   // grad_quad_3
+        const double a = -15.0 / (65536.0 * s15 * x9 * r5);
+        const double O = 48.0 * x8 * r9 + (336.0 * x6 - 288.0 * x8) * r7 + (216.0 * x4 + 144.0 * x6 + 128.0 * x8) * r5 -
+                         (180.0 * x2 - 120.0 * x4 + 640.0 * x6 - 512.0 * x8) * r3 +
+                         (135.0 + 180.0 * x2 + 480.0 * x4 - 768.0 * x6 + 2048.0 * x8) * r;
+        const double E = -96.0 * x8 * r8 - 576.0 * (x6 - x8) * r6 - (480.0 * x6 + 256.0 * x8) * r4 +
+                         (720.0 * x4 + 768.0 * x6 - 1024.0 * x8) * r2 -
+                         (-270.0 + 1080.0 * x2 + 192.0 * x4 - 512.0 * x6 + 4096.0 * x8);
+        const double P = -96.0 * x10 * r10 + (-720.0 * x8 + 960.0 * x10) * r8 -
+                         (720.0 * x6 - 1920.0 * x8 + 2560.0 * x10) * r6 + (360.0 * x4 - 1440.0 * x6 + 3840.0 * x8) * r4 -
+                         (270.0 * x2 - 1440.0 * x4 + 5760.0 * x6) * r2 + 135.0 - 900.0 * x2 + 4800.0 * x4;
+        c.Ap = a * (O + E);
+        c.Am = a * (O - E);
+        c.A0 = 15.0 / (32768.0 * s15 * x9 * r4) *
+               (48.0 * x8 * r8 + (336.0 * x6 - 480.0 * x8) * r6 + (216.0 * x4 - 720.0 * x6 + 1280.0 * x8) * r4 -
+                (180.0 * x2 - 840.0 * x4 + 2560.0 * x6) * r2 + 135.0 - 900.0 * x2 + 4800.0 * x4);
+        c.Bp = c.Bm = -15.0 / (131072.0 * pi * x10 * r5) * (P + 16384.0 * x10);
+        c.B0 = -15.0 / (65536.0 * pi * x10 * r5) * (-P);
+        c.reg = -15.0 * std::pow(r2 - 4.0, 3) * (3.0 * r4 + 6.0 * r2 + 8.0) / (2048.0 * pi * r5);
+        out[5] = img_sum(r, xi, c);
+    }
+    {   // grad_quad_4
+        const double a = -15.0 / (65536.0 * s15 * x9 * r5);
+        const double O = 144.0 * x8 * r9 + (288.0 * x6 + 96.0 * x8) * r7 - (432.0 * x4 - 912.0 * x6 + 896.0 * x8) * r5 +
+                         (720.0 * x2 - 2280.0 * x4 + 4480.0 * x6 - 3584.0 * x8) * r3 -
+                         (945.0 + 1260.0 * x2 + 3360.0 * x4 - 5376.0 * x6 + 14336.0 * x8) * r;
+        const double E = -288.0 * x8 * r8 - (288.0 * x6 + 192.0 * x8) * r6 + (720.0 * x4 - 480.0 * x6 + 1792.0 * x8) * r4 -
+                         (1080.0 * x2 + 2160.0 * x4 + 5376.0 * x6 - 7168.0 * x8) * r2 -
+                         (1890.0 - 7560.0 * x2 - 1344.0 * x4 + 3584.0 * x6 - 28672.0 * x8);
+        const double P = 288.0 * x10 * r10 + (720.0 * x8 - 960.0 * x10) * r8 - (720.0 * x6 - 1920.0 * x8 + 2560.0 * x10) * r6 +
+                         (1080.0 * x4 - 4320.0 * x6 + 11520.0 * x8) * r4 - (1350.0 * x2 - 7200.0 * x4 + 28800.0 * x6) * r2 +
+                         945.0 - 6300.0 * x2 + 33600.0 * x4;
+        c.Ap = a * (O + E);
+        c.Am = a * (O - E);
+        c.A0 = 15.0 / (32768.0 * s15 * x9 * r4) *
+               (144.0 * x8 * r8 + (288.0 * x6 - 480.0 * x8) * r6 - (432.0 * x4 - 1200.0 * x6 + 1280.0 * x8) * r4 +
+                (720.0 * x2 - 3000.0 * x4 + 6400.0 * x6) * r2 - 945.0 + 6300.0 * x2 - 33600.0 * x4);
+        c.Bp = c.Bm = 15.0 / (131072.0 * pi * x10 * r5) * (P + 114688.0 * x10);
+        c.B0 = -15.0 / (65536.0 * pi * x10 * r5) * P;
+        c.reg = -15.0 * (3584.0 - 80.0 * r6 - 30.0 * r8 + 9.0 * r10) / (2048.0 * pi * r5);
+        out[6] = img_sum(r, xi, c);
+    }
+}
+
+// Gradient self term, the quadrupole tables' entry 0 (:465-466).
+static double quad_self_grad(double xi) {
+    const double x2 = xi * xi, x4 = x2 * x2, x5 = x4 * xi, sp = std::sqrt(kPi);
+    return 0.5 / kPi *
+           (3.0 * (3.0 - 10.0 * x2 + 20.0 * x4) / (8.0 * sp * x5) -
+            3.0 * (3.0 + 2.0 * x2 + 4.0 * x4) * std::exp(-4.0 * x2) / (8.0 * sp * x5) + 3.0 * std::erfc(2.0 * xi));
+}
+
 // The reference samples the pair functions on r = 1 + 0.001 j (j < 9000) and
 // interpolates linearly, clamping at r >= 9.999 (:92-97, :668-685).  The pair
 // values are therefore computed here the same way, from the same samples.
@@ -266,6 +437,22 @@ struct PairTable {
         for (int j = 0; j < n; ++j) {
             r[j + 1] = 1.0 + j * 0.001;
             dipole_pair_functions(r[j + 1], xi, perp[j + 1], para[j + 1]);
+        }
+    }
+    // quadrupole tables on the same samples: field_quad_1..3 (entry 0 = 0) and
+    // grad_quad_1..4 (entry 0 = the gradient self term); built on first use
+    std::vector<double> q[7];
+    double self_grad = 0.0;
+    void build_quads(double xi) {
+        if (!q[0].empty()) return;
+        const size_t ts = r.size();
+        for (auto& t : q) t.assign(ts, 0.0);
+        self_grad = quad_self_grad(xi);
+        for (int t = 3; t < 7; ++t) q[t][0] = self_grad;
+        for (size_t e = 1; e < ts; ++e) {
+            double v[7];
+            quad_pair_functions(r[e], xi, v);
+            for (int t = 0; t < 7; ++t) q[t][e] = v[t];
         }
     }
     double interp(const std::vector<double>& y, double x) const {
@@ -2854,6 +3041,332 @@ __global__ __launch_bounds__(256) void poly_gather_kernel(SlotTable tab, int B, 
 }
 
 // ---------------------------------------------------------------------------
+// Quadrupole kernels (monodisperse, full grid [B][8][nx*ny*nz]: the field's 3
+// components, then the 5 quadrupole-gradient components)
+// ---------------------------------------------------------------------------
+
+// Per voxel: khat (3), sc, scQ, scGP, scGQ, Qfactor (5), Qdot (5) (:1301-1404).
+__global__ void quad_kspace_kernel(double* __restrict__ k64, float* __restrict__ k32, KParams q) {
+    const size_t V = (size_t)q.nx * q.ny * q.nz;
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < V; v += (size_t)gridDim.x * blockDim.x) {
+        const int ix = (int)(v / ((size_t)q.ny * q.nz)), iy = (int)((v / q.nz) % q.ny), iz = (int)(v % q.nz);
+        const double fx = ix <= (q.nx - 1) / 2 ? ix : ix - q.nx;
+        const double fy = iy <= (q.ny - 1) / 2 ? iy : iy - q.ny;
+        const double fz = iz <= (q.nz - 1) / 2 ? iz : iz - q.nz;
+        const double Gx = fx * 2.0 * kPi / q.Lx, Gy = fy * 2.0 * kPi / q.Ly, Gz = fz * 2.0 * kPi / q.Lz;
+        const double kx = Gx - q.kbx, ky = Gy - q.kby, kz = Gz;
+        const double k2 = kx * kx + ky * ky + kz * kz, k = sqrt(k2);
+        double o[16];
+        for (int c = 0; c < 16; ++c) o[c] = 0.0;
+        if (k >= 1e-12) {
+            const double h0 = kx / k, h1 = ky / k, h2 = kz / k;
+            const double etak2 = k2 - (Gx * Gx * q.etax + Gy * Gy * q.etay + Gz * Gz * q.etaz);
+            double sk, ck;
+            sincos(k, &sk, &ck);
+            const double t = sk / k - ck;
+            const double ex = exp(-etak2 / (4.0 * q.xi * q.xi));
+            const double j1 = t / k, j2 = (3.0 / k2 - 1.0) * sk / k - 3.0 * ck / k2;
+            const double e2 = ex / k2 * q.invV;
+            o[0] = h0;
+            o[1] = h1;
+            o[2] = h2;
+            o[3] = 9.0 * t * t * ex / (k2 * k2) * q.invV;
+            o[4] = -22.5 * j1 * j2 * e2;
+            o[5] = 45.0 * j1 * j2 * e2;
+            o[6] = 112.5 * j2 * j2 * e2;
+            o[7] = h0 * h0 - 1.0 / 3.0;
+            o[8] = h0 * h1;
+            o[9] = h0 * h2;
+            o[10] = h1 * h1 - 1.0 / 3.0;
+            o[11] = h1 * h2;
+            o[12] = h0 * h0 - h2 * h2;
+            o[13] = 2.0 * h0 * h1;
+            o[14] = 2.0 * h0 * h2;
+            o[15] = h1 * h1 - h2 * h2;
+        }
+        // Qdot[4] = 2 h1 h2 does not fit the 16 slots: the scale kernel forms it from khat
+        for (int c = 0; c < 16; ++c) {
+            k64[16 * v + c] = o[c];
+            k32[16 * v + c] = (float)o[c];
+        }
+    }
+}
+
+// Owner-computes spread of dipoles (field grid) and quadrupoles (gradient grid)
+// with the Gaussian weights: a thread per voxel and G right-hand sides.
+template <typename R, int G>
+__global__ __launch_bounds__(256) void quad_spread_kernel(SlotTable tab, int B, int N, int V, int SS,
+                                                          const int* __restrict__ vox_start,
+                                                          const int* __restrict__ vox_items,
+                                                          const R* __restrict__ sw, const int* __restrict__ qmap,
+                                                          typename C2<R>::T* __restrict__ out) {
+    using CT = typename C2<R>::T;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const int g0 = blockIdx.y * G;
+    R ar[G][8], ai[G][8];
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) ar[j][c] = ai[j][c] = R(0);
+    const int e0 = vox_start[v], e1 = vox_start[v + 1];
+    for (int e = e0; e < e1; ++e) {
+        const int item = vox_items[e];
+        const int pid = item / SS;
+        const R w = sw[item];
+        const int q = qmap[pid];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            if (g0 + j < B) {
+                const double2* x = tab.x[g0 + j];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double2 pv = x[3 * (size_t)pid + c];
+                    ar[j][c] += w * (R)pv.x;
+                    ai[j][c] += w * (R)pv.y;
+                }
+                if (q >= 0) {
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) {
+                        const double2 qv = x[3 * (size_t)N + 5 * (size_t)q + c];
+                        ar[j][3 + c] += w * (R)qv.x;
+                        ai[j][3 + c] += w * (R)qv.y;
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+        if (g0 + j < B) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                CT o;
+                o.x = ar[j][c];
+                o.y = ai[j][c];
+                out[((size_t)(g0 + j) * 8 + c) * V + v] = o;
+            }
+        }
+    }
+}
+
+// Joint k-space coupling of field and gradient grids (scale_kernel_joint,
+// :1435-1487): Edk = E.khat, GQ = G.Qdot; E <- (sc Edk + i scQ GQ) khat,
+// G <- (i scGP Edk + scGQ GQ) Qfactor.
+template <typename R>
+__global__ __launch_bounds__(256) void quad_scale_kernel(typename C2<R>::T* __restrict__ grid,
+                                                         const R* __restrict__ kq, size_t V, size_t total) {
+    using CT = typename C2<R>::T;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t b = e / V, v = e - b * V;
+        CT* g = grid + b * 8 * V + v;
+        const R* k = kq + 16 * v;
+        const R h0 = k[0], h1 = k[1], h2 = k[2];
+        const R qd[5] = {k[12], k[13], k[14], k[15], R(2) * h1 * h2};
+        CT f[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) f[c] = g[c * V];
+        const R er = f[0].x * h0 + f[1].x * h1 + f[2].x * h2, ei = f[0].y * h0 + f[1].y * h1 + f[2].y * h2;
+        R gr = R(0), gi = R(0);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            gr += f[3 + c].x * qd[c];
+            gi += f[3 + c].y * qd[c];
+        }
+        // i a (x + i y) = -a y + i a x
+        const R sr = k[3] * er - k[4] * gi, si = k[3] * ei + k[4] * gr;
+        const R tr = -k[5] * ei + k[6] * gr, ti = k[5] * er + k[6] * gi;
+        const R hh[3] = {h0, h1, h2};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            CT o;
+            o.x = sr * hh[c];
+            o.y = si * hh[c];
+            g[c * V] = o;
+        }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            CT o;
+            o.x = tr * k[7 + c];
+            o.y = ti * k[7 + c];
+            g[(3 + c) * V] = o;
+        }
+    }
+}
+
+// A wave per (particle i, group of G right-hand sides): reciprocal field and,
+// for a quadrupole particle, gradient (h^3 sum of the Gaussian weights times the
+// grids), the real-space couplings of every neighbour's dipole and quadrupole
+// (dipole -> field, quadrupole -> field, dipole -> gradient, quadrupole ->
+// gradient, :1631-1869 with the cross-term signs of tests/quadrupole_field.py:214-215),
+// the self terms (sc + self_perp for the field, 2.5 sc + self_grad / 2 for the
+// gradient) and the optional left Jacobi scaling; FP64 sums, one reduction.
+// Pair data [16]: u (3), perp, para - perp, cos, sin, field_quad_1..3, grad_quad_1..4.
+template <typename R, int G>
+__global__ __launch_bounds__(256) void quad_gather_kernel(SlotTable tab, int B, int parts, int N, int V, int SS,
+                                                          const typename C2<R>::T* __restrict__ grid,
+                                                          const R* __restrict__ sw, const int* __restrict__ pidx,
+                                                          const int* __restrict__ qmap, double h3,
+                                                          const int* __restrict__ nb_start,
+                                                          const int* __restrict__ nb_j,
+                                                          const double* __restrict__ pairq,
+                                                          const double2* __restrict__ selfc, double self_perp,
+                                                          double self_grad, int jacobi) {
+    using CT = typename C2<R>::T;
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= N) return;  // whole wave
+    const int g0 = blockIdx.y * G;
+    const int qi = qmap[i];
+    const int nc = qi >= 0 ? 8 : 3;
+    double acc[G][8][2];
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[j][c][0] = acc[j][c][1] = 0.0;
+    if (parts & MPM_PART_RECIP) {
+        for (int o = lane; o < SS; o += 64) {
+            const size_t item = (size_t)i * SS + o;
+            const int v = pidx[item];
+            const double w = (double)sw[item] * h3;
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                if (g0 + j < B) {
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        if (c < nc) {
+                            const CT gv = grid[((size_t)(g0 + j) * 8 + c) * V + v];
+                            acc[j][c][0] += w * (double)gv.x;
+                            acc[j][c][1] += w * (double)gv.y;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (parts & MPM_PART_REAL) {
+        const int k0 = nb_start[i], k1 = nb_start[i + 1];
+        const double I5[5] = {1.0, 0.0, 0.0, 1.0, 0.0};
+        for (int k = k0 + lane; k < k1; k += 64) {
+            const int jn = nb_j[k];
+            const int qs = qmap[jn];
+            const double* pc = pairq + 16 * (size_t)k;
+            const double u[3] = {pc[0], pc[1], pc[2]};
+            const double cp = pc[3], cd = pc[4], ec = pc[5], es = pc[6];
+            const double t0 = pc[7], t1 = pc[8], t2 = pc[9], g1 = pc[10], g2 = pc[11], g3 = pc[12], g4 = pc[13];
+            const double rr[5] = {u[0] * u[0], u[0] * u[1], u[0] * u[2], u[1] * u[1], u[1] * u[2]};
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                if (g0 + j < B) {
+                    const double2* x = tab.x[g0 + j];
+                    const double2 pp[3] = {x[3 * (size_t)jn], x[3 * (size_t)jn + 1], x[3 * (size_t)jn + 2]};
+                    double f[8][2];
+                    const double rpr = pp[0].x * u[0] + pp[1].x * u[1] + pp[2].x * u[2];
+                    const double rpi = pp[0].y * u[0] + pp[1].y * u[1] + pp[2].y * u[2];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        f[c][0] = cp * pp[c].x + cd * rpr * u[c];
+                        f[c][1] = cp * pp[c].y + cd * rpi * u[c];
+                    }
+#pragma unroll
+                    for (int c = 3; c < 8; ++c) f[c][0] = f[c][1] = 0.0;
+                    double Qr[3][2] = {{0, 0}, {0, 0}, {0, 0}}, Qrr[2] = {0, 0};
+                    double2 qq[5];
+                    if (qs >= 0) {
+#pragma unroll
+                        for (int c = 0; c < 5; ++c) qq[c] = x[3 * (size_t)N + 5 * (size_t)qs + c];
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const double q0 = h ? qq[0].y : qq[0].x, q1 = h ? qq[1].y : qq[1].x,
+                                         q2 = h ? qq[2].y : qq[2].x, q3 = h ? qq[3].y : qq[3].x,
+                                         q4 = h ? qq[4].y : qq[4].x;
+                            Qr[0][h] = q0 * u[0] + q1 * u[1] + q2 * u[2];
+                            Qr[1][h] = q1 * u[0] + q3 * u[1] + q4 * u[2];
+                            Qr[2][h] = q2 * u[0] + q4 * u[1] - (q0 + q3) * u[2];
+                            Qrr[h] = Qr[0][h] * u[0] + Qr[1][h] * u[1] + Qr[2][h] * u[2];
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) f[c][h] += 0.5 * (Qrr[h] * t0 * u[c] + 2.0 * t1 * Qr[c][h]);
+                        }
+                    }
+                    if (qi >= 0) {
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const double rp = h ? rpi : rpr;
+                            const double p0 = h ? pp[0].y : pp[0].x, p1 = h ? pp[1].y : pp[1].x,
+                                         p2 = h ? pp[2].y : pp[2].x;
+                            const double PrrP[5] = {2.0 * p0 * u[0], p0 * u[1] + p1 * u[0], p0 * u[2] + p2 * u[0],
+                                                    2.0 * p1 * u[1], p1 * u[2] + p2 * u[1]};
+#pragma unroll
+                            for (int c = 0; c < 5; ++c)
+                                f[3 + c][h] = -(rp * (t0 * rr[c] + (t1 + t2) * I5[c]) + PrrP[c] * t1);
+                            if (qs >= 0) {
+                                const double qv[5] = {h ? qq[0].y : qq[0].x, h ? qq[1].y : qq[1].x, h ? qq[2].y : qq[2].x,
+                                                      h ? qq[3].y : qq[3].x, h ? qq[4].y : qq[4].x};
+                                const double QrrQ[5] = {2.0 * Qr[0][h] * u[0], Qr[0][h] * u[1] + Qr[1][h] * u[0],
+                                                        Qr[0][h] * u[2] + Qr[2][h] * u[0], 2.0 * Qr[1][h] * u[1],
+                                                        Qr[1][h] * u[2] + Qr[2][h] * u[1]};
+#pragma unroll
+                                for (int c = 0; c < 5; ++c)
+                                    f[3 + c][h] += 0.5 * (Qrr[h] * ((g1 + g3) * I5[c] + g4 * rr[c]) + qv[c] * g2 +
+                                                          2.0 * g3 * QrrQ[c]);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        acc[j][c][0] += f[c][0] * ec - f[c][1] * es;
+                        acc[j][c][1] += f[c][0] * es + f[c][1] * ec;
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            acc[j][c][0] = wave_sum(acc[j][c][0]);
+            acc[j][c][1] = wave_sum(acc[j][c][1]);
+        }
+    if (lane >= G * 8) return;
+    const int j = lane / 8, c = lane % 8;  // lane (j, c) stores component c of right-hand side g0 + j
+    if (g0 + j >= B || c >= nc) return;
+    double yr = 0.0, yi = 0.0;
+    const double2* xb = nullptr;
+    double2* yb = nullptr;
+    int row = 0;
+#pragma unroll
+    for (int jj = 0; jj < G; ++jj) {  // uniform indices into the argument, selected per lane
+        if (jj == j) {
+            xb = tab.x[g0 + jj];
+            yb = tab.y[g0 + jj];
+            row = tab.row[g0 + jj];
+        }
+#pragma unroll
+        for (int cc = 0; cc < 8; ++cc)
+            if (jj == j && cc == c) {
+                yr = acc[jj][cc][0];
+                yi = acc[jj][cc][1];
+            }
+    }
+    const size_t e = c < 3 ? 3 * (size_t)i + c : 3 * (size_t)N + 5 * (size_t)qi + (c - 3);
+    const double2 xv = xb[e];
+    const double2 sc = selfc[(size_t)row * N + i];
+    if (parts & MPM_PART_REAL) {
+        const double fr = c < 3 ? sc.x + self_perp : 2.5 * sc.x + 0.5 * self_grad;
+        const double fi = c < 3 ? sc.y : 2.5 * sc.y;
+        yr += fr * xv.x - fi * xv.y;
+        yi += fr * xv.y + fi * xv.x;
+    }
+    double2 inv = make_double2(1.0, 0.0);
+    if (jacobi) {
+        const double den = sc.x * sc.x + sc.y * sc.y;
+        if (den > 1e-30) inv = make_double2(sc.x / den, -sc.y / den);
+    }
+    yb[e] = make_double2(yr * inv.x - yi * inv.y, yr * inv.y + yi * inv.x);
+}
+
+// ---------------------------------------------------------------------------
 // EwaldOp
 // ---------------------------------------------------------------------------
 EwaldOp::EwaldOp(const mpm_op_desc& d) {
@@ -2996,6 +3509,86 @@ void EwaldOp::setup_poly_stencils(const std::vector<double>& pos, int Nn) {
     d_selfp.upload(sp.data(), sp.size());
 }
 
+void EwaldOp::set_quads(const int* idx, int64_t nq) {
+    MPM_REQUIRE(!poly, MPM_ERR_UNSUPPORTED, "quadrupoles on the polydisperse operator are not provided by this build");
+    MPM_REQUIRE(N == 0, MPM_ERR_RUNTIME, "set quadrupoles before the particle positions");
+    MPM_REQUIRE(nq < 0 || idx || nq == 0, MPM_ERR_INVALID_ARGUMENT, "null quadrupole indices");
+    quads = true;
+    quads_all = nq < 0;
+    qreq_h.assign(idx && nq > 0 ? idx : nullptr, idx && nq > 0 ? idx + nq : nullptr);
+    force_full = 1;  // the slab kernels carry dipoles only
+    table->build_quads(xi);
+    build_kspace();
+}
+
+// Per-frame stencils with quadrupoles: the W^3 points of every particle, their
+// wrapped voxels and Gaussian weights C exp(-2 xi^2 sum gd^2/eta) (the dipole
+// weights, monodisperse...cu:687-816), and the voxel -> entry lists.
+void EwaldOp::setup_quad_stencils(const std::vector<double>& pos, int Nn) {
+    const int W = S.width, half = S.half;
+    const int SS = W * W * W;
+    MPM_REQUIRE((int64_t)Nn * SS < (1LL << 31), MPM_ERR_UNSUPPORTED, "too many stencil points");
+    const double prod_eta = S.eta[0] * S.eta[1] * S.eta[2];
+    const double Cs = std::pow(2.0 * xi * xi / kPi, 1.5) * std::sqrt(1.0 / prod_eta);
+    std::vector<double> sw((size_t)Nn * SS);
+    std::vector<int> pidx((size_t)Nn * SS), cnt(V + 1, 0);
+    for (int i = 0; i < Nn; ++i) {
+        int g[3];
+        double dd[3];
+        for (int d = 0; d < 3; ++d) {
+            const double p = pos[(size_t)d * Nn + i];
+            g[d] = (int)std::round(p / S.h[d]);
+            dd[d] = g[d] * S.h[d] - p;
+        }
+        int o = 0;
+        for (int ox = -half; ox <= half; ++ox)
+            for (int oy = -half; oy <= half; ++oy)
+                for (int oz = -half; oz <= half; ++oz, ++o) {
+                    const int off[3] = {ox, oy, oz};
+                    double a = 0.0;
+                    int w[3];
+                    for (int d = 0; d < 3; ++d) {
+                        w[d] = ((g[d] + off[d]) % S.n[d] + S.n[d]) % S.n[d];
+                        const double gd = dd[d] + off[d] * S.h[d];
+                        a += gd * gd / S.eta[d];
+                    }
+                    const size_t item = (size_t)i * SS + o;
+                    const int v = (w[0] * S.n[1] + w[1]) * S.n[2] + w[2];
+                    pidx[item] = v;
+                    sw[item] = Cs * std::exp(-2.0 * xi * xi * a);
+                    cnt[v + 1]++;
+                }
+    }
+    for (size_t v = 0; v < V; ++v) cnt[v + 1] += cnt[v];
+    std::vector<int> items((size_t)Nn * SS);
+    {
+        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+        for (size_t item = 0; item < (size_t)Nn * SS; ++item) items[fill[pidx[item]]++] = (int)item;
+    }
+    std::vector<float> sw32(sw.begin(), sw.end());
+    d_sw64.upload(sw.data(), sw.size());
+    d_sw32.upload(sw32.data(), sw32.size());
+    d_pidx.upload(pidx.data(), pidx.size());
+    d_vox_start.upload(cnt.data(), cnt.size());
+    d_vox_items.upload(items.data(), items.size());
+    // quadrupole slots
+    qidx_h.clear();
+    if (quads_all)
+        for (int i = 0; i < Nn; ++i) qidx_h.push_back(i);
+    else
+        qidx_h = qreq_h;
+    std::vector<int> qmap(Nn, -1);
+    for (size_t q = 0; q < qidx_h.size(); ++q) {
+        MPM_REQUIRE(qidx_h[q] >= 0 && qidx_h[q] < Nn, MPM_ERR_INVALID_ARGUMENT, "quadrupole index out of range");
+        MPM_REQUIRE(qmap[qidx_h[q]] < 0, MPM_ERR_INVALID_ARGUMENT, "repeated quadrupole index");
+        qmap[qidx_h[q]] = (int)q;
+    }
+    std::vector<int> qi = qidx_h;
+    if (qi.empty()) qi.push_back(0);
+    d_qidx.upload(qi.data(), qi.size());
+    d_qmap.upload(qmap.data(), qmap.size());
+}
+
 EwaldOp::~EwaldOp() = default;
 
 KParams EwaldOp::kparams() const {
@@ -3024,10 +3617,15 @@ void EwaldOp::build_kspace() {
 }
 
 void EwaldOp::ensure_kspace_table() {
-    const int key = poly ? 5000 : (!slab ? 0 : (zcontig ? 2 + Zs : 1) + (kt ? 1000 : 0));
+    const int key = quads ? 6000 : (poly ? 5000 : (!slab ? 0 : (zcontig ? 2 + Zs : 1) + (kt ? 1000 : 0)));
     if (kspace_ready && kspace_key == key) return;
     const int threads = 256;
-    if (poly) {
+    if (quads) {
+        d_kq64.resize(16 * V);
+        d_kq32.resize(16 * V);
+        const int blocks = (int)std::min<size_t>((V + threads - 1) / threads, 65535);
+        hipLaunchKernelGGL(quad_kspace_kernel, dim3(blocks), dim3(threads), 0, nullptr, d_kq64.p, d_kq32.p, kparams());
+    } else if (poly) {
         d_k64.resize(V);
         d_k32.resize(V);
         const int blocks = (int)std::min<size_t>((V + threads - 1) / threads, 65535);
@@ -3080,14 +3678,17 @@ void EwaldOp::set_bloch(double kx, double ky) {
 
 void EwaldOp::build_pair_coefs() {
     const size_t M = h_pair_f.size();
-    std::vector<double> c(std::max<size_t>(M, 1) * 8, 0.0);
+    const int stride = (quads && h_pair_q.size() == 7 * M) ? 16 : 8;  // quadrupoles: + 7 table values
+    std::vector<double> c(std::max<size_t>(M, 1) * stride, 0.0);
     const bool bloch = kb[0] != 0.0 || kb[1] != 0.0;
     for (size_t k = 0; k < M; ++k) {
         const double* r = &h_pair_r[3 * k];
         const double d = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
         const double dmin = poly ? 1e-6 : 1.0;  // d_eff of the unit vector
         const double de = d < dmin ? dmin : d;
-        double* o = &c[8 * k];
+        double* o = &c[stride * k];
+        if (stride == 16)
+            for (int t = 0; t < 7; ++t) o[7 + t] = h_pair_q[7 * k + t];
         o[0] = r[0] / de;
         o[1] = r[1] / de;
         o[2] = r[2] / de;
@@ -3309,7 +3910,7 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     }
     std::vector<int> nstart(Nn + 1, 0), nj;
     std::vector<double2> nf;
-    std::vector<double> pr;
+    std::vector<double> pr, pq;
     nj.reserve((size_t)Nn * 64);
     nf.reserve((size_t)Nn * 64);
     pr.reserve((size_t)Nn * 64 * 3);
@@ -3328,6 +3929,8 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         } else {
             const double de = dist < 1.0 ? 1.0 : dist;
             nf.push_back(make_double2(tab.interp(tab.perp, de), tab.interp(tab.para, de)));
+            if (quads)
+                for (int t = 0; t < 7; ++t) pq.push_back(tab.interp(tab.q[t], de));
         }
         // x_j - x_i, the orientation the gather uses
         double rj[3] = {pos[j] - pos[i], pos[Nn + j] - pos[Nn + i], pos[2 * Nn + j] - pos[2 * Nn + i]};
@@ -3369,6 +3972,7 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         nf.push_back(make_double2(0, 0));
     }
     h_pair_r = std::move(pr);
+    h_pair_q = std::move(pq);
     h_pair_f = nf;
     h_pair_f.resize(num_pairs);
     d_nb_j.upload(nj.data(), nj.size());
@@ -3423,7 +4027,7 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
                 pass(lf->inv, a, g, g, 0, 0);
             }
         } else {  // [comps][nx][ny][nz]: z, y, x lines in place (one component for the polydisperse grid)
-            const long long nc = poly ? 1 : 3;
+            const long long nc = poly ? 1 : (quads ? 8 : 3);
             lf->rhs_elems = nc * nxy * nzz;
             const LineGeom gz{nc * nxy, 1, 0, nzz, 1};
             const LineGeom gy{nc * nx * nzz, nzz, 1, (long long)ny * nzz, nzz};
@@ -3447,12 +4051,18 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
             fft = std::make_unique<FftPlans>(std::vector<size_t>{(size_t)S.n[1], (size_t)S.n[0]}, 3 * Zs);
         else
             fft = std::make_unique<FftPlans>(std::vector<size_t>{(size_t)S.n[2], (size_t)S.n[1], (size_t)S.n[0]},
-                                             poly ? 1 : 3);
+                                             poly ? 1 : (quads ? 8 : 3));
         fft_key = key;
     }
     if (poly) {
         MPM_REQUIRE(lfft != nullptr, MPM_ERR_UNSUPPORTED, "grid length not supported by the built-in FFT");
         setup_poly_stencils(pos, Nn);
+    }
+    if (quads) {
+        MPM_REQUIRE(lfft != nullptr && fft_backend == 0, MPM_ERR_UNSUPPORTED,
+                    "quadrupoles run on the built-in FFT only");
+        setup_quad_stencils(pos, Nn);
+        build_pair_coefs();  // with the quadrupole table values
     }
     // default self coefficients: zero (reference base_electric_field.cu allocates them zeroed)
     if (self_rows == 0 || d_selfc.n != (size_t)self_rows * Nn) {
@@ -3471,6 +4081,7 @@ void EwaldOp::set_self_coef(const double* coef, int rows) {
 
 size_t EwaldOp::grid_elems_per_rhs() const {
     if (poly) return V;
+    if (quads) return 8 * V;
     return slab ? 3 * (size_t)Zs * S.n[0] * nyp : 3 * V;
 }
 size_t EwaldOp::kgrid_elems_per_rhs() const { return 3 * (size_t)Zs * kspace_cols(); }
@@ -3735,6 +4346,50 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
 static_assert(kMaxSlots % 4 == 0, "the polydisperse kernels take right-hand sides in groups of 4");
 
 template <typename R>
+void EwaldOp::launch_quads(const SlotTable& t, int B, int parts, hipStream_t s, int jacobi) {
+    using CT = typename C2<R>::T;
+    const bool fp32 = std::is_same<R, float>::value;
+    CT* grid = fp32 ? reinterpret_cast<CT*>(d_grid32.p) : reinterpret_cast<CT*>(d_grid64.p);
+    const R* sw = fp32 ? reinterpret_cast<const R*>(d_sw32.p) : reinterpret_cast<const R*>(d_sw64.p);
+    const int SS = S.width * S.width * S.width;
+    const int Vi = (int)V, Ni = (int)N;
+    constexpr int G = 2;
+    if (parts & MPM_PART_RECIP) {
+        {
+            ProfScope ps("spread", s, B);
+            ps.kernel("quad_spread_kernel", {type_name<R>(), "2"});
+            hipLaunchKernelGGL((quad_spread_kernel<R, G>), dim3((unsigned)((V + 255) / 256), (unsigned)((B + G - 1) / G)),
+                               dim3(256), 0, s, t, B, Ni, Vi, SS, d_vox_start.p, d_vox_items.p, sw, d_qmap.p, grid);
+            MPM_HIP(hipGetLastError());
+        }
+        {
+            ProfScope ps("fft", s, B);
+            lfft->exec<R>(grid, grid, B, true, s);
+        }
+        {
+            ProfScope ps("scale", s, B);
+            ps.kernel("quad_scale_kernel", {type_name<R>()});
+            const R* kq = fp32 ? reinterpret_cast<const R*>(d_kq32.p) : reinterpret_cast<const R*>(d_kq64.p);
+            const size_t total = (size_t)B * V;
+            const int blocks = (int)std::min<size_t>((total + 255) / 256, 16384);
+            hipLaunchKernelGGL((quad_scale_kernel<R>), dim3(blocks), dim3(256), 0, s, grid, kq, V, total);
+            MPM_HIP(hipGetLastError());
+        }
+        {
+            ProfScope ps("fft", s, B);
+            lfft->exec<R>(grid, grid, B, false, s);
+        }
+    }
+    ProfScope ps("gather", s, B);
+    ps.kernel("quad_gather_kernel", {type_name<R>(), "2"});
+    const double h3 = S.h[0] * S.h[1] * S.h[2];
+    hipLaunchKernelGGL((quad_gather_kernel<R, G>), dim3((unsigned)((Ni + 3) / 4), (unsigned)((B + G - 1) / G)),
+                       dim3(256), 0, s, t, B, parts, Ni, Vi, SS, grid, sw, d_pidx.p, d_qmap.p, h3, d_nb_start.p,
+                       d_nb_j.p, d_pairc.p, d_selfc.p, self_perp, table->self_grad, jacobi);
+    MPM_HIP(hipGetLastError());
+}
+
+template <typename R>
 void EwaldOp::launch_poly(const SlotTable& t, int B, int parts, hipStream_t s, int jacobi) {
     using CT = typename C2<R>::T;
     const bool fp32 = std::is_same<R, float>::value;
@@ -3796,6 +4451,11 @@ void EwaldOp::apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t
         if (kt) d_kgrid64.reserve(kelems);
     }
     const int jac = jacobi ? 1 : 0;
+    if (quads) {
+        if (fp32) launch_quads<float>(t, B, parts, s, jac);
+        else launch_quads<double>(t, B, parts, s, jac);
+        return;
+    }
     if (poly) {
         if (fp32) launch_poly<float>(t, B, parts, s, jac);
         else launch_poly<double>(t, B, parts, s, jac);
@@ -3812,7 +4472,7 @@ void EwaldOp::apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t
 
 void EwaldOp::apply_contiguous(const double2* x, double2* y, int B, bool fp32, int parts, hipStream_t s,
                                const int* rows) {
-    const size_t n = 3 * (size_t)N;
+    const size_t n = n_unknowns();
     for (int b0 = 0; b0 < B; b0 += kMaxSlots) {
         const int cnt = std::min(kMaxSlots, B - b0);
         SlotTable t;
@@ -3844,6 +4504,7 @@ void EwaldOp::info(mpm_op_info& o) const {
     o.slab_planes = slab ? Zs : 0;
     o.fft_backend = (lfft && fft_backend == 0) ? 0 : 1;
     o.polydisperse = poly ? 1 : 0;
+    o.num_quads = (int64_t)qidx_h.size();
     o.num_unique_radii = poly ? (int)uniq_h.size() : 1;
 }
 

@@ -230,7 +230,8 @@ __device__ inline double2 jac_inv(double2 sc) {
 __global__ __launch_bounds__(256) void residual_kernel(Ptrs p, size_t n, const double2* __restrict__ rhs_base,
                                                        double2* __restrict__ true_base,
                                                        const double2* __restrict__ selfc,
-                                                       const int* __restrict__ rows, int N, int jacobi) {
+                                                       const int* __restrict__ rows, int N, int jacobi,
+                                                       const int* __restrict__ qidx) {
     const int slot = blockIdx.y;
     const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (e >= n) return;
@@ -240,7 +241,11 @@ __global__ __launch_bounds__(256) void residual_kernel(Ptrs p, size_t n, const d
     const double2 t = make_double2(bv.x - ax.x, bv.y - ax.y);
     true_base[(size_t)s * n + e] = t;
     double2 r = t;
-    if (jacobi) r = cmul(t, jac_inv(selfc[(size_t)rows[s] * N + e / 3]));
+    if (jacobi) {  // the particle of component e: dipoles [N][3], then quadrupoles [Q][5]
+        const size_t n3 = 3 * (size_t)N;
+        const int part = e < n3 ? (int)(e / 3) : qidx[(e - n3) / 5];
+        r = cmul(t, jac_inv(selfc[(size_t)rows[s] * N + part]));
+    }
     p.b[slot][e] = r;
 }
 
@@ -492,7 +497,7 @@ struct Ctx {
     double* host_pinned = nullptr;  // 2*kMaxSlots doubles
     Ctx(EwaldOp& o, const mpm_solve_desc& dd, hipStream_t st) : op(o), d(dd), s(st) {
         N = (int)op.N;
-        n = 3 * (size_t)N;
+        n = op.n_unknowns();
         MPM_HIP(hipHostMalloc((void**)&host_pinned, 4 * kMaxSlots * sizeof(double)));
     }
     ~Ctx() {
@@ -640,7 +645,7 @@ static void gmres_group(Ctx& C, const double2* b, double2* x, int B, mpm_solve_r
             Ptrs p = make_ptrs(active, [&](int q) { return W.w.p + q * n; }, [&](int q) { return Vk(q, 0); });
             hipLaunchKernelGGL(residual_kernel, grid_for(n, (int)active.size()), dim3(256), 0, C.s, p, n,
                                b, W.t.p, C.op.self_coef(), W.rows.p, C.N,
-                               d.jacobi);
+                               d.jacobi, C.op.quad_idx_dev());
             MPM_HIP(hipGetLastError());
             // norms of the true residual (W.t) and then of the preconditioned one (V_0), so
             // that W.nrm ends up holding ||r||^2 for the V_0 / g set-up below
@@ -768,7 +773,7 @@ static void gmres_group(Ctx& C, const double2* b, double2* x, int B, mpm_solve_r
         C.apply(need, xs, ys, false);
         Ptrs p = make_ptrs(need, [&](int q) { return W.w.p + q * n; }, [&](int q) { return W.w.p + q * n; });
         hipLaunchKernelGGL(residual_kernel, grid_for(n, (int)need.size()), dim3(256), 0, C.s, p, n,
-                           b, W.t.p, C.op.self_coef(), W.rows.p, C.N, d.jacobi);
+                           b, W.t.p, C.op.self_coef(), W.rows.p, C.N, d.jacobi, C.op.quad_idx_dev());
         MPM_HIP(hipGetLastError());
         C.norms(p, (int)need.size());
         std::vector<double> rn2;
@@ -911,7 +916,7 @@ static void bicgstab_group(Ctx& C, const double2* b, double2* x, int B, mpm_solv
         }
         Ptrs p = make_ptrs(active, [&](int q) { return vec(W.w, q); }, [&](int q) { return vec(W.r, q); });
         hipLaunchKernelGGL(residual_kernel, grid_for(n, (int)active.size()), dim3(256), 0, C.s, p, n,
-                           b, W.t.p, C.op.self_coef(), W.rows.p, C.N, d.jacobi);
+                           b, W.t.p, C.op.self_coef(), W.rows.p, C.N, d.jacobi, C.op.quad_idx_dev());
         MPM_HIP(hipGetLastError());
         std::vector<double> rn;
         norms_of(active, W.r, rn);
@@ -1014,7 +1019,7 @@ static void bicgstab_group(Ctx& C, const double2* b, double2* x, int B, mpm_solv
         C.apply(need, xs, ys, false);
         Ptrs p = make_ptrs(need, [&](int q) { return vec(W.w, q); }, [&](int q) { return vec(W.r, q); });
         hipLaunchKernelGGL(residual_kernel, grid_for(n, (int)need.size()), dim3(256), 0, C.s, p, n,
-                           b, W.t.p, C.op.self_coef(), W.rows.p, C.N, d.jacobi);
+                           b, W.t.p, C.op.self_coef(), W.rows.p, C.N, d.jacobi, C.op.quad_idx_dev());
         MPM_HIP(hipGetLastError());
         std::vector<double> rn, tn;
         norms_of(need, W.r, rn);
@@ -1046,7 +1051,7 @@ void solve_batched(EwaldOp& op, const mpm_solve_desc& d, const double2* b, doubl
     MPM_REQUIRE(d.precision >= 0 && d.precision <= 2, MPM_ERR_INVALID_ARGUMENT, "unknown precision");
     MPM_REQUIRE(op.self_rows == 1 || op.self_rows >= B, MPM_ERR_INVALID_ARGUMENT,
                 "need one self-coefficient row per system (or a single shared row)");
-    const size_t n = 3 * (size_t)op.N;
+    const size_t n = op.n_unknowns();
     Ctx C(op, d, s);
     for (int b0 = 0; b0 < B; b0 += kMaxSlots) {
         const int cnt = std::min(kMaxSlots, B - b0);

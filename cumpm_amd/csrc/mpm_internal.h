@@ -141,6 +141,11 @@ class EwaldOp {
     void apply_contiguous(const double2* x, double2* y, int B, bool fp32, int parts,
                           hipStream_t s, const int* rows = nullptr);
     void info(mpm_op_info& out) const;
+    // quadrupoles on particles idx[nq] (every particle when nq < 0); before set_positions.
+    // Unknowns per right-hand side then are [3N dipole | 5Q quadrupole] components.
+    void set_quads(const int* idx, int64_t nq);
+    size_t n_unknowns() const { return 3 * (size_t)N + 5 * qidx_h.size(); }
+    const int* quad_idx_dev() const { return quads ? d_qidx.p : nullptr; }
 
     int64_t N = 0;
     int self_rows = 0;
@@ -229,6 +234,20 @@ class EwaldOp {
     int fft_backend = 0;         // 0: built-in LDS FFT, 1: rocFFT
     std::unique_ptr<PairTable> table;
     std::vector<double> last_xyz;  // positions of the current precalcs
+
+    // ---- quadrupoles (monodisperse) ----
+    bool quads = false, quads_all = false;
+    std::vector<int> qreq_h;          // requested indices (empty with quads_all)
+    std::vector<int> qidx_h;          // [Q] particle of each quadrupole
+    DevBuf<int> d_qidx, d_qmap;       // [Q], [N] (quadrupole slot or -1)
+    std::vector<double> h_pair_q;     // [M][7] quadrupole table values per pair
+    DevBuf<double> d_sw64;            // [N][W^3] Gaussian weights (spread constant included)
+    DevBuf<float> d_sw32;
+    DevBuf<double> d_kq64;            // [V][16] k-space: khat, sc, scQ, scGP, scGQ, Qfactor[5], Qdot[5]
+    DevBuf<float> d_kq32;
+    void setup_quad_stencils(const std::vector<double>& pos, int Nn);
+    template <typename R>
+    void launch_quads(const SlotTable& t, int B, int parts, hipStream_t s, int jacobi);
 
     // ---- polydisperse operator ----
     bool poly = false;

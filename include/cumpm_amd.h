@@ -87,6 +87,7 @@ typedef struct {
     int fft_backend;     /* FFT in use: 0 = built-in, 1 = rocFFT */
     int polydisperse;    /* 1 for an operator made by mpm_op_create_poly */
     int num_unique_radii;
+    int64_t num_quads;   /* Q: particles with quadrupoles (unknowns per right-hand side: 3N + 5Q) */
 } mpm_op_info;
 
 /* Monodisperse_Ewald_Electric_Field(box_x, box_y, box_z, errortol, xi, SOLVER_AX, radius=1)
@@ -108,6 +109,16 @@ mpm_status mpm_op_get_info(const mpm_op* op, mpm_op_info* info);
  * lazy per-frame precalcs of monodisperse_ewald_electric_field.cu:2485-2503 (neighbour list,
  * real-space pair functions, spread/contract stencils).  xyz: host, [N][3], nondimensional. */
 mpm_status mpm_op_set_positions(mpm_op* op, const double* xyz, int64_t n_particles);
+
+/* solve_quadrupoles / quad_idxs of the Monodisperse_Ewald_Electric_Field constructor
+ * (monodisperse_ewald_electric_field.cu:28-74; kernels :1435-1487, :1528-1629, :1631-1869,
+ * :1871-1999): quadrupoles on particles idx[nq] (every particle when nq < 0).  Call before
+ * mpm_op_set_positions.  Vectors then hold 3N dipole components followed by 5 components
+ * (xx, xy, xz, yy, yz of the traceless moment) per quadrupole, in idx order; the operator
+ * returns field and field gradient in the same layout.  Full-grid layout and built-in FFT.
+ * The dipole-quadrupole real-space couplings take the signs of the reference's Python
+ * formulation (tests/quadrupole_field.py:214-215); the CUDA kernel has both flipped. */
+mpm_status mpm_op_set_quads(mpm_op* op, const int* idx, int64_t nq);
 
 /* setBlochWavevector — src/cpp/electric_field/ewald_electric_field_base.h:136-142. */
 mpm_status mpm_op_set_bloch(mpm_op* op, double kx, double ky);

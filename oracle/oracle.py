@@ -382,6 +382,7 @@ class PolyOperator(Operator):
         if not self._h:
             raise RuntimeError(err.value.decode())
         self.N = 0
+        self.Q = 0
         self.radii = r
 
     def set_positions(self, pos):

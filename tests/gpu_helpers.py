@@ -9,7 +9,7 @@ from cumpm_amd import _capi as C
 class Op:
     """Owns an mpm_op plus device buffers; every call goes through the C ABI."""
 
-    def __init__(self, box, errortol, xi, grid=None, support=0, layout=0, fft_backend=0, radii=None):
+    def __init__(self, box, errortol, xi, grid=None, support=0, layout=0, fft_backend=0, radii=None, quads=None):
         d = C.OpDesc()
         d.layout = int(layout)
         d.fft_backend = int(fft_backend)
@@ -26,6 +26,13 @@ class Op:
             r = np.ascontiguousarray(radii, np.float64)
             C.check(C.load().mpm_op_create_poly(ctypes.byref(d), C.dptr(r), r.size, ctypes.byref(self.h)))
         self.N = 0
+        self.Q = 0
+        self._quads = quads
+        if quads is True:
+            C.check(C.load().mpm_op_set_quads(self.h, None, -1))
+        elif quads is not None:
+            idx = (ctypes.c_int * len(quads))(*[int(v) for v in quads])
+            C.check(C.load().mpm_op_set_quads(self.h, idx, len(quads)))
 
     def __del__(self):
         if getattr(self, "h", None) and C._lib is not None:
@@ -40,6 +47,14 @@ class Op:
         pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
         self.N = pos.shape[0]
         C.check(C.load().mpm_op_set_positions(self.h, C.dptr(pos), self.N))
+        self.Q = int(self.info().num_quads)
+
+    @property
+    def n(self):
+        return 3 * self.N + 5 * self.Q
+
+    def _shape(self, B):
+        return (B, self.N, 3) if self.Q == 0 else (B, self.n)
 
     def set_bloch(self, kx, ky):
         C.check(C.load().mpm_op_set_bloch(self.h, float(kx), float(ky)))
@@ -49,7 +64,8 @@ class Op:
         C.check(C.load().mpm_op_set_self_coef(self.h, C.dptr(rows.view(np.float64)), rows.shape[0]))
 
     def apply(self, p, precision=C.PREC_DOUBLE, parts=C.PART_ALL):
-        p = np.ascontiguousarray(p, np.complex128).reshape(-1, self.N, 3)
+        p = np.ascontiguousarray(p, np.complex128)
+        p = p.reshape(self._shape(p.size // self.n))
         B = p.shape[0]
         xb = C.DeviceBuffer(p.nbytes)
         yb = C.DeviceBuffer(p.nbytes)
@@ -65,7 +81,8 @@ class Op:
 
     def solve(self, b, x0=None, tol=1e-8, method=C.GMRES, restart=50, maxiter=1000, jacobi=True,
               precision=C.SOLVE_DOUBLE, _entry="mpm_solve"):
-        b = np.ascontiguousarray(b, np.complex128).reshape(-1, self.N, 3)
+        b = np.ascontiguousarray(b, np.complex128)
+        b = b.reshape(self._shape(b.size // self.n))
         B = b.shape[0]
         x0 = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, np.complex128).reshape(b.shape)
         bb = C.DeviceBuffer(b.nbytes)

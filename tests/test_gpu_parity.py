@@ -525,3 +525,79 @@ def test_dev_polydisperse_field_matches_monodisperse_for_identical_radii():
         np.testing.assert_allclose(tot, ef.compute_real_field() + ef.compute_inverse_field(), rtol=1e-12, atol=1e-12)
         fields.append(tot)
     np.testing.assert_allclose(fields[1], fields[0], rtol=1e-5, atol=1e-5)
+
+
+def _quad_geom():
+    L = np.array([22.0, 23.0, 24.0])
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(0, 20, (10, 3))
+    pos[1] = pos[0] + [1.5, 0.4, 0.3]   # overlapping
+    pos[2] = pos[0] + [0.0, 2.6, 0.2]   # close
+    return L, pos
+
+
+@pytest.mark.parametrize("quads,bloch", [(True, None), ([0, 3, 7], (0.12, -0.2))])
+def test_quad_apply_fp64_matches_oracle(quads, bloch):
+    L, pos = _quad_geom()
+    _case(f"quad apply fp64 quads={quads} bloch={bloch}")
+    op = Op(L, 1e-8, 0.5, quads=quads)
+    ref = O.Operator(L, 1e-8, 0.5)
+    if bloch:
+        op.set_bloch(*bloch)
+        ref.set_bloch(*bloch)
+    op.set_positions(pos)
+    ref.set_positions(pos)
+    ref.set_quads(None if quads is True else quads)
+    assert op.n == ref.n and op.info().num_quads == ref.Q
+    rng = np.random.default_rng(51)
+    B = 3
+    p = rng.standard_normal((B, op.n)) + 1j * rng.standard_normal((B, op.n))
+    rows = rng.standard_normal((B, 10)) + 1j * rng.standard_normal((B, 10))
+    op.set_self_coef(rows)
+    for parts in (C.PART_REAL, C.PART_RECIP, C.PART_ALL):
+        got = op.apply(p, C.PREC_DOUBLE, parts)
+        for b in range(B):
+            assert rel(got[b], ref.apply(p[b], rows[b], parts)) < 1e-11, (parts, b)
+    got32 = op.apply(p, C.PREC_MIXED)
+    assert rel(got32, op.apply(p, C.PREC_DOUBLE)) < 1e-4
+
+
+def test_quad_golden_fixture_through_solve():
+    """Reference tests/test_dipole_solvers.py:63-148: GMRES on the device against the
+    fixture's dense solve (its bar 1e-5)."""
+    _case("quad golden fixture")
+    ref = np.load(os.path.join(GOLDEN, "ref_quadrupoles_monodisperse.npz"))
+    op = Op([100.0] * 3, 1e-6, 0.4, quads=True)
+    op.set_positions(np.array([[10.0, 10.0, 10.0], [20.0, 20.0, 20.0]]))
+    assert op.n == 16
+    sc = O.self_coefficient(np.full(2, 4.5 + 0.2j))
+    op.set_self_coef(np.stack([sc] * 3))
+    b = np.zeros((3, 16), complex)
+    for dim in range(3):
+        b[dim, dim:6:3] = 1.0
+    x, reps = op.solve(b, tol=1e-12)
+    for dim in range(3):
+        assert reps[dim]["converged"]
+        np.testing.assert_allclose(x[dim, :6].reshape(2, 3), ref["dips_py_all"][dim], rtol=1e-7, atol=1e-10)
+        np.testing.assert_allclose(x[dim, 6:].reshape(2, 5), ref["quads_py_all"][dim], rtol=1e-7, atol=1e-10)
+
+
+@pytest.mark.parametrize("precision", [C.SOLVE_DOUBLE, C.SOLVE_REFINED])
+def test_quad_solve_matches_oracle(precision):
+    L, pos = _quad_geom()
+    _case(f"quad solve precision={precision}")
+    op = Op(L, 1e-8, 0.5, quads=True)
+    op.set_positions(pos)
+    sc = O.self_coefficient(np.full(10, -2.0 + 0.5j))
+    op.set_self_coef(sc)
+    b = np.zeros((2, op.n), complex)
+    b[0, 0:30:3] = 1.0
+    b[1, 2:30:3] = 1.0
+    x, reps = op.solve(b, tol=1e-10, precision=precision)
+    ref = O.Operator(L, 1e-8, 0.5)
+    ref.set_positions(pos)
+    ref.set_quads()
+    for q in range(2):
+        xr, rr = ref.solve(b[q], sc, tol=1e-10)
+        assert reps[q]["converged"] and rr["converged"]
+        assert rel(x[q], xr) < 1e-6
