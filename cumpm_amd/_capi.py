@@ -67,6 +67,7 @@ class DipoleDesc(ctypes.Structure):
                 ("xi", ctypes.c_double), ("tol", ctypes.c_double), ("quiet", ctypes.c_int),
                 ("guess", ctypes.c_int), ("method", ctypes.c_int), ("field_type", ctypes.c_int),
                 ("E0", _dp), ("num_E0", _i64), ("E0_len", _i64), ("solve_quadrupoles", ctypes.c_int),
+                ("quad_idxs", ctypes.POINTER(ctypes.c_int)), ("num_quad_idxs", _i64),
                 ("precision", ctypes.c_int), ("refined", ctypes.c_int), ("batched", ctypes.c_int),
                 ("restart", ctypes.c_int), ("maxiter", ctypes.c_int), ("grid", ctypes.c_int * 3),
                 ("support", ctypes.c_int), ("ewald_tol", ctypes.c_double)]
@@ -98,6 +99,8 @@ _SIGS = {
     "mpm_dipole_solver_get_dims": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i64)]),
     "mpm_dipole_solver_get_eff_polarizability": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
     "mpm_dipole_solver_get_dipoles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
+    "mpm_dipole_solver_get_num_quads": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i64)]),
+    "mpm_dipole_solver_get_quadrupoles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
     "mpm_dipole_solver_get_reports": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolveReport)]),
     "mpm_dipole_solver_get_op_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OpInfo)]),
     "mpm_dipole_solver_set_use_jacobi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),

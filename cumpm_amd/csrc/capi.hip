@@ -75,13 +75,16 @@ struct mpm_dipole_solver {
     int grid[3] = {0, 0, 0};
     int support = 0;
     double ewald_tol = 0.0;
+    bool solve_quads = false;
+    std::vector<int> quad_idxs;  // empty with solve_quads: every particle
     // state
-    size_t N = 0, W = 0, K = 0, frames = 0;
+    size_t N = 0, W = 0, K = 0, Q = 0, frames = 0;
     double length_scale = 1.0, eps_scale = 1.0, vol_frac = 0.0;
     std::unique_ptr<mpm::EwaldOp> op;
     bool use_poly = false;
     std::vector<cplx> dips;      // [F][W][N][K][3]
     std::vector<cplx> avg_dips;  // [F][W][K][3]
+    std::vector<cplx> quads;     // [F][W][Q][K][5]
     std::vector<mpm_solve_report> reports;  // [F][W][K]
     double solve_ms = 0, setup_ms = 0, total_ms = 0;
     mpm::DevBuf<double2> d_b, d_x;
@@ -93,6 +96,8 @@ struct mpm_dipole_solver {
 
     void set_dims(size_t n) {
         N = n;
+        // dipole_solver.cu:131-133
+        Q = solve_quads ? (quad_idxs.empty() ? N : quad_idxs.size()) : 0;
         if (radius.empty())
             radius.assign(N, 1.0);
         else if (radius.size() == 1)
@@ -116,13 +121,24 @@ struct mpm_dipole_solver {
             else if (row.size() != N)
                 throw mpm::Error(MPM_ERR_RUNTIME, "The number of particles is inconsistent with eps_p provided.");
         }
+        if (Q > 0 && use_poly)
+            throw mpm::Error(MPM_ERR_UNSUPPORTED,
+                             "quadrupoles with the polydisperse field are not provided by this build");
+        for (int q : quad_idxs)
+            if (q < 0 || (size_t)q >= N) throw mpm::Error(MPM_ERR_INVALID_ARGUMENT, "quadrupole index out of range");
         K = E0.size();
+        const size_t nu = n_unknowns();
         for (size_t k = 0; k < K; ++k)
-            if (E0[k].size() != 3 && E0[k].size() != 3 * N)
+            if (E0[k].size() != 3 && E0[k].size() != 3 * N && E0[k].size() != nu)
                 throw mpm::Error(MPM_ERR_RUNTIME, "Incident field E0[" + std::to_string(k) + "] has size " +
                                                       std::to_string(E0[k].size()) + ", but expected 3, " +
-                                                      std::to_string(3 * N) + ", or " + std::to_string(3 * N) + ".");
+                                                      std::to_string(3 * N) + ", or " + std::to_string(nu) + ".");
     }
+
+    // unknowns of one system: N dipoles (3 each) then Q quadrupoles (5 each)
+    size_t n_unknowns() const { return 3 * N + 5 * Q; }
+    // particle carrying quadrupole slot q
+    size_t quad_particle(size_t q) const { return quad_idxs.empty() ? q : (size_t)quad_idxs[q]; }
 
     void nondimensionalize() {
         length_scale = *std::min_element(radius.begin(), radius.end());
@@ -138,9 +154,9 @@ struct mpm_dipole_solver {
         vol_frac = bv == 0.0 ? 0.0 : (4.0 / 3.0) * M_PI * s3 / bv;
     }
 
-    // guess layout [p][K][3]
+    // guess layout [p][K][3] then [q][K][5] (dipole_solver.cu:244-261); quadrupoles start at zero
     std::vector<cplx> mean_field(size_t w) const {
-        std::vector<cplx> g(N * K * 3, 0.0);
+        std::vector<cplx> g(N * K * 3 + Q * K * 5, 0.0);
         for (size_t p = 0; p < N; ++p) {
             const cplx beta = (eps_p[w][p] - 1.0) / (eps_p[w][p] + 2.0);
             const cplx val = 4.0 * M_PI * std::pow(radius[p], 3.0) * beta / (1.0 - beta * vol_frac);
@@ -152,27 +168,43 @@ struct mpm_dipole_solver {
         }
         return g;
     }
-    // frame_dip: [W][N][K][3] of the current frame
-    std::vector<cplx> guess_for(size_t w, const std::vector<cplx>& fd) const {
-        const size_t blk = N * K * 3;
+    // fd: [W][N][K][3] and fq: [W][Q][K][5] of the current frame (dipole_solver.cu:233-345)
+    std::vector<cplx> guess_for(size_t w, const std::vector<cplx>& fd, const std::vector<cplx>& fq) const {
+        const size_t blk = N * K * 3, qblk = Q * K * 5;
         if (guess == MPM_GUESS_MEAN_FIELD || w < 1) return mean_field(w);
-        if (guess == MPM_GUESS_PREVIOUS || w < 2)
-            return std::vector<cplx>(fd.begin() + (w - 1) * blk, fd.begin() + w * blk);
-        std::vector<cplx> g(blk);
+        std::vector<cplx> g(blk + qblk);
+        if (guess == MPM_GUESS_PREVIOUS || w < 2) {
+            std::copy(fd.begin() + (w - 1) * blk, fd.begin() + w * blk, g.begin());
+            std::copy(fq.begin() + (w - 1) * qblk, fq.begin() + w * qblk, g.begin() + blk);
+            return g;
+        }
+        // extrapolate in the inverse polarisability X = (eps + 2) / (eps - 1) of the owning particle
         auto X = [](cplx e) { return (e - 1.0 == 0.0) ? cplx(0.0) : (e + 2.0) / (e - 1.0); };
-        for (size_t p = 0; p < N; ++p) {
+        auto extrapolate = [&](size_t p, const cplx* v1, const cplx* v2, cplx* out, size_t len) {
             cplx run = X(eps_p[w - 1][p]) - X(eps_p[w - 2][p]);
             cplx new_run = X(eps_p[w][p]) - X(eps_p[w - 1][p]);
             if (std::abs(run) < 1e-12) {
                 run = 1.0;
                 new_run = 0.0;
             }
-            for (size_t q = 0; q < K * 3; ++q) {
-                const cplx v1 = fd[(w - 1) * blk + p * K * 3 + q], v2 = fd[(w - 2) * blk + p * K * 3 + q];
-                g[p * K * 3 + q] = v1 + new_run * (v1 - v2) / run;
-            }
-        }
+            for (size_t i = 0; i < len; ++i) out[i] = v1[i] + new_run * (v1[i] - v2[i]) / run;
+        };
+        for (size_t p = 0; p < N; ++p)
+            extrapolate(p, &fd[(w - 1) * blk + p * K * 3], &fd[(w - 2) * blk + p * K * 3], &g[p * K * 3], K * 3);
+        for (size_t q = 0; q < Q; ++q)
+            extrapolate(quad_particle(q), &fq[(w - 1) * qblk + q * K * 5], &fq[(w - 2) * qblk + q * K * 5],
+                        &g[blk + q * K * 5], K * 5);
         return g;
+    }
+
+    // one system's start vector from a guess laid out as mean_field()
+    std::vector<cplx> guess_column(const std::vector<cplx>& g, size_t k) const {
+        std::vector<cplx> x0(n_unknowns());
+        for (size_t p = 0; p < N; ++p)
+            for (int c = 0; c < 3; ++c) x0[3 * p + c] = g[p * K * 3 + k * 3 + c];
+        for (size_t q = 0; q < Q; ++q)
+            for (int c = 0; c < 5; ++c) x0[3 * N + 5 * q + c] = g[N * K * 3 + q * K * 5 + k * 5 + c];
+        return x0;
     }
 
     std::vector<cplx> self_coef(size_t w) const {
@@ -185,10 +217,15 @@ struct mpm_dipole_solver {
         return sc;
     }
 
+    // dipole_solver.cu:368-390: E0 of length 3 (uniform), 3N (per particle) or 3N+5Q (full
+    // vector); the quadrupole rows are zero unless given
     std::vector<cplx> rhs(size_t k) const {
-        std::vector<cplx> e(3 * N);
-        for (size_t p = 0; p < N; ++p)
-            for (int c = 0; c < 3; ++c) e[3 * p + c] = E0[k].size() == 3 ? E0[k][c] : E0[k][3 * p + c];
+        std::vector<cplx> e(n_unknowns(), 0.0);
+        if (E0[k].size() == e.size() && e.size() != 3)
+            std::copy(E0[k].begin(), E0[k].end(), e.begin());
+        else
+            for (size_t p = 0; p < N; ++p)
+                for (int c = 0; c < 3; ++c) e[3 * p + c] = E0[k].size() == 3 ? E0[k][c] : E0[k][3 * p + c];
         return e;
     }
 
@@ -225,6 +262,9 @@ struct mpm_dipole_solver {
                 op = std::make_unique<mpm::EwaldOp>(od, radius.data(), (int64_t)N);
             else
                 op = std::make_unique<mpm::EwaldOp>(od);
+            if (Q > 0)
+                op->set_quads(quad_idxs.empty() ? nullptr : quad_idxs.data(),
+                              quad_idxs.empty() ? -1 : (int64_t)quad_idxs.size());
         } else if (N != n) {
             throw mpm::Error(MPM_ERR_RUNTIME, "The number of particles has changed!");
         }
@@ -236,16 +276,17 @@ struct mpm_dipole_solver {
             xyz[3 * i + 2] = z[i] / length_scale;
         }
         op->set_positions(xyz.data(), (int64_t)N);
-        const size_t n3 = 3 * N;
-        // initial guess: mean field, or the previous frame's first frequency
+        const size_t n3 = n_unknowns();
+        // initial guess: mean field, or the previous frame's first frequency (dipole_solver.cu:578-592)
         std::vector<cplx> init;
         if (frames == 1) {
             init = mean_field(0);
         } else {
-            const size_t off = (frames - 2) * W * N * K * 3;
+            const size_t off = (frames - 2) * W * N * K * 3, qoff = (frames - 2) * W * Q * K * 5;
             init.assign(dips.begin() + off, dips.begin() + off + N * K * 3);
+            init.insert(init.end(), quads.begin() + qoff, quads.begin() + qoff + Q * K * 5);
         }
-        std::vector<cplx> fd(W * N * K * 3, 0.0);
+        std::vector<cplx> fd(W * N * K * 3, 0.0), fq(W * Q * K * 5, 0.0);
         std::vector<mpm_solve_report> reps(W * K);
         MPM_HIP(hipDeviceSynchronize());
         auto t1 = std::chrono::steady_clock::now();
@@ -260,6 +301,8 @@ struct mpm_dipole_solver {
         auto store = [&](size_t w, size_t k, const std::vector<cplx>& sol) {
             for (size_t p = 0; p < N; ++p)
                 for (int c = 0; c < 3; ++c) fd[(w * N + p) * K * 3 + k * 3 + c] = sol[3 * p + c];
+            for (size_t q = 0; q < Q; ++q)
+                for (int c = 0; c < 5; ++c) fq[(w * Q + q) * K * 5 + k * 5 + c] = sol[3 * N + 5 * q + c];
         };
         if (!batched) {
             print("Wavenumber:");
@@ -270,12 +313,10 @@ struct mpm_dipole_solver {
                 print(std::to_string(w) + " of " + std::to_string(W));
                 auto sc = self_coef(w);
                 op->set_self_coef(reinterpret_cast<const double*>(sc.data()), 1);
-                const std::vector<cplx> g = (w == 0) ? init : guess_for(w, fd);
+                const std::vector<cplx> g = (w == 0) ? init : guess_for(w, fd, fq);
                 for (size_t k = 0; k < K; ++k) {
                     auto b = rhs(k);
-                    std::vector<cplx> x0(n3);
-                    for (size_t p = 0; p < N; ++p)
-                        for (int c = 0; c < 3; ++c) x0[3 * p + c] = g[p * K * 3 + k * 3 + c];
+                    const std::vector<cplx> x0 = guess_column(g, k);
                     MPM_HIP(hipMemcpy(d_b.p, b.data(), n3 * sizeof(double2), hipMemcpyHostToDevice));
                     MPM_HIP(hipMemcpy(d_x.p, x0.data(), n3 * sizeof(double2), hipMemcpyHostToDevice));
                     timed_solve(d_b.p, d_x.p, 1, &reps[w * K + k]);
@@ -295,8 +336,8 @@ struct mpm_dipole_solver {
                 for (size_t k = 0; k < K; ++k) {
                     auto b = rhs(k);
                     std::copy(b.begin(), b.end(), b_all.begin() + (w * K + k) * n3);
-                    for (size_t p = 0; p < N; ++p)
-                        for (int c = 0; c < 3; ++c) x_all[(w * K + k) * n3 + 3 * p + c] = g[p * K * 3 + k * 3 + c];
+                    const std::vector<cplx> x0 = guess_column(g, k);
+                    std::copy(x0.begin(), x0.end(), x_all.begin() + (w * K + k) * n3);
                 }
                 std::copy(sc.begin(), sc.end(), sc_all.begin() + w * N);
             }
@@ -331,6 +372,7 @@ struct mpm_dipole_solver {
                     avg_dips.push_back(s / (double)N);
                 }
         dips.insert(dips.end(), fd.begin(), fd.end());
+        quads.insert(quads.end(), fq.begin(), fq.end());
         reports.insert(reports.end(), reps.begin(), reps.end());
     }
 };
@@ -464,7 +506,8 @@ mpm_status mpm_dipole_solver_create(const mpm_dipole_desc* d, mpm_dipole_solver*
         MPM_REQUIRE(d && out, MPM_ERR_INVALID_ARGUMENT, "null argument");
         MPM_REQUIRE(d->num_wavevectors >= 1 && d->eps_cols >= 1 && d->eps_p, MPM_ERR_INVALID_ARGUMENT,
                     "eps_p is empty.");
-        MPM_REQUIRE(!d->solve_quadrupoles, MPM_ERR_UNSUPPORTED, "quadrupoles are not provided by this build");
+        MPM_REQUIRE(d->num_quad_idxs >= 0 && (d->num_quad_idxs == 0 || d->quad_idxs), MPM_ERR_INVALID_ARGUMENT,
+                    "null quadrupole indices");
         MPM_REQUIRE(d->tol > 0.0, MPM_ERR_INVALID_ARGUMENT, "tol must be positive");
         auto s = std::make_unique<mpm_dipole_solver>();
         s->box.assign(d->box, d->box + 3);
@@ -507,6 +550,8 @@ mpm_status mpm_dipole_solver_create(const mpm_dipole_desc* d, mpm_dipole_solver*
         for (int k = 0; k < 3; ++k) s->grid[k] = d->grid[k];
         s->support = d->support;
         s->ewald_tol = d->ewald_tol;
+        s->solve_quads = d->solve_quadrupoles != 0;
+        if (s->solve_quads && d->num_quad_idxs > 0) s->quad_idxs.assign(d->quad_idxs, d->quad_idxs + d->num_quad_idxs);
         *out = s.release();
     });
 }
@@ -558,6 +603,25 @@ mpm_status mpm_dipole_solver_get_dipoles(const mpm_dipole_solver* s, int physica
     });
 }
 
+mpm_status mpm_dipole_solver_get_num_quads(const mpm_dipole_solver* s, int64_t* num_quads) {
+    return guarded([&] {
+        MPM_REQUIRE(s && num_quads, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        *num_quads = (int64_t)s->Q;
+    });
+}
+
+mpm_status mpm_dipole_solver_get_quadrupoles(const mpm_dipole_solver* s, int physical, double* out) {
+    return guarded([&] {
+        MPM_REQUIRE(s && out, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        // dipole_solver.cu:640-647: quadrupoles scale with eps_m a^4
+        const double f = physical ? s->eps_scale * std::pow(s->length_scale, 4.0) : 1.0;
+        for (size_t q = 0; q < s->quads.size(); ++q) {
+            out[2 * q] = s->quads[q].real() * f;
+            out[2 * q + 1] = s->quads[q].imag() * f;
+        }
+    });
+}
+
 mpm_status mpm_dipole_solver_get_reports(const mpm_dipole_solver* s, mpm_solve_report* out) {
     return guarded([&] {
         MPM_REQUIRE(s && out, MPM_ERR_INVALID_ARGUMENT, "null argument");
@@ -602,6 +666,7 @@ mpm_status mpm_dipole_solver_reset(mpm_dipole_solver* s) {
         MPM_REQUIRE(s, MPM_ERR_INVALID_ARGUMENT, "null argument");
         s->frames = 0;
         s->dips.clear();
+        s->quads.clear();
         s->avg_dips.clear();
         s->reports.clear();
     });

@@ -51,8 +51,6 @@ class MonodisperseEwaldElectricField:
             mode = FieldCalcMode.SOLVER_AX if calc_inter_dipole else FieldCalcMode.INTERACTION_FIELD
         if mode != FieldCalcMode.SOLVER_AX:
             raise NotImplementedError("INTERACTION_FIELD mode (near fields) is not provided by this build")
-        if solve_quadrupoles:
-            raise NotImplementedError("quadrupoles are not provided by this build yet")
         self._prec = _parse_precision(precision)
         d = C.OpDesc()
         d.box[:] = [float(box_x), float(box_y), float(box_z)]
@@ -62,6 +60,15 @@ class MonodisperseEwaldElectricField:
             d.grid[:] = [int(g) for g in grid]
         d.support = int(support)
         self._h = self._create(d)
+        self._solve_quads = bool(solve_quadrupoles)
+        self._Q = 0
+        self._quad = None     # host copy [Q][5] complex
+        if self._solve_quads:
+            if quad_idxs is None or len(quad_idxs) == 0:
+                check(C.load().mpm_op_set_quads(self._h, None, -1))
+            else:
+                idx = (ctypes.c_int * len(quad_idxs))(*[int(i) for i in quad_idxs])
+                check(C.load().mpm_op_set_quads(self._h, idx, len(quad_idxs)))
         self._box = (float(box_x), float(box_y), float(box_z))
         self._errortol = float(errortol)
         self._xi = float(xi)
@@ -162,11 +169,11 @@ class MonodisperseEwaldElectricField:
 
     @property
     def solve_quadrupoles(self):
-        return False
+        return self._solve_quads
 
     @property
     def num_quads(self):
-        return 0
+        return self._Q
 
     @property
     def num_grid(self):
@@ -189,11 +196,13 @@ class MonodisperseEwaldElectricField:
         check(C.load().mpm_op_set_positions(self._h, C.dptr(xyz), x.size))
         if x.size != self._N:
             self._N = x.size
+            self._Q = int(self._info().num_quads)
             self._dip = np.zeros((self._N, 3), np.complex128)
+            self._quad = np.zeros((self._Q, 5), np.complex128)
             self._sc = np.zeros(self._N, np.complex128)
-            self._epoint = np.zeros((self._N, 3), np.complex128)
-            self._xbuf = C.DeviceBuffer(self._dip.nbytes)
-            self._ybuf = C.DeviceBuffer(self._dip.nbytes)
+            self._epoint = np.zeros(3 * self._N + 5 * self._Q, np.complex128)
+            self._xbuf = C.DeviceBuffer(self._epoint.nbytes)
+            self._ybuf = C.DeviceBuffer(self._epoint.nbytes)
         self._push_self()
 
     def update_dipoles(self, dip_x, dip_y, dip_z):
@@ -210,6 +219,23 @@ class MonodisperseEwaldElectricField:
         """Convenience: p of shape (N, 3) complex."""
         p = np.asarray(p, np.complex128).reshape(self._N, 3)
         self._dip = p.copy()
+
+    def update_quadrupoles(self, quad_1, quad_2, quad_3, quad_4, quad_5):
+        z = np.zeros(self._Q)
+        self.update_quadrupoles_complex(quad_1, z, quad_2, z, quad_3, z, quad_4, z, quad_5, z)
+
+    def update_quadrupoles_complex(self, quad_1r, quad_1i, quad_2r, quad_2i, quad_3r, quad_3i, quad_4r, quad_4i,
+                                   quad_5r, quad_5i):
+        """Components (Qxx, Qxy, Qxz, Qyy, Qyz) of the quadrupole subset, Qzz = -(Qxx + Qyy)."""
+        comps = [_vec(v) for v in (quad_1r, quad_1i, quad_2r, quad_2i, quad_3r, quad_3i, quad_4r, quad_4i,
+                                   quad_5r, quad_5i)]
+        if any(c.size != self._Q for c in comps):
+            raise ValueError("quadrupole component vectors must match the number of quadrupoles")
+        self._quad = np.stack([comps[2 * i] + 1j * comps[2 * i + 1] for i in range(5)], axis=1)
+
+    def set_quadrupoles(self, q):
+        """Convenience: q of shape (Q, 5) complex."""
+        self._quad = np.asarray(q, np.complex128).reshape(self._Q, 5).copy()
 
     def set_self_coef(self, self_coef_r, self_coef_i=None):
         if self_coef_i is None:
@@ -237,9 +263,10 @@ class MonodisperseEwaldElectricField:
     def _apply(self, parts):
         if not self._N:
             raise RuntimeError("electricField: particle coordinates have not been set")
-        self._xbuf.upload(np.ascontiguousarray(self._dip))
+        x = np.concatenate([self._dip.reshape(-1), self._quad.reshape(-1)])
+        self._xbuf.upload(np.ascontiguousarray(x))
         check(C.load().mpm_op_apply(self._h, self._xbuf.ptr, self._ybuf.ptr, 1, int(self._prec), int(parts), None))
-        out = np.zeros((self._N, 3), np.complex128)
+        out = np.zeros_like(x)
         self._ybuf.download(out)
         self._epoint = out
 
@@ -262,11 +289,19 @@ class MonodisperseEwaldElectricField:
         self.calculate()
         total = self.get_epoint_host()
         real = self.compute_real_field()
+        if isinstance(total, tuple):
+            return total[0] - real[0], total[1] - real[1]
         return total - real
 
     def get_epoint_host(self, reshape=True):
+        """Field at the particles, (N, 3); with quadrupoles the pair ((N, 3), (Q, 5)) of field and
+        field gradient (dev/electric_field.py:816-842 of the reference)."""
         e = self._epoint.copy()
-        return e if reshape else e.reshape(-1)
+        if not reshape:
+            return e
+        if self._solve_quads and self._Q > 0:
+            return e[:3 * self._N].reshape(self._N, 3), e[3 * self._N:].reshape(self._Q, 5)
+        return e.reshape(self._N, 3)
 
     def get_dipoles_complex_host(self):
         p = self._dip

@@ -120,14 +120,15 @@ class dipole_solver:
                 E0_list = [[complex(x) for x in block.ravel()] for block in E0_arr]
             else:
                 raise ValueError("E0 must be a 1D, 2D, or 3D array")
+        # solver.py:145-154 of the reference: True = every particle, an iterable = that subset
+        quad_idxs_list = []
         if isinstance(quadrupoles, bool):
             solve_quadrupoles = quadrupoles
         elif np.iterable(quadrupoles):
             solve_quadrupoles = True
+            quad_idxs_list = [int(i) for i in quadrupoles]
         else:
             raise TypeError("quadrupoles must be a boolean or a list/iterable of particle indices")
-        if solve_quadrupoles:
-            raise NotImplementedError("quadrupoles are not provided by this build yet")
 
         eps_arr = np.asarray(eps_p)
         if eps_arr.ndim == 0 or (eps_arr.ndim == 1 and eps_arr.size == 1):
@@ -144,6 +145,7 @@ class dipole_solver:
         # keep the host arrays alive until create() has copied them
         self._eps = np.ascontiguousarray(eps2)
         self._rad = np.ascontiguousarray(radius_list, np.float64)
+        self._qidx = (ctypes.c_int * max(1, len(quad_idxs_list)))(*quad_idxs_list)
         if E0_list:
             lens = {len(r) for r in E0_list}
             if len(lens) != 1:
@@ -167,7 +169,9 @@ class dipole_solver:
         if self._E0 is not None:
             d.E0 = C.dptr(self._E0.view(np.float64))
             d.num_E0, d.E0_len = self._E0.shape
-        d.solve_quadrupoles = 0
+        d.solve_quadrupoles = int(bool(solve_quadrupoles))
+        d.quad_idxs = self._qidx
+        d.num_quad_idxs = len(quad_idxs_list)
         d.precision = prec
         d.refined = int(refined)
         d.batched = int(bool(batched))
@@ -220,8 +224,20 @@ class dipole_solver:
         check(C.load().mpm_dipole_solver_get_dipoles(self._h, int(bool(physical)), C.dptr(out.view(np.float64))))
         return np.squeeze(out)
 
+    def get_num_quads(self):
+        v = C._i64()
+        check(C.load().mpm_dipole_solver_get_num_quads(self._h, ctypes.byref(v)))
+        return int(v.value)
+
     def get_quadrupoles(self, physical=True):
-        raise NotImplementedError("quadrupoles are not provided by this build yet")
+        """Quadrupoles (F, W, Q, K, 5) of the quadrupole subset, squeezed; the five
+        components are (Qxx, Qxy, Qxz, Qyy, Qyz) of the traceless symmetric tensor,
+        Qzz = -(Qxx + Qyy)."""
+        F, W, N, K = self._dims()
+        out = np.zeros((F, W, self.get_num_quads(), K, 5), np.complex128)
+        check(C.load().mpm_dipole_solver_get_quadrupoles(self._h, int(bool(physical)),
+                                                         C.dptr(out.view(np.float64))))
+        return np.squeeze(out)
 
     def get_cap_dip(self):
         """Deprecated. Use get_dipoles and get_eff_polarizability instead."""

@@ -189,8 +189,10 @@ typedef struct {
     int field_type;           /* mpm_field_type */
     const double* E0;         /* host complex [K][E0_len] (copied); K = 0 => identity (3 RHS) */
     int64_t num_E0;           /* K */
-    int64_t E0_len;           /* 3 or 3N */
-    int solve_quadrupoles;    /* must be 0 in this build */
+    int64_t E0_len;           /* 3, 3N or 3N + 5Q (quadrupole rows zero unless given) */
+    int solve_quadrupoles;    /* 1 => unknowns [3N dipoles | 5Q quadrupoles] (monodisperse field only) */
+    const int* quad_idxs;     /* host [num_quad_idxs] particles carrying quadrupoles (copied); */
+    int64_t num_quad_idxs;    /*   0 with solve_quadrupoles => every particle (Q = N)            */
     int precision;            /* mpm_precision: DOUBLE => MPM_SOLVE_DOUBLE, AUTO/MIXED => MPM_SOLVE_MIXED */
     int refined;              /* 1 => MPM_SOLVE_REFINED instead of MPM_SOLVE_MIXED (ignored for DOUBLE) */
     int batched;              /* 0 = reference order (one RHS at a time); 1 = all W x K RHS in one solve */
@@ -213,6 +215,11 @@ mpm_status mpm_dipole_solver_get_eff_polarizability(const mpm_dipole_solver* s, 
                                                     double* out);
 /* get_dipoles: host complex [F][W][N][K][3] (dipole_solver.cu:627-635). */
 mpm_status mpm_dipole_solver_get_dipoles(const mpm_dipole_solver* s, int physical, double* out);
+/* Number Q of particles carrying quadrupoles (0 unless solve_quadrupoles; get_num_quads). */
+mpm_status mpm_dipole_solver_get_num_quads(const mpm_dipole_solver* s, int64_t* num_quads);
+/* get_quadrupoles: host complex [F][W][Q][K][5], physical units scale by eps_m a_min^4
+ * (dipole_solver.cu:640-647). */
+mpm_status mpm_dipole_solver_get_quadrupoles(const mpm_dipole_solver* s, int physical, double* out);
 /* Per-solve reports, [F][W][K] (this build's addition: the reference reports none). */
 mpm_status mpm_dipole_solver_get_reports(const mpm_dipole_solver* s, mpm_solve_report* out);
 /* Ewald operator of the solver (valid after the first compute), for inspection. */

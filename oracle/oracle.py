@@ -251,15 +251,18 @@ def self_coefficient(eps, radius=1.0):
 
 
 class DipoleSolver:
-    """Restatement of Dipole_Solver for one or more frames (monodisperse Ewald, dipoles).
+    """Restatement of Dipole_Solver for one or more frames (Ewald field, dipoles and optional
+    quadrupoles).
 
     eps_p: (W, N) complex in the medium's units (divided by eps_m here);
-    E0: (K, 3) or (K, 3N) incident fields.  Physical outputs as the reference.
+    E0: (K, 3), (K, 3N) or (K, 3N + 5Q) incident fields; quadrupoles: False, True (every
+    particle) or the indices of the particles that carry one (dipole_solver.cu:131-182).
+    Physical outputs as the reference.
     """
 
     def __init__(self, box, eps_p, radius, eps_m=1.0, xi=0.5, tol=1e-3, guess_type="derivative",
                  solver_type="gmres", E0=None, restart=50, maxiter=1000, jacobi=True,
-                 grid=None, P=None, ewald_tol=None, field_type="auto"):
+                 grid=None, P=None, ewald_tol=None, field_type="auto", quadrupoles=False):
         self.box = np.asarray(box, np.float64)
         self.eps_p = np.asarray(eps_p, np.complex128)
         self.radius = np.asarray(radius, np.float64)  # scalar or one per particle
@@ -277,6 +280,11 @@ class DipoleSolver:
         self.P = P
         self.ewald_tol = ewald_tol
         self.op = None
+        if isinstance(quadrupoles, bool):
+            self.quad_idxs = None if quadrupoles else []
+        else:
+            self.quad_idxs = [int(i) for i in quadrupoles]
+        self.quads = []      # per frame (W, Q, K, 5)
         self.dips = []       # per frame (W, N, K, 3)
         self.alpha = []      # per frame (W, K, 3)
         self.reports = []
@@ -300,23 +308,40 @@ class DipoleSolver:
             self.op = PolyOperator(self.L, self.ewald_tol or self.tol, self.xi, self.a, self.grid, self.P)
         else:
             self.op = Operator(self.L, self.ewald_tol or self.tol, self.xi, self.grid, self.P)
+        if self.quad_idxs is None or self.quad_idxs:
+            if poly:
+                raise NotImplementedError("quadrupoles with the polydisperse field")
+            self.quad_owner = np.arange(N) if self.quad_idxs is None else np.asarray(self.quad_idxs)
+            self.quad_set = True
 
     def _rhs(self, N):
+        """(K, n) right-hand sides; dipole_solver.cu:368-390 (quadrupole rows zero unless given)."""
         K = self.E0.shape[0]
-        out = np.zeros((K, N, 3), np.complex128)
+        n = self.op.n
+        out = np.zeros((K, n), np.complex128)
         for k in range(K):
-            e = self.E0[k]
-            out[k] = np.broadcast_to(e, (N, 3)) if e.size == 3 else e.reshape(N, 3)
+            e = self.E0[k].reshape(-1)
+            if e.size == 3:
+                out[k, :3 * N] = np.tile(e, N)
+            elif e.size == 3 * N:
+                out[k, :3 * N] = e
+            elif e.size == n:
+                out[k] = e
+            else:
+                raise RuntimeError(f"Incident field E0[{k}] has size {e.size}")
         return out
 
     def _mean_field(self, w, N):
-        K = self.E0.shape[0]
+        """dipole_solver.cu:244-261: alpha_MF E0 on the dipoles, zero quadrupoles."""
         beta = (self.eps[w] - 1.0) / (self.eps[w] + 2.0)
         val = 4.0 * np.pi * self.a ** 3 * beta / (1.0 - beta * self.vol_frac)  # per particle
-        rhs = self._rhs(N)  # (K, N, 3)
-        return val[None, :, None] * rhs
+        g = self._rhs(N).copy()
+        g[:, :3 * N] *= np.repeat(val, 3)[None, :]
+        g[:, 3 * N:] = 0.0
+        return g
 
     def _guess(self, w, prev, N):
+        """dipole_solver.cu:233-345 on (K, n) vectors; a quadrupole follows its particle's eps."""
         if self.guess_type in ("mean_field", "mean-field") or w < 1:
             return self._mean_field(w, N)
         if self.guess_type == "previous" or w < 2:
@@ -331,31 +356,42 @@ class DipoleSolver:
         small = np.abs(run) < 1e-12
         run = np.where(small, 1.0, run)
         new_run = np.where(small, 0.0, new_run)
+        ratio = new_run / run  # per particle
+        per_row = np.repeat(ratio, 3)
+        if self.op.Q:
+            per_row = np.concatenate([per_row, np.repeat(ratio[self.quad_owner], 5)])
         rise = prev[w - 1] - prev[w - 2]
-        return prev[w - 1] + (new_run / run)[None, :, None] * rise
+        return prev[w - 1] + per_row[None, :] * rise
 
     def compute(self, positions):
         pos = np.asarray(positions, np.float64).reshape(-1, 3)
         N = pos.shape[0]
         if self.op is None:
             self._setup(N)
+            if getattr(self, "quad_set", False):
+                self.op.set_positions(pos / self.length_scale)
+                self.op.set_quads(self.quad_idxs)
         self.op.set_positions(pos / self.length_scale)
         K = self.E0.shape[0]
         rhs = self._rhs(N)
-        frame = np.zeros((self.W, K, N, 3), np.complex128)
+        n = self.op.n
+        frame = np.zeros((self.W, K, n), np.complex128)
         reps = []
         for w in range(self.W):
             sc = self_coefficient(self.eps[w], self.a)
             if w == 0 and self.dips:
-                guess = np.transpose(self.dips[-1][0], (1, 0, 2))  # previous frame, omega 0
+                guess = self._last_first  # previous frame, omega 0
             else:
                 guess = self._guess(w, frame, N)
             for k in range(K):
-                x, rep = self.op.solve(rhs[k].reshape(-1), sc, guess[k].reshape(-1), self.tol,
+                x, rep = self.op.solve(rhs[k], sc, guess[k], self.tol,
                                        self.solver_type, self.restart, self.maxiter, self.jacobi)
-                frame[w, k] = x.reshape(N, 3)
+                frame[w, k] = x.reshape(-1)
                 reps.append(rep)
-        dips = np.transpose(frame, (0, 2, 1, 3))  # (W, N, K, 3)
+        self._last_first = frame[0].copy()
+        dips = np.transpose(frame[:, :, :3 * N].reshape(self.W, K, N, 3), (0, 2, 1, 3))  # (W, N, K, 3)
+        Q = self.op.Q
+        self.quads.append(np.transpose(frame[:, :, 3 * N:].reshape(self.W, K, Q, 5), (0, 2, 1, 3)))
         self.dips.append(dips)
         self.alpha.append(dips.mean(axis=1))
         self.reports.append(reps)
@@ -367,6 +403,11 @@ class DipoleSolver:
     def get_dipoles(self, physical=True):
         f = self.eps_m * self.length_scale ** 3 if physical else 1.0
         return np.squeeze(np.stack(self.dips) * f)
+
+    def get_quadrupoles(self, physical=True):
+        """(F, W, Q, K, 5), squeezed; physical units scale by eps_m a_min^4 (dipole_solver.cu:640-647)."""
+        f = self.eps_m * self.length_scale ** 4 if physical else 1.0
+        return np.squeeze(np.stack(self.quads) * f)
 
 
 class PolyOperator(Operator):

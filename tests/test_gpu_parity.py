@@ -601,3 +601,110 @@ def test_quad_solve_matches_oracle(precision):
         xr, rr = ref.solve(b[q], sc, tol=1e-10)
         assert reps[q]["converged"] and rr["converged"]
         assert rel(x[q], xr) < 1e-6
+
+
+@pytest.mark.parametrize("precision,batched", [("double", False), ("refined", True)])
+def test_dipole_solver_quadrupole_fixture(precision, batched):
+    """Reference tests/test_dipole_solvers.py:63-148 through the product API (its bar 1e-5)."""
+    import cumpm_amd
+    _case(f"driver quad fixture {precision} batched={batched}")
+    ref = np.load(os.path.join(GOLDEN, "ref_quadrupoles_monodisperse.npz"))
+    s = cumpm_amd.dipole_solver(np.array([100.0] * 3), 4.5 + 0.2j, radius=1.0, eps_m=1.0, xi=0.4, tol=1e-8,
+                                field_type="monodisperse", quiet=True, quadrupoles=True, precision=precision,
+                                batched=batched)
+    s.compute(np.array([[10.0, 10.0, 10.0], [20.0, 20.0, 20.0]]))
+    dips, quads = s.get_dipoles(), s.get_quadrupoles()
+    assert dips.shape == (2, 3, 3) and quads.shape == (2, 3, 5)
+    for dim in range(3):
+        np.testing.assert_allclose(dips[:, dim], ref["dips_py_all"][dim], rtol=1e-5, atol=1e-5)
+        np.testing.assert_allclose(quads[:, dim], ref["quads_py_all"][dim], rtol=1e-5, atol=1e-5)
+
+
+def _quad_driver_case(a=1.0, eps_m=1.0):
+    L, pos = _quad_geom()
+    omega = np.linspace(0.3, 0.7, 4)
+    eps = (4.0 - 1.7 ** 2 / (omega ** 2 + 1j * omega * 0.08)) * eps_m
+    pos2 = pos + np.random.default_rng(11).normal(0.0, 0.05, pos.shape)
+    return L * a, np.stack([pos, pos2]) * a, eps
+
+
+@pytest.mark.parametrize("quads,batched,precision", [(True, False, "double"), ([0, 3, 7], False, "double"),
+                                                     ([0, 3, 7], True, "refined")])
+def test_dipole_solver_quadrupoles_match_oracle_driver(quads, batched, precision):
+    """Two frames, four frequencies, derivative guesses with quadrupoles (dipole_solver.cu:233-345,
+    :578-601) against the oracle's driver; overlapping and close pairs included."""
+    import cumpm_amd
+    _case(f"driver quads={quads} batched={batched} {precision}")
+    L, frames, eps = _quad_driver_case()
+    s = cumpm_amd.dipole_solver(L, eps, radius=1.0, tol=1e-8, quiet=True, quadrupoles=quads, precision=precision,
+                                batched=batched)
+    s.compute(frames)
+    od = O.DipoleSolver(L, eps.reshape(-1, 1), 1.0, tol=1e-8, quadrupoles=quads)
+    for f in frames:
+        od.compute(f)
+    Q = 10 if quads is True else 3
+    assert s.get_num_quads() == Q
+    q = s.get_quadrupoles()
+    assert q.shape == (2, 4, Q, 3, 5)
+    assert rel(s.get_dipoles(), od.get_dipoles()) < 1e-6
+    assert rel(q, od.get_quadrupoles()) < 1e-6
+    assert rel(s.get_eff_polarizability(), od.get_eff_polarizability()) < 1e-6
+    assert all(r["converged"] for fr in s.get_reports() for w in fr for r in w)
+
+
+def test_dipole_solver_quadrupole_units_and_full_rhs():
+    """Physical units: lengths x a and eps x eps_m leave the nondimensional problem unchanged,
+    so dipoles scale by eps_m a^3 and quadrupoles by eps_m a^4 (dipole_solver.cu:627-647).
+    A full-length E0 (3N + 5Q, quadrupole rows nonzero) matches the oracle's driver."""
+    import cumpm_amd
+    _case("driver quad units / full rhs")
+    out = []
+    for a, eps_m in ((1.0, 1.0), (2.0, 1.5)):
+        L, frames, eps = _quad_driver_case(a, eps_m)
+        s = cumpm_amd.dipole_solver(L, eps[:2], radius=a, eps_m=eps_m, tol=1e-8, quiet=True, quadrupoles=[2, 5],
+                                    precision="double")
+        s.compute(frames[0])
+        out.append((s.get_dipoles(), s.get_quadrupoles()))
+    assert rel(out[1][0], out[0][0] * 1.5 * 2.0 ** 3) < 1e-9
+    assert rel(out[1][1], out[0][1] * 1.5 * 2.0 ** 4) < 1e-9
+    L, frames, eps = _quad_driver_case()
+    rng = np.random.default_rng(4)
+    E0 = rng.standard_normal((2, 45)) + 1j * rng.standard_normal((2, 45))
+    s = cumpm_amd.dipole_solver(L, eps[:2], radius=1.0, tol=1e-8, quiet=True, quadrupoles=[2, 5, 8], E0=E0,
+                                precision="double")
+    s.compute(frames[0])
+    od = O.DipoleSolver(L, eps[:2].reshape(-1, 1), 1.0, tol=1e-8, quadrupoles=[2, 5, 8], E0=E0)
+    od.compute(frames[0])
+    assert rel(s.get_dipoles(), od.get_dipoles()) < 1e-6
+    assert rel(s.get_quadrupoles(), od.get_quadrupoles()) < 1e-6
+
+
+def test_dev_field_quadrupoles_match_oracle():
+    """Dev API with quadrupoles: get_epoint_host gives (field (N, 3), gradient (Q, 5)) as the
+    reference's dev/electric_field.py:816-842; real + inverse = total."""
+    from cumpm_amd.dev import MonodisperseEwaldElectricField
+    _case("dev quads")
+    L, pos = _quad_geom()
+    ef = MonodisperseEwaldElectricField(*L, 1e-8, 0.5, True, solve_quadrupoles=True, quad_idxs=[0, 3, 7],
+                                        precision="double")
+    ef.update_particle_coordinates(pos[:, 0], pos[:, 1], pos[:, 2])
+    assert ef.solve_quadrupoles and ef.num_quads == 3
+    rng = np.random.default_rng(2)
+    p = rng.standard_normal((10, 3)) + 1j * rng.standard_normal((10, 3))
+    q = rng.standard_normal((3, 5)) + 1j * rng.standard_normal((3, 5))
+    sc = rng.standard_normal(10) + 1j * rng.standard_normal(10)
+    ef.set_dipoles(p)
+    ef.update_quadrupoles_complex(*[v for c in range(5) for v in (q[:, c].real, q[:, c].imag)])
+    ef.set_self_coef(sc.real, sc.imag)
+    ef.calculate()
+    E, G = ef.get_epoint_host()
+    ref = O.Operator(L, 1e-8, 0.5)
+    ref.set_positions(pos)
+    ref.set_quads([0, 3, 7])
+    y = ref.apply(np.concatenate([p.reshape(-1), q.reshape(-1)]), sc)
+    assert rel(E, y[:30].reshape(10, 3)) < 1e-10
+    assert rel(G, y[30:].reshape(3, 5)) < 1e-10
+    Er, Gr = ef.compute_real_field()
+    Ei, Gi = ef.compute_inverse_field()
+    np.testing.assert_allclose(Er + Ei, E, rtol=1e-12, atol=1e-12 * np.abs(E).max())
+    np.testing.assert_allclose(Gr + Gi, G, rtol=1e-12, atol=1e-12 * np.abs(G).max())

@@ -82,3 +82,43 @@ def test_quadrupole_subset_and_split():
     op.set_quads([0, 3, 7])
     v = np.random.default_rng(1).standard_normal(op.n) + 0j
     np.testing.assert_allclose(op.apply(v, 1.0), op.apply(v, 1.0, 1) + op.apply(v, 1.0, 2), rtol=1e-12, atol=1e-12)
+
+
+def test_oracle_driver_quadrupole_fixture():
+    """The oracle's driver (quadrupoles=True, xi 0.4, tol 1e-8) against the same fixture,
+    as reference tests/test_dipole_solvers.py:63-148 drives cuMPM.dipole_solver (its bar 1e-5).
+    The Ewald accuracy is the fixture's 1e-6 (a coarser grid than tol would give, for time)."""
+    ref = np.load(os.path.join(GOLDEN, "ref_quadrupoles_monodisperse.npz"))
+    s = O.DipoleSolver([100.0] * 3, 4.5 + 0.2j, 1.0, 1.0, xi=0.4, tol=1e-8, field_type="monodisperse",
+                       quadrupoles=True, ewald_tol=1e-6)
+    s.compute(np.array([[10.0, 10.0, 10.0], [20.0, 20.0, 20.0]]))
+    dips, quads = s.get_dipoles(), s.get_quadrupoles()
+    assert dips.shape == (2, 3, 3) and quads.shape == (2, 3, 5)
+    for dim in range(3):
+        np.testing.assert_allclose(dips[:, dim], ref["dips_py_all"][dim], rtol=1e-5, atol=1e-5)
+        np.testing.assert_allclose(quads[:, dim], ref["quads_py_all"][dim], rtol=1e-5, atol=1e-5)
+
+
+def test_oracle_driver_quadrupole_rhs_layouts():
+    """E0 of length 3, 3N and 3N + 5Q give the same solution when they describe the same field
+    (dipole_solver.cu:368-390: quadrupole rows are zero unless given); a full-length E0 with
+    quadrupole rows solves A x = b for that b."""
+    L = [22.0, 23.0, 24.0]
+    pos = np.random.default_rng(5).uniform(0, 20, (6, 3))
+    e3 = np.array([[1.0, 0.5j, 0.2]])
+    outs = []
+    for E0 in (e3, np.tile(e3, (1, 6)), np.concatenate([np.tile(e3, (1, 6)), np.zeros((1, 10))], axis=1)):
+        s = O.DipoleSolver(L, [-2.0 + 0.5j], 1.0, tol=1e-10, quadrupoles=[1, 4], E0=E0)
+        s.compute(pos)
+        outs.append((s.get_dipoles(), s.get_quadrupoles()))
+    for d, q in outs[1:]:
+        np.testing.assert_allclose(d, outs[0][0], rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(q, outs[0][1], rtol=1e-12, atol=1e-14)
+    b = np.random.default_rng(3).standard_normal(28) + 0j
+    s = O.DipoleSolver(L, [-2.0 + 0.5j], 1.0, tol=1e-12, quadrupoles=[1, 4], E0=b[None, :])
+    s.compute(pos)
+    x = np.concatenate([s.get_dipoles(physical=False).reshape(-1), s.get_quadrupoles(physical=False).reshape(-1)])
+    op = O.Operator(L, 1e-12, 0.5)
+    op.set_positions(pos)
+    op.set_quads([1, 4])
+    assert np.abs(op.apply(x, O.self_coefficient(np.full(6, -2.0 + 0.5j))) - b).max() < 1e-9 * np.abs(b).max()
