@@ -22,14 +22,20 @@
 //                       (unique radii), 190-350 (finite-sphere spread/contract weights),
 //                       565-640 (scalar k-space coefficient), 654-828 (contract, self,
 //                       real space), 1434-1698 (pair functions and tables), 2039-2105 (sizing)
+//   quadrupoles         monodisperse_ewald_electric_field.cu:199-470 (seven pair tables and
+//                       the gradient self term), 1301-1487 (joint k-space coupling),
+//                       1631-1869 (real-space couplings); the cross-term signs follow the
+//                       reference's Python formulation tests/quadrupole_field.py:214-215
 // The FFT is this file's own mixed-radix + Bluestein transform (the reference
 // uses cuFFT Z2Z, unnormalised, forward e^{-i}, inverse e^{+i}).
 //
-// Pinning (see tests/test_oracle*.py): golden fixture
-// tests/data/ref_dipole_monodisperse.npz, numpy.fft, a textbook point-dipole
-// Ewald lattice sum (non-overlapping spheres), quadrature of the
-// overlapping-sphere field (1 <= r < 2), xi-independence of real+recip, and a
-// dense np.linalg.solve for the Krylov solvers.
+// Pinning (see tests/test_oracle*.py): golden fixtures
+// tests/golden/ref_dipole_monodisperse.npz and ref_quadrupoles_monodisperse.npz,
+// numpy.fft, a textbook point-dipole Ewald lattice sum (non-overlapping spheres),
+// quadrature of the overlapping-sphere field (1 <= r < 2), xi-independence of
+// real+recip (every dipole and quadrupole coupling), and a dense
+// np.linalg.solve for the Krylov solvers.  Parity unpinned: the overlap
+// polynomials (r < 2) of the seven quadrupole tables.
 
 #include <algorithm>
 #include <cmath>

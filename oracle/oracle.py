@@ -9,8 +9,9 @@ Besides thin wrappers over the C++ operator and Krylov solvers, this module
 restates the reference's ``Dipole_Solver`` driver
 (src/cpp/dipole_solver.cu:131-607): nondimensionalisation (:184-205), volume
 fraction (:207-221), the mean-field / previous / derivative initial guesses
-(:245-346), the frequency x polarisation loops (:361-489) and the physical
-rescaling of the outputs (:609-647).
+(:245-346), the frequency x polarisation loops (:361-489), the quadrupole
+unknowns, right-hand sides and guesses (:131-182, :233-345, :361-435) and the
+physical rescaling of the outputs (:609-647).
 """
 from __future__ import annotations
 
