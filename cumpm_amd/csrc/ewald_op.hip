@@ -1769,6 +1769,8 @@ __global__ __launch_bounds__(256) void scale_kernel(typename C2<R>::T* __restric
 // sum order is fixed (deterministic).
 constexpr int kMfmaRY = 16;    // y voxels per region (the MFMA N dimension)
 constexpr size_t kMaxFusedLds = 64 * 1024;  // LDS of one xtoeplitz_kernel block
+// Radius classes of the polydisperse operator up to which pair tables are built.
+constexpr int kPolyTableClasses = 64;
 constexpr int kMfmaKC = 64;    // candidates staged per chunk
 
 typedef float mpm_f32x4 __attribute__((ext_vector_type(4)));
@@ -3394,14 +3396,40 @@ EwaldOp::EwaldOp(const mpm_op_desc& d, const double* radii, int64_t nr) : EwaldO
     poly = true;
     force_full = 1;
     rad_h.assign(radii, radii + nr);
+    // radius classes: merged within 1e-5 in input order (polydisperse...cu:40-83)
     for (double r : rad_h) {
         MPM_REQUIRE(std::isfinite(r) && r > 0.0, MPM_ERR_INVALID_ARGUMENT, "radii must be positive");
+        if (poly_exact) continue;
         bool seen = false;
         for (double u : uniq_h) seen = seen || std::abs(u - r) < 1e-5;
         if (!seen) uniq_h.push_back(r);
+        poly_exact = uniq_h.size() > (size_t)kPolyTableClasses;
+    }
+    const double amax = *std::max_element(rad_h.begin(), rad_h.end());
+    compute_sizing_poly(L, errortol, xi, amax, d.grid, d.support, S);
+    peta = S.eta[0];
+    poff = -S.half;
+    V = (size_t)S.n[0] * S.n[1] * S.n[2];
+    if (poly_exact) {
+        // U (U + 1) / 2 tables of ~(rc - 1) / 0.001 samples each would not fit; evaluate
+        // every pair at its own radii instead, and count the classes for the record
+        std::vector<double> sr(rad_h);
+        std::sort(sr.begin(), sr.end());
+        poly_classes = 1;
+        for (size_t i = 1, k = 0; i < sr.size(); ++i)
+            if (sr[i] - sr[k] >= 1e-5) {
+                ++poly_classes;
+                k = i;
+            }
+        uniq_h.clear();
+        pself_h.resize(nr);
+        for (int64_t i = 0; i < nr; ++i) pself_h[i] = poly_self_term(rad_h[i], rad_h[i], xi);
+        self_perp = poly_self_term(sr[0], sr[0], xi);
+        build_kspace();
+        return;
     }
     std::sort(uniq_h.begin(), uniq_h.end());
-    MPM_REQUIRE(uniq_h.size() <= 64, MPM_ERR_UNSUPPORTED, "more than 64 distinct radii");
+    poly_classes = (int)uniq_h.size();
     ridx_h.assign(nr, 0);
     for (int64_t i = 0; i < nr; ++i)
         for (size_t u = 0; u < uniq_h.size(); ++u)
@@ -3409,10 +3437,6 @@ EwaldOp::EwaldOp(const mpm_op_desc& d, const double* radii, int64_t nr) : EwaldO
                 ridx_h[i] = (int)u;
                 break;
             }
-    compute_sizing_poly(L, errortol, xi, uniq_h.back(), d.grid, d.support, S);
-    peta = S.eta[0];
-    poff = -S.half;
-    V = (size_t)S.n[0] * S.n[1] * S.n[2];
     // tables of every radius pair: entry 0 the pair's self value, then r = 1 + 0.001 k up to rc
     const double rcv = S.rc < 1.001 ? 1.001 : S.rc;
     const size_t steps = (size_t)std::ceil((rcv - 1.0) / 0.001);
@@ -3435,7 +3459,30 @@ EwaldOp::EwaldOp(const mpm_op_desc& d, const double* radii, int64_t nr) : EwaldO
                                     pt_para[(k + 1) * pt_ncol + col]);
         }
     self_perp = uself_h[0];
+    pself_h.resize(nr);
+    for (int64_t i = 0; i < nr; ++i) pself_h[i] = uself_h[ridx_h[i]];
     build_kspace();
+}
+
+// Pair values of particles i, j at distance r (>= 1e-6): the radius classes' tables, or
+// with more classes than tables allow the closed forms at the pair's own radii, with
+// the tables' rule below r = 1 (linear between the r -> 0 value and the value at 1).
+void EwaldOp::poly_pair_value(double r, int i, int j, double& perp, double& para) const {
+    if (!poly_exact) {
+        perp = poly_lookup(pt_perp, r, ridx_h[i], ridx_h[j]);
+        para = poly_lookup(pt_para, r, ridx_h[i], ridx_h[j]);
+        return;
+    }
+    const double a = rad_h[i], b = rad_h[j];
+    if (r >= 1.0) {
+        poly_pair_functions(r, a, b, xi, perp, para);
+        return;
+    }
+    const double s0 = poly_self_term(a, b, xi);
+    double p1, q1;
+    poly_pair_functions(1.0, a, b, xi, p1, q1);
+    perp = s0 * (1.0 - r) + p1 * r;
+    para = s0 * (1.0 - r) + q1 * r;
 }
 
 // Table value of radius classes (u, v) at distance r: pair column of the upper
@@ -3500,7 +3547,7 @@ void EwaldOp::setup_poly_stencils(const std::vector<double>& pos, int Nn) {
     }
     std::vector<float> pw32(pw.begin(), pw.end());
     std::vector<double> sp(Nn);
-    for (int i = 0; i < Nn; ++i) sp[i] = uself_h[ridx_h[i]];
+    for (int i = 0; i < Nn; ++i) sp[i] = pself_h[i];
     d_pw64.upload(pw.data(), pw.size());
     d_pw32.upload(pw32.data(), pw32.size());
     d_pidx.upload(pidx.data(), pidx.size());
@@ -3924,8 +3971,9 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         nj.push_back(j);
         if (poly) {  // d_eff = max(d, 1e-6) and the radius pair's table
             const double de = dist < 1e-6 ? 1e-6 : dist;
-            nf.push_back(make_double2(poly_lookup(pt_perp, de, ridx_h[i], ridx_h[j]),
-                                      poly_lookup(pt_para, de, ridx_h[i], ridx_h[j])));
+            double fp, fa;
+            poly_pair_value(de, i, j, fp, fa);
+            nf.push_back(make_double2(fp, fa));
         } else {
             const double de = dist < 1.0 ? 1.0 : dist;
             nf.push_back(make_double2(tab.interp(tab.perp, de), tab.interp(tab.para, de)));
@@ -4505,7 +4553,7 @@ void EwaldOp::info(mpm_op_info& o) const {
     o.fft_backend = (lfft && fft_backend == 0) ? 0 : 1;
     o.polydisperse = poly ? 1 : 0;
     o.num_quads = (int64_t)qidx_h.size();
-    o.num_unique_radii = poly ? (int)uniq_h.size() : 1;
+    o.num_unique_radii = poly ? poly_classes : 1;
 }
 
 }  // namespace mpm

@@ -802,6 +802,9 @@ static void gmres_group(Ctx& C, const double2* b, double2* x, int B, mpm_solve_r
 // ---------------------------------------------------------------------------
 // BiCGSTAB (host scalars, batched vector work)
 // ---------------------------------------------------------------------------
+// Refinement cycles of the refined-precision BiCGSTAB (each restarts from the iterate).
+constexpr int kMaxBicgCycles = 20;
+
 static void bicgstab_group(Ctx& C, const double2* b, double2* x, int B, mpm_solve_report* rep) {
     Work& W = C.W;
     const size_t n = C.n;
@@ -894,13 +897,23 @@ static void bicgstab_group(Ctx& C, const double2* b, double2* x, int B, mpm_solv
         C.read_scalars(W.nrm.p, all, bn);
         for (int q = 0; q < B; ++q) bnorm[q] = std::sqrt(bn[q]);
     }
-    std::vector<int> active;
+    std::vector<int> active, pending;
     for (int q = 0; q < B; ++q) {
         if (bnorm[q] == 0.0)
             MPM_HIP(hipMemsetAsync(x + q * n, 0, n * sizeof(double2), C.s));
         else
-            active.push_back(q);
+            pending.push_back(q);
     }
+    // Refined precision: FP32 applies inside the iteration, and a system whose recurrence
+    // residual met the tolerance restarts from its iterate unless the FP64 residual
+    // M^-1 (b - A x) also does (iterative refinement with BiCGSTAB as the inner solver).
+    // DOUBLE and MIXED run one pass, as the reference.
+    const bool refine = fp32 && !outer32;
+    std::vector<char> by_recurrence(B, 0);
+    int it_total = 0;
+    for (int cycle = 0; !pending.empty(); ++cycle) {
+    active.swap(pending);
+    pending.clear();
     // r = M^-1 (b - A x0)
     if (!active.empty()) {
         std::vector<const double2*> xs;
@@ -931,7 +944,8 @@ static void bicgstab_group(Ctx& C, const double2* b, double2* x, int B, mpm_solv
         MPM_HIP(hipMemsetAsync(vec(W.vb, q), 0, n * sizeof(double2), C.s));
     }
     std::vector<cplx> rho(B, 1.0), alpha(B, 1.0), omega(B, 1.0);
-    for (int it = 0; it < d.maxiter && !active.empty(); ++it) {
+    for (int q = 0; q < B; ++q) by_recurrence[q] = 0;
+    for (int it = 0; it_total < d.maxiter && !active.empty(); ++it, ++it_total) {
         std::vector<cplx> rho_new;
         dots_of(active, W.r0b, W.r, 1, rho_new);
         std::vector<int> go;
@@ -971,6 +985,7 @@ static void bicgstab_group(Ctx& C, const double2* b, double2* x, int B, mpm_solv
         norms_of(active, W.sb, sn);
         std::vector<int> cont, stop_s;
         for (int q : active) (sn[q] < tol * bnorm[q] ? stop_s : cont).push_back(q);
+        for (int q : stop_s) by_recurrence[q] = 1;
         if (!stop_s.empty()) {  // x += alpha p
             Ptrs pa = make_ptrs(stop_s, [&](int q) { return vec(W.pb, q); }, [&](int q) { return x + q * n; });
             hipLaunchKernelGGL(axpy_kernel, grid_for(n, (int)stop_s.size()), dim3(256), 0, C.s, pa, n, W.alpha.p);
@@ -999,11 +1014,19 @@ static void bicgstab_group(Ctx& C, const double2* b, double2* x, int B, mpm_solv
         norms_of(active, W.r, rn);
         go.clear();
         for (int q : active) {
-            if (rn[q] < tol * bnorm[q]) continue;
+            if (rn[q] < tol * bnorm[q]) {
+                by_recurrence[q] = 1;
+                continue;
+            }
             if (std::abs(omega[q]) < 1e-16) continue;
             go.push_back(q);
         }
         active.swap(go);
+    }
+    // systems stopped by their recurrence residual: the next cycle checks them in FP64
+    if (refine && it_total < d.maxiter && cycle + 1 < kMaxBicgCycles)
+        for (int q = 0; q < B; ++q)
+            if (by_recurrence[q]) pending.push_back(q);
     }
     // final FP64 residuals (diagnostic)
     std::vector<int> need;

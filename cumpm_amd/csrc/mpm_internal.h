@@ -258,6 +258,11 @@ class EwaldOp {
     std::vector<double> pt_perp, pt_para;  // [entry][pair column]
     std::vector<double> uself_h;     // real-space self term per unique radius
     int pt_ncol = 0;
+    // more radius classes than kPolyTableClasses: no tables, the pair functions of
+    // each pair's own radii in closed form (the tables' limit without interpolation)
+    bool poly_exact = false;
+    int poly_classes = 0;            // distinct radii (merged within 1e-5)
+    std::vector<double> pself_h;     // [N] real-space self term of each particle
     double peta = 0;                 // scalar spectral split of the finite-sphere kernel
     int poff = 0;                    // first stencil offset, -floor((P - 1) / 2)
     DevBuf<double> d_selfp;          // [N] real-space self term of each particle
@@ -267,6 +272,7 @@ class EwaldOp {
     DevBuf<int> d_vox_start;         // [V + 1] per voxel, its (particle, point) entries
     DevBuf<int> d_vox_items;         //   entry = particle * P^3 + point, ascending
     double poly_lookup(const std::vector<double>& tab, double r, int u, int v) const;
+    void poly_pair_value(double r, int i, int j, double& perp, double& para) const;
     void setup_poly_stencils(const std::vector<double>& pos, int Nn);
 
     friend struct OpAccess;

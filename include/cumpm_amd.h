@@ -48,8 +48,10 @@ typedef enum {
 typedef enum {
     MPM_SOLVE_DOUBLE = 0,  /* every apply FP64 (reference precision="double") */
     MPM_SOLVE_MIXED = 1,   /* every apply FP32-reciprocal (reference AUTO/MIXED) */
-    MPM_SOLVE_REFINED = 2  /* FP32-reciprocal Arnoldi applies and initial residual; FP64 residual b - Ax at
-                              the end of every cycle decides convergence */
+    MPM_SOLVE_REFINED = 2  /* GMRES: FP32-reciprocal Arnoldi applies and initial residual; FP64 residual
+                              b - Ax at the end of every cycle decides convergence.  BiCGSTAB: FP32-
+                              reciprocal iteration; a system whose recurrence residual meets tol restarts
+                              from its iterate until the FP64 residual does (at most 20 passes) */
 } mpm_solve_precision;
 
 typedef enum { MPM_GMRES = 0, MPM_BICGSTAB = 1 } mpm_method;

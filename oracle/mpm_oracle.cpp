@@ -684,6 +684,9 @@ struct Op {
     std::vector<int> ridx;
     std::vector<double> pr, pperp, ppara, uself;  // r samples, tables [entry][col], self per unique radius
     int ncol = 0;
+    // more than 64 radius classes: no tables, each pair's closed forms at its own radii
+    bool exact = false;
+    std::vector<double> pself;  // [N] self term of each particle
     std::vector<double> pw;  // [i][o][3] spread/contract weight vectors
     // quadrupoles (monodisperse): particles qidx[q] carry 5 traceless components
     // (xx, xy, xz, yy, yz) after the 3N dipole unknowns; qmap[i] = q or -1
@@ -892,8 +895,22 @@ static void build_neighbors(Op& op) {
             double d = std::sqrt(rx * rx + ry * ry + rz * rz);
             if (d < op.S.rc && op.poly) {
                 const double de = d < 1e-6 ? 1e-6 : d;
-                op.nb_perp[s + k] = poly_interp(op, de, op.ridx[i], op.ridx[j], op.pperp);
-                op.nb_para[s + k] = poly_interp(op, de, op.ridx[i], op.ridx[j], op.ppara);
+                if (!op.exact) {
+                    op.nb_perp[s + k] = poly_interp(op, de, op.ridx[i], op.ridx[j], op.pperp);
+                    op.nb_para[s + k] = poly_interp(op, de, op.ridx[i], op.ridx[j], op.ppara);
+                } else {
+                    // closed forms at the pair's radii; below r = 1 the tables' linear rule
+                    const double a = op.radius[i], b = op.radius[j];
+                    double fp, fa;
+                    poly_pair_functions(de < 1.0 ? 1.0 : de, a, b, op.xi, fp, fa);
+                    if (de < 1.0) {
+                        const double s0 = poly_self(a, b, op.xi);
+                        fp = s0 * (1.0 - de) + fp * de;
+                        fa = s0 * (1.0 - de) + fa * de;
+                    }
+                    op.nb_perp[s + k] = fp;
+                    op.nb_para[s + k] = fa;
+                }
             } else if (d < op.S.rc) {
                 double de = d < 1.0 ? 1.0 : d;
                 op.nb_perp[s + k] = interp(de, op.T.r, op.T.perp);
@@ -1076,7 +1093,7 @@ static void apply(const Op& op, const cplx* self_coef, const cplx* p, cplx* E, i
     }
     if (parts & 1) {
         for (int i = 0; i < N; ++i) {
-            const cplx f = self_coef[i] + (op.poly ? op.uself[op.ridx[i]] : op.T.self_perp);
+            const cplx f = self_coef[i] + (op.poly ? op.pself[i] : op.T.self_perp);
             for (int c = 0; c < 3; ++c) E[3 * i + c] += f * p[3 * i + c];
         }
         for (int q = 0; q < Q; ++q) {  // :1615-1627
@@ -1416,9 +1433,25 @@ void* orc_poly_create(const double* L, double errortol, double xi, const double*
     op->poly = true;
     op->radius.assign(radii, radii + N);
     for (double r : op->radius) {
+        if (op->exact) break;
         bool seen = false;
         for (double u : op->uniq) seen = seen || std::abs(u - r) < 1e-5;
         if (!seen) op->uniq.push_back(r);
+        op->exact = op->uniq.size() > 64;
+    }
+    if (op->exact) {
+        const double amax = N > 0 ? *std::max_element(op->radius.begin(), op->radius.end()) : 1.0;
+        std::string err;
+        if (compute_sizing_poly(L, errortol, xi, amax, grid_override, P_override, op->S, err)) {
+            if (errbuf && errlen > 0) std::snprintf(errbuf, errlen, "%s", err.c_str());
+            delete op;
+            return nullptr;
+        }
+        op->uniq.clear();
+        op->pself.resize(N);
+        for (int i = 0; i < N; ++i) op->pself[i] = poly_self(op->radius[i], op->radius[i], xi);
+        compute_scale(*op);
+        return op;
     }
     std::sort(op->uniq.begin(), op->uniq.end());
     op->ridx.assign(N, 0);
@@ -1435,6 +1468,8 @@ void* orc_poly_create(const double* L, double errortol, double xi, const double*
         return nullptr;
     }
     build_poly_tables(*op);
+    op->pself.resize(N);
+    for (int i = 0; i < N; ++i) op->pself[i] = op->uself[op->ridx[i]];
     compute_scale(*op);
     return op;
 }
@@ -1520,7 +1555,12 @@ void orc_op_info(void* h, int* n, double* h_out, double* eta, int* P, int* width
     *P = op.S.P;
     *width = op.S.width;
     *rc = op.S.rc;
-    *self_perp = op.poly ? op.uself[0] : op.T.self_perp;
+    if (op.poly && op.exact) {
+        const double amin = *std::min_element(op.radius.begin(), op.radius.end());
+        *self_perp = poly_self(amin, amin, op.xi);
+    } else {
+        *self_perp = op.poly ? op.uself[0] : op.T.self_perp;
+    }
     *num_pairs = op.N ? op.nb_start[op.N] : 0;
 }
 

@@ -31,18 +31,18 @@ def point_dipole_field(pos, p, box, alpha=None, real_images=3, kmax_factor=6.5):
         alpha = 5.5 / L.min()
     E = np.zeros((N, 3), complex)
     rng = range(-real_images, real_images + 1)
+    shifts = np.array(list(itertools.product(rng, rng, rng)), float) * L  # (S, 3)
     for i in range(N):
-        for j in range(N):
-            for n in itertools.product(rng, rng, rng):
-                R = pos[i] - pos[j] + np.array(n) * L
-                d = np.linalg.norm(R)
-                if i == j and d == 0.0:
-                    continue
-                ar = alpha * d
-                g = 2.0 * ar / np.sqrt(np.pi) * np.exp(-ar * ar)
-                B = (erfc(ar) + g) / d ** 3
-                C = (3.0 * erfc(ar) + g * (3.0 + 2.0 * ar * ar)) / d ** 5
-                E[i] += -B * p[j] + C * np.dot(R, p[j]) * R
+        R = (pos[i] - pos)[:, None, :] + shifts[None, :, :]  # (N, S, 3)
+        d = np.linalg.norm(R, axis=-1)
+        keep = d > 0.0  # drops only the own dipole (i == j, zero shift)
+        ds = np.where(keep, d, 1.0)
+        ar = alpha * ds
+        g = 2.0 * ar / np.sqrt(np.pi) * np.exp(-ar * ar)
+        B = np.where(keep, (erfc(ar) + g) / ds ** 3, 0.0)
+        C = np.where(keep, (3.0 * erfc(ar) + g * (3.0 + 2.0 * ar * ar)) / ds ** 5, 0.0)
+        Rp = np.einsum("jsc,jc->js", R, p)
+        E[i] += -(B.sum(axis=1)[:, None] * p).sum(axis=0) + np.einsum("js,jsc->c", C * Rp, R)
     # reciprocal space, k != 0
     kmax = kmax_factor * alpha * 2.0
     m = [int(np.ceil(kmax * L[d] / (2 * np.pi))) for d in range(3)]

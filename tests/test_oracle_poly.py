@@ -126,3 +126,48 @@ def test_sizing_rule():
     h = np.array([30.0, 31.0, 32.0]) / n
     assert info["P"] == int(np.ceil(2 / h.mean() - 2 * np.log(1e-6) / np.pi))
     assert info["eta"][0] == pytest.approx(info["P"] * (h.mean() * 0.5) ** 2 / np.pi)
+
+
+def _continuous_radii_lattice(n=70, seed=4):
+    """n spheres of n distinct radii in [1, 2] on a jittered 5 x 5 x 4 lattice of spacing 5
+    (no overlaps): more radius classes than pair tables are built for."""
+    rng = np.random.default_rng(seed)
+    sites = np.stack(np.meshgrid(np.arange(5), np.arange(5), np.arange(4), indexing="ij"), -1).reshape(-1, 3)
+    sites = sites[rng.permutation(len(sites))[:n]] * 5.0 + 2.5
+    pos = sites + rng.uniform(-0.4, 0.4, sites.shape)
+    rad = 1.0 + rng.random(n)
+    rad[0] = 1.0
+    return np.array([25.0, 25.0, 20.0]), pos, rad
+
+
+def test_textbook_lattice_sum_continuous_radii():
+    """70 distinct radii (no tables; each pair's closed forms at its own radii): the
+    non-overlapping operator equals the textbook lattice sum without the tables'
+    interpolation error (the same bound as the tables' case: what remains is the Ewald
+    truncation of both sums)."""
+    L, pos, rad = _continuous_radii_lattice()
+    assert len(np.unique(np.round(rad, 5))) > 64
+    rng = np.random.default_rng(2)
+    p = rng.standard_normal((len(rad), 3)) + 1j * rng.standard_normal((len(rad), 3))
+    ref = -T.point_dipole_field(pos, p, L) + p / (4 * np.pi * rad[:, None] ** 3)
+    op = O.PolyOperator(L, 1e-10, 0.5, rad)
+    op.set_positions(pos)
+    A = op.apply(p, 0.0)
+    assert np.abs(A - ref).max() / np.abs(ref).max() < 2e-7
+
+
+def test_total_operator_independent_of_xi_continuous_radii():
+    """Overlapping pairs of 70 distinct radii (closed forms at r >= 1): real + reciprocal
+    does not depend on xi."""
+    L, pos, rad = _continuous_radii_lattice(seed=6)
+    pos[1] = pos[0] + [1.3, 0.9, 0.4]     # overlapping (d = 1.63 < a0 + a1)
+    pos[2] = pos[0] + [-2.1, 0.3, -0.2]   # close
+    rng = np.random.default_rng(3)
+    p = rng.standard_normal((len(rad), 3)) + 1j * rng.standard_normal((len(rad), 3))
+    sc = rng.standard_normal(len(rad)) + 1j * rng.standard_normal(len(rad))
+    out = []
+    for xi in (0.5, 0.6):
+        op = O.PolyOperator(L, 1e-10, xi, rad)
+        op.set_positions(pos)
+        out.append(op.apply(p, sc))
+    assert np.abs(out[0] - out[1]).max() / np.abs(out[0]).max() < 1e-8
