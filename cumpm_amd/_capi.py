@@ -82,6 +82,9 @@ _SIGS = {
     "mpm_op_create_poly": (ctypes.c_int, [ctypes.POINTER(OpDesc), _dp, _i64, ctypes.POINTER(ctypes.c_void_p)]),
     "mpm_op_destroy": (None, [ctypes.c_void_p]),
     "mpm_op_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OpInfo)]),
+    "mpm_op_get_pair_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _i64, ctypes.POINTER(_i64)]),
+    "mpm_op_get_neighbors": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_int),
+                                            _dp, _dp, _i64, ctypes.POINTER(_i64)]),
     "mpm_op_set_positions": (ctypes.c_int, [ctypes.c_void_p, _dp, _i64]),
     "mpm_op_set_bloch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
     "mpm_op_set_self_coef": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),

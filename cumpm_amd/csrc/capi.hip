@@ -432,6 +432,22 @@ mpm_status mpm_op_get_info(const mpm_op* op, mpm_op_info* info) {
     });
 }
 
+mpm_status mpm_op_get_pair_table(const mpm_op* op, int which, double* r, double* values, int64_t cap,
+                                 int64_t* size) {
+    return guarded([&] {
+        MPM_REQUIRE(op && size, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        *size = op->impl->pair_table(which, r, values, cap);
+    });
+}
+
+mpm_status mpm_op_get_neighbors(const mpm_op* op, int64_t* start, int* j, double* perp, double* para,
+                                int64_t cap, int64_t* num_pairs) {
+    return guarded([&] {
+        MPM_REQUIRE(op && num_pairs, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        *num_pairs = op->impl->neighbors(start, j, perp, para, cap);
+    });
+}
+
 mpm_status mpm_op_set_positions(mpm_op* op, const double* xyz, int64_t n) {
     return guarded([&] {
         MPM_REQUIRE(op && xyz, MPM_ERR_INVALID_ARGUMENT, "null argument");

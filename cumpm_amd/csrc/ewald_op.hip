@@ -3485,6 +3485,38 @@ void EwaldOp::poly_pair_value(double r, int i, int j, double& perp, double& para
     para = s0 * (1.0 - r) + q1 * r;
 }
 
+// Pair table `which` (0 perp, 1 para, 2..8 the quadrupole tables) on its samples r; the
+// table's length is returned and the arrays written when cap allows.
+int64_t EwaldOp::pair_table(int which, double* r, double* v, int64_t cap) const {
+    MPM_REQUIRE(!poly, MPM_ERR_UNSUPPORTED, "the polydisperse operator keeps one table per radius pair");
+    MPM_REQUIRE(which >= 0 && which <= 8, MPM_ERR_INVALID_ARGUMENT, "table index out of range");
+    const std::vector<double>& t = which == 0 ? table->perp : which == 1 ? table->para : table->q[which - 2];
+    MPM_REQUIRE(!t.empty(), MPM_ERR_RUNTIME, "quadrupole tables exist after mpm_op_set_quads");
+    const int64_t n = (int64_t)table->r.size();
+    if (cap >= n) {
+        if (r) std::copy(table->r.begin(), table->r.end(), r);
+        if (v) std::copy(t.begin(), t.end(), v);
+    }
+    return n;
+}
+
+// Neighbour list of the current positions (CSR over particles, j != i, minimum-image
+// distance below rc) and each pair's (perp, para) values; the pair count is returned
+// and the arrays written when cap allows.
+int64_t EwaldOp::neighbors(int64_t* start, int* j, double* perp, double* para, int64_t cap) const {
+    MPM_REQUIRE(N > 0, MPM_ERR_RUNTIME, "particle coordinates have not been set");
+    if (cap >= num_pairs) {
+        if (start)
+            for (size_t i = 0; i < h_nb_start.size(); ++i) start[i] = h_nb_start[i];
+        for (int64_t k = 0; k < num_pairs; ++k) {
+            if (j) j[k] = h_nb_j[k];
+            if (perp) perp[k] = h_pair_f[k].x;
+            if (para) para[k] = h_pair_f[k].y;
+        }
+    }
+    return num_pairs;
+}
+
 // Table value of radius classes (u, v) at distance r: pair column of the upper
 // triangle (row-major), the monodisperse interpolation rule (clamped at the last
 // sample).  polydisperse...cu:56-68, 168-188.
@@ -4003,6 +4035,8 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         nstart[i + 1] = (int)nj.size();
     }
     num_pairs = (int64_t)nj.size();
+    h_nb_start = nstart;
+    h_nb_j = nj;
     d_pos.upload(pos.data(), pos.size());
     d_base.upload(base.data(), base.size());
     d_w64.upload(w64.data(), w64.size());

@@ -141,6 +141,10 @@ class EwaldOp {
     void apply_contiguous(const double2* x, double2* y, int B, bool fp32, int parts,
                           hipStream_t s, const int* rows = nullptr);
     void info(mpm_op_info& out) const;
+    // host copies for inspection: the monodisperse pair tables (r, perp, para; table 2..8 =
+    // the quadrupole tables when built) and the current neighbour list with its pair values
+    int64_t pair_table(int which, double* r, double* v, int64_t cap) const;
+    int64_t neighbors(int64_t* start, int* j, double* perp, double* para, int64_t cap) const;
     // quadrupoles on particles idx[nq] (every particle when nq < 0); before set_positions.
     // Unknowns per right-hand side then are [3N dipole | 5Q quadrupole] components.
     void set_quads(const int* idx, int64_t nq);
@@ -193,6 +197,7 @@ class EwaldOp {
     // per pair, for the thread-per-(particle, RHS) finish: (ux, uy, uz, perp,
     // para - perp, cos, sin, 0) with u = r / max(|r|, 1), r = x_j - x_i (min
     // image) and the Bloch phase -(kb . r); rebuilt when positions or kb change
+    std::vector<int> h_nb_start, h_nb_j;  // neighbour list CSR of the current positions
     std::vector<double> h_pair_r;  // [M][3]
     std::vector<double2> h_pair_f;
     DevBuf<double> d_pairc;        // [M][8]

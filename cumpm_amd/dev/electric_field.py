@@ -303,6 +303,67 @@ class MonodisperseEwaldElectricField:
             return e[:3 * self._N].reshape(self._N, 3), e[3 * self._N:].reshape(self._Q, 5)
         return e.reshape(self._N, 3)
 
+    # ---- host copies of the operator's tables and per-frame precalculations ----
+    def _table(self, which):
+        size = C._i64()
+        lib = C.load()
+        check(lib.mpm_op_get_pair_table(self._h, int(which), None, None, 0, ctypes.byref(size)))
+        r = np.zeros(size.value)
+        v = np.zeros(size.value)
+        check(lib.mpm_op_get_pair_table(self._h, int(which), C.dptr(r), C.dptr(v), size.value, ctypes.byref(size)))
+        return r, v
+
+    def get_real_space_tables_host(self):
+        """(r_table, field_dip_1, field_dip_2): samples 0, 1.000 ... 9.999 and the perpendicular /
+        parallel pair functions (entry 0 the self term)."""
+        r, perp = self._table(0)
+        return r, perp, self._table(1)[1]
+
+    def _neighbors(self):
+        if not self._N:
+            raise RuntimeError("electricField: particle coordinates have not been set")
+        m = C._i64()
+        lib = C.load()
+        check(lib.mpm_op_get_neighbors(self._h, None, None, None, None, 0, ctypes.byref(m)))
+        M = m.value
+        start = np.zeros(self._N + 1, np.int64)
+        j = np.zeros(max(M, 1), np.int32)
+        perp = np.zeros(max(M, 1))
+        para = np.zeros(max(M, 1))
+        check(lib.mpm_op_get_neighbors(self._h, start.ctypes.data_as(ctypes.POINTER(C._i64)),
+                                       j.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), C.dptr(perp), C.dptr(para),
+                                       M, ctypes.byref(m)))
+        return start, j[:M], perp[:M], para[:M]
+
+    @property
+    def max_neighbors(self):
+        start = self._neighbors()[0]
+        return int(np.diff(start).max()) if self._N else 0
+
+    def get_neighbor_list_host(self):
+        """(list, counts): row i of list (length max_neighbors, padded with -1) holds the
+        neighbours j != i of particle i within rc (the same pairs as the reference; the order
+        within a row may differ)."""
+        start, j, _, _ = self._neighbors()
+        counts = np.diff(start).astype(np.int32)
+        width = int(counts.max()) if counts.size else 0
+        out = np.full((self._N, width), -1, np.int32)
+        for i in range(self._N):
+            out[i, :counts[i]] = j[start[i]:start[i + 1]]
+        return out.reshape(-1), counts
+
+    def get_real_space_precalcs_host(self):
+        """(self_perp, perp, para): the self term and each neighbour pair's perpendicular /
+        parallel values at d_eff = max(d, 1), in get_neighbor_list_host's row order."""
+        _, _, perp, para = self._neighbors()
+        return float(self._info().self_perp), perp, para
+
+    def compute_neighbor_list(self):
+        """The neighbour list, stencils and pair values are built by update_particle_coordinates."""
+
+    compute_precalculations = compute_real_space_tables = compute_neighbor_list
+    spread_precalcs = contract_precalcs = real_space_precalcs = compute_neighbor_list
+
     def get_dipoles_complex_host(self):
         p = self._dip
         return (p[:, 0].real.copy(), p[:, 0].imag.copy(), p[:, 1].real.copy(), p[:, 1].imag.copy(),

@@ -102,10 +102,26 @@ mpm_status mpm_op_create(const mpm_op_desc* desc, mpm_op** out);
  * tables :1434-1698.  radii: host [n_radii], nondimensional; radii within 1e-5 share a class.
  * rc = max(sqrt(-ln tol)/xi, 2 a_max); P = ceil(2/h_mean - 2 ln tol/pi) unless desc->support > 0.
  * Full-grid layout and the built-in FFT only (desc->layout and desc->fft_backend must be 0 or
- * request them); MPM_ERR_UNSUPPORTED for more than 64 radius classes. */
+ * request them).  Up to 64 radius classes get pair tables as the reference's; with more, each
+ * pair's closed forms are evaluated at its own radii (the reference's tables would need
+ * U(U+1)/2 columns). */
 mpm_status mpm_op_create_poly(const mpm_op_desc* desc, const double* radii, int64_t n_radii, mpm_op** out);
 void mpm_op_destroy(mpm_op* op);
 mpm_status mpm_op_get_info(const mpm_op* op, mpm_op_info* info);
+/* getRealSpaceTablesHost (ewald_electric_field_base.cu:101-110), monodisperse operator: the
+ * samples r (entry 0 = 0, then 1 + 0.001 k) and table `which` (0 = field_dip_1 / perp,
+ * 1 = field_dip_2 / para, 2..4 = field_quad_1..3, 5..8 = grad_quad_1..4 after
+ * mpm_op_set_quads).  *size = table length; r and values (either may be NULL) are written
+ * when cap >= *size. */
+mpm_status mpm_op_get_pair_table(const mpm_op* op, int which, double* r, double* values, int64_t cap,
+                                 int64_t* size);
+/* getNeighborListHost / getRealSpacePrecalcsHost (ewald_electric_field_base.cu:92-99,
+ * monodisperse_ewald_electric_field.cu:1138-1160): the neighbour list of the current positions
+ * as CSR, start[N + 1] and j[M] (every j != i at minimum-image distance below rc; the reference
+ * keeps the same pairs in padded rows), and each pair's perp/para values.  *num_pairs = M; the
+ * arrays (any may be NULL) are written when cap >= M. */
+mpm_status mpm_op_get_neighbors(const mpm_op* op, int64_t* start, int* j, double* perp, double* para,
+                                int64_t cap, int64_t* num_pairs);
 
 /* updateParticleCoordinates — src/cpp/electric_field/base_electric_field.cu:40-118 plus the
  * lazy per-frame precalcs of monodisperse_ewald_electric_field.cu:2485-2503 (neighbour list,

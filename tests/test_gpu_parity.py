@@ -708,3 +708,44 @@ def test_dev_field_quadrupoles_match_oracle():
     Ei, Gi = ef.compute_inverse_field()
     np.testing.assert_allclose(Er + Ei, E, rtol=1e-12, atol=1e-12 * np.abs(E).max())
     np.testing.assert_allclose(Gr + Gi, G, rtol=1e-12, atol=1e-12 * np.abs(G).max())
+
+
+def test_dev_host_getters_tables_and_neighbor_list():
+    """getRealSpaceTablesHost / getNeighborListHost / getRealSpacePrecalcsHost through the
+    dev API: the tables equal the oracle's (independent closed forms, 1e-13), the neighbour
+    rows hold exactly the pairs within rc (brute-force minimum image), and each pair's values
+    are the tables interpolated at d_eff = max(d, 1) (monodisperse...cu:877-958)."""
+    from cumpm_amd.dev import MonodisperseEwaldElectricField
+    _case("dev host getters")
+    box, tol, xi, pos, grid, P = _geom("random3d")
+    pos = pos.copy()
+    pos[1] = pos[0] + [0.6, 0.2, 0.1]  # d < 1: d_eff = 1
+    ef = MonodisperseEwaldElectricField(*box, tol, xi, True, precision="double")
+    ef.update_particle_coordinates(pos[:, 0], pos[:, 1], pos[:, 2])
+    r, d1, d2 = ef.get_real_space_tables_host()
+    rr, pe, pa, sp = O.tables(xi)
+    np.testing.assert_array_equal(r, rr)
+    np.testing.assert_allclose(d1, pe, rtol=1e-13, atol=1e-16)
+    np.testing.assert_allclose(d2, pa, rtol=1e-13, atol=1e-16)
+    lst, counts = ef.get_neighbor_list_host()
+    N = len(pos)
+    width = lst.size // N
+    assert width == ef.max_neighbors == counts.max()
+    rc = ef.rc
+    self_perp, perp, para = ef.get_real_space_precalcs_host()
+    assert self_perp == pytest.approx(sp, rel=1e-13)
+    k = 0
+    for i in range(N):
+        dv = pos[i] - pos
+        dv -= box * np.round(dv / box)
+        d = np.linalg.norm(dv, axis=1)
+        expect = sorted(j for j in range(N) if j != i and d[j] < rc)
+        row = lst[i * width:i * width + counts[i]]
+        assert sorted(row.tolist()) == expect
+        assert np.all(lst[i * width + counts[i]:(i + 1) * width] == -1)
+        for j in row:
+            de = max(d[j], 1.0)
+            assert perp[k] == pytest.approx(np.interp(de, rr[1:], pe[1:]), rel=1e-12, abs=1e-16)
+            assert para[k] == pytest.approx(np.interp(de, rr[1:], pa[1:]), rel=1e-12, abs=1e-16)
+            k += 1
+    assert k == perp.size
