@@ -96,6 +96,13 @@ _SIGS = {
     "mpm_solve_adjoint": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolveDesc), ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(SolveReport),
                                          ctypes.c_void_p]),
+    "mpm_near_field_create": (ctypes.c_int, [_dp, _dp, _dp, _i64, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_void_p)]),
+    "mpm_near_field_destroy": (None, [ctypes.c_void_p]),
+    "mpm_near_field_set_dipoles": (ctypes.c_int, [ctypes.c_void_p, _dp, _i64]),
+    "mpm_near_field_set_dipole_positions": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _i64]),
+    "mpm_near_field_set_field_points": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _i64]),
+    "mpm_near_field_calculate": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "mpm_dipole_solver_create": (ctypes.c_int, [ctypes.POINTER(DipoleDesc), ctypes.POINTER(ctypes.c_void_p)]),
     "mpm_dipole_solver_destroy": (None, [ctypes.c_void_p]),
     "mpm_dipole_solver_compute": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _i64]),

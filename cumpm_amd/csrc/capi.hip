@@ -378,9 +378,141 @@ struct mpm_dipole_solver {
 };
 
 // ---------------------------------------------------------------------------
+// Near field (src/cpp/near_field.cu:14-213): the field that dipoles at given
+// positions induce at arbitrary field points, as the intensity |E0 + E_ind|^2.
+// The field points join the operator as sites without dipoles, so that row f of
+// A [p; 0] (self coefficients zero) is the sum over particles of the pair terms
+// at (f, particle) plus the Gaussian contraction of the reciprocal grid at f:
+// minus the induced field there, exactly the INTERACTION_FIELD evaluation
+// (monodisperse_ewald_electric_field.cu:1631-1869, 2385-2411).
+// ---------------------------------------------------------------------------
+struct mpm_near_field {
+    double box[3];
+    cplx E0[3];
+    std::vector<double> radius;
+    double xi = 0.5, errortol = 1e-3;
+    int field_type = MPM_FIELD_AUTO;
+    std::vector<cplx> dip;                      // [N][3]
+    std::vector<double> dpos[3], fpos[3];       // dipole and field-point coordinates
+    double length_scale = 1.0;
+    std::unique_ptr<mpm::EwaldOp> op;
+    mpm::DevBuf<double2> d_x, d_y;
+
+    void calculate(double* intensity) {
+        const size_t N = dip.size() / 3, M = fpos[0].size();
+        if (fpos[0].empty() || dpos[0].empty() || dip.empty())
+            throw mpm::Error(MPM_ERR_RUNTIME,
+                             "Near_Field: must set dipoles, dipole positions, and field points before calculation.");
+        if (dpos[0].size() != N)
+            throw mpm::Error(MPM_ERR_RUNTIME,
+                             "Near_Field: dipole position size is inconsistent with dipole vector size.");
+        if (!op) {
+            if (radius.empty())
+                radius.assign(N, 1.0);
+            else if (radius.size() == 1)
+                radius.assign(N, radius[0]);
+            else if (radius.size() != N)
+                throw mpm::Error(MPM_ERR_RUNTIME,
+                                 "The number of particles is inconsistent with the number of radii provided.");
+            bool mono = true;
+            for (size_t i = 1; i < N; ++i) mono = mono && radius[i] == radius[0];
+            if (field_type == MPM_FIELD_DIRECT)
+                throw mpm::Error(MPM_ERR_UNSUPPORTED, "the direct (open-boundary) field is not provided by this build");
+            if (field_type == MPM_FIELD_POLYDISPERSE || (field_type == MPM_FIELD_AUTO && !mono))
+                throw mpm::Error(MPM_ERR_UNSUPPORTED,
+                                 "the near field of the polydisperse operator is not provided by this build");
+            length_scale = radius[0];  // near_field.cu:120 (the first radius)
+            mpm_op_desc od;
+            std::memset(&od, 0, sizeof(od));
+            for (int d = 0; d < 3; ++d) od.box[d] = box[d] / length_scale;
+            od.errortol = errortol;
+            od.xi = xi;
+            op = std::make_unique<mpm::EwaldOp>(od);
+        }
+        const size_t T = N + M;
+        std::vector<double> xyz(3 * T);
+        for (size_t i = 0; i < T; ++i)
+            for (int d = 0; d < 3; ++d)
+                xyz[3 * i + d] = (i < N ? dpos[d][i] : fpos[d][i - N]) / length_scale;
+        op->set_positions(xyz.data(), (int64_t)T);
+        std::vector<cplx> zero(T, 0.0);
+        op->set_self_coef(reinterpret_cast<const double*>(zero.data()), 1);
+        std::vector<cplx> x(3 * T, 0.0);
+        std::copy(dip.begin(), dip.end(), x.begin());
+        d_x.reserve(3 * T);
+        d_y.reserve(3 * T);
+        MPM_HIP(hipMemcpy(d_x.p, x.data(), 3 * T * sizeof(double2), hipMemcpyHostToDevice));
+        op->apply_contiguous(d_x.p, d_y.p, 1, false, MPM_PART_ALL, nullptr);
+        MPM_HIP(hipGetLastError());
+        std::vector<cplx> y(3 * T);
+        MPM_HIP(hipMemcpy(y.data(), d_y.p, 3 * T * sizeof(double2), hipMemcpyDeviceToHost));
+        for (size_t f = 0; f < M; ++f) {
+            double acc = 0.0;
+            for (int c = 0; c < 3; ++c) acc += std::norm(E0[c] - y[3 * (N + f) + c]);
+            intensity[f] = acc;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
 // extern "C"
 // ---------------------------------------------------------------------------
 extern "C" {
+
+mpm_status mpm_near_field_create(const double box[3], const double* E0, const double* radius, int64_t num_radii,
+                                 double xi, double errortol, int field_type, mpm_near_field** out) {
+    return guarded([&] {
+        MPM_REQUIRE(box && E0 && out, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        MPM_REQUIRE(num_radii >= 0 && (num_radii == 0 || radius), MPM_ERR_INVALID_ARGUMENT, "null radius");
+        MPM_REQUIRE(field_type >= 0 && field_type <= 3, MPM_ERR_RUNTIME, "Unknown field_type");
+        auto nf = std::make_unique<mpm_near_field>();
+        for (int d = 0; d < 3; ++d) {
+            nf->box[d] = box[d];
+            nf->E0[d] = cplx(E0[2 * d], E0[2 * d + 1]);
+        }
+        nf->radius.assign(radius, radius + num_radii);
+        for (double r : nf->radius) MPM_REQUIRE(r > 0.0, MPM_ERR_INVALID_ARGUMENT, "radii must be positive");
+        nf->xi = xi;
+        nf->errortol = errortol;
+        nf->field_type = field_type;
+        *out = nf.release();
+    });
+}
+
+void mpm_near_field_destroy(mpm_near_field* nf) { delete nf; }
+
+mpm_status mpm_near_field_set_dipoles(mpm_near_field* nf, const double* dip, int64_t n) {
+    return guarded([&] {
+        MPM_REQUIRE(nf && (dip || n == 0) && n >= 0, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        nf->dip.resize(3 * n);
+        for (int64_t k = 0; k < 3 * n; ++k) nf->dip[k] = cplx(dip[2 * k], dip[2 * k + 1]);
+    });
+}
+
+mpm_status mpm_near_field_set_dipole_positions(mpm_near_field* nf, const double* x, const double* y, const double* z,
+                                               int64_t n) {
+    return guarded([&] {
+        MPM_REQUIRE(nf && ((x && y && z) || n == 0) && n >= 0, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        const double* c[3] = {x, y, z};
+        for (int d = 0; d < 3; ++d) nf->dpos[d].assign(c[d], c[d] + n);
+    });
+}
+
+mpm_status mpm_near_field_set_field_points(mpm_near_field* nf, const double* x, const double* y, const double* z,
+                                           int64_t n) {
+    return guarded([&] {
+        MPM_REQUIRE(nf && ((x && y && z) || n == 0) && n >= 0, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        const double* c[3] = {x, y, z};
+        for (int d = 0; d < 3; ++d) nf->fpos[d].assign(c[d], c[d] + n);
+    });
+}
+
+mpm_status mpm_near_field_calculate(mpm_near_field* nf, double* intensity) {
+    return guarded([&] {
+        MPM_REQUIRE(nf && intensity, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        nf->calculate(intensity);
+    });
+}
 
 const char* mpm_status_string(mpm_status s) {
     switch (s) {

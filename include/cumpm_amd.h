@@ -254,6 +254,26 @@ mpm_status mpm_dipole_solver_get_timing(const mpm_dipole_solver* s, double* solv
  * first frame does.  No counterpart in the reference (its results only accumulate). */
 mpm_status mpm_dipole_solver_reset(mpm_dipole_solver* s);
 
+/* ---- Near field (src/cpp/near_field.h:12-57, near_field_bindings.cpp:8-25) ---- */
+typedef struct mpm_near_field mpm_near_field;
+/* Near_Field(box, E0, radius, xi, errortol, field_type): box physical [3], E0 host complex [3],
+ * radius host [num_radii] (0 => 1.0; lengths are scaled by the first radius), field_type one of
+ * mpm_field_type; this build evaluates the monodisperse field only: AUTO with equal radii, or
+ * MONODISPERSE. */
+mpm_status mpm_near_field_create(const double box[3], const double* E0, const double* radius, int64_t num_radii,
+                                 double xi, double errortol, int field_type, mpm_near_field** out);
+void mpm_near_field_destroy(mpm_near_field* nf);
+/* set_dipoles: host complex [n][3] (copied). */
+mpm_status mpm_near_field_set_dipoles(mpm_near_field* nf, const double* dip, int64_t n);
+mpm_status mpm_near_field_set_dipole_positions(mpm_near_field* nf, const double* x, const double* y, const double* z,
+                                               int64_t n);
+mpm_status mpm_near_field_set_field_points(mpm_near_field* nf, const double* x, const double* y, const double* z,
+                                           int64_t n);
+/* calculate (near_field.cu:105-213): intensity[f] = |E0 + E_ind(f)|^2 at every field point, E_ind
+ * the periodic Ewald field of the dipoles (volume-averaged over a unit sphere at f, the pair
+ * functions of the solver; FP64 throughout, where the reference defaults to an FP32 grid). */
+mpm_status mpm_near_field_calculate(mpm_near_field* nf, double* intensity);
+
 /* ---- Plumbing ---- */
 const char* mpm_status_string(mpm_status s);
 /* Message of the last failing call on this thread. */

@@ -749,3 +749,49 @@ def test_dev_host_getters_tables_and_neighbor_list():
             assert para[k] == pytest.approx(np.interp(de, rr[1:], pa[1:]), rel=1e-12, abs=1e-16)
             k += 1
     assert k == perp.size
+
+
+def test_near_field_golden_fixture_and_basic():
+    """Reference tests/test_near_fields.py:5-80 through the product API: the pyMPM fixture (its
+    bar 1e-4; here to 1e-8 of the oracle) and the basic shape / sign checks."""
+    import cumpm_amd
+    _case("near field fixture")
+    ref = np.load(os.path.join(GOLDEN, "ref_near_field.npz"))["intensity_py"]
+    box = np.array([50.0, 50.0, 50.0])
+    dip_pos = np.array([[0.0, 0, 0], [10.0, 0, 0], [0, 10.0, 0], [0, 0, 10.0]])
+    dip = np.array([[0.1 + 0.05j, 0, 0], [-0.1, 0.02j, 0], [0, 0.05, -0.01j], [0.01, 0, 0.1j]])
+    fp = np.array([[5.0, 0, 0], [0, 5.0, 0], [2.5, 2.5, 2.5]])
+    nf = cumpm_amd.Near_Field(box, [1.0, 0.0, 0.0], radius=1.0, dip=dip, dip_pos=dip_pos, field_points=fp)
+    I = nf.calculate()
+    np.testing.assert_allclose(I, ref, rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(I, O.near_field_intensity(box, [1.0, 0, 0], 1.0, dip, dip_pos, fp), rtol=1e-10)
+    nf2 = cumpm_amd.Near_Field(box, [1.0, 0.0, 0.0], radius=1.0, dip=[[0.1, 0, 0], [-0.1, 0, 0]],
+                               dip_pos=[[0.0, 0, 0], [10.0, 0, 0]], field_points=[[5.0, 0, 0], [0, 5.0, 0]])
+    I2 = nf2.calculate()
+    assert I2.shape == (2,) and not np.any(np.isnan(I2)) and np.all(I2 >= 0.0)
+
+
+def test_near_field_matches_oracle_with_overlap_and_radius():
+    """|E_ind|^2 (E0 = 0) at field points inside, on and outside the spheres (d < 1, 1 < d < 2,
+    d > 2), radius 2.5 (lengths scaled by it), errortol 1e-8: the oracle to 1e-10; the input
+    errors of near_field.cu:105-140."""
+    import cumpm_amd
+    _case("near field vs oracle")
+    rng = np.random.default_rng(12)
+    box = np.array([60.0, 62.0, 64.0])
+    dip_pos = rng.uniform(0, 55, (12, 3))
+    dip = rng.standard_normal((12, 3)) + 1j * rng.standard_normal((12, 3))
+    fp = np.concatenate([dip_pos[:3] + [[0.5, 0.3, 0.2], [2.0, 1.5, 0.0], [0.0, 0.0, 0.0]],
+                         rng.uniform(0, 55, (6, 3))])
+    nf = cumpm_amd.Near_Field(box, [0.0, 0.0, 0.0], radius=2.5, dip=dip, dip_pos=dip_pos, field_points=fp,
+                              errortol=1e-8)
+    I = nf.calculate()
+    ref = O.near_field_intensity(box, [0.0, 0, 0], 2.5, dip, dip_pos, fp, errortol=1e-8)
+    assert rel(I, ref) < 1e-10
+    with pytest.raises(RuntimeError, match="must set"):
+        cumpm_amd.Near_Field(box, [1.0, 0, 0], dip=dip, dip_pos=dip_pos).calculate()
+    with pytest.raises(RuntimeError, match="inconsistent"):
+        cumpm_amd.Near_Field(box, [1.0, 0, 0], dip=dip, dip_pos=dip_pos[:5], field_points=fp).calculate()
+    with pytest.raises(NotImplementedError):
+        cumpm_amd.Near_Field(box, [1.0, 0, 0], radius=np.linspace(1, 2, 12), dip=dip, dip_pos=dip_pos,
+                             field_points=fp).calculate()

@@ -143,3 +143,35 @@ def test_krylov_matches_dense_solve(method, tol, bound):
     x, rep = op.solve(b, sc, tol=tol, method=method, restart=8)
     assert rep["converged"]
     assert np.abs(x - x_ref).max() / np.abs(x_ref).max() < bound
+
+
+def test_near_field_golden_fixture():
+    """Reference tests/test_near_fields.py:31-80: the oracle's near field against the pyMPM
+    fixture (its bar 1e-4; the restatement agrees to 1e-9)."""
+    ref = np.load(os.path.join(GOLDEN, "ref_near_field.npz"))["intensity_py"]
+    box = [50.0] * 3
+    dip_pos = np.array([[0.0, 0, 0], [10.0, 0, 0], [0, 10.0, 0], [0, 0, 10.0]])
+    dip = np.array([[0.1 + 0.05j, 0, 0], [-0.1, 0.02j, 0], [0, 0.05, -0.01j], [0.01, 0, 0.1j]])
+    fp = np.array([[5.0, 0, 0], [0, 5.0, 0], [2.5, 2.5, 2.5]])
+    I = O.near_field_intensity(box, [1.0, 0, 0], 1.0, dip, dip_pos, fp)
+    np.testing.assert_allclose(I, ref, rtol=1e-9, atol=1e-12)
+
+
+def test_near_field_is_the_periodic_point_dipole_field():
+    """Field points at least 2 from every sphere centre: the volume average over a unit sphere
+    of a uniformly polarised sphere's field is its point-dipole field at the centre, so the
+    near field equals the textbook periodic point-dipole sum there (1e-7 at errortol 1e-10)."""
+    L = np.array([24.0, 25.0, 26.0])
+    rng = np.random.default_rng(3)
+    dip_pos = np.array([[2.0, 3.0, 4.0], [9.0, 3.5, 4.2], [4.0, 11.0, 13.0], [18.0, 18.0, 19.0]])
+    p = rng.standard_normal((4, 3)) + 1j * rng.standard_normal((4, 3))
+    fp = np.array([[5.5, 3.2, 4.1], [12.0, 12.0, 12.0], [20.0, 6.0, 2.0]])
+    sites = np.concatenate([dip_pos, fp])
+    ref = T.point_dipole_field(sites, np.concatenate([p, np.zeros((3, 3))]), L)[4:]
+    op = O.Operator(L, 1e-10, 0.5)
+    op.set_positions(sites)
+    E_ind = -op.apply(np.concatenate([p, np.zeros((3, 3), complex)]), 0.0)[4:]
+    assert np.abs(E_ind - ref).max() / np.abs(ref).max() < 1e-7
+    E0 = np.array([0.3, 1.0j, 0.2])
+    I = O.near_field_intensity(L, E0, 1.0, p, dip_pos, fp, errortol=1e-10)
+    np.testing.assert_allclose(I, np.sum(np.abs(E0 + ref) ** 2, axis=1), rtol=1e-9)
