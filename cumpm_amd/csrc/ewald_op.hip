@@ -1946,7 +1946,8 @@ __global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, 
 // slot of the region (the CSR position of the pair); the per-particle sum over
 // its regions happens in the finish step, in a fixed order.
 template <typename R, int RX, int ZSB>
-__global__ __launch_bounds__(256) void contract_mfma_kernel(int B, SpreadGeom g, const R* __restrict__ regw,
+__global__ __launch_bounds__(256, sizeof(R) == 4 ? 3 : 2) void contract_mfma_kernel(int B, SpreadGeom g,
+                                                                                    const R* __restrict__ regw,
                                                             const int* __restrict__ reg_start,
                                                             const int* __restrict__ reg_items, int nry, int N,
                                                             const typename C2<R>::T* __restrict__ grid,

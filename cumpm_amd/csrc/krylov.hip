@@ -76,18 +76,24 @@ __global__ __launch_bounds__(256) void dots_kernel(Ptrs p, int kk, size_t stride
     double sr[kDotGroup], si[kDotGroup];
 #pragma unroll
     for (int i = 0; i < kDotGroup; ++i) sr[i] = si[i] = 0.0;
-    for (size_t e = e0 + threadIdx.x; e < e1; e += 256) {
+    // two elements per thread and step, so that twice the loads are in flight
+    for (size_t e = e0 + threadIdx.x; e < e1; e += 512) {
+        const size_t e2 = e + 256;
+        const bool two = e2 < e1;
         const double2 bv = Bv[e];
+        const double2 bv2 = two ? Bv[e2] : make_double2(0.0, 0.0);
         if (mode == 2) {
-            sr[0] += bv.x * bv.x + bv.y * bv.y;
+            sr[0] += (bv.x * bv.x + bv.y * bv.y) + (bv2.x * bv2.x + bv2.y * bv2.y);
         } else {
 #pragma unroll
             for (int i = 0; i < kDotGroup; ++i) {
                 if (i < ni) {
                     const double2 av = A[(size_t)i * stride + e];
+                    const double2 av2 = two ? A[(size_t)i * stride + e2] : make_double2(0.0, 0.0);
                     const double2 t = (mode == 0) ? cmulc(av, bv) : cmul(av, bv);
-                    sr[i] += t.x;
-                    si[i] += t.y;
+                    const double2 t2 = (mode == 0) ? cmulc(av2, bv2) : cmul(av2, bv2);
+                    sr[i] += t.x + t2.x;
+                    si[i] += t.y + t2.y;
                 }
             }
         }
@@ -179,6 +185,7 @@ __global__ __launch_bounds__(256) void cgs_axpy_kernel(Ptrs p, int kk, size_t st
     if (e < n) {
         const double2* V = p.a[slot];
         double2 acc = p.b[slot][e];
+#pragma unroll 4
         for (int i = 0; i < kk; ++i) {
             const double2 t = cmul(s_c[i], V[(size_t)i * stride + e]);
             acc.x -= t.x;
