@@ -118,7 +118,7 @@ def kernel_bytes(name, info, N, B, pairs):
 # profiles/hex4000_sweep_batched_refined.md.
 MEASURED_TRAFFIC = {
     "xtoeplitz_kernel<float, 10, 2, true>": ((107.4 + 74.5) * 1e6, 14.22),
-    "contract_mfma_kernel<float, 4, 10>": ((110.1 + 8.4) * 1e6, 18.12),
+    "contract_mfma_kernel<float, 4, 10>": ((108.7 + 8.3) * 1e6, 18.12),
 }
 
 
