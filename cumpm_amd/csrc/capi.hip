@@ -32,6 +32,22 @@ std::string profile_names();
 }  // namespace mpm
 
 namespace {
+// Quadrupole block of the metric M = diag(I_3N, W / 2) under which M A is Hermitian:
+// W is the Frobenius product of traceless symmetric tensors in the components
+// (xx, xy, xz, yy, yz), Q:G = 2 xx Gxx + xx Gyy + yy Gxx + 2 yy Gyy + 2 (xy Gxy + xz Gxz
+// + yz Gyz).  inverse 0: v_q <- (W / 2) v_q; 1: v_q <- (W / 2)^-1 v_q (only the (xx, yy)
+// pair mixes; the other components are unchanged).
+__global__ void __launch_bounds__(256) quad_metric_kernel(double2* __restrict__ v, int B, size_t n, size_t off,
+                                                          size_t Q, int inverse) {
+    const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= (size_t)B * Q) return;
+    double2* e = v + (t / Q) * n + off + 5 * (t % Q);
+    const double2 xx = e[0], yy = e[3];
+    const double a = inverse ? 4.0 / 3.0 : 1.0, c = inverse ? -2.0 / 3.0 : 0.5;
+    e[0] = make_double2(a * xx.x + c * yy.x, a * xx.y + c * yy.y);
+    e[3] = make_double2(c * xx.x + a * yy.x, c * xx.y + a * yy.y);
+}
+
 
 thread_local std::string g_last_error;
 
@@ -644,8 +660,28 @@ mpm_status mpm_solve_adjoint(mpm_op* op, const mpm_solve_desc* desc, const doubl
                 }
             }
         } conj(*op->impl, s);
-        mpm::solve_batched(*op->impl, *desc, reinterpret_cast<const double2*>(g), reinterpret_cast<double2*>(y), B,
-                           reports, s);
+        mpm::EwaldOp& E = *op->impl;
+        const size_t Q = E.num_quad_slots();
+        if (Q == 0) {
+            mpm::solve_batched(E, *desc, reinterpret_cast<const double2*>(g), reinterpret_cast<double2*>(y), B,
+                               reports, s);
+            return;
+        }
+        // With quadrupoles A = M^-1 H, H = M A Hermitian up to the self coefficients, so
+        // A^H = M conj-self(A) M^-1: solve conj-self(A) z = M^-1 g, then y = M z.
+        const size_t n = E.n_unknowns(), off = 3 * (size_t)E.N;
+        mpm::DevBuf<double2> gt;
+        gt.reserve((size_t)B * n);
+        MPM_HIP(hipMemcpyAsync(gt.p, g, (size_t)B * n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        const dim3 grid((unsigned)(((size_t)B * Q + 255) / 256));
+        double2* yv = reinterpret_cast<double2*>(y);
+        hipLaunchKernelGGL(quad_metric_kernel, grid, dim3(256), 0, s, gt.p, B, n, off, Q, 1);
+        hipLaunchKernelGGL(quad_metric_kernel, grid, dim3(256), 0, s, yv, B, n, off, Q, 1);
+        MPM_HIP(hipGetLastError());
+        mpm::solve_batched(E, *desc, gt.p, yv, B, reports, s);
+        hipLaunchKernelGGL(quad_metric_kernel, grid, dim3(256), 0, s, yv, B, n, off, Q, 0);
+        MPM_HIP(hipGetLastError());
+        MPM_HIP(hipStreamSynchronize(s));
     });
 }
 

@@ -149,6 +149,7 @@ class EwaldOp {
     // Unknowns per right-hand side then are [3N dipole | 5Q quadrupole] components.
     void set_quads(const int* idx, int64_t nq);
     size_t n_unknowns() const { return 3 * (size_t)N + 5 * qidx_h.size(); }
+    size_t num_quad_slots() const { return qidx_h.size(); }
     const int* quad_idx_dev() const { return quads ? d_qidx.p : nullptr; }
 
     int64_t N = 0;

@@ -178,9 +178,12 @@ mpm_status mpm_solve(mpm_op* op, const mpm_solve_desc* desc, const double* b, do
 
 /* Adjoint solve A^H y = g for B systems (the gradient solve of an autograd caller;
  * mpm_solve_adjoint of the proposed C ABI, no counterpart in the reference code).
- * The field operator is Hermitian (real symmetric without a Bloch wave vector),
- * so A^H differs from A only in its conjugated self coefficients: the same
- * batched solver runs with them.  Arguments as mpm_solve, with g for b and y for x. */
+ * The dipole field operator is Hermitian (real symmetric without a Bloch wave vector),
+ * so A^H differs from A only in its conjugated self coefficients: the same batched
+ * solver runs with them.  With quadrupoles M A is Hermitian for M = diag(I, W/2), W the
+ * Frobenius product of traceless tensors in (xx, xy, xz, yy, yz), and the solve runs on
+ * M^-1 g and returns y = M z.  Arguments as mpm_solve, with g for b and y for x (y holds
+ * the initial guess on entry); reports refer to the solved system. */
 mpm_status mpm_solve_adjoint(mpm_op* op, const mpm_solve_desc* desc, const double* g, double* y, int B,
                              mpm_solve_report* reports, void* stream);
 

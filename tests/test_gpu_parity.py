@@ -795,3 +795,29 @@ def test_near_field_matches_oracle_with_overlap_and_radius():
     with pytest.raises(NotImplementedError):
         cumpm_amd.Near_Field(box, [1.0, 0, 0], radius=np.linspace(1, 2, 12), dip=dip, dip_pos=dip_pos,
                              field_points=fp).calculate()
+
+
+@pytest.mark.parametrize("bloch", [(0.0, 0.0), (0.12, -0.2)])
+def test_adjoint_solve_with_quadrupoles(bloch):
+    """With quadrupoles the field operator is not Hermitian in the (xx, xy, xz, yy, yz)
+    components; M A is, for M = diag(I, W/2) with W the Frobenius product of traceless
+    tensors, so mpm_solve_adjoint solves through M: checked by <A x, y> = <x, g> for random
+    x, with and without a Bloch wave vector."""
+    L, pos = _quad_geom()
+    _case(f"adjoint quads bloch={bloch}")
+    rng = np.random.default_rng(23)
+    op = Op(L, 1e-8, 0.5, quads=[0, 3, 7])
+    op.set_positions(pos)
+    rows = np.stack([O.self_coefficient(np.full(10, e)) for e in (-2.2 + 0.3j, 3.0 + 0.2j)])
+    op.set_self_coef(rows)
+    op.set_bloch(*bloch)
+    n = op.n
+    g = rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))
+    y, reps = op.solve_adjoint(g, tol=1e-11, restart=60)
+    assert all(r["converged"] for r in reps), reps
+    x = rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))
+    ax = op.apply(x)
+    for q in range(2):
+        lhs = np.vdot(ax[q], y[q])
+        rhs = np.vdot(x[q], g[q])
+        assert abs(lhs - rhs) < 1e-8 * abs(rhs), (q, lhs, rhs)
