@@ -2565,10 +2565,12 @@ __global__ __launch_bounds__(256) void gather_kernel(
     const double* __restrict__ pos, const int* __restrict__ nb_start, const int* __restrict__ nb_j,
     const double2* __restrict__ nb_f, double Lx, double Ly, double Lz, double kbx, double kby,
     const double2* __restrict__ selfc, double self_perp, int jacobi, const R* __restrict__ partial,
-    const int* __restrict__ ps_start, const int* __restrict__ ps_items) {
+    const int* __restrict__ ps_start, const int* __restrict__ ps_items, const int* __restrict__ order) {
     __shared__ int s_off[kMaxWidth * kMaxWidth * kMaxWidth];
     const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    // waves visit the particles in spatial order (neighbouring waves share stencil voxels)
+    const int wv = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int i = wv < N ? order[wv] : N;
     const int width = g.width, half = g.half;
     {   // stencil offsets (ox, oy, oz) packed, in the order matching the grid layout
         const int S0 = width * width * width;
@@ -2938,11 +2940,13 @@ __global__ __launch_bounds__(256) void poly_gather_kernel(SlotTable tab, int B, 
                                                           const int* __restrict__ nb_j,
                                                           const double* __restrict__ pairc,
                                                           const double2* __restrict__ selfc,
-                                                          const double* __restrict__ selfp, int jacobi) {
+                                                          const double* __restrict__ selfp, int jacobi,
+                                                          const int* __restrict__ order) {
     using CT = typename C2<R>::T;
     const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= N) return;  // whole wave
+    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv >= N) return;  // whole wave
+    const int i = order[wv];  // spatial order: neighbouring waves share stencil voxels
     const int g0 = blockIdx.y * G;
     double acc[G][3][2];
 #pragma unroll
@@ -3214,11 +3218,13 @@ __global__ __launch_bounds__(256) void quad_gather_kernel(SlotTable tab, int B, 
                                                           const int* __restrict__ nb_j,
                                                           const double* __restrict__ pairq,
                                                           const double2* __restrict__ selfc, double self_perp,
-                                                          double self_grad, int jacobi) {
+                                                          double self_grad, int jacobi,
+                                                          const int* __restrict__ order) {
     using CT = typename C2<R>::T;
     const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= N) return;  // whole wave
+    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv >= N) return;  // whole wave
+    const int i = order[wv];  // spatial order: neighbouring waves share stencil voxels
     const int g0 = blockIdx.y * G;
     const int qi = qmap[i];
     const int nc = qi >= 0 ? 8 : 3;
@@ -4038,6 +4044,25 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     num_pairs = (int64_t)nj.size();
     h_nb_start = nstart;
     h_nb_j = nj;
+    // visiting order of the wave-per-particle kernels: columns of 8 x 8 voxels in x and
+    // y, ascending z within a column, so that consecutive waves share stencil voxels
+    {
+        std::vector<int64_t> key(Nn);
+        const int64_t nby = (S.n[1] + 7) / 8;
+        for (int i = 0; i < Nn; ++i) {
+            int gv[3];
+            for (int d = 0; d < 3; ++d) {
+                double x = pos[(size_t)d * Nn + i];
+                x -= L[d] * std::floor(x / L[d]);
+                gv[d] = std::min(S.n[d] - 1, std::max(0, (int)(x / S.h[d])));
+            }
+            key[i] = ((int64_t)(gv[0] >> 3) * nby + (gv[1] >> 3)) * S.n[2] + gv[2];
+        }
+        std::vector<int> ord(Nn);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key[a] < key[b]; });
+        d_order.upload(ord.data(), ord.size());
+    }
     d_pos.upload(pos.data(), pos.size());
     d_base.upload(base.data(), base.size());
     d_w64.upload(w64.data(), w64.size());
@@ -4416,12 +4441,14 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
         if (B == 1)
             hipLaunchKernelGGL((gather_kernel<R, 1, SLAB>), dim3(blocks, 1), dim3(256), 0, s, t, B, parts, grid, g,
                                d_zmap.p, d_base.p, w, Ni, h3, d_pos.p, d_nb_start.p, d_nb_j.p, d_nb_f.p, L[0], L[1],
-                               L[2], kb[0], kb[1], d_selfc.p, self_perp, jacobi, part, d_ps_start.p, d_ps_items.p);
+                               L[2], kb[0], kb[1], d_selfc.p, self_perp, jacobi, part, d_ps_start.p, d_ps_items.p,
+                               d_order.p);
         else
             hipLaunchKernelGGL((gather_kernel<R, 4, SLAB>), dim3(blocks, (B + 3) / 4), dim3(256), 0, s, t, B, parts,
                                grid, g,
                                d_zmap.p, d_base.p, w, Ni, h3, d_pos.p, d_nb_start.p, d_nb_j.p, d_nb_f.p, L[0], L[1],
-                               L[2], kb[0], kb[1], d_selfc.p, self_perp, jacobi, part, d_ps_start.p, d_ps_items.p);
+                               L[2], kb[0], kb[1], d_selfc.p, self_perp, jacobi, part, d_ps_start.p, d_ps_items.p,
+                               d_order.p);
         MPM_HIP(hipGetLastError());
     }
 }
@@ -4468,7 +4495,7 @@ void EwaldOp::launch_quads(const SlotTable& t, int B, int parts, hipStream_t s, 
     const double h3 = S.h[0] * S.h[1] * S.h[2];
     hipLaunchKernelGGL((quad_gather_kernel<R, G>), dim3((unsigned)((Ni + 3) / 4), (unsigned)((B + G - 1) / G)),
                        dim3(256), 0, s, t, B, parts, Ni, Vi, SS, grid, sw, d_pidx.p, d_qmap.p, h3, d_nb_start.p,
-                       d_nb_j.p, d_pairc.p, d_selfc.p, self_perp, table->self_grad, jacobi);
+                       d_nb_j.p, d_pairc.p, d_selfc.p, self_perp, table->self_grad, jacobi, d_order.p);
     MPM_HIP(hipGetLastError());
 }
 
@@ -4513,7 +4540,7 @@ void EwaldOp::launch_poly(const SlotTable& t, int B, int parts, hipStream_t s, i
     const double h3 = S.h[0] * S.h[1] * S.h[2];
     hipLaunchKernelGGL((poly_gather_kernel<R, G>), dim3((unsigned)((Ni + 3) / 4), (unsigned)((B + G - 1) / G)),
                        dim3(256), 0, s, t, B, parts, Ni, Vi, SS, grid, pw, d_pidx.p, h3, d_nb_start.p, d_nb_j.p,
-                       d_pairc.p, d_selfc.p, d_selfp.p, jacobi);
+                       d_pairc.p, d_selfc.p, d_selfp.p, jacobi, d_order.p);
     MPM_HIP(hipGetLastError());
 }
 

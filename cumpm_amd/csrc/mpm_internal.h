@@ -199,6 +199,7 @@ class EwaldOp {
     // para - perp, cos, sin, 0) with u = r / max(|r|, 1), r = x_j - x_i (min
     // image) and the Bloch phase -(kb . r); rebuilt when positions or kb change
     std::vector<int> h_nb_start, h_nb_j;  // neighbour list CSR of the current positions
+    DevBuf<int> d_order;                  // [N] particles in spatial order (gather waves)
     std::vector<double> h_pair_r;  // [M][3]
     std::vector<double2> h_pair_f;
     DevBuf<double> d_pairc;        // [M][8]
