@@ -48,6 +48,81 @@ def hex_monolayer_config():
     return pos, box, a, omega, eps
 
 
+def drude(omega):
+    return 4.0 - 1.7 ** 2 / (omega ** 2 + 1j * omega * 0.08)
+
+
+def secondary_workload(name):
+    """The other benchmark workloads (parity cases; timed with --workload): returns
+    (positions, solver keyword arguments, right-hand sides per sweep, description)."""
+    if name == "sc512":
+        g = np.arange(8) * 3.0 + 1.5
+        pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+        kw = dict(box=[24.0] * 3, eps_p=-2.0 + 0.5j, radius=1.0, E0=[[1.0, 0.0, 0.0]], tol=1e-8, restart=30,
+                  grid=(32, 32, 32), support=8)
+        return pos, kw, 1, "simple cubic 8^3, spacing 3, L=24, eps=-2+0.5i, E0=x, grid 32^3, support 8, tol 1e-8"
+    if name == "hex4000-quads":
+        pos, box, a, omega, eps = hex_monolayer_config()
+        kw = dict(box=box, eps_p=eps, radius=a, tol=1e-6, field_type="monodisperse", quadrupoles=True)
+        return pos, kw, 30, "hex monolayer N=4000 with quadrupoles on every particle (n=32000), 10 x 3 RHS"
+    if name == "poly1000":
+        rng = np.random.default_rng(3)
+        L = 76.35
+        rad = rng.uniform(1.0, 4.0, 1000)
+        pos = np.zeros((1000, 3))
+        k = 0
+        while k < 1000:  # random sequential addition, periodic
+            c = rng.uniform(0.0, L, 3)
+            d = pos[:k] - c
+            d -= L * np.round(d / L)
+            if np.all(np.einsum("ij,ij->i", d, d) >= (rad[:k] + rad[k]) ** 2):
+                pos[k] = c
+                k += 1
+        kw = dict(box=[L] * 3, eps_p=drude(np.linspace(0.3, 1.0, 20)), radius=rad, tol=1e-6)
+        return pos, kw, 60, "polydisperse N=1000, radii U(1,4) nm, RSA in 76.35 nm cube, 20 x 3 RHS"
+    if name == "random100k":
+        L, n1 = 111.769, 47
+        rng = np.random.default_rng(5)
+        g = (np.arange(n1) + 0.5) * (L / n1)
+        sites = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+        pos = sites[rng.permutation(len(sites))[:100000]] + rng.uniform(-0.18, 0.18, (100000, 3))
+        kw = dict(box=[L] * 3, eps_p=np.full(4, -3.0 + 0.3j), radius=1.0, xi=np.sqrt(-np.log(1e-6)) / 4.0,
+                  tol=1e-6, grid=(256, 256, 256), support=8)
+        return pos, kw, 12, ("N=100000, a=1, volume fraction 0.3 (jittered lattice subset), 256^3, support 8, "
+                             "rc=4, 4 identical frequencies x 3 RHS")
+    raise SystemExit(f"unknown workload {name}")
+
+
+def run_secondary(cumpm_amd, name, steps, solver_type):
+    """Batched refined sweep (timed over `steps` solves after one warm-up) beside the plain
+    per-frequency FP64 sweep in reference order; one JSON line."""
+    pos, kw, nrhs, desc = secondary_workload(name)
+    out = {}
+    for label, prec, batched, n in (("batched_refined", "refined", True, steps), ("plain_fp64", "double", False, 1)):
+        s = cumpm_amd.dipole_solver(quiet=True, precision=prec, batched=batched, solver_type=solver_type, **kw)
+        s.compute(pos)
+        ms = []
+        for _ in range(n):
+            s.reset()
+            s.compute(pos)
+            ms.append(s.get_timing()[0])
+        reps = [r for w in s.get_reports()[0] for r in w]
+        it = [r["iters"] for r in reps]
+        out[label] = {"value": nrhs / (np.mean(ms) / 1e3), "ms_per_sweep": float(np.mean(ms)),
+                      "iters_min": min(it), "iters_max": max(it),
+                      "converged": all(r["converged"] for r in reps),
+                      "max_relres_pre": max(r["relres_pre"] for r in reps)}
+        out[label + "_dips"] = s.get_dipoles(physical=False)
+    dref = out.pop("plain_fp64_dips")
+    dmix = out.pop("batched_refined_dips")
+    line = {"metric": f"RHS-solves/sec, {name} ({solver_type}), batched+mixed vs plain FP64", "unit": "RHS-solves/s",
+            "value": out["batched_refined"]["value"], "higher_is_better": True, "n_gpus": 1, "steps": steps,
+            "data": "synthetic", "config": {"workload": desc},
+            "speedup_vs_plain_fp64": out["batched_refined"]["value"] / out["plain_fp64"]["value"],
+            "dipole_diff_rel_to_max": float(np.abs(dmix - dref).max() / np.abs(dref).max()), **out}
+    print(json.dumps(line))
+
+
 def make_solver(cumpm_amd, box, eps, a, precision, batched, tol):
     return cumpm_amd.dipole_solver(box, eps, radius=a, eps_m=1.0, xi=0.5, tol=tol, quiet=True,
                                    solver_type="gmres", field_type="monodisperse", precision=precision,
@@ -196,6 +271,12 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 reference-order solve")
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup + steps of the batched solve only (for rocprofv3)")
+    ap.add_argument("--workload", default="hex4000",
+                    choices=["hex4000", "sc512", "hex4000-quads", "poly1000", "random100k"],
+                    help="hex4000 (default) is the headline measurement; the others time the remaining "
+                         "benchmark workloads (batched refined vs plain FP64) and print one JSON line")
+    ap.add_argument("--solver", default="gmres", choices=["gmres", "bicgstab"],
+                    help="Krylov method of a --workload other than hex4000")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device copy (the known-traffic kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -211,6 +292,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(_capi)
+        return
+    if args.workload != "hex4000":
+        run_secondary(cumpm_amd, args.workload, max(args.steps, 1), args.solver)
         return
 
     pos, box, a, omega, eps = hex_monolayer_config()
