@@ -955,14 +955,31 @@ __device__ inline void stage_generic(typename C2<R>::T* src, typename C2<R>::T* 
         fdivmod(rest, Rx, rr, jd, q);
         const CT* sp = src + li * n + jd * Ns + a;
         R ar = R(0), ai = R(0);
-        int kq = 0;
-        for (int r = 0; r < Rx; ++r) {
-            const CT x = sp[r * m];
-            const CT w = roots[kq];
-            ar += x.x * w.x - x.y * w.y;
-            ai += x.x * w.y + x.y * w.x;
-            kq += q;
-            if (kq >= Rx) kq -= Rx;
+        if (Rx & 1) {
+            // odd radix: inputs r and Rx - r meet the roots w and conj(w), so
+            // x_r w + x_{Rx-r} conj(w) = (x_r + x_{Rx-r}) Re w + i (x_r - x_{Rx-r}) Im w
+            const CT x0 = sp[0];
+            ar = x0.x;
+            ai = x0.y;
+            int kq = q;
+            for (int r = 1; 2 * r < Rx; ++r) {
+                const CT xa = sp[r * m], xb = sp[(Rx - r) * m];
+                const CT w = roots[kq];
+                ar += (xa.x + xb.x) * w.x - (xa.y - xb.y) * w.y;
+                ai += (xa.y + xb.y) * w.x + (xa.x - xb.x) * w.y;
+                kq += q;
+                if (kq >= Rx) kq -= Rx;
+            }
+        } else {
+            int kq = 0;
+            for (int r = 0; r < Rx; ++r) {
+                const CT x = sp[r * m];
+                const CT w = roots[kq];
+                ar += x.x * w.x - x.y * w.y;
+                ai += x.x * w.y + x.y * w.x;
+                kq += q;
+                if (kq >= Rx) kq -= Rx;
+            }
         }
         CT o;
         o.x = ar;
