@@ -948,6 +948,43 @@ __device__ inline void stage_generic(typename C2<R>::T* src, typename C2<R>::T* 
         if (r > 0 && a > 0) src[e] = cmul_t(src[e], tw[a * r * step]);
     }
     __syncthreads();
+    if (Rx & 1) {
+        // odd radix: inputs r and Rx - r meet the roots w and conj(w), and outputs q and
+        // Rx - q the roots w and conj(w): one thread forms both outputs from the sums
+        // (x_r + x_{Rx-r}) Re w and (x_r - x_{Rx-r}) Im w
+        const int H = (Rx + 1) / 2, per_line = m * H;
+        const int items = nl * per_line;
+        for (int e = threadIdx.x; e < items; e += blockDim.x) {
+            const int li = e / per_line;
+            const int t = e - li * per_line;
+            const int g = t / H, q = t - g * H;
+            const int jd = g / Ns, a = g - jd * Ns;
+            const CT* sp = src + li * n + jd * Ns + a;
+            const CT x0 = sp[0];
+            R sr = R(0), si = R(0), dr = R(0), di = R(0);
+            int kq = q;
+            for (int r = 1; 2 * r < Rx; ++r) {
+                const CT xa = sp[r * m], xb = sp[(Rx - r) * m];
+                const CT w = roots[kq];
+                sr += (xa.x + xb.x) * w.x;
+                si += (xa.y + xb.y) * w.x;
+                dr += (xa.y - xb.y) * w.y;
+                di += (xa.x - xb.x) * w.y;
+                kq += q;
+                if (kq >= Rx) kq -= Rx;
+            }
+            CT o;
+            o.x = x0.x + sr - dr;
+            o.y = x0.y + si + di;
+            dst[li * n + (jd * Rx + q) * Ns + a] = o;  // natural (li, o) order
+            if (q > 0) {
+                o.x = x0.x + sr + dr;
+                o.y = x0.y + si - di;
+                dst[li * n + (jd * Rx + Rx - q) * Ns + a] = o;
+            }
+        }
+        return;
+    }
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         int li, oi, rest, a, jd, q;
         fdivmod(e, n, rn, li, oi);
@@ -955,31 +992,14 @@ __device__ inline void stage_generic(typename C2<R>::T* src, typename C2<R>::T* 
         fdivmod(rest, Rx, rr, jd, q);
         const CT* sp = src + li * n + jd * Ns + a;
         R ar = R(0), ai = R(0);
-        if (Rx & 1) {
-            // odd radix: inputs r and Rx - r meet the roots w and conj(w), so
-            // x_r w + x_{Rx-r} conj(w) = (x_r + x_{Rx-r}) Re w + i (x_r - x_{Rx-r}) Im w
-            const CT x0 = sp[0];
-            ar = x0.x;
-            ai = x0.y;
-            int kq = q;
-            for (int r = 1; 2 * r < Rx; ++r) {
-                const CT xa = sp[r * m], xb = sp[(Rx - r) * m];
-                const CT w = roots[kq];
-                ar += (xa.x + xb.x) * w.x - (xa.y - xb.y) * w.y;
-                ai += (xa.y + xb.y) * w.x + (xa.x - xb.x) * w.y;
-                kq += q;
-                if (kq >= Rx) kq -= Rx;
-            }
-        } else {
-            int kq = 0;
-            for (int r = 0; r < Rx; ++r) {
-                const CT x = sp[r * m];
-                const CT w = roots[kq];
-                ar += x.x * w.x - x.y * w.y;
-                ai += x.x * w.y + x.y * w.x;
-                kq += q;
-                if (kq >= Rx) kq -= Rx;
-            }
+        int kq = 0;
+        for (int r = 0; r < Rx; ++r) {
+            const CT x = sp[r * m];
+            const CT w = roots[kq];
+            ar += x.x * w.x - x.y * w.y;
+            ai += x.x * w.y + x.y * w.x;
+            kq += q;
+            if (kq >= Rx) kq -= Rx;
         }
         CT o;
         o.x = ar;
