@@ -98,9 +98,14 @@ def run_secondary(cumpm_amd, name, steps, solver_type):
     per-frequency FP64 sweep in reference order; one JSON line."""
     pos, kw, nrhs, desc = secondary_workload(name)
     out = {}
+    from cumpm_amd import _capi
+    s = None
     for label, prec, batched, n in (("batched_refined", "refined", True, steps), ("plain_fp64", "double", False, 1)):
+        s = None  # the previous solver's device memory is released before this one is measured
+        base = _capi.memory_stats(reset_peak=True)[0]
         s = cumpm_amd.dipole_solver(quiet=True, precision=prec, batched=batched, solver_type=solver_type, **kw)
         s.compute(pos)
+        peak = _capi.memory_stats()[1] - base
         ms = []
         for _ in range(n):
             s.reset()
@@ -111,7 +116,8 @@ def run_secondary(cumpm_amd, name, steps, solver_type):
         out[label] = {"value": nrhs / (np.mean(ms) / 1e3), "ms_per_sweep": float(np.mean(ms)),
                       "iters_min": min(it), "iters_max": max(it),
                       "converged": all(r["converged"] for r in reps),
-                      "max_relres_pre": max(r["relres_pre"] for r in reps)}
+                      "max_relres_pre": max(r["relres_pre"] for r in reps),
+                      "peak_device_bytes": peak}
         out[label + "_dips"] = s.get_dipoles(physical=False)
     dref = out.pop("plain_fp64_dips")
     dmix = out.pop("batched_refined_dips")

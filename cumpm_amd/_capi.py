@@ -103,6 +103,7 @@ _SIGS = {
     "mpm_near_field_set_dipole_positions": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _i64]),
     "mpm_near_field_set_field_points": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _i64]),
     "mpm_near_field_calculate": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "mpm_memory_stats": (ctypes.c_int, [ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.c_int]),
     "mpm_dipole_solver_create": (ctypes.c_int, [ctypes.POINTER(DipoleDesc), ctypes.POINTER(ctypes.c_void_p)]),
     "mpm_dipole_solver_destroy": (None, [ctypes.c_void_p]),
     "mpm_dipole_solver_compute": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _i64]),
@@ -228,3 +229,10 @@ def profile_names():
     buf = ctypes.create_string_buffer(need.value + 64)
     check(load().mpm_profile_names(buf, len(buf), ctypes.byref(need)))
     return [n for n in buf.value.decode().split("\n") if n]
+
+
+def memory_stats(reset_peak=False):
+    """(current, peak) device bytes held by the library's own allocations."""
+    cur, peak = _i64(), _i64()
+    check(load().mpm_memory_stats(ctypes.byref(cur), ctypes.byref(peak), int(bool(reset_peak))))
+    return int(cur.value), int(peak.value)

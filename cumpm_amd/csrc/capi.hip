@@ -856,6 +856,15 @@ mpm_status mpm_dipole_solver_reset(mpm_dipole_solver* s) {
     });
 }
 
+mpm_status mpm_memory_stats(int64_t* current_bytes, int64_t* peak_bytes, int reset_peak) {
+    return guarded([&] {
+        MPM_REQUIRE(current_bytes && peak_bytes, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        *current_bytes = mpm::g_dev_bytes.load();
+        *peak_bytes = mpm::g_dev_peak.load();
+        if (reset_peak) mpm::g_dev_peak.store(*current_bytes);
+    });
+}
+
 mpm_status mpm_buffer_alloc(size_t bytes, void** dev) {
     return guarded([&] {
         MPM_REQUIRE(dev, MPM_ERR_INVALID_ARGUMENT, "null argument");

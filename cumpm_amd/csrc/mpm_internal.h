@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <initializer_list>
 #include <map>
+#include <atomic>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -41,6 +42,17 @@ struct Error : std::runtime_error {
         if (!(cond)) throw ::mpm::Error(status, msg); \
     } while (0)
 
+// Device bytes held by the library's own allocations (operators, solver and driver
+// buffers), and their high-water mark since the last reset.
+inline std::atomic<int64_t> g_dev_bytes{0};
+inline std::atomic<int64_t> g_dev_peak{0};
+inline void note_alloc(int64_t bytes) {
+    const int64_t now = g_dev_bytes.fetch_add(bytes) + bytes;
+    int64_t peak = g_dev_peak.load();
+    while (now > peak && !g_dev_peak.compare_exchange_weak(peak, now)) {
+    }
+}
+
 // Owning device allocation.
 template <typename T>
 struct DevBuf {
@@ -51,14 +63,20 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            g_dev_bytes.fetch_sub((int64_t)(n * sizeof(T)));
+        }
         p = nullptr;
         n = 0;
     }
     void resize(size_t count) {
         if (count == n && p) return;
         release();
-        if (count) MPM_HIP(hipMalloc(&p, count * sizeof(T)));
+        if (count) {
+            MPM_HIP(hipMalloc(&p, count * sizeof(T)));
+            note_alloc((int64_t)(count * sizeof(T)));
+        }
         n = count;
     }
     // grow-only variant for scratch buffers

@@ -283,6 +283,11 @@ const char* mpm_status_string(mpm_status s);
 const char* mpm_last_error(void);
 /* Device buffers for callers without an allocator of their own (the Python layer). */
 mpm_status mpm_buffer_alloc(size_t bytes, void** dev);
+/* Device memory held by the library's own allocations (operators with their grids, tables
+ * and per-frame data, solver workspaces, driver buffers; not mpm_buffer_alloc buffers or
+ * rocFFT's internal work areas) and its high-water mark; reset_peak != 0 restarts the mark
+ * at the current value.  No counterpart in the reference. */
+mpm_status mpm_memory_stats(int64_t* current_bytes, int64_t* peak_bytes, int reset_peak);
 mpm_status mpm_buffer_free(void* dev);
 mpm_status mpm_buffer_upload(void* dev, const void* host, size_t bytes);
 mpm_status mpm_buffer_download(void* host, const void* dev, size_t bytes);

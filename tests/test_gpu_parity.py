@@ -821,3 +821,25 @@ def test_adjoint_solve_with_quadrupoles(bloch):
         lhs = np.vdot(ax[q], y[q])
         rhs = np.vdot(x[q], g[q])
         assert abs(lhs - rhs) < 1e-8 * abs(rhs), (q, lhs, rhs)
+
+
+def test_memory_stats_track_operator_allocations():
+    """mpm_memory_stats: an operator's grids and tables show up in the current bytes, the
+    high-water mark covers a solve's workspace, and destroying the operator returns them."""
+    _case("memory stats")
+    base, _ = C.memory_stats(reset_peak=True)
+    box, tol, xi, pos, grid, P = _geom("random3d")
+    op = Op(box, tol, xi)
+    op.set_positions(pos)
+    op.set_self_coef(O.self_coefficient(np.full(len(pos), -2.0 + 0.5j)))
+    held, _ = C.memory_stats()
+    assert held > base
+    b = np.zeros((2, len(pos), 3), complex)
+    b[:, :, 0] = 1.0
+    op.solve(b, tol=1e-8)
+    cur, peak = C.memory_stats()
+    assert peak >= cur >= held
+    del op
+    import gc
+    gc.collect()
+    assert C.memory_stats()[0] <= base + (cur - held)
