@@ -289,7 +289,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the sweep's 30 independent systems fit one MI355X, and "
-              "this build has no multi-GPU path (sharding would split the right-hand sides with no "
+              "this library has no multi-GPU path (sharding would split the right-hand sides with no "
               "communication).  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 

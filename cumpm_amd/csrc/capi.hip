@@ -128,7 +128,7 @@ struct mpm_dipole_solver {
         // dipole_solver.cu:527-536: auto selects the polydisperse operator iff radii differ
         use_poly = field_type == MPM_FIELD_POLYDISPERSE || (field_type == MPM_FIELD_AUTO && !mono);
         if (field_type == MPM_FIELD_DIRECT)
-            throw mpm::Error(MPM_ERR_UNSUPPORTED, "the direct (open-boundary) field is not provided by this build");
+            throw mpm::Error(MPM_ERR_UNSUPPORTED, "the direct (open-boundary) field is not provided by this library");
         if (eps_p.empty()) throw mpm::Error(MPM_ERR_RUNTIME, "eps_p is empty.");
         W = eps_p.size();
         for (auto& row : eps_p) {
@@ -139,7 +139,7 @@ struct mpm_dipole_solver {
         }
         if (Q > 0 && use_poly)
             throw mpm::Error(MPM_ERR_UNSUPPORTED,
-                             "quadrupoles with the polydisperse field are not provided by this build");
+                             "quadrupoles with the polydisperse field are not provided by this library");
         for (int q : quad_idxs)
             if (q < 0 || (size_t)q >= N) throw mpm::Error(MPM_ERR_INVALID_ARGUMENT, "quadrupole index out of range");
         K = E0.size();
@@ -433,10 +433,10 @@ struct mpm_near_field {
             bool mono = true;
             for (size_t i = 1; i < N; ++i) mono = mono && radius[i] == radius[0];
             if (field_type == MPM_FIELD_DIRECT)
-                throw mpm::Error(MPM_ERR_UNSUPPORTED, "the direct (open-boundary) field is not provided by this build");
+                throw mpm::Error(MPM_ERR_UNSUPPORTED, "the direct (open-boundary) field is not provided by this library");
             if (field_type == MPM_FIELD_POLYDISPERSE || (field_type == MPM_FIELD_AUTO && !mono))
                 throw mpm::Error(MPM_ERR_UNSUPPORTED,
-                                 "the near field of the polydisperse operator is not provided by this build");
+                                 "the near field of the polydisperse operator is not provided by this library");
             length_scale = radius[0];  // near_field.cu:120 (the first radius)
             mpm_op_desc od;
             std::memset(&od, 0, sizeof(od));

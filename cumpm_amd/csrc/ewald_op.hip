@@ -3633,7 +3633,7 @@ void EwaldOp::setup_poly_stencils(const std::vector<double>& pos, int Nn) {
 }
 
 void EwaldOp::set_quads(const int* idx, int64_t nq) {
-    MPM_REQUIRE(!poly, MPM_ERR_UNSUPPORTED, "quadrupoles on the polydisperse operator are not provided by this build");
+    MPM_REQUIRE(!poly, MPM_ERR_UNSUPPORTED, "quadrupoles on the polydisperse operator are not provided by this library");
     MPM_REQUIRE(N == 0, MPM_ERR_RUNTIME, "set quadrupoles before the particle positions");
     MPM_REQUIRE(nq < 0 || idx || nq == 0, MPM_ERR_INVALID_ARGUMENT, "null quadrupole indices");
     quads = true;

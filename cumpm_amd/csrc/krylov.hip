@@ -588,7 +588,7 @@ static void gmres_group(Ctx& C, const double2* b, double2* x, int B, mpm_solve_r
     W.sn.reserve((size_t)B * m);
     W.kk.reserve(B);
     W.rows.upload(C.rows_h.data(), B, C.s);
-    MPM_REQUIRE(m <= 63, MPM_ERR_INVALID_ARGUMENT, "GMRES restart must be <= 63 in this build");
+    MPM_REQUIRE(m <= 63, MPM_ERR_INVALID_ARGUMENT, "GMRES restart must be <= 63 in this library");
     const bool inner32 = d.precision != MPM_SOLVE_DOUBLE;
     const bool outer32 = d.precision == MPM_SOLVE_MIXED;
     const size_t vstride = (size_t)(m + 1) * n;

@@ -50,7 +50,7 @@ class MonodisperseEwaldElectricField:
         if isinstance(calc_inter_dipole, bool):
             mode = FieldCalcMode.SOLVER_AX if calc_inter_dipole else FieldCalcMode.INTERACTION_FIELD
         if mode != FieldCalcMode.SOLVER_AX:
-            raise NotImplementedError("INTERACTION_FIELD mode (near fields) is not provided by this build")
+            raise NotImplementedError("INTERACTION_FIELD mode (near fields) is not provided by this library")
         self._prec = _parse_precision(precision)
         d = C.OpDesc()
         d.box[:] = [float(box_x), float(box_y), float(box_z)]

@@ -31,7 +31,7 @@ typedef enum {
     MPM_ERR_RUNTIME = 2,          /* maps to RuntimeError (reference: std::runtime_error) */
     MPM_ERR_HIP = 3,              /* HIP runtime failure (the reference exit()s the process instead) */
     MPM_ERR_FFT = 4,              /* rocFFT failure */
-    MPM_ERR_UNSUPPORTED = 5,      /* feature of the reference not yet provided by this build */
+    MPM_ERR_UNSUPPORTED = 5,      /* feature of the reference not yet provided by this library */
     MPM_ERR_OUT_OF_MEMORY = 6
 } mpm_status;
 
@@ -241,7 +241,7 @@ mpm_status mpm_dipole_solver_get_num_quads(const mpm_dipole_solver* s, int64_t* 
 /* get_quadrupoles: host complex [F][W][Q][K][5], physical units scale by eps_m a_min^4
  * (dipole_solver.cu:640-647). */
 mpm_status mpm_dipole_solver_get_quadrupoles(const mpm_dipole_solver* s, int physical, double* out);
-/* Per-solve reports, [F][W][K] (this build's addition: the reference reports none). */
+/* Per-solve reports, [F][W][K] (an addition: the reference reports none). */
 mpm_status mpm_dipole_solver_get_reports(const mpm_dipole_solver* s, mpm_solve_report* out);
 /* Ewald operator of the solver (valid after the first compute), for inspection. */
 mpm_status mpm_dipole_solver_get_op_info(const mpm_dipole_solver* s, mpm_op_info* info);
@@ -261,7 +261,7 @@ mpm_status mpm_dipole_solver_reset(mpm_dipole_solver* s);
 typedef struct mpm_near_field mpm_near_field;
 /* Near_Field(box, E0, radius, xi, errortol, field_type): box physical [3], E0 host complex [3],
  * radius host [num_radii] (0 => 1.0; lengths are scaled by the first radius), field_type one of
- * mpm_field_type; this build evaluates the monodisperse field only: AUTO with equal radii, or
+ * mpm_field_type; this library evaluates the monodisperse field only: AUTO with equal radii, or
  * MONODISPERSE. */
 mpm_status mpm_near_field_create(const double box[3], const double* E0, const double* radius, int64_t num_radii,
                                  double xi, double errortol, int field_type, mpm_near_field** out);

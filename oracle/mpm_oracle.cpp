@@ -223,7 +223,7 @@ struct Sizing {
 
 // monodisperse_ewald_electric_field.cu:546-601.  grid_override / P_override
 // replace the computed grid or support when positive (the reference ties them
-// to errortol; the override is this build's decoupling knob).
+// to errortol; the overrides decouple them from it, as the GPU operator's do).
 static int compute_sizing(const double L[3], double errortol, double xi,
                           const int* grid_override, int P_override, Sizing& S,
                           std::string& err) {
