@@ -8,4 +8,13 @@ from .solver import dipole_solver, drude_dielectric, PrecisionMode
 from .near_field import Near_Field
 from . import dev
 
-__all__ = ["dipole_solver", "drude_dielectric", "PrecisionMode", "Near_Field", "dev"]
+
+class EELS:
+    """The reference's electron energy-loss driver (src/cuMPM/eels.py) is not part of this
+    library: constructing one raises NotImplementedError."""
+
+    def __init__(self, *args, **kwargs):
+        raise NotImplementedError("EELS is not provided by this library")
+
+
+__all__ = ["dipole_solver", "drude_dielectric", "PrecisionMode", "Near_Field", "EELS", "dev"]

@@ -94,6 +94,8 @@ def test_argument_validation_like_reference():
         cumpm_amd.Near_Field([10.0] * 3, [1.0, 0, 0], field_type="spectral")
     with pytest.raises(ValueError, match="multiple of 3"):
         cumpm_amd.Near_Field([10.0] * 3, [1.0, 0, 0], dip=[1.0, 0.0])
+    with pytest.raises(NotImplementedError):
+        cumpm_amd.EELS([10.0] * 3, [4.0], [0.1], 0.3)
     cumpm_amd.dipole_solver([10.0] * 3, -2.0, quadrupoles=True, quiet=True)
     cumpm_amd.dipole_solver([10.0] * 3, -2.0, quadrupoles=(0, 2), quiet=True)
     s = cumpm_amd.dipole_solver([10.0] * 3, [-2.0 + 0.1j, -1.0 + 0.1j], E0=[[1, 0, 0], [0, 0, 1]], quiet=True)
