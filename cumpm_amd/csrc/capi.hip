@@ -263,28 +263,49 @@ struct mpm_dipole_solver {
         if (n == 0) return;
         auto t0 = std::chrono::steady_clock::now();
         if (N == 0) {
-            set_dims(n);
-            nondimensionalize();
-            mpm_op_desc od;
-            std::memset(&od, 0, sizeof(od));
-            for (int d = 0; d < 3; ++d) {
-                od.box[d] = box[d];
-                od.grid[d] = grid[d];
+            // set-up is all or nothing: on failure (a box too small for the cutoff, bad
+            // radii, device errors) the configuration is restored and N stays 0, so a
+            // retried compute() starts again from the physical inputs
+            const std::vector<double> box0 = box, radius0 = radius;
+            const std::vector<std::vector<cplx>> eps0 = eps_p;
+            const double eps_m0 = eps_m;
+            try {
+                set_dims(n);
+                nondimensionalize();
+                mpm_op_desc od;
+                std::memset(&od, 0, sizeof(od));
+                for (int d = 0; d < 3; ++d) {
+                    od.box[d] = box[d];
+                    od.grid[d] = grid[d];
+                }
+                od.errortol = ewald_tol > 0 ? ewald_tol : tol;
+                od.xi = xi;
+                od.support = support;
+                std::unique_ptr<mpm::EwaldOp> o;
+                if (use_poly)
+                    o = std::make_unique<mpm::EwaldOp>(od, radius.data(), (int64_t)N);
+                else
+                    o = std::make_unique<mpm::EwaldOp>(od);
+                if (Q > 0)
+                    o->set_quads(quad_idxs.empty() ? nullptr : quad_idxs.data(),
+                                 quad_idxs.empty() ? -1 : (int64_t)quad_idxs.size());
+                op = std::move(o);
+            } catch (...) {
+                box = box0;
+                radius = radius0;
+                eps_p = eps0;
+                eps_m = eps_m0;
+                N = W = K = Q = 0;
+                length_scale = eps_scale = 1.0;
+                vol_frac = 0.0;
+                use_poly = false;
+                op.reset();
+                throw;
             }
-            od.errortol = ewald_tol > 0 ? ewald_tol : tol;
-            od.xi = xi;
-            od.support = support;
-            if (use_poly)
-                op = std::make_unique<mpm::EwaldOp>(od, radius.data(), (int64_t)N);
-            else
-                op = std::make_unique<mpm::EwaldOp>(od);
-            if (Q > 0)
-                op->set_quads(quad_idxs.empty() ? nullptr : quad_idxs.data(),
-                              quad_idxs.empty() ? -1 : (int64_t)quad_idxs.size());
         } else if (N != n) {
             throw mpm::Error(MPM_ERR_RUNTIME, "The number of particles has changed!");
         }
-        frames++;
+        // frames counts committed frames only: it grows after this frame's results are stored
         std::vector<double> xyz(3 * N);
         for (size_t i = 0; i < N; ++i) {
             xyz[3 * i] = x[i] / length_scale;
@@ -295,10 +316,10 @@ struct mpm_dipole_solver {
         const size_t n3 = n_unknowns();
         // initial guess: mean field, or the previous frame's first frequency (dipole_solver.cu:578-592)
         std::vector<cplx> init;
-        if (frames == 1) {
+        if (frames == 0) {
             init = mean_field(0);
         } else {
-            const size_t off = (frames - 2) * W * N * K * 3, qoff = (frames - 2) * W * Q * K * 5;
+            const size_t off = (frames - 1) * W * N * K * 3, qoff = (frames - 1) * W * Q * K * 5;
             init.assign(dips.begin() + off, dips.begin() + off + N * K * 3);
             init.insert(init.end(), quads.begin() + qoff, quads.begin() + qoff + Q * K * 5);
         }
@@ -380,16 +401,20 @@ struct mpm_dipole_solver {
         total_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
         solve_ms = device_ms;
         // alpha_eff: mean dipole per (w, k)
+        std::vector<cplx> avg(W * K * 3);
         for (size_t w = 0; w < W; ++w)
             for (size_t k = 0; k < K; ++k)
                 for (int c = 0; c < 3; ++c) {
                     cplx s = 0.0;
                     for (size_t p = 0; p < N; ++p) s += fd[(w * N + p) * K * 3 + k * 3 + c];
-                    avg_dips.push_back(s / (double)N);
+                    avg[(w * K + k) * 3 + c] = s / (double)N;
                 }
+        // commit the frame
+        avg_dips.insert(avg_dips.end(), avg.begin(), avg.end());
         dips.insert(dips.end(), fd.begin(), fd.end());
         quads.insert(quads.end(), fq.begin(), fq.end());
         reports.insert(reports.end(), reps.begin(), reps.end());
+        frames++;
     }
 };
 

@@ -3796,7 +3796,10 @@ void EwaldOp::set_bloch(double kx, double ky) {
     kb[0] = kx;
     kb[1] = ky;
     build_kspace();
-    if (N > 0) build_pair_coefs();
+    if (N > 0) {
+        build_pair_coefs();
+        ensure_kspace_table();
+    }
 }
 
 void EwaldOp::build_pair_coefs() {
@@ -3849,7 +3852,8 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     if (n == N && last_xyz.size() == (size_t)(3 * n) &&
         std::memcmp(last_xyz.data(), xyz, 3 * n * sizeof(double)) == 0)
         return;
-    last_xyz.assign(xyz, xyz + 3 * n);
+    // the cache of precalcs is valid only once this call has finished
+    last_xyz.clear();
     N = n;
     const int W = S.width, half = S.half;
     std::vector<double> pos(3 * (size_t)Nn);
@@ -4215,6 +4219,10 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         d_selfc.resize(Nn);
         MPM_HIP(hipMemset(d_selfc.p, 0, Nn * sizeof(double2)));
     }
+    // k-space tables of the layout just chosen: built here, so that applies allocate
+    // nothing beyond their grids
+    ensure_kspace_table();
+    last_xyz.assign(xyz, xyz + 3 * n);
 }
 
 void EwaldOp::set_self_coef(const double* coef, int rows) {

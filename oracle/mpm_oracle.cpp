@@ -33,9 +33,12 @@
 // tests/golden/ref_dipole_monodisperse.npz and ref_quadrupoles_monodisperse.npz,
 // numpy.fft, a textbook point-dipole Ewald lattice sum (non-overlapping spheres),
 // quadrature of the overlapping-sphere field (1 <= r < 2), xi-independence of
-// real+recip (every dipole and quadrupole coupling), and a dense
-// np.linalg.solve for the Krylov solvers.  Parity unpinned: the overlap
-// polynomials (r < 2) of the seven quadrupole tables.
+// real+recip (every dipole and quadrupole coupling), the quadrupole tables
+// (overlap polynomials and the gradient self term included) against an
+// electrostatic model of the polarised spheres evaluated by real- and k-space
+// quadrature, and a dense np.linalg.solve for the Krylov solvers.  Parity
+// unpinned: results at the cuFFT boundary (no reference test holds them; the
+// FFT is pinned by numpy.fft).
 
 #include <algorithm>
 #include <cmath>

@@ -102,3 +102,18 @@ def test_argument_validation_like_reference():
     assert s.use_jacobi_precond is True
     s.use_jacobi_precond = False
     assert s.use_jacobi_precond is False
+
+
+def test_failed_setup_leaves_solver_retryable():
+    # a box too small for the real-space cutoff fails in the operator constructor (before any
+    # device work); the solver must stay unconfigured, so a retry fails the same way instead of
+    # using a half-built state (radius 2: a second nondimensionalisation would change the box)
+    s = cumpm_amd.dipole_solver([10.0] * 3, -2.0 + 0.1j, radius=2.0, tol=1e-6, quiet=True)
+    pos = np.array([[1.0, 1.0, 1.0], [6.0, 6.0, 6.0]])
+    with pytest.raises(RuntimeError, match="Real space cutoff") as e1:
+        s.compute(pos)
+    assert s._dims()[0] == 0 and s._dims()[2] == 0
+    with pytest.raises(RuntimeError, match="Real space cutoff") as e2:
+        s.compute(pos)
+    assert str(e1.value) == str(e2.value)
+    assert s._dims()[0] == 0 and s._dims()[2] == 0

@@ -124,6 +124,98 @@ def test_hex_monolayer_sweep_refined_vs_fp64():
     assert np.all(_pre_residual(op, xs, bs, sc) < 1.01e-6)
 
 
+def _hex4000_golden():
+    import os
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", "hex4000_resonant.npz"))
+
+
+def test_hex_monolayer_full_size_applies_match_oracle():
+    """The headline geometry at its own size on the GPU path it runs: the 9-plane slab of the
+    210 x 114 x 61 grid (built-in FFT, matrix-core spread and contraction, Toeplitz plane
+    coupling; FP32 the fused x-transform/Toeplitz kernel).  FP64 applies against the oracle to
+    1e-10 for a narrow (B = 3) and a wide (B = 5) batch with per-system self coefficients; FP32
+    applies within the reference's 1e-4."""
+    g = _hex4000_golden()
+    L, P, eps = g["box"], g["positions"], g["eps"]
+    _case("hex monolayer N=4000 full-size applies vs oracle")
+    op = Op(L, 1e-6, 0.5)
+    op.set_positions(P)
+    info = op.info()
+    assert tuple(info.grid) == (210, 114, 61) and info.slab_planes == 9
+    ref = O.Operator(L, 1e-6, 0.5)
+    ref.set_positions(P)
+    assert ref.info()["grid"] == (210, 114, 61) and ref.info()["num_pairs"] == info.num_pairs
+    rng = np.random.default_rng(42)
+    N = len(P)
+    for B in (3, 5):
+        x = rng.standard_normal((B, N, 3)) + 1j * rng.standard_normal((B, N, 3))
+        rows = np.stack([np.full(N, O.self_coefficient(eps[(3 * b) % 10])) for b in range(B)])
+        op.set_self_coef(rows)
+        got = op.apply(x, C.PREC_DOUBLE)
+        got32 = op.apply(x, C.PREC_MIXED)
+        for b in range(B):
+            want = ref.apply(x[b], rows[b])
+            assert rel(got[b], want) < 1e-10, (B, b)
+            assert rel(got32[b], want) < 1e-4, (B, b)
+
+
+def test_hex_monolayer_resonant_systems_match_oracle_solutions():
+    """Three near-resonant systems of the headline sweep (frequency index 6 and 7) against the
+    oracle's solutions at tol 1e-11 (tests/golden/make_hex4000_golden.py): the batched
+    FP32-inner / FP64-refined sweep of all 30 systems at tol 1e-6 within 1e-6, and an FP64
+    batched solve of the three at tol 1e-10 within 1e-8."""
+    g = _hex4000_golden()
+    L, P, eps, systems, xg = g["box"], g["positions"], g["eps"], g["systems"], g["x"]
+    N = len(P)
+    _case("hex monolayer N=4000 refined sweep vs oracle solutions")
+    s = cumpm_amd.dipole_solver(L * 5.0, eps, radius=5.0, tol=1e-6, quiet=True, field_type="monodisperse",
+                                precision="refined", batched=True)
+    s.compute(P * 5.0)
+    d = s.get_dipoles(physical=False)  # (W, N, K, 3)
+    for q, (w, k) in enumerate(systems):
+        assert rel(d[w, :, k], xg[q]) < 1e-6, (w, k, rel(d[w, :, k], xg[q]))
+    _case("hex monolayer N=4000 FP64 solves at tol 1e-10 vs oracle solutions")
+    op = Op(L, 1e-6, 0.5)
+    op.set_positions(P)
+    rows = np.stack([np.full(N, O.self_coefficient(eps[w])) for w, _ in systems])
+    op.set_self_coef(rows)
+    b = np.zeros((len(systems), N, 3), np.complex128)
+    for q, (_, k) in enumerate(systems):
+        b[q, :, k] = 1.0
+    x, reps = op.solve(b, tol=1e-10, precision=C.SOLVE_DOUBLE, maxiter=5000)
+    assert all(r["converged"] for r in reps)
+    for q in range(len(systems)):
+        assert rel(x[q], xg[q]) < 1e-8, (q, rel(x[q], xg[q]))
+
+
+def test_driver_frame_counter_survives_a_failed_frame():
+    """A frame whose positions are rejected (non-finite) leaves no trace: the next valid frame
+    warm-starts from the last committed one and the effective polarisability averages the
+    committed frames only."""
+    _case("driver failed frame")
+    rng = np.random.default_rng(8)
+    pos1 = rng.uniform(0, 20, (12, 3))
+    pos3 = pos1 + rng.normal(0, 0.05, pos1.shape)
+    eps = np.array([-2.5 + 0.4j, -2.0 + 0.5j])
+    s = cumpm_amd.dipole_solver([20.0] * 3, eps, tol=1e-8, quiet=True, precision="double")
+    s.compute(pos1)
+    bad = pos1.copy()
+    bad[3, 1] = np.nan
+    with pytest.raises(ValueError, match="non-finite"):
+        s.compute(bad)
+    assert s._dims()[0] == 1
+    s.compute(pos3)
+    assert s._dims()[0] == 2
+    d = s.get_dipoles(physical=False)
+    assert d.shape == (2, 2, 12, 3, 3)
+    alpha = s.get_eff_polarizability(physical=False)
+    np.testing.assert_allclose(alpha, d.mean(axis=2).mean(axis=0), rtol=1e-12, atol=1e-14)
+    od = O.DipoleSolver([20.0] * 3, eps[:, None], 1.0, tol=1e-8)
+    od.compute(pos1)
+    od.compute(pos3)
+    assert rel(d, np.stack(od.dips)) < 1e-6
+
+
 def _rsa_poly(N=1000, L=76.35, seed=3):
     rng = np.random.default_rng(seed)
     rad = rng.uniform(1.0, 4.0, N)

@@ -8,7 +8,8 @@
   quadrupole -> field, dipole -> gradient, quadrupole -> gradient) with pairs
   inside the cutoff, overlapping ones included: this pins the seven quadrupole
   real-space tables' screened parts against the reciprocal sum.  Their
-  overlap polynomials (r < 2) do not depend on xi: parity unpinned there.
+  overlap polynomials (r < 2) do not depend on xi; tests/test_oracle_quad_overlap.py
+  pins them against an electrostatic model by quadrature.
 """
 import os
 
