@@ -90,6 +90,8 @@ _SIGS = {
     "mpm_op_set_self_coef": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
     "mpm_op_apply": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "mpm_op_apply_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "mpm_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolveDesc), ctypes.c_void_p,
                                  ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(SolveReport),
                                  ctypes.c_void_p]),

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <iostream>
+#include <mutex>
 
 #include "mpm_internal.h"
 
@@ -496,6 +497,39 @@ struct mpm_near_field {
 };
 
 // ---------------------------------------------------------------------------
+// Scratch of mpm_op_apply_noalloc: the reciprocal grids of one apply live in this
+// file-scope device array instead of the operator's own (allocated on demand)
+// buffers.  Every GPU the library runs on holds its own copy until the process
+// ends.  In host code `scratch` names a host-side stand-in, not device memory: the
+// array's address on each device comes from hipGetSymbolAddress (scratch_on_device)
+// and only that address reaches kernels.  Calls on one device share the array, so
+// calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+__device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch array's address on the current device, looked up once per device.
+char* scratch_on_device() {
+    int dev = 0;
+    MPM_HIP(hipGetDevice(&dev));
+    MPM_REQUIRE(dev >= 0 && dev < kMaxDevices, MPM_ERR_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        MPM_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        MPM_REQUIRE(ptr != nullptr, MPM_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = static_cast<char*>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // extern "C"
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -654,6 +688,31 @@ mpm_status mpm_op_apply(mpm_op* op, const double* x, double* y, int B, int preci
                     "need one self-coefficient row per right-hand side (or a single shared row)");
         E.apply_contiguous(reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y), B,
                            precision != MPM_PREC_DOUBLE, parts, static_cast<hipStream_t>(stream));
+        MPM_HIP(hipGetLastError());
+    });
+}
+
+mpm_status mpm_op_apply_noalloc(mpm_op* op, const double* x, double* y, int B, int precision, int parts,
+                                void* stream) {
+    return guarded([&] {
+        MPM_REQUIRE(op && x && y, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        MPM_REQUIRE(B >= 1, MPM_ERR_INVALID_ARGUMENT, "B must be >= 1");
+        MPM_REQUIRE(parts >= 1 && parts <= 3, MPM_ERR_INVALID_ARGUMENT, "parts must be 1, 2 or 3");
+        MPM_REQUIRE(precision >= 0 && precision <= 2, MPM_ERR_INVALID_ARGUMENT, "unknown precision");
+        auto& E = *op->impl;
+        MPM_REQUIRE(E.N > 0, MPM_ERR_RUNTIME, "apply before set_positions");
+        MPM_REQUIRE(E.self_rows == 1 || E.self_rows >= B, MPM_ERR_INVALID_ARGUMENT,
+                    "need one self-coefficient row per right-hand side (or a single shared row)");
+        const bool fp32 = precision != MPM_PREC_DOUBLE;
+        const size_t need = E.scratch_bytes(std::min(B, mpm::kMaxSlots), fp32);
+        MPM_REQUIRE(need <= kNoAllocScratchBytes, MPM_ERR_INVALID_ARGUMENT,
+                    "mpm_op_apply_noalloc: " + std::to_string(need) + " bytes of grid scratch needed, " +
+                        std::to_string(kNoAllocScratchBytes) + " available (use fewer right-hand sides or mpm_op_apply)");
+        mpm::EwaldOp::Workspace ws;
+        ws.base = scratch_on_device();
+        ws.bytes = kNoAllocScratchBytes;
+        E.apply_contiguous(reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y), B, fp32, parts,
+                           static_cast<hipStream_t>(stream), nullptr, &ws);
         MPM_HIP(hipGetLastError());
     });
 }

@@ -4245,7 +4245,7 @@ template <typename R, bool SLAB>
 void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int jacobi) {
     using CT = typename C2<R>::T;
     const bool fp32 = std::is_same<R, float>::value;
-    CT* grid = fp32 ? reinterpret_cast<CT*>(d_grid32.p) : reinterpret_cast<CT*>(d_grid64.p);
+    CT* grid = reinterpret_cast<CT*>(cur_grid);
     const R* w = fp32 ? reinterpret_cast<const R*>(d_w32.p) : reinterpret_cast<const R*>(d_w64.p);
     SpreadGeom g;
     g.nx = S.n[0];
@@ -4304,7 +4304,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
         }
         const size_t bytes_per_rhs = grid_elems_per_rhs() * sizeof(CT);
         CT* kgrid = grid;
-        if (kt) kgrid = fp32 ? reinterpret_cast<CT*>(d_kgrid32.p) : reinterpret_cast<CT*>(d_kgrid64.p);
+        if (kt) kgrid = reinterpret_cast<CT*>(cur_kgrid);
         // contiguous slab, transposed k-space rows, a two-stage x transform whose rows
         // fit in LDS: x transform, plane coupling and inverse x transform in one kernel
         const int xaxis = (SLAB && zcontig && kt && lfft && fft_backend == 0) ? lfft->kt_x : -1;
@@ -4504,7 +4504,7 @@ template <typename R>
 void EwaldOp::launch_quads(const SlotTable& t, int B, int parts, hipStream_t s, int jacobi) {
     using CT = typename C2<R>::T;
     const bool fp32 = std::is_same<R, float>::value;
-    CT* grid = fp32 ? reinterpret_cast<CT*>(d_grid32.p) : reinterpret_cast<CT*>(d_grid64.p);
+    CT* grid = reinterpret_cast<CT*>(cur_grid);
     const R* sw = fp32 ? reinterpret_cast<const R*>(d_sw32.p) : reinterpret_cast<const R*>(d_sw64.p);
     const int SS = S.width * S.width * S.width;
     const int Vi = (int)V, Ni = (int)N;
@@ -4548,7 +4548,7 @@ template <typename R>
 void EwaldOp::launch_poly(const SlotTable& t, int B, int parts, hipStream_t s, int jacobi) {
     using CT = typename C2<R>::T;
     const bool fp32 = std::is_same<R, float>::value;
-    CT* grid = fp32 ? reinterpret_cast<CT*>(d_grid32.p) : reinterpret_cast<CT*>(d_grid64.p);
+    CT* grid = reinterpret_cast<CT*>(cur_grid);
     const R* pw = fp32 ? reinterpret_cast<const R*>(d_pw32.p) : reinterpret_cast<const R*>(d_pw64.p);
     const int SS = S.P * S.P * S.P;
     const int Vi = (int)V;
@@ -4589,21 +4589,62 @@ void EwaldOp::launch_poly(const SlotTable& t, int B, int parts, hipStream_t s, i
     MPM_HIP(hipGetLastError());
 }
 
-void EwaldOp::apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t s, bool jacobi) {
+__global__ void zero_words_kernel(uint4* __restrict__ p, size_t n) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        p[e] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t EwaldOp::scratch_bytes(int B, bool fp32) const {
+    const size_t cs = fp32 ? sizeof(float2) : sizeof(double2);
+    return align256((size_t)B * grid_elems_per_rhs() * cs) + (kt ? align256((size_t)B * kgrid_elems_per_rhs() * cs) : 0);
+}
+
+bool EwaldOp::builtin_fft() const { return lfft && fft_backend == 0; }
+
+void EwaldOp::apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t s, bool jacobi,
+                    const Workspace* ws) {
     MPM_REQUIRE(N > 0, MPM_ERR_RUNTIME, "apply before set_positions");
     MPM_REQUIRE(B >= 1 && B <= kMaxSlots, MPM_ERR_INVALID_ARGUMENT, "bad slot count");
     for (int b = 0; b < B; ++b)
         MPM_REQUIRE(t.row[b] >= 0 && t.row[b] < self_rows, MPM_ERR_INVALID_ARGUMENT,
                     "self-coefficient row out of range");
-    if (parts & MPM_PART_RECIP) ensure_kspace_table();
     const size_t elems = (size_t)B * grid_elems_per_rhs();
     const size_t kelems = (size_t)B * kgrid_elems_per_rhs();
-    if (fp32) {
-        d_grid32.reserve(elems);
-        if (kt) d_kgrid32.reserve(kelems);
+    if (ws) {
+        // caller's workspace: nothing may allocate, so the k-space tables must already be
+        // current (set_positions and set_bloch build them) and the FFT must be the built-in one
+        MPM_REQUIRE(!(parts & MPM_PART_RECIP) || (kspace_ready && builtin_fft()), MPM_ERR_UNSUPPORTED,
+                    "an apply without allocation needs the built-in FFT and current k-space tables");
+        const size_t need = scratch_bytes(B, fp32);
+        MPM_REQUIRE(need <= ws->bytes, MPM_ERR_INVALID_ARGUMENT,
+                    "the apply needs " + std::to_string(need) + " bytes of scratch, more than the " +
+                        std::to_string(ws->bytes) + " available");
+        const size_t cs = fp32 ? sizeof(float2) : sizeof(double2);
+        cur_grid = ws->base;
+        cur_kgrid = kt ? ws->base + align256(elems * cs) : nullptr;
+        // the grids hold the previous call's data: start from zeros as freshly cleared memory
+        if (parts & MPM_PART_RECIP) {
+            const size_t words = need / sizeof(uint4);
+            const int blocks = (int)std::min<size_t>((words + 255) / 256, 8192);
+            hipLaunchKernelGGL(zero_words_kernel, dim3(blocks), dim3(256), 0, s, reinterpret_cast<uint4*>(ws->base),
+                               words);
+            MPM_HIP(hipGetLastError());
+        }
     } else {
-        d_grid64.reserve(elems);
-        if (kt) d_kgrid64.reserve(kelems);
+        if (parts & MPM_PART_RECIP) ensure_kspace_table();
+        if (fp32) {
+            d_grid32.reserve(elems);
+            if (kt) d_kgrid32.reserve(kelems);
+            cur_grid = d_grid32.p;
+            cur_kgrid = d_kgrid32.p;
+        } else {
+            d_grid64.reserve(elems);
+            if (kt) d_kgrid64.reserve(kelems);
+            cur_grid = d_grid64.p;
+            cur_kgrid = d_kgrid64.p;
+        }
     }
     const int jac = jacobi ? 1 : 0;
     if (quads) {
@@ -4626,7 +4667,7 @@ void EwaldOp::apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t
 }
 
 void EwaldOp::apply_contiguous(const double2* x, double2* y, int B, bool fp32, int parts, hipStream_t s,
-                               const int* rows) {
+                               const int* rows, const Workspace* ws) {
     const size_t n = n_unknowns();
     for (int b0 = 0; b0 < B; b0 += kMaxSlots) {
         const int cnt = std::min(kMaxSlots, B - b0);
@@ -4636,7 +4677,7 @@ void EwaldOp::apply_contiguous(const double2* x, double2* y, int B, bool fp32, i
             t.y[j] = y + (size_t)(b0 + j) * n;
             t.row[j] = rows ? rows[b0 + j] : (self_rows > 1 ? b0 + j : 0);
         }
-        apply(t, cnt, fp32, parts, s);
+        apply(t, cnt, fp32, parts, s, false, ws);
     }
 }
 

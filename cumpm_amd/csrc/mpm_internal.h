@@ -152,12 +152,23 @@ class EwaldOp {
     void set_self_coef(const double* coef, int rows);
     // conjugates the device self coefficients in place (the adjoint operator's)
     void conj_self_coef(hipStream_t s);
+    // Device memory a caller lends to an apply for its reciprocal grids (base 256-byte
+    // aligned); with one, apply allocates nothing.
+    struct Workspace {
+        char* base = nullptr;
+        size_t bytes = 0;
+    };
     // y[b] = A x[b] (or M^-1 A x[b] with the left Jacobi preconditioner) for the slots
-    // in `t` (count B <= kMaxSlots)
-    void apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t s, bool jacobi = false);
+    // in `t` (count B <= kMaxSlots); the grids live in `ws` when given (then the k-space
+    // tables must be current and the FFT the built-in one), else in the operator's buffers
+    void apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t s, bool jacobi = false,
+               const Workspace* ws = nullptr);
     // convenience: contiguous [B][3N] buffers, chunked by kMaxSlots
     void apply_contiguous(const double2* x, double2* y, int B, bool fp32, int parts,
-                          hipStream_t s, const int* rows = nullptr);
+                          hipStream_t s, const int* rows = nullptr, const Workspace* ws = nullptr);
+    // workspace bytes an apply of B (<= kMaxSlots) right-hand sides needs
+    size_t scratch_bytes(int B, bool fp32) const;
+    bool builtin_fft() const;
     void info(mpm_op_info& out) const;
     // host copies for inspection: the monodisperse pair tables (r, perp, para; table 2..8 =
     // the quadrupole tables when built) and the current neighbour list with its pair values
@@ -248,7 +259,10 @@ class EwaldOp {
     DevBuf<float> d_k32;
     DevBuf<double2> d_toe64;
     DevBuf<float2> d_toe32;
-    // grids
+    // grids (cur_grid / cur_kgrid: those of the apply in flight, the operator's own buffers
+    // or the caller's workspace)
+    void* cur_grid = nullptr;
+    void* cur_kgrid = nullptr;
     DevBuf<double2> d_grid64;
     DevBuf<float2> d_grid32;
     DevBuf<double2> d_kgrid64;   // slab k-space buffer (transposed columns)

@@ -151,6 +151,19 @@ mpm_status mpm_op_set_self_coef(mpm_op* op, const double* coef, int n_rows);
 mpm_status mpm_op_apply(mpm_op* op, const double* x, double* y, int B, int precision, int parts,
                         void* stream);
 
+/* mpm_op_apply that allocates no device memory.  No counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The reciprocal grids of the apply live in a
+ * library-owned static device array of 64 MiB (each GPU the library runs on holds its own
+ * copy until the process ends); sizes that need more (grid elements x right-hand sides,
+ * up to 32 right-hand sides per pass) fail with MPM_ERR_INVALID_ARGUMENT.  The operator's
+ * k-space tables are built by mpm_op_set_positions / mpm_op_set_bloch, so after those the
+ * apply calls no allocating routine; it needs the built-in FFT (MPM_ERR_UNSUPPORTED with
+ * rocFFT).  Results equal mpm_op_apply's.  Calls on one GPU share the array: calls that
+ * overlap on one GPU (two streams, two threads) race. */
+mpm_status mpm_op_apply_noalloc(mpm_op* op, const double* x, double* y, int B, int precision, int parts,
+                                void* stream);
+
 typedef struct {
     int method;     /* mpm_method */
     int restart;    /* GMRES restart length (reference default 50) */
