@@ -73,6 +73,18 @@ class DipoleDesc(ctypes.Structure):
                 ("support", ctypes.c_int), ("ewald_tol", ctypes.c_double)]
 
 
+class EelsDesc(ctypes.Structure):
+    _fields_ = [("box", ctypes.c_double * 3), ("eps_p", _dp), ("num_wavevectors", _i64), ("eps_cols", _i64),
+                ("omega", _dp), ("num_omega", _i64), ("v", ctypes.c_double), ("radius", _dp),
+                ("num_radii", _i64), ("eps_m", ctypes.c_double), ("xi", ctypes.c_double),
+                ("tol", ctypes.c_double), ("quiet", ctypes.c_int), ("guess", ctypes.c_int),
+                ("method", ctypes.c_int), ("field_type", ctypes.c_int), ("integration_step", ctypes.c_double),
+                ("solve_quadrupoles", ctypes.c_int), ("quad_idxs", ctypes.POINTER(ctypes.c_int)),
+                ("num_quad_idxs", _i64), ("asm_flag", ctypes.c_int), ("asm_nx", ctypes.c_int),
+                ("asm_ny", ctypes.c_int), ("precision", ctypes.c_int), ("refined", ctypes.c_int),
+                ("batched", ctypes.c_int), ("restart", ctypes.c_int), ("maxiter", ctypes.c_int)]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "mpm_status_string": (ctypes.c_char_p, [ctypes.c_int]),
@@ -80,6 +92,7 @@ _SIGS = {
     "mpm_op_create": (ctypes.c_int, [ctypes.POINTER(OpDesc), ctypes.POINTER(ctypes.c_void_p)]),
     "mpm_op_set_quads": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), _i64]),
     "mpm_op_create_poly": (ctypes.c_int, [ctypes.POINTER(OpDesc), _dp, _i64, ctypes.POINTER(ctypes.c_void_p)]),
+    "mpm_op_create_direct": (ctypes.c_int, [_dp, _i64, ctypes.POINTER(ctypes.c_void_p)]),
     "mpm_op_destroy": (None, [ctypes.c_void_p]),
     "mpm_op_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OpInfo)]),
     "mpm_op_get_pair_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _i64, ctypes.POINTER(_i64)]),
@@ -120,6 +133,18 @@ _SIGS = {
     "mpm_dipole_solver_get_use_jacobi": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "mpm_dipole_solver_get_timing": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "mpm_dipole_solver_reset": (ctypes.c_int, [ctypes.c_void_p]),
+    "mpm_eels_solver_create": (ctypes.c_int, [ctypes.POINTER(EelsDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    "mpm_eels_solver_destroy": (None, [ctypes.c_void_p]),
+    "mpm_eels_solver_compute": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _i64]),
+    "mpm_eels_solver_get_dims": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i64)]),
+    "mpm_eels_solver_get_eels": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "mpm_eels_solver_get_dipoles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
+    "mpm_eels_solver_get_quadrupoles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
+    "mpm_eels_solver_get_incident_field": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "mpm_eels_solver_get_reports": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _i64, ctypes.POINTER(_i64)]),
+    "mpm_eels_solver_set_use_jacobi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "mpm_eels_solver_get_use_jacobi": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "mpm_eels_solver_get_timing": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "mpm_buffer_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
     "mpm_buffer_free": (ctypes.c_int, [ctypes.c_void_p]),
     "mpm_buffer_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),

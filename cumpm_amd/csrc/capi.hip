@@ -50,27 +50,8 @@ __global__ void __launch_bounds__(256) quad_metric_kernel(double2* __restrict__ 
 }
 
 
-thread_local std::string g_last_error;
-
-template <typename F>
-mpm_status guarded(F&& f) {
-    try {
-        f();
-        g_last_error.clear();
-        return MPM_OK;
-    } catch (const mpm::Error& e) {
-        g_last_error = e.what();
-        return e.status;
-    } catch (const std::bad_alloc&) {
-        g_last_error = "host allocation failed";
-        return MPM_ERR_OUT_OF_MEMORY;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return MPM_ERR_RUNTIME;
-    }
-}
-
 using mpm::cplx;
+using mpm::guarded;
 
 }  // namespace
 
@@ -98,7 +79,7 @@ struct mpm_dipole_solver {
     size_t N = 0, W = 0, K = 0, Q = 0, frames = 0;
     double length_scale = 1.0, eps_scale = 1.0, vol_frac = 0.0;
     std::unique_ptr<mpm::EwaldOp> op;
-    bool use_poly = false;
+    bool use_poly = false, use_direct = false;
     std::vector<cplx> dips;      // [F][W][N][K][3]
     std::vector<cplx> avg_dips;  // [F][W][K][3]
     std::vector<cplx> quads;     // [F][W][Q][K][5]
@@ -128,8 +109,7 @@ struct mpm_dipole_solver {
             throw mpm::Error(MPM_ERR_RUNTIME, "field_type was set to 'monodisperse' but radii are not identical.");
         // dipole_solver.cu:527-536: auto selects the polydisperse operator iff radii differ
         use_poly = field_type == MPM_FIELD_POLYDISPERSE || (field_type == MPM_FIELD_AUTO && !mono);
-        if (field_type == MPM_FIELD_DIRECT)
-            throw mpm::Error(MPM_ERR_UNSUPPORTED, "the direct (open-boundary) field is not provided by this library");
+        use_direct = field_type == MPM_FIELD_DIRECT;
         if (eps_p.empty()) throw mpm::Error(MPM_ERR_RUNTIME, "eps_p is empty.");
         W = eps_p.size();
         for (auto& row : eps_p) {
@@ -283,7 +263,9 @@ struct mpm_dipole_solver {
                 od.xi = xi;
                 od.support = support;
                 std::unique_ptr<mpm::EwaldOp> o;
-                if (use_poly)
+                if (use_direct)
+                    o = std::make_unique<mpm::EwaldOp>(mpm::EwaldOp::DirectTag{}, radius.data(), (int64_t)N);
+                else if (use_poly)
                     o = std::make_unique<mpm::EwaldOp>(od, radius.data(), (int64_t)N);
                 else
                     o = std::make_unique<mpm::EwaldOp>(od);
@@ -299,7 +281,7 @@ struct mpm_dipole_solver {
                 N = W = K = Q = 0;
                 length_scale = eps_scale = 1.0;
                 vol_frac = 0.0;
-                use_poly = false;
+                use_poly = use_direct = false;
                 op.reset();
                 throw;
             }
@@ -437,6 +419,8 @@ struct mpm_near_field {
     std::vector<cplx> dip;                      // [N][3]
     std::vector<double> dpos[3], fpos[3];       // dipole and field-point coordinates
     double length_scale = 1.0;
+    bool direct = false;
+    size_t op_sites = 0;  // direct field: particles + field points the operator's radii cover
     std::unique_ptr<mpm::EwaldOp> op;
     mpm::DevBuf<double2> d_x, d_y;
 
@@ -458,19 +442,29 @@ struct mpm_near_field {
                                  "The number of particles is inconsistent with the number of radii provided.");
             bool mono = true;
             for (size_t i = 1; i < N; ++i) mono = mono && radius[i] == radius[0];
-            if (field_type == MPM_FIELD_DIRECT)
-                throw mpm::Error(MPM_ERR_UNSUPPORTED, "the direct (open-boundary) field is not provided by this library");
-            if (field_type == MPM_FIELD_POLYDISPERSE || (field_type == MPM_FIELD_AUTO && !mono))
+            direct = field_type == MPM_FIELD_DIRECT;
+            if (!direct && (field_type == MPM_FIELD_POLYDISPERSE || (field_type == MPM_FIELD_AUTO && !mono)))
                 throw mpm::Error(MPM_ERR_UNSUPPORTED,
                                  "the near field of the polydisperse operator is not provided by this library");
             length_scale = radius[0];  // near_field.cu:120 (the first radius)
-            mpm_op_desc od;
-            std::memset(&od, 0, sizeof(od));
-            for (int d = 0; d < 3; ++d) od.box[d] = box[d] / length_scale;
-            od.errortol = errortol;
-            od.xi = xi;
-            op = std::make_unique<mpm::EwaldOp>(od);
+            if (direct) {
+                // the particles' radii then unit radii for the field points (sites without dipoles:
+                // their radius enters only their own self term, which multiplies a zero dipole)
+                std::vector<double> rr(N + M, 1.0);
+                for (size_t i = 0; i < N; ++i) rr[i] = radius[i] / length_scale;
+                op = std::make_unique<mpm::EwaldOp>(mpm::EwaldOp::DirectTag{}, rr.data(), (int64_t)rr.size());
+                op_sites = N + M;
+            } else {
+                mpm_op_desc od;
+                std::memset(&od, 0, sizeof(od));
+                for (int d = 0; d < 3; ++d) od.box[d] = box[d] / length_scale;
+                od.errortol = errortol;
+                od.xi = xi;
+                op = std::make_unique<mpm::EwaldOp>(od);
+            }
         }
+        MPM_REQUIRE(!direct || op_sites == N + M, MPM_ERR_RUNTIME,
+                    "Near_Field: the direct field keeps the particle and field-point counts of its first calculation");
         const size_t T = N + M;
         std::vector<double> xyz(3 * T);
         for (size_t i = 0; i < T; ++i)
@@ -602,7 +596,7 @@ const char* mpm_status_string(mpm_status s) {
     return "unknown status";
 }
 
-const char* mpm_last_error(void) { return g_last_error.c_str(); }
+const char* mpm_last_error(void) { return mpm::g_last_error.c_str(); }
 
 mpm_status mpm_op_create(const mpm_op_desc* desc, mpm_op** out) {
     return guarded([&] {
@@ -620,6 +614,15 @@ mpm_status mpm_op_create_poly(const mpm_op_desc* desc, const double* radii, int6
         auto h = std::make_unique<mpm_op>();
         h->impl = std::make_unique<mpm::EwaldOp>(*desc, radii, n_radii);
         *out = h.release();
+    });
+}
+
+mpm_status mpm_op_create_direct(const double* radii, int64_t n_radii, mpm_op** out) {
+    return guarded([&] {
+        MPM_REQUIRE(out && radii && n_radii > 0, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        auto o = std::make_unique<mpm_op>();
+        o->impl = std::make_unique<mpm::EwaldOp>(mpm::EwaldOp::DirectTag{}, radii, n_radii);
+        *out = o.release();
     });
 }
 

@@ -3413,6 +3413,203 @@ __global__ __launch_bounds__(256) void quad_gather_kernel(SlotTable tab, int B, 
 }
 
 // ---------------------------------------------------------------------------
+// Direct (open-boundary) field in the solver's mode, after
+// src/cpp/electric_field/direct_electric_field.cu:37-300 (direct_field_kernel,
+// SOLVER_AX).  A thread per target particle and G right-hand sides; the sources
+// pass through LDS in tiles of kDirectTile (positions, radii, dipoles and
+// quadrupoles of the G systems), so each source's geometry (r, u, 1/r^n) is
+// computed once for all G systems.  For a source i != j: r = x_i - x_j, clamped to
+// the source radius when shorter (a vanishing r keeps no direction), the point
+// dipole field c (3 (p.u) u - p) / r^3 with c = -1/(4 pi); with quadrupoles the
+// point quadrupole field (3c / r^4)(2.5 s u - v), v = Qm u, s = u.v, Qm =
+// -[[Q0, Q1, Q2], [Q1, Q3, Q4], [Q2, Q4, -Q0 - Q3]] in the Ewald operator's
+// components, and at quadrupole targets the field gradients (d_x E_x, d_x E_y,
+// d_x E_z, d_y E_y, d_y E_z): of a dipole (-3c / r^4)((p.u)(I - 5 u u) + p u^T +
+// u p^T), of a quadrupole (-3c / r^5)(5 (v u^T + u v^T) + 2.5 s I - 17.5 s u u -
+// Qm) (the derivative of its field; the reference kernel doubles the off-diagonal
+// source components and differs in sign here).  Self terms (sc + 1/(4 pi a^3)) p
+// and (2.5 sc / a^2 + 3/(4 pi a^5)) Q in FP64; left Jacobi divides by sc.
+// ---------------------------------------------------------------------------
+constexpr int kDirectTile = 128;
+
+template <typename R, int G, bool QUADS>
+__global__ __launch_bounds__(256) void direct_kernel(SlotTable tab, int B, int N, const double* __restrict__ pos,
+                                                     const double* __restrict__ rad, const int* __restrict__ qmap,
+                                                     const double2* __restrict__ selfc, int jacobi) {
+    constexpr int T = kDirectTile;
+    constexpr int QT = QUADS ? T : 1;
+    __shared__ R sx[T], sy[T], sz[T], sa[T];
+    __shared__ R sp[G][3][2][T];
+    __shared__ R sq[QUADS ? G : 1][5][2][QT];
+    __shared__ int sqm[QT];
+    const int tid = threadIdx.x;
+    const int j = blockIdx.x * blockDim.x + tid;
+    const int g0 = blockIdx.y * G;
+    const bool act = j < N;
+    R tx = 0, ty = 0, tz = 0;
+    int qt = -1;
+    if (act) {
+        tx = (R)pos[j];
+        ty = (R)pos[(size_t)N + j];
+        tz = (R)pos[2 * (size_t)N + j];
+        if (QUADS) qt = qmap[j];
+    }
+    R E[G][3][2];
+    R Gq[G][5][2];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) E[g][c][0] = E[g][c][1] = R(0);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) Gq[g][c][0] = Gq[g][c][1] = R(0);
+    }
+    const R cf = R(-1.0 / (4.0 * kPi));
+    const R eps_orig = sizeof(R) == 4 ? R(1e-6) : R(1e-9);
+    const R eps_sing = sizeof(R) == 4 ? R(1e-12) : R(1e-18);
+    for (int t0 = 0; t0 < N; t0 += T) {
+        __syncthreads();
+        if (tid < T) {
+            const int src = t0 + tid;
+            const bool ok = src < N;
+            sx[tid] = ok ? (R)pos[src] : R(0);
+            sy[tid] = ok ? (R)pos[(size_t)N + src] : R(0);
+            sz[tid] = ok ? (R)pos[2 * (size_t)N + src] : R(0);
+            sa[tid] = ok ? (R)rad[src] : R(0);
+            const int qs = (QUADS && ok) ? qmap[src] : -1;
+            if (QUADS) sqm[tid] = qs;
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const bool has = ok && g0 + g < B;
+                const double2* x = has ? tab.x[g0 + g] : nullptr;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double2 v = has ? x[3 * (size_t)src + c] : make_double2(0.0, 0.0);
+                    sp[g][c][0][tid] = (R)v.x;
+                    sp[g][c][1][tid] = (R)v.y;
+                }
+                if (QUADS) {
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) {
+                        const double2 v = (has && qs >= 0) ? x[3 * (size_t)N + 5 * (size_t)qs + c] : make_double2(0.0, 0.0);
+                        sq[g][c][0][tid] = (R)v.x;
+                        sq[g][c][1][tid] = (R)v.y;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (!act) continue;
+        const int cnt = min(T, N - t0);
+        for (int k = 0; k < cnt; ++k) {
+            if (t0 + k == j) continue;
+            R rx = sx[k] - tx, ry = sy[k] - ty, rz = sz[k] - tz;
+            R r2 = rx * rx + ry * ry + rz * rz;
+            const R ai = sa[k];
+            if (r2 < ai * ai) {
+                const R r0 = sqrt(r2);
+                if (r0 > eps_orig) {
+                    const R f = ai / r0;
+                    rx *= f;
+                    ry *= f;
+                    rz *= f;
+                }
+                r2 = ai * ai;
+            }
+            if (r2 < eps_sing) continue;
+            const R d = sqrt(r2);
+            const R id = R(1) / d;
+            const R u[3] = {rx * id, ry * id, rz * id};
+            const R id3 = id * id * id;
+            const R fd = cf * id3;
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const R p[3] = {sp[g][0][h][k], sp[g][1][h][k], sp[g][2][h][k]};
+                    const R pu = p[0] * u[0] + p[1] * u[1] + p[2] * u[2];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) E[g][c][h] += fd * (R(3) * pu * u[c] - p[c]);
+                }
+            if (QUADS) {
+                const int qs = sqm[k];
+                const R id4 = id3 * id, id5 = id4 * id;
+                const R rr[5] = {u[0] * u[0], u[0] * u[1], u[0] * u[2], u[1] * u[1], u[1] * u[2]};
+                const R I5[5] = {R(1), R(0), R(0), R(1), R(0)};
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        if (qs >= 0) {
+                            const R q0 = sq[g][0][h][k], q1 = sq[g][1][h][k], q2 = sq[g][2][h][k],
+                                    q3 = sq[g][3][h][k], q4 = sq[g][4][h][k];
+                            // v = Qm u with Qm = -[[q0, q1, q2], [q1, q3, q4], [q2, q4, -q0 - q3]]
+                            const R v[3] = {-(q0 * u[0] + q1 * u[1] + q2 * u[2]),
+                                            -(q1 * u[0] + q3 * u[1] + q4 * u[2]),
+                                            -(q2 * u[0] + q4 * u[1] - (q0 + q3) * u[2])};
+                            const R sv = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+                            const R fq = R(3) * cf * id4;
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) E[g][c][h] += fq * (R(2.5) * sv * u[c] - v[c]);
+                            if (qt >= 0) {
+                                const R vu[5] = {R(2) * v[0] * u[0], v[0] * u[1] + v[1] * u[0], v[0] * u[2] + v[2] * u[0],
+                                                 R(2) * v[1] * u[1], v[1] * u[2] + v[2] * u[1]};
+                                const R Qc[5] = {-q0, -q1, -q2, -q3, -q4};
+                                const R fg = R(-3) * cf * id5;
+#pragma unroll
+                                for (int c = 0; c < 5; ++c)
+                                    Gq[g][c][h] += fg * (R(5) * vu[c] + R(2.5) * sv * I5[c] - R(17.5) * sv * rr[c] - Qc[c]);
+                            }
+                        }
+                        if (qt >= 0) {
+                            const R p[3] = {sp[g][0][h][k], sp[g][1][h][k], sp[g][2][h][k]};
+                            const R pu = p[0] * u[0] + p[1] * u[1] + p[2] * u[2];
+                            const R pv[5] = {R(2) * p[0] * u[0], p[0] * u[1] + p[1] * u[0], p[0] * u[2] + p[2] * u[0],
+                                             R(2) * p[1] * u[1], p[1] * u[2] + p[2] * u[1]};
+                            const R fg = R(-3) * cf * id4;
+#pragma unroll
+                            for (int c = 0; c < 5; ++c) Gq[g][c][h] += fg * (pu * (I5[c] - R(5) * rr[c]) + pv[c]);
+                        }
+                    }
+            }
+        }
+    }
+    if (!act) return;
+    const double aj = rad[j];
+    const double a3 = aj * aj * aj;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        if (g0 + g >= B) break;
+        const double2* x = tab.x[g0 + g];
+        double2* y = tab.y[g0 + g];
+        const double2 sc = selfc[(size_t)tab.row[g0 + g] * N + j];
+        double2 inv = make_double2(1.0, 0.0);
+        if (jacobi) {
+            const double den = sc.x * sc.x + sc.y * sc.y;
+            if (den > 1e-30) inv = make_double2(sc.x / den, -sc.y / den);
+        }
+        const double sr = sc.x + 1.0 / (4.0 * kPi * a3), si = sc.y;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double2 xv = x[3 * (size_t)j + c];
+            const double yr = (double)E[g][c][0] + sr * xv.x - si * xv.y;
+            const double yi = (double)E[g][c][1] + sr * xv.y + si * xv.x;
+            y[3 * (size_t)j + c] = make_double2(yr * inv.x - yi * inv.y, yr * inv.y + yi * inv.x);
+        }
+        if (QUADS && qt >= 0) {
+            const double qr = 2.5 * sc.x / (aj * aj) + 3.0 / (4.0 * kPi * a3 * aj * aj), qi = 2.5 * sc.y / (aj * aj);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                const size_t e = 3 * (size_t)N + 5 * (size_t)qt + c;
+                const double2 xv = x[e];
+                const double yr = (double)Gq[g][c][0] + qr * xv.x - qi * xv.y;
+                const double yi = (double)Gq[g][c][1] + qr * xv.y + qi * xv.x;
+                y[e] = make_double2(yr * inv.x - yi * inv.y, yr * inv.y + yi * inv.x);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // EwaldOp
 // ---------------------------------------------------------------------------
 EwaldOp::EwaldOp(const mpm_op_desc& d) {
@@ -3430,6 +3627,18 @@ EwaldOp::EwaldOp(const mpm_op_desc& d) {
     V = (size_t)S.n[0] * S.n[1] * S.n[2];
     table = std::make_unique<PairTable>(xi);
     build_kspace();
+}
+
+EwaldOp::EwaldOp(DirectTag, const double* radii, int64_t nr) {
+    MPM_REQUIRE(radii && nr > 0, MPM_ERR_INVALID_ARGUMENT, "no radii");
+    for (int64_t i = 0; i < nr; ++i)
+        MPM_REQUIRE(std::isfinite(radii[i]) && radii[i] > 0.0, MPM_ERR_INVALID_ARGUMENT, "radii must be positive");
+    direct = true;
+    rad_h.assign(radii, radii + nr);
+    L[0] = L[1] = L[2] = 0.0;
+    errortol = 0.0;
+    xi = 0.0;
+    V = 0;
 }
 
 EwaldOp::EwaldOp(const mpm_op_desc& d, const double* radii, int64_t nr) : EwaldOp(d) {
@@ -3532,6 +3741,7 @@ void EwaldOp::poly_pair_value(double r, int i, int j, double& perp, double& para
 // Pair table `which` (0 perp, 1 para, 2..8 the quadrupole tables) on its samples r; the
 // table's length is returned and the arrays written when cap allows.
 int64_t EwaldOp::pair_table(int which, double* r, double* v, int64_t cap) const {
+    MPM_REQUIRE(!direct, MPM_ERR_UNSUPPORTED, "the direct field has no real-space tables");
     MPM_REQUIRE(!poly, MPM_ERR_UNSUPPORTED, "the polydisperse operator keeps one table per radius pair");
     MPM_REQUIRE(which >= 0 && which <= 8, MPM_ERR_INVALID_ARGUMENT, "table index out of range");
     const std::vector<double>& t = which == 0 ? table->perp : which == 1 ? table->para : table->q[which - 2];
@@ -3548,6 +3758,7 @@ int64_t EwaldOp::pair_table(int which, double* r, double* v, int64_t cap) const 
 // distance below rc) and each pair's (perp, para) values; the pair count is returned
 // and the arrays written when cap allows.
 int64_t EwaldOp::neighbors(int64_t* start, int* j, double* perp, double* para, int64_t cap) const {
+    MPM_REQUIRE(!direct, MPM_ERR_UNSUPPORTED, "the direct field has no neighbour list");
     MPM_REQUIRE(N > 0, MPM_ERR_RUNTIME, "particle coordinates have not been set");
     if (cap >= num_pairs) {
         if (start)
@@ -3639,6 +3850,7 @@ void EwaldOp::set_quads(const int* idx, int64_t nq) {
     quads = true;
     quads_all = nq < 0;
     qreq_h.assign(idx && nq > 0 ? idx : nullptr, idx && nq > 0 ? idx + nq : nullptr);
+    if (direct) return;
     force_full = 1;  // the slab kernels carry dipoles only
     table->build_quads(xi);
     build_kspace();
@@ -3694,7 +3906,10 @@ void EwaldOp::setup_quad_stencils(const std::vector<double>& pos, int Nn) {
     d_pidx.upload(pidx.data(), pidx.size());
     d_vox_start.upload(cnt.data(), cnt.size());
     d_vox_items.upload(items.data(), items.size());
-    // quadrupole slots
+    build_quad_slots(Nn);
+}
+
+void EwaldOp::build_quad_slots(int Nn) {
     qidx_h.clear();
     if (quads_all)
         for (int i = 0; i < Nn; ++i) qidx_h.push_back(i);
@@ -3793,6 +4008,7 @@ void EwaldOp::conj_self_coef(hipStream_t s) {
 
 void EwaldOp::set_bloch(double kx, double ky) {
     if (kx == kb[0] && ky == kb[1]) return;
+    MPM_REQUIRE(!direct, MPM_ERR_UNSUPPORTED, "the direct field has no Bloch wave vector");
     kb[0] = kx;
     kb[1] = ky;
     build_kspace();
@@ -3848,6 +4064,21 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     const int Nn = (int)n;
     for (int64_t q = 0; q < 3 * n; ++q)
         MPM_REQUIRE(std::isfinite(xyz[q]), MPM_ERR_INVALID_ARGUMENT, "non-finite position");
+    if (direct) {
+        MPM_REQUIRE(rad_h.size() == 1 || (int64_t)rad_h.size() == n, MPM_ERR_INVALID_ARGUMENT,
+                    "the direct field needs one radius per particle (or one for all)");
+        std::vector<double> pos(3 * (size_t)Nn), rr(Nn);
+        for (int i = 0; i < Nn; ++i) {
+            for (int d = 0; d < 3; ++d) pos[(size_t)d * Nn + i] = xyz[3 * (size_t)i + d];
+            rr[i] = rad_h.size() == 1 ? rad_h[0] : rad_h[i];
+        }
+        N = 0;
+        d_pos.upload(pos.data(), pos.size());
+        d_drad.upload(rr.data(), rr.size());
+        if (quads) build_quad_slots(Nn);
+        N = n;
+        return;
+    }
     // unchanged positions: the per-frame precalcs are still valid
     if (n == N && last_xyz.size() == (size_t)(3 * n) &&
         std::memcmp(last_xyz.data(), xyz, 3 * n * sizeof(double)) == 0)
@@ -4589,6 +4820,27 @@ void EwaldOp::launch_poly(const SlotTable& t, int B, int parts, hipStream_t s, i
     MPM_HIP(hipGetLastError());
 }
 
+template <typename R>
+void EwaldOp::launch_direct(const SlotTable& t, int B, int parts, hipStream_t s, int jacobi) {
+    MPM_REQUIRE(parts & MPM_PART_REAL, MPM_ERR_INVALID_ARGUMENT,
+                "the direct field has no reciprocal part (parts must include MPM_PART_REAL)");
+    ProfScope ps("direct", s, B);
+    const int Ni = (int)N;
+    const dim3 blk(256);
+    if (quads) {
+        constexpr int G = 2;
+        ps.kernel("direct_kernel", {type_name<R>(), "2", "true"});
+        hipLaunchKernelGGL((direct_kernel<R, G, true>), dim3((unsigned)((Ni + 255) / 256), (unsigned)((B + G - 1) / G)),
+                           blk, 0, s, t, B, Ni, d_pos.p, d_drad.p, d_qmap.p, d_selfc.p, jacobi);
+    } else {
+        constexpr int G = 4;
+        ps.kernel("direct_kernel", {type_name<R>(), "4", "false"});
+        hipLaunchKernelGGL((direct_kernel<R, G, false>), dim3((unsigned)((Ni + 255) / 256), (unsigned)((B + G - 1) / G)),
+                           blk, 0, s, t, B, Ni, d_pos.p, d_drad.p, nullptr, d_selfc.p, jacobi);
+    }
+    MPM_HIP(hipGetLastError());
+}
+
 __global__ void zero_words_kernel(uint4* __restrict__ p, size_t n) {
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
         p[e] = make_uint4(0u, 0u, 0u, 0u);
@@ -4612,7 +4864,9 @@ void EwaldOp::apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t
                     "self-coefficient row out of range");
     const size_t elems = (size_t)B * grid_elems_per_rhs();
     const size_t kelems = (size_t)B * kgrid_elems_per_rhs();
-    if (ws) {
+    if (direct) {
+        // no grids
+    } else if (ws) {
         // caller's workspace: nothing may allocate, so the k-space tables must already be
         // current (set_positions and set_bloch build them) and the FFT must be the built-in one
         MPM_REQUIRE(!(parts & MPM_PART_RECIP) || (kspace_ready && builtin_fft()), MPM_ERR_UNSUPPORTED,
@@ -4647,6 +4901,11 @@ void EwaldOp::apply(const SlotTable& t, int B, bool fp32, int parts, hipStream_t
         }
     }
     const int jac = jacobi ? 1 : 0;
+    if (direct) {
+        if (fp32) launch_direct<float>(t, B, parts, s, jac);
+        else launch_direct<double>(t, B, parts, s, jac);
+        return;
+    }
     if (quads) {
         if (fp32) launch_quads<float>(t, B, parts, s, jac);
         else launch_quads<double>(t, B, parts, s, jac);

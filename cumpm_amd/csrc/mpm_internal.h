@@ -42,6 +42,28 @@ struct Error : std::runtime_error {
         if (!(cond)) throw ::mpm::Error(status, msg); \
     } while (0)
 
+// Message of the last failing C-ABI call on this thread, and the wrapper every entry
+// point runs its body in: exceptions become an mpm_status and a message.
+inline thread_local std::string g_last_error;
+
+template <typename F>
+mpm_status guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return MPM_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return MPM_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return MPM_ERR_RUNTIME;
+    }
+}
+
 // Device bytes held by the library's own allocations (operators, solver and driver
 // buffers), and their high-water mark since the last reset.
 inline std::atomic<int64_t> g_dev_bytes{0};
@@ -146,7 +168,12 @@ class EwaldOp {
     // polydisperse operator: one radius per particle (nondimensional), merged
     // within 1e-5 into unique radii; positions must then come with the same N
     EwaldOp(const mpm_op_desc& d, const double* radii, int64_t n_radii);
+    // direct (open-boundary) field: no grid, no cutoff; radii one per particle or one for
+    // all (nondimensional)
+    struct DirectTag {};
+    EwaldOp(DirectTag, const double* radii, int64_t n_radii);
     ~EwaldOp();
+    bool is_direct() const { return direct; }
     void set_positions(const double* xyz, int64_t N);
     void set_bloch(double kx, double ky);
     void set_self_coef(const double* coef, int rows);
@@ -287,6 +314,13 @@ class EwaldOp {
     void setup_quad_stencils(const std::vector<double>& pos, int Nn);
     template <typename R>
     void launch_quads(const SlotTable& t, int B, int parts, hipStream_t s, int jacobi);
+
+    // ---- direct field ----
+    bool direct = false;
+    DevBuf<double> d_drad;           // [N] radius of each particle
+    template <typename R>
+    void launch_direct(const SlotTable& t, int B, int parts, hipStream_t s, int jacobi);
+    void build_quad_slots(int Nn);   // qidx_h, d_qidx, d_qmap from the requested indices
 
     // ---- polydisperse operator ----
     bool poly = false;

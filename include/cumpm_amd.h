@@ -106,6 +106,16 @@ mpm_status mpm_op_create(const mpm_op_desc* desc, mpm_op** out);
  * pair's closed forms are evaluated at its own radii (the reference's tables would need
  * U(U+1)/2 columns). */
 mpm_status mpm_op_create_poly(const mpm_op_desc* desc, const double* radii, int64_t n_radii, mpm_op** out);
+/* Direct_Electric_Field(radius, SOLVER_AX, solve_quadrupoles, quad_idxs) —
+ * src/cpp/electric_field/direct_electric_field.cu:37-300, 418-427: the open-boundary point
+ * multipole field, O(N^2) per apply, no grid and no cutoff.  radii: host, one per particle
+ * or one for all (nondimensional); pairs closer than the source's radius are evaluated at
+ * that radius.  Quadrupoles (mpm_op_set_quads) couple in the Ewald operator's conventions:
+ * the reference kernel doubles the off-diagonal components of a quadrupole source and its
+ * quadrupole-to-gradient term is not the derivative of the quadrupole's field; here both are
+ * the large-box limit of the Ewald operator.  MPM_PREC_MIXED evaluates the pair sums in FP32
+ * (as the reference's default), self terms in FP64.  mpm_op_set_bloch: unsupported. */
+mpm_status mpm_op_create_direct(const double* radii, int64_t n_radii, mpm_op** out);
 void mpm_op_destroy(mpm_op* op);
 mpm_status mpm_op_get_info(const mpm_op* op, mpm_op_info* info);
 /* getRealSpaceTablesHost (ewald_electric_field_base.cu:101-110), monodisperse operator: the
@@ -269,6 +279,72 @@ mpm_status mpm_dipole_solver_get_timing(const mpm_dipole_solver* s, double* solv
  * plans and per-frame precalcs; the next compute() starts from the mean-field guess as a
  * first frame does.  No counterpart in the reference (its results only accumulate). */
 mpm_status mpm_dipole_solver_reset(mpm_dipole_solver* s);
+
+/* ---- EELS driver (src/cpp/eels_solver.cu, eels_bindings.cpp:61-194, src/cuMPM/eels.py) ---- */
+typedef struct mpm_eels_solver mpm_eels_solver;
+
+typedef struct {
+    double box[3];            /* physical */
+    const double* eps_p;      /* host complex [W][eps_cols], eps_cols = 1 or N (copied) */
+    int64_t num_wavevectors;  /* W */
+    int64_t eps_cols;
+    const double* omega;      /* host [num_omega >= W] wavenumbers in inverse length units (copied) */
+    int64_t num_omega;
+    double v;                 /* beam speed as a fraction of c, 0 < v < 1 */
+    const double* radius;     /* host [num_radii] (copied); num_radii 0 => 1.0 */
+    int64_t num_radii;
+    double eps_m;
+    double xi;
+    double tol;               /* Krylov tolerance and Ewald accuracy */
+    int quiet;
+    int guess;                /* mpm_guess */
+    int method;               /* mpm_method */
+    int field_type;           /* mpm_field_type; DIRECT = the open-boundary field */
+    double integration_step;  /* kept for the reference's signature (scaled, unused) */
+    int solve_quadrupoles;
+    const int* quad_idxs;     /* host [num_quad_idxs] (copied); 0 with solve_quadrupoles => all */
+    int64_t num_quad_idxs;
+    int asm_flag;             /* aperiodic sampling: average over asm_nx x asm_ny Bloch points */
+    int asm_nx, asm_ny;
+    int precision;            /* mpm_precision; the reference defaults to AUTO = FP32 reciprocal */
+    int refined;              /* 1 => MPM_SOLVE_REFINED instead of MPM_SOLVE_MIXED (ignored for DOUBLE) */
+    int batched;              /* 0 = reference order (one frequency at a time with its guesses);
+                                 1 = every frequency of a Bloch point in one batched solve */
+    int restart;              /* 0 => 50 */
+    int maxiter;              /* 0 => 1000 */
+} mpm_eels_desc;
+
+/* EELS_Solver(box, eps_p, omega, v, radius, eps_m, xi, tol, quiet, guess_type, solver_type,
+ * field_type, integration_step, solve_quadrupoles, quad_idxs, asm_flag, asm_Nx, asm_Ny,
+ * precision) — eels_solver.cu:118-233.  Fails with MPM_ERR_INVALID_ARGUMENT for v outside
+ * (0, 1) (the reference then evaluates Bessel functions of imaginary argument). */
+mpm_status mpm_eels_solver_create(const mpm_eels_desc* desc, mpm_eels_solver** out);
+void mpm_eels_solver_destroy(mpm_eels_solver* s);
+/* EELS_Solver::compute (eels_solver.cu:438-880): one beam position epos (x, y), physical units,
+ * and the particle coordinates; appends one frame (the loss spectrum, dipoles and quadrupoles).
+ * The incident field is the relativistic field of the beam (Bessel K0/K1 on the device), each
+ * quadrupole's incident gradient its 24-point surface average; the loss is the reciprocity sum
+ * sum Im(p . conj E) (+ quadrupole terms) / (2 pi^2). */
+mpm_status mpm_eels_solver_compute(mpm_eels_solver* s, const double epos[2], const double* x, const double* y,
+                                   const double* z, int64_t n_particles);
+/* dims = {frames, wavevectors, particles, quadrupoles} */
+mpm_status mpm_eels_solver_get_dims(const mpm_eels_solver* s, int64_t dims[4]);
+/* get_eels: host [F][W] (eels_bindings.cpp:9-19). */
+mpm_status mpm_eels_solver_get_eels(const mpm_eels_solver* s, double* out);
+/* get_dipoles: host complex [F][W][N][3]; physical scales by eps_m a_min^3 (eels_solver.h:155-163). */
+mpm_status mpm_eels_solver_get_dipoles(const mpm_eels_solver* s, int physical, double* out);
+/* get_quadrupoles: host complex [F][W][Q][5]; physical scales by eps_m a_min^4. */
+mpm_status mpm_eels_solver_get_quadrupoles(const mpm_eels_solver* s, int physical, double* out);
+/* Incident field of the last frame's last Bloch point, host complex [W][3N + 5Q] (nondimensional;
+ * no counterpart in the reference). */
+mpm_status mpm_eels_solver_get_incident_field(const mpm_eels_solver* s, double* out);
+/* Per-solve reports, [F][Bloch points][W]: *count entries, written when cap >= *count. */
+mpm_status mpm_eels_solver_get_reports(const mpm_eels_solver* s, mpm_solve_report* out, int64_t cap,
+                                       int64_t* count);
+mpm_status mpm_eels_solver_set_use_jacobi(mpm_eels_solver* s, int enable);
+mpm_status mpm_eels_solver_get_use_jacobi(const mpm_eels_solver* s, int* enable);
+mpm_status mpm_eels_solver_get_timing(const mpm_eels_solver* s, double* solve_ms, double* setup_ms,
+                                      double* total_ms);
 
 /* ---- Near field (src/cpp/near_field.h:12-57, near_field_bindings.cpp:8-25) ---- */
 typedef struct mpm_near_field mpm_near_field;

@@ -450,3 +450,289 @@ class PolyOperator(Operator):
         if pos.shape[0] != self.radii.size:
             raise ValueError("one radius per particle")
         return super().set_positions(pos)
+
+
+class DirectOperator:
+    """A p for the direct (open-boundary) field in the solver's mode, dense numpy restatement
+    of src/cpp/electric_field/direct_electric_field.cu:37-300 (direct_field_kernel, mode
+    SOLVER_AX): for every target j and source i != j, r = x_i - x_j clamped to the source
+    radius when shorter (a zero-length r keeps its direction, i.e. none), the point dipole
+    field c (3 (p.u) u - p) / r^3 with c = -1/(4 pi), with quadrupoles the point quadrupole
+    field, and for quadrupole targets the dipole and quadrupole field gradients; the self
+    terms (sc + 1/(4 pi a^3)) p and (2.5 sc / a^2 + 3/(4 pi a^5)) Q.
+
+    Quadrupole couplings follow the Ewald operator's conventions, not the direct kernel's:
+    the components (Q0..Q4) = (xx, xy, xz, yy, yz) enter the fields as the traceless matrix
+    Qm = -[[Q0, Q1, Q2], [Q1, Q3, Q4], [Q2, Q4, -Q0 - Q3]] (the kernel doubles the off-diagonal
+    entries, :206-212), and the gradient of a quadrupole's field is the derivative of that
+    field, (-3c / r^5) [5 (v u^T + u v^T) + 2.5 s I - 17.5 s u u^T - Qm] with v = Qm u,
+    s = u.v, in the components (xx, xy, xz, yy, yz) of d_target E (the kernel's :253-272
+    differs in sign and in the Qm term).  With these the direct field is the large-box limit
+    of the Ewald operator for every coupling, which the reference's kernel is not.
+    """
+
+    def __init__(self, radii, quad_idxs=None):
+        self.radii = np.asarray(radii, np.float64).reshape(-1)
+        self.quad_idxs = quad_idxs  # None: no quadrupoles; True: every particle; or indices
+        self.N = 0
+        self.Q = 0
+
+    @property
+    def n(self):
+        return 3 * self.N + 5 * self.Q
+
+    def set_positions(self, pos):
+        self.pos = np.asarray(pos, np.float64).reshape(-1, 3)
+        self.N = self.pos.shape[0]
+        a = np.broadcast_to(self.radii, (self.N,)) if self.radii.size == 1 else self.radii
+        if a.size != self.N:
+            raise ValueError("one radius per particle")
+        self.a = np.asarray(a, np.float64)
+        if self.quad_idxs is None or self.quad_idxs is False:
+            self.owner = np.zeros(0, int)
+        elif self.quad_idxs is True:
+            self.owner = np.arange(self.N)
+        else:
+            self.owner = np.asarray(self.quad_idxs, int)
+        self.Q = self.owner.size
+        self.qmap = -np.ones(self.N, int)
+        self.qmap[self.owner] = np.arange(self.Q)
+
+    @staticmethod
+    def _qmat(q):
+        return -np.array([[q[0], q[1], q[2]],
+                          [q[1], q[3], q[4]],
+                          [q[2], q[4], -q[0] - q[3]]])
+
+    def apply(self, x, self_coef):
+        N, Q = self.N, self.Q
+        x = np.asarray(x, np.complex128).reshape(-1)
+        sc = np.broadcast_to(np.asarray(self_coef, np.complex128), (N,))
+        p = x[:3 * N].reshape(N, 3)
+        qv = x[3 * N:].reshape(Q, 5)
+        E = np.zeros((N, 3), np.complex128)
+        G = np.zeros((Q, 5), np.complex128)
+        c = -1.0 / (4.0 * np.pi)
+        I5 = np.array([1.0, 0, 0, 1.0, 0])
+        for j in range(N):
+            qt = self.qmap[j]
+            for i in range(N):
+                if i == j:
+                    continue
+                r = self.pos[i] - self.pos[j]
+                r2 = r @ r
+                ai = self.a[i]
+                if r2 < ai * ai:
+                    if np.sqrt(r2) > 1e-9:
+                        r = r * np.sqrt(ai * ai / r2)
+                    r2 = ai * ai
+                if r2 < 1e-18:
+                    continue
+                d = np.sqrt(r2)
+                u = r / d
+                pu = p[i] @ u
+                E[j] += c / d ** 3 * (3.0 * pu * u - p[i])
+                if Q == 0:
+                    continue
+                qs = self.qmap[i]
+                Qm = self._qmat(qv[qs]) if qs >= 0 else np.zeros((3, 3), np.complex128)
+                v = Qm @ u
+                s = u @ v
+                E[j] += 3.0 * c / d ** 4 * (2.5 * s * u - v)
+                if qt < 0:
+                    continue
+                rr = np.array([u[0] * u[0], u[0] * u[1], u[0] * u[2], u[1] * u[1], u[1] * u[2]])
+                sym = lambda a_, b_: np.array([2 * a_[0] * b_[0], a_[0] * b_[1] + a_[1] * b_[0],
+                                               a_[0] * b_[2] + a_[2] * b_[0], 2 * a_[1] * b_[1],
+                                               a_[1] * b_[2] + a_[2] * b_[1]])
+                Gd = pu * (-5.0 * rr + I5) + sym(p[i], u)
+                G[qt] += -3.0 * c / d ** 4 * Gd
+                Qc = np.array([Qm[0, 0], Qm[0, 1], Qm[0, 2], Qm[1, 1], Qm[1, 2]])
+                Gq = 5.0 * sym(v, u) + 2.5 * s * I5 - 17.5 * s * rr - Qc
+                G[qt] += -3.0 * c / d ** 5 * Gq
+        a3 = self.a ** 3
+        E += ((sc + 1.0 / (4.0 * np.pi * a3))[:, None]) * p
+        if Q:
+            ao = self.a[self.owner]
+            G += ((2.5 * sc[self.owner] / ao ** 2 + 3.0 / (4.0 * np.pi * ao ** 5))[:, None]) * qv
+        return np.concatenate([E.reshape(-1), G.reshape(-1)])
+
+    def matrix(self, self_coef):
+        n = self.n
+        A = np.zeros((n, n), np.complex128)
+        e = np.zeros(n, np.complex128)
+        for k in range(n):
+            e[:] = 0.0
+            e[k] = 1.0
+            A[:, k] = self.apply(e, self_coef)
+        return A
+
+    def solve(self, b, self_coef, **_):
+        """Exact solution of A x = b (dense LU; sizes where the direct field is used are small)."""
+        return np.linalg.solve(self.matrix(self_coef), np.asarray(b, np.complex128).reshape(-1))
+
+
+# ---------------------------------------------------------------------------
+# EELS (src/cpp/eels_solver.cu:241-880)
+# ---------------------------------------------------------------------------
+def surface_normals24():
+    """The 24 surface points of the quadrupole gradient rule: a Fibonacci sphere with
+    z_k = 1 - (2k + 1)/24 and azimuth -(k + 1/2) pi (3 - sqrt 5) (eels_solver.cu:1-27)."""
+    k = np.arange(24)
+    z = 1.0 - (2.0 * k + 1.0) / 24.0
+    r = np.sqrt(1.0 - z * z)
+    a = -(k + 0.5) * np.pi * (3.0 - np.sqrt(5.0))
+    return np.stack([r * np.cos(a), r * np.sin(a), z], axis=1)
+
+
+def beam_field(points, epos, omega, v, sigma, gamma=None):
+    """Relativistic field of a charge moving along z through (epos[0], epos[1]) at speed v
+    (units of c), frequency omega (wavenumber), at points (M, 3) (eels_solver.cu:610-640):
+    E_perp = -A K1(xi) r/r_s, E_z = A (i/gamma) K0(xi), A = omega/(v^2 gamma) e^{2 pi i omega
+    z / v}, xi = 2 pi omega r_s / (v gamma), r_s = sqrt(r_perp^2 + sigma^2)."""
+    import scipy.special as sp
+    if gamma is None:
+        gamma = 1.0 / np.sqrt(1.0 - v * v)
+    p = np.asarray(points, np.float64).reshape(-1, 3)
+    dx = p[:, 0] - epos[0]
+    dy = p[:, 1] - epos[1]
+    rs = np.sqrt(dx * dx + dy * dy + sigma * sigma)
+    xi = 2.0 * np.pi * omega * rs / (v * gamma)
+    A = omega / (v * v * gamma) * np.exp(1j * 2.0 * np.pi * omega * p[:, 2] / v)
+    E = np.zeros((p.shape[0], 3), np.complex128)
+    E[:, 0] = -A * sp.kv(1, xi) * dx / rs
+    E[:, 1] = -A * sp.kv(1, xi) * dy / rs
+    E[:, 2] = A * 1j / gamma * sp.kv(0, xi)
+    return E
+
+
+def beam_gradient(centres, epos, omega, v, sigma, R=1.0):
+    """Volume-averaged gradient <d_i E_j> of the beam field over spheres of radius R by the
+    24-point surface rule 3/(24 R) sum_k E(x + R n_k) n_k^T, as the five components
+    (Gxx - Gzz, Gxy, Gxz, Gyy - Gzz, Gyz) of its symmetric part (eels_solver.cu:646-706)."""
+    n = surface_normals24()
+    c = np.asarray(centres, np.float64).reshape(-1, 3)
+    out = np.zeros((c.shape[0], 5), np.complex128)
+    for i in range(c.shape[0]):
+        E = beam_field(c[i][None, :] + R * n, epos, omega, v, sigma)  # (24, 3)
+        T = 3.0 / (24.0 * R) * (n.T @ E)  # T[i][j] = sum_k n_i E_j
+        out[i] = [T[0, 0] - T[2, 2], 0.5 * (T[0, 1] + T[1, 0]), 0.5 * (T[0, 2] + T[2, 0]),
+                  T[1, 1] - T[2, 2], 0.5 * (T[1, 2] + T[2, 1])]
+    return out
+
+
+class EELSSolver:
+    """Restatement of EELS_Solver for one frame per beam position (no aperiodic sampling).
+
+    box, positions, radius, epos in physical units; omega wavenumbers; eps_p (W, N) or (W, 1)
+    relative to vacuum (divided by eps_m here).  field: "direct", "monodisperse" or
+    "polydisperse".  Solutions are exact (dense LU) for the direct field and GMRES(50) to tol
+    for the Ewald fields; the loss is the reciprocity sum (eels_solver.cu:844-880)."""
+
+    def __init__(self, box, eps_p, omega, v, radius=1.0, eps_m=1.0, xi=0.5, tol=1e-8, field="direct",
+                 quad_idxs=None, asm=None):
+        self.box = np.asarray(box, np.float64)
+        self.eps_p = np.asarray(eps_p, np.complex128)
+        if self.eps_p.ndim == 1:
+            self.eps_p = self.eps_p[:, None]
+        self.omega = np.asarray(omega, np.float64)
+        self.v = float(v)
+        self.radius = radius
+        self.eps_m = float(eps_m)
+        self.xi = xi
+        self.tol = tol
+        self.field = field
+        self.quad_idxs = quad_idxs  # None: no quadrupoles, True: every particle, or indices
+        self.asm = asm              # None, or (Nx, Ny) Bloch points of the aperiodic sampling
+        self.eels, self.dips, self.quads, self.einc = [], [], [], []
+
+    def compute(self, epos, positions):
+        pos = np.asarray(positions, np.float64).reshape(-1, 3)
+        N = pos.shape[0]
+        rad = np.broadcast_to(np.asarray(self.radius, np.float64), (N,)).astype(np.float64)
+        ls = rad.min()
+        a = rad / ls
+        L = self.box / ls
+        x = pos / ls
+        om = self.omega * ls
+        eps = np.broadcast_to(self.eps_p, (self.eps_p.shape[0], N)) / self.eps_m
+        W = eps.shape[0]
+        e = np.asarray(epos, np.float64) / ls
+        sigma = 0.5 / ls
+        if self.field == "direct":
+            op = DirectOperator(a, self.quad_idxs if self.quad_idxs is not None else None)
+            op.set_positions(x)
+        elif self.field == "polydisperse":
+            op = PolyOperator(L, self.tol, self.xi, a)
+            op.set_positions(x)
+        else:
+            op = Operator(L, self.tol, self.xi)
+            op.set_positions(x)
+            if self.quad_idxs is not None:
+                op.set_quads(None if self.quad_idxs is True else self.quad_idxs)
+        Q = op.Q
+        owner = (np.arange(N) if self.quad_idxs is True else np.asarray(self.quad_idxs, int)) if Q else np.zeros(0, int)
+        n = 3 * N + 5 * Q
+        gamma = 1.0 / np.sqrt(1.0 - self.v ** 2)
+        # aperiodic sampling (eels_solver.cu:521-537, 584-607, 858-872): Bloch points k = -pi/L +
+        # (i + 1/2) dk, the beam's images within 12 v gamma / (2 pi w) summed with the phase
+        # e^{i k.(x - R)}, the loss and dipoles averaged over the points
+        Nx, Ny = self.asm if self.asm else (1, 1)
+        dkx, dky = (2 * np.pi / (L[0] * Nx), 2 * np.pi / (L[1] * Ny)) if self.asm else (0.0, 0.0)
+        fe = np.zeros(W)
+        fd = np.zeros((W, N, 3), np.complex128)
+        fq = np.zeros((W, Q, 5), np.complex128)
+        fi = []
+        for ix in range(Nx):
+            for iy in range(Ny):
+                kx = -np.pi / L[0] + (ix + 0.5) * dkx if self.asm else 0.0
+                ky = -np.pi / L[1] + (iy + 0.5) * dky if self.asm else 0.0
+                if self.asm:
+                    op.set_bloch(kx, ky)
+                fi = []
+                for w in range(W):
+                    E = np.zeros(n, np.complex128)
+                    if self.asm:
+                        rd = 12.0 * abs(self.v * gamma / (2 * np.pi * om[w]))
+                        rx, ry = int(np.ceil(rd / L[0])), int(np.ceil(rd / L[1]))
+                    else:
+                        rx = ry = 0
+                    for jx in range(-rx, rx + 1):
+                        for jy in range(-ry, ry + 1):
+                            R = np.array([jx * L[0], jy * L[1]])
+                            ph = np.exp(1j * (kx * (x[:, 0] - R[0]) + ky * (x[:, 1] - R[1])))
+                            E[:3 * N] += (beam_field(x, e + R, om[w], self.v, sigma) * ph[:, None]).reshape(-1)
+                            if Q:
+                                E[3 * N:] += (beam_gradient(x[owner], e + R, om[w], self.v, sigma)
+                                              * ph[owner][:, None]).reshape(-1)
+                    sc = -3.0 / (4.0 * np.pi * (1.0 - eps[w]))  # without a^-3 (eels_solver.cu:730-733)
+                    if isinstance(op, DirectOperator):
+                        p = op.solve(E, sc)
+                    else:
+                        p, _ = op.solve(E, sc, tol=self.tol)
+                    sm = np.sum((p[:3 * N] * np.conj(E[:3 * N])).imag)
+                    if Q:
+                        wq = np.tile([1.0, 2.0, 2.0, 1.0, 2.0], Q)
+                        sm += np.sum((wq * p[3 * N:] * np.conj(E[3 * N:])).imag) / 6.0
+                    fe[w] += sm / (2.0 * np.pi ** 2)
+                    fd[w] += p[:3 * N].reshape(N, 3)
+                    fq[w] += p[3 * N:].reshape(Q, 5)
+                    fi.append(E)
+        f = 1.0 / (Nx * Ny)
+        fe, fd, fq = fe * f, fd * f, fq * f
+        self.eels.append(np.array(fe))
+        self.dips.append(np.array(fd))
+        self.quads.append(np.array(fq))
+        self.einc.append(np.array(fi))
+        self.length_scale = ls
+
+    def get_eels(self):
+        return np.squeeze(np.array(self.eels))
+
+    def get_dipoles(self, physical=True):
+        f = self.eps_m * self.length_scale ** 3 if physical else 1.0
+        return np.squeeze(np.array(self.dips) * f)
+
+    def get_quadrupoles(self, physical=True):
+        f = self.eps_m * self.length_scale ** 4 if physical else 1.0
+        return np.squeeze(np.array(self.quads) * f)

@@ -43,8 +43,8 @@ def _c_struct_sizes():
 #include <stdio.h>
 #include "cumpm_amd.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu\n", sizeof(mpm_op_desc), sizeof(mpm_op_info), sizeof(mpm_solve_desc),
-         sizeof(mpm_solve_report), sizeof(mpm_dipole_desc));
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(mpm_op_desc), sizeof(mpm_op_info), sizeof(mpm_solve_desc),
+         sizeof(mpm_solve_report), sizeof(mpm_dipole_desc), sizeof(mpm_eels_desc));
   return 0;
 }
 '''
@@ -60,7 +60,7 @@ int main(void) {
 def test_ctypes_structs_match_header():
     sizes = _c_struct_sizes()
     py = [ctypes.sizeof(C.OpDesc), ctypes.sizeof(C.OpInfo), ctypes.sizeof(C.SolveDesc),
-          ctypes.sizeof(C.SolveReport), ctypes.sizeof(C.DipoleDesc)]
+          ctypes.sizeof(C.SolveReport), ctypes.sizeof(C.DipoleDesc), ctypes.sizeof(C.EelsDesc)]
     assert sizes == py
 
 
@@ -94,8 +94,19 @@ def test_argument_validation_like_reference():
         cumpm_amd.Near_Field([10.0] * 3, [1.0, 0, 0], field_type="spectral")
     with pytest.raises(ValueError, match="multiple of 3"):
         cumpm_amd.Near_Field([10.0] * 3, [1.0, 0, 0], dip=[1.0, 0.0])
-    with pytest.raises(NotImplementedError):
-        cumpm_amd.EELS([10.0] * 3, [4.0], [0.1], 0.3)
+    # reference tests/test_eels_solver.py:179-181 and src/cuMPM/eels.py:85-104
+    with pytest.raises(ValueError, match="Both split_dist and N_split must be specified"):
+        cumpm_amd.EELS(box=[50, 50, 50], eps_p=[4.0], omega=[0.1], v=0.3, split_dist=2.0)
+    with pytest.raises(ValueError, match="split_method"):
+        cumpm_amd.EELS([50.0] * 3, [4.0], [0.1], 0.3, split_method="hex")
+    with pytest.raises(ValueError, match="field_type"):
+        cumpm_amd.EELS([50.0] * 3, [4.0], [0.1], 0.3, field_type="spectral")
+    with pytest.raises(ValueError, match="v must lie"):
+        cumpm_amd.EELS([50.0] * 3, [4.0], [0.1], 1.2)
+    e = cumpm_amd.EELS([50.0] * 3, [4.0 + 0.1j], [0.1], 0.3, quiet=True)
+    assert e.use_jacobi_precond is True
+    e.use_jacobi_precond = False
+    assert e.use_jacobi_precond is False
     cumpm_amd.dipole_solver([10.0] * 3, -2.0, quadrupoles=True, quiet=True)
     cumpm_amd.dipole_solver([10.0] * 3, -2.0, quadrupoles=(0, 2), quiet=True)
     s = cumpm_amd.dipole_solver([10.0] * 3, [-2.0 + 0.1j, -1.0 + 0.1j], E0=[[1, 0, 0], [0, 0, 1]], quiet=True)
