@@ -133,6 +133,9 @@ _SIGS = {
     "mpm_dipole_solver_get_use_jacobi": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "mpm_dipole_solver_get_timing": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "mpm_dipole_solver_reset": (ctypes.c_int, [ctypes.c_void_p]),
+    "mpm_ebeam_field": (ctypes.c_int, [_dp, _i64, ctypes.POINTER(ctypes.c_int), _i64, _dp, _i64, ctypes.c_double,
+                                       _dp, ctypes.c_double, _dp]),
+    "mpm_op_set_self_coef_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "mpm_eels_solver_create": (ctypes.c_int, [ctypes.POINTER(EelsDesc), ctypes.POINTER(ctypes.c_void_p)]),
     "mpm_eels_solver_destroy": (None, [ctypes.c_void_p]),
     "mpm_eels_solver_compute": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _i64]),

@@ -679,6 +679,13 @@ mpm_status mpm_op_set_self_coef(mpm_op* op, const double* coef, int rows) {
     });
 }
 
+mpm_status mpm_op_set_self_coef_device(mpm_op* op, const double* coef, int rows, void* stream) {
+    return guarded([&] {
+        MPM_REQUIRE(op && coef, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        op->impl->set_self_coef_device(coef, rows, static_cast<hipStream_t>(stream));
+    });
+}
+
 mpm_status mpm_op_apply(mpm_op* op, const double* x, double* y, int B, int precision, int parts,
                         void* stream) {
     return guarded([&] {

@@ -652,6 +652,53 @@ mpm_status mpm_eels_solver_create(const mpm_eels_desc* d, mpm_eels_solver** out)
 
 void mpm_eels_solver_destroy(mpm_eels_solver* s) { delete s; }
 
+mpm_status mpm_ebeam_field(const double* xyz, int64_t n_particles, const int* quad_particles, int64_t n_quads,
+                           const double* omega, int64_t n_omega, double v, const double epos[2], double sigma,
+                           double* out) {
+    return guarded([&] {
+        MPM_REQUIRE(xyz && omega && epos && out && n_particles > 0 && n_omega > 0, MPM_ERR_INVALID_ARGUMENT,
+                    "null or empty argument");
+        MPM_REQUIRE(n_quads >= 0 && (n_quads == 0 || quad_particles), MPM_ERR_INVALID_ARGUMENT, "null quadrupoles");
+        MPM_REQUIRE(v > 0.0 && v < 1.0, MPM_ERR_INVALID_ARGUMENT, "v must lie in (0, 1)");
+        const size_t N = (size_t)n_particles, Q = (size_t)n_quads, W = (size_t)n_omega, nu = 3 * N + 5 * Q;
+        std::vector<double> soa(3 * N);
+        for (size_t i = 0; i < N; ++i)
+            for (int d = 0; d < 3; ++d) soa[d * N + i] = xyz[3 * i + d];
+        std::vector<int> qown(std::max<size_t>(Q, 1), 0);
+        for (size_t q = 0; q < Q; ++q) {
+            MPM_REQUIRE(quad_particles[q] >= 0 && (size_t)quad_particles[q] < N, MPM_ERR_INVALID_ARGUMENT,
+                        "quadrupole particle out of range");
+            qown[q] = quad_particles[q];
+        }
+        std::vector<int> rmax(2 * W, 0);
+        mpm::DevBuf<double> d_pos, d_om;
+        mpm::DevBuf<int> d_rmax, d_q;
+        mpm::DevBuf<double2> d_out;
+        d_pos.upload(soa.data(), soa.size());
+        d_om.upload(omega, W);
+        d_rmax.upload(rmax.data(), rmax.size());
+        d_q.upload(qown.data(), qown.size());
+        d_out.resize(W * nu);
+        mpm::BeamGeom g;
+        g.ex = epos[0];
+        g.ey = epos[1];
+        g.v = v;
+        g.gamma = 1.0 / std::sqrt(1.0 - v * v);
+        g.sigma = sigma;
+        g.kx = g.ky = 0.0;
+        g.Lx = g.Ly = 0.0;
+        g.N = (int)N;
+        g.Q = (int)Q;
+        g.W = (int)W;
+        g.n = (int64_t)nu;
+        const int64_t threads = (int64_t)(N + Q) * W;
+        hipLaunchKernelGGL(mpm::ebeam_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, nullptr, d_pos.p,
+                           d_om.p, d_rmax.p, d_q.p, g, d_out.p);
+        MPM_HIP(hipGetLastError());
+        MPM_HIP(hipMemcpy(out, d_out.p, W * nu * sizeof(double2), hipMemcpyDeviceToHost));
+    });
+}
+
 mpm_status mpm_eels_solver_compute(mpm_eels_solver* s, const double epos[2], const double* x, const double* y,
                                    const double* z, int64_t n) {
     return guarded([&] {

@@ -4463,6 +4463,14 @@ void EwaldOp::set_self_coef(const double* coef, int rows) {
     self_rows = rows;
 }
 
+void EwaldOp::set_self_coef_device(const double* coef, int rows, hipStream_t s) {
+    MPM_REQUIRE(N > 0, MPM_ERR_RUNTIME, "set positions before the self coefficients");
+    MPM_REQUIRE(rows >= 1, MPM_ERR_INVALID_ARGUMENT, "need at least one row of self coefficients");
+    d_selfc.resize((size_t)rows * N);
+    MPM_HIP(hipMemcpyAsync(d_selfc.p, coef, (size_t)rows * N * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    self_rows = rows;
+}
+
 size_t EwaldOp::grid_elems_per_rhs() const {
     if (poly) return V;
     if (quads) return 8 * V;

@@ -177,6 +177,8 @@ class EwaldOp {
     void set_positions(const double* xyz, int64_t N);
     void set_bloch(double kx, double ky);
     void set_self_coef(const double* coef, int rows);
+    // the same from device memory, enqueued on s
+    void set_self_coef_device(const double* coef, int rows, hipStream_t s);
     // conjugates the device self coefficients in place (the adjoint operator's)
     void conj_self_coef(hipStream_t s);
     // Device memory a caller lends to an apply for its reciprocal grids (base 256-byte

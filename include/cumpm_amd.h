@@ -154,6 +154,9 @@ mpm_status mpm_op_set_bloch(mpm_op* op, double kx, double ky);
 /* setSelfCoef — src/cpp/electric_field/base_electric_field.cu (setSelfCoef), one row of N
  * complex coefficients per right-hand side: coef is host [n_rows][N] complex. */
 mpm_status mpm_op_set_self_coef(mpm_op* op, const double* coef, int n_rows);
+/* The same with coef in device memory ([n_rows][N] complex), copied on `stream` (for callers
+ * that keep their parameters resident, e.g. an autograd function; no counterpart). */
+mpm_status mpm_op_set_self_coef_device(mpm_op* op, const double* coef, int n_rows, void* stream);
 
 /* calculate() — monodisperse_ewald_electric_field.cu:2211-2411 for B right-hand sides at
  * once: y[b] = A x[b] with self-coefficient row b (row 0 if only one row was set).
@@ -341,6 +344,14 @@ mpm_status mpm_eels_solver_get_incident_field(const mpm_eels_solver* s, double* 
 /* Per-solve reports, [F][Bloch points][W]: *count entries, written when cap >= *count. */
 mpm_status mpm_eels_solver_get_reports(const mpm_eels_solver* s, mpm_solve_report* out, int64_t cap,
                                        int64_t* count);
+/* The EELS driver's incident field on its own (no counterpart in the reference as an entry
+ * point; eels_solver.cu:588-706): for nondimensional particle positions xyz [N][3] (host), the
+ * particles quad_particles [Q] that carry quadrupoles, W wavenumbers omega (times the length
+ * unit), beam speed v / c, beam position epos and probe softening sigma (nondimensional),
+ * out: host complex [W][3N + 5Q], field rows then gradient rows as the solver's unknowns. */
+mpm_status mpm_ebeam_field(const double* xyz, int64_t n_particles, const int* quad_particles, int64_t n_quads,
+                           const double* omega, int64_t n_omega, double v, const double epos[2], double sigma,
+                           double* out);
 mpm_status mpm_eels_solver_set_use_jacobi(mpm_eels_solver* s, int enable);
 mpm_status mpm_eels_solver_get_use_jacobi(const mpm_eels_solver* s, int* enable);
 mpm_status mpm_eels_solver_get_timing(const mpm_eels_solver* s, double* solve_ms, double* setup_ms,
