@@ -24,6 +24,7 @@ PREC_DOUBLE, PREC_MIXED, PREC_AUTO = 0, 1, 2
 SOLVE_DOUBLE, SOLVE_MIXED, SOLVE_REFINED = 0, 1, 2
 GMRES, BICGSTAB = 0, 1
 PART_REAL, PART_RECIP, PART_ALL = 1, 2, 3
+STAGE_SPREAD, STAGE_SCALE, STAGE_CONTRACT, STAGE_REAL_SPACE = 1, 2, 3, 4
 GUESS_DERIVATIVE, GUESS_MEAN_FIELD, GUESS_PREVIOUS = 0, 1, 2
 FIELD_AUTO, FIELD_MONODISPERSE, FIELD_POLYDISPERSE, FIELD_DIRECT = 0, 1, 2, 3
 
@@ -103,6 +104,11 @@ _SIGS = {
     "mpm_op_set_self_coef": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
     "mpm_op_apply": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "mpm_op_grid_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]),
+    "mpm_op_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_int, ctypes.c_void_p]),
+    "mpm_op_device_pointer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p),
+                                             ctypes.POINTER(ctypes.c_size_t)]),
     "mpm_op_apply_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "mpm_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolveDesc), ctypes.c_void_p,

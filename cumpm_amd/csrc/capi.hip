@@ -702,6 +702,34 @@ mpm_status mpm_op_apply(mpm_op* op, const double* x, double* y, int B, int preci
     });
 }
 
+mpm_status mpm_op_grid_bytes(const mpm_op* op, int precision, size_t* bytes) {
+    return guarded([&] {
+        MPM_REQUIRE(op && bytes, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        MPM_REQUIRE(op->impl->N > 0, MPM_ERR_RUNTIME, "electricField: particle coordinates have not been set");
+        *bytes = op->impl->stage_grid_bytes(precision != MPM_PREC_DOUBLE);
+    });
+}
+
+mpm_status mpm_op_stage(mpm_op* op, int stage, const double* x, void* grid, double* y, int precision,
+                        void* stream) {
+    return guarded([&] {
+        MPM_REQUIRE(op, MPM_ERR_INVALID_ARGUMENT, "null operator");
+        MPM_REQUIRE(precision >= 0 && precision <= 2, MPM_ERR_INVALID_ARGUMENT, "unknown precision");
+        op->impl->stage(stage, reinterpret_cast<const double2*>(x), grid, reinterpret_cast<double2*>(y),
+                        precision != MPM_PREC_DOUBLE, static_cast<hipStream_t>(stream));
+        MPM_HIP(hipGetLastError());
+    });
+}
+
+mpm_status mpm_op_device_pointer(const mpm_op* op, const char* name, void** ptr, size_t* bytes) {
+    return guarded([&] {
+        MPM_REQUIRE(op && name && ptr, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        size_t b = 0;
+        *ptr = op->impl->device_pointer(name, b);
+        if (bytes) *bytes = b;
+    });
+}
+
 mpm_status mpm_op_apply_noalloc(mpm_op* op, const double* x, double* y, int B, int precision, int parts,
                                 void* stream) {
     return guarded([&] {

@@ -4506,173 +4506,178 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
     const R* regw = fp32 ? reinterpret_cast<const R*>(d_regw32.p) : reinterpret_cast<const R*>(d_regw64.p);
     const int Ni = (int)N;
     if (parts & MPM_PART_RECIP) {
-        if (SLAB && reg_rx > 0) {
-            ProfScope ps("spread", s, B);
-            constexpr int CPB = 32;  // SpreadShape<R, ...>::CPB
-            const dim3 gs((unsigned)(reg_nrx * reg_nry), (unsigned)((6 * B + CPB - 1) / CPB));
-#define MPM_SM(RXV, ZSV)                                                                                          \
-    case ZSV:                                                                                                     \
-        ps.kernel("spread_mfma_kernel", {type_name<R>(), #RXV, #ZSV});                                            \
-        hipLaunchKernelGGL((spread_mfma_kernel<R, RXV, ZSV>), gs, dim3(256), 0, s, t, B, g, regw, d_reg_start.p, \
-                           d_reg_items.p, reg_nry, Ni, grid);                                                    \
-        break;
-            switch (zs_bucket(Zs)) {
-                MPM_SM(4, 4)
-                MPM_SM(4, 6)
-                MPM_SM(4, 8)
-                MPM_SM(4, 10)
-                MPM_SM(2, 12)
-                MPM_SM(2, 14)
-                MPM_SM(2, 16)
-                default:
-                    throw Error(MPM_ERR_RUNTIME, "slab size out of range");
-            }
-#undef MPM_SM
-            MPM_HIP(hipGetLastError());
-        } else {
-            ProfScope ps("spread", s, B);
-            ps.kernel("spread_kernel", {type_name<R>(), B == 1 ? "1" : "4", tbool(SLAB)});
-            const int nt = ntiles[0] * ntiles[1] * n_ztiles;
-            if (B == 1)
-                hipLaunchKernelGGL((spread_kernel<R, 1, SLAB>), dim3(nt, 1), dim3(256), 0, s, t, B, g, d_base.p, w,
-                                   d_bin_start.p, d_bin_items.p, d_ztiles.p, d_zmap.p, Ni, grid);
-            else
-                hipLaunchKernelGGL((spread_kernel<R, 4, SLAB>), dim3(nt, (B + 3) / 4), dim3(256), 0, s, t, B, g,
-                                   d_base.p, w, d_bin_start.p, d_bin_items.p, d_ztiles.p, d_zmap.p, Ni, grid);
-            MPM_HIP(hipGetLastError());
-        }
-        const size_t bytes_per_rhs = grid_elems_per_rhs() * sizeof(CT);
-        CT* kgrid = grid;
-        if (kt) kgrid = reinterpret_cast<CT*>(cur_kgrid);
-        // contiguous slab, transposed k-space rows, a two-stage x transform whose rows
-        // fit in LDS: x transform, plane coupling and inverse x transform in one kernel
-        const int xaxis = (SLAB && zcontig && kt && lfft && fft_backend == 0) ? lfft->kt_x : -1;
-        const size_t xlds = (size_t)(3 * Zs + 2) * S.n[0] * sizeof(CT);
-        const bool fusedx = fp32 && xaxis >= 0 && lfft->axes[xaxis]->two && xlds <= kMaxFusedLds;
-        {
-            ProfScope ps("fft", s, B);
-            if (lfft && fft_backend == 0)
-                lfft->exec<R>(grid, kgrid, B, true, s, fusedx ? xaxis : -1);
-            else
-                fft->exec(grid, bytes_per_rhs, B, fp32, true, s);
-        }
-        if (fusedx) {
-            ProfScope ps("zfused", s, B);
-            const CT* T = reinterpret_cast<const CT*>(d_toe32.p);
-            const LineAxis& ax = *lfft->axes[xaxis];
-            const CT* tw = reinterpret_cast<const CT*>(ax.tw32.p);
-            const bool ri = (S.n[2] & 1) != 0;
-            const dim3 gx((unsigned)(8 * ((S.n[1] + 7) / 8) * B));  // XCD-major row order
-#define MPM_XT(ZSV)                                                                                              \
-    case ZSV:                                                                                                    \
-        ps.kernel("xtoeplitz_kernel", {type_name<R>(), #ZSV, "2", tbool(ri)});                                   \
-        if (ri)                                                                                                  \
-            hipLaunchKernelGGL((xtoeplitz_kernel<R, ZSV, 2, true>), gx, dim3(256), xlds, s, kgrid, Zs, S.n[1],   \
-                               nxp, S.n[0], ax.plan.radix[0], ax.plan.radix[1], tw, T);                         \
-        else                                                                                                     \
-            hipLaunchKernelGGL((xtoeplitz_kernel<R, ZSV, 2, false>), gx, dim3(256), xlds, s, kgrid, Zs, S.n[1],  \
-                               nxp, S.n[0], ax.plan.radix[0], ax.plan.radix[1], tw, T);                         \
-        break;
-            if constexpr (std::is_same<R, float>::value) {
+        if (phases & 1) {
+            if (SLAB && reg_rx > 0) {
+                ProfScope ps("spread", s, B);
+                constexpr int CPB = 32;  // SpreadShape<R, ...>::CPB
+                const dim3 gs((unsigned)(reg_nrx * reg_nry), (unsigned)((6 * B + CPB - 1) / CPB));
+    #define MPM_SM(RXV, ZSV)                                                                                          \
+        case ZSV:                                                                                                     \
+            ps.kernel("spread_mfma_kernel", {type_name<R>(), #RXV, #ZSV});                                            \
+            hipLaunchKernelGGL((spread_mfma_kernel<R, RXV, ZSV>), gs, dim3(256), 0, s, t, B, g, regw, d_reg_start.p, \
+                               d_reg_items.p, reg_nry, Ni, grid);                                                    \
+            break;
                 switch (zs_bucket(Zs)) {
-                    MPM_XT(4)
-                    MPM_XT(6)
-                    MPM_XT(8)
-                    MPM_XT(10)
-                    MPM_XT(12)
-                    MPM_XT(14)
-                    MPM_XT(16)
+                    MPM_SM(4, 4)
+                    MPM_SM(4, 6)
+                    MPM_SM(4, 8)
+                    MPM_SM(4, 10)
+                    MPM_SM(2, 12)
+                    MPM_SM(2, 14)
+                    MPM_SM(2, 16)
                     default:
                         throw Error(MPM_ERR_RUNTIME, "slab size out of range");
                 }
+    #undef MPM_SM
+                MPM_HIP(hipGetLastError());
+            } else {
+                ProfScope ps("spread", s, B);
+                ps.kernel("spread_kernel", {type_name<R>(), B == 1 ? "1" : "4", tbool(SLAB)});
+                const int nt = ntiles[0] * ntiles[1] * n_ztiles;
+                if (B == 1)
+                    hipLaunchKernelGGL((spread_kernel<R, 1, SLAB>), dim3(nt, 1), dim3(256), 0, s, t, B, g, d_base.p, w,
+                                       d_bin_start.p, d_bin_items.p, d_ztiles.p, d_zmap.p, Ni, grid);
+                else
+                    hipLaunchKernelGGL((spread_kernel<R, 4, SLAB>), dim3(nt, (B + 3) / 4), dim3(256), 0, s, t, B, g,
+                                       d_base.p, w, d_bin_start.p, d_bin_items.p, d_ztiles.p, d_zmap.p, Ni, grid);
+                MPM_HIP(hipGetLastError());
             }
-#undef MPM_XT
-            MPM_HIP(hipGetLastError());
-        } else if (SLAB && zcontig) {
-            ProfScope ps("zfused", s, B);
-            const int ncols = kspace_cols();
-            const CT* T = reinterpret_cast<const CT*>(fp32 ? static_cast<const void*>(d_toe32.p)
-                                                           : static_cast<const void*>(d_toe64.p));
-            const int rb = B >= 16 ? 16 : (B >= 4 ? 4 : 1);
-            const int cb = 256 / rb;
-            const dim3 gridz((ncols + cb - 1) / cb, (B + rb - 1) / rb);
-            const bool ri = (S.n[2] & 1) != 0;  // odd nz: real / imaginary table entries
-#define MPM_ZT(ZSV)                                                                                           \
-    case ZSV:                                                                                                 \
-        ps.kernel("ztoeplitz_kernel", {type_name<R>(), #ZSV, std::to_string(ztoe_as<R, ZSV>()), tbool(ri)}); \
-        if (ri)                                                                                               \
-            hipLaunchKernelGGL((ztoeplitz_kernel<R, ZSV, ztoe_as<R, ZSV>(), true>), gridz, dim3(256), 0, s,   \
-                               kgrid, B, ncols, Zs, rb, T);                                                   \
-        else                                                                                                  \
-            hipLaunchKernelGGL((ztoeplitz_kernel<R, ZSV, ztoe_as<R, ZSV>(), false>), gridz, dim3(256), 0, s,  \
-                               kgrid, B, ncols, Zs, rb, T);                                                   \
-        break;
-            switch (zs_bucket(Zs)) {
-                MPM_ZT(4)
-                MPM_ZT(6)
-                MPM_ZT(8)
-                MPM_ZT(10)
-                MPM_ZT(12)
-                MPM_ZT(14)
-                MPM_ZT(16)
-                default:
-                    throw Error(MPM_ERR_RUNTIME, "slab size out of range");
-            }
-#undef MPM_ZT
-            MPM_HIP(hipGetLastError());
-        } else if (SLAB) {
-            ProfScope ps("zfused", s, B);
-            const int ncols = kspace_cols();
-            const int zsb = zs_bucket(Zs);
-            const size_t lds = zfused_lds(zsb, S.n[2], sizeof(R));
-            const R* kc = fp32 ? reinterpret_cast<const R*>(d_k32.p) : reinterpret_cast<const R*>(d_k64.p);
-            const int rb = B >= 16 ? 16 : (B >= 4 ? 4 : 1);
-            const dim3 gridz((ncols + 256 / rb - 1) / (256 / rb), (B + rb - 1) / rb);
-#define MPM_ZF2(ZSV, RBV)                                                                                     \
-    do {                                                                                                      \
-        ps.kernel("zfused_kernel", {type_name<R>(), #ZSV, #RBV});                                             \
-        hipLaunchKernelGGL((zfused_kernel<R, ZSV, RBV>), gridz, dim3(256), lds, s, kgrid, B, ncols, S.n[2],  \
-                           Zs, d_zlist.p, kc);                                                                \
-    } while (0)
-#define MPM_ZF(ZSV)                     \
-    case ZSV:                           \
-        if (rb == 16)                   \
-            MPM_ZF2(ZSV, 16);           \
-        else if (rb == 4)               \
-            MPM_ZF2(ZSV, 4);            \
-        else                            \
-            MPM_ZF2(ZSV, 1);            \
-        break;
-            switch (zsb) {
-                MPM_ZF(4)
-                MPM_ZF(6)
-                MPM_ZF(8)
-                MPM_ZF(10)
-                MPM_ZF(12)
-                MPM_ZF(14)
-                MPM_ZF(16)
-                default:
-                    throw Error(MPM_ERR_RUNTIME, "slab size out of range");
-            }
-#undef MPM_ZF
-#undef MPM_ZF2
-            MPM_HIP(hipGetLastError());
-        } else {
-            ProfScope ps("scale", s, B);
-            ps.kernel("scale_kernel", {type_name<R>()});
-            const int blocks = (int)std::min<size_t>((V + 255) / 256, 8192);
-            const R* kc = fp32 ? reinterpret_cast<const R*>(d_k32.p) : reinterpret_cast<const R*>(d_k64.p);
-            hipLaunchKernelGGL(scale_kernel<R>, dim3(blocks), dim3(256), 0, s, grid, kc, V, B);
-            MPM_HIP(hipGetLastError());
         }
-        {
-            ProfScope ps("fft", s, B);
-            if (lfft && fft_backend == 0)
-                lfft->exec<R>(grid, kgrid, B, false, s, fusedx ? xaxis : -1);
-            else
-                fft->exec(grid, bytes_per_rhs, B, fp32, false, s);
+        if (phases & 2) {
+            const size_t bytes_per_rhs = grid_elems_per_rhs() * sizeof(CT);
+            CT* kgrid = grid;
+            if (kt) kgrid = reinterpret_cast<CT*>(cur_kgrid);
+            // contiguous slab, transposed k-space rows, a two-stage x transform whose rows
+            // fit in LDS: x transform, plane coupling and inverse x transform in one kernel
+            const int xaxis = (SLAB && zcontig && kt && lfft && fft_backend == 0) ? lfft->kt_x : -1;
+            const size_t xlds = (size_t)(3 * Zs + 2) * S.n[0] * sizeof(CT);
+            const bool fusedx = fp32 && xaxis >= 0 && lfft->axes[xaxis]->two && xlds <= kMaxFusedLds;
+            {
+                ProfScope ps("fft", s, B);
+                if (lfft && fft_backend == 0)
+                    lfft->exec<R>(grid, kgrid, B, true, s, fusedx ? xaxis : -1);
+                else
+                    fft->exec(grid, bytes_per_rhs, B, fp32, true, s);
+            }
+            if (fusedx) {
+                ProfScope ps("zfused", s, B);
+                const CT* T = reinterpret_cast<const CT*>(d_toe32.p);
+                const LineAxis& ax = *lfft->axes[xaxis];
+                const CT* tw = reinterpret_cast<const CT*>(ax.tw32.p);
+                const bool ri = (S.n[2] & 1) != 0;
+                const dim3 gx((unsigned)(8 * ((S.n[1] + 7) / 8) * B));  // XCD-major row order
+    #define MPM_XT(ZSV)                                                                                              \
+        case ZSV:                                                                                                    \
+            ps.kernel("xtoeplitz_kernel", {type_name<R>(), #ZSV, "2", tbool(ri)});                                   \
+            if (ri)                                                                                                  \
+                hipLaunchKernelGGL((xtoeplitz_kernel<R, ZSV, 2, true>), gx, dim3(256), xlds, s, kgrid, Zs, S.n[1],   \
+                                   nxp, S.n[0], ax.plan.radix[0], ax.plan.radix[1], tw, T);                         \
+            else                                                                                                     \
+                hipLaunchKernelGGL((xtoeplitz_kernel<R, ZSV, 2, false>), gx, dim3(256), xlds, s, kgrid, Zs, S.n[1],  \
+                                   nxp, S.n[0], ax.plan.radix[0], ax.plan.radix[1], tw, T);                         \
+            break;
+                if constexpr (std::is_same<R, float>::value) {
+                    switch (zs_bucket(Zs)) {
+                        MPM_XT(4)
+                        MPM_XT(6)
+                        MPM_XT(8)
+                        MPM_XT(10)
+                        MPM_XT(12)
+                        MPM_XT(14)
+                        MPM_XT(16)
+                        default:
+                            throw Error(MPM_ERR_RUNTIME, "slab size out of range");
+                    }
+                }
+    #undef MPM_XT
+                MPM_HIP(hipGetLastError());
+            } else if (SLAB && zcontig) {
+                ProfScope ps("zfused", s, B);
+                const int ncols = kspace_cols();
+                const CT* T = reinterpret_cast<const CT*>(fp32 ? static_cast<const void*>(d_toe32.p)
+                                                               : static_cast<const void*>(d_toe64.p));
+                const int rb = B >= 16 ? 16 : (B >= 4 ? 4 : 1);
+                const int cb = 256 / rb;
+                const dim3 gridz((ncols + cb - 1) / cb, (B + rb - 1) / rb);
+                const bool ri = (S.n[2] & 1) != 0;  // odd nz: real / imaginary table entries
+    #define MPM_ZT(ZSV)                                                                                           \
+        case ZSV:                                                                                                 \
+            ps.kernel("ztoeplitz_kernel", {type_name<R>(), #ZSV, std::to_string(ztoe_as<R, ZSV>()), tbool(ri)}); \
+            if (ri)                                                                                               \
+                hipLaunchKernelGGL((ztoeplitz_kernel<R, ZSV, ztoe_as<R, ZSV>(), true>), gridz, dim3(256), 0, s,   \
+                                   kgrid, B, ncols, Zs, rb, T);                                                   \
+            else                                                                                                  \
+                hipLaunchKernelGGL((ztoeplitz_kernel<R, ZSV, ztoe_as<R, ZSV>(), false>), gridz, dim3(256), 0, s,  \
+                                   kgrid, B, ncols, Zs, rb, T);                                                   \
+            break;
+                switch (zs_bucket(Zs)) {
+                    MPM_ZT(4)
+                    MPM_ZT(6)
+                    MPM_ZT(8)
+                    MPM_ZT(10)
+                    MPM_ZT(12)
+                    MPM_ZT(14)
+                    MPM_ZT(16)
+                    default:
+                        throw Error(MPM_ERR_RUNTIME, "slab size out of range");
+                }
+    #undef MPM_ZT
+                MPM_HIP(hipGetLastError());
+            } else if (SLAB) {
+                ProfScope ps("zfused", s, B);
+                const int ncols = kspace_cols();
+                const int zsb = zs_bucket(Zs);
+                const size_t lds = zfused_lds(zsb, S.n[2], sizeof(R));
+                const R* kc = fp32 ? reinterpret_cast<const R*>(d_k32.p) : reinterpret_cast<const R*>(d_k64.p);
+                const int rb = B >= 16 ? 16 : (B >= 4 ? 4 : 1);
+                const dim3 gridz((ncols + 256 / rb - 1) / (256 / rb), (B + rb - 1) / rb);
+    #define MPM_ZF2(ZSV, RBV)                                                                                     \
+        do {                                                                                                      \
+            ps.kernel("zfused_kernel", {type_name<R>(), #ZSV, #RBV});                                             \
+            hipLaunchKernelGGL((zfused_kernel<R, ZSV, RBV>), gridz, dim3(256), lds, s, kgrid, B, ncols, S.n[2],  \
+                               Zs, d_zlist.p, kc);                                                                \
+        } while (0)
+    #define MPM_ZF(ZSV)                     \
+        case ZSV:                           \
+            if (rb == 16)                   \
+                MPM_ZF2(ZSV, 16);           \
+            else if (rb == 4)               \
+                MPM_ZF2(ZSV, 4);            \
+            else                            \
+                MPM_ZF2(ZSV, 1);            \
+            break;
+                switch (zsb) {
+                    MPM_ZF(4)
+                    MPM_ZF(6)
+                    MPM_ZF(8)
+                    MPM_ZF(10)
+                    MPM_ZF(12)
+                    MPM_ZF(14)
+                    MPM_ZF(16)
+                    default:
+                        throw Error(MPM_ERR_RUNTIME, "slab size out of range");
+                }
+    #undef MPM_ZF
+    #undef MPM_ZF2
+                MPM_HIP(hipGetLastError());
+            } else {
+                ProfScope ps("scale", s, B);
+                ps.kernel("scale_kernel", {type_name<R>()});
+                const int blocks = (int)std::min<size_t>((V + 255) / 256, 8192);
+                const R* kc = fp32 ? reinterpret_cast<const R*>(d_k32.p) : reinterpret_cast<const R*>(d_k64.p);
+                hipLaunchKernelGGL(scale_kernel<R>, dim3(blocks), dim3(256), 0, s, grid, kc, V, B);
+                MPM_HIP(hipGetLastError());
+            }
+            {
+                ProfScope ps("fft", s, B);
+                if (lfft && fft_backend == 0)
+                    lfft->exec<R>(grid, kgrid, B, false, s, fusedx ? xaxis : -1);
+                else
+                    fft->exec(grid, bytes_per_rhs, B, fp32, false, s);
+            }
         }
     }
+    if (!(phases & 4)) return;
     const R* part = nullptr;
     if (SLAB && reg_rx > 0 && (parts & MPM_PART_RECIP)) {
         ProfScope ps("contract", s, B);
@@ -4748,14 +4753,14 @@ void EwaldOp::launch_quads(const SlotTable& t, int B, int parts, hipStream_t s, 
     const int SS = S.width * S.width * S.width;
     const int Vi = (int)V, Ni = (int)N;
     constexpr int G = 2;
-    if (parts & MPM_PART_RECIP) {
-        {
-            ProfScope ps("spread", s, B);
-            ps.kernel("quad_spread_kernel", {type_name<R>(), "2"});
-            hipLaunchKernelGGL((quad_spread_kernel<R, G>), dim3((unsigned)((V + 255) / 256), (unsigned)((B + G - 1) / G)),
-                               dim3(256), 0, s, t, B, Ni, Vi, SS, d_vox_start.p, d_vox_items.p, sw, d_qmap.p, grid);
-            MPM_HIP(hipGetLastError());
-        }
+    if ((parts & MPM_PART_RECIP) && (phases & 1)) {
+        ProfScope ps("spread", s, B);
+        ps.kernel("quad_spread_kernel", {type_name<R>(), "2"});
+        hipLaunchKernelGGL((quad_spread_kernel<R, G>), dim3((unsigned)((V + 255) / 256), (unsigned)((B + G - 1) / G)),
+                           dim3(256), 0, s, t, B, Ni, Vi, SS, d_vox_start.p, d_vox_items.p, sw, d_qmap.p, grid);
+        MPM_HIP(hipGetLastError());
+    }
+    if ((parts & MPM_PART_RECIP) && (phases & 2)) {
         {
             ProfScope ps("fft", s, B);
             lfft->exec<R>(grid, grid, B, true, s);
@@ -4774,6 +4779,7 @@ void EwaldOp::launch_quads(const SlotTable& t, int B, int parts, hipStream_t s, 
             lfft->exec<R>(grid, grid, B, false, s);
         }
     }
+    if (!(phases & 4)) return;
     ProfScope ps("gather", s, B);
     ps.kernel("quad_gather_kernel", {type_name<R>(), "2"});
     const double h3 = S.h[0] * S.h[1] * S.h[2];
@@ -4793,14 +4799,14 @@ void EwaldOp::launch_poly(const SlotTable& t, int B, int parts, hipStream_t s, i
     const int Vi = (int)V;
     const int Ni = (int)N;
     constexpr int G = 4;
-    if (parts & MPM_PART_RECIP) {
-        {
-            ProfScope ps("spread", s, B);
-            ps.kernel("poly_spread_kernel", {type_name<R>(), "4"});
-            hipLaunchKernelGGL((poly_spread_kernel<R, G>), dim3((unsigned)((V + 255) / 256), (unsigned)((B + G - 1) / G)),
-                               dim3(256), 0, s, t, B, Vi, SS, d_vox_start.p, d_vox_items.p, pw, grid);
-            MPM_HIP(hipGetLastError());
-        }
+    if ((parts & MPM_PART_RECIP) && (phases & 1)) {
+        ProfScope ps("spread", s, B);
+        ps.kernel("poly_spread_kernel", {type_name<R>(), "4"});
+        hipLaunchKernelGGL((poly_spread_kernel<R, G>), dim3((unsigned)((V + 255) / 256), (unsigned)((B + G - 1) / G)),
+                           dim3(256), 0, s, t, B, Vi, SS, d_vox_start.p, d_vox_items.p, pw, grid);
+        MPM_HIP(hipGetLastError());
+    }
+    if ((parts & MPM_PART_RECIP) && (phases & 2)) {
         {
             ProfScope ps("fft", s, B);
             lfft->exec<R>(grid, grid, B, true, s);
@@ -4819,6 +4825,7 @@ void EwaldOp::launch_poly(const SlotTable& t, int B, int parts, hipStream_t s, i
             lfft->exec<R>(grid, grid, B, false, s);
         }
     }
+    if (!(phases & 4)) return;
     ProfScope ps("gather", s, B);
     ps.kernel("poly_gather_kernel", {type_name<R>(), "4"});
     const double h3 = S.h[0] * S.h[1] * S.h[2];
@@ -4946,6 +4953,123 @@ void EwaldOp::apply_contiguous(const double2* x, double2* y, int B, bool fp32, i
         }
         apply(t, cnt, fp32, parts, s, false, ws);
     }
+}
+
+__global__ void add_kernel(double2* __restrict__ y, const double2* __restrict__ t, size_t n) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        y[e].x += t[e].x;
+        y[e].y += t[e].y;
+    }
+}
+
+size_t EwaldOp::stage_grid_bytes(bool fp32) const {
+    return grid_elems_per_rhs() * (fp32 ? sizeof(float2) : sizeof(double2));
+}
+
+void EwaldOp::stage(int which, const double2* x, void* grid, double2* y, bool fp32, hipStream_t s) {
+    MPM_REQUIRE(N > 0, MPM_ERR_RUNTIME, "electricField: particle coordinates have not been set");
+    MPM_REQUIRE(which >= 1 && which <= 4, MPM_ERR_INVALID_ARGUMENT, "unknown stage");
+    MPM_REQUIRE(self_rows >= 1, MPM_ERR_RUNTIME, "set the self coefficients first");
+    const bool recip = which != 4;
+    MPM_REQUIRE(!(recip && direct), MPM_ERR_UNSUPPORTED, "the direct field has no reciprocal-space stages");
+    MPM_REQUIRE(!recip || grid, MPM_ERR_INVALID_ARGUMENT, "stage needs a grid buffer");
+    MPM_REQUIRE(which == 2 || (x != nullptr && (which == 1 || y != nullptr)), MPM_ERR_INVALID_ARGUMENT,
+                "stage needs the dipoles (and, to add into, the field)");
+    if (recip) ensure_kspace_table();
+    // spread writes the grid, scale transforms it in place; contract and real space add into y
+    SlotTable t;
+    t.x[0] = x;
+    t.row[0] = 0;
+    const size_t n = n_unknowns();
+    DevBuf<double2> tmp;
+    if (which >= 3) {
+        tmp.resize(n);
+        t.y[0] = tmp.p;
+    } else {
+        t.y[0] = nullptr;
+    }
+    cur_grid = grid;
+    cur_kgrid = nullptr;
+    if (kt && which == 2) {
+        const size_t ke = kgrid_elems_per_rhs();
+        if (fp32) {
+            d_kgrid32.reserve(ke);
+            cur_kgrid = d_kgrid32.p;
+        } else {
+            d_kgrid64.reserve(ke);
+            cur_kgrid = d_kgrid64.p;
+        }
+    }
+    struct Phases {
+        int& p;
+        Phases(int& q, int v) : p(q) { p = v; }
+        ~Phases() { p = 7; }
+    } ph(phases, which == 1 ? 1 : which == 2 ? 2 : 4);
+    const int parts = which == 4 ? MPM_PART_REAL : MPM_PART_RECIP;
+    if (direct) {
+        if (fp32) launch_direct<float>(t, 1, parts, s, 0);
+        else launch_direct<double>(t, 1, parts, s, 0);
+    } else if (quads) {
+        if (fp32) launch_quads<float>(t, 1, parts, s, 0);
+        else launch_quads<double>(t, 1, parts, s, 0);
+    } else if (poly) {
+        if (fp32) launch_poly<float>(t, 1, parts, s, 0);
+        else launch_poly<double>(t, 1, parts, s, 0);
+    } else if (fp32) {
+        if (slab) launch<float, true>(t, 1, parts, s, 0);
+        else launch<float, false>(t, 1, parts, s, 0);
+    } else {
+        if (slab) launch<double, true>(t, 1, parts, s, 0);
+        else launch<double, false>(t, 1, parts, s, 0);
+    }
+    if (which >= 3) {
+        hipLaunchKernelGGL(add_kernel, dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)), dim3(256), 0, s, y,
+                           tmp.p, n);
+        MPM_HIP(hipGetLastError());
+        MPM_HIP(hipStreamSynchronize(s));  // tmp is released on return
+    }
+}
+
+void* EwaldOp::device_pointer(const std::string& name, size_t& bytes) const {
+    auto give = [&](const void* p, size_t b) {
+        bytes = p ? b : 0;
+        return const_cast<void*>(p);
+    };
+    const size_t Nn = (size_t)N;
+    if (name == "x_part") return give(d_pos.p, Nn * sizeof(double));
+    if (name == "y_part") return give(d_pos.p ? d_pos.p + Nn : nullptr, Nn * sizeof(double));
+    if (name == "z_part") return give(d_pos.p ? d_pos.p + 2 * Nn : nullptr, Nn * sizeof(double));
+    if (name == "self_coef") return give(d_selfc.p, d_selfc.n * sizeof(double2));
+    if (name == "radius") return give(d_drad.p, d_drad.n * sizeof(double));
+    if (name == "spread_coef" || name == "contract_coef") {
+        if (poly) return give(d_pw64.p, d_pw64.n * sizeof(double));
+        if (quads) return give(d_sw64.p, d_sw64.n * sizeof(double));
+        return give(d_w64.p, d_w64.n * sizeof(double));
+    }
+    if (name == "spread_coef_fp32" || name == "contract_coef_fp32") {
+        if (poly) return give(d_pw32.p, d_pw32.n * sizeof(float));
+        if (quads) return give(d_sw32.p, d_sw32.n * sizeof(float));
+        return give(d_w32.p, d_w32.n * sizeof(float));
+    }
+    if (name == "spread_idxs" || name == "contract_idxs") {
+        if (poly || quads) return give(d_pidx.p, d_pidx.n * sizeof(int));
+        return give(d_base.p, d_base.n * sizeof(int));
+    }
+    if (name == "particle_index") return give(d_order.p, d_order.n * sizeof(int));
+    if (name == "neighbor_start") return give(d_nb_start.p, d_nb_start.n * sizeof(int));
+    if (name == "neighbor_list") return give(d_nb_j.p, d_nb_j.n * sizeof(int));
+    if (name == "perp_para") return give(d_nb_f.p, d_nb_f.n * sizeof(double2));
+    if (name == "pair_coef") return give(d_pairc.p, d_pairc.n * sizeof(double));
+    if (name == "self_perp") return give(d_selfp.p, d_selfp.n * sizeof(double));
+    if (name == "scale_coef") {
+        if (quads) return give(d_kq64.p, d_kq64.n * sizeof(double));
+        if (slab && zcontig) return give(d_toe64.p, d_toe64.n * sizeof(double2));
+        return give(d_k64.p, d_k64.n * sizeof(double));
+    }
+    if (name == "fe_grid") return give(cur_grid, 0);
+    if (name == "fes_grid") return give(d_kgrid64.p ? (void*)d_kgrid64.p : (void*)d_kgrid32.p,
+                                        d_kgrid64.p ? d_kgrid64.n * sizeof(double2) : d_kgrid32.n * sizeof(float2));
+    throw Error(MPM_ERR_INVALID_ARGUMENT, "unknown device buffer name '" + name + "'");
 }
 
 void EwaldOp::info(mpm_op_info& o) const {

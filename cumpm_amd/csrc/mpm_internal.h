@@ -195,6 +195,14 @@ class EwaldOp {
     // convenience: contiguous [B][3N] buffers, chunked by kMaxSlots
     void apply_contiguous(const double2* x, double2* y, int B, bool fp32, int parts,
                           hipStream_t s, const int* rows = nullptr, const Workspace* ws = nullptr);
+    // One stage of an apply of one right-hand side on caller buffers (the reference's
+    // pipeline spread / scale / contract / realSpace): 1 spread x into grid, 2 transform,
+    // Green's tensor and inverse transform of grid in place, 3 y += contraction of grid,
+    // 4 y += real space and self terms of x (self-coefficient row 0, no preconditioner)
+    void stage(int which, const double2* x, void* grid, double2* y, bool fp32, hipStream_t s);
+    size_t stage_grid_bytes(bool fp32) const;
+    // device address and size of a named internal buffer (null when this layout has none)
+    void* device_pointer(const std::string& name, size_t& bytes) const;
     // workspace bytes an apply of B (<= kMaxSlots) right-hand sides needs
     size_t scratch_bytes(int B, bool fp32) const;
     bool builtin_fft() const;
@@ -292,6 +300,7 @@ class EwaldOp {
     // or the caller's workspace)
     void* cur_grid = nullptr;
     void* cur_kgrid = nullptr;
+    int phases = 7;  // parts of a launch: 1 spread, 2 transforms and Green's tensor, 4 contraction
     DevBuf<double2> d_grid64;
     DevBuf<float2> d_grid32;
     DevBuf<double2> d_kgrid64;   // slab k-space buffer (transposed columns)

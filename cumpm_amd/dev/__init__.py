@@ -1,3 +1,5 @@
-from .electric_field import MonodisperseEwaldElectricField, PolydisperseEwaldElectricField, FieldCalcMode
+from .electric_field import (DirectElectricField, FieldCalcMode, MonodisperseEwaldElectricField,
+                             PolydisperseEwaldElectricField)
 
-__all__ = ["MonodisperseEwaldElectricField", "PolydisperseEwaldElectricField", "FieldCalcMode"]
+__all__ = ["DirectElectricField", "MonodisperseEwaldElectricField", "PolydisperseEwaldElectricField",
+           "FieldCalcMode"]

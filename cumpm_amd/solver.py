@@ -19,10 +19,13 @@ from ._capi import check
 
 
 class PrecisionMode:
-    """Reference enum _cuMPM.PrecisionMode (ewald_electric_field_base.h; AUTO == MIXED)."""
+    """Reference enum _cuMPM.PrecisionMode (ewald_electric_field_base.h; AUTO == MIXED; FP32 and
+    FP64 are the direct field's names for MIXED and DOUBLE)."""
     AUTO = C.PREC_AUTO
     MIXED = C.PREC_MIXED
     DOUBLE = C.PREC_DOUBLE
+    FP32 = C.PREC_MIXED
+    FP64 = C.PREC_DOUBLE
 
 
 def drude_dielectric(omega, gamma, omega_p, eps_inf):

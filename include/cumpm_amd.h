@@ -164,6 +164,36 @@ mpm_status mpm_op_set_self_coef_device(mpm_op* op, const double* coef, int n_row
 mpm_status mpm_op_apply(mpm_op* op, const double* x, double* y, int B, int precision, int parts,
                         void* stream);
 
+/* The reference's dev-API pipeline (Ewald_Electric_Field_Base::spread / scale / contract /
+ * realSpace, electric_field_bindings.cpp:239-258) for one right-hand side on caller buffers:
+ * MPM_STAGE_SPREAD writes x (device, [3N + 5Q] complex) onto grid; MPM_STAGE_SCALE transforms
+ * grid, applies the Green's tensor and transforms back, in place; MPM_STAGE_CONTRACT adds the
+ * contraction of grid at the particles to y; MPM_STAGE_REAL_SPACE adds the real-space pair
+ * terms and self terms of x (self-coefficient row 0) to y.  SPREAD + SCALE + CONTRACT +
+ * REAL_SPACE on a zeroed y equals mpm_op_apply.  grid: device, mpm_op_grid_bytes bytes, in
+ * this library's layout (full grid [3][nx][ny][nz], or the slab [3][Zs][nx][nyp] of
+ * mpm_op_info.slab_planes occupied z-planes, rows padded to nyp = 16 ceil(ny / 16); with
+ * quadrupoles [8][nx][ny][nz]; polydisperse one scalar grid [nx][ny][nz]); complex float for
+ * MIXED/AUTO, complex double for DOUBLE.  The direct field has REAL_SPACE only. */
+enum { MPM_STAGE_SPREAD = 1, MPM_STAGE_SCALE = 2, MPM_STAGE_CONTRACT = 3, MPM_STAGE_REAL_SPACE = 4 };
+mpm_status mpm_op_grid_bytes(const mpm_op* op, int precision, size_t* bytes);
+mpm_status mpm_op_stage(mpm_op* op, int stage, const double* x, void* grid, double* y, int precision,
+                        void* stream);
+/* getDev* of the reference (electric_field_bindings.cpp:142-164): the device address of an
+ * internal buffer by name, valid until the next set_positions / set_bloch / destroy, and its
+ * size in bytes (0 and NULL when this operator's layout holds no such buffer):
+ * "x_part", "y_part", "z_part" (double [N] each), "self_coef" (complex double [rows][N]),
+ * "radius" (direct field), "spread_coef" / "contract_coef" (FP64 stencil weights: per-axis
+ * [N][3][width] monodisperse, [N][width^3] with quadrupoles, [N][P^3][3] polydisperse; the
+ * same with "_fp32"), "spread_idxs" / "contract_idxs" (int base voxel [3][N], or wrapped voxel
+ * per stencil point), "particle_index" (int [N] spatial order of the gathers),
+ * "neighbor_start" (int [N + 1] CSR), "neighbor_list" (int [M]), "perp_para" (complex-double
+ * pairs (perp, para) [M]), "pair_coef" (double per-pair coefficients), "self_perp"
+ * (polydisperse, double [N]), "scale_coef" (k-space table), "fe_grid" (the grid of the last
+ * apply), "fes_grid" (the slab's transposed k-space buffer).  Unknown names fail with
+ * MPM_ERR_INVALID_ARGUMENT. */
+mpm_status mpm_op_device_pointer(const mpm_op* op, const char* name, void** ptr, size_t* bytes);
+
 /* mpm_op_apply that allocates no device memory.  No counterpart in the reference: it is
  * for callers that cannot allocate (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  The reciprocal grids of the apply live in a
