@@ -1,6 +1,7 @@
-// Batched, device-resident Krylov solvers for A x = b over up to kMaxSlots
-// systems that share one Ewald operator (each with its own self-coefficient
-// row, i.e. its own frequency).
+// Batched, device-resident Krylov solvers for A x = b over many systems that
+// share one Ewald operator (each with its own self-coefficient row, i.e. its own
+// frequency): GMRES streams any number of systems through kMaxSlots slots,
+// BiCGSTAB runs them in groups of kMaxSlots.
 //
 // Semantics follow the reference solvers system by system:
 //   GMRES(m)  src/cpp/numerical_solver/gmres_solver.cu:56-208 — left Jacobi,
@@ -18,7 +19,8 @@
 // with one re-orthogonalisation pass (CGS2: two batched GEMV sweeps instead
 // of k+1 dependent dot/axpy pairs), numerically equivalent to the
 // reference's modified Gram-Schmidt.  Converged systems retire: they leave
-// the slot list and cost nothing in later applies.
+// the slot list, cost nothing in later applies, and (GMRES) hand their slot to
+// the next pending system.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,6 +41,11 @@ struct Ptrs {
     double2* a[kMaxSlots];
     double2* b[kMaxSlots];
     int sys[kMaxSlots];  // system index (row into per-system device arrays)
+    // per-slot data of the streaming GMRES (kernels templated on VAR read these):
+    // basis vectors to project against, right-hand side, self-coefficient row
+    int k[kMaxSlots];
+    const double2* c[kMaxSlots];
+    int row[kMaxSlots];
 };
 
 __device__ inline double2 cmul(double2 a, double2 b) {
@@ -60,11 +67,14 @@ __device__ inline double wsum(double v) {
 // basis vectors launches chunks * systems * ceil(k / kDotGroup) blocks and every
 // thread keeps kDotGroup independent loads in flight.
 constexpr int kDotGroup = 8;
-__global__ __launch_bounds__(256) void dots_kernel(Ptrs p, int kk, size_t stride, size_t n, size_t chunk,
-                                                   int mode, double2* __restrict__ partial) {
+template <bool VAR>
+__global__ __launch_bounds__(256) void dots_kernel_t(Ptrs p, int kmax, size_t stride, size_t n, size_t chunk,
+                                                     int mode, double2* __restrict__ partial) {
     __shared__ double s_red[4][2 * kDotGroup];
-    const int ng = (kk + kDotGroup - 1) / kDotGroup;
+    const int ng = (kmax + kDotGroup - 1) / kDotGroup;
     const int slot = blockIdx.y / ng, grp = blockIdx.y % ng;
+    const int kk = VAR ? p.k[slot] : kmax;  // VAR: each slot its own count (block-uniform exit)
+    if (grp * kDotGroup >= kk) return;
     const int blk = blockIdx.x;
     const size_t e0 = (size_t)blk * chunk;
     const size_t e1 = min(n, e0 + chunk);
@@ -125,6 +135,8 @@ __global__ __launch_bounds__(256) void dots_kernel(Ptrs p, int kk, size_t stride
     }
 }
 
+#define dots_kernel dots_kernel_t<false>
+
 // out[sys][i] (+)= sum_blk partial[slot][blk][i]
 __global__ void reduce_kernel(Ptrs p, int nslots, int kk, int nblk, const double2* __restrict__ partial,
                               double2* __restrict__ out, int out_stride, int accumulate,
@@ -154,13 +166,15 @@ __global__ void reduce_kernel(Ptrs p, int nslots, int kk, int nblk, const double
 // into h[sys] (pass 0) or adds it (pass 1).  With pnorm (the second pass), each
 // block also writes the sum of |w_new|^2 over its elements to
 // pnorm[slot * pstride + block], the partial norms of the next basis vector.
-__global__ __launch_bounds__(256) void cgs_axpy_kernel(Ptrs p, int kk, size_t stride, size_t n, int nblk,
-                                                       const double2* __restrict__ partial, double2* __restrict__ h,
-                                                       int hstride, int accumulate, double* __restrict__ pnorm,
-                                                       int pstride) {
+template <bool VAR>
+__global__ __launch_bounds__(256) void cgs_axpy_kernel_t(Ptrs p, int kk_arg, size_t stride, size_t n, int nblk,
+                                                         const double2* __restrict__ partial, double2* __restrict__ h,
+                                                         int hstride, int accumulate, double* __restrict__ pnorm,
+                                                         int pstride) {
     __shared__ double2 s_c[64];
     __shared__ double s_n[4];
     const int slot = blockIdx.y;
+    const int kk = VAR ? p.k[slot] : kk_arg;
     if (threadIdx.x < kk) {
         double sr = 0.0, si = 0.0;
         for (int b = 0; b < nblk; ++b) {
@@ -202,6 +216,8 @@ __global__ __launch_bounds__(256) void cgs_axpy_kernel(Ptrs p, int kk, size_t st
     }
 }
 
+#define cgs_axpy_kernel cgs_axpy_kernel_t<false>
+
 // b = a / sqrt(nn) with nn = sum_blk pnorm[slot][blk] (summed by the first wave of
 // every block in a fixed order); block 0 stores nn into norm2[sys].
 __global__ __launch_bounds__(256) void scale_norm_kernel(Ptrs p, size_t n, const double* __restrict__ pnorm,
@@ -234,16 +250,18 @@ __device__ inline double2 jac_inv(double2 sc) {
 
 // r = M^-1 (b - Ax) (or b - Ax without Jacobi); also keeps t = b - Ax for the true residual.
 // a = Ax, b = r (out);  rhs/true_out by system.
-__global__ __launch_bounds__(256) void residual_kernel(Ptrs p, size_t n, const double2* __restrict__ rhs_base,
-                                                       double2* __restrict__ true_base,
-                                                       const double2* __restrict__ selfc,
-                                                       const int* __restrict__ rows, int N, int jacobi,
-                                                       const int* __restrict__ qidx) {
+// VAR: the right-hand side p.c[slot] and self-coefficient row p.row[slot] per slot.
+template <bool VAR>
+__global__ __launch_bounds__(256) void residual_kernel_t(Ptrs p, size_t n, const double2* __restrict__ rhs_base,
+                                                         double2* __restrict__ true_base,
+                                                         const double2* __restrict__ selfc,
+                                                         const int* __restrict__ rows, int N, int jacobi,
+                                                         const int* __restrict__ qidx) {
     const int slot = blockIdx.y;
     const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (e >= n) return;
     const int s = p.sys[slot];
-    const double2 bv = rhs_base[(size_t)s * n + e];
+    const double2 bv = VAR ? p.c[slot][e] : rhs_base[(size_t)s * n + e];
     const double2 ax = p.a[slot][e];
     const double2 t = make_double2(bv.x - ax.x, bv.y - ax.y);
     true_base[(size_t)s * n + e] = t;
@@ -251,10 +269,12 @@ __global__ __launch_bounds__(256) void residual_kernel(Ptrs p, size_t n, const d
     if (jacobi) {  // the particle of component e: dipoles [N][3], then quadrupoles [Q][5]
         const size_t n3 = 3 * (size_t)N;
         const int part = e < n3 ? (int)(e / 3) : qidx[(e - n3) / 5];
-        r = cmul(t, jac_inv(selfc[(size_t)rows[s] * N + part]));
+        const int row = VAR ? p.row[slot] : rows[s];
+        r = cmul(t, jac_inv(selfc[(size_t)row * N + part]));
     }
     p.b[slot][e] = r;
 }
+#define residual_kernel residual_kernel_t<false>
 
 // b = a * (1/norm[sys]) when norm > 0
 __global__ __launch_bounds__(256) void scale_copy_kernel(Ptrs p, size_t n, const double* __restrict__ norm2) {
@@ -281,10 +301,11 @@ __global__ __launch_bounds__(256) void axpy_kernel(Ptrs p, size_t n, const doubl
 // GMRES column update + Givens rotations on the device, one wave per system.
 // H: [sys][(m+1) x m] row-major (row i, column k); h: new column coefficients [sys][m+1];
 // hn2: squared norm of the orthogonalised vector.  gmres_solver.cu:139-178.
-__global__ __launch_bounds__(64) void givens_kernel(Ptrs p, int nslots, int k, int m, double2* __restrict__ H,
-                                                    const double2* __restrict__ h, const double* __restrict__ hn2,
-                                                    double2* __restrict__ g, double* __restrict__ cs,
-                                                    double2* __restrict__ sn, double* __restrict__ resid) {
+template <bool VAR>
+__global__ __launch_bounds__(64) void givens_kernel_t(Ptrs p, int nslots, int k_arg, int m, double2* __restrict__ H,
+                                                      const double2* __restrict__ h, const double* __restrict__ hn2,
+                                                      double2* __restrict__ g, double* __restrict__ cs,
+                                                      double2* __restrict__ sn, double* __restrict__ resid) {
     // one wave per system: the new Hessenberg column and the earlier rotations are
     // staged in LDS, one lane applies the rotations in order
     __shared__ double2 s_col[64 + 2];
@@ -294,6 +315,7 @@ __global__ __launch_bounds__(64) void givens_kernel(Ptrs p, int nslots, int k, i
     if (slot >= nslots) return;
     const int lane = threadIdx.x;
     const int s = p.sys[slot];
+    const int k = VAR ? p.k[slot] - 1 : k_arg;  // VAR: the column this slot adds
     double2* Hs = H + (size_t)s * (m + 1) * m;
     const double2* hs = h + (size_t)s * (m + 1);
     double2* gs = g + (size_t)s * (m + 1);
@@ -351,6 +373,8 @@ __global__ __launch_bounds__(64) void givens_kernel(Ptrs p, int nslots, int k, i
     __syncthreads();
     for (int i = lane; i <= k + 1; i += 64) Hs[i * m + k] = s_col[i];
 }
+
+#define givens_kernel givens_kernel_t<false>
 
 // y = H^-1 g (upper triangular, kk columns per system); gmres_solver.cu:187-195.
 // One wave per system: the upper triangle is staged in LDS (packed by rows), then
@@ -488,7 +512,7 @@ struct Work {
     int B = 0, m = 0;
     DevBuf<double2> V, w, r, t, H, g, h, y, partial;
     DevBuf<double2> pb, sb, vb, tb, r0b, alpha, omega, beta;
-    DevBuf<double> nrm, resid, cs, pnorm;
+    DevBuf<double> nrm, tn, resid, cs, pnorm;
     DevBuf<int> rows, kk;
     DevBuf<double2> sn;
 };
@@ -524,8 +548,8 @@ struct Ctx {
         op.apply(t, (int)sys.size(), fp32, MPM_PART_ALL, s, jacobi);
     }
 
-    // squared norms of vectors b[slot] -> W.nrm[sys]
-    void norms(Ptrs& p, int ns) {
+    // squared norms of vectors b[slot] -> out[sys] (W.nrm by default)
+    void norms(Ptrs& p, int ns, double* out = nullptr) {
         const int nb = nblk_for(n);
         hipLaunchKernelGGL(dots_kernel, dots_grid(n, ns, 1), dim3(256), 0, s, p, 1, (size_t)0, n, chunk_for(n), 2, W.partial.p);
         MPM_HIP(hipGetLastError());
@@ -533,10 +557,10 @@ struct Ctx {
         hipLaunchKernelGGL(reduce_kernel, dim3((ns + 255) / 256), dim3(256), 0, s, p, ns, 1, nb,
                            W.partial.p, W.h.p, W.m + 1, 0, (double2*)nullptr);
         MPM_HIP(hipGetLastError());
-        // gather real parts into nrm[sys]
-        extract(p, ns);
+        // gather real parts into out[sys]
+        extract(p, ns, out ? out : W.nrm.p);
     }
-    void extract(Ptrs& p, int ns);
+    void extract(Ptrs& p, int ns, double* out);
 
     void read_scalars(const double* dev, const std::vector<int>& sys, std::vector<double>& out) {
         // dev indexed by system; copy the whole [B] array (small)
@@ -554,215 +578,275 @@ __global__ void extract_kernel(Ptrs p, int ns, const double2* __restrict__ h, in
     out[p.sys[slot]] = h[(size_t)p.sys[slot] * hs].x;
 }
 
-void Ctx::extract(Ptrs& p, int ns) {
-    hipLaunchKernelGGL(extract_kernel, dim3((ns + 63) / 64), dim3(64), 0, s, p, ns, W.h.p, W.m + 1, W.nrm.p);
+void Ctx::extract(Ptrs& p, int ns, double* out) {
+    hipLaunchKernelGGL(extract_kernel, dim3((ns + 63) / 64), dim3(64), 0, s, p, ns, W.h.p, W.m + 1, out);
     MPM_HIP(hipGetLastError());
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// GMRES
+// GMRES, streamed over slots
 // ---------------------------------------------------------------------------
-static void gmres_group(Ctx& C, const double2* b, double2* x, int B, mpm_solve_report* rep) {
+// Up to kMaxSlots systems occupy the slots at once; every workspace array (Krylov
+// basis, Hessenberg matrix, rotations, residual vectors) is indexed by slot, and a
+// system that finishes frees its slot for the next pending one, so that batches
+// of any size run as one continuous stream.  Each system follows the reference's
+// sequence on its own: outer residual M^-1 (b - A x) and its check, then an
+// Arnoldi cycle of up to m steps (leaving early on the Givens estimate), the
+// update x += V y, and again the outer residual.  A step of the stream advances
+// every occupied slot by one stage: one batched apply for the slots that need a
+// residual (FP32 and FP64 ones separately) and one for the slots in a cycle (each
+// at its own column), then the Gram-Schmidt, Givens and residual work of all of
+// them, and one read-back of the per-slot scalars that decide the next stage.
+static void gmres_stream(Ctx& C, const double2* b, double2* x, int B, mpm_solve_report* rep) {
     Work& W = C.W;
     const size_t n = C.n;
     const mpm_solve_desc& d = C.d;
     const int m = (int)std::min<size_t>(n, (size_t)std::max(1, d.restart));
-    W.n = n;
-    W.B = B;
-    W.m = m;
-    W.V.reserve((size_t)B * (m + 1) * n);
-    W.w.reserve((size_t)B * n);
-    W.t.reserve((size_t)B * n);
-    W.H.reserve((size_t)B * (m + 1) * m);
-    W.g.reserve((size_t)B * (m + 1));
-    W.h.reserve((size_t)B * (m + 1));
-    W.y.reserve((size_t)B * (m + 1));
-    W.partial.reserve((size_t)kMaxSlots * kRedBlocks * 64);
-    const int nbx = (int)((n + 255) / 256);  // blocks of the element-wise kernels
-    W.pnorm.reserve((size_t)kMaxSlots * nbx);
-    W.nrm.reserve(B);
-    W.resid.reserve(B);
-    W.cs.reserve((size_t)B * m);
-    W.sn.reserve((size_t)B * m);
-    W.kk.reserve(B);
-    W.rows.upload(C.rows_h.data(), B, C.s);
     MPM_REQUIRE(m <= 63, MPM_ERR_INVALID_ARGUMENT, "GMRES restart must be <= 63 in this library");
+    const int S = std::min(B, kMaxSlots);
+    W.n = n;
+    W.B = S;
+    W.m = m;
+    W.V.reserve((size_t)S * (m + 1) * n);
+    W.w.reserve((size_t)S * n);
+    W.t.reserve((size_t)S * n);
+    W.H.reserve((size_t)S * (m + 1) * m);
+    W.g.reserve((size_t)S * (m + 1));
+    W.h.reserve((size_t)S * (m + 1));
+    W.y.reserve((size_t)S * (m + 1));
+    W.partial.reserve((size_t)kMaxSlots * kRedBlocks * 64);
+    const int nbx = (int)((n + 255) / 256);
+    W.pnorm.reserve((size_t)kMaxSlots * nbx);
+    // per-slot scalars in one block, read back with one copy per step: ||M^-1 r||^2 (also the
+    // squared norm of the new Arnoldi vector), ||r||^2 of the outer residual, Givens estimate
+    W.tn.reserve(3 * kMaxSlots);
+    double* const s_nrm = W.tn.p;
+    double* const s_tn = W.tn.p + kMaxSlots;
+    double* const s_res = W.tn.p + 2 * kMaxSlots;
+    W.cs.reserve((size_t)S * m);
+    W.sn.reserve((size_t)S * m);
+    W.kk.reserve(S);
     const bool inner32 = d.precision != MPM_SOLVE_DOUBLE;
     const bool outer32 = d.precision == MPM_SOLVE_MIXED;
+    const double tol = d.tol;
     const size_t vstride = (size_t)(m + 1) * n;
-    auto Vk = [&](int sys, int k) { return W.V.p + (size_t)sys * vstride + (size_t)k * n; };
+    auto Vk = [&](int slot, int k) { return W.V.p + (size_t)slot * vstride + (size_t)k * n; };
+    auto row_of = [&](int q) { return C.op.self_rows > 1 ? q : 0; };
 
-    std::vector<int> all(B);
-    std::iota(all.begin(), all.end(), 0);
-    std::vector<int> iters(B, 0), matvecs(B, 0), mv64(B, 0), done(B, 0), conv(B, 0), fresh64(B, 0);
+    enum { PENDING = 0, RESIDUAL = 1, ARNOLDI = 2, DONE = 3 };
+    std::vector<int> st(B, PENDING), kcol(B, 0), first(B, 1), iters(B, 0), matvecs(B, 0), mv64(B, 0), conv(B, 0),
+        fresh64(B, 0), res32(B, 0);
     std::vector<double> bnorm(B, 0.0), last_pre(B, 0.0), last_true(B, 0.0);
+    std::vector<int> slot_sys(S, -1), slot_of(B, -1);
 
-    auto make_ptrs = [&](const std::vector<int>& sys, auto a_of, auto b_of) {
+    // squared norms of the vectors vec(q), q in sys (placed in slots 0.., |sys| <= S)
+    auto norms_of = [&](const std::vector<int>& sys, auto vec, std::vector<double>& out) {
         Ptrs p;
         for (size_t j = 0; j < sys.size(); ++j) {
-            p.a[j] = a_of(sys[j]);
-            p.b[j] = b_of(sys[j]);
-            p.sys[j] = sys[j];
+            p.a[j] = p.b[j] = vec(sys[j]);
+            p.sys[j] = (int)j;
         }
-        return p;
+        C.norms(p, (int)sys.size(), s_nrm);
+        MPM_HIP(hipMemcpyAsync(C.host_pinned, s_nrm, sys.size() * sizeof(double), hipMemcpyDeviceToHost, C.s));
+        MPM_HIP(hipStreamSynchronize(C.s));
+        out.assign(C.host_pinned, C.host_pinned + sys.size());
     };
-    // ||b||
-    {
-        Ptrs p = make_ptrs(all, [&](int q) { return (double2*)b + q * n; },
-                           [&](int q) { return (double2*)b + q * n; });
-        C.norms(p, B);
+    // ||b|| in chunks of S; zero right-hand sides give x = 0 at once
+    for (int q0 = 0; q0 < B; q0 += S) {
+        std::vector<int> sys;
+        for (int q = q0; q < std::min(B, q0 + S); ++q) sys.push_back(q);
         std::vector<double> nb2;
-        C.read_scalars(W.nrm.p, all, nb2);
-        for (int q = 0; q < B; ++q) {
-            bnorm[q] = std::sqrt(nb2[q]);
+        norms_of(sys, [&](int q) { return (double2*)b + (size_t)q * n; }, nb2);
+        for (size_t j = 0; j < sys.size(); ++j) {
+            const int q = sys[j];
+            bnorm[q] = std::sqrt(nb2[j]);
             if (bnorm[q] == 0.0) {
-                MPM_HIP(hipMemsetAsync(x + q * n, 0, n * sizeof(double2), C.s));
-                done[q] = 1;
+                MPM_HIP(hipMemsetAsync(x + (size_t)q * n, 0, n * sizeof(double2), C.s));
+                st[q] = DONE;
                 conv[q] = 1;
                 fresh64[q] = 1;
             }
         }
     }
-    std::vector<int> active;
-    for (int q = 0; q < B; ++q)
-        if (!done[q]) active.push_back(q);
-
-    const double tol = d.tol;
-    // REFINED: the initial residual may carry the FP32 operator's error (the
-    // end-of-cycle residual is FP64 and alone decides convergence), so the first
-    // outer residual is FP32 and is never taken as converged
-    bool first = true;
-    while (!active.empty()) {
-        const bool res32 = outer32 || (inner32 && first);
-        // ---- outer residual r = M^-1 (b - A x) ----
-        {
-            std::vector<const double2*> xs;
-            std::vector<double2*> ys;
-            for (int q : active) {
-                xs.push_back(x + q * n);
-                ys.push_back(W.w.p + q * n);
+    size_t next_pending = 0;
+    auto admit = [&]() {
+        for (int sl = 0; sl < S; ++sl) {
+            if (slot_sys[sl] >= 0) continue;
+            while (next_pending < (size_t)B && st[next_pending] != PENDING) ++next_pending;
+            if (next_pending >= (size_t)B) return;
+            const int q = (int)next_pending++;
+            slot_sys[sl] = q;
+            slot_of[q] = sl;
+            st[q] = RESIDUAL;
+        }
+    };
+    auto release = [&](int q) {
+        slot_sys[slot_of[q]] = -1;
+        slot_of[q] = -1;
+    };
+    auto apply_slots = [&](const std::vector<int>& sys, auto x_of, bool fp32, bool jac) {
+        if (sys.empty()) return;
+        SlotTable t;
+        for (size_t j = 0; j < sys.size(); ++j) {
+            t.x[j] = x_of(sys[j]);
+            t.y[j] = W.w.p + (size_t)slot_of[sys[j]] * n;
+            t.row[j] = row_of(sys[j]);
+        }
+        C.op.apply(t, (int)sys.size(), fp32, MPM_PART_ALL, C.s, jac);
+        for (int q : sys) {
+            matvecs[q]++;
+            if (!fp32) mv64[q]++;
+        }
+    };
+    // slot-indexed pointer tables: a[j] / b[j] of system sys[j], sys field = its slot
+    auto ptrs = [&](const std::vector<int>& sys, auto a_of, auto b_of) {
+        Ptrs p;
+        for (size_t j = 0; j < sys.size(); ++j) {
+            const int q = sys[j];
+            p.a[j] = a_of(q);
+            p.b[j] = b_of(q);
+            p.sys[j] = slot_of[q];
+            p.k[j] = kcol[q] + 1;
+            p.c[j] = b + (size_t)q * n;
+            p.row[j] = row_of(q);
+        }
+        return p;
+    };
+    const int nb = nblk_for(n);
+    // Residual checks are batched: a system whose cycle ended waits while others are still
+    // cycling, unless systems are queued for slots and a quarter of the slots wait (a
+    // residual apply costs about as much for a few systems as for many).
+    const int batch_wait = std::max(1, S / 4);
+    while (true) {
+        admit();
+        std::vector<int> R32, R64, A, waiting;
+        for (int sl = 0; sl < S; ++sl) {
+            const int q = slot_sys[sl];
+            if (q < 0) continue;
+            if (st[q] == RESIDUAL)
+                waiting.push_back(q);
+            else
+                A.push_back(q);
+        }
+        bool queued = false;
+        for (size_t q = next_pending; q < (size_t)B && !queued; ++q) queued = st[q] == PENDING;
+        if (A.empty() || (queued && (int)waiting.size() >= batch_wait)) {
+            for (int q : waiting) {
+                res32[q] = outer32 || (inner32 && first[q]);
+                (res32[q] ? R32 : R64).push_back(q);
             }
-            C.apply(active, xs, ys, res32);
-            for (int q : active) {
-                matvecs[q]++;
-                if (!res32) mv64[q]++;
-            }
-            Ptrs p = make_ptrs(active, [&](int q) { return W.w.p + q * n; }, [&](int q) { return Vk(q, 0); });
-            hipLaunchKernelGGL(residual_kernel, grid_for(n, (int)active.size()), dim3(256), 0, C.s, p, n,
-                               b, W.t.p, C.op.self_coef(), W.rows.p, C.N,
-                               d.jacobi, C.op.quad_idx_dev());
+        }
+        if (R32.empty() && R64.empty() && A.empty()) break;
+        std::vector<int> R(R32);
+        R.insert(R.end(), R64.begin(), R64.end());
+        // ---- applies: A x for the residual slots, M^-1 A V_k for the cycling ones ----
+        apply_slots(R32, [&](int q) { return (const double2*)(x + (size_t)q * n); }, true, false);
+        apply_slots(R64, [&](int q) { return (const double2*)(x + (size_t)q * n); }, false, false);
+        apply_slots(A, [&](int q) { return (const double2*)Vk(slot_of[q], kcol[q]); }, inner32, d.jacobi != 0);
+        const int nr = (int)R.size(), na = (int)A.size();
+        // ---- outer residuals: V_0 = M^-1 (b - A x), t = b - A x, their norms ----
+        if (nr) {
+            Ptrs p = ptrs(R, [&](int q) { return W.w.p + (size_t)slot_of[q] * n; },
+                          [&](int q) { return Vk(slot_of[q], 0); });
+            hipLaunchKernelGGL(residual_kernel_t<true>, grid_for(n, nr), dim3(256), 0, C.s, p, n, b, W.t.p,
+                               C.op.self_coef(), W.rows.p, C.N, d.jacobi, C.op.quad_idx_dev());
             MPM_HIP(hipGetLastError());
-            // norms of the true residual (W.t) and then of the preconditioned one (V_0), so
-            // that W.nrm ends up holding ||r||^2 for the V_0 / g set-up below
-            Ptrs pt = make_ptrs(active, [&](int q) { return W.t.p + q * n; }, [&](int q) { return W.t.p + q * n; });
-            C.norms(pt, (int)active.size());
-            std::vector<double> tn2;
-            C.read_scalars(W.nrm.p, active, tn2);
-            Ptrs pn = make_ptrs(active, [&](int q) { return Vk(q, 0); }, [&](int q) { return Vk(q, 0); });
-            C.norms(pn, (int)active.size());
-            std::vector<double> rn2;
-            C.read_scalars(W.nrm.p, active, rn2);
-            std::vector<int> still;
-            for (int q : active) {
-                const double rn = std::sqrt(rn2[q]);
-                last_pre[q] = rn / bnorm[q];
-                last_true[q] = std::sqrt(tn2[q]) / bnorm[q];
-                fresh64[q] = !res32;
-                if (rn < tol * bnorm[q] && (!(res32 && !outer32) || rn == 0.0)) {
-                    done[q] = 1;
-                    conv[q] = 1;
-                } else if (iters[q] >= d.maxiter) {
-                    done[q] = 1;  // maxiter exhausted (reference returns the partial x silently)
-                } else {
-                    still.push_back(q);
-                }
-            }
-            active.swap(still);
-            first = false;
-            if (active.empty()) break;
-            // V_0 = r / ||r||, g = (||r||, 0, ...)
-            Ptrs pv = make_ptrs(active, [&](int q) { return Vk(q, 0); }, [&](int q) { return Vk(q, 0); });
-            hipLaunchKernelGGL(scale_copy_kernel, grid_for(n, (int)active.size()), dim3(256), 0, C.s, pv, n, W.nrm.p);
-            hipLaunchKernelGGL(ginit_kernel, dim3((int)active.size()), dim3(64), 0, C.s, pv, (int)active.size(), m,
-                               W.g.p, W.nrm.p);
+            Ptrs pt = ptrs(R, [&](int q) { return W.t.p + (size_t)slot_of[q] * n; },
+                           [&](int q) { return W.t.p + (size_t)slot_of[q] * n; });
+            C.norms(pt, nr, s_tn);
+            Ptrs pn = ptrs(R, [&](int q) { return Vk(slot_of[q], 0); }, [&](int q) { return Vk(slot_of[q], 0); });
+            C.norms(pn, nr, s_nrm);
             MPM_HIP(hipGetLastError());
         }
-        // ---- Arnoldi cycle ----
-        std::vector<int> cyc = active;           // systems in this cycle
-        std::vector<int> kk(B, 0);               // columns produced per system
-        std::vector<int> in_cycle = cyc;
-        for (int k = 0; k < m && !in_cycle.empty(); ++k) {
-            const int ns = (int)in_cycle.size();
-            // w = M^-1 A V_k
-            {
-                std::vector<const double2*> xs;
-                std::vector<double2*> ys;
-                for (int q : in_cycle) {
-                    xs.push_back(Vk(q, k));
-                    ys.push_back(W.w.p + q * n);
-                }
-                C.apply(in_cycle, xs, ys, inner32, d.jacobi != 0);
-                for (int q : in_cycle) {
-                    matvecs[q]++;
-                    if (!inner32) mv64[q]++;
-                }
-            }
-            // CGS2: h = V^H w ; w -= V h ; h2 = V^H w ; w -= V h2 ; h += h2 (the
-            // projections summed inside the update; the second update also leaves
-            // the partial norms of w)
-            Ptrs pw = make_ptrs(in_cycle, [&](int q) { return Vk(q, 0); }, [&](int q) { return W.w.p + q * n; });
-            const int nb = nblk_for(n);
+        // ---- Arnoldi step of the cycling slots: CGS2, normalisation, Givens ----
+        if (na) {
+            int kmax = 0;
+            for (int q : A) kmax = std::max(kmax, kcol[q] + 1);
+            Ptrs pw = ptrs(A, [&](int q) { return Vk(slot_of[q], 0); }, [&](int q) { return W.w.p + (size_t)slot_of[q] * n; });
             for (int pass = 0; pass < 2; ++pass) {
                 ProfScope ps("krylov", C.s);
-                hipLaunchKernelGGL(dots_kernel, dots_grid(n, ns, k + 1), dim3(256), 0, C.s, pw, k + 1, n, n,
+                hipLaunchKernelGGL(dots_kernel_t<true>, dots_grid(n, na, kmax), dim3(256), 0, C.s, pw, kmax, n, n,
                                    chunk_for(n), 0, W.partial.p);
-                hipLaunchKernelGGL(cgs_axpy_kernel, grid_for(n, ns), dim3(256), 0, C.s, pw, k + 1, n, n, nb,
+                hipLaunchKernelGGL(cgs_axpy_kernel_t<true>, grid_for(n, na), dim3(256), 0, C.s, pw, kmax, n, n, nb,
                                    (const double2*)W.partial.p, W.h.p, m + 1, pass,
                                    pass == 1 ? W.pnorm.p : (double*)nullptr, nbx);
                 MPM_HIP(hipGetLastError());
             }
-            // ||w||^2 -> nrm, V_{k+1} = w / ||w||
-            Ptrs pc = make_ptrs(in_cycle, [&](int q) { return W.w.p + q * n; }, [&](int q) { return Vk(q, k + 1); });
-            hipLaunchKernelGGL(scale_norm_kernel, grid_for(n, ns), dim3(256), 0, C.s, pc, n, (const double*)W.pnorm.p,
-                               nbx, nbx, W.nrm.p);
-            hipLaunchKernelGGL(givens_kernel, dim3(ns), dim3(64), 0, C.s, pw, ns, k, m, W.H.p, W.h.p,
-                               W.nrm.p, W.g.p, W.cs.p, W.sn.p, W.resid.p);
+            Ptrs pc = ptrs(A, [&](int q) { return W.w.p + (size_t)slot_of[q] * n; },
+                           [&](int q) { return Vk(slot_of[q], kcol[q] + 1); });
+            hipLaunchKernelGGL(scale_norm_kernel, grid_for(n, na), dim3(256), 0, C.s, pc, n, (const double*)W.pnorm.p,
+                               nbx, nbx, s_nrm);
+            hipLaunchKernelGGL(givens_kernel_t<true>, dim3(na), dim3(64), 0, C.s, pw, na, 0, m, W.H.p, W.h.p,
+                               (const double*)s_nrm, W.g.p, W.cs.p, W.sn.p, s_res);
             MPM_HIP(hipGetLastError());
-            std::vector<double> res;
-            C.read_scalars(W.resid.p, in_cycle, res);
-            std::vector<int> next;
-            for (int q : in_cycle) {
-                iters[q]++;
-                kk[q] = k + 1;
-                const bool conv = res[q] < tol * bnorm[q];
-                if (!conv && k + 1 < m && iters[q] < d.maxiter) next.push_back(q);
-            }
-            in_cycle.swap(next);
         }
-        // ---- end of cycle: y = H^-1 g, x += V y ----
-        {
-            const int ns = (int)cyc.size();
-            Ptrs px = make_ptrs(cyc, [&](int q) { return Vk(q, 0); }, [&](int q) { return x + q * n; });
-            std::vector<int> kkB(B, 0);
-            for (int q : cyc) kkB[q] = kk[q];
-            W.kk.upload(kkB.data(), B, C.s);
+        // ---- one read-back: ||M^-1 r||^2, ||r||^2 of the residual slots, Givens estimates ----
+        double* hp = C.host_pinned;
+        MPM_HIP(hipMemcpyAsync(hp, s_nrm, 3 * kMaxSlots * sizeof(double), hipMemcpyDeviceToHost, C.s));
+        MPM_HIP(hipStreamSynchronize(C.s));
+        // ---- residual slots: converged, out of iterations, or a new cycle ----
+        std::vector<int> start;
+        for (int q : R) {
+            const int sl = slot_of[q];
+            const double rn = std::sqrt(hp[sl]);
+            last_pre[q] = rn / bnorm[q];
+            last_true[q] = std::sqrt(hp[kMaxSlots + sl]) / bnorm[q];
+            fresh64[q] = !res32[q];
+            // the initial REFINED residual is FP32 and never taken as converged
+            if (rn < tol * bnorm[q] && (!(res32[q] && !outer32) || rn == 0.0)) {
+                st[q] = DONE;
+                conv[q] = 1;
+            } else if (iters[q] >= d.maxiter) {
+                st[q] = DONE;  // maxiter exhausted (reference returns the partial x silently)
+            } else {
+                st[q] = ARNOLDI;
+                kcol[q] = 0;
+                start.push_back(q);
+            }
+            first[q] = 0;
+        }
+        // ---- cycling slots: the Givens estimate decides whether the cycle goes on ----
+        std::vector<int> finish;
+        for (int q : A) {
+            const int sl = slot_of[q];
+            iters[q]++;
+            const bool est = hp[2 * kMaxSlots + sl] < tol * bnorm[q];
+            if (est || kcol[q] + 1 >= m || iters[q] >= d.maxiter)
+                finish.push_back(q);
+            else
+                kcol[q]++;
+        }
+        // new cycles: V_0 = r / ||r||, g = (||r||, 0, ...)
+        if (!start.empty()) {
+            Ptrs pv = ptrs(start, [&](int q) { return Vk(slot_of[q], 0); }, [&](int q) { return Vk(slot_of[q], 0); });
+            const int ns = (int)start.size();
+            hipLaunchKernelGGL(scale_copy_kernel, grid_for(n, ns), dim3(256), 0, C.s, pv, n, (const double*)s_nrm);
+            hipLaunchKernelGGL(ginit_kernel, dim3(ns), dim3(64), 0, C.s, pv, ns, m, W.g.p, (const double*)s_nrm);
+            MPM_HIP(hipGetLastError());
+        }
+        // ended cycles: y = H^-1 g, x += V y, then the outer residual again
+        if (!finish.empty()) {
+            const int ns = (int)finish.size();
+            // columns per slot, staged in the pinned block after the three read-back rows (the
+            // next write to it follows the next step's stream synchronisation)
+            int* kk = reinterpret_cast<int*>(hp + 3 * kMaxSlots);
+            std::fill(kk, kk + S, 0);
+            for (int q : finish) kk[slot_of[q]] = kcol[q] + 1;
+            MPM_HIP(hipMemcpyAsync(W.kk.p, kk, S * sizeof(int), hipMemcpyHostToDevice, C.s));
+            Ptrs px = ptrs(finish, [&](int q) { return Vk(slot_of[q], 0); }, [&](int q) { return x + (size_t)q * n; });
             hipLaunchKernelGGL(backsolve_kernel, dim3(ns), dim3(64), (size_t)m * (m + 1) / 2 * sizeof(double2), C.s,
-                               px, ns,
-                               W.kk.p, m, W.H.p,
-                               W.g.p, W.y.p);
+                               px, ns, W.kk.p, m, W.H.p, W.g.p, W.y.p);
             hipLaunchKernelGGL(xupdate_kernel, grid_for(n, ns), dim3(256), 0, C.s, px, n, n, W.kk.p, W.y.p, m + 1);
             MPM_HIP(hipGetLastError());
+            for (int q : finish) {
+                fresh64[q] = 0;
+                st[q] = iters[q] < d.maxiter ? RESIDUAL : DONE;
+            }
         }
-        // systems that ran out of iterations leave without a further residual check
-        std::vector<int> still;
-        for (int q : cyc) {
-            fresh64[q] = 0;
-            if (iters[q] < d.maxiter) still.push_back(q);
-            else done[q] = 1;
-        }
-        active.swap(still);
+        for (int sl = 0; sl < S; ++sl)
+            if (slot_sys[sl] >= 0 && st[slot_sys[sl]] == DONE) release(slot_sys[sl]);
     }
 
     // Diagnostic FP64 residuals for systems whose last residual was not computed with
@@ -770,28 +854,30 @@ static void gmres_group(Ctx& C, const double2* b, double2* x, int B, mpm_solve_r
     std::vector<int> need;
     for (int q = 0; q < B; ++q)
         if (bnorm[q] > 0.0 && !fresh64[q]) need.push_back(q);
-    if (!need.empty()) {
-        std::vector<const double2*> xs;
-        std::vector<double2*> ys;
-        for (int q : need) {
-            xs.push_back(x + q * n);
-            ys.push_back(W.w.p + q * n);
+    for (size_t q0 = 0; q0 < need.size(); q0 += S) {
+        std::vector<int> sys(need.begin() + q0, need.begin() + std::min(need.size(), q0 + S));
+        for (size_t j = 0; j < sys.size(); ++j) {
+            slot_sys[j] = sys[j];
+            slot_of[sys[j]] = (int)j;
         }
-        C.apply(need, xs, ys, false);
-        Ptrs p = make_ptrs(need, [&](int q) { return W.w.p + q * n; }, [&](int q) { return W.w.p + q * n; });
-        hipLaunchKernelGGL(residual_kernel, grid_for(n, (int)need.size()), dim3(256), 0, C.s, p, n,
-                           b, W.t.p, C.op.self_coef(), W.rows.p, C.N, d.jacobi, C.op.quad_idx_dev());
+        SlotTable t;
+        for (size_t j = 0; j < sys.size(); ++j) {
+            t.x[j] = x + (size_t)sys[j] * n;
+            t.y[j] = W.w.p + j * n;
+            t.row[j] = row_of(sys[j]);
+        }
+        C.op.apply(t, (int)sys.size(), false, MPM_PART_ALL, C.s, false);
+        Ptrs p = ptrs(sys, [&](int q) { return W.w.p + (size_t)slot_of[q] * n; },
+                      [&](int q) { return W.w.p + (size_t)slot_of[q] * n; });
+        hipLaunchKernelGGL(residual_kernel_t<true>, grid_for(n, (int)sys.size()), dim3(256), 0, C.s, p, n, b, W.t.p,
+                           C.op.self_coef(), W.rows.p, C.N, d.jacobi, C.op.quad_idx_dev());
         MPM_HIP(hipGetLastError());
-        C.norms(p, (int)need.size());
-        std::vector<double> rn2;
-        C.read_scalars(W.nrm.p, need, rn2);
-        Ptrs pt = make_ptrs(need, [&](int q) { return W.t.p + q * n; }, [&](int q) { return W.t.p + q * n; });
-        C.norms(pt, (int)need.size());
-        std::vector<double> tn2;
-        C.read_scalars(W.nrm.p, need, tn2);
-        for (int q : need) {
-            last_pre[q] = std::sqrt(rn2[q]) / bnorm[q];
-            last_true[q] = std::sqrt(tn2[q]) / bnorm[q];
+        std::vector<double> rn2, tn2;
+        norms_of(sys, [&](int q) { return W.w.p + (size_t)slot_of[q] * n; }, rn2);
+        norms_of(sys, [&](int q) { return W.t.p + (size_t)slot_of[q] * n; }, tn2);
+        for (size_t j = 0; j < sys.size(); ++j) {
+            last_pre[sys[j]] = std::sqrt(rn2[j]) / bnorm[sys[j]];
+            last_true[sys[j]] = std::sqrt(tn2[j]) / bnorm[sys[j]];
         }
     }
     if (rep) {
@@ -1083,14 +1169,16 @@ void solve_batched(EwaldOp& op, const mpm_solve_desc& d, const double2* b, doubl
                 "need one self-coefficient row per system (or a single shared row)");
     const size_t n = op.n_unknowns();
     Ctx C(op, d, s);
+    if (d.method == MPM_GMRES) {
+        gmres_stream(C, b, x, B, reports);
+        MPM_HIP(hipStreamSynchronize(s));
+        return;
+    }
     for (int b0 = 0; b0 < B; b0 += kMaxSlots) {
         const int cnt = std::min(kMaxSlots, B - b0);
         C.rows_h.assign(cnt, 0);
         for (int j = 0; j < cnt; ++j) C.rows_h[j] = op.self_rows > 1 ? b0 + j : 0;
-        if (d.method == MPM_GMRES)
-            gmres_group(C, b + b0 * n, x + b0 * n, cnt, reports ? reports + b0 : nullptr);
-        else
-            bicgstab_group(C, b + b0 * n, x + b0 * n, cnt, reports ? reports + b0 : nullptr);
+        bicgstab_group(C, b + b0 * n, x + b0 * n, cnt, reports ? reports + b0 : nullptr);
     }
     MPM_HIP(hipStreamSynchronize(s));
 }

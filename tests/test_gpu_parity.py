@@ -230,6 +230,43 @@ def test_batched_equals_one_at_a_time():
         assert abs(rb[q]["iters"] - r1[0]["iters"]) <= 1
 
 
+@pytest.mark.parametrize("precision", [C.SOLVE_DOUBLE, C.SOLVE_REFINED])
+def test_stream_of_more_systems_than_slots(precision):
+    """70 systems (more than the 32 slots) in one GMRES call: retired systems' slots are
+    refilled, every system follows its own restart cycles (restart 12, so most need several),
+    and each result equals its solve alone; some systems hit maxiter while others continue."""
+    box, tol, xi, pos, grid, P = _geom("hex_slab")
+    _case(f"stream of 70 systems precision={precision}")
+    N = len(pos)
+    rng = np.random.default_rng(9)
+    B = 70
+    eps = rng.uniform(-4.0, -1.2, B) + 1j * rng.uniform(0.3, 1.0, B)
+    rows = np.stack([O.self_coefficient(np.full(N, e)) for e in eps])
+    b = rng.standard_normal((B, N, 3)) + 1j * rng.standard_normal((B, N, 3))
+    b[5] = 0.0  # a zero right-hand side retires at once
+    op = Op(box, tol, xi)
+    op.set_positions(pos)
+    op.set_self_coef(rows)
+    xb, rb = op.solve(b, tol=1e-9, restart=12, maxiter=600, precision=precision)
+    assert np.all(xb[5] == 0) and rb[5]["converged"] and rb[5]["iters"] == 0
+    for q in (0, 1, 17, 33, 40, 63, 69):
+        op1 = Op(box, tol, xi)
+        op1.set_positions(pos)
+        op1.set_self_coef(rows[q])
+        x1, r1 = op1.solve(b[q:q + 1], tol=1e-9, restart=12, maxiter=600, precision=precision)
+        assert rb[q]["converged"] and r1[0]["converged"]
+        assert rel(xb[q], x1[0]) < 1e-7, (q, rel(xb[q], x1[0]))
+        assert abs(rb[q]["iters"] - r1[0]["iters"]) <= 1
+    assert max(r["iters"] for r in rb) > 12  # several restart cycles happened
+    # a maxiter budget that some systems exhaust while the others converge
+    x2, r2 = op.solve(b, tol=1e-9, restart=12, maxiter=30, precision=precision)
+    it = np.array([r["iters"] for r in r2])
+    assert it.max() <= 30 and any(not r["converged"] for r in r2) and any(r["converged"] for r in r2)
+    for q, r in enumerate(r2):
+        if r["converged"] and q != 5:
+            assert r["relres_pre"] < 1e-9
+
+
 def test_refined_precision_reaches_fp64_accuracy():
     box, tol, xi, pos, grid, P = _geom("hex_slab")
     _case("refined")
