@@ -129,10 +129,10 @@ def run_secondary(cumpm_amd, name, steps, solver_type):
     print(json.dumps(line))
 
 
-def make_solver(cumpm_amd, box, eps, a, precision, batched, tol):
+def make_solver(cumpm_amd, box, eps, a, precision, batched, tol, autotune=False):
     return cumpm_amd.dipole_solver(box, eps, radius=a, eps_m=1.0, xi=0.5, tol=tol, quiet=True,
                                    solver_type="gmres", field_type="monodisperse", precision=precision,
-                                   batched=batched)
+                                   batched=batched, autotune=autotune)
 
 
 def run_once(s, pos):
@@ -275,6 +275,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 reference-order solve")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default layout and FFT instead of timing the candidates on the first frame")
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup + steps of the batched solve only (for rocprofv3)")
     ap.add_argument("--workload", default="hex4000",
@@ -309,7 +311,8 @@ def main():
     log = lambda *m: print("[bench]", *m, file=sys.stderr, flush=True)
 
     # ---- batched + mixed (refined) ----
-    s_mix = make_solver(cumpm_amd, box, eps, a, "refined", True, args.tol)
+    tune = not args.no_autotune
+    s_mix = make_solver(cumpm_amd, box, eps, a, "refined", True, args.tol, tune)
     if args.profile_only:  # per-kernel launch and RHS counts of every solve below
         _capi.profile_reset()
         _capi.profile_enable(True)
@@ -358,7 +361,7 @@ def main():
     # ---- plain FP64, reference order ----
     plain = None
     if not args.no_plain:
-        s_plain = make_solver(cumpm_amd, box, eps, a, "double", False, args.tol)
+        s_plain = make_solver(cumpm_amd, box, eps, a, "double", False, args.tol, tune)
         run_once(s_plain, pos)  # warm the FFT plans
         p_ms, _, _ = run_once(s_plain, pos)
         reps_plain = s_plain.get_reports()[0]
@@ -371,7 +374,7 @@ def main():
         }
         log(f"plain fp64: {p_ms:.1f} ms/sweep -> {plain['value']:.1f} RHS-solves/s")
         # batched FP64 (same batched code path, FP64 applies only)
-        s_b64 = make_solver(cumpm_amd, box, eps, a, "double", True, args.tol)
+        s_b64 = make_solver(cumpm_amd, box, eps, a, "double", True, args.tol, tune)
         run_once(s_b64, pos)
         b64_ms, _, _ = run_once(s_b64, pos)
         plain["batched_fp64_value"] = B / (b64_ms / 1e3)
@@ -434,6 +437,7 @@ def main():
             "grid": list(info.grid), "stencil": int(info.stencil_width), "rc": info.rc,
             "slab_planes": int(info.slab_planes), "fft": "built-in" if info.fft_backend == 0 else "rocFFT",
             "rhs": B, "particles": N, "parallelism": "single GPU, all RHS batched",
+            "autotune": s_mix.get_tune_info() if tune else None,
         },
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": (value / plain["value"]) if plain else None,

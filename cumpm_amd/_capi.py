@@ -25,6 +25,7 @@ SOLVE_DOUBLE, SOLVE_MIXED, SOLVE_REFINED = 0, 1, 2
 GMRES, BICGSTAB = 0, 1
 PART_REAL, PART_RECIP, PART_ALL = 1, 2, 3
 STAGE_SPREAD, STAGE_SCALE, STAGE_CONTRACT, STAGE_REAL_SPACE = 1, 2, 3, 4
+TUNE_GRID, TUNE_FORCE = 1, 2
 GUESS_DERIVATIVE, GUESS_MEAN_FIELD, GUESS_PREVIOUS = 0, 1, 2
 FIELD_AUTO, FIELD_MONODISPERSE, FIELD_POLYDISPERSE, FIELD_DIRECT = 0, 1, 2, 3
 
@@ -71,7 +72,18 @@ class DipoleDesc(ctypes.Structure):
                 ("quad_idxs", ctypes.POINTER(ctypes.c_int)), ("num_quad_idxs", _i64),
                 ("precision", ctypes.c_int), ("refined", ctypes.c_int), ("batched", ctypes.c_int),
                 ("restart", ctypes.c_int), ("maxiter", ctypes.c_int), ("grid", ctypes.c_int * 3),
-                ("support", ctypes.c_int), ("ewald_tol", ctypes.c_double)]
+                ("support", ctypes.c_int), ("ewald_tol", ctypes.c_double), ("autotune", ctypes.c_int)]
+
+
+class TuneInfo(ctypes.Structure):
+    _fields_ = [("layout", ctypes.c_int), ("fft_backend", ctypes.c_int), ("grid", ctypes.c_int * 3),
+                ("best_ms", ctypes.c_double), ("default_ms", ctypes.c_double), ("candidates", ctypes.c_int),
+                ("cached", ctypes.c_int)]
+
+    def as_dict(self):
+        return {"layout": self.layout, "fft_backend": self.fft_backend, "grid": list(self.grid),
+                "best_ms": self.best_ms, "default_ms": self.default_ms, "candidates": self.candidates,
+                "cached": bool(self.cached)}
 
 
 class EelsDesc(ctypes.Structure):
@@ -138,6 +150,9 @@ _SIGS = {
     "mpm_dipole_solver_set_use_jacobi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "mpm_dipole_solver_get_use_jacobi": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "mpm_dipole_solver_get_timing": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "mpm_dipole_solver_get_tune_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TuneInfo)]),
+    "mpm_op_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(TuneInfo)]),
     "mpm_dipole_solver_reset": (ctypes.c_int, [ctypes.c_void_p]),
     "mpm_ebeam_field": (ctypes.c_int, [_dp, _i64, ctypes.POINTER(ctypes.c_int), _i64, _dp, _i64, ctypes.c_double,
                                        _dp, ctypes.c_double, _dp]),

@@ -73,6 +73,8 @@ struct mpm_dipole_solver {
     int grid[3] = {0, 0, 0};
     int support = 0;
     double ewald_tol = 0.0;
+    int autotune = 0;         // 0 off, 1 layout and FFT, 2 also grid growth
+    mpm_tune_info tune{};     // the first frame's choice
     bool solve_quads = false;
     std::vector<int> quad_idxs;  // empty with solve_quads: every particle
     // state
@@ -296,6 +298,15 @@ struct mpm_dipole_solver {
             xyz[3 * i + 2] = z[i] / length_scale;
         }
         op->set_positions(xyz.data(), (int64_t)N);
+        if (autotune && !use_direct && frames == 0 && tune.candidates == 0) {
+            const int B = batched ? (int)std::min<size_t>(W * K, (size_t)mpm::kMaxSlots) : 1;
+            mpm::autotune(*op, B, precision != MPM_PREC_DOUBLE, autotune == 2 ? MPM_TUNE_GRID : 0, nullptr, tune);
+            print("Autotune: layout " + std::to_string(tune.layout) + ", fft_backend " +
+                  std::to_string(tune.fft_backend) + ", grid " + std::to_string(tune.grid[0]) + "x" +
+                  std::to_string(tune.grid[1]) + "x" + std::to_string(tune.grid[2]) + " (" +
+                  std::to_string(tune.best_ms) + " ms per apply, default " + std::to_string(tune.default_ms) +
+                  " ms)");
+        }
         const size_t n3 = n_unknowns();
         // initial guess: mean field, or the previous frame's first frequency (dipole_solver.cu:578-592)
         std::vector<cplx> init;
@@ -730,6 +741,15 @@ mpm_status mpm_op_device_pointer(const mpm_op* op, const char* name, void** ptr,
     });
 }
 
+mpm_status mpm_op_autotune(mpm_op* op, int B, int precision, int flags, mpm_tune_info* info) {
+    return guarded([&] {
+        MPM_REQUIRE(op && info, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        MPM_REQUIRE(precision >= 0 && precision <= 2, MPM_ERR_INVALID_ARGUMENT, "unknown precision");
+        MPM_REQUIRE((flags & ~(MPM_TUNE_GRID | MPM_TUNE_FORCE)) == 0, MPM_ERR_INVALID_ARGUMENT, "unknown flags");
+        mpm::autotune(*op->impl, B, precision != MPM_PREC_DOUBLE, flags, nullptr, *info);
+    });
+}
+
 mpm_status mpm_op_apply_noalloc(mpm_op* op, const double* x, double* y, int B, int precision, int parts,
                                 void* stream) {
     return guarded([&] {
@@ -856,6 +876,8 @@ mpm_status mpm_dipole_solver_create(const mpm_dipole_desc* d, mpm_dipole_solver*
         for (int k = 0; k < 3; ++k) s->grid[k] = d->grid[k];
         s->support = d->support;
         s->ewald_tol = d->ewald_tol;
+        MPM_REQUIRE(d->autotune >= 0 && d->autotune <= 2, MPM_ERR_INVALID_ARGUMENT, "autotune must be 0, 1 or 2");
+        s->autotune = d->autotune;
         s->solve_quads = d->solve_quadrupoles != 0;
         if (s->solve_quads && d->num_quad_idxs > 0) s->quad_idxs.assign(d->quad_idxs, d->quad_idxs + d->num_quad_idxs);
         *out = s.release();
@@ -954,6 +976,13 @@ mpm_status mpm_dipole_solver_get_use_jacobi(const mpm_dipole_solver* s, int* ena
     return guarded([&] {
         MPM_REQUIRE(s && enable, MPM_ERR_INVALID_ARGUMENT, "null argument");
         *enable = s->use_jacobi ? 1 : 0;
+    });
+}
+
+mpm_status mpm_dipole_solver_get_tune_info(const mpm_dipole_solver* s, mpm_tune_info* info) {
+    return guarded([&] {
+        MPM_REQUIRE(s && info, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        *info = s->tune;
     });
 }
 

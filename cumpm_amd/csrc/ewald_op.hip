@@ -4955,6 +4955,36 @@ void EwaldOp::apply_contiguous(const double2* x, double2* y, int B, bool fp32, i
     }
 }
 
+void EwaldOp::configure(int layout, int backend, const int* grid) {
+    MPM_REQUIRE(!direct, MPM_ERR_UNSUPPORTED, "the direct field has no reciprocal-space configuration");
+    MPM_REQUIRE(layout == 0 || layout == 1, MPM_ERR_INVALID_ARGUMENT, "layout must be 0 or 1");
+    MPM_REQUIRE(backend == 0 || backend == 1, MPM_ERR_INVALID_ARGUMENT, "fft_backend must be 0 or 1");
+    const bool regrid = grid && (grid[0] != S.n[0] || grid[1] != S.n[1] || grid[2] != S.n[2]);
+    if (poly || quads)
+        MPM_REQUIRE(layout == 1 && backend == 0 && !regrid, MPM_ERR_UNSUPPORTED,
+                    "the polydisperse and quadrupole operators run on the full grid with the built-in FFT");
+    if (regrid) {
+        Sizing T;
+        compute_sizing(L, errortol, xi, grid, S.P, T);  // same support, spacing from the new grid
+        MPM_REQUIRE(T.width <= kMaxWidth, MPM_ERR_UNSUPPORTED, "stencil too wide");
+        S = T;
+        V = (size_t)S.n[0] * S.n[1] * S.n[2];
+    }
+    force_full = layout;
+    fft_backend = backend;
+    lfft.reset();
+    fft.reset();
+    fft_key = -2;
+    kspace_ready = false;
+    kspace_key = -1;
+    if (N > 0) {
+        const std::vector<double> xyz = last_xyz;
+        const int64_t n = N;
+        last_xyz.clear();
+        set_positions(xyz.data(), n);
+    }
+}
+
 __global__ void add_kernel(double2* __restrict__ y, const double2* __restrict__ t, size_t n) {
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
         y[e].x += t[e].x;

@@ -203,6 +203,17 @@ class EwaldOp {
     size_t stage_grid_bytes(bool fp32) const;
     // device address and size of a named internal buffer (null when this layout has none)
     void* device_pointer(const std::string& name, size_t& bytes) const;
+    // Reciprocal-space configuration: layout 0 automatic / 1 full grid, FFT backend 0
+    // built-in / 1 rocFFT, and (grid != null) the grid dimensions at the same support;
+    // rebuilds the per-frame data of the current positions.  Every configuration is the
+    // same operator up to rounding, except a new grid (a finer discretisation).
+    void configure(int layout, int backend, const int* grid = nullptr);
+    int layout_setting() const { return force_full; }
+    int backend_setting() const { return fft_backend; }
+    const int* grid_dims() const { return S.n; }
+    bool is_poly() const { return poly; }
+    bool has_quads() const { return quads; }
+    bool is_slab() const { return slab; }
     // workspace bytes an apply of B (<= kMaxSlots) right-hand sides needs
     size_t scratch_bytes(int B, bool fp32) const;
     bool builtin_fft() const;
@@ -365,5 +376,9 @@ class EwaldOp {
 // Batched Krylov solve (krylov.hip).
 void solve_batched(EwaldOp& op, const mpm_solve_desc& d, const double2* b, double2* x, int B,
                    mpm_solve_report* reports, hipStream_t s);
+
+// Times the reciprocal-space configurations of `op` on B right-hand sides and leaves the
+// fastest configured (tune.hip); MPM_TUNE_* flags.
+void autotune(EwaldOp& op, int B, bool fp32, int flags, hipStream_t s, mpm_tune_info& out);
 
 }  // namespace mpm

@@ -1,0 +1,170 @@
+// Autotuner for the reciprocal-space configuration of an operator.
+//
+// The reference has one fixed pipeline; here the same discrete operator can run
+// on the z-plane slab or the full grid, with the built-in line FFT or rocFFT, and
+// (when the caller allows it) on a grid grown to lengths whose prime factors are at
+// most 7.  For a given operator, batch size and precision, every valid candidate
+// is configured, warmed up and timed over a few batched applies with HIP events on
+// the launch stream; the fastest is kept.  Choices are cached per (grid, particle
+// count, power-of-two batch bucket, precision, grid growth) for the life of the
+// process, so later operators of the same shape skip the timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "mpm_internal.h"
+
+namespace mpm {
+namespace {
+
+struct Key {
+    int n[3];
+    int64_t N;
+    int bucket, fp32, grow, kind;
+    bool operator<(const Key& o) const {
+        return std::tie(n[0], n[1], n[2], N, bucket, fp32, grow, kind) <
+               std::tie(o.n[0], o.n[1], o.n[2], o.N, o.bucket, o.fp32, o.grow, o.kind);
+    }
+};
+struct Choice {
+    int layout, backend, grid[3];
+    double best_ms, default_ms;
+    int candidates;
+};
+std::mutex g_mutex;
+std::map<Key, Choice> g_cache;
+
+// smallest length >= n whose prime factors are 2, 3, 5 and 7
+int smooth_at_least(int n) {
+    for (int m = std::max(n, 1);; ++m) {
+        int r = m;
+        for (int p : {2, 3, 5, 7})
+            while (r % p == 0) r /= p;
+        if (r == 1) return m;
+    }
+}
+
+__global__ void fill_kernel(double2* __restrict__ v, size_t n) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        v[e] = make_double2(std::sin(0.37 * (double)(e % 1013)), std::cos(0.21 * (double)(e % 577)));
+}
+
+}  // namespace
+
+void autotune(EwaldOp& op, int B, bool fp32, int flags, hipStream_t s, mpm_tune_info& out) {
+    MPM_REQUIRE(op.N > 0, MPM_ERR_RUNTIME, "autotune before set_positions");
+    MPM_REQUIRE(!op.is_direct(), MPM_ERR_UNSUPPORTED, "the direct field has nothing to tune");
+    MPM_REQUIRE(B >= 1, MPM_ERR_INVALID_ARGUMENT, "B must be >= 1");
+    const int Bt = std::min(B, kMaxSlots);
+    const bool grow = (flags & MPM_TUNE_GRID) != 0;
+    const bool force = (flags & MPM_TUNE_FORCE) != 0;
+    const int* n0 = op.grid_dims();
+    Key key{{n0[0], n0[1], n0[2]}, op.N, 1, fp32 ? 1 : 0, grow ? 1 : 0, op.is_poly() ? 2 : (op.has_quads() ? 1 : 0)};
+    while (key.bucket < Bt) key.bucket <<= 1;
+    std::memset(&out, 0, sizeof(out));
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        auto it = g_cache.find(key);
+        if (!force && it != g_cache.end()) {
+            const Choice& c = it->second;
+            op.configure(c.layout, c.backend, c.grid);
+            out.layout = c.layout;
+            out.fft_backend = c.backend;
+            for (int d = 0; d < 3; ++d) out.grid[d] = c.grid[d];
+            out.best_ms = c.best_ms;
+            out.default_ms = c.default_ms;
+            out.candidates = c.candidates;
+            out.cached = 1;
+            return;
+        }
+    }
+    // candidates: the current configuration first (the default), then the others
+    struct Cand {
+        int layout, backend, grid[3];
+    };
+    const int g0[3] = {n0[0], n0[1], n0[2]};
+    std::vector<Cand> cands;
+    auto add = [&](int l, int b, const int* g) {
+        for (const Cand& c : cands)
+            if (c.layout == l && c.backend == b && c.grid[0] == g[0] && c.grid[1] == g[1] && c.grid[2] == g[2]) return;
+        cands.push_back(Cand{l, b, {g[0], g[1], g[2]}});
+    };
+    add(op.layout_setting(), op.backend_setting(), g0);
+    if (!op.is_poly() && !op.has_quads()) {
+        int gs[3];
+        for (int d = 0; d < 3; ++d) gs[d] = smooth_at_least(g0[d]);
+        for (int l = 0; l < 2; ++l)
+            for (int b = 0; b < 2; ++b) {
+                add(l, b, g0);
+                if (grow) add(l, b, gs);
+            }
+    }
+    const size_t nu = op.n_unknowns();
+    DevBuf<double2> x, y;
+    x.resize((size_t)Bt * nu);
+    y.resize((size_t)Bt * nu);
+    hipLaunchKernelGGL(fill_kernel, dim3(1024), dim3(256), 0, s, x.p, (size_t)Bt * nu);
+    MPM_HIP(hipGetLastError());
+    const std::vector<int> rows(Bt, 0);  // every timed system on self-coefficient row 0
+    hipEvent_t e0, e1;
+    MPM_HIP(hipEventCreate(&e0));
+    MPM_HIP(hipEventCreate(&e1));
+    std::vector<double> ms(cands.size(), 1e300);
+    int best = 0;
+    try {
+        for (size_t c = 0; c < cands.size(); ++c) {
+            try {
+                op.configure(cands[c].layout, cands[c].backend, cands[c].grid);
+            } catch (const Error& e) {
+                if (e.status == MPM_ERR_UNSUPPORTED || e.status == MPM_ERR_INVALID_ARGUMENT) continue;
+                throw;
+            }
+            // a layout request the geometry cannot take (no slab) repeats another candidate
+            if (cands[c].layout == 0 && !op.is_slab() && c > 0) {
+                bool dup = false;
+                for (size_t k = 0; k < c; ++k)
+                    dup = dup || (cands[k].layout == 1 && cands[k].backend == cands[c].backend &&
+                                  std::equal(cands[k].grid, cands[k].grid + 3, cands[c].grid));
+                if (dup) continue;
+            }
+            op.apply_contiguous(x.p, y.p, Bt, fp32, MPM_PART_ALL, s, rows.data());  // warm-up: plans, caches
+            const int reps = 3;
+            MPM_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r) op.apply_contiguous(x.p, y.p, Bt, fp32, MPM_PART_ALL, s, rows.data());
+            MPM_HIP(hipEventRecord(e1, s));
+            MPM_HIP(hipEventSynchronize(e1));
+            float t = 0.f;
+            MPM_HIP(hipEventElapsedTime(&t, e0, e1));
+            ms[c] = t / reps;
+            if (ms[c] < ms[best]) best = (int)c;
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const Cand& b = cands[best];
+    op.configure(b.layout, b.backend, b.grid);
+    Choice ch{b.layout, b.backend, {b.grid[0], b.grid[1], b.grid[2]}, ms[best], ms[0], (int)cands.size()};
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        g_cache[key] = ch;
+    }
+    out.layout = ch.layout;
+    out.fft_backend = ch.backend;
+    for (int d = 0; d < 3; ++d) out.grid[d] = ch.grid[d];
+    out.best_ms = ch.best_ms;
+    out.default_ms = ch.default_ms;
+    out.candidates = ch.candidates;
+    out.cached = 0;
+}
+
+}  // namespace mpm

@@ -6,7 +6,10 @@ of libcumpm_amd (include/cumpm_amd.h).  Additions, all keyword-only and off
 by default: ``batched`` (all frequency x polarisation systems in one batched
 GPU solve), ``precision="refined"`` (FP32 reciprocal space inside the Krylov
 iterations, FP64 residual at every restart), ``restart``/``maxiter``,
-Ewald overrides ``grid``/``support``/``ewald_tol``, and ``get_reports()``.
+Ewald overrides ``grid``/``support``/``ewald_tol``, ``autotune`` (time the
+reciprocal-space layouts and FFT back-ends on the first frame and keep the
+fastest; ``autotune="grid"`` may also grow the grid to 7-smooth lengths), and
+``get_reports()``.
 """
 from __future__ import annotations
 
@@ -95,7 +98,9 @@ class dipole_solver:
     def __init__(self, box, eps_p, radius=1.0, eps_m=1.0, xi=0.5, tol=1e-3, quiet=False,
                  guess_type="derivative", solver_type="gmres", field_type="auto", E0=None,
                  quadrupoles=False, precision=None, *, batched=False, restart=50, maxiter=1000,
-                 grid=None, support=0, ewald_tol=0.0):
+                 grid=None, support=0, ewald_tol=0.0, autotune=False):
+        if autotune not in (False, True, "grid"):
+            raise ValueError(f"autotune must be False, True or 'grid', got {autotune!r}")
         if field_type not in ("auto", "monodisperse", "polydisperse", "direct"):
             raise ValueError(
                 f"field_type must be 'auto', 'monodisperse', 'polydisperse', or 'direct', got {field_type}")
@@ -184,6 +189,7 @@ class dipole_solver:
             d.grid[:] = [int(g) for g in grid]
         d.support = int(support)
         d.ewald_tol = float(ewald_tol)
+        d.autotune = 2 if autotune == "grid" else int(bool(autotune))
         h = ctypes.c_void_p()
         check(C.load().mpm_dipole_solver_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h
@@ -260,6 +266,14 @@ class dipole_solver:
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         check(C.load().mpm_dipole_solver_get_timing(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
+
+    def get_tune_info(self):
+        """The autotuner's choice on the first frame: layout, fft_backend, grid, best_ms and
+        default_ms per apply of one solve's batch, candidates, cached (zeros when autotune is
+        off)."""
+        info = C.TuneInfo()
+        check(C.load().mpm_dipole_solver_get_tune_info(self._h, ctypes.byref(info)))
+        return info.as_dict()
 
     def reset(self):
         """Drop stored frames (keeps the operator and FFT plans); the next compute() is a

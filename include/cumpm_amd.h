@@ -207,6 +207,28 @@ mpm_status mpm_op_device_pointer(const mpm_op* op, const char* name, void** ptr,
 mpm_status mpm_op_apply_noalloc(mpm_op* op, const double* x, double* y, int B, int precision, int parts,
                                 void* stream);
 
+/* Autotuner.  No counterpart in the reference, whose pipeline is fixed: the operator can run
+ * on the z-plane slab or the full grid (layout), with the built-in FFT or rocFFT
+ * (fft_backend), and with MPM_TUNE_GRID on a grid grown per axis to the next length whose
+ * prime factors are at most 7 (a finer discretisation: results change within the Ewald
+ * tolerance).  Each valid configuration is timed with HIP events over warm applies of
+ * min(B, 32) right-hand sides in `precision`; the fastest stays configured on op.  Choices
+ * are cached for the process per (grid, N, power-of-two batch bucket, precision, operator
+ * kind, MPM_TUNE_GRID); MPM_TUNE_FORCE times again.  The polydisperse and quadrupole
+ * operators have one configuration (reported, not timed away).  Needs positions set; the
+ * direct field fails with MPM_ERR_UNSUPPORTED. */
+enum { MPM_TUNE_GRID = 1, MPM_TUNE_FORCE = 2 };
+typedef struct {
+    int layout;         /* chosen mpm_op_desc.layout (0 automatic / slab, 1 full grid) */
+    int fft_backend;    /* chosen mpm_op_desc.fft_backend */
+    int grid[3];        /* grid in use after tuning */
+    double best_ms;     /* time of one apply of the batch in the chosen configuration */
+    double default_ms;  /* the same in the configuration the operator had before */
+    int candidates;     /* configurations considered */
+    int cached;         /* 1 when the choice came from the process cache (no timing) */
+} mpm_tune_info;
+mpm_status mpm_op_autotune(mpm_op* op, int B, int precision, int flags, mpm_tune_info* info);
+
 typedef struct {
     int method;     /* mpm_method */
     int restart;    /* GMRES restart length (reference default 50) */
@@ -278,6 +300,9 @@ typedef struct {
     int grid[3];              /* Ewald overrides (0 = reference rule) */
     int support;
     double ewald_tol;         /* 0 => tol (the reference ties the Ewald accuracy to tol) */
+    int autotune;             /* 0 off; 1 => mpm_op_autotune on the first frame, batch = the
+                                 systems of one solve, precision = the solve's; 2 => the same
+                                 with MPM_TUNE_GRID (no counterpart in the reference) */
 } mpm_dipole_desc;
 
 mpm_status mpm_dipole_solver_create(const mpm_dipole_desc* desc, mpm_dipole_solver** out);
@@ -308,6 +333,9 @@ mpm_status mpm_dipole_solver_get_use_jacobi(const mpm_dipole_solver* s, int* ena
  * precalcs), and the whole call. */
 mpm_status mpm_dipole_solver_get_timing(const mpm_dipole_solver* s, double* solve_ms, double* setup_ms,
                                         double* total_ms);
+/* The autotuner's result on the first frame (all zero while desc->autotune is 0 or before
+ * the first compute). */
+mpm_status mpm_dipole_solver_get_tune_info(const mpm_dipole_solver* s, mpm_tune_info* info);
 /* Forget the stored frames (results and warm-start state) but keep the operator, its FFT
  * plans and per-frame precalcs; the next compute() starts from the mean-field guess as a
  * first frame does.  No counterpart in the reference (its results only accumulate). */

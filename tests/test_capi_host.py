@@ -43,8 +43,8 @@ def _c_struct_sizes():
 #include <stdio.h>
 #include "cumpm_amd.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(mpm_op_desc), sizeof(mpm_op_info), sizeof(mpm_solve_desc),
-         sizeof(mpm_solve_report), sizeof(mpm_dipole_desc), sizeof(mpm_eels_desc));
+  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(mpm_op_desc), sizeof(mpm_op_info), sizeof(mpm_solve_desc),
+         sizeof(mpm_solve_report), sizeof(mpm_dipole_desc), sizeof(mpm_eels_desc), sizeof(mpm_tune_info));
   return 0;
 }
 '''
@@ -60,7 +60,8 @@ int main(void) {
 def test_ctypes_structs_match_header():
     sizes = _c_struct_sizes()
     py = [ctypes.sizeof(C.OpDesc), ctypes.sizeof(C.OpInfo), ctypes.sizeof(C.SolveDesc),
-          ctypes.sizeof(C.SolveReport), ctypes.sizeof(C.DipoleDesc), ctypes.sizeof(C.EelsDesc)]
+          ctypes.sizeof(C.SolveReport), ctypes.sizeof(C.DipoleDesc), ctypes.sizeof(C.EelsDesc),
+          ctypes.sizeof(C.TuneInfo)]
     assert sizes == py
 
 
