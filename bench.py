@@ -193,14 +193,53 @@ def kernel_bytes(name, info, N, B, pairs):
     return algorithmic_bytes(kernel_class(name), info, N, B, "<float" in name, pairs)
 
 
-# HBM bytes per dispatch (FETCH_SIZE + WRITE_SIZE, FETCH_SIZE scaled by the copy-kernel
-# calibration) measured with rocprofv3 --pmc on this workload, and the mean right-hand
-# sides per dispatch of that kernel in the profiled run; from
-# profiles/hex4000_sweep_batched_refined.md.
-MEASURED_TRAFFIC = {
-    "xtoeplitz_kernel<float, 10, 2, true>": ((107.4 + 74.5) * 1e6, 14.22),
-    "contract_mfma_kernel<float, 4, 10>": ((108.7 + 8.3) * 1e6, 18.12),
-}
+def _pmc_run(counter, args, outdir, name, timeout_s=120):
+    """One rocprofv3 --pmc pass of a single counter over `python3 bench.py <args>` in a child
+    process (this process stays idle meanwhile); returns {kernel name: [value per dispatch]}."""
+    import csv
+    import subprocess
+    cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", outdir, "-o", name, "--",
+           sys.executable, os.path.abspath(__file__)] + args
+    proc = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, start_new_session=True)
+    try:
+        _, err = proc.communicate(timeout=timeout_s)
+    except subprocess.TimeoutExpired:
+        os.killpg(proc.pid, 9)
+        proc.communicate()
+        raise RuntimeError(f"rocprofv3 --pmc {counter} exceeded {timeout_s} s")
+    if proc.returncode != 0:
+        raise RuntimeError(f"rocprofv3 --pmc {counter} failed ({proc.returncode}): {err.decode()[-300:]}")
+    out = {}
+    with open(os.path.join(outdir, f"{name}_counter_collection.csv")) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] == counter:
+                out.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    return out
+
+
+def measure_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` (a profiler-style name without the argument list) and
+    the right-hand sides per dispatch they were measured at: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch) from two rocprofv3 passes over one --profile-only solve of this workload, each
+    scaled by what the same counter reads for the library's plain copy kernel of known size
+    (--copy-calibration)."""
+    import tempfile
+    with tempfile.TemporaryDirectory(prefix="bench_pmc_") as d:
+        prof = ["--profile-only", "--no-autotune", "--steps", "1", "--warmup", "1"]
+        cal = ["--copy-calibration"]
+        copy_bytes = float(256 << 20)
+        scale = {}
+        for c in ("FETCH_SIZE", "WRITE_SIZE"):
+            vals = [v for k, vs in _pmc_run(c, cal, d, "cal_" + c).items() if "profile_copy_kernel" in k for v in vs]
+            scale[c] = copy_bytes / (np.mean(vals) * 1024.0)
+        total = 0.0
+        for c in ("FETCH_SIZE", "WRITE_SIZE"):
+            vals = [v for k, vs in _pmc_run(c, prof, d, "op_" + c).items()
+                    if k.replace("void mpm::", "").split("(")[0] == kernel for v in vs]
+            if not vals:
+                raise RuntimeError(f"no {kernel} dispatch in the --pmc {c} pass")
+            total += np.mean(vals) * 1024.0 * scale[c]
+    return total, scale
 
 
 def cpu_baseline(pos, box, a, eps, iters_needed, budget_s=20.0):
@@ -275,6 +314,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 reference-order solve")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc passes that measure roofline.traffic")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default layout and FFT instead of timing the candidates on the first frame")
     ap.add_argument("--profile-only", action="store_true",
@@ -401,12 +442,20 @@ def main():
     kd = kernels[dom]
     bytes_per_launch = kernel_bytes(dom, info, N, kd["mean_rhs"], pairs)
     achieved = bytes_per_launch / (kd["avg_ms"] / 1e3) / 1e9
-    traffic = None
-    if dom in MEASURED_TRAFFIC:  # scaled to this run's right-hand sides per launch
-        t_bytes, t_rhs = MEASURED_TRAFFIC[dom]
-        traffic = t_bytes * kd["mean_rhs"] / t_rhs
+    traffic, traffic_note = None, None
+    if not args.no_traffic:
+        # the --profile-only solve is this run's solve (same systems, same deferral), so its
+        # dispatches of `dom` carry the same right-hand sides per launch on average
+        try:
+            traffic, tscale = measure_traffic(dom)
+            traffic_note = (f"rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch of {dom} over one --profile-only "
+                            f"solve, scaled by the copy-kernel factors {tscale['FETCH_SIZE']:.3f} (fetch) and "
+                            f"{tscale['WRITE_SIZE']:.3f} (write)")
+            log(f"traffic: {traffic / 1e6:.1f} MB per launch ({traffic / bytes_per_launch:.2f}x algorithmic)")
+        except Exception as e:  # no profiler on this machine, or a failed pass: reported, not fatal
+            traffic_note = f"not measured: {e}"
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note, "kernel": dom,
                 "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": kd["avg_ms"],
                 "mean_rhs_per_launch": kd["mean_rhs"], "launches": kd["launches"]}
 

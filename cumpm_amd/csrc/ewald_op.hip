@@ -2302,7 +2302,10 @@ __global__ void kspace_slab_kernel(double* __restrict__ k64, float* __restrict__
 // a complex symmetric 3x3 per (d, column) with T_{-d} = conj(T_d) (the projector
 // sc k^ k^T is real).  Table layout [d][6][column], d = 0..Zs-1, components
 // (xx, yy, zz, xy, xz, yz), built in FP64 once per grid / Bloch vector / slab size.
-__global__ void ztoeplitz_table_kernel(double2* __restrict__ t64, float2* __restrict__ t32, KParams q, int Zs) {
+// t32ri (odd nz only, else null): the same table in FP32 as six floats per (d, column),
+// [Zs][ncols][6] = (Re xx, Re yy, Re zz, Re xy, Im xz, Im yz), the only nonzero parts then.
+__global__ void ztoeplitz_table_kernel(double2* __restrict__ t64, float2* __restrict__ t32, float* __restrict__ t32ri,
+                                       KParams q, int Zs) {
     const int ncols = kcols(q);
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const int d = blockIdx.y;
@@ -2328,6 +2331,15 @@ __global__ void ztoeplitz_table_kernel(double2* __restrict__ t64, float2* __rest
         const size_t e = ((size_t)d * 6 + c) * ncols + col;
         t64[e] = make_double2(tr[c], ti[c]);
         t32[e] = make_float2((float)tr[c], (float)ti[c]);
+    }
+    if (t32ri) {
+        float* o = t32ri + ((size_t)d * ncols + col) * 6;
+        o[0] = (float)tr[0];
+        o[1] = (float)tr[1];
+        o[2] = (float)tr[2];
+        o[3] = (float)tr[3];
+        o[4] = (float)ti[4];
+        o[5] = (float)ti[5];
     }
 }
 
@@ -2435,6 +2447,75 @@ __device__ __forceinline__ void toeplitz_column(typename C2<R>::T* base, int ld,
     }
 }
 
+// One plane pair of the odd-nz FP32 coupling: output plane o gets T f[b] with T's real
+// entries (xx, yy, zz, xy) and imaginary entries S (xz, yz), S = +1 for T_d, -1 for
+// T_{-d} = conj(T_d); every accumulator is a chain of FMAs.
+template <int ZS, int S>
+__device__ __forceinline__ void couple_ri32(float (&ar)[3][ZS], float (&ai)[3][ZS], const float2 (&f)[3][ZS], int o,
+                                            int b, float rxx, float ryy, float rzz, float rxy, float ixz, float iyz) {
+    const float2 fx = f[0][b], fy = f[1][b], fz = f[2][b];
+    const float sxz = S > 0 ? ixz : -ixz, syz = S > 0 ? iyz : -iyz;
+    ar[0][o] = fmaf(rxx, fx.x, ar[0][o]);
+    ar[0][o] = fmaf(rxy, fy.x, ar[0][o]);
+    ar[0][o] = fmaf(-sxz, fz.y, ar[0][o]);
+    ai[0][o] = fmaf(rxx, fx.y, ai[0][o]);
+    ai[0][o] = fmaf(rxy, fy.y, ai[0][o]);
+    ai[0][o] = fmaf(sxz, fz.x, ai[0][o]);
+    ar[1][o] = fmaf(rxy, fx.x, ar[1][o]);
+    ar[1][o] = fmaf(ryy, fy.x, ar[1][o]);
+    ar[1][o] = fmaf(-syz, fz.y, ar[1][o]);
+    ai[1][o] = fmaf(rxy, fx.y, ai[1][o]);
+    ai[1][o] = fmaf(ryy, fy.y, ai[1][o]);
+    ai[1][o] = fmaf(syz, fz.x, ai[1][o]);
+    ar[2][o] = fmaf(rzz, fz.x, ar[2][o]);
+    ar[2][o] = fmaf(-sxz, fx.y, ar[2][o]);
+    ar[2][o] = fmaf(-syz, fy.y, ar[2][o]);
+    ai[2][o] = fmaf(rzz, fz.y, ai[2][o]);
+    ai[2][o] = fmaf(sxz, fx.x, ai[2][o]);
+    ai[2][o] = fmaf(syz, fy.x, ai[2][o]);
+}
+
+// toeplitz_column for FP32 and an odd nz on the compact table: Tc points at the column's
+// six floats of [Zs][ncols][6], dstride = 6 ncols.  Each T_d is loaded once and applied to
+// the pairs at separations +d and -d; all ZS output planes accumulate in one pass.
+template <int ZS>
+__device__ __forceinline__ void toeplitz_column_ri32(float2* base, int ld, int Zs, const float* __restrict__ Tc,
+                                                     size_t dstride) {
+    float2 f[3][ZS];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int j = 0; j < ZS; ++j) f[c][j] = j < Zs ? base[(size_t)(c * Zs + j) * ld] : make_float2(0.f, 0.f);
+    float ar[3][ZS], ai[3][ZS];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < ZS; ++k) ar[c][k] = ai[c][k] = 0.f;
+#pragma unroll
+    for (int d = 0; d < ZS; ++d) {
+        if (d < Zs) {
+            const float* tp = Tc + (size_t)d * dstride;
+            const float2 t01 = *reinterpret_cast<const float2*>(tp);
+            const float2 t23 = *reinterpret_cast<const float2*>(tp + 2);
+            const float2 t45 = *reinterpret_cast<const float2*>(tp + 4);
+#pragma unroll
+            for (int a = d; a < ZS; ++a) {
+                if (a < Zs) {
+                    couple_ri32<ZS, 1>(ar, ai, f, a, a - d, t01.x, t01.y, t23.x, t23.y, t45.x, t45.y);
+                    if (d > 0) couple_ri32<ZS, -1>(ar, ai, f, a - d, a, t01.x, t01.y, t23.x, t23.y, t45.x, t45.y);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < ZS; ++a) {
+        if (a < Zs) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) base[(size_t)(c * Zs + a) * ld] = make_float2(ar[c][a], ai[c][a]);
+        }
+    }
+}
+
 // Applies the Toeplitz plane coupling in place on the slab [B][3][Zs][column].
 // One thread per (column, right-hand side); a block covers (256 / rb) columns x rb
 // right-hand sides so the rb threads of a column share the table lines in cache.
@@ -2494,11 +2575,15 @@ __device__ __forceinline__ void dif_stage(typename C2<R>::T* x, int base, int st
     for (int k = 0; k < P; ++k) x[base + k * stride] = v[k];
 }
 
+// Up to 10 planes, three blocks per CU fit the LDS (the 210-point lines of a 9-plane slab
+// take 45 KB per block): the register budget is held to the matching three waves per SIMD
+// (at most 168 VGPRs).  Thicker slabs hold more planes in registers and run two.
 template <typename R, int ZS, int CLS, bool RI>
-__global__ __launch_bounds__(256) void xtoeplitz_kernel(typename C2<R>::T* __restrict__ kbuf, int Zs, int ny,
+__global__ __launch_bounds__(256, ZS <= 10 ? 3 : 2) void xtoeplitz_kernel(typename C2<R>::T* __restrict__ kbuf, int Zs, int ny,
                                                         int nxp, int n, int P1, int P2,
                                                         const typename C2<R>::T* __restrict__ twiddle,
-                                                        const typename C2<R>::T* __restrict__ T) {
+                                                        const typename C2<R>::T* __restrict__ T,
+                                                        const float* __restrict__ Tri) {
     using CT = typename C2<R>::T;
     constexpr int AS = ztoe_as<R, ZS>();
     extern __shared__ unsigned char s_raw[];
@@ -2551,8 +2636,12 @@ __global__ __launch_bounds__(256) void xtoeplitz_kernel(typename C2<R>::T* __res
     }
     __syncthreads();
     const int ncols = ny * nxp;
-    for (int kx = tid; kx < n; kx += blockDim.x)
-        toeplitz_column<R, ZS, AS, RI>(X + kx, n, Zs, T + (size_t)ky * nxp + kx, ncols);
+    for (int kx = tid; kx < n; kx += blockDim.x) {
+        if constexpr (RI && std::is_same<R, float>::value)
+            toeplitz_column_ri32<ZS>(X + kx, n, Zs, Tri + ((size_t)ky * nxp + kx) * 6, (size_t)ncols * 6);
+        else
+            toeplitz_column<R, ZS, AS, RI>(X + kx, n, Zs, T + (size_t)ky * nxp + kx, ncols);
+    }
     __syncthreads();
     // inverse, radix P1, in place: X[li n + j2 + P2 r] -> DFT -> * conj W_n^{j2 k1}
     for (int w = tid; w < nl * P2; w += blockDim.x) {
@@ -3973,8 +4062,11 @@ void EwaldOp::ensure_kspace_table() {
         const size_t ncols = (size_t)kspace_cols();
         d_toe64.resize((size_t)Zs * 6 * ncols);
         d_toe32.resize((size_t)Zs * 6 * ncols);
+        const bool ri = (S.n[2] & 1) != 0;
+        if (ri) d_toe32ri.resize((size_t)Zs * 6 * ncols);
         hipLaunchKernelGGL(ztoeplitz_table_kernel, dim3((unsigned)((ncols + threads - 1) / threads), Zs),
-                           dim3(threads), 0, nullptr, d_toe64.p, d_toe32.p, kparams(), Zs);
+                           dim3(threads), 0, nullptr, d_toe64.p, d_toe32.p, ri ? d_toe32ri.p : nullptr, kparams(),
+                           Zs);
     } else {
         // [kz][column][4] over the slab's k-space columns, or [voxel][4] on the full grid
         const size_t nk = slab ? (size_t)kspace_cols() * S.n[2] : V;
@@ -4571,10 +4663,10 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
             ps.kernel("xtoeplitz_kernel", {type_name<R>(), #ZSV, "2", tbool(ri)});                                   \
             if (ri)                                                                                                  \
                 hipLaunchKernelGGL((xtoeplitz_kernel<R, ZSV, 2, true>), gx, dim3(256), xlds, s, kgrid, Zs, S.n[1],   \
-                                   nxp, S.n[0], ax.plan.radix[0], ax.plan.radix[1], tw, T);                         \
+                                   nxp, S.n[0], ax.plan.radix[0], ax.plan.radix[1], tw, T, d_toe32ri.p);            \
             else                                                                                                     \
                 hipLaunchKernelGGL((xtoeplitz_kernel<R, ZSV, 2, false>), gx, dim3(256), xlds, s, kgrid, Zs, S.n[1],  \
-                                   nxp, S.n[0], ax.plan.radix[0], ax.plan.radix[1], tw, T);                         \
+                                   nxp, S.n[0], ax.plan.radix[0], ax.plan.radix[1], tw, T, nullptr);                \
             break;
                 if constexpr (std::is_same<R, float>::value) {
                     switch (zs_bucket(Zs)) {

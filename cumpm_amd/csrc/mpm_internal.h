@@ -307,6 +307,7 @@ class EwaldOp {
     DevBuf<float> d_k32;
     DevBuf<double2> d_toe64;
     DevBuf<float2> d_toe32;
+    DevBuf<float> d_toe32ri;  // odd nz: [Zs][ncols][6] real xx, yy, zz, xy and imaginary xz, yz
     // grids (cur_grid / cur_kgrid: those of the apply in flight, the operator's own buffers
     // or the caller's workspace)
     void* cur_grid = nullptr;
