@@ -93,14 +93,23 @@ def secondary_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def run_secondary(cumpm_amd, name, steps, solver_type):
+def run_secondary(cumpm_amd, name, steps, solver_type, profile_only=False):
     """Batched refined sweep (timed over `steps` solves after one warm-up) beside the plain
-    per-frequency FP64 sweep in reference order; one JSON line."""
+    per-frequency FP64 sweep in reference order; one JSON line.  profile_only: the batched
+    refined solves only (the run a profiler wraps)."""
     pos, kw, nrhs, desc = secondary_workload(name)
     out = {}
     from cumpm_amd import _capi
     s = None
-    for label, prec, batched, n in (("batched_refined", "refined", True, steps), ("plain_fp64", "double", False, 1)):
+    legs = [("batched_refined", "refined", True, steps), ("plain_fp64", "double", False, 1)]
+    if profile_only:
+        s = cumpm_amd.dipole_solver(quiet=True, precision="refined", batched=True, solver_type=solver_type, **kw)
+        for _ in range(steps + 1):
+            s.reset()
+            s.compute(pos)
+        print(json.dumps({"workload": desc, "solves": steps + 1, "ms_per_sweep": s.get_timing()[0]}))
+        return
+    for label, prec, batched, n in legs:
         s = None  # the previous solver's device memory is released before this one is measured
         base = _capi.memory_stats(reset_peak=True)[0]
         s = cumpm_amd.dipole_solver(quiet=True, precision=prec, batched=batched, solver_type=solver_type, **kw)
@@ -343,7 +352,7 @@ def main():
         copy_calibration(_capi)
         return
     if args.workload != "hex4000":
-        run_secondary(cumpm_amd, args.workload, max(args.steps, 1), args.solver)
+        run_secondary(cumpm_amd, args.workload, max(args.steps, 1), args.solver, args.profile_only)
         return
 
     pos, box, a, omega, eps = hex_monolayer_config()

@@ -151,6 +151,8 @@ _SIGS = {
     "mpm_dipole_solver_get_use_jacobi": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "mpm_dipole_solver_get_timing": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "mpm_dipole_solver_get_tune_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TuneInfo)]),
+    "mpm_op_radius_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "mpm_op_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.POINTER(TuneInfo)]),
     "mpm_dipole_solver_reset": (ctypes.c_int, [ctypes.c_void_p]),

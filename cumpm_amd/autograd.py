@@ -137,6 +137,109 @@ class _Solve(torch.autograd.Function):
         return gb, gsc, None
 
 
+class PolyOperator:
+    """The polydisperse Ewald operator at fixed positions with the particle radii as an input
+    of the differentiable solve (`solve(b, sc, radii)`), in the units of box, positions and
+    radii as given (no rescaling: pick units where the radii are of order one).  The grid
+    and support are fixed at construction (from the initial radii), so that radius changes
+    move the operator smoothly; the real-space cutoff is max(sqrt(-ln ewald_tol) / xi,
+    2 a_max), which moves with the largest radius only when 2 a_max sets it.  Dipoles only
+    (the polydisperse operator carries no quadrupoles), no Bloch wave vector."""
+
+    def __init__(self, box, positions, radii, xi=0.5, ewald_tol=1e-6, grid=None, support=0, tol=1e-10, restart=50,
+                 maxiter=2000, precision=C.SOLVE_DOUBLE):
+        load_with_torch()
+        self.radius = 1.0
+        self.positions = np.ascontiguousarray(np.asarray(positions, np.float64).reshape(-1, 3))
+        self.N = self.positions.shape[0]
+        self.Q = 0
+        self.n = 3 * self.N
+        self.quad_owner = np.zeros(0, np.int64)
+        self.box = [float(v) for v in box]
+        self.xi = float(xi)
+        self.ewald_tol = float(ewald_tol)
+        self.grid = None if grid is None else [int(g) for g in grid]
+        self.support = int(support)
+        self.desc = C.SolveDesc(method=C.GMRES, restart=int(restart), maxiter=int(maxiter), tol=float(tol), jacobi=1,
+                                precision=int(precision))
+        self.reports = []
+        self.h = None
+        self._radii = None
+        self._build(np.asarray(radii, np.float64).reshape(-1))
+        info = C.OpInfo()
+        check(C.load().mpm_op_get_info(self.h, ctypes.byref(info)))
+        self.grid = [int(g) for g in info.grid]
+        self.support = int(info.support)
+
+    def _build(self, radii):
+        if radii.shape != (self.N,):
+            raise ValueError(f"need {self.N} radii")
+        if self._radii is not None and np.array_equal(radii, self._radii):
+            return
+        lib = C.load()
+        d = C.OpDesc()
+        d.box[:] = self.box
+        d.errortol = self.ewald_tol
+        d.xi = self.xi
+        if self.grid is not None:
+            d.grid[:] = self.grid
+        d.support = self.support
+        r = np.ascontiguousarray(radii, np.float64)
+        h = ctypes.c_void_p()
+        check(lib.mpm_op_create_poly(ctypes.byref(d), C.dptr(r), self.N, ctypes.byref(h)))
+        try:
+            check(lib.mpm_op_set_positions(h, C.dptr(self.positions), self.N))
+        except Exception:
+            lib.mpm_op_destroy(h)
+            raise
+        self.__del__()
+        self.h = h
+        self._radii = r.copy()
+
+    __del__ = Operator.__del__
+    _solve = Operator._solve
+    dipoles = Operator.dipoles
+
+    def solve(self, b, sc, radii):
+        """x = A(sc, radii)^-1 b for B systems: b (B, 3N), sc (B, N) complex128 and radii (N,)
+        float64, CUDA tensors; differentiable in all three."""
+        if b.dtype != torch.complex128 or sc.dtype != torch.complex128 or radii.dtype != torch.float64:
+            raise TypeError("b and sc must be complex128, radii float64")
+        if b.shape[-1] != self.n or sc.shape != (b.shape[0], self.N) or radii.shape != (self.N,):
+            raise ValueError(f"b must be (B, {self.n}), sc (B, {self.N}) and radii ({self.N},)")
+        if not (b.is_cuda and sc.is_cuda and radii.is_cuda):
+            raise ValueError("b, sc and radii must live on the GPU")
+        return _PolySolve.apply(b, sc, radii, self)
+
+
+class _PolySolve(torch.autograd.Function):
+    """Backward: grad_b and grad_sc as for _Solve; the operator part of the radius gradient
+    is dL/da_i = -Re(sum_b grad_b^H (dA/da_i) x_b) (mpm_op_radius_gradient)."""
+
+    @staticmethod
+    def forward(ctx, b, sc, radii, op):
+        op._build(radii.detach().cpu().numpy())
+        x = op._solve("mpm_solve", b, sc)
+        ctx.op = op
+        ctx.save_for_backward(x, sc, radii)
+        return x
+
+    @staticmethod
+    def backward(ctx, gx):
+        x, sc, radii = ctx.saved_tensors
+        op = ctx.op
+        op._build(radii.detach().cpu().numpy())  # another forward may have moved the radii
+        gb = op._solve("mpm_solve_adjoint", gx.contiguous(), sc)
+        N = op.N
+        gsc = -(torch.conj(x) * gb).reshape(-1, N, 3).sum(-1)
+        out = torch.empty(N, dtype=torch.float64, device=x.device)
+        xc = x.detach().contiguous()
+        stream = ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)
+        check(C.load().mpm_op_radius_gradient(op.h, ctypes.c_void_p(xc.data_ptr()), ctypes.c_void_p(gb.data_ptr()),
+                                              xc.shape[0], ctypes.c_void_p(out.data_ptr()), 0, stream))
+        return gb, gsc, -out, None
+
+
 def drude(omega, eps_inf, wp, gamma, eps_m=1.0):
     """(W, N) permittivities relative to the medium: eps_inf - wp^2 / (w^2 + i w gamma), every
     parameter a (N,) tensor (or scalar), omega (W,)."""
@@ -146,7 +249,8 @@ def drude(omega, eps_inf, wp, gamma, eps_m=1.0):
 
 
 def self_coefficient(eps, radius=1.0):
-    """-3 / (4 pi (1 - eps) a^3) (dipole_solver.cu:453-461)."""
+    """-3 / (4 pi (1 - eps) a^3) (dipole_solver.cu:453-461); radius a scalar or a tensor
+    broadcasting against eps (per-particle radii: (N,))."""
     return -3.0 / (4.0 * np.pi * (1.0 - eps)) / radius ** 3
 
 

@@ -741,6 +741,15 @@ mpm_status mpm_op_device_pointer(const mpm_op* op, const char* name, void** ptr,
     });
 }
 
+mpm_status mpm_op_radius_gradient(mpm_op* op, const double* x, const double* g, int B, double* out, int accumulate,
+                                  void* stream) {
+    return guarded([&] {
+        MPM_REQUIRE(op && x && g && out, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        op->impl->radius_gradient(reinterpret_cast<const double2*>(x), reinterpret_cast<const double2*>(g), B, out,
+                                  accumulate != 0, static_cast<hipStream_t>(stream));
+    });
+}
+
 mpm_status mpm_op_autotune(mpm_op* op, int B, int precision, int flags, mpm_tune_info* info) {
     return guarded([&] {
         MPM_REQUIRE(op && info, MPM_ERR_INVALID_ARGUMENT, "null argument");

@@ -160,24 +160,69 @@ static void compute_sizing_poly(const double L[3], double errortol, double xi, d
     S.width = S.P;
 }
 
+// Forward-mode dual number v + d eps (eps^2 = 0): the closed forms below, written once
+// for a generic scalar, give their derivative with respect to one radius when that radius
+// is seeded with d = 1 (radius gradients of the polydisperse operator).
+namespace dual {  // found by argument-dependent lookup only: no overload hides ::exp, ::min, ...
+struct Dual {
+    double v = 0.0, d = 0.0;
+    Dual() = default;
+    Dual(double x) : v(x) {}  // NOLINT: constants promote
+    Dual(double x, double dx) : v(x), d(dx) {}
+};
+static inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+static inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+static inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
+static inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+static inline Dual operator/(Dual a, Dual b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+static inline Dual operator+(Dual a, double b) { return {a.v + b, a.d}; }
+static inline Dual operator+(double a, Dual b) { return {a + b.v, b.d}; }
+static inline Dual operator-(Dual a, double b) { return {a.v - b, a.d}; }
+static inline Dual operator-(double a, Dual b) { return {a - b.v, -b.d}; }
+static inline Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
+static inline Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
+static inline Dual operator/(Dual a, double b) { return {a.v / b, a.d / b}; }
+static inline Dual operator/(double a, Dual b) { return {a / b.v, -a * b.d / (b.v * b.v)}; }
+static inline Dual& operator+=(Dual& a, Dual b) { return a = a + b; }
+static inline bool operator<(Dual a, Dual b) { return a.v < b.v; }
+static inline bool operator>=(Dual a, Dual b) { return a.v >= b.v; }
+static inline Dual exp(Dual a) {
+    const double e = std::exp(a.v);
+    return {e, e * a.d};
+}
+static inline Dual erfc(Dual a) { return {std::erfc(a.v), -2.0 / std::sqrt(kPi) * std::exp(-a.v * a.v) * a.d}; }
+static inline Dual erf(Dual a) { return {std::erf(a.v), 2.0 / std::sqrt(kPi) * std::exp(-a.v * a.v) * a.d}; }
+static inline Dual sqrt(Dual a) {
+    const double q = std::sqrt(a.v);
+    return {q, 0.5 * a.d / q};
+}
+static inline Dual abs(Dual a) { return a.v < 0.0 ? -a : a; }
+static inline Dual min(Dual a, Dual b) { return a.v <= b.v ? a : b; }
+static inline Dual ipow(Dual a, int n) { return {std::pow(a.v, n), n * std::pow(a.v, n - 1) * a.d}; }
+}  // namespace dual
+using dual::Dual;
+static inline double ipow(double a, int n) { return std::pow(a, n); }
+
 // Real-space pair functions of uniformly polarised spheres of radii a, b at
 // centre distance r: odd extensions in the radii,
 //   f = sum_{s,t = +-1} [A(r; s a, t b) exp(-(r + s a + t b)^2 xi^2) + C(r; s a, t b) erfc((r + s a + t b) xi)]
 //       + the direct overlap term,
 // A and C carrying the factor 1/(a^3 b^3) of their own signed arguments.  :1434-1621.
+template <typename T>
 struct PolyTerm {
-    double Ap, Cp, Aa, Ca;  // perpendicular and parallel coefficients
+    T Ap, Cp, Aa, Ca;  // perpendicular and parallel coefficients
 };
-static PolyTerm poly_terms(double r, double a, double b, double xi) {
+template <typename T>
+static PolyTerm<T> poly_terms(T r, T a, T b, double xi) {
     const double x2 = xi * xi, x4 = x2 * x2, x5 = x4 * xi, x6 = x4 * x2;
-    const double r2 = r * r, r3 = r2 * r, r4 = r2 * r2, r5 = r4 * r, r6 = r4 * r2;
-    const double a2 = a * a, b2 = b * b, a3 = a2 * a, b3 = b2 * b, s = a + b, s3 = s * s * s;
-    const double q = a2 - 4.0 * a * b + b2, d2 = (a2 - b2) * (a2 - b2);
-    const double c3 = 4.0 * a3 - 3.0 * a2 * b - 3.0 * a * b2 + 4.0 * b3;
-    const double e3 = a3 - 3.0 * a2 * b - 3.0 * a * b2 + b3;
-    const double m2 = a2 - a * b + b2;
-    const double de = std::pow(kPi, 1.5) * a3 * b3 * r3 * x5, dc = kPi * a3 * b3 * r3 * x6;
-    PolyTerm t;
+    const T r2 = r * r, r3 = r2 * r, r4 = r2 * r2, r5 = r4 * r, r6 = r4 * r2;
+    const T a2 = a * a, b2 = b * b, a3 = a2 * a, b3 = b2 * b, s = a + b, s3 = s * s * s;
+    const T q = a2 - 4.0 * a * b + b2, d2 = (a2 - b2) * (a2 - b2);
+    const T c3 = 4.0 * a3 - 3.0 * a2 * b - 3.0 * a * b2 + 4.0 * b3;
+    const T e3 = a3 - 3.0 * a2 * b - 3.0 * a * b2 + b3;
+    const T m2 = a2 - a * b + b2;
+    const T de = std::pow(kPi, 1.5) * a3 * b3 * r3 * x5, dc = kPi * a3 * b3 * r3 * x6;
+    PolyTerm<T> t;
     t.Ap = (4.0 * x4 * r5 - 4.0 * s * x4 * r4 + (16.0 * x2 + 8.0 * (-4.0 * a2 + a * b - 4.0 * b2) * x4) * r3 +
             (-12.0 * s * x2 - 8.0 * c3 * x4) * r2 + (3.0 - 12.0 * m2 * x2 - 4.0 * s * s * q * x4) * r + 3.0 * s +
             4.0 * c3 * x2 + 4.0 * s3 * q * x4) / (1024.0 * de);
@@ -192,23 +237,27 @@ static PolyTerm poly_terms(double r, double a, double b, double xi) {
     return t;
 }
 
-static void poly_pair_functions(double r, double a, double b, double xi, double& fperp, double& fpara) {
-    double fp = 0.0, fa = 0.0;
+template <typename T>
+static void poly_pair_functions(T r, T a, T b, double xi, T& fperp, T& fpara) {
+    using std::abs;
+    using std::erfc;
+    using std::exp;
+    T fp = 0.0, fa = 0.0;
     for (int sa = -1; sa <= 1; sa += 2)
         for (int sb = -1; sb <= 1; sb += 2) {
-            const double u = r + sa * a + sb * b;
-            const double e = std::exp(-u * u * xi * xi), c = std::erfc(u * xi);
-            const PolyTerm t = poly_terms(r, sa * a, sb * b, xi);
+            const T u = r + (double)sa * a + (double)sb * b;
+            const T e = exp(-u * u * xi * xi), c = erfc(u * xi);
+            const PolyTerm<T> t = poly_terms<T>(r, (double)sa * a, (double)sb * b, xi);
             fp += t.Ap * e + t.Cp * c;
             fa += t.Aa * e + t.Ca * c;
         }
-    const double r3 = r * r * r, a3 = a * a * a, b3 = b * b * b;
-    if (r < a + b && r >= std::abs(a - b)) {  // lens-shaped overlap
-        const double w = a + b - r;
+    const T r3 = r * r * r, a3 = a * a * a, b3 = b * b * b;
+    if (r < a + b && r >= abs(a - b)) {  // lens-shaped overlap
+        const T w = a + b - r;
         fp += w * w * w / (128.0 * kPi * a3 * b3 * r3) *
               (-r3 - 3.0 * (a + b) * r * r + 3.0 * (a * a - 4.0 * a * b + b * b) * r + a3 - 3.0 * a * a * b -
                3.0 * a * b * b + b3);
-        fa += -(std::pow(a + b, 4) * (a * a - 4.0 * a * b + b * b) - 8.0 * (a3 + b3) * r3 +
+        fa += -(ipow(a + b, 4) * (a * a - 4.0 * a * b + b * b) - 8.0 * (a3 + b3) * r3 +
                 9.0 * (a * a + b * b) * r3 * r - 2.0 * r3 * r3) / (64.0 * kPi * a3 * b3 * r3);
     } else if (r < b - a) {  // sphere a inside sphere b
         fp += (r3 - b3) / (4.0 * kPi * b3 * r3);
@@ -223,28 +272,35 @@ static void poly_pair_functions(double r, double a, double b, double xi, double&
 
 // r -> 0 limit (concentric spheres): a radius pair's table entry 0, and for a = b
 // a particle's real-space self term.  :1611-1620.
-static double poly_self_term(double a, double b, double xi) {
-    const double x2 = xi * xi, x3 = x2 * xi, a3 = a * a * a, b3 = b * b * b;
-    return ((1.0 - 2.0 * a * a * x2 + 2.0 * a * b * x2 - 2.0 * b * b * x2) * std::exp(-(a + b) * (a + b) * x2) +
-            (-1.0 + 2.0 * a * a * x2 + 2.0 * a * b * x2 + 2.0 * b * b * x2) * std::exp(-(a - b) * (a - b) * x2)) /
+template <typename T>
+static T poly_self_term(T a, T b, double xi) {
+    using std::erf;
+    using std::exp;
+    using std::min;
+    const double x2 = xi * xi, x3 = x2 * xi;
+    const T a3 = a * a * a, b3 = b * b * b;
+    return ((1.0 - 2.0 * a * a * x2 + 2.0 * a * b * x2 - 2.0 * b * b * x2) * exp(-(a + b) * (a + b) * x2) +
+            (-1.0 + 2.0 * a * a * x2 + 2.0 * a * b * x2 + 2.0 * b * b * x2) * exp(-(a - b) * (a - b) * x2)) /
                (16.0 * std::pow(kPi, 1.5) * a3 * b3 * x3) +
-           ((a3 - b3) * std::erf((a - b) * xi) - (a3 + b3) * std::erf((a + b) * xi)) / (8.0 * kPi * a3 * b3) +
-           std::min(a3, b3) / (4.0 * kPi * a3 * b3);
+           ((a3 - b3) * erf((a - b) * xi) - (a3 + b3) * erf((a + b) * xi)) / (8.0 * kPi * a3 * b3) +
+           min(a3, b3) / (4.0 * kPi * a3 * b3);
 }
 
 // Finite-sphere spread/contract weight vector of a particle of radius a at grid
 // offset gd (grid point minus centre): C(|gd|) gd / |gd|, with the reference's
 // separate form below |gd| = 1e-6.  :243-262.
-static void poly_weight(const double gd[3], double a, double xi, double eta, double w[3]) {
+template <typename T>
+static void poly_weight(const double gd[3], T a, double xi, double eta, T w[3]) {
+    using std::exp;
     const double d = std::sqrt(gd[0] * gd[0] + gd[1] * gd[1] + gd[2] * gd[2]);
     const double x2 = xi * xi;
-    double c;
+    T c;
     if (d >= 1e-6) {
-        const double k = 3.0 / (8.0 * std::sqrt(2.0) * std::pow(kPi, 1.5) * a * a * a * std::sqrt(eta) * xi * d * d);
-        c = k * ((eta + 4.0 * a * x2 * d) * std::exp(-2.0 * (d + a) * (d + a) * x2 / eta) -
-                 (eta - 4.0 * a * x2 * d) * std::exp(-2.0 * (d - a) * (d - a) * x2 / eta)) / d;
+        const T k = 3.0 / (8.0 * std::sqrt(2.0) * std::pow(kPi, 1.5) * a * a * a * std::sqrt(eta) * xi * d * d);
+        c = k * ((eta + 4.0 * a * x2 * d) * exp(-2.0 * (d + a) * (d + a) * x2 / eta) -
+                 (eta - 4.0 * a * x2 * d) * exp(-2.0 * (d - a) * (d - a) * x2 / eta)) / d;
     } else {
-        c = 8.0 * std::sqrt(2.0) * std::pow(xi, 5.0) * std::exp(-2.0 * a * x2 / eta) /
+        c = 8.0 * std::sqrt(2.0) * std::pow(xi, 5.0) * exp(-2.0 * a * x2 / eta) /
             (std::pow(kPi, 1.5) * std::pow(eta, 2.5));
     }
     for (int q = 0; q < 3; ++q) w[q] = c * gd[q];
@@ -3052,6 +3108,78 @@ __global__ __launch_bounds__(256) void poly_scale_kernel(typename C2<R>::T* __re
     }
 }
 
+// Radius gradient of the polydisperse operator, a wave per particle i (all B systems):
+//   out_i = sum_b Re(g_b^H (dA/da_i) x_b)
+// with three parts.  Reciprocal: h^3 sum_o dC_i(o) . (conj(g_i) phx[v] + x_i conj(phg[v])),
+// phx / phg the convolved spread grids of x / g (the kernel is real and even, so the
+// derivative of i's spread weight, seen through g^H, contracts against conj(phg)).  Real
+// space: for each neighbour j, D = dperp I + d(para - perp) u u^T (the row's derivative),
+// Re(g_i^H D x_j) + Re(g_j^H D x_i).  Self: dself_i Re(g_i^H x_i).  FP64 throughout.
+__global__ __launch_bounds__(256) void poly_rgrad_kernel(int B, int N, int V, int SS, size_t n3,
+                                                         const double2* __restrict__ x, const double2* __restrict__ g,
+                                                         const double2* __restrict__ phx,
+                                                         const double2* __restrict__ phg,
+                                                         const double* __restrict__ dpw, const int* __restrict__ pidx,
+                                                         double h3, const int* __restrict__ nb_start,
+                                                         const int* __restrict__ nb_j, const double* __restrict__ pairc,
+                                                         const double2* __restrict__ dpair,
+                                                         const double* __restrict__ dps, double* __restrict__ out,
+                                                         int accumulate) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= N) return;  // whole wave
+    double acc = 0.0, rec = 0.0;
+    for (int o = lane; o < SS; o += 64) {
+        const size_t item = (size_t)i * SS + o;
+        const int v = pidx[item];
+        const double w[3] = {dpw[3 * item], dpw[3 * item + 1], dpw[3 * item + 2]};
+        for (int b = 0; b < B; ++b) {
+            const double2 fx = phx[(size_t)b * V + v], fg = phg[(size_t)b * V + v];
+            const double2* xb = x + (size_t)b * n3 + 3 * (size_t)i;
+            const double2* gb = g + (size_t)b * n3 + 3 * (size_t)i;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                rec += w[c] * (gb[c].x * fx.x + gb[c].y * fx.y + xb[c].x * fg.x + xb[c].y * fg.y);
+        }
+    }
+    acc = h3 * rec;
+    for (int k = nb_start[i] + lane; k < nb_start[i + 1]; k += 64) {
+        const int j = nb_j[k];
+        const double* pc = pairc + 8 * (size_t)k;
+        const double u[3] = {pc[0], pc[1], pc[2]};
+        const double2 dp = dpair[k];
+        for (int b = 0; b < B; ++b) {
+            const double2* xi_ = x + (size_t)b * n3 + 3 * (size_t)i;
+            const double2* gi_ = g + (size_t)b * n3 + 3 * (size_t)i;
+            const double2* xj_ = x + (size_t)b * n3 + 3 * (size_t)j;
+            const double2* gj_ = g + (size_t)b * n3 + 3 * (size_t)j;
+            double uxj_r = 0, uxj_i = 0, uxi_r = 0, uxi_i = 0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                uxj_r += u[c] * xj_[c].x;
+                uxj_i += u[c] * xj_[c].y;
+                uxi_r += u[c] * xi_[c].x;
+                uxi_i += u[c] * xi_[c].y;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double djr = dp.x * xj_[c].x + dp.y * uxj_r * u[c], dji = dp.x * xj_[c].y + dp.y * uxj_i * u[c];
+                const double dir = dp.x * xi_[c].x + dp.y * uxi_r * u[c], dii = dp.x * xi_[c].y + dp.y * uxi_i * u[c];
+                acc += gi_[c].x * djr + gi_[c].y * dji + gj_[c].x * dir + gj_[c].y * dii;
+            }
+        }
+    }
+    if (lane == 0)
+        for (int b = 0; b < B; ++b) {
+            const double2* xb = x + (size_t)b * n3 + 3 * (size_t)i;
+            const double2* gb = g + (size_t)b * n3 + 3 * (size_t)i;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc += dps[i] * (gb[c].x * xb[c].x + gb[c].y * xb[c].y);
+        }
+    acc = wave_sum(acc);
+    if (lane == 0) out[i] = accumulate ? out[i] + acc : acc;
+}
+
 // A wave per (particle i, group of G right-hand sides): the reciprocal field
 // h^3 sum_o C_i(o) S[v(i, o)] over the particle's P^3 stencil points, the
 // real-space sum over its neighbours with the pair coefficients of the
@@ -3889,6 +4017,8 @@ void EwaldOp::setup_poly_stencils(const std::vector<double>& pos, int Nn) {
     std::vector<double> pw((size_t)Nn * SS * 3);
     std::vector<int> pidx((size_t)Nn * SS);
     std::vector<int> cnt(V + 1, 0);
+    pst_g.resize(3 * (size_t)Nn);
+    pst_dd.resize(3 * (size_t)Nn);
     for (int i = 0; i < Nn; ++i) {
         int g[3];
         double dd[3];
@@ -3896,6 +4026,8 @@ void EwaldOp::setup_poly_stencils(const std::vector<double>& pos, int Nn) {
             const double p = pos[(size_t)d * Nn + i];
             g[d] = (int)std::round(p / S.h[d]);
             dd[d] = g[d] * S.h[d] - p;
+            pst_g[3 * (size_t)i + d] = g[d];
+            pst_dd[3 * (size_t)i + d] = dd[d];
         }
         int o = 0;
         for (int ox = poff; ox < poff + P; ++ox)
@@ -3930,6 +4062,68 @@ void EwaldOp::setup_poly_stencils(const std::vector<double>& pos, int Nn) {
     d_vox_start.upload(cnt.data(), cnt.size());
     d_vox_items.upload(items.data(), items.size());
     d_selfp.upload(sp.data(), sp.size());
+    ++pst_serial;
+}
+
+// d/da_i of the pair values of particles i, j at distance r (>= 1e-6), from the closed
+// forms at the pair's own radii (the limit of the radius-class tables), with the rule
+// below r = 1 (linear between the r -> 0 value and the value at 1).
+void EwaldOp::poly_pair_dvalue(double r, int i, int j, double& dperp, double& dpara) const {
+    const Dual a(rad_h[i], 1.0), b(rad_h[j], 0.0);
+    Dual perp, para;
+    if (r >= 1.0) {
+        poly_pair_functions<Dual>(Dual(r), a, b, xi, perp, para);
+    } else {
+        const Dual s0 = poly_self_term<Dual>(a, b, xi);
+        Dual p1, q1;
+        poly_pair_functions<Dual>(Dual(1.0), a, b, xi, p1, q1);
+        perp = s0 * (1.0 - r) + p1 * r;
+        para = s0 * (1.0 - r) + q1 * r;
+    }
+    dperp = perp.d;
+    dpara = para.d;
+}
+
+// Radius derivatives of the current frame: every stencil point's weight vector, every
+// neighbour pair's coefficients (for the row's own radius) and every real-space self term.
+void EwaldOp::ensure_rgrad_data() {
+    if (rg_serial == pst_serial) return;
+    const int Nn = (int)N;
+    const int P = S.P, SS = P * P * P;
+    std::vector<double> dpw((size_t)Nn * SS * 3);
+    for (int i = 0; i < Nn; ++i) {
+        const Dual a(rad_h[i], 1.0);
+        int o = 0;
+        for (int ox = poff; ox < poff + P; ++ox)
+            for (int oy = poff; oy < poff + P; ++oy)
+                for (int oz = poff; oz < poff + P; ++oz, ++o) {
+                    const int off[3] = {ox, oy, oz};
+                    double gd[3];
+                    for (int d = 0; d < 3; ++d) gd[d] = pst_dd[3 * (size_t)i + d] + off[d] * S.h[d];
+                    Dual w[3];
+                    poly_weight<Dual>(gd, a, xi, peta, w);
+                    for (int q = 0; q < 3; ++q) dpw[((size_t)i * SS + o) * 3 + q] = w[q].d;
+                }
+    }
+    const size_t M = h_nb_j.size();
+    std::vector<double2> dpair(std::max<size_t>(M, 1), make_double2(0.0, 0.0));
+    for (int i = 0; i < Nn; ++i)
+        for (int k = h_nb_start[i]; k < h_nb_start[i + 1]; ++k) {
+            const double* r = &h_pair_r[3 * (size_t)k];
+            const double d = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+            double dp, dq;
+            poly_pair_dvalue(d < 1e-6 ? 1e-6 : d, i, h_nb_j[k], dp, dq);
+            dpair[k] = make_double2(dp, dq - dp);
+        }
+    std::vector<double> dps(Nn);
+    for (int i = 0; i < Nn; ++i) {
+        const Dual a(rad_h[i], 1.0);
+        dps[i] = poly_self_term<Dual>(a, a, xi).d;
+    }
+    d_dpw.upload(dpw.data(), dpw.size());
+    d_dpair.upload(dpair.data(), dpair.size());
+    d_dps.upload(dps.data(), dps.size());
+    rg_serial = pst_serial;
 }
 
 void EwaldOp::set_quads(const int* idx, int64_t nq) {
@@ -3956,6 +4150,8 @@ void EwaldOp::setup_quad_stencils(const std::vector<double>& pos, int Nn) {
     const double Cs = std::pow(2.0 * xi * xi / kPi, 1.5) * std::sqrt(1.0 / prod_eta);
     std::vector<double> sw((size_t)Nn * SS);
     std::vector<int> pidx((size_t)Nn * SS), cnt(V + 1, 0);
+    pst_g.resize(3 * (size_t)Nn);
+    pst_dd.resize(3 * (size_t)Nn);
     for (int i = 0; i < Nn; ++i) {
         int g[3];
         double dd[3];
@@ -3963,6 +4159,8 @@ void EwaldOp::setup_quad_stencils(const std::vector<double>& pos, int Nn) {
             const double p = pos[(size_t)d * Nn + i];
             g[d] = (int)std::round(p / S.h[d]);
             dd[d] = g[d] * S.h[d] - p;
+            pst_g[3 * (size_t)i + d] = g[d];
+            pst_dd[3 * (size_t)i + d] = dd[d];
         }
         int o = 0;
         for (int ox = -half; ox <= half; ++ox)
@@ -4925,6 +5123,48 @@ void EwaldOp::launch_poly(const SlotTable& t, int B, int parts, hipStream_t s, i
                        dim3(256), 0, s, t, B, parts, Ni, Vi, SS, grid, pw, d_pidx.p, h3, d_nb_start.p, d_nb_j.p,
                        d_pairc.p, d_selfc.p, d_selfp.p, jacobi, d_order.p);
     MPM_HIP(hipGetLastError());
+}
+
+void EwaldOp::radius_gradient(const double2* x, const double2* g, int B, double* out, bool accumulate,
+                              hipStream_t s) {
+    MPM_REQUIRE(poly, MPM_ERR_UNSUPPORTED, "radius gradients need the polydisperse operator");
+    MPM_REQUIRE(N > 0, MPM_ERR_RUNTIME, "radius gradient before set_positions");
+    MPM_REQUIRE(kb[0] == 0.0 && kb[1] == 0.0, MPM_ERR_UNSUPPORTED,
+                "radius gradients are provided without a Bloch wave vector");
+    MPM_REQUIRE(B >= 1, MPM_ERR_INVALID_ARGUMENT, "B must be >= 1");
+    ensure_rgrad_data();
+    ensure_kspace_table();
+    const int chunk = 8;
+    const int Bc = std::min(B, chunk);
+    d_rgx.reserve((size_t)Bc * V);
+    d_rgg.reserve((size_t)Bc * V);
+    const size_t n3 = n_unknowns();
+    const int SS = S.P * S.P * S.P;
+    const double h3 = S.h[0] * S.h[1] * S.h[2];
+    struct PhaseReset {
+        int& p;
+        ~PhaseReset() { p = 7; }
+    } reset{phases};
+    for (int b0 = 0; b0 < B; b0 += chunk) {
+        const int cnt = std::min(chunk, B - b0);
+        SlotTable tx, tg;
+        for (int j = 0; j < cnt; ++j) {
+            tx.x[j] = x + (size_t)(b0 + j) * n3;
+            tg.x[j] = g + (size_t)(b0 + j) * n3;
+            tx.y[j] = tg.y[j] = nullptr;
+            tx.row[j] = tg.row[j] = 0;
+        }
+        phases = 3;  // spread, transform, scale, inverse transform: the convolved grid
+        cur_grid = d_rgx.p;
+        launch_poly<double>(tx, cnt, MPM_PART_RECIP, s, 0);
+        cur_grid = d_rgg.p;
+        launch_poly<double>(tg, cnt, MPM_PART_RECIP, s, 0);
+        phases = 7;
+        hipLaunchKernelGGL(poly_rgrad_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, s, cnt, (int)N, (int)V, SS,
+                           n3, x + (size_t)b0 * n3, g + (size_t)b0 * n3, d_rgx.p, d_rgg.p, d_dpw.p, d_pidx.p, h3,
+                           d_nb_start.p, d_nb_j.p, d_pairc.p, d_dpair.p, d_dps.p, out, (accumulate || b0 > 0) ? 1 : 0);
+        MPM_HIP(hipGetLastError());
+    }
 }
 
 template <typename R>

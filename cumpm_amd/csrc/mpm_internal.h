@@ -179,6 +179,11 @@ class EwaldOp {
     void set_self_coef(const double* coef, int rows);
     // the same from device memory, enqueued on s
     void set_self_coef_device(const double* coef, int rows, hipStream_t s);
+    // Polydisperse operator, no Bloch vector: out[i] (+)= sum_b Re(g_b^H (dA/da_i) x_b) for
+    // device x, g [B][3N], the derivative of the operator (pair functions, real-space self
+    // term, finite-sphere spread/contract weights; not the self coefficients) with respect
+    // to particle i's radius.  accumulate: add to out instead of overwriting it.
+    void radius_gradient(const double2* x, const double2* g, int B, double* out, bool accumulate, hipStream_t s);
     // conjugates the device self coefficients in place (the adjoint operator's)
     void conj_self_coef(hipStream_t s);
     // Device memory a caller lends to an apply for its reciprocal grids (base 256-byte
@@ -370,6 +375,16 @@ class EwaldOp {
     double poly_lookup(const std::vector<double>& tab, double r, int u, int v) const;
     void poly_pair_value(double r, int i, int j, double& perp, double& para) const;
     void setup_poly_stencils(const std::vector<double>& pos, int Nn);
+    // radius derivatives of the current frame, built on the first radius_gradient call
+    std::vector<int> pst_g;          // [N][3] nearest grid point of each particle
+    std::vector<double> pst_dd;      // [N][3] its offset from the particle
+    int64_t pst_serial = 0, rg_serial = -1;
+    DevBuf<double> d_dpw;            // [N][P^3][3] d(weight vector)/da
+    DevBuf<double2> d_dpair;         // per CSR entry (d perp, d (para - perp)) / d a_row
+    DevBuf<double> d_dps;            // [N] d(real-space self term)/da
+    DevBuf<double2> d_rgx, d_rgg;    // reciprocal fields of x and g
+    void poly_pair_dvalue(double r, int i, int j, double& dperp, double& dpara) const;
+    void ensure_rgrad_data();
 
     friend struct OpAccess;
 };

@@ -207,6 +207,18 @@ mpm_status mpm_op_device_pointer(const mpm_op* op, const char* name, void** ptr,
 mpm_status mpm_op_apply_noalloc(mpm_op* op, const double* x, double* y, int B, int precision, int parts,
                                 void* stream);
 
+/* Radius derivative of the polydisperse operator, for the radius gradients of an autograd
+ * caller (no counterpart in the reference code; the derivatives of its pair functions
+ * polydisperse_ewald_electric_field.cu:1434-1621, real-space self term :1611-1620 and
+ * finite-sphere weights :243-262 with respect to the radius):
+ *   out[i] = sum_b Re(g_b^H (dA/da_i) x_b)      (accumulate = 1: out[i] += the same)
+ * for x, g device [B][3N] complex and out device double [N]; A is the operator without its
+ * self coefficients (their radius dependence is the caller's).  Pair derivatives come from
+ * the closed forms at each pair's own radii (the limit of the radius-class tables).  The
+ * polydisperse operator only (else MPM_ERR_UNSUPPORTED), without a Bloch wave vector. */
+mpm_status mpm_op_radius_gradient(mpm_op* op, const double* x, const double* g, int B, double* out, int accumulate,
+                                  void* stream);
+
 /* Autotuner.  No counterpart in the reference, whose pipeline is fixed: the operator can run
  * on the z-plane slab or the full grid (layout), with the built-in FFT or rocFFT
  * (fft_backend), and with MPM_TUNE_GRID on a grid grown per axis to the next length whose
