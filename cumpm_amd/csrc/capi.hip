@@ -88,6 +88,25 @@ struct mpm_dipole_solver {
     std::vector<mpm_solve_report> reports;  // [F][W][K]
     double solve_ms = 0, setup_ms = 0, total_ms = 0;
     mpm::DevBuf<double2> d_b, d_x;
+    // batched mode: right-hand sides (on the device), self-coefficient rows (in the operator)
+    // and mean-field guesses depend on the configuration only, so they are built once
+    bool batch_ready = false;
+    std::vector<cplx> mf_all;         // [B][n] mean-field guesses
+    cplx* h_stage = nullptr;          // pinned staging for the guesses and solutions
+    size_t h_stage_n = 0;
+    cplx* stage(size_t n) {
+        if (n > h_stage_n) {
+            if (h_stage) (void)hipHostFree(h_stage);
+            h_stage = nullptr;
+            h_stage_n = 0;
+            MPM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_stage), n * sizeof(cplx), hipHostMallocDefault));
+            h_stage_n = n;
+        }
+        return h_stage;
+    }
+    ~mpm_dipole_solver() {
+        if (h_stage) (void)hipHostFree(h_stage);
+    }
     int indent = 0;
 
     void print(const std::string& m) const {
@@ -244,6 +263,7 @@ struct mpm_dipole_solver {
 
     void compute(const double* x, const double* y, const double* z, size_t n) {
         if (n == 0) return;
+        mpm::HostScope whole("host_compute");
         auto t0 = std::chrono::steady_clock::now();
         if (N == 0) {
             // set-up is all or nothing: on failure (a box too small for the cutoff, bad
@@ -275,6 +295,7 @@ struct mpm_dipole_solver {
                     o->set_quads(quad_idxs.empty() ? nullptr : quad_idxs.data(),
                                  quad_idxs.empty() ? -1 : (int64_t)quad_idxs.size());
                 op = std::move(o);
+                batch_ready = false;
             } catch (...) {
                 box = box0;
                 radius = radius0;
@@ -297,7 +318,10 @@ struct mpm_dipole_solver {
             xyz[3 * i + 1] = y[i] / length_scale;
             xyz[3 * i + 2] = z[i] / length_scale;
         }
-        op->set_positions(xyz.data(), (int64_t)N);
+        {
+            mpm::HostScope hs("host_set_positions");
+            op->set_positions(xyz.data(), (int64_t)N);
+        }
         if (autotune && !use_direct && frames == 0 && tune.candidates == 0) {
             const int B = batched ? (int)std::min<size_t>(W * K, (size_t)mpm::kMaxSlots) : 1;
             mpm::autotune(*op, B, precision != MPM_PREC_DOUBLE, autotune == 2 ? MPM_TUNE_GRID : 0, nullptr, tune);
@@ -329,7 +353,7 @@ struct mpm_dipole_solver {
             MPM_HIP(hipDeviceSynchronize());
             device_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
         };
-        auto store = [&](size_t w, size_t k, const std::vector<cplx>& sol) {
+        auto store = [&](size_t w, size_t k, const cplx* sol) {
             for (size_t p = 0; p < N; ++p)
                 for (int c = 0; c < 3; ++c) fd[(w * N + p) * K * 3 + k * 3 + c] = sol[3 * p + c];
             for (size_t q = 0; q < Q; ++q)
@@ -353,41 +377,52 @@ struct mpm_dipole_solver {
                     timed_solve(d_b.p, d_x.p, 1, &reps[w * K + k]);
                     std::vector<cplx> sol(n3);
                     MPM_HIP(hipMemcpy(sol.data(), d_x.p, n3 * sizeof(double2), hipMemcpyDeviceToHost));
-                    store(w, k, sol);
+                    store(w, k, sol.data());
                 }
             }
             indent--;
         } else {
             const size_t B = W * K;
-            std::vector<cplx> sc_all(W * N);
-            std::vector<cplx> b_all(B * n3), x_all(B * n3);
-            for (size_t w = 0; w < W; ++w) {
-                auto sc = self_coef(w);
-                const std::vector<cplx> g = (w == 0) ? init : mean_field(w);
-                for (size_t k = 0; k < K; ++k) {
-                    auto b = rhs(k);
-                    std::copy(b.begin(), b.end(), b_all.begin() + (w * K + k) * n3);
-                    const std::vector<cplx> x0 = guess_column(g, k);
-                    std::copy(x0.begin(), x0.end(), x_all.begin() + (w * K + k) * n3);
+            mpm::HostLaps laps;
+            if (!batch_ready) {
+                std::vector<cplx> b_all(B * n3), rows(B * N);
+                mf_all.assign(B * n3, 0.0);
+                for (size_t w = 0; w < W; ++w) {
+                    const std::vector<cplx> sc = self_coef(w), g = mean_field(w);
+                    for (size_t k = 0; k < K; ++k) {
+                        const size_t sys = w * K + k;
+                        const std::vector<cplx> b = rhs(k), x0 = guess_column(g, k);
+                        std::copy(b.begin(), b.end(), b_all.begin() + sys * n3);
+                        std::copy(x0.begin(), x0.end(), mf_all.begin() + sys * n3);
+                        // one self-coefficient row per system (rows repeat across polarisations)
+                        std::copy(sc.begin(), sc.end(), rows.begin() + sys * N);
+                    }
                 }
-                std::copy(sc.begin(), sc.end(), sc_all.begin() + w * N);
+                d_b.reserve(B * n3);
+                MPM_HIP(hipMemcpy(d_b.p, b_all.data(), B * n3 * sizeof(double2), hipMemcpyHostToDevice));
+                op->set_self_coef(reinterpret_cast<const double*>(rows.data()), (int)B);
+                batch_ready = true;
             }
-            // one self-coefficient row per system (rows repeat across polarisations)
-            std::vector<cplx> rows(B * N);
-            for (size_t w = 0; w < W; ++w)
-                for (size_t k = 0; k < K; ++k)
-                    std::copy(sc_all.begin() + w * N, sc_all.begin() + (w + 1) * N, rows.begin() + (w * K + k) * N);
-            op->set_self_coef(reinterpret_cast<const double*>(rows.data()), (int)B);
-            d_b.reserve(B * n3);
+            // guesses: the mean field, except frequency 0 after the first frame, which starts
+            // from the previous frame's solution (dipole_solver.cu:578-592)
+            cplx* xs = stage(B * n3);
+            std::copy(mf_all.begin(), mf_all.end(), xs);
+            if (frames > 0)
+                for (size_t k = 0; k < K; ++k) {
+                    const std::vector<cplx> x0 = guess_column(init, k);
+                    std::copy(x0.begin(), x0.end(), xs + k * n3);
+                }
+            laps.lap("host_rhs_guesses");
             d_x.reserve(B * n3);
-            MPM_HIP(hipMemcpy(d_b.p, b_all.data(), B * n3 * sizeof(double2), hipMemcpyHostToDevice));
-            MPM_HIP(hipMemcpy(d_x.p, x_all.data(), B * n3 * sizeof(double2), hipMemcpyHostToDevice));
+            MPM_HIP(hipMemcpy(d_x.p, xs, B * n3 * sizeof(double2), hipMemcpyHostToDevice));
+            laps.lap("host_upload");
             timed_solve(d_b.p, d_x.p, (int)B, reps.data());
-            MPM_HIP(hipMemcpy(x_all.data(), d_x.p, B * n3 * sizeof(double2), hipMemcpyDeviceToHost));
+            laps.lap("host_solve_wall");
+            MPM_HIP(hipMemcpy(xs, d_x.p, B * n3 * sizeof(double2), hipMemcpyDeviceToHost));
+            laps.lap("host_download");
             for (size_t w = 0; w < W; ++w)
-                for (size_t k = 0; k < K; ++k)
-                    store(w, k, std::vector<cplx>(x_all.begin() + (w * K + k) * n3,
-                                                  x_all.begin() + (w * K + k + 1) * n3));
+                for (size_t k = 0; k < K; ++k) store(w, k, xs + (w * K + k) * n3);
+            laps.lap("host_store");
         }
         MPM_HIP(hipDeviceSynchronize());
         auto t2 = std::chrono::steady_clock::now();

@@ -31,6 +31,7 @@
 #include <functional>
 #include <mutex>
 #include <numeric>
+#include <thread>
 
 #include "mpm_internal.h"
 
@@ -568,6 +569,32 @@ ProfScope::~ProfScope() {
     (void)hipEventRecord(e1, stream);
     std::lock_guard<std::mutex> g(g_prof_mu);
     g_pending.push_back({name, kname, units, e0, e1});
+}
+
+HostScope::HostScope(const char* n) : name(n), on(g_prof_on) {
+    if (on) t0 = std::chrono::steady_clock::now();
+}
+HostScope::~HostScope() {
+    if (!on) return;
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    auto& e = g_prof[name];
+    e.ms += ms;
+    e.count += 1;
+}
+
+HostLaps::HostLaps() : on(g_prof_on) {
+    if (on) t = std::chrono::steady_clock::now();
+}
+void HostLaps::lap(const char* name) {
+    if (!on) return;
+    const auto now = std::chrono::steady_clock::now();
+    const double ms = std::chrono::duration<double, std::milli>(now - t).count();
+    t = now;
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    auto& e = g_prof[name];
+    e.ms += ms;
+    e.count += 1;
 }
 
 void profile_flush() {
@@ -3827,6 +3854,41 @@ __global__ __launch_bounds__(256) void direct_kernel(SlotTable tab, int B, int N
 }
 
 // ---------------------------------------------------------------------------
+// Host threads for the per-frame set-up
+// ---------------------------------------------------------------------------
+// Threads for n items with at least `grain` items each: up to 16 (one GPU's share of the
+// host) and the machine's hardware threads.
+static int host_threads(int n, int grain) {
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    return std::max(1, std::min({hw, 16, n / std::max(1, grain)}));
+}
+// First item of chunk c of n items split into T contiguous chunks.
+static int chunk_begin(int n, int T, int c) { return (int)((int64_t)n * c / T); }
+// f(begin, end, chunk) on T threads over contiguous chunks of [0, n); exceptions thrown by
+// any chunk are rethrown after every thread has finished.
+template <typename F>
+static void parallel_for(int n, int T, F&& f) {
+    if (T <= 1) {
+        f(0, n, 0);
+        return;
+    }
+    std::vector<std::thread> th;
+    std::vector<std::exception_ptr> err(T);
+    th.reserve(T);
+    for (int c = 0; c < T; ++c)
+        th.emplace_back([&, c] {
+            try {
+                f(chunk_begin(n, T, c), chunk_begin(n, T, c + 1), c);
+            } catch (...) {
+                err[c] = std::current_exception();
+            }
+        });
+    for (auto& t : th) t.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+}
+
+// ---------------------------------------------------------------------------
 // EwaldOp
 // ---------------------------------------------------------------------------
 EwaldOp::EwaldOp(const mpm_op_desc& d) {
@@ -4376,6 +4438,7 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     // the cache of precalcs is valid only once this call has finished
     last_xyz.clear();
     N = n;
+    HostLaps laps;
     const int W = S.width, half = S.half;
     std::vector<double> pos(3 * (size_t)Nn);
     std::vector<int> base(3 * (size_t)Nn);
@@ -4401,6 +4464,7 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
             }
         }
     }
+    laps.lap("host_setup_weights");
     // ---- reciprocal layout: slab of occupied z-planes, or the full grid ----
     const int nz = S.n[2];
     std::vector<char> occ(nz, 0);
@@ -4502,22 +4566,25 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
             for (int o = off; o < W; o += nd) acc += w64[((size_t)pid * 3 + d) * W + o];
             return acc;
         };
-        for (int r = 0; r < nreg; ++r) {
-            const int x0 = (r / reg_nry) * RX, y0 = (r % reg_nry) * RY;
-            for (int q = rs[r]; q < rs[r + 1]; ++q) {
-                const int pid = ri[q];
-                double* row = &rw64[(size_t)q * NW];
-                for (int j = 0; j < RX; ++j)
-                    if (x0 + j < S.n[0]) row[X + j] = wsum(pid, 0, x0 + j);
-                for (int p = 0; p < Zs; ++p) row[Zo + p] = wsum(pid, 2, zl[p]);
-                for (int j = 0; j < RY; ++j)
-                    if (y0 + j < S.n[1]) row[Yo + j] = wsum(pid, 1, y0 + j);
+        parallel_for(nreg, host_threads(nreg, 16), [&](int r0, int r1, int) {
+            for (int r = r0; r < r1; ++r) {
+                const int x0 = (r / reg_nry) * RX, y0 = (r % reg_nry) * RY;
+                for (int q = rs[r]; q < rs[r + 1]; ++q) {
+                    const int pid = ri[q];
+                    double* row = &rw64[(size_t)q * NW];
+                    for (int j = 0; j < RX; ++j)
+                        if (x0 + j < S.n[0]) row[X + j] = wsum(pid, 0, x0 + j);
+                    for (int p = 0; p < Zs; ++p) row[Zo + p] = wsum(pid, 2, zl[p]);
+                    for (int j = 0; j < RY; ++j)
+                        if (y0 + j < S.n[1]) row[Yo + j] = wsum(pid, 1, y0 + j);
+                }
             }
-        }
+        });
         std::vector<float> rw32(rw64.begin(), rw64.end());
         d_regw64.upload(rw64.data(), rw64.size());
         d_regw32.upload(rw32.data(), rw32.size());
     }
+    laps.lap("host_setup_regions");
     std::vector<int> ztl;  // z-tiles that hold at least one computed voxel
     for (int t = 0; t < ntiles[2]; ++t) {
         bool any = !slab;
@@ -4536,6 +4603,7 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         std::vector<int> fill(start.begin(), start.end() - 1);
         for (int i = 0; i < Nn; ++i) items[fill[bin_of[i]]++] = i;
     }
+    laps.lap("host_setup_bins");
     // real-space neighbours: all j != i with min-image distance < rc (neighbor_list.cu
     // keeps d^2 <= rc^2 and the kernel then skips d >= rc; pairs at exactly rc add zero)
     const PairTable& tab = *table;
@@ -4556,13 +4624,19 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
             members[(size_t)(cz * nc[1] + cy) * nc[0] + cx].push_back(j);
         }
     }
-    std::vector<int> nstart(Nn + 1, 0), nj;
-    std::vector<double2> nf;
-    std::vector<double> pr, pq;
-    nj.reserve((size_t)Nn * 64);
-    nf.reserve((size_t)Nn * 64);
-    pr.reserve((size_t)Nn * 64 * 3);
-    auto consider = [&](int i, int j) {
+    // particles split into contiguous chunks, one per host thread; each chunk's pairs are
+    // appended in particle order, so the list is the same for any thread count
+    struct Chunk {
+        std::vector<int> nj;
+        std::vector<double2> nf;
+        std::vector<double> pr, pq;
+    };
+    std::vector<int> nstart(Nn + 1, 0);
+    auto consider = [&](Chunk& ch, int i, int j) {
+        std::vector<int>& nj = ch.nj;
+        std::vector<double2>& nf = ch.nf;
+        std::vector<double>& pr = ch.pr;
+        std::vector<double>& pq = ch.pq;
         double r[3] = {pos[i] - pos[j], pos[Nn + i] - pos[Nn + j], pos[2 * Nn + i] - pos[2 * Nn + j]};
         for (int d = 0; d < 3; ++d) r[d] -= L[d] * std::round(r[d] / L[d]);
         const double d2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
@@ -4586,26 +4660,56 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         for (int d = 0; d < 3; ++d) rj[d] -= L[d] * std::round(rj[d] / L[d]);
         pr.insert(pr.end(), rj, rj + 3);
     };
-    for (int i = 0; i < Nn; ++i) {
-        if (cells) {
-            const int cx = cell_coord(pos[i], 0), cy = cell_coord(pos[Nn + i], 1), cz = cell_coord(pos[2 * Nn + i], 2);
-            for (int dz = -1; dz <= 1; ++dz)
-                for (int dy = -1; dy <= 1; ++dy)
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        const int a = (cx + dx + nc[0]) % nc[0], b = (cy + dy + nc[1]) % nc[1],
-                                  c = (cz + dz + nc[2]) % nc[2];
-                        for (int j : members[(size_t)(c * nc[1] + b) * nc[0] + a])
-                            if (j != i) consider(i, j);
-                    }
-        } else {
-            for (int j = 0; j < Nn; ++j)
-                if (j != i) consider(i, j);
+    const int nthreads = host_threads(Nn, 256);
+    std::vector<Chunk> chunks(nthreads);
+    parallel_for(Nn, nthreads, [&](int i0, int i1, int t) {
+        Chunk& ch = chunks[t];
+        const size_t guess = (size_t)(i1 - i0) * 64;
+        ch.nj.reserve(guess);
+        ch.nf.reserve(guess);
+        ch.pr.reserve(3 * guess);
+        for (int i = i0; i < i1; ++i) {
+            if (cells) {
+                const int cx = cell_coord(pos[i], 0), cy = cell_coord(pos[Nn + i], 1),
+                          cz = cell_coord(pos[2 * Nn + i], 2);
+                for (int dz = -1; dz <= 1; ++dz)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            const int a = (cx + dx + nc[0]) % nc[0], b = (cy + dy + nc[1]) % nc[1],
+                                      c = (cz + dz + nc[2]) % nc[2];
+                            for (int j : members[(size_t)(c * nc[1] + b) * nc[0] + a])
+                                if (j != i) consider(ch, i, j);
+                        }
+            } else {
+                for (int j = 0; j < Nn; ++j)
+                    if (j != i) consider(ch, i, j);
+            }
+            nstart[i + 1] = (int)ch.nj.size();  // chunk-local; made global below
         }
-        nstart[i + 1] = (int)nj.size();
+    });
+    std::vector<int> nj;
+    std::vector<double2> nf;
+    std::vector<double> pr, pq;
+    {
+        size_t tot = 0;
+        for (const Chunk& ch : chunks) tot += ch.nj.size();
+        nj.reserve(tot);
+        nf.reserve(tot);
+        pr.reserve(3 * tot);
+        for (int c = 0; c < nthreads; ++c) {
+            const int off = (int)nj.size();
+            for (int i = chunk_begin(Nn, nthreads, c); i < chunk_begin(Nn, nthreads, c + 1); ++i) nstart[i + 1] += off;
+            const Chunk& ch = chunks[c];
+            nj.insert(nj.end(), ch.nj.begin(), ch.nj.end());
+            nf.insert(nf.end(), ch.nf.begin(), ch.nf.end());
+            pr.insert(pr.end(), ch.pr.begin(), ch.pr.end());
+            pq.insert(pq.end(), ch.pq.begin(), ch.pq.end());
+        }
     }
     num_pairs = (int64_t)nj.size();
     h_nb_start = nstart;
     h_nb_j = nj;
+    laps.lap("host_setup_neighbors");
     // visiting order of the wave-per-particle kernels: columns of 8 x 8 voxels in x and
     // y, ascending z within a column, so that consecutive waves share stencil voxels
     {
@@ -4740,9 +4844,11 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         d_selfc.resize(Nn);
         MPM_HIP(hipMemset(d_selfc.p, 0, Nn * sizeof(double2)));
     }
+    laps.lap("host_setup_uploads_fft");
     // k-space tables of the layout just chosen: built here, so that applies allocate
     // nothing beyond their grids
     ensure_kspace_table();
+    laps.lap("host_setup_kspace");
     last_xyz.assign(xyz, xyz + 3 * n);
 }
 

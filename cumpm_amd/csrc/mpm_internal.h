@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <complex>
 #include <cstdint>
 #include <initializer_list>
@@ -127,6 +128,23 @@ struct ProfScope {
     int64_t units;
     hipStream_t stream;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+// Host wall time of a bracketed region (per-frame set-up, transfers, driver work), recorded
+// under `name` while profiling is on; no device synchronisation of its own.
+struct HostScope {
+    explicit HostScope(const char* name);
+    ~HostScope();
+    const char* name;
+    std::chrono::steady_clock::time_point t0;
+    bool on;
+};
+// Consecutive host regions: lap(name) records the time since the previous lap (or since
+// construction) under `name` while profiling is on.
+struct HostLaps {
+    HostLaps();
+    void lap(const char* name);
+    std::chrono::steady_clock::time_point t;
+    bool on;
 };
 template <typename R>
 inline std::string type_name() { return std::is_same<R, float>::value ? "float" : "double"; }
