@@ -136,6 +136,7 @@ _SIGS = {
     "mpm_near_field_set_dipole_positions": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _i64]),
     "mpm_near_field_set_field_points": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _i64]),
     "mpm_near_field_calculate": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "mpm_near_field_set_precision": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "mpm_memory_stats": (ctypes.c_int, [ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.c_int]),
     "mpm_dipole_solver_create": (ctypes.c_int, [ctypes.POINTER(DipoleDesc), ctypes.POINTER(ctypes.c_void_p)]),
     "mpm_dipole_solver_destroy": (None, [ctypes.c_void_p]),

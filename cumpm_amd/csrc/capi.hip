@@ -462,11 +462,12 @@ struct mpm_near_field {
     std::vector<double> radius;
     double xi = 0.5, errortol = 1e-3;
     int field_type = MPM_FIELD_AUTO;
+    int precision = MPM_PREC_AUTO;  // the reference's fields default to an FP32 reciprocal grid
     std::vector<cplx> dip;                      // [N][3]
     std::vector<double> dpos[3], fpos[3];       // dipole and field-point coordinates
     double length_scale = 1.0;
-    bool direct = false;
-    size_t op_sites = 0;  // direct field: particles + field points the operator's radii cover
+    bool direct = false, poly = false;
+    size_t op_sites = 0;  // direct / polydisperse field: particles + field points the radii cover
     std::unique_ptr<mpm::EwaldOp> op;
     mpm::DevBuf<double2> d_x, d_y;
 
@@ -489,11 +490,30 @@ struct mpm_near_field {
             bool mono = true;
             for (size_t i = 1; i < N; ++i) mono = mono && radius[i] == radius[0];
             direct = field_type == MPM_FIELD_DIRECT;
-            if (!direct && (field_type == MPM_FIELD_POLYDISPERSE || (field_type == MPM_FIELD_AUTO && !mono)))
-                throw mpm::Error(MPM_ERR_UNSUPPORTED,
-                                 "the near field of the polydisperse operator is not provided by this library");
+            poly = !direct && (field_type == MPM_FIELD_POLYDISPERSE || (field_type == MPM_FIELD_AUTO && !mono));
             length_scale = radius[0];  // near_field.cu:120 (the first radius)
-            if (direct) {
+            if (poly) {
+                // The field points are sites without dipoles.  Their radii follow the reference's
+                // INTERACTION_FIELD evaluation by index: field point f takes particle f's radius
+                // for f < N; beyond, its spread weight takes the first radius
+                // (polydisperse_ewald_electric_field.cu:297) and its pair functions the smallest
+                // radius class (:402, classes sorted ascending).
+                const double rmin = *std::min_element(radius.begin(), radius.end());
+                std::vector<double> rr(N + M), wr(N + M);
+                for (size_t i = 0; i < N + M; ++i) {
+                    const size_t f = i - N;
+                    rr[i] = (i < N ? radius[i] : (f < N ? radius[f] : rmin)) / length_scale;
+                    wr[i] = (i < N ? radius[i] : (f < N ? radius[f] : radius[0])) / length_scale;
+                }
+                mpm_op_desc od;
+                std::memset(&od, 0, sizeof(od));
+                for (int d = 0; d < 3; ++d) od.box[d] = box[d] / length_scale;
+                od.errortol = errortol;
+                od.xi = xi;
+                op = std::make_unique<mpm::EwaldOp>(od, rr.data(), (int64_t)rr.size());
+                op->set_weight_radii(wr.data(), (int64_t)wr.size());
+                op_sites = N + M;
+            } else if (direct) {
                 // the particles' radii then unit radii for the field points (sites without dipoles:
                 // their radius enters only their own self term, which multiplies a zero dipole)
                 std::vector<double> rr(N + M, 1.0);
@@ -509,8 +529,9 @@ struct mpm_near_field {
                 op = std::make_unique<mpm::EwaldOp>(od);
             }
         }
-        MPM_REQUIRE(!direct || op_sites == N + M, MPM_ERR_RUNTIME,
-                    "Near_Field: the direct field keeps the particle and field-point counts of its first calculation");
+        MPM_REQUIRE(!(direct || poly) || op_sites == N + M, MPM_ERR_RUNTIME,
+                    "Near_Field: the direct and polydisperse fields keep the particle and field-point counts of "
+                    "their first calculation");
         const size_t T = N + M;
         std::vector<double> xyz(3 * T);
         for (size_t i = 0; i < T; ++i)
@@ -524,7 +545,7 @@ struct mpm_near_field {
         d_x.reserve(3 * T);
         d_y.reserve(3 * T);
         MPM_HIP(hipMemcpy(d_x.p, x.data(), 3 * T * sizeof(double2), hipMemcpyHostToDevice));
-        op->apply_contiguous(d_x.p, d_y.p, 1, false, MPM_PART_ALL, nullptr);
+        op->apply_contiguous(d_x.p, d_y.p, 1, precision != MPM_PREC_DOUBLE && !direct, MPM_PART_ALL, nullptr);
         MPM_HIP(hipGetLastError());
         std::vector<cplx> y(3 * T);
         MPM_HIP(hipMemcpy(y.data(), d_y.p, 3 * T * sizeof(double2), hipMemcpyDeviceToHost));
@@ -619,6 +640,14 @@ mpm_status mpm_near_field_set_field_points(mpm_near_field* nf, const double* x, 
         MPM_REQUIRE(nf && ((x && y && z) || n == 0) && n >= 0, MPM_ERR_INVALID_ARGUMENT, "null argument");
         const double* c[3] = {x, y, z};
         for (int d = 0; d < 3; ++d) nf->fpos[d].assign(c[d], c[d] + n);
+    });
+}
+
+mpm_status mpm_near_field_set_precision(mpm_near_field* nf, int precision) {
+    return guarded([&] {
+        MPM_REQUIRE(nf, MPM_ERR_INVALID_ARGUMENT, "null argument");
+        MPM_REQUIRE(precision >= 0 && precision <= 2, MPM_ERR_INVALID_ARGUMENT, "unknown precision");
+        nf->precision = precision;
     });
 }
 

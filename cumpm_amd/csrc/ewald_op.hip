@@ -4105,7 +4105,7 @@ void EwaldOp::setup_poly_stencils(const std::vector<double>& pos, int Nn) {
                     const size_t item = (size_t)i * SS + o;
                     const int v = (w[0] * S.n[1] + w[1]) * S.n[2] + w[2];
                     pidx[item] = v;
-                    poly_weight(gd, rad_h[i], xi, peta, &pw[3 * item]);
+                    poly_weight(gd, wrad_h.empty() ? rad_h[i] : wrad_h[i], xi, peta, &pw[3 * item]);
                     cnt[v + 1]++;
                 }
     }
@@ -4125,6 +4125,15 @@ void EwaldOp::setup_poly_stencils(const std::vector<double>& pos, int Nn) {
     d_vox_items.upload(items.data(), items.size());
     d_selfp.upload(sp.data(), sp.size());
     ++pst_serial;
+}
+
+void EwaldOp::set_weight_radii(const double* r, int64_t n) {
+    MPM_REQUIRE(poly, MPM_ERR_UNSUPPORTED, "weight radii belong to the polydisperse operator");
+    MPM_REQUIRE(r && n == (int64_t)rad_h.size(), MPM_ERR_INVALID_ARGUMENT, "one weight radius per site");
+    for (int64_t i = 0; i < n; ++i)
+        MPM_REQUIRE(std::isfinite(r[i]) && r[i] > 0.0, MPM_ERR_INVALID_ARGUMENT, "radii must be positive");
+    wrad_h.assign(r, r + n);
+    last_xyz.clear();  // the next set_positions rebuilds the stencils
 }
 
 // d/da_i of the pair values of particles i, j at distance r (>= 1e-6), from the closed
@@ -5234,6 +5243,7 @@ void EwaldOp::launch_poly(const SlotTable& t, int B, int parts, hipStream_t s, i
 void EwaldOp::radius_gradient(const double2* x, const double2* g, int B, double* out, bool accumulate,
                               hipStream_t s) {
     MPM_REQUIRE(poly, MPM_ERR_UNSUPPORTED, "radius gradients need the polydisperse operator");
+    MPM_REQUIRE(wrad_h.empty(), MPM_ERR_UNSUPPORTED, "radius gradients need one radius per particle");
     MPM_REQUIRE(N > 0, MPM_ERR_RUNTIME, "radius gradient before set_positions");
     MPM_REQUIRE(kb[0] == 0.0 && kb[1] == 0.0, MPM_ERR_UNSUPPORTED,
                 "radius gradients are provided without a Bloch wave vector");

@@ -202,6 +202,9 @@ class EwaldOp {
     // term, finite-sphere spread/contract weights; not the self coefficients) with respect
     // to particle i's radius.  accumulate: add to out instead of overwriting it.
     void radius_gradient(const double2* x, const double2* g, int B, double* out, bool accumulate, hipStream_t s);
+    // polydisperse operator: spread/contract weight radii that differ from the pair-function
+    // radii (one per site; set before set_positions)
+    void set_weight_radii(const double* r, int64_t n);
     // conjugates the device self coefficients in place (the adjoint operator's)
     void conj_self_coef(hipStream_t s);
     // Device memory a caller lends to an apply for its reciprocal grids (base 256-byte
@@ -394,6 +397,9 @@ class EwaldOp {
     void poly_pair_value(double r, int i, int j, double& perp, double& para) const;
     void setup_poly_stencils(const std::vector<double>& pos, int Nn);
     // radius derivatives of the current frame, built on the first radius_gradient call
+    // spread/contract weight radii when they differ from the pair-function radii (the near
+    // field's field points, whose two radii follow different rules); empty: rad_h
+    std::vector<double> wrad_h;
     std::vector<int> pst_g;          // [N][3] nearest grid point of each particle
     std::vector<double> pst_dd;      // [N][3] its offset from the particle
     int64_t pst_serial = 0, rg_serial = -1;

@@ -2,7 +2,8 @@
 
 Same interface as the reference's ``cuMPM.Near_Field`` (src/cuMPM/near_field.py:5-126) over
 the C ABI (``mpm_near_field_*``); the field is the periodic Ewald field of the operator, with
-the field points taking part as sites that carry no dipole.
+the field points taking part as sites that carry no dipole.  Addition: ``precision``
+("auto" / None as the reference's default FP32 reciprocal grid, "mixed", or "double").
 """
 import ctypes
 
@@ -24,10 +25,10 @@ def _coords(arr, what):
 
 class Near_Field:
     r"""Local field intensity :math:`\lvert \mathbf{E}_0 + \mathbf{E}_{\text{ind}} \rvert^2` at target
-    points from complex point dipoles under periodic boundary conditions (monodisperse field)."""
+    points from complex point dipoles under periodic boundary conditions."""
 
     def __init__(self, box, E0, radius=1.0, dip=None, dip_pos=None, field_points=None, xi=0.5, errortol=1e-3,
-                 field_type="auto"):
+                 field_type="auto", *, precision=None):
         self.box = [float(x) for x in box]
         self.E0 = [complex(x) for x in E0]
         if len(self.box) != 3:
@@ -48,6 +49,10 @@ class Near_Field:
                                              self.xi, self.errortol, _FIELD[field_type], ctypes.byref(h)))
         self._h = h
         self._M = 0
+        prec = {None: C.PREC_AUTO, "auto": C.PREC_AUTO, "mixed": C.PREC_MIXED, "double": C.PREC_DOUBLE}
+        if precision not in prec:
+            raise ValueError(f"precision must be None, 'auto', 'mixed' or 'double', got {precision!r}")
+        check(C.load().mpm_near_field_set_precision(h, prec[precision]))
         if dip is not None:
             self.set_dipoles(dip)
         if dip_pos is not None:

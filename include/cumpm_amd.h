@@ -430,9 +430,9 @@ mpm_status mpm_eels_solver_get_timing(const mpm_eels_solver* s, double* solve_ms
 /* ---- Near field (src/cpp/near_field.h:12-57, near_field_bindings.cpp:8-25) ---- */
 typedef struct mpm_near_field mpm_near_field;
 /* Near_Field(box, E0, radius, xi, errortol, field_type): box physical [3], E0 host complex [3],
- * radius host [num_radii] (0 => 1.0; lengths are scaled by the first radius), field_type one of
- * mpm_field_type; this library evaluates the monodisperse field only: AUTO with equal radii, or
- * MONODISPERSE. */
+ * radius host [num_radii] (0 => 1.0; lengths are scaled by the first radius), field_type a value
+ * of mpm_field_type: AUTO selects the polydisperse field when the radii differ and DIRECT the
+ * open-boundary field (near_field.cu:105-147). */
 mpm_status mpm_near_field_create(const double box[3], const double* E0, const double* radius, int64_t num_radii,
                                  double xi, double errortol, int field_type, mpm_near_field** out);
 void mpm_near_field_destroy(mpm_near_field* nf);
@@ -442,9 +442,16 @@ mpm_status mpm_near_field_set_dipole_positions(mpm_near_field* nf, const double*
                                                int64_t n);
 mpm_status mpm_near_field_set_field_points(mpm_near_field* nf, const double* x, const double* y, const double* z,
                                            int64_t n);
+/* Reciprocal-space precision of calculate (mpm_precision): AUTO (the default) and MIXED use an
+ * FP32 grid as the reference's fields do by default; DOUBLE runs FP64 throughout.  No
+ * counterpart in the reference's Near_Field, whose fields take their default. */
+mpm_status mpm_near_field_set_precision(mpm_near_field* nf, int precision);
 /* calculate (near_field.cu:105-213): intensity[f] = |E0 + E_ind(f)|^2 at every field point, E_ind
- * the periodic Ewald field of the dipoles (volume-averaged over a unit sphere at f, the pair
- * functions of the solver; FP64 throughout, where the reference defaults to an FP32 grid). */
+ * the periodic Ewald field of the dipoles: the field points join the operator as sites without
+ * dipoles (volume-averaged over a sphere at f, the pair functions of the solver).  With the
+ * polydisperse field, field point f takes particle f's radius for f < N; beyond, the first
+ * radius for its spread weight and the smallest for its pair functions, as the reference's
+ * INTERACTION_FIELD kernels index them (polydisperse_ewald_electric_field.cu:297, 402). */
 mpm_status mpm_near_field_calculate(mpm_near_field* nf, double* intensity);
 
 /* ---- Plumbing ---- */

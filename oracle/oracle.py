@@ -251,17 +251,36 @@ def self_coefficient(eps, radius=1.0):
     return -3.0 / (4.0 * np.pi * diff) / np.asarray(radius, dtype=np.float64) ** 3
 
 
-def near_field_intensity(box, E0, radius, dip, dip_pos, field_points, xi=0.5, errortol=1e-3):
-    """|E0 + E_ind|^2 at the field points (src/cpp/near_field.cu:105-213, monodisperse):
-    lengths over the first radius; E_ind(f) = -(row f of A [p; 0]) with the field points as
-    sites without dipoles and zero self coefficients, i.e. the Gaussian contraction of the
-    reciprocal grid at f plus the pair terms of every particle within rc at d_eff = max(d, 1)
-    (the INTERACTION_FIELD evaluation, monodisperse_ewald_electric_field.cu:1631-1869,
-    2385-2411, with its final negation)."""
-    a = float(np.atleast_1d(radius)[0])
+def near_field_intensity(box, E0, radius, dip, dip_pos, field_points, xi=0.5, errortol=1e-3, field_type="auto"):
+    """|E0 + E_ind|^2 at the field points (src/cpp/near_field.cu:105-213): lengths over the
+    first radius; E_ind(f) = -(row f of A [p; 0]) with the field points as sites without dipoles
+    and zero self coefficients, i.e. the contraction of the reciprocal grid at f plus the pair
+    terms of every particle within rc (the INTERACTION_FIELD evaluation,
+    monodisperse_ewald_electric_field.cu:1631-1869, 2385-2411, with its final negation).
+    Polydisperse (field_type "polydisperse", or "auto" with unequal radii,
+    near_field.cu:121-147): field point f is a site of particle f's radius for f < N; beyond,
+    the reference's kernels take the first radius for its spread weight
+    (polydisperse_ewald_electric_field.cu:297) and the smallest radius class for its pair
+    functions (:402); this restatement covers inputs where the two agree (M <= N, or the first
+    radius the smallest) and raises otherwise."""
+    r = np.atleast_1d(np.asarray(radius, np.float64))
+    a = float(r[0])
     dip = np.asarray(dip, np.complex128).reshape(-1, 3)
     N = dip.shape[0]
     fp = np.asarray(field_points, np.float64).reshape(-1, 3)
+    M = fp.shape[0]
+    if r.size == 1:
+        r = np.full(N, a)
+    poly = field_type == "polydisperse" or (field_type == "auto" and np.any(r != r[0]))
+    if poly:
+        if M > N and r[0] != r.min():
+            raise NotImplementedError("field points beyond the N-th take two different radii in the reference")
+        fr = np.array([r[f] if f < N else r.min() for f in range(M)])
+        op = PolyOperator(np.asarray(box, np.float64) / a, errortol, xi, np.concatenate([r, fr]) / a)
+        op.set_positions(np.concatenate([np.asarray(dip_pos, np.float64).reshape(-1, 3), fp]) / a)
+        x = np.concatenate([dip, np.zeros_like(fp, dtype=np.complex128)])
+        y = op.apply(x, 0.0)[N:]
+        return np.sum(np.abs(np.asarray(E0, np.complex128)[None, :] - y) ** 2, axis=1)
     op = Operator(np.asarray(box, np.float64) / a, errortol, xi)
     op.set_positions(np.concatenate([np.asarray(dip_pos, np.float64).reshape(-1, 3), fp]) / a)
     x = np.concatenate([dip, np.zeros_like(fp, dtype=np.complex128)])

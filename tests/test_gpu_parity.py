@@ -798,10 +798,15 @@ def test_near_field_golden_fixture_and_basic():
     dip_pos = np.array([[0.0, 0, 0], [10.0, 0, 0], [0, 10.0, 0], [0, 0, 10.0]])
     dip = np.array([[0.1 + 0.05j, 0, 0], [-0.1, 0.02j, 0], [0, 0.05, -0.01j], [0.01, 0, 0.1j]])
     fp = np.array([[5.0, 0, 0], [0, 5.0, 0], [2.5, 2.5, 2.5]])
-    nf = cumpm_amd.Near_Field(box, [1.0, 0.0, 0.0], radius=1.0, dip=dip, dip_pos=dip_pos, field_points=fp)
+    nf = cumpm_amd.Near_Field(box, [1.0, 0.0, 0.0], radius=1.0, dip=dip, dip_pos=dip_pos, field_points=fp,
+                              precision="double")
     I = nf.calculate()
     np.testing.assert_allclose(I, ref, rtol=1e-4, atol=1e-4)
     np.testing.assert_allclose(I, O.near_field_intensity(box, [1.0, 0, 0], 1.0, dip, dip_pos, fp), rtol=1e-10)
+    # the reference's default: an FP32 reciprocal grid
+    I32 = cumpm_amd.Near_Field(box, [1.0, 0.0, 0.0], radius=1.0, dip=dip, dip_pos=dip_pos, field_points=fp).calculate()
+    np.testing.assert_allclose(I32, ref, rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(I32, I, rtol=1e-5)
     nf2 = cumpm_amd.Near_Field(box, [1.0, 0.0, 0.0], radius=1.0, dip=[[0.1, 0, 0], [-0.1, 0, 0]],
                                dip_pos=[[0.0, 0, 0], [10.0, 0, 0]], field_points=[[5.0, 0, 0], [0, 5.0, 0]])
     I2 = nf2.calculate()
@@ -821,7 +826,7 @@ def test_near_field_matches_oracle_with_overlap_and_radius():
     fp = np.concatenate([dip_pos[:3] + [[0.5, 0.3, 0.2], [2.0, 1.5, 0.0], [0.0, 0.0, 0.0]],
                          rng.uniform(0, 55, (6, 3))])
     nf = cumpm_amd.Near_Field(box, [0.0, 0.0, 0.0], radius=2.5, dip=dip, dip_pos=dip_pos, field_points=fp,
-                              errortol=1e-8)
+                              errortol=1e-8, precision="double")
     I = nf.calculate()
     ref = O.near_field_intensity(box, [0.0, 0, 0], 2.5, dip, dip_pos, fp, errortol=1e-8)
     assert rel(I, ref) < 1e-10
@@ -829,9 +834,55 @@ def test_near_field_matches_oracle_with_overlap_and_radius():
         cumpm_amd.Near_Field(box, [1.0, 0, 0], dip=dip, dip_pos=dip_pos).calculate()
     with pytest.raises(RuntimeError, match="inconsistent"):
         cumpm_amd.Near_Field(box, [1.0, 0, 0], dip=dip, dip_pos=dip_pos[:5], field_points=fp).calculate()
-    with pytest.raises(NotImplementedError):
-        cumpm_amd.Near_Field(box, [1.0, 0, 0], radius=np.linspace(1, 2, 12), dip=dip, dip_pos=dip_pos,
-                             field_points=fp).calculate()
+    with pytest.raises(ValueError):
+        cumpm_amd.Near_Field(box, [1.0, 0, 0], dip=dip, dip_pos=dip_pos, field_points=fp, precision="single")
+
+
+@pytest.mark.parametrize("first_smallest", [True, False])
+def test_near_field_polydisperse_matches_oracle(first_smallest):
+    """Polydisperse Near_Field (near_field.cu:121-147; field points as sites of the reference's
+    radii: particle f's radius for f < N, beyond it the first radius for the spread weight and the
+    smallest for the pair functions) against the oracle at FP64 (1e-10) and the default FP32 grid
+    (1e-5).  With the first radius not the smallest, the oracle covers the field points f < N."""
+    import cumpm_amd
+    _case(f"near field polydisperse first_smallest={first_smallest}")
+    rng = np.random.default_rng(14)
+    box = np.array([40.0, 41.0, 42.0])
+    N = 10
+    dip_pos = rng.uniform(0, 38, (N, 3))
+    dip = rng.standard_normal((N, 3)) + 1j * rng.standard_normal((N, 3))
+    r = rng.uniform(1.0, 2.0, N)
+    if first_smallest:
+        r[0] = 0.9
+    M = 14 if first_smallest else 8
+    fp = rng.uniform(0, 38, (M, 3))
+    fp[1] = dip_pos[4] + [0.6, 0.2, 0.1]   # inside a sphere
+    fp[2] = dip_pos[5] + [2.2, 1.0, 0.0]   # near one
+    nf = cumpm_amd.Near_Field(box, [0.3, 0.0, 0.1j], radius=r, dip=dip, dip_pos=dip_pos, field_points=fp,
+                              errortol=1e-8, precision="double")
+    I = nf.calculate()
+    ref = O.near_field_intensity(box, [0.3, 0.0, 0.1j], r, dip, dip_pos, fp, errortol=1e-8)
+    assert rel(I, ref) < 1e-10
+    I32 = cumpm_amd.Near_Field(box, [0.3, 0.0, 0.1j], radius=r, dip=dip, dip_pos=dip_pos, field_points=fp,
+                               errortol=1e-8).calculate()
+    assert rel(I32, I) < 1e-5
+    # forced polydisperse with equal radii: the monodisperse field to the Ewald tolerance at the
+    # field points outside every sphere (inside one, the monodisperse tables clamp r < 1 to r = 1
+    # where the polydisperse ones interpolate from the r -> 0 limit)
+    Ip = cumpm_amd.Near_Field(box, [0.3, 0.0, 0.1j], radius=np.ones(N), dip=dip, dip_pos=dip_pos, field_points=fp,
+                              errortol=1e-8, field_type="polydisperse", precision="double").calculate()
+    Im = cumpm_amd.Near_Field(box, [0.3, 0.0, 0.1j], radius=1.0, dip=dip, dip_pos=dip_pos, field_points=fp,
+                              errortol=1e-8, precision="double").calculate()
+    dmin = np.array([np.min(np.linalg.norm(dip_pos - f, axis=1)) for f in fp])
+    out = dmin > 2.0
+    assert out.sum() >= M - 2
+    assert rel(Ip[out], Im[out]) < 1e-5
+    if not first_smallest:
+        # two radii for the field points beyond the N-th: runs, finite
+        more = cumpm_amd.Near_Field(box, [0.3, 0.0, 0.1j], radius=r, dip=dip, dip_pos=dip_pos,
+                                    field_points=np.concatenate([fp, fp + 0.5]), errortol=1e-8).calculate()
+        assert more.shape == (2 * M,) and np.all(np.isfinite(more))
+        np.testing.assert_allclose(more[:M], I32, rtol=1e-5)
 
 
 @pytest.mark.parametrize("bloch", [(0.0, 0.0), (0.12, -0.2)])

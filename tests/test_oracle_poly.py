@@ -171,3 +171,29 @@ def test_total_operator_independent_of_xi_continuous_radii():
         op.set_positions(pos)
         out.append(op.apply(p, sc))
     assert np.abs(out[0] - out[1]).max() / np.abs(out[0]).max() < 1e-8
+
+
+def test_polydisperse_near_field_reduces_to_monodisperse_and_is_xi_free():
+    """The polydisperse near field (field points as sites of the reference's radii) with equal
+    radii equals the monodisperse near field, and with unequal radii it does not depend on the
+    Ewald splitting (both to the Ewald tolerance)."""
+    rng = np.random.default_rng(31)
+    box = np.array([22.0, 23.0, 24.0])
+    N, M = 8, 5
+    pos = rng.uniform(0, 20, (N, 3))
+    dip = rng.standard_normal((N, 3)) + 1j * rng.standard_normal((N, 3))
+    fp = rng.uniform(0, 20, (M, 3))
+    fp[0] = pos[2] + [1.3, 0.4, 0.0]  # overlapping the particle's sphere
+    E0 = [0.0, 0.0, 0.0]
+    mono = O.near_field_intensity(box, E0, 1.0, dip, pos, fp, errortol=1e-8)
+    poly = O.near_field_intensity(box, E0, np.ones(N), dip, pos, fp, errortol=1e-8, field_type="polydisperse")
+    np.testing.assert_allclose(poly, mono, rtol=2e-6)
+    r = rng.uniform(0.9, 1.3, N)
+    r[0] = r.min() - 0.05
+    a = O.near_field_intensity(box, E0, r, dip, pos, fp, xi=0.45, errortol=1e-9)
+    b = O.near_field_intensity(box, E0, r, dip, pos, fp, xi=0.6, errortol=1e-9)
+    np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-7 * np.abs(b).max())
+    with pytest.raises(NotImplementedError):
+        r2 = r.copy()
+        r2[0] = r.max() + 0.1
+        O.near_field_intensity(box, E0, r2, dip, pos, np.concatenate([fp, fp + 0.5]), errortol=1e-6)
