@@ -274,10 +274,22 @@ def cpu_baseline(pos, box, a, eps, iters_needed, budget_s=20.0):
     t_solve = time.perf_counter() - t0
     per_matvec = t_solve / max(rep["matvecs"] + 1, 1)  # +1: the diagnostic final residual
     full = per_matvec * (iters_needed + 2)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {
         "value": 1.0 / full,
         "unit": "RHS-solves/s",
         "cores": O.num_threads(),
+        "cpu_model": model,
+        "nproc": os.cpu_count(),
+        "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
         "kind": "port",
         "sample": (f"oracle FP64 GMRES(50)+Jacobi on 1 RHS (omega index {w}, x-pol) capped at {steps} "
                    f"Arnoldi steps ({t_solve:.1f} s, {per_matvec * 1e3:.0f} ms per operator apply incl. "

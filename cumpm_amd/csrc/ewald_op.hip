@@ -1,27 +1,34 @@
-// Monodisperse spectral-Ewald dipole operator y = A x for a batch of
-// right-hand sides on gfx950.
+// Spectral-Ewald field operator y = A x for a batch of right-hand sides on gfx950:
+// monodisperse dipoles, monodisperse quadrupoles, the polydisperse (finite-sphere)
+// operator and the direct open-boundary field.
 //
-// A = diag(self_coef_b + self_perp) + G_real + G_recip, the operator of
-// src/cpp/electric_field/monodisperse_ewald_electric_field.cu (SOLVER_AX mode,
-// dipoles).  The arithmetic follows the reference; the data movement does not:
+// A = diag(self_coef_b + self term) + G_real + G_recip, the operator of
+// src/cpp/electric_field/monodisperse_ewald_electric_field.cu (SOLVER_AX and
+// INTERACTION_FIELD modes), polydisperse_ewald_electric_field.cu and
+// direct_electric_field.cu.  The arithmetic follows the reference; the data movement
+// does not:
 //
-//  * spread  — owner-computes gather: a 256-thread workgroup owns a 4x8x8
-//              voxel tile and pulls in the particles binned around it (LDS
-//              staged), so every grid value is written exactly once, with no
-//              atomics and no separate zero-fill; the reference scatters with
-//              LDS/global atomics after a memset (:1160-1298, :2000-2048).
-//              Gaussian weights are separable, exp(-2 xi^2 sum_d gd_d^2/eta_d)
-//              = prod_d exp(-2 xi^2 gd_d^2/eta_d), so each particle carries
-//              3 x width weights instead of width^3.
-//  * FFT     — rocFFT, 3-D, in place, one plan per (precision, batch bucket),
-//              component-major grids [rhs][component][voxel].
-//  * scale   — k-space Green's tensor projection f <- sc (f.k^) k^ (:1406-1432)
-//              with (k^, sc) packed per voxel and reused across the batch.
-//  * gather  — one wavefront per particle fuses the Gaussian contraction
-//              (:1488-1525), the real-space neighbour sum (:1631-1869, written
-//              as a gather over a symmetric CSR list so it needs no atomics)
-//              and the self term (:1589-1629); a single wave reduction and a
-//              single store per particle and RHS.
+//  * layout  — thin films run on a slab of the occupied z-planes (the z transform is
+//              replaced by a Toeplitz coupling of those planes); other geometries,
+//              quadrupoles and the polydisperse operator on the full grid.
+//  * spread  — slab: region contractions on the matrix cores (spread_mfma_kernel,
+//              separable weights V^T W^T per region, every voxel written once); full
+//              grid: owner-computes tiles that pull in the binned particles (no atomics,
+//              no zero-fill), in place of the reference's atomic scatter (:1160-1298).
+//  * FFT     — the built-in batched LDS line FFT (register radices 2-23, a generic
+//              stage beyond), 2-D on the slab with a transposed k-space buffer and
+//              3-D on the full grid; rocFFT as an alternative back-end.
+//  * scale   — full grid: the Green's tensor projection per voxel (:1406-1432); slab:
+//              xtoeplitz_kernel fuses the x transform, the plane coupling and the
+//              inverse x transform (FP32), ztoeplitz / zfused kernels otherwise.
+//  * gather  — slab: matrix-core region contractions (contract_mfma_kernel) plus
+//              finish_kernel (slot sums, real-space neighbour sum over a symmetric CSR
+//              list, self term, Jacobi); full grid: a wavefront per particle fusing the
+//              contraction (:1488-1525), the real-space sum (:1631-1869) and the self
+//              term (:1589-1629); one store per particle and right-hand side.
+//  * precalc — per frame on the host (neighbour list and region rows on host threads):
+//              stencils, bins, regions, CSR neighbours with per-pair coefficients.
+
 #include <hip/hip_runtime.h>
 #include <rocfft/rocfft.h>
 
