@@ -30,6 +30,7 @@
 //              stencils, bins, regions, CSR neighbours with per-pair coefficients.
 
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 #include <rocfft/rocfft.h>
 
 #include <algorithm>
@@ -4046,6 +4047,7 @@ int64_t EwaldOp::pair_table(int which, double* r, double* v, int64_t cap) const 
 int64_t EwaldOp::neighbors(int64_t* start, int* j, double* perp, double* para, int64_t cap) const {
     MPM_REQUIRE(!direct, MPM_ERR_UNSUPPORTED, "the direct field has no neighbour list");
     MPM_REQUIRE(N > 0, MPM_ERR_RUNTIME, "particle coordinates have not been set");
+    fetch_pairs();
     if (cap >= num_pairs) {
         if (start)
             for (size_t i = 0; i < h_nb_start.size(); ++i) start[i] = h_nb_start[i];
@@ -4228,8 +4230,6 @@ void EwaldOp::setup_quad_stencils(const std::vector<double>& pos, int Nn) {
     const double Cs = std::pow(2.0 * xi * xi / kPi, 1.5) * std::sqrt(1.0 / prod_eta);
     std::vector<double> sw((size_t)Nn * SS);
     std::vector<int> pidx((size_t)Nn * SS), cnt(V + 1, 0);
-    pst_g.resize(3 * (size_t)Nn);
-    pst_dd.resize(3 * (size_t)Nn);
     for (int i = 0; i < Nn; ++i) {
         int g[3];
         double dd[3];
@@ -4237,8 +4237,6 @@ void EwaldOp::setup_quad_stencils(const std::vector<double>& pos, int Nn) {
             const double p = pos[(size_t)d * Nn + i];
             g[d] = (int)std::round(p / S.h[d]);
             dd[d] = g[d] * S.h[d] - p;
-            pst_g[3 * (size_t)i + d] = g[d];
-            pst_dd[3 * (size_t)i + d] = dd[d];
         }
         int o = 0;
         for (int ox = -half; ox <= half; ++ox)
@@ -4386,7 +4384,229 @@ void EwaldOp::set_bloch(double kx, double ky) {
     }
 }
 
+// ---- neighbour list and pair data on the device (monodisperse operator) ----
+//
+// The host keeps the O(N) cell lists (members in ascending index) and each particle's
+// cell; the device visits, for every particle i, the 27 cells around its own in the
+// host's order (dz, dy, dx = -1..1) and their members in order, so the CSR it builds is
+// the host's.  Pass 0 counts each particle's pairs, pass 1 fills them in: the neighbour,
+// x_j - x_i (minimum image), the table values (perp, para) at d_eff = max(d, 1) and, with
+// quadrupoles, the seven quadrupole table values (monodisperse_ewald_electric_field.cu
+// :877-958, neighbor_list.cu:242-420).
+struct NbGeom {
+    double L[3];
+    double rc, rc2;
+    int nc[3];
+    int cells;
+    int ts;  // table samples
+    int nq;  // quadrupole tables (0 or 7)
+};
+
+__device__ inline double tab_interp(const double* __restrict__ y, int ts, double x) {
+    // PairTable::interp on the samples 0, 1 + 0.001 k (the table's r array)
+    if (x <= 0.0) return y[0];
+    const double rlast = 1.0 + (ts - 2) * 0.001;
+    if (x >= rlast) return y[ts - 1];
+    if (x < 1.0) return y[0] * (1.0 - x) + y[1] * x;
+    const size_t idx = 1 + (size_t)((x - 1.0) / 0.001);
+    if (idx >= (size_t)ts - 1) return y[ts - 1];
+    const double r0 = 1.0 + (idx - 1) * 0.001, r1 = 1.0 + idx * 0.001;
+    const double t = (x - r0) / (r1 - r0);
+    return y[idx] * (1.0 - t) + y[idx + 1] * t;
+}
+
+__global__ __launch_bounds__(256) void nb_search_kernel(int pass, int N, const double* __restrict__ pos, NbGeom g,
+                                                        const int* __restrict__ cellc,
+                                                        const int* __restrict__ cell_start,
+                                                        const int* __restrict__ cell_items,
+                                                        const int* __restrict__ nb_start, int* __restrict__ count,
+                                                        int* __restrict__ nb_j, double* __restrict__ pair_r,
+                                                        double2* __restrict__ nb_f, double* __restrict__ pair_q,
+                                                        const double* __restrict__ tab) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double xi = pos[i], yi = pos[N + i], zi = pos[2 * N + i];
+    int k = pass ? nb_start[i] : 0;
+    int cnt = 0;
+    auto consider = [&](int j) {
+        double r[3] = {xi - pos[j], yi - pos[N + j], zi - pos[2 * N + j]};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) r[d] -= g.L[d] * round(r[d] / g.L[d]);
+        const double d2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        if (!(d2 <= g.rc2)) return;
+        const double dist = sqrt(d2);
+        if (!(dist < g.rc)) return;
+        ++cnt;
+        if (!pass) return;
+        nb_j[k] = j;
+        const double de = dist < 1.0 ? 1.0 : dist;
+        nb_f[k] = make_double2(tab_interp(tab, g.ts, de), tab_interp(tab + g.ts, g.ts, de));
+        for (int t = 0; t < g.nq; ++t) pair_q[7 * (size_t)k + t] = tab_interp(tab + (size_t)(2 + t) * g.ts, g.ts, de);
+        double rj[3] = {pos[j] - xi, pos[N + j] - yi, pos[2 * N + j] - zi};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            rj[d] -= g.L[d] * round(rj[d] / g.L[d]);
+            pair_r[3 * (size_t)k + d] = rj[d];
+        }
+        ++k;
+    };
+    if (g.cells) {
+        const int cx = cellc[3 * i], cy = cellc[3 * i + 1], cz = cellc[3 * i + 2];
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int a = (cx + dx + g.nc[0]) % g.nc[0], b = (cy + dy + g.nc[1]) % g.nc[1],
+                              c = (cz + dz + g.nc[2]) % g.nc[2];
+                    const int cell = (c * g.nc[1] + b) * g.nc[0] + a;
+                    for (int m = cell_start[cell]; m < cell_start[cell + 1]; ++m) {
+                        const int j = cell_items[m];
+                        if (j != i) consider(j);
+                    }
+                }
+    } else {
+        for (int j = 0; j < N; ++j)
+            if (j != i) consider(j);
+    }
+    if (!pass) count[i] = cnt;
+}
+
+// Per-pair coefficients (u = (x_j - x_i) / d_eff, perp, para - perp, the Bloch phase
+// e^{-i k_b.r}, and with quadrupoles the seven table values) from the pair data, the
+// device form of build_pair_coefs.
+__global__ __launch_bounds__(256) void pair_coefs_kernel(int64_t M, const double* __restrict__ pair_r,
+                                                         const double2* __restrict__ nb_f,
+                                                         const double* __restrict__ pair_q, int stride, double dmin,
+                                                         double kbx, double kby, double* __restrict__ out) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < M; k += (int64_t)gridDim.x * blockDim.x) {
+        const double* r = pair_r + 3 * k;
+        const double d = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        const double de = d < dmin ? dmin : d;
+        double* o = out + stride * k;
+        if (stride == 16)
+            for (int t = 0; t < 7; ++t) o[7 + t] = pair_q[7 * k + t];
+        o[0] = r[0] / de;
+        o[1] = r[1] / de;
+        o[2] = r[2] / de;
+        const double2 f = nb_f[k];
+        o[3] = f.x;
+        o[4] = f.y - f.x;
+        o[5] = 1.0;
+        o[6] = 0.0;
+        if (kbx != 0.0 || kby != 0.0) {
+            double sn, cs;
+            sincos(-(kbx * r[0] + kby * r[1]), &sn, &cs);
+            o[5] = cs;
+            o[6] = sn;
+        }
+    }
+}
+
+void EwaldOp::search_pairs_device(const std::vector<double>& pos, int Nn, bool cells, const int nc[3],
+                                  const std::vector<int>& cell_start, const std::vector<int>& cell_items,
+                                  const std::vector<int>& cellc) {
+    const PairTable& tab = *table;
+    const int ts = (int)tab.r.size();
+    const int nq = quads ? 7 : 0;
+    if (ptab_nt != 2 + nq || d_ptab.n != (size_t)(2 + nq) * ts) {
+        std::vector<double> t((size_t)(2 + nq) * ts);
+        std::copy(tab.perp.begin(), tab.perp.end(), t.begin());
+        std::copy(tab.para.begin(), tab.para.end(), t.begin() + ts);
+        for (int q = 0; q < nq; ++q) std::copy(tab.q[q].begin(), tab.q[q].end(), t.begin() + (size_t)(2 + q) * ts);
+        d_ptab.upload(t.data(), t.size());
+        ptab_nt = 2 + nq;
+    }
+    d_pos.upload(pos.data(), pos.size());
+    if (cells) {
+        d_cell_start.upload(cell_start.data(), cell_start.size());
+        d_cell_items.upload(cell_items.data(), std::max<size_t>(cell_items.size(), 1));
+        d_cellc.upload(cellc.data(), cellc.size());
+    }
+    NbGeom g;
+    for (int d = 0; d < 3; ++d) {
+        g.L[d] = L[d];
+        g.nc[d] = nc[d];
+    }
+    g.rc = S.rc;
+    g.rc2 = S.rc * S.rc;
+    g.cells = cells ? 1 : 0;
+    g.ts = ts;
+    g.nq = nq;
+    d_nb_count.resize((size_t)Nn + 1);
+    d_nb_start.resize((size_t)Nn + 1);
+    const dim3 grid((unsigned)((Nn + 255) / 256));
+    hipLaunchKernelGGL(nb_search_kernel, grid, dim3(256), 0, nullptr, 0, Nn, d_pos.p, g, d_cellc.p, d_cell_start.p,
+                       d_cell_items.p, (const int*)nullptr, d_nb_count.p, (int*)nullptr, (double*)nullptr,
+                       (double2*)nullptr, (double*)nullptr, d_ptab.p);
+    MPM_HIP(hipGetLastError());
+    MPM_HIP(hipMemsetAsync(d_nb_count.p + Nn, 0, sizeof(int), nullptr));
+    size_t tmp = 0;
+    MPM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_nb_count.p, d_nb_start.p, Nn + 1, nullptr));
+    d_scan_tmp.reserve(std::max<size_t>(tmp, 1));
+    MPM_HIP(hipcub::DeviceScan::ExclusiveSum(d_scan_tmp.p, tmp, d_nb_count.p, d_nb_start.p, Nn + 1, nullptr));
+    int total = 0;
+    MPM_HIP(hipMemcpy(&total, d_nb_start.p + Nn, sizeof(int), hipMemcpyDeviceToHost));
+    num_pairs = total;
+    const size_t M = std::max<size_t>((size_t)total, 1);
+    d_nb_j.resize(M);
+    d_pair_r.resize(3 * M);
+    d_nb_f.resize(M);
+    if (nq) d_pairq.resize(7 * M);
+    hipLaunchKernelGGL(nb_search_kernel, grid, dim3(256), 0, nullptr, 1, Nn, d_pos.p, g, d_cellc.p, d_cell_start.p,
+                       d_cell_items.p, (const int*)d_nb_start.p, (int*)nullptr, d_nb_j.p, d_pair_r.p, d_nb_f.p,
+                       nq ? d_pairq.p : (double*)nullptr, d_ptab.p);
+    MPM_HIP(hipGetLastError());
+    if (total == 0) {  // the kernels read one entry: a harmless zero pair
+        MPM_HIP(hipMemsetAsync(d_nb_j.p, 0, sizeof(int), nullptr));
+        MPM_HIP(hipMemsetAsync(d_nb_f.p, 0, sizeof(double2), nullptr));
+        MPM_HIP(hipMemsetAsync(d_pair_r.p, 0, 3 * sizeof(double), nullptr));
+    }
+    pairs_dev = true;
+    pairs_host_valid = false;
+    h_nb_start.clear();
+    h_nb_j.clear();
+    h_pair_r.clear();
+    h_pair_f.clear();
+    h_pair_q.clear();
+}
+
+// Host copies of the device-built neighbour list and pair data (getters, diagnostics).
+void EwaldOp::fetch_pairs() const {
+    if (!pairs_dev || pairs_host_valid) return;
+    const int Nn = (int)N;
+    const size_t M = (size_t)num_pairs;
+    h_nb_start.resize((size_t)Nn + 1);
+    h_nb_j.resize(M);
+    h_pair_r.resize(3 * M);
+    h_pair_f.resize(M);
+    MPM_HIP(hipMemcpy(h_nb_start.data(), d_nb_start.p, ((size_t)Nn + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    if (M) {
+        MPM_HIP(hipMemcpy(h_nb_j.data(), d_nb_j.p, M * sizeof(int), hipMemcpyDeviceToHost));
+        MPM_HIP(hipMemcpy(h_pair_r.data(), d_pair_r.p, 3 * M * sizeof(double), hipMemcpyDeviceToHost));
+        MPM_HIP(hipMemcpy(h_pair_f.data(), d_nb_f.p, M * sizeof(double2), hipMemcpyDeviceToHost));
+    }
+    if (quads) {
+        h_pair_q.resize(7 * M);
+        if (M) MPM_HIP(hipMemcpy(h_pair_q.data(), d_pairq.p, 7 * M * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    pairs_host_valid = true;
+}
+
 void EwaldOp::build_pair_coefs() {
+    if (pairs_dev) {
+        const int64_t M = num_pairs;
+        const int stride = quads ? 16 : 8;
+        d_pairc.resize((size_t)std::max<int64_t>(M, 1) * stride);
+        if (M > 0) {
+            const int blocks = (int)std::min<int64_t>((M + 255) / 256, 16384);
+            hipLaunchKernelGGL(pair_coefs_kernel, dim3(blocks), dim3(256), 0, nullptr, M, d_pair_r.p, d_nb_f.p,
+                               quads ? d_pairq.p : (const double*)nullptr, stride, 1.0, kb[0], kb[1], d_pairc.p);
+            MPM_HIP(hipGetLastError());
+        } else {
+            MPM_HIP(hipMemset(d_pairc.p, 0, (size_t)stride * sizeof(double)));
+        }
+        MPM_HIP(hipDeviceSynchronize());
+        return;
+    }
     const size_t M = h_pair_f.size();
     const int stride = (quads && h_pair_q.size() == 7 * M) ? 16 : 8;  // quadrupoles: + 7 table values
     std::vector<double> c(std::max<size_t>(M, 1) * stride, 0.0);
@@ -4462,24 +4682,26 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     std::vector<float> w32((size_t)Nn * 3 * W);
     const double prod_eta = S.eta[0] * S.eta[1] * S.eta[2];
     const double Cs = std::pow(2.0 * xi * xi / kPi, 1.5) * std::sqrt(1.0 / prod_eta);
-    for (int i = 0; i < Nn; ++i) {
-        for (int d = 0; d < 3; ++d) {
-            const double p = xyz[3 * (size_t)i + d];
-            pos[(size_t)d * Nn + i] = p;
-            const double gd = std::round(p / S.h[d]);
-            MPM_REQUIRE(std::fabs(gd) < 1e9, MPM_ERR_INVALID_ARGUMENT, "position too far from the box");
-            const long long gi = (long long)gd;
-            const double dd = gi * S.h[d] - p;
-            base[(size_t)d * Nn + i] = (int)(((gi % S.n[d]) + S.n[d]) % S.n[d]);
-            for (int o = -half; o <= half; ++o) {
-                const double x = dd + o * S.h[d];
-                double wv = std::exp(-2.0 * xi * xi * x * x / S.eta[d]);
-                if (d == 0) wv *= Cs;
-                w64[((size_t)i * 3 + d) * W + o + half] = wv;
-                w32[((size_t)i * 3 + d) * W + o + half] = (float)wv;
+    parallel_for(Nn, host_threads(Nn, 2048), [&](int i0, int i1, int) {
+        for (int i = i0; i < i1; ++i) {
+            for (int d = 0; d < 3; ++d) {
+                const double p = xyz[3 * (size_t)i + d];
+                pos[(size_t)d * Nn + i] = p;
+                const double gd = std::round(p / S.h[d]);
+                MPM_REQUIRE(std::fabs(gd) < 1e9, MPM_ERR_INVALID_ARGUMENT, "position too far from the box");
+                const long long gi = (long long)gd;
+                const double dd = gi * S.h[d] - p;
+                base[(size_t)d * Nn + i] = (int)(((gi % S.n[d]) + S.n[d]) % S.n[d]);
+                for (int o = -half; o <= half; ++o) {
+                    const double x = dd + o * S.h[d];
+                    double wv = std::exp(-2.0 * xi * xi * x * x / S.eta[d]);
+                    if (d == 0) wv *= Cs;
+                    w64[((size_t)i * 3 + d) * W + o + half] = wv;
+                    w32[((size_t)i * 3 + d) * W + o + half] = (float)wv;
+                }
             }
         }
-    }
+    });
     laps.lap("host_setup_weights");
     // ---- reciprocal layout: slab of occupied z-planes, or the full grid ----
     const int nz = S.n[2];
@@ -4633,98 +4855,120 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         int c = (int)std::floor(wv / (L[d] / nc[d]));
         return std::min(std::max(c, 0), nc[d] - 1);
     };
+    std::vector<int> cellc;  // each particle's cell (x, y, z)
     if (cells) {
         members.resize((size_t)nc[0] * nc[1] * nc[2]);
+        cellc.resize(3 * (size_t)Nn);
         for (int j = 0; j < Nn; ++j) {
             const int cx = cell_coord(pos[j], 0), cy = cell_coord(pos[Nn + j], 1), cz = cell_coord(pos[2 * Nn + j], 2);
             members[(size_t)(cz * nc[1] + cy) * nc[0] + cx].push_back(j);
+            cellc[3 * (size_t)j] = cx;
+            cellc[3 * (size_t)j + 1] = cy;
+            cellc[3 * (size_t)j + 2] = cz;
         }
     }
-    // particles split into contiguous chunks, one per host thread; each chunk's pairs are
-    // appended in particle order, so the list is the same for any thread count
-    struct Chunk {
-        std::vector<int> nj;
-        std::vector<double2> nf;
-        std::vector<double> pr, pq;
-    };
+    // The monodisperse operator searches its pairs on the device from the host's cell lists
+    // (the table values need no host work); the polydisperse operator's pair values come from
+    // closed forms at each pair's radii, evaluated on host threads.
+    const bool dev_pairs = !poly;
     std::vector<int> nstart(Nn + 1, 0);
-    auto consider = [&](Chunk& ch, int i, int j) {
-        std::vector<int>& nj = ch.nj;
-        std::vector<double2>& nf = ch.nf;
-        std::vector<double>& pr = ch.pr;
-        std::vector<double>& pq = ch.pq;
-        double r[3] = {pos[i] - pos[j], pos[Nn + i] - pos[Nn + j], pos[2 * Nn + i] - pos[2 * Nn + j]};
-        for (int d = 0; d < 3; ++d) r[d] -= L[d] * std::round(r[d] / L[d]);
-        const double d2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-        if (!(d2 <= rc2)) return;
-        const double dist = std::sqrt(d2);
-        if (!(dist < rc)) return;
-        nj.push_back(j);
-        if (poly) {  // d_eff = max(d, 1e-6) and the radius pair's table
-            const double de = dist < 1e-6 ? 1e-6 : dist;
-            double fp, fa;
-            poly_pair_value(de, i, j, fp, fa);
-            nf.push_back(make_double2(fp, fa));
-        } else {
-            const double de = dist < 1.0 ? 1.0 : dist;
-            nf.push_back(make_double2(tab.interp(tab.perp, de), tab.interp(tab.para, de)));
-            if (quads)
-                for (int t = 0; t < 7; ++t) pq.push_back(tab.interp(tab.q[t], de));
-        }
-        // x_j - x_i, the orientation the gather uses
-        double rj[3] = {pos[j] - pos[i], pos[Nn + j] - pos[Nn + i], pos[2 * Nn + j] - pos[2 * Nn + i]};
-        for (int d = 0; d < 3; ++d) rj[d] -= L[d] * std::round(rj[d] / L[d]);
-        pr.insert(pr.end(), rj, rj + 3);
-    };
-    const int nthreads = host_threads(Nn, 256);
-    std::vector<Chunk> chunks(nthreads);
-    parallel_for(Nn, nthreads, [&](int i0, int i1, int t) {
-        Chunk& ch = chunks[t];
-        const size_t guess = (size_t)(i1 - i0) * 64;
-        ch.nj.reserve(guess);
-        ch.nf.reserve(guess);
-        ch.pr.reserve(3 * guess);
-        for (int i = i0; i < i1; ++i) {
-            if (cells) {
-                const int cx = cell_coord(pos[i], 0), cy = cell_coord(pos[Nn + i], 1),
-                          cz = cell_coord(pos[2 * Nn + i], 2);
-                for (int dz = -1; dz <= 1; ++dz)
-                    for (int dy = -1; dy <= 1; ++dy)
-                        for (int dx = -1; dx <= 1; ++dx) {
-                            const int a = (cx + dx + nc[0]) % nc[0], b = (cy + dy + nc[1]) % nc[1],
-                                      c = (cz + dz + nc[2]) % nc[2];
-                            for (int j : members[(size_t)(c * nc[1] + b) * nc[0] + a])
-                                if (j != i) consider(ch, i, j);
-                        }
-            } else {
-                for (int j = 0; j < Nn; ++j)
-                    if (j != i) consider(ch, i, j);
-            }
-            nstart[i + 1] = (int)ch.nj.size();  // chunk-local; made global below
-        }
-    });
     std::vector<int> nj;
     std::vector<double2> nf;
     std::vector<double> pr, pq;
-    {
-        size_t tot = 0;
-        for (const Chunk& ch : chunks) tot += ch.nj.size();
-        nj.reserve(tot);
-        nf.reserve(tot);
-        pr.reserve(3 * tot);
-        for (int c = 0; c < nthreads; ++c) {
-            const int off = (int)nj.size();
-            for (int i = chunk_begin(Nn, nthreads, c); i < chunk_begin(Nn, nthreads, c + 1); ++i) nstart[i + 1] += off;
-            const Chunk& ch = chunks[c];
-            nj.insert(nj.end(), ch.nj.begin(), ch.nj.end());
-            nf.insert(nf.end(), ch.nf.begin(), ch.nf.end());
-            pr.insert(pr.end(), ch.pr.begin(), ch.pr.end());
-            pq.insert(pq.end(), ch.pq.begin(), ch.pq.end());
+    if (dev_pairs) {
+        std::vector<int> cs, ci;
+        if (cells) {
+            cs.assign(members.size() + 1, 0);
+            for (size_t c = 0; c < members.size(); ++c) cs[c + 1] = cs[c] + (int)members[c].size();
+            ci.reserve(Nn);
+            for (const auto& m : members) ci.insert(ci.end(), m.begin(), m.end());
         }
+        search_pairs_device(pos, Nn, cells, nc, cs, ci, cellc);
+    } else {
+        pairs_dev = false;
+        pairs_host_valid = true;
+        // particles split into contiguous chunks, one per host thread; each chunk's pairs are
+        // appended in particle order, so the list is the same for any thread count
+        struct Chunk {
+            std::vector<int> nj;
+            std::vector<double2> nf;
+            std::vector<double> pr, pq;
+        };
+        auto consider = [&](Chunk& ch, int i, int j) {
+            std::vector<int>& nj = ch.nj;
+            std::vector<double2>& nf = ch.nf;
+            std::vector<double>& pr = ch.pr;
+            std::vector<double>& pq = ch.pq;
+            double r[3] = {pos[i] - pos[j], pos[Nn + i] - pos[Nn + j], pos[2 * Nn + i] - pos[2 * Nn + j]};
+            for (int d = 0; d < 3; ++d) r[d] -= L[d] * std::round(r[d] / L[d]);
+            const double d2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+            if (!(d2 <= rc2)) return;
+            const double dist = std::sqrt(d2);
+            if (!(dist < rc)) return;
+            nj.push_back(j);
+            if (poly) {  // d_eff = max(d, 1e-6) and the radius pair's table
+                const double de = dist < 1e-6 ? 1e-6 : dist;
+                double fp, fa;
+                poly_pair_value(de, i, j, fp, fa);
+                nf.push_back(make_double2(fp, fa));
+            } else {
+                const double de = dist < 1.0 ? 1.0 : dist;
+                nf.push_back(make_double2(tab.interp(tab.perp, de), tab.interp(tab.para, de)));
+                if (quads)
+                    for (int t = 0; t < 7; ++t) pq.push_back(tab.interp(tab.q[t], de));
+            }
+            // x_j - x_i, the orientation the gather uses
+            double rj[3] = {pos[j] - pos[i], pos[Nn + j] - pos[Nn + i], pos[2 * Nn + j] - pos[2 * Nn + i]};
+            for (int d = 0; d < 3; ++d) rj[d] -= L[d] * std::round(rj[d] / L[d]);
+            pr.insert(pr.end(), rj, rj + 3);
+        };
+        const int nthreads = host_threads(Nn, 256);
+        std::vector<Chunk> chunks(nthreads);
+        parallel_for(Nn, nthreads, [&](int i0, int i1, int t) {
+            Chunk& ch = chunks[t];
+            const size_t guess = (size_t)(i1 - i0) * 64;
+            ch.nj.reserve(guess);
+            ch.nf.reserve(guess);
+            ch.pr.reserve(3 * guess);
+            for (int i = i0; i < i1; ++i) {
+                if (cells) {
+                    const int cx = cell_coord(pos[i], 0), cy = cell_coord(pos[Nn + i], 1),
+                              cz = cell_coord(pos[2 * Nn + i], 2);
+                    for (int dz = -1; dz <= 1; ++dz)
+                        for (int dy = -1; dy <= 1; ++dy)
+                            for (int dx = -1; dx <= 1; ++dx) {
+                                const int a = (cx + dx + nc[0]) % nc[0], b = (cy + dy + nc[1]) % nc[1],
+                                          c = (cz + dz + nc[2]) % nc[2];
+                                for (int j : members[(size_t)(c * nc[1] + b) * nc[0] + a])
+                                    if (j != i) consider(ch, i, j);
+                            }
+                } else {
+                    for (int j = 0; j < Nn; ++j)
+                        if (j != i) consider(ch, i, j);
+                }
+                nstart[i + 1] = (int)ch.nj.size();  // chunk-local; made global below
+            }
+        });
+        {
+            size_t tot = 0;
+            for (const Chunk& ch : chunks) tot += ch.nj.size();
+            nj.reserve(tot);
+            nf.reserve(tot);
+            pr.reserve(3 * tot);
+            for (int c = 0; c < nthreads; ++c) {
+                const int off = (int)nj.size();
+                for (int i = chunk_begin(Nn, nthreads, c); i < chunk_begin(Nn, nthreads, c + 1); ++i) nstart[i + 1] += off;
+                const Chunk& ch = chunks[c];
+                nj.insert(nj.end(), ch.nj.begin(), ch.nj.end());
+                nf.insert(nf.end(), ch.nf.begin(), ch.nf.end());
+                pr.insert(pr.end(), ch.pr.begin(), ch.pr.end());
+                pq.insert(pq.end(), ch.pq.begin(), ch.pq.end());
+            }
+        }
+        num_pairs = (int64_t)nj.size();
+        h_nb_start = nstart;
+        h_nb_j = nj;
     }
-    num_pairs = (int64_t)nj.size();
-    h_nb_start = nstart;
-    h_nb_j = nj;
     laps.lap("host_setup_neighbors");
     // visiting order of the wave-per-particle kernels: columns of 8 x 8 voxels in x and
     // y, ascending z within a column, so that consecutive waves share stencil voxels
@@ -4745,7 +4989,7 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key[a] < key[b]; });
         d_order.upload(ord.data(), ord.size());
     }
-    d_pos.upload(pos.data(), pos.size());
+    if (!dev_pairs) d_pos.upload(pos.data(), pos.size());
     d_base.upload(base.data(), base.size());
     d_w64.upload(w64.data(), w64.size());
     d_w32.upload(w32.data(), w32.size());
@@ -4756,17 +5000,19 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     d_zmap.upload(zmap.data(), zmap.size());
     if (zl.empty()) zl.push_back(0);
     d_zlist.upload(zl.data(), zl.size());
-    d_nb_start.upload(nstart.data(), nstart.size());
-    if (nj.empty()) {
-        nj.push_back(0);
-        nf.push_back(make_double2(0, 0));
+    if (!dev_pairs) {
+        d_nb_start.upload(nstart.data(), nstart.size());
+        if (nj.empty()) {
+            nj.push_back(0);
+            nf.push_back(make_double2(0, 0));
+        }
+        h_pair_r = std::move(pr);
+        h_pair_q = std::move(pq);
+        h_pair_f = nf;
+        h_pair_f.resize(num_pairs);
+        d_nb_j.upload(nj.data(), nj.size());
+        d_nb_f.upload(nf.data(), nf.size());
     }
-    h_pair_r = std::move(pr);
-    h_pair_q = std::move(pq);
-    h_pair_f = nf;
-    h_pair_f.resize(num_pairs);
-    d_nb_j.upload(nj.data(), nj.size());
-    d_nb_f.upload(nf.data(), nf.size());
     build_pair_coefs();
     // FFT plans for the chosen layout.  With the built-in transform the slab rows
     // are padded to whole 128-byte cache lines (16 FP32 / 8 FP64 complex; 16 for

@@ -301,11 +301,26 @@ class EwaldOp {
     // per pair, for the thread-per-(particle, RHS) finish: (ux, uy, uz, perp,
     // para - perp, cos, sin, 0) with u = r / max(|r|, 1), r = x_j - x_i (min
     // image) and the Bloch phase -(kb . r); rebuilt when positions or kb change
-    std::vector<int> h_nb_start, h_nb_j;  // neighbour list CSR of the current positions
+    mutable std::vector<int> h_nb_start, h_nb_j;  // neighbour list CSR of the current positions
     DevBuf<int> d_order;                  // [N] particles in spatial order (gather waves)
-    std::vector<double> h_pair_r;  // [M][3]
-    std::vector<double2> h_pair_f;
+    mutable std::vector<double> h_pair_r;  // [M][3]
+    mutable std::vector<double2> h_pair_f;
     DevBuf<double> d_pairc;        // [M][8]
+    // monodisperse operator: the neighbour list and pair data are built on the device
+    // (search_pairs_device); the host copies h_nb_start / h_nb_j / h_pair_* are then
+    // fetched only when asked for (fetch_pairs)
+    bool pairs_dev = false;
+    mutable bool pairs_host_valid = true;
+    DevBuf<double> d_pair_r;       // [M][3] x_j - x_i
+    DevBuf<double> d_pairq;        // [M][7] quadrupole table values
+    DevBuf<double> d_ptab;         // [2 + 7][ts] pair tables (perp, para, quadrupole) on the device
+    int ptab_nt = 0;
+    DevBuf<int> d_cell_start, d_cell_items, d_cellc, d_nb_count;
+    DevBuf<char> d_scan_tmp;
+    void search_pairs_device(const std::vector<double>& pos, int Nn, bool cells, const int nc[3],
+                             const std::vector<int>& cell_start, const std::vector<int>& cell_items,
+                             const std::vector<int>& cellc);
+    void fetch_pairs() const;
     void build_pair_coefs();
     DevBuf<double2> d_selfc;     // [rows][N]
     bool zcontig = false;        // slab planes form one cyclic run (zlist[a] = z0 + a mod nz)
@@ -355,7 +370,7 @@ class EwaldOp {
     std::vector<int> qreq_h;          // requested indices (empty with quads_all)
     std::vector<int> qidx_h;          // [Q] particle of each quadrupole
     DevBuf<int> d_qidx, d_qmap;       // [Q], [N] (quadrupole slot or -1)
-    std::vector<double> h_pair_q;     // [M][7] quadrupole table values per pair
+    mutable std::vector<double> h_pair_q;     // [M][7] quadrupole table values per pair
     DevBuf<double> d_sw64;            // [N][W^3] Gaussian weights (spread constant included)
     DevBuf<float> d_sw32;
     DevBuf<double> d_kq64;            // [V][16] k-space: khat, sc, scQ, scGP, scGQ, Qfactor[5], Qdot[5]
