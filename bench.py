@@ -93,7 +93,7 @@ def secondary_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def run_secondary(cumpm_amd, name, steps, solver_type, profile_only=False):
+def run_secondary(cumpm_amd, name, steps, solver_type, profile_only=False, autotune=False):
     """Batched refined sweep (timed over `steps` solves after one warm-up) beside the plain
     per-frequency FP64 sweep in reference order; one JSON line.  profile_only: the batched
     refined solves only (the run a profiler wraps)."""
@@ -103,7 +103,8 @@ def run_secondary(cumpm_amd, name, steps, solver_type, profile_only=False):
     s = None
     legs = [("batched_refined", "refined", True, steps), ("plain_fp64", "double", False, 1)]
     if profile_only:
-        s = cumpm_amd.dipole_solver(quiet=True, precision="refined", batched=True, solver_type=solver_type, **kw)
+        s = cumpm_amd.dipole_solver(quiet=True, precision="refined", batched=True, solver_type=solver_type,
+                                    autotune=autotune, **kw)
         for _ in range(steps + 1):
             s.reset()
             s.compute(pos)
@@ -112,9 +113,12 @@ def run_secondary(cumpm_amd, name, steps, solver_type, profile_only=False):
     for label, prec, batched, n in legs:
         s = None  # the previous solver's device memory is released before this one is measured
         base = _capi.memory_stats(reset_peak=True)[0]
-        s = cumpm_amd.dipole_solver(quiet=True, precision=prec, batched=batched, solver_type=solver_type, **kw)
+        s = cumpm_amd.dipole_solver(quiet=True, precision=prec, batched=batched, solver_type=solver_type,
+                                    autotune=autotune, **kw)
         s.compute(pos)
         peak = _capi.memory_stats()[1] - base
+        if autotune:
+            out[label + "_autotune"] = s.get_tune_info()
         ms = []
         for _ in range(n):
             s.reset()
@@ -337,6 +341,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 reference-order solve")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure roofline.traffic")
+    ap.add_argument("--tune-grid", action="store_true",
+                    help="secondary workloads: let the autotuner also grow the grid to 7-smooth lengths "
+                         "(a finer discretisation, within the Ewald tolerance)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default layout and FFT instead of timing the candidates on the first frame")
     ap.add_argument("--profile-only", action="store_true",
@@ -364,7 +371,8 @@ def main():
         copy_calibration(_capi)
         return
     if args.workload != "hex4000":
-        run_secondary(cumpm_amd, args.workload, max(args.steps, 1), args.solver, args.profile_only)
+        run_secondary(cumpm_amd, args.workload, max(args.steps, 1), args.solver, args.profile_only,
+                      "grid" if args.tune_grid else (not args.no_autotune))
         return
 
     pos, box, a, omega, eps = hex_monolayer_config()

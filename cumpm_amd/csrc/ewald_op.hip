@@ -5662,8 +5662,9 @@ void EwaldOp::configure(int layout, int backend, const int* grid) {
     MPM_REQUIRE(backend == 0 || backend == 1, MPM_ERR_INVALID_ARGUMENT, "fft_backend must be 0 or 1");
     const bool regrid = grid && (grid[0] != S.n[0] || grid[1] != S.n[1] || grid[2] != S.n[2]);
     if (poly || quads)
-        MPM_REQUIRE(layout == 1 && backend == 0 && !regrid, MPM_ERR_UNSUPPORTED,
+        MPM_REQUIRE(layout == 1 && backend == 0, MPM_ERR_UNSUPPORTED,
                     "the polydisperse and quadrupole operators run on the full grid with the built-in FFT");
+    MPM_REQUIRE(!(poly && regrid), MPM_ERR_UNSUPPORTED, "the polydisperse operator keeps its grid");
     if (regrid) {
         Sizing T;
         compute_sizing(L, errortol, xi, grid, S.P, T);  // same support, spacing from the new grid

@@ -96,14 +96,16 @@ void autotune(EwaldOp& op, int B, bool fp32, int flags, hipStream_t s, mpm_tune_
         cands.push_back(Cand{l, b, {g[0], g[1], g[2]}});
     };
     add(op.layout_setting(), op.backend_setting(), g0);
+    int gs[3];
+    for (int d = 0; d < 3; ++d) gs[d] = smooth_at_least(g0[d]);
     if (!op.is_poly() && !op.has_quads()) {
-        int gs[3];
-        for (int d = 0; d < 3; ++d) gs[d] = smooth_at_least(g0[d]);
         for (int l = 0; l < 2; ++l)
             for (int b = 0; b < 2; ++b) {
                 add(l, b, g0);
                 if (grow) add(l, b, gs);
             }
+    } else if (op.has_quads() && grow) {
+        add(1, 0, gs);  // quadrupoles: full grid and built-in FFT, on a grown grid
     }
     const size_t nu = op.n_unknowns();
     DevBuf<double2> x, y;

@@ -226,9 +226,10 @@ mpm_status mpm_op_radius_gradient(mpm_op* op, const double* x, const double* g, 
  * tolerance).  Each valid configuration is timed with HIP events over warm applies of
  * min(B, 32) right-hand sides in `precision`; the fastest stays configured on op.  Choices
  * are cached for the process per (grid, N, power-of-two batch bucket, precision, operator
- * kind, MPM_TUNE_GRID); MPM_TUNE_FORCE times again.  The polydisperse and quadrupole
- * operators have one configuration (reported, not timed away).  Needs positions set; the
- * direct field fails with MPM_ERR_UNSUPPORTED. */
+ * kind, MPM_TUNE_GRID); MPM_TUNE_FORCE times again.  The quadrupole operator runs on the
+ * full grid with the built-in FFT, so its only candidate besides the current one is the grown
+ * grid (MPM_TUNE_GRID); the polydisperse operator has one configuration.  Needs positions
+ * set; the direct field fails with MPM_ERR_UNSUPPORTED. */
 enum { MPM_TUNE_GRID = 1, MPM_TUNE_FORCE = 2 };
 typedef struct {
     int layout;         /* chosen mpm_op_desc.layout (0 automatic / slab, 1 full grid) */

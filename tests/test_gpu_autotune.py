@@ -131,6 +131,33 @@ def test_quadrupole_and_poly_operators_have_one_configuration():
     assert rel(p.apply(x), y0) < 1e-13
 
 
+def test_quadrupole_grid_growth_equals_operator_on_grown_grid_and_oracle():
+    # quadrupoles: the grown grid is their only other candidate (full grid, built-in FFT)
+    print("autotune quads grid growth", flush=True)
+    rng = np.random.default_rng(6)
+    L = 20.0
+    pos = rng.uniform(0, L, (30, 3))
+    q = Op([L, L, L], 1e-8, 0.5, grid=(43, 43, 43), quads=[0, 5, 7])
+    q.set_positions(pos)
+    rows = rng.standard_normal((2, 30)) + 1j * rng.standard_normal((2, 30))
+    q.set_self_coef(rows)
+    P = q.info().support
+    t = _tune(q, 2, C.PREC_DOUBLE, C.TUNE_GRID)
+    assert t["candidates"] == 2 and (t["layout"], t["fft_backend"]) == (1, 0)
+    g = t["grid"]
+    assert g in ([43] * 3, [45] * 3)
+    x = _rhs(rng, 2, q.n)
+    y = q.apply(x)
+    ref = Op([L, L, L], 1e-8, 0.5, grid=g, support=P, quads=[0, 5, 7])
+    ref.set_positions(pos)
+    ref.set_self_coef(rows)
+    assert rel(y, ref.apply(x)) < 1e-12
+    o = O.Operator([L, L, L], 1e-8, 0.5, grid=g, P=P)
+    o.set_positions(pos)
+    o.set_quads([0, 5, 7])
+    assert rel(y[0], o.apply(x[0], rows[0])) < 1e-11
+
+
 def test_dipole_solver_autotune_same_results():
     print("autotune dipole_solver", flush=True)
     pos, box = hex_monolayer(16, 12, 2.2, jitter=0.05, seed=5)
