@@ -199,6 +199,7 @@ def kernel_class(name):
     return {"spread_mfma_kernel": "spread", "spread_kernel": "spread", "line_fft2_kernel": "fft_axis",
             "line_fft_kernel": "fft_axis", "ztoeplitz_kernel": "zfused", "zfused_kernel": "zfused", "xtoeplitz_kernel": "xtoeplitz",
             "scale_kernel": "scale", "contract_mfma_kernel": "contract", "contract_nb_mfma_kernel": "contract",
+            "contract_full_mfma_kernel": "contract",
             "finish_kernel": "gather", "gather_kernel": "gather"}[base]
 
 

@@ -1900,6 +1900,11 @@ constexpr size_t kMaxFusedLds = 64 * 1024;  // LDS of one xtoeplitz_kernel block
 // Radius classes of the polydisperse operator up to which pair tables are built.
 constexpr int kPolyTableClasses = 64;
 constexpr int kMfmaKC = 64;    // candidates staged per chunk
+// Planes (y) per region on the full grid and candidate blocks of 16 per chunk of
+// its contraction: 4 x 8 x 16-voxel regions, about 28 candidates each in a
+// packing of volume fraction 0.3 with 9-point stencils on a 256^3 grid.
+constexpr int kFullRegionPlanes = 8;
+constexpr int kFullContractNB = 2;
 
 typedef float mpm_f32x4 __attribute__((ext_vector_type(4)));
 typedef double mpm_f64x4 __attribute__((ext_vector_type(4)));
@@ -1955,38 +1960,139 @@ struct RegionW {
     static constexpr int NW = Y + kMfmaRY;
 };
 
-// Copies the precomputed region-local weight rows of a chunk of cnt <= kMfmaKC
+// Copies the precomputed region-local weight rows of a chunk of cnt <= KC
 // candidates into LDS (zero rows past cnt) and ends with a barrier.
-template <typename R, int RX, int ZSB>
+// v where c holds, else +0, by a bit mask rather than a select: the compiler turns a
+// select of a loaded value into a branch around the load, which then waits for
+// the load at the branch's end.
+__device__ __forceinline__ float keep_if(float v, bool c) {
+    return __uint_as_float(__float_as_uint(v) & (c ? 0xffffffffu : 0u));
+}
+__device__ __forceinline__ double keep_if(double v, bool c) {
+    return __longlong_as_double(__double_as_longlong(v) & (c ? -1LL : 0LL));
+}
+
+// Blocks of 256 threads; every load is issued before the first store (one memory
+// latency per chunk, not one per element).
+template <typename R, int RX, int ZSB, int KC = kMfmaKC>
 __device__ inline void stage_weights(R* s_w, int cnt, const R* __restrict__ regw) {
     using L = RegionW<RX, ZSB>;
+    constexpr int T = (KC * L::NW + 255) / 256;
     const int n = cnt * L::NW;
-    for (int e = threadIdx.x; e < kMfmaKC * L::NW; e += blockDim.x) s_w[e] = e < n ? regw[e] : R(0);
+    R v[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int e = (int)threadIdx.x + 256 * t;
+        v[t] = e < n ? regw[e] : R(0);
+    }
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int e = (int)threadIdx.x + 256 * t;
+        if (e < KC * L::NW) s_w[e] = v[t];
+    }
     __syncthreads();
 }
 
+// Plane loop of the matrix-core contractions, software-pipelined over two operand
+// buffers: the loads of the next plane are in flight while the current plane's
+// products run.  load(p, buf) issues plane p's loads into buf, mma(buf, p)
+// consumes them; planes first, first + step, ... below end.  The loop body has no
+// branch between a load and its use (a branch would let the compiler sink the
+// loads next to their use): the last pair loads its second plane again, and an
+// odd plane count ends with one more product.  Scheduling barriers keep each group
+// of loads whole and ahead of the other buffer's products, so that the products
+// wait only for their own (older) loads.
+template <typename Buf, typename LoadF, typename MmaF>
+__device__ __forceinline__ void plane_pipeline(int first, int step, int end, Buf& ga, Buf& gb, LoadF&& load,
+                                               MmaF&& mma) {
+    if (first >= end) return;
+    const int n = (end - first + step - 1) / step;
+    const int last = first + (n - 1) * step;
+    load(first, ga);
+    int i = 0, p = first;
+    for (; i + 1 < n; i += 2, p += 2 * step) {
+        __builtin_amdgcn_sched_barrier(0);
+        load(p + step, gb);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(ga, p);
+        __builtin_amdgcn_sched_barrier(0);
+        load(min(p + 2 * step, last), ga);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(gb, p + step);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (i < n) mma(ga, p);
+}
+
+// Vector form of the contractions' grid operands: the four voxels a lane feeds into
+// the four products of a plane and x are rows 4 r + u (r = lane / 16, u = 0..3) of
+// its real column (even lane) or imaginary column (odd lane); the two lanes of a
+// complex column load rows (4 r, 4 r + 1) and (4 r + 2, 4 r + 3) as whole complex
+// pairs v = (re, im, re, im) and swap halves with a neighbour-lane move, so that one
+// 16-byte load per lane reads the 16 rows of eight complex components (one whole
+// 128-byte line each for FP32).  g receives rows 4 r + u of this lane's part.
+template <typename R, typename V4>
+__device__ __forceinline__ void pair_rows(const V4& v, int part, R (&g)[4]) {
+    const R s0 = part ? v[0] : v[1], s1 = part ? v[2] : v[3];
+    const R t0 = __shfl_xor(s0, 1, 64), t1 = __shfl_xor(s1, 1, 64);
+    g[0] = part ? t0 : v[0];
+    g[1] = part ? t1 : v[2];
+    g[2] = part ? v[1] : t0;
+    g[3] = part ? v[3] : t1;
+}
+
+// Where the regions sit in a grid buffer.  A region is RX consecutive x, up to
+// pblk consecutive planes and 16 consecutive rows, rows contiguous in memory:
+// in the slab layout the planes are the occupied z-planes (all in one region)
+// and the rows y; on the full grid the planes are y (blocks of pblk) and the
+// rows z.  Regions are numbered (x block, plane block, row block), row block
+// fastest.  Strides are in complex elements.
+struct RegionGeom {
+    int nx, nrow, nplane;  // extents along x, rows, planes
+    int nrr, npb, pblk;    // row blocks, plane blocks, planes per region
+    unsigned xs, ps, cs;   // strides of x, plane, component (b, c)
+};
+
+struct RegionAt {
+    int x0, plane0, row0, np;  // origin and number of planes present
+};
+
+template <int RX>
+__device__ inline RegionAt region_at(const RegionGeom& rg, int reg) {
+    const int per_x = rg.npb * rg.nrr;
+    const int rx = reg / per_x, rem = reg - rx * per_x;
+    const int pb = rem / rg.nrr, rr = rem - pb * rg.nrr;
+    RegionAt a;
+    a.x0 = rx * RX;
+    a.plane0 = pb * rg.pblk;
+    a.row0 = rr * kMfmaRY;
+    a.np = min(rg.pblk, rg.nplane - a.plane0);
+    return a;
+}
+
 template <typename R, int RX, int ZSB>
-__global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, SpreadGeom g,
+__global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, RegionGeom rg,
                                                           const R* __restrict__ regw,
                                                           const int* __restrict__ reg_start,
-                                                          const int* __restrict__ reg_items, int nry, int N,
+                                                          const int* __restrict__ reg_items, int N,
                                                           typename C2<R>::T* __restrict__ out) {
     using CT = typename C2<R>::T;
     using M = Mfma<R>;
     using Sh = SpreadShape<R, RX, ZSB>;
-    constexpr int RY = kMfmaRY, KC = kMfmaKC, CPB = Sh::CPB, ZH = Sh::ZH;
+    constexpr int KC = kMfmaKC, CPB = Sh::CPB, ZH = Sh::ZH;
     using L = RegionW<RX, ZSB>;
     __shared__ R s_v[KC][CPB + 1];
     __shared__ __align__(16) R s_w[KC][L::NW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int cg = wave % (CPB / 16), pg = wave / (CPB / 16);
     const int reg = blockIdx.x;
-    const int x0 = (reg / nry) * RX, y0 = (reg % nry) * RY;
+    const RegionAt ra = region_at<RX>(rg, reg);
+    const int x0 = ra.x0, y0 = ra.row0;
     const int col_base = blockIdx.y * CPB;
     const int wcol0 = col_base + cg * 16;
     const int ncols = 6 * B;
-    const int Zs = g.Zs;
-    const int p0 = pg * ZH;  // this wave's slab planes [p0, p0 + ZH)
+    const int Zs = ra.np;    // planes of this region
+    const int p0 = pg * ZH;  // this wave's planes [p0, p0 + ZH) of the region
     const int k0 = reg_start[reg], k1 = reg_start[reg + 1];
 
     typename M::V4 acc[RX][ZH];
@@ -1998,18 +2104,22 @@ __global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, 
     // staged columns of this thread: s_v[q][scol] for q = tid / CPB + (256 / CPB) t
     constexpr int QS = 256 / CPB;
     const int scol = tid % CPB, sqc = col_base + scol;
-    const double* xcol = nullptr;
-    if (sqc < ncols) xcol = reinterpret_cast<const double*>(tab.x[sqc / 6]) + 2 * ((sqc >> 1) % 3) + (sqc & 1);
+    // columns past the batch read column 0 and stage zeros
+    const bool scok = sqc < ncols;
+    const int sq = scok ? sqc : 0;
+    const double* xcol = reinterpret_cast<const double*>(tab.x[sq / 6]) + 2 * ((sq >> 1) % 3) + (sq & 1);
     for (int c0 = k0; c0 < k1; c0 += KC) {
         const int cnt = min(KC, k1 - c0);
         __syncthreads();
-        // values: s_v[k][col] = component (b, c), real or imaginary part, of candidate k
+        // values: s_v[k][col] = component (b, c), real or imaginary part, of candidate k;
+        // all candidate ids first, then all values (two memory latencies per chunk)
+        int id[KC / QS];
+#pragma unroll
+        for (int t = 0; t < KC / QS; ++t) id[t] = reg_items[c0 + min(tid / CPB + QS * t, cnt - 1)];
         R vv[KC / QS];
 #pragma unroll
-        for (int t = 0; t < KC / QS; ++t) {
-            const int q = tid / CPB + QS * t;
-            vv[t] = (xcol && q < cnt) ? (R)xcol[6 * (size_t)reg_items[c0 + q]] : R(0);
-        }
+        for (int t = 0; t < KC / QS; ++t)
+            vv[t] = keep_if((R)xcol[6 * (size_t)id[t]], scok && tid / CPB + QS * t < cnt);
         stage_weights<R, RX, ZSB>(&s_w[0][0], cnt, regw + (size_t)c0 * L::NW);
 #pragma unroll
         for (int t = 0; t < KC / QS; ++t) s_v[tid / CPB + QS * t][scol] = vv[t];
@@ -2035,27 +2145,26 @@ __global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, 
         }
     }
     // lane holds complex columns 2 (lane >> 4) + h of the wave (entries 2 h, 2 h + 1)
-    // at y = y0 + (lane & 15)
+    // at row y0 + (lane & 15)
     const int y = y0 + (lane & 15);
-    if (y >= g.ny || wcol0 >= ncols) return;
-    const size_t plane = (size_t)g.nx * g.nyp;
+    if (y >= rg.nrow || wcol0 >= ncols) return;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int qc = wcol0 + 4 * (lane >> 4) + 2 * h;
         if (qc >= ncols) continue;
         const int b = qc / 6, c = (qc >> 1) % 3;
-        CT* o = out + ((size_t)b * 3 + c) * Zs * plane + y;
+        CT* o = out + ((size_t)b * 3 + c) * rg.cs + (size_t)ra.plane0 * rg.ps + y;
 #pragma unroll
         for (int xi = 0; xi < RX; ++xi) {
             const int x = x0 + xi;
-            if (x >= g.nx) continue;
+            if (x >= rg.nx) continue;
 #pragma unroll
             for (int p = 0; p < ZH; ++p)
                 if (p0 + p < Zs) {
                     CT v;
                     v.x = acc[xi][p][2 * h];
                     v.y = acc[xi][p][2 * h + 1];
-                    o[(p0 + p) * plane + (size_t)x * g.nyp] = v;
+                    o[(size_t)(p0 + p) * rg.ps + (size_t)x * rg.xs] = v;
                 }
         }
     }
@@ -2073,86 +2182,131 @@ __global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, 
 // without barriers after the one weight-staging step.  D goes to the particle's
 // slot of the region (the CSR position of the pair); the per-particle sum over
 // its regions happens in the finish step, in a fixed order.
-template <typename R, int RX, int ZSB>
-__global__ __launch_bounds__(256, sizeof(R) == 4 ? 3 : 2) void contract_mfma_kernel(int B, SpreadGeom g,
+template <typename R, int RX, int ZSB, int NB, bool VEC>
+__global__ __launch_bounds__(256, sizeof(R) == 4 ? 3 : 2) void contract_mfma_kernel(int B, RegionGeom rg,
                                                                                     const R* __restrict__ regw,
                                                             const int* __restrict__ reg_start,
-                                                            const int* __restrict__ reg_items, int nry, int N,
+                                                            const int* __restrict__ reg_items, int N,
                                                             const typename C2<R>::T* __restrict__ grid,
                                                             R* __restrict__ partial, int ncg) {
     using M = Mfma<R>;
     using V4 = typename M::V4;
-    constexpr int RY = kMfmaRY, KC = kMfmaKC;
+    constexpr int KC = 16 * NB;  // candidates per chunk, in NB blocks of 16
     using L = RegionW<RX, ZSB>;
-    constexpr int NB = KC / 16;  // candidate blocks of 16
     __shared__ __align__(16) R s_w[KC][L::NW];
     __shared__ V4 s_red[4][NB][64];  // per-wave partial accumulators
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // 4 waves = ncg column groups x (4 / ncg) plane groups
     const int npg = 4 / ncg, cg = wave % ncg, pg = wave / ncg;
     const int reg = blockIdx.x;
-    const int x0 = (reg / nry) * RX, y0 = (reg % nry) * RY;
+    const RegionAt ra = region_at<RX>(rg, reg);
+    const int x0 = ra.x0, y0 = ra.row0;
     const int wcol0 = (blockIdx.y * ncg + cg) * 16;
     const int ncols = 6 * B;
     const bool active = wcol0 < ncols;
-    const int Zs = g.Zs;
-    const unsigned plane = (unsigned)(g.nx * g.nyp);
+    const int Zs = ra.np;  // planes of this region
+    const unsigned plane = rg.ps;
     const int k0 = reg_start[reg], k1 = reg_start[reg + 1];
 
     // this lane's B-operand column and its rows of the region, as 32-bit offsets
-    // (in R) from the uniform grid base: the grid holds fewer than 2^31 complex
+    // (in R) from the uniform grid base: the launch keeps 6 B x (complex per
+    // component) below 2^32
     const R* gf = reinterpret_cast<const R*>(grid);
     const int qc = wcol0 + (lane & 15);
     const bool cvalid = qc < ncols;
-    unsigned coff = 0, vmask = 0;  // column offset; bit 4 xi + u: (x0 + xi, y0 + 4 u + lane / 16) inside
+    // column offset at the region origin (cbase, an in-grid voxel of every plane of the
+    // region) and at this lane's row; bit 4 xi + u of vmask: (x0 + xi, y0 + 4 u + lane / 16)
+    // inside the grid.  VEC (pair_rows): the lane's complex pair at rows y0 + 4 r + 2 part
+    // and the next, bit xi of vmask: (x0 + xi, y0 + 4 r + 2 part) inside; even rows keep
+    // both rows of a pair inside together.
+    const int r = lane >> 4, part = lane & 1;
+    unsigned cbase = 0, coff = 0, vmask = 0;
     if (cvalid) {
         const int b = qc / 6, c = (qc >> 1) % 3;
-        coff = 2u * (unsigned)((b * 3 + c) * Zs) * plane + (unsigned)(qc & 1) +
-               2u * (unsigned)(x0 * g.nyp + y0 + (lane >> 4));
+        cbase = 2u * (unsigned)(b * 3 + c) * rg.cs + (VEC ? 0u : (unsigned)(qc & 1)) +
+                2u * ((unsigned)ra.plane0 * rg.ps + (unsigned)x0 * rg.xs + (unsigned)y0);
+        if (VEC) {
+            coff = cbase + 2u * (unsigned)(4 * r + 2 * part);
 #pragma unroll
-        for (int xi = 0; xi < RX; ++xi)
+            for (int xi = 0; xi < RX; ++xi)
+                if (x0 + xi < rg.nx && y0 + 4 * r + 2 * part < rg.nrow) vmask |= 1u << xi;
+        } else {
+            coff = cbase + 2u * (unsigned)(lane >> 4);
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (x0 + xi < g.nx && y0 + 4 * u + (lane >> 4) < g.ny) vmask |= 1u << (4 * xi + u);
+            for (int xi = 0; xi < RX; ++xi)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (x0 + xi < rg.nx && y0 + 4 * u + (lane >> 4) < rg.nrow) vmask |= 1u << (4 * xi + u);
+        }
     }
-    const unsigned xstep = 2u * (unsigned)g.nyp;
+    const unsigned xstep = 2u * rg.xs;
 
     for (int c0 = k0; c0 < k1; c0 += KC) {
         const int cnt = min(KC, k1 - c0);
         __syncthreads();
-        stage_weights<R, RX, ZSB>(&s_w[0][0], cnt, regw + (size_t)c0 * L::NW);
+        stage_weights<R, RX, ZSB, KC>(&s_w[0][0], cnt, regw + (size_t)c0 * L::NW);
         V4 acc[NB];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) acc[nb] = V4{R(0), R(0), R(0), R(0)};
         if (active) {
-            // per-lane weights: candidate k = 16 nb + (lane & 15) at y = 4 u + (lane >> 4), x = x0 + xi
+            // per-lane weights: candidate k = 16 nb + (lane & 15) at x = x0 + xi and rows
+            // y0 + 4 u + r (VEC: y0 + 4 r + u)
             R wyl[NB][4], wxl[NB][RX];
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) {
                 const int k = nb * 16 + (lane & 15);
 #pragma unroll
-                for (int u = 0; u < 4; ++u) wyl[nb][u] = s_w[k][L::Y + 4 * u + (lane >> 4)];
+                for (int u = 0; u < 4; ++u) wyl[nb][u] = s_w[k][L::Y + (VEC ? 4 * r + u : 4 * u + r)];
 #pragma unroll
                 for (int xi = 0; xi < RX; ++xi) wxl[nb][xi] = s_w[k][L::X + xi];
             }
+            if constexpr (VEC) {
+                struct PlaneBuf {
+                    V4 g[RX];
+                };
+                auto load_plane = [&](int p, PlaneBuf& dst) {
+                    const unsigned poff = coff + 2u * (unsigned)p * plane, pin = cbase + 2u * (unsigned)p * plane;
+#pragma unroll
+                    for (int xi = 0; xi < RX; ++xi) {
+                        const bool in = (vmask >> xi) & 1u;
+                        dst.g[xi] = *reinterpret_cast<const V4*>(gf + (in ? poff + xi * xstep : pin));
+                    }
+                };
+                auto mma = [&](PlaneBuf& pb, int p) {
+                    R wz[NB];
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) wz[nb] = s_w[nb * 16 + (lane & 15)][L::Z + p];
+#pragma unroll
+                    for (int xi = 0; xi < RX; ++xi) {
+                        R gv[4];
+                        pair_rows<R>(pb.g[xi], part, gv);
+#pragma unroll
+                        for (int nb = 0; nb < NB; ++nb) {
+                            const R wxz = wxl[nb][xi] * wz[nb];
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) acc[nb] = M::mac(wyl[nb][u] * wxz, gv[u], acc[nb]);
+                        }
+                    }
+                };
+                PlaneBuf ga, gb;
+                plane_pipeline(pg, npg, Zs, ga, gb, load_plane, mma);
+            } else {
             // the next plane's operands are loaded while this plane's products run
             auto load_plane = [&](int p, R (&dst)[RX][4]) {
-                const unsigned poff = coff + 2u * (unsigned)p * plane;
+                const unsigned poff = coff + 2u * (unsigned)p * plane, pin = cbase + 2u * (unsigned)p * plane;
 #pragma unroll
                 for (int xi = 0; xi < RX; ++xi)
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        // lanes outside the grid read their own in-grid voxel and drop it:
-                        // no divergent branch around the load
+                        // lanes outside the grid read the region's origin instead, which
+                        // meets a zero weight (the weight rows are zero past the grid's
+                        // edges): no branch around the load and no use of it before the
+                        // products
                         const bool in = (vmask >> (4 * xi + u)) & 1u;
-                        const R v = gf[in ? poff + xi * xstep + 8u * u : poff];
-                        dst[xi][u] = in ? v : R(0);
+                        dst[xi][u] = gf[in ? poff + xi * xstep + 8u * u : pin];
                     }
             };
-            R gv[RX][4], gn[RX][4];
-            if (pg < Zs) load_plane(pg, gv);
-            for (int p = pg; p < Zs; p += npg) {
-                if (p + npg < Zs) load_plane(p + npg, gn);
+            auto mma = [&](R (&gv)[RX][4], int p) {
                 R wz[NB];
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb) wz[nb] = s_w[nb * 16 + (lane & 15)][L::Z + p];
@@ -2166,10 +2320,9 @@ __global__ __launch_bounds__(256, sizeof(R) == 4 ? 3 : 2) void contract_mfma_ker
 #pragma unroll
                         for (int u = 0; u < 4; ++u) acc[nb] = M::mac(wyl[nb][u] * wxz, gv[xi][u], acc[nb]);
                     }
-#pragma unroll
-                for (int xi = 0; xi < RX; ++xi)
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) gv[xi][u] = gn[xi][u];
+            };
+            R ga[RX][4], gb[RX][4];
+            plane_pipeline(pg, npg, Zs, ga, gb, load_plane, mma);
             }
         }
         // plane groups of one column group are summed in plane-group order
@@ -2188,9 +2341,9 @@ __global__ __launch_bounds__(256, sizeof(R) == 4 ? 3 : 2) void contract_mfma_ker
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int k = nb * 16 + M::drow(lane >> 4, r);
-                    if (k < cnt) partial[(size_t)(c0 + k) * ncols + qc] = acc[nb][r];
+                for (int q = 0; q < 4; ++q) {
+                    const int k = nb * 16 + M::drow(lane >> 4, q);
+                    if (k < cnt) partial[(size_t)(c0 + k) * ncols + qc] = acc[nb][q];
                 }
         }
     }
@@ -2202,49 +2355,53 @@ __global__ __launch_bounds__(256, sizeof(R) == 4 ? 3 : 2) void contract_mfma_ker
 // spread over four waves; the next plane's operands are loaded while the
 // current plane's products run.
 template <typename R, int RX, int ZSB>
-__global__ __launch_bounds__(256) void contract_nb_mfma_kernel(int B, SpreadGeom g, const R* __restrict__ regw,
+__global__ __launch_bounds__(256) void contract_nb_mfma_kernel(int B, RegionGeom rg, const R* __restrict__ regw,
                                                                const int* __restrict__ reg_start,
-                                                               const int* __restrict__ reg_items, int nry, int N,
+                                                               const int* __restrict__ reg_items, int N,
                                                                const typename C2<R>::T* __restrict__ grid,
                                                                R* __restrict__ partial) {
     using M = Mfma<R>;
     using V4 = typename M::V4;
-    constexpr int RY = kMfmaRY, KC = kMfmaKC;
+    constexpr int KC = kMfmaKC;
     using L = RegionW<RX, ZSB>;
     __shared__ __align__(16) R s_w[KC][L::NW];
     const int tid = threadIdx.x, lane = tid & 63, nb = tid >> 6;  // candidate block of this wave
     const int reg = blockIdx.x;
-    const int x0 = (reg / nry) * RX, y0 = (reg % nry) * RY;
+    const RegionAt ra = region_at<RX>(rg, reg);
+    const int x0 = ra.x0, y0 = ra.row0;
     const int wcol0 = blockIdx.y * 16;
     const int ncols = 6 * B;
-    const int Zs = g.Zs;
-    const unsigned plane = (unsigned)(g.nx * g.nyp);
+    const int Zs = ra.np;  // planes of this region
+    const unsigned plane = rg.ps;
     const int k0 = reg_start[reg], k1 = reg_start[reg + 1];
     const R* gf = reinterpret_cast<const R*>(grid);
     const int qc = wcol0 + (lane & 15);
     const bool cvalid = qc < ncols;
-    unsigned coff = 0, vmask = 0;  // column offset; bit 4 xi + u: (x0 + xi, y0 + 4 u + lane / 16) inside
+    // column offset at the region origin (cbase, an in-grid voxel of every plane of the
+    // region) and at this lane's row; bit 4 xi + u of vmask: (x0 + xi, y0 + 4 u + lane / 16)
+    // inside the grid
+    unsigned cbase = 0, coff = 0, vmask = 0;
     if (cvalid) {
         const int b = qc / 6, c = (qc >> 1) % 3;
-        coff = 2u * (unsigned)((b * 3 + c) * Zs) * plane + (unsigned)(qc & 1) +
-               2u * (unsigned)(x0 * g.nyp + y0 + (lane >> 4));
+        cbase = 2u * (unsigned)(b * 3 + c) * rg.cs + (unsigned)(qc & 1) +
+                2u * ((unsigned)ra.plane0 * rg.ps + (unsigned)x0 * rg.xs + (unsigned)y0);
+        coff = cbase + 2u * (unsigned)(lane >> 4);
 #pragma unroll
         for (int xi = 0; xi < RX; ++xi)
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                if (x0 + xi < g.nx && y0 + 4 * u + (lane >> 4) < g.ny) vmask |= 1u << (4 * xi + u);
+                if (x0 + xi < rg.nx && y0 + 4 * u + (lane >> 4) < rg.nrow) vmask |= 1u << (4 * xi + u);
     }
-    const unsigned xstep = 2u * (unsigned)g.nyp;
+    const unsigned xstep = 2u * rg.xs;
     auto load_plane = [&](int p, R (&dst)[RX][4]) {
-        const unsigned poff = coff + 2u * (unsigned)p * plane;
+        const unsigned poff = coff + 2u * (unsigned)p * plane, pin = cbase + 2u * (unsigned)p * plane;
 #pragma unroll
         for (int xi = 0; xi < RX; ++xi)
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                // lanes outside the grid read their own in-grid voxel and drop it
+                // lanes outside the grid read the region's origin, which meets a zero weight
                 const bool in = (vmask >> (4 * xi + u)) & 1u;
-                const R v = gf[in ? poff + xi * xstep + 8u * u : poff];
-                dst[xi][u] = in ? v : R(0);
+                dst[xi][u] = gf[in ? poff + xi * xstep + 8u * u : pin];
             }
     };
     for (int c0 = k0; c0 < k1; c0 += KC) {
@@ -2259,10 +2416,7 @@ __global__ __launch_bounds__(256) void contract_nb_mfma_kernel(int B, SpreadGeom
 #pragma unroll
         for (int xi = 0; xi < RX; ++xi) wxl[xi] = s_w[k][L::X + xi];
         V4 acc = V4{R(0), R(0), R(0), R(0)};
-        R gv[RX][4], gn[RX][4];
-        load_plane(0, gv);
-        for (int p = 0; p < Zs; ++p) {
-            if (p + 1 < Zs) load_plane(p + 1, gn);
+        auto mma = [&](R (&gv)[RX][4], int p) {
             const R wz = s_w[k][L::Z + p];
 #pragma unroll
             for (int xi = 0; xi < RX; ++xi) {
@@ -2270,11 +2424,9 @@ __global__ __launch_bounds__(256) void contract_nb_mfma_kernel(int B, SpreadGeom
 #pragma unroll
                 for (int u = 0; u < 4; ++u) acc = M::mac(wyl[u] * wxz, gv[xi][u], acc);
             }
-#pragma unroll
-            for (int xi = 0; xi < RX; ++xi)
-#pragma unroll
-                for (int u = 0; u < 4; ++u) gv[xi][u] = gn[xi][u];
-        }
+        };
+        R ga[RX][4], gb[RX][4];
+        plane_pipeline(0, 1, Zs, ga, gb, load_plane, mma);
         // lane holds D[16 nb + drow(lane >> 4, r)][lane & 15]
         if (cvalid) {
 #pragma unroll
@@ -2282,6 +2434,135 @@ __global__ __launch_bounds__(256) void contract_nb_mfma_kernel(int B, SpreadGeom
                 const int kk = nb * 16 + M::drow(lane >> 4, r);
                 if (kk < cnt) partial[(size_t)(c0 + kk) * ncols + qc] = acc[r];
             }
+        }
+    }
+}
+
+// Full-grid contraction on the matrix cores: the products of contract_mfma_kernel
+// over RX x PB x 16-voxel regions (x, y planes, z rows), one wave per (region,
+// column group of 16), work item = region x column groups + group, so that the
+// groups of a region run in neighbouring waves and share its weights in the caches.
+// Waves are independent (no LDS, no barriers).
+//
+// Memory access is in whole 16-byte vectors: the four voxels a lane feeds into the
+// four products of a plane and x are rows 4 r + u (r = lane / 16, u = 0..3), and the
+// two lanes of a complex column pair (real part j even, imaginary part j + 1) load
+// rows (4 r, 4 r + 1) and (4 r + 2, 4 r + 3) as complex pairs, then swap halves with
+// a neighbour-lane move; so one load instruction per x and plane reads the 16 rows
+// of eight complex components, one whole 128-byte line each.  The candidates'
+// weights come the same way, a vector each for x, the planes and the lane's rows.
+// Needs an even number of rows (z) and 16-byte aligned grid and weight rows.
+// The next plane's operands are loaded while the current plane's products run.
+template <typename R, int RX, int PB, int NB>
+__global__ __launch_bounds__(256) void contract_full_mfma_kernel(int B, RegionGeom rg, const R* __restrict__ regw,
+                                                                 const int* __restrict__ reg_start, int nitems,
+                                                                 int ngrp, const typename C2<R>::T* __restrict__ grid,
+                                                                 R* __restrict__ partial) {
+    using M = Mfma<R>;
+    using V4 = typename M::V4;  // four R: a complex pair of the grid, or four weights
+    using L = RegionW<RX, PB>;
+    static_assert(RX == 4 && PB == 8, "one weight vector for x, two for the planes");
+    static_assert(L::Z == 4 && L::Y == 12 && L::NW == 28, "weight row: x at 0, planes at 4, rows at 12");
+    constexpr int KC = 16 * NB;
+    const int lane = threadIdx.x & 63;
+    const int item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (item >= nitems) return;  // the whole wave leaves: nothing below synchronises
+    const int reg = item / ngrp;
+    const RegionAt ra = region_at<RX>(rg, reg);
+    const int x0 = ra.x0, y0 = ra.row0, np = ra.np;
+    const int ncols = 6 * B;
+    const int r = lane >> 4, part = lane & 1;
+    const int qc = (item - reg * ngrp) * 16 + (lane & 15);
+    const bool cvalid = qc < ncols;
+    const R* gf = reinterpret_cast<const R*>(grid);
+    // 32-bit offsets in R from the uniform grid base (the launch keeps them below 2^32):
+    // cbase = the component's voxel (x0, plane0, y0), an in-grid pair of rows of every
+    // plane; this lane loads rows y0 + 4 r + 2 part and the next (vmask bit xi: inside)
+    unsigned cbase = 2u * ((unsigned)ra.plane0 * rg.ps + (unsigned)x0 * rg.xs + (unsigned)y0), coff = cbase,
+             vmask = 0;
+    if (cvalid) {
+        const int b = qc / 6, c = (qc >> 1) % 3;
+        cbase += 2u * (unsigned)(b * 3 + c) * rg.cs;
+        coff = cbase + 2u * (unsigned)(4 * r + 2 * part);
+#pragma unroll
+        for (int xi = 0; xi < RX; ++xi)
+            if (x0 + xi < rg.nx && y0 + 4 * r + 2 * part < rg.nrow) vmask |= 1u << xi;
+    }
+    const unsigned xstep = 2u * rg.xs, pstep = 2u * rg.ps;
+    const int k0 = reg_start[reg], k1 = reg_start[reg + 1];
+    struct PlaneBuf {
+        V4 g[RX];
+    };
+    for (int c0 = k0; c0 < k1; c0 += KC) {
+        const int cnt = min(KC, k1 - c0);
+        // this lane's candidates k = 16 nb + (lane & 15): x weights, plane weights (zero
+        // past the region's planes and for candidates past cnt) and rows 4 r .. 4 r + 3
+        R wyl[NB][4], wxl[NB][RX], wzl[NB][PB];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const int k = nb * 16 + (lane & 15);
+            const bool kin = k < cnt;
+            const V4* wr = reinterpret_cast<const V4*>(regw + (size_t)(c0 + (kin ? k : 0)) * L::NW);
+            const V4 wx = wr[0], wz0 = wr[1], wz1 = wr[2], wy = wr[3 + r];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                wxl[nb][q] = wx[q];
+                wzl[nb][q] = keep_if(wz0[q], kin && q < np);
+                wzl[nb][4 + q] = keep_if(wz1[q], kin && 4 + q < np);
+                wyl[nb][q] = wy[q];
+            }
+        }
+        // an opaque zero, redefined per chunk: keeps the compiler from hoisting all eight
+        // planes' loads out of the chunk loop (128 registers, no overlap with the products)
+        unsigned salt = 0;
+        asm volatile("" : "+v"(salt));
+        auto load_plane = [&](int p, PlaneBuf& dst) {
+            const int pc = min(p, np - 1);  // planes past the region's reload its last one
+            const unsigned poff = coff + salt + (unsigned)pc * pstep, pin = cbase + salt + (unsigned)pc * pstep;
+#pragma unroll
+            for (int xi = 0; xi < RX; ++xi) {
+                const bool in = (vmask >> xi) & 1u;
+                dst.g[xi] = *reinterpret_cast<const V4*>(gf + (in ? poff + xi * xstep : pin));
+            }
+        };
+        V4 acc[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[nb] = V4{R(0), R(0), R(0), R(0)};
+        auto mma = [&](PlaneBuf& pb, int p) {
+#pragma unroll
+            for (int xi = 0; xi < RX; ++xi) {
+                R gv[4];
+                pair_rows<R>(pb.g[xi], part, gv);
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) {
+                    const R wxz = wxl[nb][xi] * wzl[nb][p];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) acc[nb] = M::mac(wyl[nb][u] * wxz, gv[u], acc[nb]);
+                }
+            }
+        };
+        PlaneBuf ga, gb;
+        load_plane(0, ga);
+#pragma unroll
+        for (int p = 0; p < PB; p += 2) {
+            __builtin_amdgcn_sched_barrier(0);
+            load_plane(p + 1, gb);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(ga, p);
+            __builtin_amdgcn_sched_barrier(0);
+            if (p + 2 < PB) load_plane(p + 2, ga);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(gb, p + 1);
+        }
+        // lane holds D[16 nb + drow(lane >> 4, r)][lane & 15]
+        if (cvalid) {
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int k = nb * 16 + M::drow(lane >> 4, q);
+                    if (k < cnt) partial[(size_t)(c0 + k) * ncols + qc] = acc[nb][q];
+                }
         }
     }
 }
@@ -4744,45 +5025,66 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         nbins[d] = (S.n[d] + binsz[d] - 1) / binsz[d];
         ntiles[d] = (S.n[d] + tile[d] - 1) / tile[d];
     }
-    // regions of the matrix-core spread: every particle joins each region its
-    // xy stencil footprint meets
+    // regions of the matrix-core spread and contraction: every particle joins each
+    // region its stencil footprint meets.  Slab: x blocks of reg_rx and y blocks of
+    // 16, all slab planes in each region; full grid (dipoles of one radius): x blocks
+    // of 4, y blocks of kFullRegionPlanes planes and z blocks of 16 rows.
     reg_rx = 0;
-    if (slab) {
-        reg_rx = zs_bucket(Zs) <= 10 ? 4 : 2;
-        const int RX = reg_rx, RY = 16;
+    reg_full = false;
+    reg_npb = 1;
+    reg_pblk = 0;
+    if (slab || (!poly && !quads)) {
+        const int RX = slab ? (zs_bucket(Zs) <= 10 ? 4 : 2) : 4, RY = kMfmaRY;
+        const int arow = slab ? 1 : 2;  // axis of the 16 contiguous rows
+        const int pblk = slab ? Zs : kFullRegionPlanes;
+        reg_rx = RX;
+        reg_full = !slab;
+        reg_pblk = pblk;
         reg_nrx = (S.n[0] + RX - 1) / RX;
-        reg_nry = (S.n[1] + RY - 1) / RY;
-        const int nreg = reg_nrx * reg_nry;
-        std::vector<std::vector<int>> lists(nreg);
-        std::vector<int> rxs, rys;
-        for (int i = 0; i < Nn; ++i) {
-            auto regions_of = [&](int d, int R, std::vector<int>& outv) {
-                outv.clear();
-                const int nd = S.n[d];
-                for (int o = -half; o <= half; ++o) {
-                    const int v = (((base[(size_t)d * Nn + i] + o) % nd) + nd) % nd;
-                    const int r = v / R;
-                    if (std::find(outv.begin(), outv.end(), r) == outv.end()) outv.push_back(r);
-                }
-            };
-            regions_of(0, RX, rxs);
-            regions_of(1, RY, rys);
-            for (int rx : rxs)
-                for (int ry : rys) lists[(size_t)rx * reg_nry + ry].push_back(i);
+        reg_nry = (S.n[arow] + RY - 1) / RY;
+        reg_npb = slab ? 1 : (S.n[1] + pblk - 1) / pblk;
+        const int nreg = reg_nrx * reg_npb * reg_nry;
+        // distinct blocks of size R that the stencil of particle i meets along axis d
+        auto blocks_of = [&](int i, int d, int R, int* outv) {
+            int n = 0;
+            const int nd = S.n[d];
+            for (int o = -half; o <= half; ++o) {
+                const int v = (((base[(size_t)d * Nn + i] + o) % nd) + nd) % nd;
+                const int r = v / R;
+                bool seen = false;
+                for (int q = 0; q < n; ++q) seen = seen || outv[q] == r;
+                if (!seen) outv[n++] = r;
+            }
+            return n;
+        };
+        auto for_regions = [&](int i, auto&& f) {
+            int rxs[kMaxWidth], rys[kMaxWidth], pbs[kMaxWidth];
+            const int nrx = blocks_of(i, 0, RX, rxs);
+            const int nry = blocks_of(i, arow, RY, rys);
+            int npb = 1;
+            pbs[0] = 0;
+            if (!slab) npb = blocks_of(i, 1, pblk, pbs);
+            for (int a = 0; a < nrx; ++a)
+                for (int c = 0; c < npb; ++c)
+                    for (int e = 0; e < nry; ++e) f(((size_t)rxs[a] * reg_npb + pbs[c]) * reg_nry + rys[e]);
+        };
+        // candidates of every region in ascending particle order (count, then fill)
+        std::vector<int> rs(nreg + 1, 0);
+        for (int i = 0; i < Nn; ++i) for_regions(i, [&](size_t r) { rs[r + 1]++; });
+        for (int r = 0; r < nreg; ++r) rs[r + 1] += rs[r];
+        std::vector<int> ri(std::max(rs[nreg], 1));
+        {
+            std::vector<int> fill(rs.begin(), rs.end() - 1);
+            for (int i = 0; i < Nn; ++i) for_regions(i, [&](size_t r) { ri[fill[r]++] = i; });
         }
-        std::vector<int> rs(nreg + 1, 0), ri;
-        for (int r = 0; r < nreg; ++r) rs[r + 1] = rs[r] + (int)lists[r].size();
-        ri.reserve(rs[nreg] + 1);
-        for (int r = 0; r < nreg; ++r) ri.insert(ri.end(), lists[r].begin(), lists[r].end());
         // slots of each particle, in ascending region order
-        std::vector<int> pss(Nn + 1, 0), psi(std::max<size_t>(ri.size(), 1));
-        for (size_t q = 0; q < (size_t)rs[nreg]; ++q) pss[ri[q] + 1]++;
+        std::vector<int> pss(Nn + 1, 0), psi(std::max(rs[nreg], 1));
+        for (int q = 0; q < rs[nreg]; ++q) pss[ri[q] + 1]++;
         for (int i = 0; i < Nn; ++i) pss[i + 1] += pss[i];
         {
             std::vector<int> fill(pss.begin(), pss.end() - 1);
             for (int q = 0; q < rs[nreg]; ++q) psi[fill[ri[q]]++] = q;
         }
-        if (ri.empty()) ri.push_back(0);
         d_reg_start.upload(rs.data(), rs.size());
         d_reg_items.upload(ri.data(), ri.size());
         d_ps_start.upload(pss.data(), pss.size());
@@ -4790,9 +5092,9 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         d_partial.resize((size_t)std::max(rs[nreg], 1) * 6 * kMaxSlots);
         d_partial64.resize((size_t)std::max(rs[nreg], 1) * 6 * kMaxSlots);
         // region-local separable weights of every (region, candidate) slot, in the
-        // row layout of the kernels: x at [0, RX), slab planes at [4, 4 + ZSB),
-        // y at [4 + ZSB4, 4 + ZSB4 + 16); sums over stencil points that fold
-        const int zsb = zs_bucket(Zs), zsb4 = (zsb + 3) & ~3;
+        // row layout of the kernels: x at [0, RX), planes at [4, 4 + ZSB), rows at
+        // [4 + ZSB4, 4 + ZSB4 + 16); sums over stencil points that fold
+        const int zsb = slab ? zs_bucket(Zs) : kFullRegionPlanes, zsb4 = (zsb + 3) & ~3;
         const int NW = 4 + zsb4 + RY, X = 0, Zo = 4, Yo = 4 + zsb4;
         std::vector<double> rw64((size_t)std::max(rs[nreg], 1) * NW, 0.0);
         auto wsum = [&](int pid, int d, int v) {
@@ -4806,15 +5108,21 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         };
         parallel_for(nreg, host_threads(nreg, 16), [&](int r0, int r1, int) {
             for (int r = r0; r < r1; ++r) {
-                const int x0 = (r / reg_nry) * RX, y0 = (r % reg_nry) * RY;
+                const int per_x = reg_npb * reg_nry;
+                const int x0 = (r / per_x) * RX, pl0 = ((r % per_x) / reg_nry) * pblk,
+                          y0 = (r % reg_nry) * RY;
                 for (int q = rs[r]; q < rs[r + 1]; ++q) {
                     const int pid = ri[q];
                     double* row = &rw64[(size_t)q * NW];
                     for (int j = 0; j < RX; ++j)
                         if (x0 + j < S.n[0]) row[X + j] = wsum(pid, 0, x0 + j);
-                    for (int p = 0; p < Zs; ++p) row[Zo + p] = wsum(pid, 2, zl[p]);
+                    if (slab) {
+                        for (int p = 0; p < Zs; ++p) row[Zo + p] = wsum(pid, 2, zl[p]);
+                    } else {
+                        for (int p = 0; p < pblk && pl0 + p < S.n[1]; ++p) row[Zo + p] = wsum(pid, 1, pl0 + p);
+                    }
                     for (int j = 0; j < RY; ++j)
-                        if (y0 + j < S.n[1]) row[Yo + j] = wsum(pid, 1, y0 + j);
+                        if (y0 + j < S.n[arow]) row[Yo + j] = wsum(pid, arow, y0 + j);
                 }
             }
         });
@@ -5163,17 +5471,47 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
     g.nyp = slab ? nyp : S.n[1];
     const R* regw = fp32 ? reinterpret_cast<const R*>(d_regw32.p) : reinterpret_cast<const R*>(d_regw64.p);
     const int Ni = (int)N;
+    RegionGeom rg;
+    rg.nx = S.n[0];
+    rg.nrr = reg_nry;
+    rg.npb = reg_npb;
+    rg.pblk = reg_pblk;
+    if (SLAB) {
+        rg.nrow = S.n[1];
+        rg.nplane = Zs;
+        rg.xs = (unsigned)nyp;
+        rg.ps = (unsigned)(S.n[0] * nyp);
+        rg.cs = (unsigned)((size_t)Zs * S.n[0] * nyp);
+    } else {
+        rg.nrow = S.n[2];
+        rg.nplane = S.n[1];
+        rg.xs = (unsigned)(S.n[1] * S.n[2]);
+        rg.ps = (unsigned)S.n[2];
+        rg.cs = (unsigned)V;
+    }
+    // the matrix-core kernels address the grid with 32-bit element offsets
+    const bool regions = reg_rx > 0 && (SLAB || reg_full) &&
+                         2ull * (unsigned long long)B * grid_elems_per_rhs() < (1ull << 32);
+    const int nreg = reg_nrx * reg_npb * reg_nry;
     if (parts & MPM_PART_RECIP) {
         if (phases & 1) {
-            if (SLAB && reg_rx > 0) {
+            if (regions && !SLAB) {
                 ProfScope ps("spread", s, B);
                 constexpr int CPB = 32;  // SpreadShape<R, ...>::CPB
-                const dim3 gs((unsigned)(reg_nrx * reg_nry), (unsigned)((6 * B + CPB - 1) / CPB));
+                const dim3 gs((unsigned)nreg, (unsigned)((6 * B + CPB - 1) / CPB));
+                ps.kernel("spread_mfma_kernel", {type_name<R>(), "4", std::to_string(kFullRegionPlanes)});
+                hipLaunchKernelGGL((spread_mfma_kernel<R, 4, kFullRegionPlanes>), gs, dim3(256), 0, s, t, B, rg, regw,
+                                   d_reg_start.p, d_reg_items.p, Ni, grid);
+                MPM_HIP(hipGetLastError());
+            } else if (regions) {
+                ProfScope ps("spread", s, B);
+                constexpr int CPB = 32;  // SpreadShape<R, ...>::CPB
+                const dim3 gs((unsigned)nreg, (unsigned)((6 * B + CPB - 1) / CPB));
     #define MPM_SM(RXV, ZSV)                                                                                          \
         case ZSV:                                                                                                     \
             ps.kernel("spread_mfma_kernel", {type_name<R>(), #RXV, #ZSV});                                            \
-            hipLaunchKernelGGL((spread_mfma_kernel<R, RXV, ZSV>), gs, dim3(256), 0, s, t, B, g, regw, d_reg_start.p, \
-                               d_reg_items.p, reg_nry, Ni, grid);                                                    \
+            hipLaunchKernelGGL((spread_mfma_kernel<R, RXV, ZSV>), gs, dim3(256), 0, s, t, B, rg, regw, d_reg_start.p, \
+                               d_reg_items.p, Ni, grid);                                                             \
             break;
                 switch (zs_bucket(Zs)) {
                     MPM_SM(4, 4)
@@ -5337,25 +5675,48 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
     }
     if (!(phases & 4)) return;
     const R* part = nullptr;
-    if (SLAB && reg_rx > 0 && (parts & MPM_PART_RECIP)) {
+    if (regions && (parts & MPM_PART_RECIP)) {
         ProfScope ps("contract", s, B);
         // column groups of 16 per block: 4 when there are enough right-hand sides,
-        // else fewer, with the slab planes split over the remaining waves; one or
-        // two column groups: one wave per candidate block instead
+        // else fewer, with the region's planes split over the remaining waves; slab
+        // with one or two column groups: one wave per candidate block instead
         const int ngrp = (6 * B + 15) / 16;
         const int ncg = ngrp >= 4 ? 4 : (ngrp >= 2 ? 2 : 1);
-        const dim3 gs((unsigned)(reg_nrx * reg_nry), (unsigned)((ngrp + ncg - 1) / ncg));
-        const dim3 gn((unsigned)(reg_nrx * reg_nry), (unsigned)ngrp);
+        const dim3 gs((unsigned)nreg, (unsigned)((ngrp + ncg - 1) / ncg));
+        const dim3 gn((unsigned)nreg, (unsigned)ngrp);
         R* pbuf = fp32 ? reinterpret_cast<R*>(d_partial.p) : reinterpret_cast<R*>(d_partial64.p);
+        // whole vectors of two complex: an even row count and stride and an aligned grid
+        // keep every pair aligned and both of its rows inside or outside together
+        const bool aligned = (reinterpret_cast<uintptr_t>(grid) % 32) == 0;
+        const bool vec = aligned && (SLAB ? (nyp % 2 == 0 && S.n[1] % 2 == 0) : S.n[2] % 2 == 0);
+        if (!SLAB && vec) {
+            ps.kernel("contract_full_mfma_kernel",
+                      {type_name<R>(), "4", std::to_string(kFullRegionPlanes), std::to_string(kFullContractNB)});
+            const int nitems = nreg * ngrp;
+            hipLaunchKernelGGL((contract_full_mfma_kernel<R, 4, kFullRegionPlanes, kFullContractNB>),
+                               dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, s, B, rg, regw, d_reg_start.p, nitems,
+                               ngrp, grid, pbuf);
+        } else if (!SLAB) {
+            ps.kernel("contract_mfma_kernel", {type_name<R>(), "4", std::to_string(kFullRegionPlanes),
+                                               std::to_string(kFullContractNB), "false"});
+            hipLaunchKernelGGL((contract_mfma_kernel<R, 4, kFullRegionPlanes, kFullContractNB, false>), gs, dim3(256),
+                               0, s, B, rg, regw, d_reg_start.p, d_reg_items.p, Ni, grid, pbuf, ncg);
+        } else {
 #define MPM_CM(RXV, ZSV)                                                                                          \
     case ZSV:                                                                                                     \
-        ps.kernel(ngrp <= 2 ? "contract_nb_mfma_kernel" : "contract_mfma_kernel", {type_name<R>(), #RXV, #ZSV});   \
-        if (ngrp <= 2)                                                                                            \
-            hipLaunchKernelGGL((contract_nb_mfma_kernel<R, RXV, ZSV>), gn, dim3(256), 0, s, B, g, regw,          \
-                               d_reg_start.p, d_reg_items.p, reg_nry, Ni, grid, pbuf);                           \
-        else                                                                                                      \
-            hipLaunchKernelGGL((contract_mfma_kernel<R, RXV, ZSV>), gs, dim3(256), 0, s, B, g, regw, d_reg_start.p, \
-                               d_reg_items.p, reg_nry, Ni, grid, pbuf, ncg);                                     \
+        if (ngrp <= 2) {                                                                                          \
+            ps.kernel("contract_nb_mfma_kernel", {type_name<R>(), #RXV, #ZSV});                                   \
+            hipLaunchKernelGGL((contract_nb_mfma_kernel<R, RXV, ZSV>), gn, dim3(256), 0, s, B, rg, regw,          \
+                               d_reg_start.p, d_reg_items.p, Ni, grid, pbuf);                                    \
+        } else if (vec) {                                                                                         \
+            ps.kernel("contract_mfma_kernel", {type_name<R>(), #RXV, #ZSV, "4", "true"});                         \
+            hipLaunchKernelGGL((contract_mfma_kernel<R, RXV, ZSV, 4, true>), gs, dim3(256), 0, s, B, rg, regw,    \
+                               d_reg_start.p, d_reg_items.p, Ni, grid, pbuf, ncg);                               \
+        } else {                                                                                                  \
+            ps.kernel("contract_mfma_kernel", {type_name<R>(), #RXV, #ZSV, "4", "false"});                        \
+            hipLaunchKernelGGL((contract_mfma_kernel<R, RXV, ZSV, 4, false>), gs, dim3(256), 0, s, B, rg, regw,   \
+                               d_reg_start.p, d_reg_items.p, Ni, grid, pbuf, ncg);                               \
+        }                                                                                                         \
         break;
         switch (zs_bucket(Zs)) {
             MPM_CM(4, 4)
@@ -5369,6 +5730,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
                 throw Error(MPM_ERR_RUNTIME, "slab size out of range");
         }
 #undef MPM_CM
+        }
         MPM_HIP(hipGetLastError());
         part = pbuf;
     }

@@ -328,9 +328,12 @@ class EwaldOp {
     int nyp = 0, nxp = 0;        // slab row strides of the grid ([x][nyp]) and k-space ([ky][nxp]) buffers
     size_t kgrid_elems_per_rhs() const;
     int kspace_cols() const;
-    // matrix-core spread and contraction (slab): xy-regions of reg_rx x 16 voxels and the
-    // candidate particles of each region (CSR, ascending ids)
-    int reg_rx = 0, reg_nrx = 0, reg_nry = 0;
+    // matrix-core spread and contraction: regions of reg_rx x (reg_pblk planes) x 16 rows
+    // (slab: every occupied z-plane, rows y; full grid: planes y, rows z) and the candidate
+    // particles of each region (CSR, ascending ids); reg_nry row blocks and reg_npb plane
+    // blocks per x block of regions
+    int reg_rx = 0, reg_nrx = 0, reg_nry = 0, reg_npb = 1, reg_pblk = 0;
+    bool reg_full = false;       // the regions tile the full grid (monodisperse dipoles)
     DevBuf<int> d_reg_start;
     DevBuf<int> d_reg_items;
     DevBuf<int> d_ps_start;      // [N + 1] region slots of each particle
