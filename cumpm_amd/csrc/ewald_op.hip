@@ -2354,7 +2354,7 @@ __global__ __launch_bounds__(256, sizeof(R) == 4 ? 3 : 2) void contract_mfma_ker
 // and, when only one or two column groups exist, the region's candidates still
 // spread over four waves; the next plane's operands are loaded while the
 // current plane's products run.
-template <typename R, int RX, int ZSB>
+template <typename R, int RX, int ZSB, bool VEC>
 __global__ __launch_bounds__(256) void contract_nb_mfma_kernel(int B, RegionGeom rg, const R* __restrict__ regw,
                                                                const int* __restrict__ reg_start,
                                                                const int* __restrict__ reg_items, int N,
@@ -2379,31 +2379,29 @@ __global__ __launch_bounds__(256) void contract_nb_mfma_kernel(int B, RegionGeom
     const bool cvalid = qc < ncols;
     // column offset at the region origin (cbase, an in-grid voxel of every plane of the
     // region) and at this lane's row; bit 4 xi + u of vmask: (x0 + xi, y0 + 4 u + lane / 16)
-    // inside the grid
+    // inside the grid.  VEC (pair_rows): the lane's complex pair at rows y0 + 4 r + 2 part
+    // and the next, bit xi of vmask: (x0 + xi, y0 + 4 r + 2 part) inside.
+    const int r = lane >> 4, part = lane & 1;
     unsigned cbase = 0, coff = 0, vmask = 0;
     if (cvalid) {
         const int b = qc / 6, c = (qc >> 1) % 3;
-        cbase = 2u * (unsigned)(b * 3 + c) * rg.cs + (unsigned)(qc & 1) +
+        cbase = 2u * (unsigned)(b * 3 + c) * rg.cs + (VEC ? 0u : (unsigned)(qc & 1)) +
                 2u * ((unsigned)ra.plane0 * rg.ps + (unsigned)x0 * rg.xs + (unsigned)y0);
-        coff = cbase + 2u * (unsigned)(lane >> 4);
+        if (VEC) {
+            coff = cbase + 2u * (unsigned)(4 * r + 2 * part);
 #pragma unroll
-        for (int xi = 0; xi < RX; ++xi)
+            for (int xi = 0; xi < RX; ++xi)
+                if (x0 + xi < rg.nx && y0 + 4 * r + 2 * part < rg.nrow) vmask |= 1u << xi;
+        } else {
+            coff = cbase + 2u * (unsigned)(lane >> 4);
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (x0 + xi < rg.nx && y0 + 4 * u + (lane >> 4) < rg.nrow) vmask |= 1u << (4 * xi + u);
+            for (int xi = 0; xi < RX; ++xi)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (x0 + xi < rg.nx && y0 + 4 * u + (lane >> 4) < rg.nrow) vmask |= 1u << (4 * xi + u);
+        }
     }
     const unsigned xstep = 2u * rg.xs;
-    auto load_plane = [&](int p, R (&dst)[RX][4]) {
-        const unsigned poff = coff + 2u * (unsigned)p * plane, pin = cbase + 2u * (unsigned)p * plane;
-#pragma unroll
-        for (int xi = 0; xi < RX; ++xi)
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                // lanes outside the grid read the region's origin, which meets a zero weight
-                const bool in = (vmask >> (4 * xi + u)) & 1u;
-                dst[xi][u] = gf[in ? poff + xi * xstep + 8u * u : pin];
-            }
-    };
     for (int c0 = k0; c0 < k1; c0 += KC) {
         const int cnt = min(KC, k1 - c0);
         __syncthreads();
@@ -2412,27 +2410,65 @@ __global__ __launch_bounds__(256) void contract_nb_mfma_kernel(int B, RegionGeom
         const int k = nb * 16 + (lane & 15);
         R wyl[4], wxl[RX];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) wyl[u] = s_w[k][L::Y + 4 * u + (lane >> 4)];
+        for (int u = 0; u < 4; ++u) wyl[u] = s_w[k][L::Y + (VEC ? 4 * r + u : 4 * u + r)];
 #pragma unroll
         for (int xi = 0; xi < RX; ++xi) wxl[xi] = s_w[k][L::X + xi];
         V4 acc = V4{R(0), R(0), R(0), R(0)};
-        auto mma = [&](R (&gv)[RX][4], int p) {
-            const R wz = s_w[k][L::Z + p];
+        if constexpr (VEC) {
+            struct PlaneBuf {
+                V4 g[RX];
+            };
+            auto load_plane = [&](int p, PlaneBuf& dst) {
+                const unsigned poff = coff + 2u * (unsigned)p * plane, pin = cbase + 2u * (unsigned)p * plane;
 #pragma unroll
-            for (int xi = 0; xi < RX; ++xi) {
-                const R wxz = wxl[xi] * wz;
+                for (int xi = 0; xi < RX; ++xi) {
+                    const bool in = (vmask >> xi) & 1u;
+                    dst.g[xi] = *reinterpret_cast<const V4*>(gf + (in ? poff + xi * xstep : pin));
+                }
+            };
+            auto mma = [&](PlaneBuf& pb, int p) {
+                const R wz = s_w[k][L::Z + p];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) acc = M::mac(wyl[u] * wxz, gv[xi][u], acc);
-            }
-        };
-        R ga[RX][4], gb[RX][4];
-        plane_pipeline(0, 1, Zs, ga, gb, load_plane, mma);
-        // lane holds D[16 nb + drow(lane >> 4, r)][lane & 15]
+                for (int xi = 0; xi < RX; ++xi) {
+                    R gv[4];
+                    pair_rows<R>(pb.g[xi], part, gv);
+                    const R wxz = wxl[xi] * wz;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) acc = M::mac(wyl[u] * wxz, gv[u], acc);
+                }
+            };
+            PlaneBuf ga, gb;
+            plane_pipeline(0, 1, Zs, ga, gb, load_plane, mma);
+        } else {
+            auto load_plane = [&](int p, R (&dst)[RX][4]) {
+                const unsigned poff = coff + 2u * (unsigned)p * plane, pin = cbase + 2u * (unsigned)p * plane;
+#pragma unroll
+                for (int xi = 0; xi < RX; ++xi)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        // lanes outside the grid read the region's origin, which meets a zero weight
+                        const bool in = (vmask >> (4 * xi + u)) & 1u;
+                        dst[xi][u] = gf[in ? poff + xi * xstep + 8u * u : pin];
+                    }
+            };
+            auto mma = [&](R (&gv)[RX][4], int p) {
+                const R wz = s_w[k][L::Z + p];
+#pragma unroll
+                for (int xi = 0; xi < RX; ++xi) {
+                    const R wxz = wxl[xi] * wz;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) acc = M::mac(wyl[u] * wxz, gv[xi][u], acc);
+                }
+            };
+            R ga[RX][4], gb[RX][4];
+            plane_pipeline(0, 1, Zs, ga, gb, load_plane, mma);
+        }
+        // lane holds D[16 nb + drow(lane >> 4, q)][lane & 15]
         if (cvalid) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int kk = nb * 16 + M::drow(lane >> 4, r);
-                if (kk < cnt) partial[(size_t)(c0 + kk) * ncols + qc] = acc[r];
+            for (int q = 0; q < 4; ++q) {
+                const int kk = nb * 16 + M::drow(lane >> 4, q);
+                if (kk < cnt) partial[(size_t)(c0 + kk) * ncols + qc] = acc[q];
             }
         }
     }
@@ -5493,6 +5529,12 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
     const bool regions = reg_rx > 0 && (SLAB || reg_full) &&
                          2ull * (unsigned long long)B * grid_elems_per_rhs() < (1ull << 32);
     const int nreg = reg_nrx * reg_npb * reg_nry;
+    // the matrix-core contractions load whole vectors of two complex when an even row
+    // count and stride and an aligned grid keep every pair aligned and both of its rows
+    // inside or outside together (16-byte stores in the spread measured slower: kept at
+    // one complex per lane)
+    const bool aligned = (reinterpret_cast<uintptr_t>(grid) % 32) == 0;
+    const bool vec = aligned && (SLAB ? (nyp % 2 == 0 && S.n[1] % 2 == 0) : S.n[2] % 2 == 0);
     if (parts & MPM_PART_RECIP) {
         if (phases & 1) {
             if (regions && !SLAB) {
@@ -5685,10 +5727,6 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
         const dim3 gs((unsigned)nreg, (unsigned)((ngrp + ncg - 1) / ncg));
         const dim3 gn((unsigned)nreg, (unsigned)ngrp);
         R* pbuf = fp32 ? reinterpret_cast<R*>(d_partial.p) : reinterpret_cast<R*>(d_partial64.p);
-        // whole vectors of two complex: an even row count and stride and an aligned grid
-        // keep every pair aligned and both of its rows inside or outside together
-        const bool aligned = (reinterpret_cast<uintptr_t>(grid) % 32) == 0;
-        const bool vec = aligned && (SLAB ? (nyp % 2 == 0 && S.n[1] % 2 == 0) : S.n[2] % 2 == 0);
         if (!SLAB && vec) {
             ps.kernel("contract_full_mfma_kernel",
                       {type_name<R>(), "4", std::to_string(kFullRegionPlanes), std::to_string(kFullContractNB)});
@@ -5705,9 +5743,13 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
 #define MPM_CM(RXV, ZSV)                                                                                          \
     case ZSV:                                                                                                     \
         if (ngrp <= 2) {                                                                                          \
-            ps.kernel("contract_nb_mfma_kernel", {type_name<R>(), #RXV, #ZSV});                                   \
-            hipLaunchKernelGGL((contract_nb_mfma_kernel<R, RXV, ZSV>), gn, dim3(256), 0, s, B, rg, regw,          \
-                               d_reg_start.p, d_reg_items.p, Ni, grid, pbuf);                                    \
+            ps.kernel("contract_nb_mfma_kernel", {type_name<R>(), #RXV, #ZSV, tbool(vec)});                       \
+            if (vec)                                                                                              \
+                hipLaunchKernelGGL((contract_nb_mfma_kernel<R, RXV, ZSV, true>), gn, dim3(256), 0, s, B, rg,      \
+                                   regw, d_reg_start.p, d_reg_items.p, Ni, grid, pbuf);                          \
+            else                                                                                                  \
+                hipLaunchKernelGGL((contract_nb_mfma_kernel<R, RXV, ZSV, false>), gn, dim3(256), 0, s, B, rg,     \
+                                   regw, d_reg_start.p, d_reg_items.p, Ni, grid, pbuf);                          \
         } else if (vec) {                                                                                         \
             ps.kernel("contract_mfma_kernel", {type_name<R>(), #RXV, #ZSV, "4", "true"});                         \
             hipLaunchKernelGGL((contract_mfma_kernel<R, RXV, ZSV, 4, true>), gs, dim3(256), 0, s, B, rg, regw,    \
