@@ -1357,6 +1357,151 @@ __global__ __launch_bounds__(256) void line_fft2_kernel(const typename C2<R>::T*
     }
 }
 
+// Full grid ([B][3][nx][ny][nz], z lines contiguous): the forward z transform, the
+// Green's tensor projection f <- sc (f . k^) k^ and the inverse z transform in one
+// pass, after the forward x and y transforms and before the inverse ones (the
+// axes' transforms commute), so that the grid is read and written once for the
+// three steps instead of three times.  A block takes TL z lines of each of the
+// three components of one right-hand side (the projection couples them per
+// k-point), both register stages of each transform as line_fft2_kernel, all in one
+// LDS buffer y: forward stage 1 from the grid into y, forward stage 2 in place
+// (each butterfly writes the elements it read), the projection in place, inverse
+// stage 1 with every thread's inputs read before any output is written (one
+// butterfly per thread: 3 TL P2 <= 256), inverse stage 2 from y to the grid.  kc
+// holds (k^x, k^y, k^z, sc) per voxel.
+template <typename R, int CLS>
+__global__ __launch_bounds__(256) void zline_scale_kernel(typename C2<R>::T* grid, int nlines, int nxy, int n,
+                                                          int P1, int P2,
+                                                          const typename C2<R>::T* __restrict__ twiddle, int TL,
+                                                          const R* __restrict__ kc) {
+    using CT = typename C2<R>::T;
+    constexpr int PMAX = CLS == 0 ? 8 : (CLS == 1 ? 16 : kMaxPrimeRadix);
+    extern __shared__ unsigned char s_raw[];
+    CT* const y = reinterpret_cast<CT*>(s_raw);  // [3 TL][n]
+    CT* const tw = y + 3 * TL * n;               // forward twiddles, then their conjugates
+    CT* const twi = tw + n;
+    const int tid = threadIdx.x;
+    const int l0 = blockIdx.x * TL;
+    const int nl = min(TL, nlines - l0);
+    const int nt = 3 * nl;  // block lines t = c nl + li
+    for (int k = tid; k < n; k += blockDim.x) {
+        const CT w = twiddle[k];
+        tw[k] = w;
+        CT wc = w;
+        wc.y = -wc.y;
+        twi[k] = wc;
+    }
+    __shared__ int s_base[3 * 64];  // grid offset of each block line
+    __shared__ int s_vox[64];       // voxel (x, y, 0) of each line
+    if (tid < nt) {
+        const int c = tid / nl, li = tid - c * nl;
+        const int L = l0 + li, b = L / nxy, xy = L - b * nxy;
+        s_base[tid] = ((b * 3 + c) * nxy + xy) * n;
+        if (c == 0) s_vox[li] = xy * n;
+    }
+    __syncthreads();
+    const float r2 = 1.0f / (float)P2, r1 = 1.0f / (float)P1;
+    // forward stage 1: butterflies j < P2 of radix P1, grid -> y
+    for (int w = tid; w < nt * P2; w += blockDim.x) {
+        int t, j;
+        fdivmod(w, P2, r2, t, j);
+        switch (P1) {
+#define MPM_S1(P, C)                                                                     \
+    case P:                                                                              \
+        if constexpr (CLS >= C) fft2_s1<R, P, false>(grid, s_base[t] + j, P2, y, t * n + j * P1); \
+        break;
+            MPM_FFT2_CASES(MPM_S1)
+#undef MPM_S1
+            default:
+                break;
+        }
+    }
+    __syncthreads();
+    // forward stage 2: butterflies a < P1 of radix P2, in place in y (natural order)
+    for (int w = tid; w < nt * P1; w += blockDim.x) {
+        int t, a;
+        fdivmod(w, P1, r1, t, a);
+        switch (P2) {
+#define MPM_S2(P, C)                                                                           \
+    case P:                                                                                    \
+        if constexpr (CLS >= C) fft2_s2<R, P, false>(y, t * n + a, P1, tw, a, y, t * n + a, P1); \
+        break;
+            MPM_FFT2_CASES(MPM_S2)
+#undef MPM_S2
+            default:
+                break;
+        }
+    }
+    __syncthreads();
+    // projection per k-point of each line
+    for (int w = tid; w < nl * n; w += blockDim.x) {
+        const int li = w / n, k = w - li * n;
+        const int v = s_vox[li] + k;
+        const R kx = kc[4 * (size_t)v], ky = kc[4 * (size_t)v + 1], kz = kc[4 * (size_t)v + 2],
+                sc = kc[4 * (size_t)v + 3];
+        CT* f0 = y + li * n + k;
+        CT* f1 = f0 + nl * n;
+        CT* f2 = f1 + nl * n;
+        const CT a0 = *f0, a1 = *f1, a2 = *f2;
+        const R dr = (a0.x * kx + a1.x * ky + a2.x * kz) * sc;
+        const R di = (a0.y * kx + a1.y * ky + a2.y * kz) * sc;
+        CT o;
+        o.x = dr * kx;
+        o.y = di * kx;
+        *f0 = o;
+        o.x = dr * ky;
+        o.y = di * ky;
+        *f1 = o;
+        o.x = dr * kz;
+        o.y = di * kz;
+        *f2 = o;
+    }
+    __syncthreads();
+    // inverse stage 1: butterfly (t, j) of radix P1 reads y[t n + j + r P2] and writes
+    // y[t n + j P1 + q]; all reads precede all writes
+    {
+        const bool act = tid < nt * P2;
+        int t = 0, j = 0;
+        if (act) fdivmod(tid, P2, r2, t, j);
+        CT v[PMAX];
+#pragma unroll
+        for (int r = 0; r < PMAX; ++r)
+            if (act && r < P1) v[r] = y[t * n + j + r * P2];
+        __syncthreads();
+        if (act) {
+            switch (P1) {
+#define MPM_S1(P, C)                                                        \
+    case P:                                                                 \
+        if constexpr (CLS >= C && P <= PMAX) {                              \
+            dft_reg<R, P, true>(v);                                         \
+            _Pragma("unroll") for (int q = 0; q < P; ++q) y[t * n + j * P1 + q] = v[q]; \
+        }                                                                   \
+        break;
+                MPM_FFT2_CASES(MPM_S1)
+#undef MPM_S1
+                default:
+                    break;
+            }
+        }
+    }
+    __syncthreads();
+    // inverse stage 2: y -> grid
+    for (int w = tid; w < nt * P1; w += blockDim.x) {
+        int t, a;
+        fdivmod(w, P1, r1, t, a);
+        switch (P2) {
+#define MPM_S2(P, C)                                                                                   \
+    case P:                                                                                            \
+        if constexpr (CLS >= C) fft2_s2<R, P, true>(y, t * n + a, P1, twi, a, grid, s_base[t] + a, P1); \
+        break;
+            MPM_FFT2_CASES(MPM_S2)
+#undef MPM_S2
+            default:
+                break;
+        }
+    }
+}
+
 // Stage radices for length n: primes above kMaxPrimeRadix stand alone (generic
 // stage), primes from 17 up stand alone, and the small primes are packed,
 // largest first, into the fullest radix that stays <= kMaxCompositeRadix.
@@ -1443,6 +1588,7 @@ struct LineFft {
     std::vector<LinePass> fwd, inv;
     long long rhs_elems = 0;  // elements per RHS (grid size)
     int kt_x = -1;            // transposed slab layout: the axis of the contiguous x lines
+    int full_z = -1;          // full grid: the axis of the contiguous z lines
 
     // lines per block: as many as fit the per-block element budget (at most 64)
     template <typename R>
@@ -5417,6 +5563,7 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
             for (int d = 0; d < 3; ++d) {
                 if (n3[d] <= 1) continue;
                 const int a = axis(n3[d]);
+                if (d == 0) lf->full_z = a;
                 pass(lf->fwd, a, g3[d], g3[d], 0, 0);
                 pass(lf->inv, a, g3[d], g3[d], 0, 0);
             }
@@ -5590,14 +5737,50 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
             const int xaxis = (SLAB && zcontig && kt && lfft && fft_backend == 0) ? lfft->kt_x : -1;
             const size_t xlds = (size_t)(3 * Zs + 2) * S.n[0] * sizeof(CT);
             const bool fusedx = fp32 && xaxis >= 0 && lfft->axes[xaxis]->two && xlds <= kMaxFusedLds;
+            // full grid (FP32, built-in transform, two-stage z lines): the z transforms and
+            // the Green's tensor in one pass over z lines of three components
+            const int zaxis = (!SLAB && fp32 && lfft && fft_backend == 0) ? lfft->full_z : -1;
+            int ztl = 0;
+            if (zaxis >= 0 && lfft->axes[zaxis]->two) {
+                const LinePlan& pl = lfft->axes[zaxis]->plan;
+                const size_t per_line = 3 * (size_t)pl.n * sizeof(CT);  // one buffer, 3 components
+                // twiddles and their conjugates, and 1 KB of the kernel's static line tables
+                const size_t fixed = 2 * (size_t)pl.n * sizeof(CT) + 1024;
+                const int by_lds = kMaxFusedLds > fixed ? (int)((kMaxFusedLds - fixed) / per_line) : 0;
+                const int by_threads = 256 / (3 * pl.radix[1]);  // one inverse stage-1 butterfly per thread
+                // about 6 KB of lines per component and block: more blocks per CU keep
+                // more loads in flight (measured at 256^3: 3 lines 3.41 ms, 5 lines 3.69 ms)
+                ztl = std::min({by_lds, by_threads, std::max(1, 768 / pl.n)});
+            }
+            const bool fusedz = ztl >= 1;
+            const int skip = fusedx ? xaxis : (fusedz ? zaxis : -1);
             {
                 ProfScope ps("fft", s, B);
                 if (lfft && fft_backend == 0)
-                    lfft->exec<R>(grid, kgrid, B, true, s, fusedx ? xaxis : -1);
+                    lfft->exec<R>(grid, kgrid, B, true, s, skip);
                 else
                     fft->exec(grid, bytes_per_rhs, B, fp32, true, s);
             }
-            if (fusedx) {
+            if (fusedz) {
+                ProfScope ps("scale", s, B);
+                const LineAxis& ax = *lfft->axes[zaxis];
+                const int nxy = S.n[0] * S.n[1], nlines = nxy * B;
+                const size_t lds = (3 * (size_t)ztl * ax.plan.n + 2 * (size_t)ax.plan.n) * sizeof(CT);
+                const CT* tw = reinterpret_cast<const CT*>(ax.tw32.p);
+                const R* kc = reinterpret_cast<const R*>(d_k32.p);
+                const dim3 gz((unsigned)((nlines + ztl - 1) / ztl));
+                ps.kernel("zline_scale_kernel", {type_name<R>(), std::to_string(ax.cls)});
+                if (ax.cls == 0)
+                    hipLaunchKernelGGL((zline_scale_kernel<R, 0>), gz, dim3(256), lds, s, grid, nlines, nxy,
+                                       ax.plan.n, ax.plan.radix[0], ax.plan.radix[1], tw, ztl, kc);
+                else if (ax.cls == 1)
+                    hipLaunchKernelGGL((zline_scale_kernel<R, 1>), gz, dim3(256), lds, s, grid, nlines, nxy,
+                                       ax.plan.n, ax.plan.radix[0], ax.plan.radix[1], tw, ztl, kc);
+                else
+                    hipLaunchKernelGGL((zline_scale_kernel<R, 2>), gz, dim3(256), lds, s, grid, nlines, nxy,
+                                       ax.plan.n, ax.plan.radix[0], ax.plan.radix[1], tw, ztl, kc);
+                MPM_HIP(hipGetLastError());
+            } else if (fusedx) {
                 ProfScope ps("zfused", s, B);
                 const CT* T = reinterpret_cast<const CT*>(d_toe32.p);
                 const LineAxis& ax = *lfft->axes[xaxis];
@@ -5709,7 +5892,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
             {
                 ProfScope ps("fft", s, B);
                 if (lfft && fft_backend == 0)
-                    lfft->exec<R>(grid, kgrid, B, false, s, fusedx ? xaxis : -1);
+                    lfft->exec<R>(grid, kgrid, B, false, s, skip);
                 else
                     fft->exec(grid, bytes_per_rhs, B, fp32, false, s);
             }
