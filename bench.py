@@ -162,7 +162,8 @@ def algorithmic_bytes(cls, info, N, B, fp32, pairs):
     occupied z-planes (slab_planes * nx * ny voxels).  "fft_axisK" is one pass of
     the transform (one read and one write of the grid); "contract" reads the grid
     once and produces the reciprocal field of every particle (3 complex FP64 per
-    particle and RHS); "zfused" is the slab Green's-tensor pass (grid read and
+    particle and RHS); "zline" is the full grid's z transforms and Green's-tensor
+    projection in one pass; "zfused" is the slab Green's-tensor pass (grid read and
     write plus its coefficient table); "xtoeplitz" is the same with the slab's x
     transform and inverse x transform fused in (still one read and one write); "gather" reads the dipoles of the
     neighbours and of the particle and writes the result (pair data on the fly
@@ -184,7 +185,7 @@ def algorithmic_bytes(cls, info, N, B, fp32, pairs):
         return 2 * grid + 6 * info.slab_planes * nx * ny * c
     if cls == "zfused":
         return 2 * grid + (6 * info.slab_planes * nx * ny * c if info.slab_planes else 4 * V * r)
-    if cls == "scale":
+    if cls in ("scale", "zline"):  # zline: z transforms + projection, still one read and one write
         return 2 * grid + 4 * V * r
     if cls == "contract":
         return grid + 48 * N * B + N * (3 * W * r + 12)
@@ -194,13 +195,14 @@ def algorithmic_bytes(cls, info, N, B, fp32, pairs):
 
 
 def kernel_class(name):
-    """Kernel class of a kernel name as profilers print it (template arguments included)."""
+    """Kernel class of a kernel name as profilers print it (template arguments included);
+    "other" (no algorithmic byte model) for kernels outside the dipole operator's path."""
     base = name.split("<")[0]
     return {"spread_mfma_kernel": "spread", "spread_kernel": "spread", "line_fft2_kernel": "fft_axis",
             "line_fft_kernel": "fft_axis", "ztoeplitz_kernel": "zfused", "zfused_kernel": "zfused", "xtoeplitz_kernel": "xtoeplitz",
             "scale_kernel": "scale", "contract_mfma_kernel": "contract", "contract_nb_mfma_kernel": "contract",
-            "contract_full_mfma_kernel": "contract",
-            "finish_kernel": "gather", "gather_kernel": "gather"}[base]
+            "contract_full_mfma_kernel": "contract", "zline_scale_kernel": "zline",
+            "finish_kernel": "gather", "gather_kernel": "gather"}.get(base, "other")
 
 
 def kernel_bytes(name, info, N, B, pairs):

@@ -1370,7 +1370,7 @@ __global__ __launch_bounds__(256) void line_fft2_kernel(const typename C2<R>::T*
 // butterfly per thread: 3 TL P2 <= 256), inverse stage 2 from y to the grid.  kc
 // holds (k^x, k^y, k^z, sc) per voxel.
 template <typename R, int CLS>
-__global__ __launch_bounds__(256) void zline_scale_kernel(typename C2<R>::T* grid, int nlines, int nxy, int n,
+__global__ __launch_bounds__(256) void zline_scale_kernel(typename C2<R>::T* grid, int B, int nxy, int n,
                                                           int P1, int P2,
                                                           const typename C2<R>::T* __restrict__ twiddle, int TL,
                                                           const R* __restrict__ kc) {
@@ -1382,7 +1382,7 @@ __global__ __launch_bounds__(256) void zline_scale_kernel(typename C2<R>::T* gri
     CT* const twi = tw + n;
     const int tid = threadIdx.x;
     const int l0 = blockIdx.x * TL;
-    const int nl = min(TL, nlines - l0);
+    const int nl = min(TL, B * nxy - l0);
     const int nt = 3 * nl;  // block lines t = c nl + li
     for (int k = tid; k < n; k += blockDim.x) {
         const CT w = twiddle[k];
@@ -1391,11 +1391,13 @@ __global__ __launch_bounds__(256) void zline_scale_kernel(typename C2<R>::T* gri
         wc.y = -wc.y;
         twi[k] = wc;
     }
+    // line L = xy B + b: the right-hand sides of one (x, y) follow each other, so
+    // that its k-space coefficients are read from memory once, not once per b
     __shared__ int s_base[3 * 64];  // grid offset of each block line
     __shared__ int s_vox[64];       // voxel (x, y, 0) of each line
     if (tid < nt) {
         const int c = tid / nl, li = tid - c * nl;
-        const int L = l0 + li, b = L / nxy, xy = L - b * nxy;
+        const int L = l0 + li, xy = L / B, b = L - xy * B;
         s_base[tid] = ((b * 3 + c) * nxy + xy) * n;
         if (c == 0) s_vox[li] = xy * n;
     }
@@ -3427,7 +3429,8 @@ __global__ __launch_bounds__(256) void gather_kernel(
     }
 }
 
-// kFinishLanes lanes per (particle i, right-hand side b), lanes then b fastest:
+// kFinishLanes lanes per (particle i, right-hand side b), lanes then b fastest,
+// the particles in the spatial visiting order of the wave-per-particle kernels:
 // the reciprocal part as the sum of the particle's region contractions, the
 // real-space part over the neighbour list with the precomputed pair
 // coefficients (ux, uy, uz, perp, para - perp, cos, sin) -- the field of p_j at
@@ -3446,7 +3449,7 @@ __global__ __launch_bounds__(256) void finish_kernel(SlotTable tab, int B, int p
                                                      const int* __restrict__ nb_start, const int* __restrict__ nb_j,
                                                      const double* __restrict__ pairc,
                                                      const double2* __restrict__ selfc, double self_perp,
-                                                     int jacobi) {
+                                                     int jacobi, const int* __restrict__ order) {
     constexpr int NL = kFinishLanes;
     __shared__ const double2* s_x[kMaxSlots];
     __shared__ double2* s_y[kMaxSlots];
@@ -3461,8 +3464,10 @@ __global__ __launch_bounds__(256) void finish_kernel(SlotTable tab, int B, int p
     __syncthreads();
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int item = t / NL, sub = t % NL;  // NL divides the wave: the lanes of an item share it
-    const int i = item / B, b = item - i * B;
-    if (i >= N) return;  // whole items only: all lanes of an item leave together
+    const int ii = item / B, b = item - ii * B;
+    if (ii >= N) return;  // whole items only: all lanes of an item leave together
+    // particles in spatial order: neighbouring items share neighbours' dipoles and pair data
+    const int i = order[ii];
     const double2* xb = s_x[b];
     double ar[3] = {0.0, 0.0, 0.0}, ai[3] = {0.0, 0.0, 0.0};
     if (parts & MPM_PART_RECIP) {
@@ -5771,13 +5776,13 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
                 const dim3 gz((unsigned)((nlines + ztl - 1) / ztl));
                 ps.kernel("zline_scale_kernel", {type_name<R>(), std::to_string(ax.cls)});
                 if (ax.cls == 0)
-                    hipLaunchKernelGGL((zline_scale_kernel<R, 0>), gz, dim3(256), lds, s, grid, nlines, nxy,
+                    hipLaunchKernelGGL((zline_scale_kernel<R, 0>), gz, dim3(256), lds, s, grid, B, nxy,
                                        ax.plan.n, ax.plan.radix[0], ax.plan.radix[1], tw, ztl, kc);
                 else if (ax.cls == 1)
-                    hipLaunchKernelGGL((zline_scale_kernel<R, 1>), gz, dim3(256), lds, s, grid, nlines, nxy,
+                    hipLaunchKernelGGL((zline_scale_kernel<R, 1>), gz, dim3(256), lds, s, grid, B, nxy,
                                        ax.plan.n, ax.plan.radix[0], ax.plan.radix[1], tw, ztl, kc);
                 else
-                    hipLaunchKernelGGL((zline_scale_kernel<R, 2>), gz, dim3(256), lds, s, grid, nlines, nxy,
+                    hipLaunchKernelGGL((zline_scale_kernel<R, 2>), gz, dim3(256), lds, s, grid, B, nxy,
                                        ax.plan.n, ax.plan.radix[0], ax.plan.radix[1], tw, ztl, kc);
                 MPM_HIP(hipGetLastError());
             } else if (fusedx) {
@@ -5965,7 +5970,8 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
         const double h3 = S.h[0] * S.h[1] * S.h[2];
         const int blocks = (int)(((size_t)Ni * B * kFinishLanes + 255) / 256);
         hipLaunchKernelGGL((finish_kernel<R>), dim3(blocks), dim3(256), 0, s, t, B, parts, Ni, part, d_ps_start.p,
-                           d_ps_items.p, h3, d_nb_start.p, d_nb_j.p, d_pairc.p, d_selfc.p, self_perp, jacobi);
+                           d_ps_items.p, h3, d_nb_start.p, d_nb_j.p, d_pairc.p, d_selfc.p, self_perp, jacobi,
+                           d_order.p);
         MPM_HIP(hipGetLastError());
     } else {
         ProfScope ps("gather", s, B);
