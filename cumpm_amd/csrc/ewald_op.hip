@@ -1295,7 +1295,11 @@ __global__ __launch_bounds__(256) void line_fft2_kernel(const typename C2<R>::T*
     using CT = typename C2<R>::T;
     extern __shared__ unsigned char s_raw[];
     CT* const y = reinterpret_cast<CT*>(s_raw);
-    CT* tw = y + TL * n;
+    // LDS rows of P1 + 1 and an odd line pitch: stage 1 writes rows j (or lines, when
+    // lanes run over lines) of consecutive lanes without bank conflicts (a pitch of 16
+    // complex would put them all on one bank set)
+    const int ln = (n + P2) | 1;  // one line: P2 rows of P1 + 1, padded to odd
+    CT* tw = y + TL * ln;
     const int tid = threadIdx.x;
     const int l0 = blockIdx.x * TL;
     const int nl = min(TL, (int)gin.nlines - l0);
@@ -1320,7 +1324,7 @@ __global__ __launch_bounds__(256) void line_fft2_kernel(const typename C2<R>::T*
             int li, j;
             if (esi == 1) fdivmod(w, P2, rm, li, j);
             else fdivmod(w, nl, rnl, j, li);
-            const int ib = s_lbi[li] + j * esi, yb = li * n + j * P1;
+            const int ib = s_lbi[li] + j * esi, yb = li * ln + j * (P1 + 1);
             switch (P1) {
 #define MPM_S1(P, C)                                                                 \
     case P:                                                                          \
@@ -1342,11 +1346,11 @@ __global__ __launch_bounds__(256) void line_fft2_kernel(const typename C2<R>::T*
             int li, j;
             if (eso == 1) fdivmod(w, P1, rm, li, j);
             else fdivmod(w, nl, rnl, j, li);
-            const int yb = li * n + j, ob = s_lbo[li] + j * eso;
+            const int yb = li * ln + j, ob = s_lbo[li] + j * eso;
             switch (P2) {
 #define MPM_S2(P, C)                                                                       \
     case P:                                                                                \
-        if constexpr (CLS >= C) fft2_s2<R, P, INV>(y, yb, P1, tw, j, out, ob, P1 * eso);   \
+        if constexpr (CLS >= C) fft2_s2<R, P, INV>(y, yb, P1 + 1, tw, j, out, ob, P1 * eso); \
         break;
                 MPM_FFT2_CASES(MPM_S2)
 #undef MPM_S2
@@ -1354,6 +1358,91 @@ __global__ __launch_bounds__(256) void line_fft2_kernel(const typename C2<R>::T*
                     break;
             }
         }
+    }
+}
+
+// Persistent form of line_fft2_kernel for plans where each stage is one butterfly
+// per thread (TL P2 <= 256 and TL P1 <= 256): a block walks the sets of TL lines
+// blockIdx.x, blockIdx.x + gridDim.x, ..., and issues the stage-1 loads of its next
+// set into registers before the current set's stage 2 (LDS -> DFT -> global), so
+// that the loads of one set are in flight while the previous set is finished and
+// stored.
+template <typename R, bool INV, int CLS>
+__global__ __launch_bounds__(256) void line_fft2p_kernel(const typename C2<R>::T* in, typename C2<R>::T* out,
+                                                         LineGeom gin, LineGeom gout, int n, int P1, int P2,
+                                                         const typename C2<R>::T* __restrict__ twiddle, int TL,
+                                                         int nsets) {
+    using CT = typename C2<R>::T;
+    constexpr int PMAX = CLS == 0 ? 8 : (CLS == 1 ? 16 : kMaxPrimeRadix);
+    extern __shared__ unsigned char s_raw[];
+    CT* const y = reinterpret_cast<CT*>(s_raw);
+    const int ln = (n + P2) | 1;  // one line in LDS: P2 rows of P1 + 1, odd pitch (no bank conflicts)
+    CT* tw = y + TL * ln;
+    const int tid = threadIdx.x;
+    for (int k = tid; k < n; k += blockDim.x) {
+        CT w = twiddle[k];
+        if (INV) w.y = -w.y;
+        tw[k] = w;
+    }
+    const int esi = (int)gin.elem_stride, eso = (int)gout.elem_stride;
+    const int nlines = (int)gin.nlines;
+    // this thread's butterflies: stage 1 (line li1, j1 < P2), stage 2 (line li2, a2 < P1),
+    // lines fastest when the elements are strided (consecutive lines adjacent in memory)
+    int li1, j1, li2, a2;
+    if (esi == 1) fdivmod(tid, P2, 1.0f / (float)P2, li1, j1);
+    else fdivmod(tid, TL, 1.0f / (float)TL, j1, li1);
+    if (eso == 1) fdivmod(tid, P1, 1.0f / (float)P1, li2, a2);
+    else fdivmod(tid, TL, 1.0f / (float)TL, a2, li2);
+    const bool in1 = tid < TL * P2, in2 = tid < TL * P1;
+    CT v[PMAX];
+    auto load = [&](int set) {
+        const int l = set * TL + li1;
+        if (in1 && l < nlines) {
+            const int ib = line_base32(gin, l) + j1 * esi;
+#pragma unroll
+            for (int r = 0; r < PMAX; ++r)
+                if (r < P1) v[r] = in[ib + r * P2 * esi];
+        }
+    };
+    int set = blockIdx.x;
+    if (set < nsets) load(set);
+    __syncthreads();  // twiddles
+    for (; set < nsets; set += gridDim.x) {
+        const int l0 = set * TL;
+        // stage 1: DFT of the loaded inputs into LDS
+        if (in1 && l0 + li1 < nlines) {
+            switch (P1) {
+#define MPM_S1(P, C)                                                            \
+    case P:                                                                     \
+        if constexpr (CLS >= C && P <= PMAX) {                                  \
+            dft_reg<R, P, INV>(v);                                              \
+            _Pragma("unroll") for (int q = 0; q < P; ++q) y[li1 * ln + j1 * (P + 1) + q] = v[q]; \
+        }                                                                       \
+        break;
+                MPM_FFT2_CASES(MPM_S1)
+#undef MPM_S1
+                default:
+                    break;
+            }
+        }
+        __syncthreads();
+        // the next set's loads, in flight during this set's stage 2
+        if (set + (int)gridDim.x < nsets) load(set + gridDim.x);
+        // stage 2: LDS -> DFT -> output
+        if (in2 && l0 + li2 < nlines) {
+            const int ob = line_base32(gout, l0 + li2) + a2 * eso;
+            switch (P2) {
+#define MPM_S2(P, C)                                                                          \
+    case P:                                                                                   \
+        if constexpr (CLS >= C) fft2_s2<R, P, INV>(y, li2 * ln + a2, P1 + 1, tw, a2, out, ob, P1 * eso); \
+        break;
+                MPM_FFT2_CASES(MPM_S2)
+#undef MPM_S2
+                default:
+                    break;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -1377,8 +1466,11 @@ __global__ __launch_bounds__(256) void zline_scale_kernel(typename C2<R>::T* gri
     using CT = typename C2<R>::T;
     constexpr int PMAX = CLS == 0 ? 8 : (CLS == 1 ? 16 : kMaxPrimeRadix);
     extern __shared__ unsigned char s_raw[];
-    CT* const y = reinterpret_cast<CT*>(s_raw);  // [3 TL][n]
-    CT* const tw = y + 3 * TL * n;               // forward twiddles, then their conjugates
+    // a line in LDS is P2 rows of P1 + 1 (padded: no bank conflicts in stage 1); after
+    // the forward transform element k of a line sits at row k / P1, column k % P1
+    const int ln = (n + P2) | 1;  // odd pitch: lines of consecutive lanes on distinct banks
+    CT* const y = reinterpret_cast<CT*>(s_raw);  // [3 TL][ln]
+    CT* const tw = y + 3 * TL * ln;              // forward twiddles, then their conjugates
     CT* const twi = tw + n;
     const int tid = threadIdx.x;
     const int l0 = blockIdx.x * TL;
@@ -1410,7 +1502,7 @@ __global__ __launch_bounds__(256) void zline_scale_kernel(typename C2<R>::T* gri
         switch (P1) {
 #define MPM_S1(P, C)                                                                     \
     case P:                                                                              \
-        if constexpr (CLS >= C) fft2_s1<R, P, false>(grid, s_base[t] + j, P2, y, t * n + j * P1); \
+        if constexpr (CLS >= C) fft2_s1<R, P, false>(grid, s_base[t] + j, P2, y, t * ln + j * (P1 + 1)); \
         break;
             MPM_FFT2_CASES(MPM_S1)
 #undef MPM_S1
@@ -1419,14 +1511,15 @@ __global__ __launch_bounds__(256) void zline_scale_kernel(typename C2<R>::T* gri
         }
     }
     __syncthreads();
-    // forward stage 2: butterflies a < P1 of radix P2, in place in y (natural order)
+    // forward stage 2: butterflies a < P1 of radix P2, in place in y (element k at row
+    // k / P1, column k % P1)
     for (int w = tid; w < nt * P1; w += blockDim.x) {
         int t, a;
         fdivmod(w, P1, r1, t, a);
         switch (P2) {
 #define MPM_S2(P, C)                                                                           \
     case P:                                                                                    \
-        if constexpr (CLS >= C) fft2_s2<R, P, false>(y, t * n + a, P1, tw, a, y, t * n + a, P1); \
+        if constexpr (CLS >= C) fft2_s2<R, P, false>(y, t * ln + a, P1 + 1, tw, a, y, t * ln + a, P1 + 1); \
         break;
             MPM_FFT2_CASES(MPM_S2)
 #undef MPM_S2
@@ -1439,11 +1532,13 @@ __global__ __launch_bounds__(256) void zline_scale_kernel(typename C2<R>::T* gri
     for (int w = tid; w < nl * n; w += blockDim.x) {
         const int li = w / n, k = w - li * n;
         const int v = s_vox[li] + k;
+        int kq, ka;
+        fdivmod(k, P1, r1, kq, ka);
         const R kx = kc[4 * (size_t)v], ky = kc[4 * (size_t)v + 1], kz = kc[4 * (size_t)v + 2],
                 sc = kc[4 * (size_t)v + 3];
-        CT* f0 = y + li * n + k;
-        CT* f1 = f0 + nl * n;
-        CT* f2 = f1 + nl * n;
+        CT* f0 = y + li * ln + kq * (P1 + 1) + ka;
+        CT* f1 = f0 + nl * ln;
+        CT* f2 = f1 + nl * ln;
         const CT a0 = *f0, a1 = *f1, a2 = *f2;
         const R dr = (a0.x * kx + a1.x * ky + a2.x * kz) * sc;
         const R di = (a0.y * kx + a1.y * ky + a2.y * kz) * sc;
@@ -1468,7 +1563,11 @@ __global__ __launch_bounds__(256) void zline_scale_kernel(typename C2<R>::T* gri
         CT v[PMAX];
 #pragma unroll
         for (int r = 0; r < PMAX; ++r)
-            if (act && r < P1) v[r] = y[t * n + j + r * P2];
+            if (act && r < P1) {
+                int kq, ka;
+                fdivmod(j + r * P2, P1, r1, kq, ka);
+                v[r] = y[t * ln + kq * (P1 + 1) + ka];
+            }
         __syncthreads();
         if (act) {
             switch (P1) {
@@ -1476,7 +1575,7 @@ __global__ __launch_bounds__(256) void zline_scale_kernel(typename C2<R>::T* gri
     case P:                                                                 \
         if constexpr (CLS >= C && P <= PMAX) {                              \
             dft_reg<R, P, true>(v);                                         \
-            _Pragma("unroll") for (int q = 0; q < P; ++q) y[t * n + j * P1 + q] = v[q]; \
+            _Pragma("unroll") for (int q = 0; q < P; ++q) y[t * ln + j * (P1 + 1) + q] = v[q]; \
         }                                                                   \
         break;
                 MPM_FFT2_CASES(MPM_S1)
@@ -1494,7 +1593,7 @@ __global__ __launch_bounds__(256) void zline_scale_kernel(typename C2<R>::T* gri
         switch (P2) {
 #define MPM_S2(P, C)                                                                                   \
     case P:                                                                                            \
-        if constexpr (CLS >= C) fft2_s2<R, P, true>(y, t * n + a, P1, twi, a, grid, s_base[t] + a, P1); \
+        if constexpr (CLS >= C) fft2_s2<R, P, true>(y, t * ln + a, P1 + 1, twi, a, grid, s_base[t] + a, P1); \
         break;
             MPM_FFT2_CASES(MPM_S2)
 #undef MPM_S2
@@ -1585,6 +1684,20 @@ struct LinePass {
     int src, dst;
 };
 
+// Compute units of the current device (cached per device).
+static int device_cus() {
+    static int cached[64] = {0};
+    int dev = 0;
+    MPM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return 256;
+    if (cached[dev] == 0) {
+        int cus = 0;
+        MPM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        cached[dev] = std::max(cus, 1);
+    }
+    return cached[dev];
+}
+
 struct LineFft {
     std::vector<std::unique_ptr<LineAxis>> axes;
     std::vector<LinePass> fwd, inv;
@@ -1655,8 +1768,30 @@ struct LineFft {
             CT* out = ps.dst ? kgrid : grid;
             if (ax.two) {
                 const int TL = lines_per_block2<R>(n, ax.plan.radix[0], ax.plan.radix[1]);
-                const size_t lds = ((size_t)TL * n + n) * sizeof(CT);
-                const dim3 gd((unsigned)((gi.nlines + TL - 1) / TL));
+                const size_t lds = ((size_t)TL * ((n + ax.plan.radix[1]) | 1) + n) * sizeof(CT);  // padded rows
+                const int nsets = (int)((gi.nlines + TL - 1) / TL);
+                if (TL * ax.plan.radix[1] <= 256 && TL * ax.plan.radix[0] <= 256) {
+                    // one butterfly per thread and stage: persistent blocks, next set prefetched
+                    const int bpc = std::max(1, std::min(8, (int)((160 * 1024) / lds)));
+                    const dim3 gp((unsigned)std::min(nsets, device_cus() * bpc));
+                    pscope.kernel("line_fft2p_kernel", {type_name<R>(), tbool(!forward), std::to_string(ax.cls)});
+#define MPM_LP(INV, C)                                                                                         \
+    hipLaunchKernelGGL((line_fft2p_kernel<R, INV, C>), gp, dim3(256), lds, s, in, out, gi, go, n, ax.plan.radix[0], \
+                       ax.plan.radix[1], tw, TL, nsets)
+                    if (forward) {
+                        if (ax.cls == 0) MPM_LP(false, 0);
+                        else if (ax.cls == 1) MPM_LP(false, 1);
+                        else MPM_LP(false, 2);
+                    } else {
+                        if (ax.cls == 0) MPM_LP(true, 0);
+                        else if (ax.cls == 1) MPM_LP(true, 1);
+                        else MPM_LP(true, 2);
+                    }
+#undef MPM_LP
+                    MPM_HIP(hipGetLastError());
+                    continue;
+                }
+                const dim3 gd((unsigned)nsets);
                 pscope.kernel("line_fft2_kernel", {type_name<R>(), tbool(!forward), std::to_string(ax.cls)});
 #define MPM_L2(INV)                                                                       \
     if (ax.cls == 0) launch2<R, INV, 0>(gd, lds, s, in, out, gi, go, ax.plan, tw, TL);    \
@@ -5748,7 +5883,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
             int ztl = 0;
             if (zaxis >= 0 && lfft->axes[zaxis]->two) {
                 const LinePlan& pl = lfft->axes[zaxis]->plan;
-                const size_t per_line = 3 * (size_t)pl.n * sizeof(CT);  // one buffer, 3 components
+                const size_t per_line = 3 * (size_t)((pl.n + pl.radix[1]) | 1) * sizeof(CT);  // padded, 3 components
                 // twiddles and their conjugates, and 1 KB of the kernel's static line tables
                 const size_t fixed = 2 * (size_t)pl.n * sizeof(CT) + 1024;
                 const int by_lds = kMaxFusedLds > fixed ? (int)((kMaxFusedLds - fixed) / per_line) : 0;
@@ -5770,7 +5905,8 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
                 ProfScope ps("scale", s, B);
                 const LineAxis& ax = *lfft->axes[zaxis];
                 const int nxy = S.n[0] * S.n[1], nlines = nxy * B;
-                const size_t lds = (3 * (size_t)ztl * ax.plan.n + 2 * (size_t)ax.plan.n) * sizeof(CT);
+                const size_t lds =
+                    (3 * (size_t)ztl * ((ax.plan.n + ax.plan.radix[1]) | 1) + 2 * (size_t)ax.plan.n) * sizeof(CT);
                 const CT* tw = reinterpret_cast<const CT*>(ax.tw32.p);
                 const R* kc = reinterpret_cast<const R*>(d_k32.p);
                 const dim3 gz((unsigned)((nlines + ztl - 1) / ztl));
