@@ -5240,6 +5240,68 @@ void EwaldOp::build_pair_coefs() {
 
 // LDS bytes of zfused_kernel for slab size zs_pad and precision size rs.
 static size_t zfused_lds(int zs_pad, int nz, size_t rs) { return 2 * rs * zs_pad * nz; }
+// Region-local separable weight rows of the matrix-core kernels, one block per
+// region (grid-stride), threads over the (candidate, entry) pairs of its slots:
+// entry e of a row is the weight of axis d at voxel v summed over the stencil
+// points of the particle that land on v (sum over o = off, off + n_d, ... < W of
+// its per-axis weights, off = (v - (base - half)) mod n_d), or zero outside the
+// grid and in the row's padding.
+struct RegionWeightGeom {
+    int nreg, RX, npb, nry, pblk, arow, zs, NW, zsb4, W, half, N;
+    int n[3];
+};
+
+__global__ __launch_bounds__(256) void region_weights_kernel(RegionWeightGeom g, const int* __restrict__ reg_start,
+                                                             const int* __restrict__ reg_items,
+                                                             const int* __restrict__ base,
+                                                             const double* __restrict__ w64,
+                                                             const int* __restrict__ zlist, double* __restrict__ rw64,
+                                                             float* __restrict__ rw32) {
+    for (int r = blockIdx.x; r < g.nreg; r += gridDim.x) {
+        const int per_x = g.npb * g.nry;
+        const int x0 = (r / per_x) * g.RX, pl0 = ((r % per_x) / g.nry) * g.pblk, y0 = (r % g.nry) * kMfmaRY;
+        const int k0 = reg_start[r], k1 = reg_start[r + 1];
+        for (int e = threadIdx.x; e < (k1 - k0) * g.NW; e += blockDim.x) {
+            const int q = k0 + e / g.NW, col = e % g.NW;
+            const int pid = reg_items[q];
+            int d = -1, v = 0;
+            if (col < 4) {
+                if (col < g.RX && x0 + col < g.n[0]) {
+                    d = 0;
+                    v = x0 + col;
+                }
+            } else if (col < 4 + g.zsb4) {
+                const int p = col - 4;
+                if (g.zs > 0) {
+                    if (p < g.zs) {
+                        d = 2;
+                        v = zlist[p];
+                    }
+                } else if (p < g.pblk && pl0 + p < g.n[1]) {
+                    d = 1;
+                    v = pl0 + p;
+                }
+            } else {
+                const int j = col - 4 - g.zsb4;
+                if (y0 + j < g.n[g.arow]) {
+                    d = g.arow;
+                    v = y0 + j;
+                }
+            }
+            double acc = 0.0;
+            if (d >= 0) {
+                const int nd = g.n[d];
+                const int b0 = base[(size_t)d * g.N + pid] - g.half;
+                int off = (v - b0) % nd;
+                if (off < 0) off += nd;
+                for (int o = off; o < g.W; o += nd) acc += w64[((size_t)pid * 3 + d) * g.W + o];
+            }
+            rw64[(size_t)q * g.NW + col] = acc;
+            rw32[(size_t)q * g.NW + col] = (float)acc;
+        }
+    }
+}
+
 static int zs_bucket(int Zs) {
     static const int buckets[] = {4, 6, 8, 10, 12, 14, 16};
     for (int b : buckets)
@@ -5367,15 +5429,28 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
         reg_npb = slab ? 1 : (S.n[1] + pblk - 1) / pblk;
         const int nreg = reg_nrx * reg_npb * reg_nry;
         // distinct blocks of size R that the stencil of particle i meets along axis d
+        // (walking its W voxels with the periodic wrap; one division per axis)
         auto blocks_of = [&](int i, int d, int R, int* outv) {
             int n = 0;
             const int nd = S.n[d];
-            for (int o = -half; o <= half; ++o) {
-                const int v = (((base[(size_t)d * Nn + i] + o) % nd) + nd) % nd;
-                const int r = v / R;
-                bool seen = false;
-                for (int q = 0; q < n; ++q) seen = seen || outv[q] == r;
-                if (!seen) outv[n++] = r;
+            int v = (base[(size_t)d * Nn + i] - half) % nd;
+            if (v < 0) v += nd;
+            int r = v / R, next = (r + 1) * R, last = -1;
+            for (int o = 0; o < W; ++o) {
+                if (r != last) {
+                    bool seen = false;
+                    for (int q = 0; q < n; ++q) seen = seen || outv[q] == r;
+                    if (!seen) outv[n++] = r;
+                    last = r;
+                }
+                if (++v == nd) {
+                    v = 0;
+                    r = 0;
+                    next = R;
+                } else if (v == next) {
+                    ++r;
+                    next += R;
+                }
             }
             return n;
         };
@@ -5390,15 +5465,27 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
                 for (int c = 0; c < npb; ++c)
                     for (int e = 0; e < nry; ++e) f(((size_t)rxs[a] * reg_npb + pbs[c]) * reg_nry + rys[e]);
         };
-        // candidates of every region in ascending particle order (count, then fill)
+        // candidates of every region in ascending particle order: each host thread counts
+        // its contiguous chunk of particles, then fills after the chunks before it
+        const int nth = host_threads(Nn, 16);
+        std::vector<std::vector<int>> cnt(nth, std::vector<int>(nreg, 0));
+        parallel_for(Nn, nth, [&](int i0, int i1, int t) {
+            for (int i = i0; i < i1; ++i) for_regions(i, [&](size_t r) { cnt[t][r]++; });
+        });
         std::vector<int> rs(nreg + 1, 0);
-        for (int i = 0; i < Nn; ++i) for_regions(i, [&](size_t r) { rs[r + 1]++; });
-        for (int r = 0; r < nreg; ++r) rs[r + 1] += rs[r];
-        std::vector<int> ri(std::max(rs[nreg], 1));
-        {
-            std::vector<int> fill(rs.begin(), rs.end() - 1);
-            for (int i = 0; i < Nn; ++i) for_regions(i, [&](size_t r) { ri[fill[r]++] = i; });
+        for (int r = 0; r < nreg; ++r) {
+            int acc = rs[r];
+            for (int t = 0; t < nth; ++t) {
+                const int c = cnt[t][r];
+                cnt[t][r] = acc;  // this chunk's first position in region r
+                acc += c;
+            }
+            rs[r + 1] = acc;
         }
+        std::vector<int> ri(std::max(rs[nreg], 1));
+        parallel_for(Nn, nth, [&](int i0, int i1, int t) {
+            for (int i = i0; i < i1; ++i) for_regions(i, [&](size_t r) { ri[cnt[t][r]++] = i; });
+        });
         // slots of each particle, in ascending region order
         std::vector<int> pss(Nn + 1, 0), psi(std::max(rs[nreg], 1));
         for (int q = 0; q < rs[nreg]; ++q) pss[ri[q] + 1]++;
@@ -5407,50 +5494,21 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
             std::vector<int> fill(pss.begin(), pss.end() - 1);
             for (int q = 0; q < rs[nreg]; ++q) psi[fill[ri[q]]++] = q;
         }
-        d_reg_start.upload(rs.data(), rs.size());
-        d_reg_items.upload(ri.data(), ri.size());
-        d_ps_start.upload(pss.data(), pss.size());
-        d_ps_items.upload(psi.data(), psi.size());
-        d_partial.resize((size_t)std::max(rs[nreg], 1) * 6 * kMaxSlots);
-        d_partial64.resize((size_t)std::max(rs[nreg], 1) * 6 * kMaxSlots);
-        // region-local separable weights of every (region, candidate) slot, in the
-        // row layout of the kernels: x at [0, RX), planes at [4, 4 + ZSB), rows at
-        // [4 + ZSB4, 4 + ZSB4 + 16); sums over stencil points that fold
-        const int zsb = slab ? zs_bucket(Zs) : kFullRegionPlanes, zsb4 = (zsb + 3) & ~3;
-        const int NW = 4 + zsb4 + RY, X = 0, Zo = 4, Yo = 4 + zsb4;
-        std::vector<double> rw64((size_t)std::max(rs[nreg], 1) * NW, 0.0);
-        auto wsum = [&](int pid, int d, int v) {
-            const int nd = S.n[d];
-            const int b0 = base[(size_t)d * Nn + pid] - half;
-            int off = (v - b0) % nd;
-            if (off < 0) off += nd;
-            double acc = 0.0;
-            for (int o = off; o < W; o += nd) acc += w64[((size_t)pid * 3 + d) * W + o];
-            return acc;
-        };
-        parallel_for(nreg, host_threads(nreg, 16), [&](int r0, int r1, int) {
-            for (int r = r0; r < r1; ++r) {
-                const int per_x = reg_npb * reg_nry;
-                const int x0 = (r / per_x) * RX, pl0 = ((r % per_x) / reg_nry) * pblk,
-                          y0 = (r % reg_nry) * RY;
-                for (int q = rs[r]; q < rs[r + 1]; ++q) {
-                    const int pid = ri[q];
-                    double* row = &rw64[(size_t)q * NW];
-                    for (int j = 0; j < RX; ++j)
-                        if (x0 + j < S.n[0]) row[X + j] = wsum(pid, 0, x0 + j);
-                    if (slab) {
-                        for (int p = 0; p < Zs; ++p) row[Zo + p] = wsum(pid, 2, zl[p]);
-                    } else {
-                        for (int p = 0; p < pblk && pl0 + p < S.n[1]; ++p) row[Zo + p] = wsum(pid, 1, pl0 + p);
-                    }
-                    for (int j = 0; j < RY; ++j)
-                        if (y0 + j < S.n[arow]) row[Yo + j] = wsum(pid, arow, y0 + j);
-                }
-            }
-        });
-        std::vector<float> rw32(rw64.begin(), rw64.end());
-        d_regw64.upload(rw64.data(), rw64.size());
-        d_regw32.upload(rw32.data(), rw32.size());
+        laps.lap("host_setup_region_lists");
+        // grow-only: the slot count changes a little from frame to frame
+        d_reg_start.assign(rs.data(), rs.size());
+        d_reg_items.assign(ri.data(), ri.size());
+        d_ps_start.assign(pss.data(), pss.size());
+        d_ps_items.assign(psi.data(), psi.size());
+        // one buffer for both precisions (an apply runs in one of them at a time), with an
+        // eighth of headroom so that the next frames' slot counts fit without reallocation
+        {
+            const size_t need = (size_t)std::max(rs[nreg], 1) * 6 * kMaxSlots;
+            if (need > d_partial64.n) d_partial64.reserve(need + need / 8);
+        }
+        reg_slots = rs[nreg];
+        // the region-local weight rows are computed on the device once the per-particle
+        // weights are there (region_weights_kernel, below)
     }
     laps.lap("host_setup_regions");
     std::vector<int> ztl;  // z-tiles that hold at least one computed voxel
@@ -5630,6 +5688,35 @@ void EwaldOp::set_positions(const double* xyz, int64_t n) {
     d_zmap.upload(zmap.data(), zmap.size());
     if (zl.empty()) zl.push_back(0);
     d_zlist.upload(zl.data(), zl.size());
+    if (reg_rx > 0) {
+        // region-local separable weights of every (region, candidate) slot, in the row
+        // layout of the matrix-core kernels: x at [0, RX), planes at [4, 4 + ZSB), rows at
+        // [4 + ZSB4, 4 + ZSB4 + 16); sums over stencil points that fold
+        const int zsb = slab ? zs_bucket(Zs) : kFullRegionPlanes, zsb4 = (zsb + 3) & ~3;
+        const int NW = 4 + zsb4 + kMfmaRY;
+        const size_t nw = (size_t)std::max(reg_slots, 1) * NW;
+        if (nw > d_regw64.n) d_regw64.reserve(nw + nw / 8);
+        if (nw > d_regw32.n) d_regw32.reserve(nw + nw / 8);
+        RegionWeightGeom wg;
+        wg.nreg = reg_nrx * reg_npb * reg_nry;
+        wg.RX = reg_rx;
+        wg.npb = reg_npb;
+        wg.nry = reg_nry;
+        wg.pblk = reg_pblk;
+        wg.arow = slab ? 1 : 2;
+        wg.zs = slab ? Zs : 0;
+        wg.NW = NW;
+        wg.zsb4 = zsb4;
+        wg.W = W;
+        wg.half = half;
+        wg.N = Nn;
+        for (int d = 0; d < 3; ++d) wg.n[d] = S.n[d];
+        const int blocks = std::min(wg.nreg, 65535);
+        hipLaunchKernelGGL(region_weights_kernel, dim3((unsigned)std::max(blocks, 1)), dim3(256), 0, nullptr, wg,
+                           d_reg_start.p, d_reg_items.p, d_base.p, d_w64.p, d_zlist.p, d_regw64.p, d_regw32.p);
+        MPM_HIP(hipGetLastError());
+        MPM_HIP(hipStreamSynchronize(nullptr));
+    }
     if (!dev_pairs) {
         d_nb_start.upload(nstart.data(), nstart.size());
         if (nj.empty()) {
@@ -6050,7 +6137,7 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
         const int ncg = ngrp >= 4 ? 4 : (ngrp >= 2 ? 2 : 1);
         const dim3 gs((unsigned)nreg, (unsigned)((ngrp + ncg - 1) / ncg));
         const dim3 gn((unsigned)nreg, (unsigned)ngrp);
-        R* pbuf = fp32 ? reinterpret_cast<R*>(d_partial.p) : reinterpret_cast<R*>(d_partial64.p);
+        R* pbuf = reinterpret_cast<R*>(d_partial64.p);  // sized for FP64, shared by FP32
         if (!SLAB && vec) {
             ps.kernel("contract_full_mfma_kernel",
                       {type_name<R>(), "4", std::to_string(kFullRegionPlanes), std::to_string(kFullContractNB)});

@@ -111,6 +111,13 @@ struct DevBuf {
         if (count) MPM_HIP(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
         if (count) MPM_HIP(hipStreamSynchronize(s));
     }
+    // upload into a grow-only allocation (n is then the capacity): per-frame data whose
+    // size changes a little from frame to frame keeps its memory
+    void assign(const T* host, size_t count, hipStream_t s = nullptr) {
+        reserve(count);
+        if (count) MPM_HIP(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+        if (count) MPM_HIP(hipStreamSynchronize(s));
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -334,14 +341,14 @@ class EwaldOp {
     // blocks per x block of regions
     int reg_rx = 0, reg_nrx = 0, reg_nry = 0, reg_npb = 1, reg_pblk = 0;
     bool reg_full = false;       // the regions tile the full grid (monodisperse dipoles)
+    int reg_slots = 0;           // (region, candidate) slots of the current frame
     DevBuf<int> d_reg_start;
     DevBuf<int> d_reg_items;
     DevBuf<int> d_ps_start;      // [N + 1] region slots of each particle
     DevBuf<int> d_ps_items;      // slot ids (CSR positions of d_reg_items)
     DevBuf<double> d_regw64;     // [slot][row] region-local separable weights (FP64, FP32)
     DevBuf<float> d_regw32;
-    DevBuf<float> d_partial;     // [slot][6 B] per-region contractions (FP32, FP64)
-    DevBuf<double> d_partial64;
+    DevBuf<double> d_partial64;  // [slot][6 B] per-region contractions (FP64, or FP32 in its first half)
     // k-space tables: (khat x, y, z, scale) per voxel for the full grid and the
     // z-DFT slab kernel, or the Toeplitz plane coupling [Zs][6][nx*ny] for a
     // contiguous slab; kspace_key records which one is current
