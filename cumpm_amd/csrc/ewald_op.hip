@@ -1361,91 +1361,6 @@ __global__ __launch_bounds__(256) void line_fft2_kernel(const typename C2<R>::T*
     }
 }
 
-// Persistent form of line_fft2_kernel for plans where each stage is one butterfly
-// per thread (TL P2 <= 256 and TL P1 <= 256): a block walks the sets of TL lines
-// blockIdx.x, blockIdx.x + gridDim.x, ..., and issues the stage-1 loads of its next
-// set into registers before the current set's stage 2 (LDS -> DFT -> global), so
-// that the loads of one set are in flight while the previous set is finished and
-// stored.
-template <typename R, bool INV, int CLS>
-__global__ __launch_bounds__(256) void line_fft2p_kernel(const typename C2<R>::T* in, typename C2<R>::T* out,
-                                                         LineGeom gin, LineGeom gout, int n, int P1, int P2,
-                                                         const typename C2<R>::T* __restrict__ twiddle, int TL,
-                                                         int nsets) {
-    using CT = typename C2<R>::T;
-    constexpr int PMAX = CLS == 0 ? 8 : (CLS == 1 ? 16 : kMaxPrimeRadix);
-    extern __shared__ unsigned char s_raw[];
-    CT* const y = reinterpret_cast<CT*>(s_raw);
-    const int ln = (n + P2) | 1;  // one line in LDS: P2 rows of P1 + 1, odd pitch (no bank conflicts)
-    CT* tw = y + TL * ln;
-    const int tid = threadIdx.x;
-    for (int k = tid; k < n; k += blockDim.x) {
-        CT w = twiddle[k];
-        if (INV) w.y = -w.y;
-        tw[k] = w;
-    }
-    const int esi = (int)gin.elem_stride, eso = (int)gout.elem_stride;
-    const int nlines = (int)gin.nlines;
-    // this thread's butterflies: stage 1 (line li1, j1 < P2), stage 2 (line li2, a2 < P1),
-    // lines fastest when the elements are strided (consecutive lines adjacent in memory)
-    int li1, j1, li2, a2;
-    if (esi == 1) fdivmod(tid, P2, 1.0f / (float)P2, li1, j1);
-    else fdivmod(tid, TL, 1.0f / (float)TL, j1, li1);
-    if (eso == 1) fdivmod(tid, P1, 1.0f / (float)P1, li2, a2);
-    else fdivmod(tid, TL, 1.0f / (float)TL, a2, li2);
-    const bool in1 = tid < TL * P2, in2 = tid < TL * P1;
-    CT v[PMAX];
-    auto load = [&](int set) {
-        const int l = set * TL + li1;
-        if (in1 && l < nlines) {
-            const int ib = line_base32(gin, l) + j1 * esi;
-#pragma unroll
-            for (int r = 0; r < PMAX; ++r)
-                if (r < P1) v[r] = in[ib + r * P2 * esi];
-        }
-    };
-    int set = blockIdx.x;
-    if (set < nsets) load(set);
-    __syncthreads();  // twiddles
-    for (; set < nsets; set += gridDim.x) {
-        const int l0 = set * TL;
-        // stage 1: DFT of the loaded inputs into LDS
-        if (in1 && l0 + li1 < nlines) {
-            switch (P1) {
-#define MPM_S1(P, C)                                                            \
-    case P:                                                                     \
-        if constexpr (CLS >= C && P <= PMAX) {                                  \
-            dft_reg<R, P, INV>(v);                                              \
-            _Pragma("unroll") for (int q = 0; q < P; ++q) y[li1 * ln + j1 * (P + 1) + q] = v[q]; \
-        }                                                                       \
-        break;
-                MPM_FFT2_CASES(MPM_S1)
-#undef MPM_S1
-                default:
-                    break;
-            }
-        }
-        __syncthreads();
-        // the next set's loads, in flight during this set's stage 2
-        if (set + (int)gridDim.x < nsets) load(set + gridDim.x);
-        // stage 2: LDS -> DFT -> output
-        if (in2 && l0 + li2 < nlines) {
-            const int ob = line_base32(gout, l0 + li2) + a2 * eso;
-            switch (P2) {
-#define MPM_S2(P, C)                                                                          \
-    case P:                                                                                   \
-        if constexpr (CLS >= C) fft2_s2<R, P, INV>(y, li2 * ln + a2, P1 + 1, tw, a2, out, ob, P1 * eso); \
-        break;
-                MPM_FFT2_CASES(MPM_S2)
-#undef MPM_S2
-                default:
-                    break;
-            }
-        }
-        __syncthreads();
-    }
-}
-
 // Full grid ([B][3][nx][ny][nz], z lines contiguous): the forward z transform, the
 // Green's tensor projection f <- sc (f . k^) k^ and the inverse z transform in one
 // pass, after the forward x and y transforms and before the inverse ones (the
@@ -1684,20 +1599,6 @@ struct LinePass {
     int src, dst;
 };
 
-// Compute units of the current device (cached per device).
-static int device_cus() {
-    static int cached[64] = {0};
-    int dev = 0;
-    MPM_HIP(hipGetDevice(&dev));
-    if (dev < 0 || dev >= 64) return 256;
-    if (cached[dev] == 0) {
-        int cus = 0;
-        MPM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        cached[dev] = std::max(cus, 1);
-    }
-    return cached[dev];
-}
-
 struct LineFft {
     std::vector<std::unique_ptr<LineAxis>> axes;
     std::vector<LinePass> fwd, inv;
@@ -1770,27 +1671,6 @@ struct LineFft {
                 const int TL = lines_per_block2<R>(n, ax.plan.radix[0], ax.plan.radix[1]);
                 const size_t lds = ((size_t)TL * ((n + ax.plan.radix[1]) | 1) + n) * sizeof(CT);  // padded rows
                 const int nsets = (int)((gi.nlines + TL - 1) / TL);
-                if (TL * ax.plan.radix[1] <= 256 && TL * ax.plan.radix[0] <= 256) {
-                    // one butterfly per thread and stage: persistent blocks, next set prefetched
-                    const int bpc = std::max(1, std::min(8, (int)((160 * 1024) / lds)));
-                    const dim3 gp((unsigned)std::min(nsets, device_cus() * bpc));
-                    pscope.kernel("line_fft2p_kernel", {type_name<R>(), tbool(!forward), std::to_string(ax.cls)});
-#define MPM_LP(INV, C)                                                                                         \
-    hipLaunchKernelGGL((line_fft2p_kernel<R, INV, C>), gp, dim3(256), lds, s, in, out, gi, go, n, ax.plan.radix[0], \
-                       ax.plan.radix[1], tw, TL, nsets)
-                    if (forward) {
-                        if (ax.cls == 0) MPM_LP(false, 0);
-                        else if (ax.cls == 1) MPM_LP(false, 1);
-                        else MPM_LP(false, 2);
-                    } else {
-                        if (ax.cls == 0) MPM_LP(true, 0);
-                        else if (ax.cls == 1) MPM_LP(true, 1);
-                        else MPM_LP(true, 2);
-                    }
-#undef MPM_LP
-                    MPM_HIP(hipGetLastError());
-                    continue;
-                }
                 const dim3 gd((unsigned)nsets);
                 pscope.kernel("line_fft2_kernel", {type_name<R>(), tbool(!forward), std::to_string(ax.cls)});
 #define MPM_L2(INV)                                                                       \
@@ -5900,9 +5780,15 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
         rg.cs = (unsigned)V;
     }
     // the matrix-core kernels address the grid with 32-bit element offsets
-    const bool regions = reg_rx > 0 && (SLAB || reg_full) &&
-                         2ull * (unsigned long long)B * grid_elems_per_rhs() < (1ull << 32);
     const int nreg = reg_nrx * reg_npb * reg_nry;
+    // full grid: the region kernels need enough (region, column group) waves to fill the
+    // GPU; with fewer than a quarter of the per-particle gather's waves (small grids and
+    // batches, e.g. 512 particles on 32^3 with one right-hand side: 64 against 512) the
+    // per-particle path is faster
+    const long long region_waves = (long long)nreg * ((6 * B + 15) / 16);
+    const bool full_ok = reg_full && 4 * region_waves >= (long long)N * ((B + 3) / 4);
+    const bool regions = reg_rx > 0 && (SLAB || full_ok) &&
+                         2ull * (unsigned long long)B * grid_elems_per_rhs() < (1ull << 32);
     // the matrix-core contractions load whole vectors of two complex when an even row
     // count and stride and an aligned grid keep every pair aligned and both of its rows
     // inside or outside together (16-byte stores in the spread measured slower: kept at
