@@ -11,23 +11,32 @@
 //  * layout  — thin films run on a slab of the occupied z-planes (the z transform is
 //              replaced by a Toeplitz coupling of those planes); other geometries,
 //              quadrupoles and the polydisperse operator on the full grid.
-//  * spread  — slab: region contractions on the matrix cores (spread_mfma_kernel,
-//              separable weights V^T W^T per region, every voxel written once); full
-//              grid: owner-computes tiles that pull in the binned particles (no atomics,
-//              no zero-fill), in place of the reference's atomic scatter (:1160-1298).
+//  * spread  — region products on the matrix cores (spread_mfma_kernel, separable
+//              weights V^T W^T per region, every voxel written once): x by 16-row
+//              y regions over all slab planes, or 4 x 8 x 16 (x, y, z) regions on the
+//              full grid (monodisperse dipoles); owner-computes tiles that pull in the
+//              binned particles where the regions do not apply (small full grids,
+//              quadrupoles, polydisperse); no atomics, in place of the reference's
+//              atomic scatter (:1160-1298).
 //  * FFT     — the built-in batched LDS line FFT (register radices 2-23, a generic
-//              stage beyond), 2-D on the slab with a transposed k-space buffer and
-//              3-D on the full grid; rocFFT as an alternative back-end.
-//  * scale   — full grid: the Green's tensor projection per voxel (:1406-1432); slab:
-//              xtoeplitz_kernel fuses the x transform, the plane coupling and the
+//              stage beyond; lines held in LDS with padded rows), 2-D on the slab with a
+//              transposed k-space buffer and 3-D on the full grid; rocFFT as an
+//              alternative back-end.
+//  * scale   — full grid: the Green's tensor projection per voxel (:1406-1432), in FP32
+//              fused with the forward and inverse z transforms (zline_scale_kernel);
+//              slab: xtoeplitz_kernel fuses the x transform, the plane coupling and the
 //              inverse x transform (FP32), ztoeplitz / zfused kernels otherwise.
-//  * gather  — slab: matrix-core region contractions (contract_mfma_kernel) plus
-//              finish_kernel (slot sums, real-space neighbour sum over a symmetric CSR
-//              list, self term, Jacobi); full grid: a wavefront per particle fusing the
-//              contraction (:1488-1525), the real-space sum (:1631-1869) and the self
-//              term (:1589-1629); one store per particle and right-hand side.
-//  * precalc — per frame on the host (neighbour list and region rows on host threads):
-//              stencils, bins, regions, CSR neighbours with per-pair coefficients.
+//  * gather  — the transposes of the spread's region products on the matrix cores
+//              (contract_mfma_kernel and its narrow-batch form on the slab,
+//              contract_full_mfma_kernel on the full grid; 16-byte complex-pair loads)
+//              into per-slot partials, then finish_kernel (slot sums, real-space
+//              neighbour sum over a symmetric CSR list, self term, Jacobi); elsewhere a
+//              wavefront per particle fusing the contraction (:1488-1525), the
+//              real-space sum (:1631-1869) and the self term (:1589-1629); one store
+//              per particle and right-hand side.
+//  * precalc — per frame: the neighbour list and pair coefficients (monodisperse) and
+//              the region weight rows on the device; stencil weights, bins and region
+//              candidate lists on host threads.
 
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -2045,19 +2054,20 @@ __global__ __launch_bounds__(256) void scale_kernel(typename C2<R>::T* __restric
     }
 }
 
-// ---- spread on the matrix cores (slab layout) ----
+// ---- spread on the matrix cores ----
 //
-// For an xy-region of RX x 16 voxels over the slab planes and 16 real columns
+// For a region of RX x (planes) x 16 rows (slab: RX x 16 y over all slab planes;
+// full grid: 4 x 8 y-planes x 16 z, see RegionGeom) and 16 real columns
 // q = 6 b + 2 c + (0: real, 1: imaginary) of the right-hand sides,
 //   grid^T[q][voxel] = sum_k V[k][q] W[k][voxel],
-//   W[k][(x, y, plane)] = wx_k(x) wy_k(y) wz_k(plane)
+//   W[k][(x, plane, row)] = wx_k(x) wp_k(plane) wr_k(row)
 // over the candidate particles k of the region (every particle whose stencil
 // footprint meets it, in ascending id order), one 16x16x4 MFMA (FP32 or FP64)
 // per four candidates and (x, plane) pair: A = V^T (16 columns x 4 candidates),
-// B = W^T (4 candidates x 16 consecutive y).  The region-local separable weights
+// B = W^T (4 candidates x 16 consecutive rows).  The region-local separable weights
 // (stencil folding on small grids included) are staged per candidate chunk in
-// LDS.  Every voxel of the slab is written exactly once, without atomics; the
-// sum order is fixed (deterministic).
+// LDS.  Every voxel is written exactly once, without atomics; the sum order is
+// fixed (deterministic).
 constexpr int kMfmaRY = 16;    // y voxels per region (the MFMA N dimension)
 constexpr size_t kMaxFusedLds = 64 * 1024;  // LDS of one xtoeplitz_kernel block
 // Radius classes of the polydisperse operator up to which pair tables are built.
@@ -2333,16 +2343,17 @@ __global__ __launch_bounds__(256) void spread_mfma_kernel(SlotTable tab, int B, 
     }
 }
 
-// ---- contraction on the matrix cores (slab layout) ----
+// ---- contraction on the matrix cores ----
 //
 // The transpose of spread_mfma_kernel over the same regions: for the candidate
 // particles k of a region and 16 real columns q,
 //   D[k][q] = sum_{voxels v of the region} W[k][v] G[v][q],
 // one 16x16x4 MFMA per four voxels and 16 candidates (A = W, B = G), every
 // loaded operand feeding all candidate blocks.  Each wave reads its B operands
-// straight from the grid: lane (j, u) loads column j at y = 4 u + (lane >> 4),
-// so the four loads of a 16-voxel row consume whole cache lines; waves run
-// without barriers after the one weight-staging step.  D goes to the particle's
+// straight from the grid: lane (j, u) loads column j at row 4 u + (lane >> 4),
+// so the four loads of a 16-voxel row consume whole cache lines, or with VEC
+// (pair_rows) one 16-byte complex pair per lane and x; waves run without
+// barriers after the one weight-staging step.  D goes to the particle's
 // slot of the region (the CSR position of the pair); the per-particle sum over
 // its regions happens in the finish step, in a fixed order.
 template <typename R, int RX, int ZSB, int NB, bool VEC>
