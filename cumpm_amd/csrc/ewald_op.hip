@@ -3498,39 +3498,56 @@ __global__ __launch_bounds__(256) void finish_kernel(SlotTable tab, int B, int p
     const int i = order[ii];
     const double2* xb = s_x[b];
     double ar[3] = {0.0, 0.0, 0.0}, ai[3] = {0.0, 0.0, 0.0};
-    if (parts & MPM_PART_RECIP) {
-        const int s0 = ps_start[i], s1 = ps_start[i + 1];
-        for (int sl = s0 + sub; sl < s1; sl += NL) {
-            const R* pp = partial + (size_t)ps_items[sl] * (6 * B) + 6 * b;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                ar[c] += (double)pp[2 * c];
-                ai[c] += (double)pp[2 * c + 1];
-            }
-        }
-    }
     double br[3] = {0.0, 0.0, 0.0}, bi[3] = {0.0, 0.0, 0.0};
-    if (parts & MPM_PART_REAL) {
-        const int k0 = nb_start[i], k1 = nb_start[i + 1];
-#pragma unroll 2
-        for (int k = k0 + sub; k < k1; k += NL) {
-            const int j = nb_j[k];
-            const double* pc = pairc + 8 * (size_t)k;
-            const double ux = pc[0], uy = pc[1], uz = pc[2], cp = pc[3], cd = pc[4], ec = pc[5], es = pc[6];
-            const double2 p0 = xb[3 * (size_t)j], p1 = xb[3 * (size_t)j + 1], p2 = xb[3 * (size_t)j + 2];
-            const double rr = cd * (p0.x * ux + p1.x * uy + p2.x * uz);
-            const double ri = cd * (p0.y * ux + p1.y * uy + p2.y * uz);
-            const double u[3] = {ux, uy, uz};
-            const double2 pp[3] = {p0, p1, p2};
+    // the slot sum (the caller always asks for the reciprocal part here) and the
+    // neighbour sum; the lane's first two slots and first two neighbours are fetched
+    // together (indices, then data, read at clamped in-range positions and dropped when
+    // past the end), the rest in the loops that follow
+    const int s0 = ps_start[i], s1 = ps_start[i + 1];
+    const bool real = (parts & MPM_PART_REAL) != 0;
+    const int k0 = real ? nb_start[i] : 0, k1 = real ? nb_start[i + 1] : 0;
+    // entries past the end are masked to zero (bit masks, not branches, so that the
+    // compiler issues every load before the first use)
+    auto slot = [&](int item_slot, bool take) {
+        const R* pp = partial + (size_t)item_slot * (6 * B) + 6 * b;
+        R v[6];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double er = cp * pp[c].x + rr * u[c];
-                const double ei = cp * pp[c].y + ri * u[c];
-                br[c] += er * ec - ei * es;
-                bi[c] += er * es + ei * ec;
-            }
+        for (int e = 0; e < 6; ++e) v[e] = keep_if(pp[e], take);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            ar[c] += (double)v[2 * c];
+            ai[c] += (double)v[2 * c + 1];
         }
+    };
+    auto neighbour = [&](int k, int j, bool take) {
+        const double* pc = pairc + 8 * (size_t)k;
+        const double ux = pc[0], uy = pc[1], uz = pc[2], cp = pc[3], cd = pc[4];
+        const double ec = keep_if(pc[5], take), es = keep_if(pc[6], take);  // zero phase: no contribution
+        const double2 p0 = xb[3 * (size_t)j], p1 = xb[3 * (size_t)j + 1], p2 = xb[3 * (size_t)j + 2];
+        const double rr = cd * (p0.x * ux + p1.x * uy + p2.x * uz);
+        const double ri = cd * (p0.y * ux + p1.y * uy + p2.y * uz);
+        const double u[3] = {ux, uy, uz};
+        const double2 pp[3] = {p0, p1, p2};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double er = cp * pp[c].x + rr * u[c];
+            const double ei = cp * pp[c].y + ri * u[c];
+            br[c] += er * ec - ei * es;
+            bi[c] += er * es + ei * ec;
+        }
+    };
+    {
+        const int sa = s0 + sub, sb = sa + NL, ka = k0 + sub, kb = ka + NL;
+        const int sac = max(min(sa, s1 - 1), 0), sbc = max(min(sb, s1 - 1), 0);
+        const int kac = max(min(ka, k1 - 1), 0), kbc = max(min(kb, k1 - 1), 0);
+        const int ia = ps_items[sac], ib = ps_items[sbc], ja = nb_j[kac], jb = nb_j[kbc];
+        slot(ia, sa < s1);
+        slot(ib, sb < s1);
+        neighbour(kac, ja, ka < k1);
+        neighbour(kbc, jb, kb < k1);
     }
+    for (int sl = s0 + sub + 2 * NL; sl < s1; sl += NL) slot(ps_items[sl], true);
+    for (int k = k0 + sub + 2 * NL; k < k1; k += NL) neighbour(k, nb_j[k], true);
     // butterfly over the item's lanes: every lane ends with the same sums
 #pragma unroll
     for (int m = 1; m < NL; m <<= 1)
