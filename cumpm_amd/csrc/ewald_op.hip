@@ -5891,9 +5891,9 @@ void EwaldOp::launch(const SlotTable& t, int B, int parts, hipStream_t s, int ja
                 const size_t fixed = 2 * (size_t)pl.n * sizeof(CT) + 1024;
                 const int by_lds = kMaxFusedLds > fixed ? (int)((kMaxFusedLds - fixed) / per_line) : 0;
                 const int by_threads = 256 / (3 * pl.radix[1]);  // one inverse stage-1 butterfly per thread
-                // about 6 KB of lines per component and block: more blocks per CU keep
-                // more loads in flight (measured at 256^3: 3 lines 3.41 ms, 5 lines 3.69 ms)
-                ztl = std::min({by_lds, by_threads, std::max(1, 768 / pl.n)});
+                // about 10 KB of lines per component and block (measured at 256^3 with
+                // padded rows: 5 lines 2.39 ms, 3 lines 2.55 ms)
+                ztl = std::min({by_lds, by_threads, std::max(1, 1280 / pl.n)});
             }
             const bool fusedz = ztl >= 1;
             const int skip = fusedx ? xaxis : (fusedz ? zaxis : -1);
